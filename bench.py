@@ -1,0 +1,182 @@
+"""Benchmark: MLUPS of the projection time step on one MI355X.
+
+Workload: flow past a cylinder, 2048 x 1024 channel (32 x 16 D), Re = 100,
+uniform inflow, zero-gradient OUTLET, slip walls, D = 1 body at 4 D, v seeded
+with a 1e-3 Gaussian blob; SSP-RK3, adaptive dt (ceiling 0.02, CFL 0.4);
+pressure by multigrid-preconditioned CG to relative residual 1e-10.  Both
+precision paths run: plain FP64, and mixed (FP32 V-cycle inside FP64 CG).
+
+One "step" is one full SSP-RK3 time step (three projections) through
+Simulation.run, per-step diagnostics included, with fields resident in HBM.
+MLUPS = nx * ny * steps / wall seconds / 1e6.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+KERNELS = {0: "k_cg_update", 1: "k_mgcg_dir<double>", 2: "k_smooth_fine<double>",
+           3: "k_residual_fine<double>"}
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_path(precision: str, steps: int, warmup: int, nx: int, ny: int):
+    import numpy as np
+    import torch
+
+    from pycfd_amd import cases
+    from pycfd_amd.physics.incompressible import Simulation
+
+    cfg, body, u0, v0 = cases.cylinder(nx=nx, ny=ny, pressure_solver="mgcg",
+                                       poisson_tol=1e-10, precision=precision)
+    t0 = time.perf_counter()
+    sim = Simulation(cfg, obstacle=body, u_init=u0, v_init=v0)
+    log(f"[{precision}] setup {time.perf_counter() - t0:.1f} s "
+        f"({sim.solver.pressure_solver.hierarchy})")
+    if warmup:
+        sim.run(max_steps=warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = sim.run(max_steps=steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    hist = res.history
+    out = {
+        "mlups": nx * ny * steps / wall / 1e6,
+        "ms_per_step": 1e3 * wall / steps,
+        "max_div": float(max(hist["max_div"])),
+        "iterations_last_solve": int(sim.solver.pressure_solver.last_iterations),
+        "cd_last": float(sim.force_coefficients()[0]),
+        "t": float(sim.fields.t),
+    }
+    log(f"[{precision}] {steps} steps in {wall:.3f} s -> {out['mlups']:.1f} MLUPS, "
+        f"max|div| {out['max_div']:.2e}, {out['iterations_last_solve']} it/solve")
+    return sim, out
+
+
+def kernel_roofline(sim, which: int, reps: int = 50):
+    ms, nbytes = sim.solver.ctx.time_kernel(which, reps)
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    return {"kernel": KERNELS[which], "bound": "hbm", "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None, "avg_us": round(ms * 1e3, 3), "bytes_per_launch": nbytes}
+
+
+def cpu_baseline(nx: int, ny: int, steps: int):
+    """The CPU oracle (NumPy/SciPy FP64 restatement of the reference) on the
+    same workload, timed over `steps` steps after its own setup."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    Lx, Ly = 32.0, 16.0
+    dx, dy = Lx / nx, Ly / ny
+    offs = (np.arange(8) + 0.5) / 8
+    xf = np.concatenate(([0.0], np.cumsum(np.full(nx, dx))))
+    yf = np.concatenate(([0.0], np.cumsum(np.full(ny, dy))))
+    xf[-1], yf[-1] = Lx, Ly
+    xs = (xf[:-1, None] + offs[None, :] * dx).ravel()
+    ys = (yf[:-1, None] + offs[None, :] * dy).ravel()
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    solid = (((X - 4.0) ** 2 + (Y - 8.0) ** 2 <= 0.25).astype(float)
+             .reshape(nx, 8, ny, 8).mean(axis=(1, 3)) > 0.5)
+    case = O.Case(nx=nx, ny=ny, lx=Lx, ly=Ly, re=100.0, dt=0.02, cfl_max=0.4,
+                  kinds=("inlet", "outlet", "symmetry", "symmetry"),
+                  velocity=(1.0, 0.0, 0.0, 0.0), solid=solid, pressure_solver="mgcg",
+                  tol=1e-10)
+    Xv, Yv = np.meshgrid(case.xc, case.yf, indexing="ij")
+    v0 = 1e-3 * np.exp(-(((Xv - 5.0) / 1.0) ** 2 + ((Yv - 8.0) / 0.5) ** 2))
+    st = O.State()
+    u, v, p = O.initialize(case, st, 1.0, v0)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dt = O.compute_dt(case, u, v)
+        u, v, p = O.step(case, st, u, v, dt)
+    wall = time.perf_counter() - t0
+    return {"value": round(nx * ny * steps / wall / 1e6, 4), "unit": "MLUPS", "cores": 1,
+            "kind": "port",
+            "sample": f"{steps} RK3 steps of the {nx}x{ny} cylinder (mgcg, tol 1e-10) after "
+                      f"setup and the initial projection; {wall:.1f} s; NumPy/SciPy FP64, "
+                      "single-threaded sparse kernels"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--nx", type=int, default=2048)
+    ap.add_argument("--ny", type=int, default=1024)
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--paths", default="fp64,mixed")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path "
+              f"(--gpus {args.gpus} requested); run with --gpus 1", file=sys.stderr)
+        return 2
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP device", file=sys.stderr)
+        return 2
+    torch.cuda.set_device(0)
+    from pycfd_amd import _lib
+    _lib.load()
+
+    results = {}
+    sims = {}
+    for prec in args.paths.split(","):
+        sims[prec], results[prec] = run_path(prec, args.steps, args.warmup, args.nx, args.ny)
+    main_path = "mixed" if "mixed" in results else "fp64"
+    r = results[main_path]
+    roof = [kernel_roofline(sims["fp64" if "fp64" in sims else main_path], w) for w in KERNELS]
+    dominant = max(roof, key=lambda d: d["avg_us"] * (2 if d["kernel"] != "k_cg_update" else 1))
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline (oracle) ...")
+        cpu = cpu_baseline(args.nx, args.ny, args.cpu_steps)
+        log(f"cpu baseline {cpu['value']} MLUPS")
+    line = {
+        "metric": "MLUPS per projection step @2048x1024, mixed vs FP64 at fixed div-u; HBM GB/s",
+        "value": round(r["mlups"], 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if main_path == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"cylinder Re=100 {args.nx}x{args.ny} (32x16 D), OUTLET, "
+                               "mgcg tol 1e-10, SSP-RK3",
+                   "path": main_path, "nx": args.nx, "ny": args.ny},
+        "paths": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv)
+                      for kk, vv in v.items()} for k, v in results.items()},
+        "roofline": dominant,
+        "kernels": roof,
+        "cpu_baseline": cpu,
+    }
+    if "fp64" in results and "mixed" in results:
+        line["mixed_speedup_vs_fp64"] = round(results["mixed"]["mlups"] / results["fp64"]["mlups"], 3)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

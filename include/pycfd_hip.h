@@ -1,0 +1,216 @@
+/*
+ * pycfd_hip.h -- C-ABI of libpycfd_hip.so, the gfx950 (MI355X) engine behind
+ * the projection time step of pycfd.
+ *
+ * The reference (kiliancmd/pycfd) is pure Python, so there is no existing FFI
+ * to replace; each entry point below stands in for the Python method cited
+ * next to it, and the Python package `pycfd_amd` binds them with ctypes
+ * (pycfd_amd/_lib.py) behind the reference's own classes.
+ *
+ * Conventions
+ *   - All field pointers are DEVICE pointers into FP64 arrays in the
+ *     reference's ghosted C-order layout (pycfd/core/mesh.py:14-31):
+ *       u (nx+3, ny+2), v (nx+2, ny+3), p (nx+2, ny+2), contiguous.
+ *     Poisson vectors are the nx*ny interior cells, k = i*ny + j.
+ *   - The library never frees caller memory.  It owns its scratch buffers.
+ *   - Work is enqueued on the HIP stream given at creation; functions that
+ *     return host values synchronise that stream, the others do not.
+ *   - Every function returns a status code; pcfd_last_error() describes the
+ *     most recent failure of the calling thread.
+ *   - A context is not thread-safe: one context = one simulation = one stream.
+ */
+#ifndef PYCFD_HIP_H
+#define PYCFD_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define PCFD_OK 0
+#define PCFD_ERR_HIP 1
+#define PCFD_ERR_NOT_CONVERGED 2
+#define PCFD_ERR_STALLED 3
+#define PCFD_ERR_NONFINITE 4
+#define PCFD_ERR_BAD_ARG 5
+#define PCFD_ERR_UNSUPPORTED 6
+#define PCFD_ERR_NO_DEVICE 7
+
+/* wall kinds, pycfd/config.py:85-94 (BCKind) */
+#define PCFD_BC_NO_SLIP 0
+#define PCFD_BC_MOVING_WALL 1
+#define PCFD_BC_INLET 2
+#define PCFD_BC_OUTLET 3
+#define PCFD_BC_PRESSURE_OUTLET 4
+#define PCFD_BC_PERIODIC 5
+#define PCFD_BC_SYMMETRY 6
+
+/* pressure solvers, pycfd/config.py:74-82 (PressureSolver) */
+#define PCFD_SOLVER_DIRECT 0
+#define PCFD_SOLVER_CG 1
+#define PCFD_SOLVER_JACOBI 2
+#define PCFD_SOLVER_SOR 3
+#define PCFD_SOLVER_MULTIGRID 4
+#define PCFD_SOLVER_MGCG 5
+
+/* time schemes, pycfd/config.py:52-64 (TimeScheme) */
+#define PCFD_SCHEME_EULER 0
+#define PCFD_SCHEME_RK2 1
+#define PCFD_SCHEME_RK3 2
+
+/* boundary-pass operation codes for pcfd_mask_obstacle */
+#define PCFD_MASK_ZERO 0
+#define PCFD_MASK_RESET 1
+#define PCFD_MASK_ACCUMULATE 2
+
+typedef struct pcfd_ctx pcfd_ctx;
+
+/* One case: grid, physics, walls, immersed body.  Host pointers are copied. */
+typedef struct pcfd_case_desc {
+  int32_t nx, ny;
+  double dx, dy;                /* uniform spacing (mesh.py:313-529) */
+  double nu;                    /* config.py:248-251 */
+  double body_fx, body_fy;      /* config.py:209 */
+  int32_t time_scheme;          /* PCFD_SCHEME_* */
+  int32_t bc_kind[4];           /* PCFD_BC_*, walls left, right, bottom, top */
+  double bc_velocity[4];
+  double bc_p_ref[4];
+  const double* inlet_line[4];  /* INLET walls: full normal-velocity line, length
+                                   ny+2 (left/right) or nx+2 (bottom/top),
+                                   boundary.py:205-216; NULL otherwise */
+  const uint8_t* solid;         /* (nx*ny) 1 inside a body, or NULL */
+  const int8_t* body_id;        /* (nx*ny) body of each solid cell, or NULL */
+  int32_t n_bodies;             /* > 1 enables per-body forces (solver.py:136-146) */
+} pcfd_case_desc;
+
+typedef struct pcfd_solver_desc {
+  int32_t kind;       /* PCFD_SOLVER_* */
+  double tol;         /* relative residual (config.py:217) */
+  int32_t maxiter;    /* config.py:218 */
+  double omega;       /* SOR over-relaxation (config.py:219) or Jacobi damping */
+  int32_t sweeps;     /* V-cycle pre/post sweeps (config.py:220) */
+  int32_t precision;  /* 0: FP64 throughout; 1: FP32 V-cycle, FP64 Krylov */
+} pcfd_solver_desc;
+
+/* One level of the aggregation hierarchy (multigrid.py:293-322).  Level 0 is
+ * the fine operator, applied matrix-free: only nx, ny, fx, fy, active and
+ * weight are read for it.  A coarser level carries its Galerkin operator as a
+ * stencil dictionary: row k of the operator is row cls[k] of `table`
+ * (n_classes x n_offsets, row-major), coefficient s acting on the cell at
+ * offset (di[s], dj[s]); per class its 1/diag and active flag.  The coarsest
+ * level carries the dense pseudo-inverse (coarse_n x coarse_n, row-major) of
+ * its operator, projections included (multigrid.py:524-543). */
+typedef struct pcfd_mg_level_desc {
+  int32_t nx, ny;
+  int32_t fx, fy;              /* aggregation factors towards the next level */
+  int32_t n_offsets;
+  const int8_t* di;
+  const int8_t* dj;
+  int32_t n_classes;
+  const uint16_t* cls;         /* (nx*ny) */
+  const double* table;         /* (n_classes*n_offsets) */
+  const double* class_dinv;    /* (n_classes) */
+  const uint8_t* class_active; /* (n_classes) */
+  const uint8_t* active;       /* (nx*ny) */
+  double weight;               /* damped-Jacobi weight (multigrid.py:354-362) */
+  int32_t coarse_n;
+  const double* coarse_inv;
+} pcfd_mg_level_desc;
+
+typedef struct pcfd_solve_info {
+  int32_t iterations;  /* last_iterations (pressure.py:296) */
+  double residual;     /* last_residual, relative 2-norm (pressure.py:318-322) */
+  int32_t converged;
+  int32_t status;
+} pcfd_solve_info;
+
+/* --- lifetime --------------------------------------------------------- */
+/* ProjectionSolver.__init__ (solver.py:101-195): device state, IB face lists,
+ * Poisson operator flags (pressure.py:110-270).  hip_stream may be NULL. */
+int pcfd_ctx_create(const pcfd_case_desc* desc, void* hip_stream, pcfd_ctx** out);
+int pcfd_ctx_destroy(pcfd_ctx* ctx);
+int pcfd_last_error(char* buf, size_t len);
+int pcfd_device_count(int32_t* count);
+int pcfd_synchronize(pcfd_ctx* ctx);
+/* Host-only: the per-cell flag byte of the fine Poisson operator the device
+ * applies (bit0 solvable, bits 1-4 faces to x-1, x+1, y-1, y+1 carry flux),
+ * i.e. the sparsity of assemble_poisson_matrix (pressure.py:209-251).  Needs
+ * no device; returns the solvable-cell count in *n_solvable. */
+int pcfd_poisson_flags(const pcfd_case_desc* desc, uint8_t* flags_out, int32_t* n_solvable);
+
+/* make_pressure_solver (pressure.py:645-660) with a hierarchy built by the
+ * host setup (MultigridHierarchy._build, multigrid.py:364-395). */
+int pcfd_set_pressure_solver(pcfd_ctx* ctx, const pcfd_solver_desc* desc,
+                             const pcfd_mg_level_desc* levels, int32_t n_levels);
+
+/* --- operators of one substep ------------------------------------------- */
+/* ProjectionSolver.momentum_rhs (solver.py:501-529): rhs_u (nx+1, ny),
+ * rhs_v (nx, ny+1), contiguous. */
+int pcfd_momentum_rhs(pcfd_ctx* ctx, const double* u, const double* v, double blend,
+                      double* rhs_u, double* rhs_v);
+/* ProjectionSolver.divergence (solver.py:532-543): out (nx, ny). */
+int pcfd_divergence(pcfd_ctx* ctx, const double* u, const double* v, double* out);
+/* BoundaryManager.apply_velocity (boundary.py:375-379). */
+int pcfd_apply_velocity_bc(pcfd_ctx* ctx, double* u, double* v, int32_t predictor);
+/* BoundaryManager.enforce_global_mass_balance (boundary.py:423-487). */
+int pcfd_mass_balance(pcfd_ctx* ctx, double* u, double* v);
+/* ProjectionSolver._mask_obstacle (solver.py:553-599); mode PCFD_MASK_*. */
+int pcfd_mask_obstacle(pcfd_ctx* ctx, double* u, double* v, double dt, int32_t mode);
+/* ProjectionSolver._wrap_periodic_faces (solver.py:601-607). */
+int pcfd_wrap_faces(pcfd_ctx* ctx, double* u, double* v);
+/* BoundaryManager.apply_pressure (boundary.py:381-385) on a ghosted p. */
+int pcfd_apply_pressure_bc(pcfd_ctx* ctx, double* p);
+
+/* PoissonSolverBase.solve (pressure.py:325-327): rhs (nx, ny) -> p_out
+ * (nx, ny), normalised; p0 (nx, ny) or NULL for a zero start.  Leaves the
+ * warm-start guess of pcfd_project untouched. */
+int pcfd_pressure_solve(pcfd_ctx* ctx, const double* rhs, const double* p0, double* p_out,
+                        pcfd_solve_info* info);
+/* ProjectionSolver.project (solver.py:643-683): u, v corrected in place, p
+ * (ghosted) written; warm-started from the previous normalised pressure. */
+int pcfd_project(pcfd_ctx* ctx, double* u, double* v, double dt, double* p,
+                 pcfd_solve_info* info);
+/* ProjectionSolver.initialize (solver.py:301-329) after the caller has written
+ * the initial physical faces into u, v (ghosts zero). */
+int pcfd_initialize(pcfd_ctx* ctx, double* u, double* v, double* p, pcfd_solve_info* info);
+/* ProjectionSolver.step (solver.py:686-731): (u0, v0) -> (u, v, p). */
+int pcfd_step(pcfd_ctx* ctx, const double* u0, const double* v0, double dt, double blend,
+              double* u, double* v, double* p, pcfd_solve_info* info);
+/* Forget the warm-start pressure (ProjectionSolver._p_guess = None). */
+int pcfd_reset_guess(pcfd_ctx* ctx);
+
+/* --- reductions for TimeStepper (timestepper.py:94-142, 208-266) ----------
+ * out[0] max|u_phys| out[1] max|v_phys| out[2] non-finite count in u, v, p
+ * out[3] max|u - u_prev| out[4] max|v - v_prev| out[5] max|div| over fluid
+ * out[6] sum over cells of uc^2 + vc^2, out[7..8] body_force_reaction (fx, fy).
+ * out_host holds 9 doubles.  u_prev/v_prev/p may be NULL. */
+int pcfd_diagnostics(pcfd_ctx* ctx, const double* u, const double* v, const double* p,
+                     const double* u_prev, const double* v_prev, double* out_host);
+/* body_force_reaction then body_force_reactions (solver.py:173-178):
+ * out_host has 2 + 2*n_bodies doubles. */
+int pcfd_get_force(pcfd_ctx* ctx, double* out_host);
+
+/* --- Poisson building blocks (tests, autotuning) ----------------------- */
+/* y = A x with the fine operator (pressure.py:110-270); both (nx, ny). */
+int pcfd_poisson_apply(pcfd_ctx* ctx, const double* x, double* y);
+/* z = M r, one V-cycle from zero with the projections
+ * (MultigridHierarchy.apply, multigrid.py:460-474). */
+int pcfd_mg_apply(pcfd_ctx* ctx, const double* r, double* z);
+
+/* --- measurement -------------------------------------------------------
+ * Times `reps` back-to-back launches of one hot fine-level kernel on the
+ * context's stream with HIP events (which: 0 CG update, 1 multigrid-CG
+ * direction + SpMV, 2 damped-Jacobi sweep, 3 residual) and reports the mean
+ * duration and the kernel's algorithmic bytes per launch.  No reference
+ * counterpart: measurement only. */
+int pcfd_time_kernel(pcfd_ctx* ctx, int32_t which, int32_t reps, double* avg_ms,
+                     double* bytes_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYCFD_HIP_H */

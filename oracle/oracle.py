@@ -1,0 +1,738 @@
+"""CPU oracle of pycfd's projection time step.
+
+TEST INFRASTRUCTURE ONLY.  Nothing in the product package (pycfd_amd) imports
+this module; only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg use it, as the checker and as the timed CPU baseline.
+
+A plain FP64 NumPy/SciPy restatement of the reference's algorithm
+(kiliancmd/pycfd, pure Python), self-contained so that a mistake in the HIP
+engine cannot be mirrored here.  Each function cites the reference lines it
+follows.  Pinning (see tests/test_oracle.py):
+
+  momentum_rhs, divergence ........ golden vectors (momentum_rhs.npz), bitwise
+  apply_velocity, mass_balance,
+  mask, wrap, apply_pressure ...... golden vectors (boundary.npz), bitwise/1e-15
+  assemble ........................ symmetry, zero row sums, 5-point
+                                    coefficients, manufactured solution
+  solvers (direct, cg, sor,
+  jacobi, multigrid, mgcg) ........ golden solver outputs (multigrid.npz), a
+                                    dense solve, the closed-form manufactured
+                                    Poisson solution
+  hierarchy / V-cycle ............. golden unknowns, weights, nnz and z = M r
+                                    (multigrid.npz); dense symmetry of M
+  step / run ...................... golden trajectories (trajectory.npz),
+                                    Taylor-Green decay exp(-2 nu t)
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+WALLS = ("left", "right", "bottom", "top")
+
+
+@dataclass
+class Case:
+    """Everything the step needs (config.py:179-235, restricted to uniform meshes)."""
+
+    nx: int
+    ny: int
+    lx: float = 1.0
+    ly: float = 1.0
+    re: float = 100.0
+    u_ref: float = 1.0
+    l_ref: float = 1.0
+    body_force: tuple = (0.0, 0.0)
+    kinds: tuple = ("no_slip",) * 4          # left, right, bottom, top
+    velocity: tuple = (0.0,) * 4
+    profile: tuple = ("uniform",) * 4
+    p_ref: tuple = (0.0,) * 4
+    solid: np.ndarray | None = None
+    scheme: str = "rk3"
+    pressure_solver: str = "direct"
+    tol: float = 1e-10
+    maxiter: int = 20000
+    sor_omega: float = 1.8
+    mg_sweeps: int = 1
+    # time control (timestepper.py)
+    dt: float = 1e-3
+    adaptive_dt: bool = True
+    cfl_max: float = 0.5
+    advection: str = "central"
+    upwind_blend: float | None = None
+
+    def __post_init__(self):
+        self.dx = float(np.full(self.nx, self.lx / self.nx)[0])
+        self.dy = float(np.full(self.ny, self.ly / self.ny)[0])
+        self.nu = self.u_ref * self.l_ref / self.re
+        self.px = self.kinds[0] == "periodic"
+        self.py = self.kinds[2] == "periodic"
+        if self.solid is not None:
+            self.solid = np.asarray(self.solid, dtype=bool)
+            if not self.solid.any():
+                self.solid = None
+        # coordinates, mesh.py:353-375
+        dxc = np.full(self.nx, self.lx / self.nx)
+        dyc = np.full(self.ny, self.ly / self.ny)
+        self.xf = np.concatenate(([0.0], np.cumsum(dxc)))
+        self.yf = np.concatenate(([0.0], np.cumsum(dyc)))
+        self.xf[-1], self.yf[-1] = self.lx, self.ly
+        self.xc = 0.5 * (self.xf[:-1] + self.xf[1:])
+        self.yc = 0.5 * (self.yf[:-1] + self.yf[1:])
+
+
+def wall_index(case: Case, w: int):
+    """(axis, n_face, n_in, n_ghost, t_ghost, t_in, p_ghost, p_in, outward), boundary.py:69-80."""
+    nx, ny = case.nx, case.ny
+    return [(0, 1, 2, 0, 0, 1, 0, 1, -1),
+            (0, nx + 1, nx, nx + 2, nx + 1, nx, nx + 1, nx, 1),
+            (1, 1, 2, 0, 0, 1, 0, 1, -1),
+            (1, ny + 1, ny, ny + 2, ny + 1, ny, ny + 1, ny, 1)][w]
+
+
+def _get(a, axis, idx):
+    return a[idx, :] if axis == 0 else a[:, idx]
+
+
+def _put(a, axis, idx, val):
+    if axis == 0:
+        a[idx, :] = val
+    else:
+        a[:, idx] = val
+
+
+# --------------------------------------------------------------------------
+# Momentum right-hand side: solver.py:334-465, 501-529 (array path, uniform)
+# --------------------------------------------------------------------------
+def momentum_rhs(case: Case, u, v, blend: float = 0.0):
+    nx, ny, dx, dy = case.nx, case.ny, case.dx, case.dy
+    uc = 0.5 * (u[1:nx + 2, 0:ny + 1] + u[1:nx + 2, 1:ny + 2])   # corner u
+    vc = 0.5 * (v[0:nx + 1, 1:ny + 2] + v[1:nx + 2, 1:ny + 2])   # corner v
+    uv = uc * vc
+    ucc = 0.5 * (u[0:nx + 2, :] + u[1:nx + 3, :])
+    vcc = 0.5 * (v[:, 0:ny + 2] + v[:, 1:ny + 3])
+    ur, ul = ucc[1:nx + 2, 1:ny + 1], ucc[0:nx + 1, 1:ny + 1]
+    a_uu = (ur ** 2 - ul ** 2) / dx
+    a_uv = (uv[0:nx + 1, 1:ny + 1] - uv[0:nx + 1, 0:ny]) / dy
+    vt, vb = vcc[1:nx + 1, 1:ny + 2], vcc[1:nx + 1, 0:ny + 1]
+    a_vv = (vt ** 2 - vb ** 2) / dy
+    a_vu = (uv[1:nx + 1, 0:ny + 1] - uv[0:nx, 0:ny + 1]) / dx
+    if blend > 0.0:
+        um, u0, up = u[0:nx + 1, 1:ny + 1], u[1:nx + 2, 1:ny + 1], u[2:nx + 3, 1:ny + 1]
+        a_uu += blend / dx * (np.abs(ur) * (u0 - up) / 2.0 - np.abs(ul) * (um - u0) / 2.0)
+        ud, uu = u[1:nx + 2, 0:ny], u[1:nx + 2, 2:ny + 2]
+        a_uv += blend / dy * (np.abs(vc[0:nx + 1, 1:ny + 1]) * (u0 - uu) / 2.0
+                              - np.abs(vc[0:nx + 1, 0:ny]) * (ud - u0) / 2.0)
+        vm, v0, vp = v[1:nx + 1, 0:ny + 1], v[1:nx + 1, 1:ny + 2], v[1:nx + 1, 2:ny + 3]
+        a_vv += blend / dy * (np.abs(vt) * (v0 - vp) / 2.0 - np.abs(vb) * (vm - v0) / 2.0)
+        vl, vr = v[0:nx, 1:ny + 2], v[2:nx + 2, 1:ny + 2]
+        a_vu += blend / dx * (np.abs(uc[1:nx + 1, 0:ny + 1]) * (v0 - vr) / 2.0
+                              - np.abs(uc[0:nx, 0:ny + 1]) * (vl - v0) / 2.0)
+    dx2, dy2 = dx ** 2, dy ** 2
+    u0 = u[1:nx + 2, 1:ny + 1]
+    lap_u = ((u[0:nx + 1, 1:ny + 1] - 2.0 * u0 + u[2:nx + 3, 1:ny + 1]) / dx2
+             + (u[1:nx + 2, 0:ny] - 2.0 * u0 + u[1:nx + 2, 2:ny + 2]) / dy2)
+    v0 = v[1:nx + 1, 1:ny + 2]
+    lap_v = ((v[0:nx, 1:ny + 2] - 2.0 * v0 + v[2:nx + 2, 1:ny + 2]) / dx2
+             + (v[1:nx + 1, 0:ny + 1] - 2.0 * v0 + v[1:nx + 1, 2:ny + 3]) / dy2)
+    fx, fy = case.body_force
+    return (-(a_uu + a_uv) + case.nu * lap_u + fx,
+            -(a_vu + a_vv) + case.nu * lap_v + fy)
+
+
+def divergence(case: Case, u, v):
+    """solver.py:532-543."""
+    nx, ny = case.nx, case.ny
+    return ((u[2:nx + 2, 1:ny + 1] - u[1:nx + 1, 1:ny + 1]) / case.dx
+            + (v[1:nx + 1, 2:ny + 2] - v[1:nx + 1, 1:ny + 1]) / case.dy)
+
+
+# --------------------------------------------------------------------------
+# Boundaries: boundary.py:99-487, solver.py:553-607
+# --------------------------------------------------------------------------
+def inlet_value(case: Case, w: int, axis: int, outward: int, length: int):
+    speed = -outward * case.velocity[w]
+    if case.profile[w] == "uniform":
+        return speed
+    coord, ext = (case.yc, case.ly) if axis == 0 else (case.xc, case.lx)
+    prof = 4.0 * coord * (ext - coord) / ext ** 2
+    full = np.zeros(length)
+    full[1:1 + coord.size] = speed * prof
+    return full
+
+
+def apply_velocity(case: Case, u, v, predictor: bool) -> None:
+    for w in range(4):
+        kind = case.kinds[w]
+        if kind == "periodic":
+            continue
+        axis, nf, ni, ng, tg, ti, _, _, outward = wall_index(case, w)
+        nrm, tan = (u, v) if axis == 0 else (v, u)
+        prescribes = kind != "pressure_outlet"
+        derives = kind == "outlet"
+        if prescribes and (predictor or not derives):
+            if kind == "inlet":
+                val = inlet_value(case, w, axis, outward, _get(nrm, axis, nf).size)
+            elif kind == "outlet":
+                val = _get(nrm, axis, ni).copy()
+            else:
+                val = 0.0
+            _put(nrm, axis, nf, val)
+        if kind == "pressure_outlet":
+            _put(nrm, axis, ng, _get(nrm, axis, nf).copy())
+        else:
+            _put(nrm, axis, ng, 2.0 * _get(nrm, axis, nf) - _get(nrm, axis, ni))
+        t_in = _get(tan, axis, ti)
+        if kind == "moving_wall":
+            _put(tan, axis, tg, 2.0 * case.velocity[w] - t_in)
+        elif kind in ("no_slip", "inlet"):
+            _put(tan, axis, tg, -t_in)
+        else:
+            _put(tan, axis, tg, t_in.copy())
+    wrap_velocity(case, u, v)
+
+
+def wrap_velocity(case: Case, u, v) -> None:
+    nx, ny = case.nx, case.ny
+    if case.px:
+        u[nx + 1, :] = u[1, :]
+        u[0, :] = u[nx, :]
+        u[nx + 2, :] = u[2, :]
+        v[0, :] = v[nx, :]
+        v[nx + 1, :] = v[1, :]
+    if case.py:
+        v[:, ny + 1] = v[:, 1]
+        v[:, 0] = v[:, ny]
+        v[:, ny + 2] = v[:, 2]
+        u[:, 0] = u[:, ny]
+        u[:, ny + 1] = u[:, 1]
+
+
+def mass_balance(case: Case, u, v) -> None:
+    if "outlet" not in case.kinds:
+        return
+    nx, ny = case.nx, case.ny
+    influx = outflux = 0.0
+    faces = []
+    for w in range(4):
+        kind = case.kinds[w]
+        if kind == "periodic":
+            continue
+        axis, nf, *_, outward = wall_index(case, w)
+        face = (u[nf, :][1:ny + 1]) if axis == 0 else (v[:, nf][1:nx + 1])
+        width = case.dy if axis == 0 else case.dx
+        flux = outward * (float(face.sum()) * width)
+        if kind == "outlet":
+            outflux += flux
+            faces.append((face, width, outward))
+        else:
+            influx -= flux
+    if outflux > 1.0e-12:
+        for face, _, _ in faces:
+            face *= influx / outflux
+    else:
+        area = sum(face.size * width for face, width, _ in faces)
+        for face, _, outward in faces:
+            face += outward * (influx - outflux) / area
+
+
+def upd_ranges(case: Case):
+    """solver.py:251-263."""
+    nx, ny = case.nx, case.ny
+    k = case.kinds
+    u_lo = 1 if (case.px or k[0] == "pressure_outlet") else 2
+    u_hi = nx + 2 if (k[1] == "pressure_outlet" and not case.px) else nx + 1
+    v_lo = 1 if (case.py or k[2] == "pressure_outlet") else 2
+    v_hi = ny + 2 if (k[3] == "pressure_outlet" and not case.py) else ny + 1
+    return u_lo, u_hi, v_lo, v_hi
+
+
+def face_masks(case: Case):
+    """solver.py:62-79, 276-283."""
+    if case.solid is None:
+        return None, None
+    u_lo, u_hi, v_lo, v_hi = upd_ranges(case)
+    s = case.solid
+
+    def faces(lo, hi, axis, periodic):
+        m = np.arange(lo, hi)
+        if periodic:
+            n = s.shape[axis]
+            return np.take(s, (m - 2) % n, axis=axis) | np.take(s, (m - 1) % n, axis=axis)
+        pad = [(0, 0), (0, 0)]
+        pad[axis] = (1, 1)
+        q = np.pad(s, pad, constant_values=False)
+        return np.take(q, m - 1, axis=axis) | np.take(q, m, axis=axis)
+
+    return faces(u_lo, u_hi, 0, case.px), faces(v_lo, v_hi, 1, case.py)
+
+
+@dataclass
+class State:
+    """Mutable solver state carried across substeps (solver.py:169-179)."""
+
+    p_guess: np.ndarray | None = None
+    force: tuple = (0.0, 0.0)
+    last_iterations: int = 0
+    last_residual: float = 0.0
+    hierarchy: object = None
+    system: object = None
+    lu: object = None
+    extra: dict = field(default_factory=dict)
+
+
+def mask(case: Case, st: State, u, v, dt, reset: bool = False) -> None:
+    """solver.py:553-599 (single body)."""
+    if case.solid is None:
+        return
+    u_lo, u_hi, v_lo, v_hi = upd_ranges(case)
+    um, vm = face_masks(case)
+    uf = u[u_lo:u_hi, 1:case.ny + 1]
+    vf = v[1:case.nx + 1, v_lo:v_hi]
+    if dt is not None:
+        area = case.dx * case.dy
+        fx = float(uf[um].sum()) * area / dt
+        fy = float(vf[vm].sum()) * area / dt
+        st.force = (fx, fy) if reset else (st.force[0] + fx, st.force[1] + fy)
+    uf[um] = 0.0
+    vf[vm] = 0.0
+
+
+def wrap_faces(case: Case, u, v) -> None:
+    if case.px:
+        u[case.nx + 1, :] = u[1, :]
+    if case.py:
+        v[:, case.ny + 1] = v[:, 1]
+
+
+def apply_pressure(case: Case, p) -> None:
+    """boundary.py:166-169, 293-303, 412-420."""
+    nx, ny = case.nx, case.ny
+    for w in range(4):
+        kind = case.kinds[w]
+        if kind == "periodic":
+            continue
+        axis, *_, pg, pi, _ = wall_index(case, w)
+        if kind == "pressure_outlet":
+            _put(p, axis, pg, 2.0 * case.p_ref[w] - _get(p, axis, pi))
+        else:
+            _put(p, axis, pg, _get(p, axis, pi))
+    if case.px:
+        p[0, :] = p[nx, :]
+        p[nx + 1, :] = p[1, :]
+    if case.py:
+        p[:, 0] = p[:, ny]
+        p[:, ny + 1] = p[:, 1]
+
+
+# --------------------------------------------------------------------------
+# Poisson operator: pressure.py:110-270
+# --------------------------------------------------------------------------
+@dataclass
+class System:
+    A: sp.csr_matrix
+    fluid: np.ndarray
+    singular: bool
+    pin: int
+    bc_rhs: np.ndarray
+
+
+def assemble(case: Case) -> System:
+    nx, ny = case.nx, case.ny
+    n = nx * ny
+    solid = np.zeros((nx, ny), bool) if case.solid is None else case.solid
+    fluid = ~solid
+    cx, cy = 1.0 / (case.dx * case.dx), 1.0 / (case.dy * case.dy)
+    dirichlet = {w: case.p_ref[w] for w in range(4) if case.kinds[w] == "pressure_outlet"}
+    ii, jj = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    diag = np.zeros((nx, ny))
+    bc = np.zeros((nx, ny))
+    rows, cols, vals = [], [], []
+    for w, (di, dj, c, per) in enumerate(((-1, 0, cx, case.px), (1, 0, cx, case.px),
+                                          (0, -1, cy, case.py), (0, 1, cy, case.py))):
+        ni, nj = ii + di, jj + dj
+        if per:
+            ni, nj = ni % nx, nj % ny
+        inside = (ni >= 0) & (ni < nx) & (nj >= 0) & (nj < ny)
+        nic, njc = np.clip(ni, 0, nx - 1), np.clip(nj, 0, ny - 1)
+        act = fluid & inside & fluid[nic, njc]
+        rows.append((ii * ny + jj)[act])
+        cols.append((nic * ny + njc)[act])
+        vals.append(np.full(int(act.sum()), c))
+        diag -= c * act
+        if w in dirichlet:
+            on = fluid & ~inside
+            diag -= 2.0 * c * on
+            bc -= 2.0 * c * dirichlet[w] * on
+    solv = fluid & (diag != 0.0)
+    diag = np.where(solv, diag, 1.0)
+    bc = np.where(solv, bc, 0.0)
+    rows.append(np.arange(n))
+    cols.append(np.arange(n))
+    vals.append(diag.ravel())
+    A = sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(n, n)).tocsr()
+    A.sum_duplicates()
+    f = solv.ravel()
+    pin = int(np.flatnonzero(f)[0]) if f.any() else 0
+    return System(A, f, not dirichlet, pin, bc.ravel())
+
+
+def prepare_rhs(sysm: System, rhs):
+    """pressure.py:300-308."""
+    b = np.asarray(rhs, float).ravel() * 1.0 + sysm.bc_rhs
+    b[~sysm.fluid] = 0.0
+    if sysm.singular:
+        b[sysm.fluid] -= b[sysm.fluid].mean()
+    return b
+
+
+def normalize(sysm: System, p, shape):
+    p[~sysm.fluid] = 0.0
+    if sysm.singular and sysm.fluid.any():
+        p[sysm.fluid] -= p[sysm.fluid].mean()
+    return p.reshape(shape)
+
+
+def residual_norm(sysm: System, p, b) -> float:
+    r = b - sysm.A @ p
+    nb = float(np.linalg.norm(b))
+    return float(np.linalg.norm(r)) / (nb if nb > 1e-300 else 1.0)
+
+
+# --------------------------------------------------------------------------
+# Aggregation multigrid: multigrid.py:129-543
+# --------------------------------------------------------------------------
+class Hierarchy:
+    def __init__(self, A, shape, active, singular, sweeps=1):
+        self.sweeps = sweeps
+        self.levels = []
+        A = sp.csr_matrix(A)
+        lev = self._level(A, tuple(shape), np.asarray(active, bool), singular, None)
+        self.levels.append(lev)
+        while len(self.levels) < 20 and lev["A"].shape[0] > 96:
+            nx, ny = lev["shape"]
+            fx, fy = self._factors(lev["A"], lev["shape"])
+            ncx, ncy = max(-(-nx // fx), 1), max(-(-ny // fy), 1)
+            i, j = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+            labels = np.where(lev["active"], ((i // fx) * ncy + (j // fy)).ravel(), -1)
+            cact = np.zeros(ncx * ncy, bool)
+            cact[labels[labels >= 0]] = True
+            if lev["A"].shape[0] < 1.3 * cact.size:
+                break
+            rows = np.flatnonzero(labels >= 0)
+            P = sp.coo_matrix((np.ones(rows.size), (rows, labels[rows])),
+                              shape=(labels.size, cact.size)).tocsr()
+            P = (P - sp.diags(lev["weight"] * lev["dinv"]) @ (lev["A"] @ P)).tocsr()
+            Ac = sp.csr_matrix(P.T @ lev["A"] @ P)
+            if not cact.all():
+                Ac = (Ac + sp.diags((~cact).astype(float))).tocsr()
+            Ac.sum_duplicates()
+            lev = self._level(Ac, (ncx, ncy), cact, singular, P)
+            self.levels.append(lev)
+        last = self.levels[-1]
+        last["pin"] = int(np.flatnonzero(last["active"])[0]) if last["active"].any() else 0
+        M = last["A"]
+        if singular and M.shape[0]:
+            M = M.tolil(copy=True)
+            M.rows[last["pin"]] = [last["pin"]]
+            M.data[last["pin"]] = [1.0]
+        last["lu"] = spla.splu(sp.csc_matrix(M))
+
+    @staticmethod
+    def _factors(A, shape):
+        nx, ny = shape
+
+        def strength(off):
+            if off >= A.shape[0]:
+                return 0.0
+            band = np.abs(A.diagonal(off))
+            return float(band.mean()) if band.size else 0.0
+
+        sx, sy = strength(ny), strength(1)
+        if nx >= 2 and sx > 2.0 * sy:
+            return 2, 1
+        if ny >= 2 and sy > 2.0 * sx:
+            return 1, 2
+        return (2 if nx >= 2 else 1), (2 if ny >= 2 else 1)
+
+    @staticmethod
+    def _level(A, shape, active, singular, P):
+        d = A.diagonal()
+        dinv = 1.0 / np.where(d == 0.0, 1.0, d)
+        x = np.random.default_rng(0).standard_normal(A.shape[0])
+        x /= np.linalg.norm(x)
+        rho = 1.0
+        for _ in range(12):
+            y = dinv * (A @ x)
+            nrm = float(np.linalg.norm(y))
+            if nrm == 0.0:
+                break
+            rho = nrm
+            x = y / nrm
+        rho *= 1.05
+        w = (4.0 / 3.0) / rho if rho > 0 else 1.0
+        return {"A": A, "dinv": dinv, "weight": w, "wd": w * dinv, "active": active,
+                "shape": shape, "P": P, "singular": singular}
+
+    def _coarsest(self, lev, b):
+        rhs = np.array(b, float)
+        act = lev["active"]
+        if lev["singular"]:
+            rhs[~act] = 0.0
+            rhs[act] -= rhs[act].mean()
+            rhs[lev["pin"]] = 0.0
+        x = lev["lu"].solve(rhs)
+        x[~act] = 0.0
+        if lev["singular"] and act.any():
+            x[act] -= x[act].mean()
+        return x
+
+    def cycle(self, depth, b, x):
+        lev = self.levels[depth]
+        if depth == len(self.levels) - 1:
+            x[:] = self._coarsest(lev, b)
+            return
+        for _ in range(self.sweeps):
+            x += lev["wd"] * (b - lev["A"] @ x)
+        P = self.levels[depth + 1]["P"]
+        xc = np.zeros(P.shape[1])
+        self.cycle(depth + 1, P.T @ (b - lev["A"] @ x), xc)
+        x += P @ xc
+        for _ in range(self.sweeps):
+            x += lev["wd"] * (b - lev["A"] @ x)
+
+    def project(self, v):
+        top = self.levels[0]
+        v[~top["active"]] = 0.0
+        if top["singular"] and top["active"].any():
+            v[top["active"]] -= v[top["active"]].mean()
+
+    def apply(self, r):
+        b = np.array(r, float).ravel()
+        self.project(b)
+        x = np.zeros_like(b)
+        self.cycle(0, b, x)
+        self.project(x)
+        return x
+
+
+def solve(case: Case, st: State, rhs, p0=None):
+    """The selected solver on lap(p) = rhs (pressure.py:330-632); returns (nx, ny)."""
+    if st.system is None:
+        st.system = assemble(case)
+    S = st.system
+    shape = (case.nx, case.ny)
+    kind, tol, maxiter = case.pressure_solver, case.tol, case.maxiter
+    b = prepare_rhs(S, rhs)
+    x0 = None if p0 is None else np.asarray(p0, float).ravel().copy()
+    if kind == "direct":
+        if st.lu is None:
+            A = S.A
+            if S.singular:
+                A = A.tolil(copy=True)
+                A.rows[S.pin] = [S.pin]
+                A.data[S.pin] = [1.0]
+            st.lu = spla.splu(sp.csc_matrix(A))
+        bp = b.copy()
+        if S.singular:
+            bp[S.pin] = 0.0
+        p = st.lu.solve(bp)
+        st.last_iterations, st.last_residual = 1, residual_norm(S, p, b)
+        return normalize(S, p, shape)
+    if kind in ("multigrid", "mgcg") and st.hierarchy is None:
+        st.hierarchy = Hierarchy(S.A, shape, S.fluid, S.singular, case.mg_sweeps)
+    if kind in ("cg", "mgcg"):
+        if kind == "cg":
+            d = S.A.diagonal()
+            M = spla.LinearOperator(S.A.shape, matvec=lambda y: y / d, dtype=float)
+        else:
+            M = spla.LinearOperator(S.A.shape, matvec=st.hierarchy.apply, dtype=float)
+        its = [0]
+        p, info = spla.cg(S.A, b, x0=x0, rtol=tol, atol=0.0, maxiter=maxiter, M=M,
+                          callback=lambda _x: its.__setitem__(0, its[0] + 1))
+        st.last_iterations, st.last_residual = its[0], residual_norm(S, p, b)
+        if info != 0 and st.last_residual > max(tol * 10, 1e-8):
+            raise RuntimeError(f"{kind} failed to converge ({st.last_residual:.3e})")
+        return normalize(S, p, shape)
+    fluid = S.fluid
+    if kind in ("sor", "jacobi"):
+        p = np.zeros_like(b) if x0 is None else x0
+        p[~fluid] = 0.0
+        res = residual_norm(S, p, b)
+        every = 5 if kind == "sor" else 20
+        if kind == "sor":
+            nx, ny = shape
+            col = ((np.arange(nx)[:, None] + np.arange(ny)[None, :]) % 2).ravel()
+            red, blk = np.flatnonzero(col == 0), np.flatnonzero(col == 1)
+            A = S.A.tocsr()
+            Arb, Abr = A[red][:, blk], A[blk][:, red]
+            dd = A.diagonal()
+            w = case.sor_omega
+        else:
+            dinv = 1.0 / S.A.diagonal()
+            w = 2.0 / 3.0
+        it = 0
+        for it in range(1, maxiter + 1):
+            if kind == "sor":
+                p[red] = (1 - w) * p[red] + w * (b[red] - Arb @ p[blk]) / dd[red]
+                p[blk] = (1 - w) * p[blk] + w * (b[blk] - Abr @ p[red]) / dd[blk]
+            else:
+                p += w * dinv * (b - S.A @ p)
+            if S.singular:
+                p[fluid] -= p[fluid].mean()
+            if it % every == 0 or it == maxiter:
+                res = residual_norm(S, p, b)
+                if res < tol:
+                    break
+        st.last_iterations, st.last_residual = it, res
+        if res > tol:
+            raise RuntimeError(f"{kind} stalled ({res:.3e})")
+        return normalize(S, p, shape)
+    if kind == "multigrid":
+        H = st.hierarchy
+        bb = b.copy()
+        H.project(bb)
+        x = np.zeros_like(bb) if x0 is None else x0
+        H.project(x)
+        res = residual_norm(S, x, bb)
+        cycles = stalled = 0
+        while cycles < maxiter and res >= tol and stalled < 3:
+            prev = res
+            H.cycle(0, bb, x)
+            H.project(x)
+            res = residual_norm(S, x, bb)
+            cycles += 1
+            stalled = stalled + 1 if res > 0.995 * prev else 0
+        st.last_iterations, st.last_residual = cycles, res
+        if res > tol:
+            raise RuntimeError(f"multigrid stalled ({res:.3e})")
+        return normalize(S, x, shape)
+    raise ValueError(f"unknown pressure solver {kind!r}")
+
+
+WARM = {"direct": False, "cg": True, "jacobi": True, "sor": True, "multigrid": True,
+        "mgcg": True}
+
+
+def project(case: Case, st: State, u, v, dt):
+    """solver.py:643-683: u, v corrected in place; returns ghosted p."""
+    nx, ny = case.nx, case.ny
+    p_int = solve(case, st, divergence(case, u, v) / dt, st.p_guess)
+    if WARM[case.pressure_solver]:
+        st.p_guess = p_int
+    p = np.zeros((nx + 2, ny + 2))
+    p[1:nx + 1, 1:ny + 1] = p_int
+    apply_pressure(case, p)
+    u_lo, u_hi, v_lo, v_hi = upd_ranges(case)
+    u[u_lo:u_hi, 1:ny + 1] -= dt * (p[u_lo:u_hi, 1:ny + 1] - p[u_lo - 1:u_hi - 1, 1:ny + 1]) / case.dx
+    v[1:nx + 1, v_lo:v_hi] -= dt * (p[1:nx + 1, v_lo:v_hi] - p[1:nx + 1, v_lo - 1:v_hi - 1]) / case.dy
+    return p
+
+
+def initialize(case: Case, st: State, u_init=None, v_init=None):
+    """solver.py:301-329."""
+    nx, ny = case.nx, case.ny
+    u = np.zeros((nx + 3, ny + 2))
+    v = np.zeros((nx + 2, ny + 3))
+    if u_init is not None:
+        u[1:nx + 2, 1:ny + 1] = u_init
+    if v_init is not None:
+        v[1:nx + 1, 1:ny + 2] = v_init
+    apply_velocity(case, u, v, True)
+    mass_balance(case, u, v)
+    mask(case, st, u, v, None)
+    wrap_faces(case, u, v)
+    p = project(case, st, u, v, 1.0)
+    mask(case, st, u, v, None)
+    wrap_faces(case, u, v)
+    apply_velocity(case, u, v, False)
+    return u, v, p
+
+
+def substep(case: Case, st: State, u, v, dt, blend=0.0):
+    """solver.py:612-640."""
+    ru, rv = momentum_rhs(case, u, v, blend)
+    u_lo, u_hi, v_lo, v_hi = upd_ranges(case)
+    us, vs = u.copy(), v.copy()
+    us[u_lo:u_hi, 1:case.ny + 1] += dt * ru[u_lo - 1:u_hi - 1, :]
+    vs[1:case.nx + 1, v_lo:v_hi] += dt * rv[:, v_lo - 1:v_hi - 1]
+    apply_velocity(case, us, vs, True)
+    mass_balance(case, us, vs)
+    mask(case, st, us, vs, dt, reset=True)
+    wrap_faces(case, us, vs)
+    p = project(case, st, us, vs, dt)
+    mask(case, st, us, vs, dt, reset=False)
+    wrap_faces(case, us, vs)
+    apply_velocity(case, us, vs, False)
+    return us, vs, p
+
+
+def step(case: Case, st: State, u0, v0, dt, blend=0.0):
+    """solver.py:686-731."""
+    if case.scheme == "euler":
+        return substep(case, st, u0, v0, dt, blend)
+    if case.scheme == "rk2":
+        u1, v1, _ = substep(case, st, u0, v0, dt, blend)
+        u2, v2, p = substep(case, st, u1, v1, dt, blend)
+        u, v = 0.5 * u0 + 0.5 * u2, 0.5 * v0 + 0.5 * v2
+    else:
+        u1, v1, _ = substep(case, st, u0, v0, dt, blend)
+        ue, ve, _ = substep(case, st, u1, v1, dt, blend)
+        u2, v2 = 0.75 * u0 + 0.25 * ue, 0.75 * v0 + 0.25 * ve
+        ue2, ve2, p = substep(case, st, u2, v2, dt, blend)
+        u = (1.0 / 3.0) * u0 + (2.0 / 3.0) * ue2
+        v = (1.0 / 3.0) * v0 + (2.0 / 3.0) * ve2
+    apply_velocity(case, u, v, False)
+    wrap_faces(case, u, v)
+    return u, v, p
+
+
+def compute_dt(case: Case, u, v) -> float:
+    """timestepper.py:82-113."""
+    if not case.adaptive_dt:
+        return case.dt
+    nx, ny = case.nx, case.ny
+    umax = float(np.abs(u[1:nx + 2, 1:ny + 1]).max())
+    vmax = float(np.abs(v[1:nx + 1, 1:ny + 2]).max())
+    scale = umax / case.dx + vmax / case.dy
+    dt_conv = case.cfl_max / scale if scale > 1e-12 else float("inf")
+    visc = 0.8 / (2.0 * case.nu * (1.0 / case.dx ** 2 + 1.0 / case.dy ** 2))
+    return float(min(dt_conv, visc, case.dt))
+
+
+def run(case: Case, steps: int, u_init=None, v_init=None, st: State | None = None):
+    """Simulation(...).run(max_steps=steps) (incompressible.py:47-116,
+    timestepper.py:145-266); returns (u, v, p, history, t)."""
+    st = st or State()
+    u, v, p = initialize(case, st, u_init, v_init)
+    t = 0.0
+    hist = {"dt": [], "max_div": [], "kinetic_energy": [], "fx": [], "fy": []}
+    nx, ny = case.nx, case.ny
+    for _ in range(steps):
+        dt = compute_dt(case, u, v)
+        blend = 0.0
+        if case.advection == "upwind":
+            if case.upwind_blend is not None:
+                blend = case.upwind_blend
+            else:
+                umax = float(np.abs(u[1:nx + 2, 1:ny + 1]).max())
+                vmax = float(np.abs(v[1:nx + 1, 1:ny + 2]).max())
+                blend = float(min(1.0, 1.2 * max(umax * dt / case.dx, vmax * dt / case.dy)))
+        u, v, p = step(case, st, u, v, dt, blend)
+        t += dt
+        d = divergence(case, u, v)
+        if case.solid is not None:
+            d = d[~case.solid]
+        uc = 0.5 * (u[1:nx + 1, 1:ny + 1] + u[2:nx + 2, 1:ny + 1])
+        vc = 0.5 * (v[1:nx + 1, 1:ny + 1] + v[1:nx + 1, 2:ny + 2])
+        hist["dt"].append(dt)
+        hist["max_div"].append(float(np.abs(d).max()))
+        hist["kinetic_energy"].append(0.5 * (float(np.sum(uc ** 2 + vc ** 2)) * (case.dx * case.dy)))
+        hist["fx"].append(st.force[0])
+        hist["fy"].append(st.force[1])
+    return u, v, p, hist, t

@@ -1,0 +1,197 @@
+"""One device context of ``libpycfd_hip.so``: the handle a ProjectionSolver
+(or a standalone pressure solver) drives.  Thin ctypes plumbing only; the
+arithmetic all happens in the HIP kernels."""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from .. import _lib
+
+
+def _ptr(t) -> C.c_void_p:
+    return C.c_void_p(0 if t is None else int(t.data_ptr()))
+
+
+def require_device() -> None:
+    """Fail loudly unless a HIP device is usable through torch."""
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError(
+            "pycfd_amd needs a HIP device: torch.cuda.is_available() is False "
+            "(the HIP engine has no CPU fallback)"
+        )
+
+
+class DeviceContext:
+    """``pcfd_ctx`` plus the arrays it was created from (kept alive here)."""
+
+    def __init__(self, nx: int, ny: int, dx: float, dy: float, nu: float,
+                 body_force=(0.0, 0.0), scheme: str = "rk3",
+                 bc_kinds=("no_slip",) * 4, bc_velocity=(0.0,) * 4, bc_p_ref=(0.0,) * 4,
+                 inlet_lines=(None,) * 4, solid: np.ndarray | None = None,
+                 body_id: np.ndarray | None = None, n_bodies: int = 0) -> None:
+        require_device()
+        import torch
+        self._lib = _lib.load()
+        self.nx, self.ny = int(nx), int(ny)
+        d = _lib.CaseDesc()
+        d.nx, d.ny = self.nx, self.ny
+        d.dx, d.dy, d.nu = float(dx), float(dy), float(nu)
+        d.body_fx, d.body_fy = float(body_force[0]), float(body_force[1])
+        d.time_scheme = _lib.SCHEME_CODES[scheme]
+        self._keep = []
+        for w in range(4):
+            d.bc_kind[w] = _lib.BC_CODES[bc_kinds[w]]
+            d.bc_velocity[w] = float(bc_velocity[w])
+            d.bc_p_ref[w] = float(bc_p_ref[w])
+            line = inlet_lines[w]
+            if line is not None:
+                arr = np.ascontiguousarray(line, dtype=np.float64)
+                self._keep.append(arr)
+                d.inlet_line[w] = arr.ctypes.data_as(C.POINTER(C.c_double))
+        if solid is not None and np.any(solid):
+            sarr = np.ascontiguousarray(solid, dtype=np.uint8)
+            self._keep.append(sarr)
+            d.solid = sarr.ctypes.data_as(C.POINTER(C.c_uint8))
+        if body_id is not None:
+            barr = np.ascontiguousarray(body_id, dtype=np.int8)
+            self._keep.append(barr)
+            d.body_id = barr.ctypes.data_as(C.POINTER(C.c_int8))
+        d.n_bodies = int(n_bodies)
+        self.n_bodies = int(n_bodies) if n_bodies > 1 else 0
+        self.stream = torch.cuda.current_stream()
+        handle = C.c_void_p(0)
+        _lib.check(self._lib.pcfd_ctx_create(C.byref(d), C.c_void_p(self.stream.cuda_stream),
+                                             C.byref(handle)), "pcfd_ctx_create")
+        self.handle = handle
+        self._keep = []
+
+    def close(self) -> None:
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            self._lib.pcfd_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _call(self, name: str, *args) -> None:
+        _lib.check(getattr(self._lib, name)(self.handle, *args), name)
+
+    # -- configuration ---------------------------------------------------
+    def set_solver(self, kind: str, tol: float, maxiter: int, omega: float, sweeps: int,
+                   precision: int, levels: list[dict] | None) -> None:
+        sd = _lib.SolverDesc(_lib.SOLVER_CODES[kind], float(tol), int(maxiter), float(omega),
+                             int(sweeps), int(precision))
+        keep = []
+        if levels:
+            arr = (_lib.MGLevelDesc * len(levels))()
+            for i, L in enumerate(levels):
+                e = arr[i]
+                e.nx, e.ny, e.fx, e.fy = L["nx"], L["ny"], L["fx"], L["fy"]
+                e.weight = float(L["weight"])
+                act = np.ascontiguousarray(L["active"], dtype=np.uint8)
+                keep.append(act)
+                e.active = act.ctypes.data_as(C.POINTER(C.c_uint8))
+                if L.get("table") is not None:
+                    di = np.ascontiguousarray(L["di"], dtype=np.int8)
+                    dj = np.ascontiguousarray(L["dj"], dtype=np.int8)
+                    cls = np.ascontiguousarray(L["cls"], dtype=np.uint16)
+                    tab = np.ascontiguousarray(L["table"], dtype=np.float64)
+                    cdv = np.ascontiguousarray(L["class_dinv"], dtype=np.float64)
+                    cac = np.ascontiguousarray(L["class_active"], dtype=np.uint8)
+                    keep += [di, dj, cls, tab, cdv, cac]
+                    e.n_offsets = di.size
+                    e.di = di.ctypes.data_as(C.POINTER(C.c_int8))
+                    e.dj = dj.ctypes.data_as(C.POINTER(C.c_int8))
+                    e.n_classes = cdv.size
+                    e.cls = cls.ctypes.data_as(C.POINTER(C.c_uint16))
+                    e.table = tab.ctypes.data_as(C.POINTER(C.c_double))
+                    e.class_dinv = cdv.ctypes.data_as(C.POINTER(C.c_double))
+                    e.class_active = cac.ctypes.data_as(C.POINTER(C.c_uint8))
+                if L.get("coarse_inv") is not None:
+                    ci = np.ascontiguousarray(L["coarse_inv"], dtype=np.float64)
+                    keep.append(ci)
+                    e.coarse_n = int(ci.shape[0])
+                    e.coarse_inv = ci.ctypes.data_as(C.POINTER(C.c_double))
+            self._call("pcfd_set_pressure_solver", C.byref(sd), arr, len(levels))
+        else:
+            self._call("pcfd_set_pressure_solver", C.byref(sd), None, 0)
+
+    # -- operators -------------------------------------------------------
+    def momentum_rhs(self, u, v, blend: float, rhs_u, rhs_v) -> None:
+        self._call("pcfd_momentum_rhs", _ptr(u), _ptr(v), float(blend), _ptr(rhs_u),
+                   _ptr(rhs_v))
+
+    def divergence(self, u, v, out) -> None:
+        self._call("pcfd_divergence", _ptr(u), _ptr(v), _ptr(out))
+
+    def apply_velocity_bc(self, u, v, predictor: bool) -> None:
+        self._call("pcfd_apply_velocity_bc", _ptr(u), _ptr(v), int(bool(predictor)))
+
+    def mass_balance(self, u, v) -> None:
+        self._call("pcfd_mass_balance", _ptr(u), _ptr(v))
+
+    def mask_obstacle(self, u, v, dt: float | None, mode: int) -> None:
+        self._call("pcfd_mask_obstacle", _ptr(u), _ptr(v), float(dt or 0.0), int(mode))
+
+    def wrap_faces(self, u, v) -> None:
+        self._call("pcfd_wrap_faces", _ptr(u), _ptr(v))
+
+    def apply_pressure_bc(self, p) -> None:
+        self._call("pcfd_apply_pressure_bc", _ptr(p))
+
+    def pressure_solve(self, rhs, p0, out) -> _lib.SolveInfo:
+        info = _lib.SolveInfo()
+        self._call("pcfd_pressure_solve", _ptr(rhs), _ptr(p0), _ptr(out), C.byref(info))
+        return info
+
+    def project(self, u, v, dt: float, p) -> _lib.SolveInfo:
+        info = _lib.SolveInfo()
+        self._call("pcfd_project", _ptr(u), _ptr(v), float(dt), _ptr(p), C.byref(info))
+        return info
+
+    def initialize(self, u, v, p) -> _lib.SolveInfo:
+        info = _lib.SolveInfo()
+        self._call("pcfd_initialize", _ptr(u), _ptr(v), _ptr(p), C.byref(info))
+        return info
+
+    def step(self, u0, v0, dt: float, blend: float, u, v, p, want_info: bool = True):
+        info = _lib.SolveInfo() if want_info else None
+        self._call("pcfd_step", _ptr(u0), _ptr(v0), float(dt), float(blend), _ptr(u), _ptr(v),
+                   _ptr(p), C.byref(info) if info is not None else None)
+        return info
+
+    def reset_guess(self) -> None:
+        self._call("pcfd_reset_guess")
+
+    def diagnostics(self, u, v, p, u_prev=None, v_prev=None) -> np.ndarray:
+        out = np.zeros(16)
+        self._call("pcfd_diagnostics", _ptr(u), _ptr(v), _ptr(p), _ptr(u_prev), _ptr(v_prev),
+                   out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def force(self) -> np.ndarray:
+        out = np.zeros(2 + 2 * self.n_bodies)
+        self._call("pcfd_get_force", out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def poisson_apply(self, x, y) -> None:
+        self._call("pcfd_poisson_apply", _ptr(x), _ptr(y))
+
+    def mg_apply(self, r, z) -> None:
+        self._call("pcfd_mg_apply", _ptr(r), _ptr(z))
+
+    def time_kernel(self, which: int, reps: int) -> tuple[float, float]:
+        """(mean ms per launch, algorithmic bytes per launch) of one hot kernel."""
+        ms, b = C.c_double(0), C.c_double(0)
+        self._call("pcfd_time_kernel", int(which), int(reps), C.byref(ms), C.byref(b))
+        return float(ms.value), float(b.value)
+
+    def synchronize(self) -> None:
+        self._call("pcfd_synchronize")

@@ -1,0 +1,281 @@
+"""Aggregation-multigrid hierarchy: host setup, device V-cycle.
+
+The hierarchy is built once per operator, as in ``pycfd/core/multigrid.py``:
+geometric 2x2 / 2x1 / 1x2 aggregates that keep each level a rectangular grid
+with holes (:167-245), the 0/1 tentative prolongator smoothed by one damped
+Jacobi pass (:248-273), Galerkin coarse operators with the identity on
+dropped aggregates (:276-287, 380-383), damping weights from a seeded
+12-step power iteration (:129-157, 354-362), and the coarsest level solved
+exactly as a pinned solve followed by mean removal (:514-543).
+
+Because every level is a structured grid, each coarse operator is handed to
+the GPU as a structured stencil (one coefficient plane per stencil offset,
+coalesced along y) rather than as CSR, and the prolongator is never stored:
+the device applies ``P = (I - w D^-1 A) P_tent`` and ``P^T`` on the fly from
+the level's own operator.  The coarsest level becomes a dense FP64
+pseudo-inverse applied by one workgroup.
+"""
+
+from __future__ import annotations
+
+import logging
+from dataclasses import dataclass
+
+import numpy as np
+import scipy.sparse as sp
+
+log = logging.getLogger(__name__)
+
+COARSEST_UNKNOWNS = 96
+MAX_LEVELS = 20
+MIN_COARSENING_RATIO = 1.3
+JACOBI_WEIGHT_FACTOR = 4.0 / 3.0
+POWER_ITERATIONS = 12
+STALL_FACTOR = 0.995
+STALL_PATIENCE = 3
+ANISOTROPY_THRESHOLD = 2.0
+
+
+def spectral_radius(A: sp.csr_matrix, dinv: np.ndarray, seed: int = 0) -> float:
+    """Power-iteration estimate of rho(D^-1 A), nudged up 5% (multigrid.py:129-157)."""
+    n = A.shape[0]
+    if n == 0:
+        return 1.0
+    x = np.random.default_rng(seed).standard_normal(n)
+    nrm = float(np.linalg.norm(x))
+    if nrm == 0.0:
+        return 1.0
+    x /= nrm
+    rho = 1.0
+    for _ in range(POWER_ITERATIONS):
+        y = dinv * (A @ x)
+        nrm = float(np.linalg.norm(y))
+        if nrm == 0.0:
+            return rho
+        rho = nrm
+        x = y / nrm
+    return rho * 1.05
+
+
+def coarsening_factors(A: sp.csr_matrix, shape: tuple[int, int]) -> tuple[int, int]:
+    """Full or semi-coarsening from the operator's axis strengths (multigrid.py:167-204)."""
+    nx, ny = shape
+    if nx < 2 and ny < 2:
+        return 1, 1
+
+    def strength(offset: int) -> float:
+        if offset >= A.shape[0]:
+            return 0.0
+        band = np.abs(A.diagonal(offset))
+        return float(band.mean()) if band.size else 0.0
+
+    sx, sy = strength(ny), strength(1)
+    if nx >= 2 and sx > ANISOTROPY_THRESHOLD * sy:
+        return 2, 1
+    if ny >= 2 and sy > ANISOTROPY_THRESHOLD * sx:
+        return 1, 2
+    return (2 if nx >= 2 else 1), (2 if ny >= 2 else 1)
+
+
+def coarse_shape(shape, factors) -> tuple[int, int]:
+    return tuple(max(-(-n // f), 1) for n, f in zip(shape, factors))  # type: ignore[return-value]
+
+
+def aggregate_blocks(shape, active: np.ndarray, factors):
+    """Block labels (-1 on inactive cells) and the coarse active mask (multigrid.py:212-245)."""
+    nx, ny = shape
+    fx, fy = factors
+    ncx, ncy = coarse_shape(shape, factors)
+    i, j = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    block = ((i // fx) * ncy + (j // fy)).ravel()
+    labels = np.where(np.asarray(active).ravel(), block, -1)
+    coarse_active = np.zeros(ncx * ncy, dtype=bool)
+    coarse_active[labels[labels >= 0]] = True
+    return labels, coarse_active
+
+
+def tentative_prolongator(labels: np.ndarray, n_coarse: int) -> sp.csr_matrix:
+    rows = np.flatnonzero(labels >= 0)
+    return sp.coo_matrix((np.ones(rows.size), (rows, labels[rows])),
+                         shape=(labels.size, n_coarse)).tocsr()
+
+
+def smooth_prolongator(A, dinv, weight, P) -> sp.csr_matrix:
+    return (P - sp.diags(weight * dinv) @ (A @ P)).tocsr()
+
+
+def _identity_on_inactive(A: sp.csr_matrix, active: np.ndarray) -> sp.csr_matrix:
+    if active.all():
+        return A
+    return (A + sp.diags((~active).astype(float))).tocsr()
+
+
+@dataclass
+class Level:
+    A: sp.csr_matrix
+    dinv: np.ndarray
+    weight: float
+    active: np.ndarray
+    shape: tuple[int, int]
+    singular: bool
+    factors: tuple[int, int] = (1, 1)   # towards the next coarser level
+    pin: int = 0
+
+    @property
+    def n(self) -> int:
+        return self.A.shape[0]
+
+
+def to_class_stencil(A: sp.csr_matrix, shape: tuple[int, int], periodic: tuple[bool, bool],
+                     active: np.ndarray):
+    """CSR operator on a rectangular grid -> stencil dictionary.
+
+    Returns offsets ``(di, dj)``, the per-cell class index, the class table
+    (one row of coefficients per distinct operator row) and, per class, the
+    diagonal inverse and the active flag.
+    """
+    nx, ny = shape
+    coo = A.tocoo()
+    r, c, v = coo.row, coo.col, coo.data
+    keep = v != 0.0
+    r, c, v = r[keep], c[keep], v[keep]
+    ri, rj = np.divmod(r, ny)
+    ci, cj = np.divmod(c, ny)
+    di, dj = ci - ri, cj - rj
+    if periodic[0] and nx > 1:
+        di = (di + nx // 2) % nx - nx // 2
+    if periodic[1] and ny > 1:
+        dj = (dj + ny // 2) % ny - ny // 2
+    key = di.astype(np.int64) * 4096 + dj
+    uniq, inv = np.unique(key, return_inverse=True)
+    udi = np.round(uniq / 4096).astype(np.int64)
+    udj = uniq - udi * 4096
+    if np.abs(udi).max(initial=0) > 127 or np.abs(udj).max(initial=0) > 127:
+        raise ValueError("coarse stencil offset exceeds the int8 range")
+    S = uniq.size
+    rows = np.zeros((nx * ny, S + 1))
+    rows[r, inv] = v
+    rows[:, S] = np.asarray(active, dtype=float).ravel()
+    table, cls = np.unique(rows, axis=0, return_inverse=True)
+    if table.shape[0] > 65535:
+        raise ValueError(f"{table.shape[0]} distinct operator rows exceed the class index range")
+    centre = int(np.flatnonzero((udi == 0) & (udj == 0))[0])
+    diag = table[:, centre]
+    cdinv = 1.0 / np.where(diag == 0.0, 1.0, diag)
+    cactive = table[:, S] != 0.0
+    return (udi.astype(np.int8), udj.astype(np.int8), cls.ravel().astype(np.uint16),
+            np.ascontiguousarray(table[:, :S]), cdinv, cactive.astype(np.uint8))
+
+
+def coarsest_pseudo_inverse(level: Level) -> np.ndarray:
+    """Dense operator of the coarsest solve, projections included (multigrid.py:514-543)."""
+    n = level.n
+    A = level.A.toarray()
+    act = level.active
+    if level.singular and n:
+        M = A.copy()
+        M[level.pin, :] = 0.0
+        M[level.pin, level.pin] = 1.0
+        na = max(int(act.sum()), 1)
+        proj = np.zeros((n, n))
+        ia = np.flatnonzero(act)
+        proj[np.ix_(ia, ia)] = np.eye(ia.size) - 1.0 / na
+        pin_zero = np.eye(n)
+        pin_zero[level.pin, level.pin] = 0.0
+        return proj @ np.linalg.inv(M) @ pin_zero @ proj
+    zero_inactive = np.diag(act.astype(float))
+    return zero_inactive @ np.linalg.inv(A)
+
+
+class MultigridHierarchy:
+    """A V-cycle hierarchy for one operator (multigrid.py:324-474)."""
+
+    def __init__(self, A: sp.csr_matrix, shape: tuple[int, int], active: np.ndarray,
+                 singular: bool, presmooth: int = 1, postsmooth: int = 1,
+                 periodic: tuple[bool, bool] = (False, False)) -> None:
+        if presmooth < 1 or postsmooth < 1:
+            raise ValueError(
+                "a V-cycle needs at least one smoothing sweep on each side, got "
+                f"pre={presmooth}, post={postsmooth}")
+        if presmooth != postsmooth:
+            raise ValueError(
+                f"a V-cycle with pre={presmooth} and post={postsmooth} sweeps is "
+                "not a symmetric operator, and conjugate gradient needs one; use "
+                "an equal number on each side")
+        self.presmooth = presmooth
+        self.postsmooth = postsmooth
+        self.periodic = (bool(periodic[0]), bool(periodic[1]))
+        self.levels: list[Level] = []
+        self._build(sp.csr_matrix(A), tuple(shape), np.asarray(active, dtype=bool).ravel(),
+                    bool(singular))
+        self._device = None
+
+    def _make_level(self, A, shape, active, singular) -> Level:
+        diag = A.diagonal()
+        dinv = 1.0 / np.where(diag == 0.0, 1.0, diag)
+        rho = spectral_radius(A, dinv)
+        weight = JACOBI_WEIGHT_FACTOR / rho if rho > 0.0 else 1.0
+        return Level(A=A, dinv=dinv, weight=weight, active=active, shape=shape,
+                     singular=singular)
+
+    def _build(self, A, shape, active, singular) -> None:
+        level = self._make_level(A, shape, active, singular)
+        self.levels.append(level)
+        while len(self.levels) < MAX_LEVELS and level.n > COARSEST_UNKNOWNS:
+            fx, fy = coarsening_factors(level.A, level.shape)
+            labels, coarse_active = aggregate_blocks(level.shape, level.active, (fx, fy))
+            next_shape = coarse_shape(level.shape, (fx, fy))
+            n_coarse = coarse_active.size
+            if n_coarse == 0 or level.n < MIN_COARSENING_RATIO * n_coarse:
+                break
+            P = tentative_prolongator(labels, n_coarse)
+            P = smooth_prolongator(level.A, level.dinv, level.weight, P)
+            coarse_A = _identity_on_inactive(sp.csr_matrix(P.T @ level.A @ P), coarse_active)
+            coarse_A.sum_duplicates()
+            level.factors = (fx, fy)
+            level = self._make_level(coarse_A, next_shape, coarse_active, singular)
+            self.levels.append(level)
+        last = self.levels[-1]
+        last.pin = int(np.flatnonzero(last.active)[0]) if last.active.any() else 0
+        log.debug("multigrid hierarchy: %s", self)
+
+    @property
+    def n_levels(self) -> int:
+        return len(self.levels)
+
+    @property
+    def unknowns(self) -> list[int]:
+        return [lv.n for lv in self.levels]
+
+    @property
+    def operator_complexity(self) -> float:
+        fine = self.levels[0].A.nnz
+        return sum(lv.A.nnz for lv in self.levels) / fine if fine else 1.0
+
+    def __repr__(self) -> str:
+        return (f"MultigridHierarchy({self.n_levels} levels, {self.unknowns[0]} -> "
+                f"{self.unknowns[-1]} unknowns, complexity {self.operator_complexity:.2f})")
+
+    def device_levels(self) -> list[dict]:
+        """The hierarchy in the form ``pcfd_set_pressure_solver`` takes."""
+        if self._device is not None:
+            return self._device
+        out = []
+        for l, lv in enumerate(self.levels):
+            d = {"nx": lv.shape[0], "ny": lv.shape[1], "fx": lv.factors[0],
+                 "fy": lv.factors[1], "weight": lv.weight,
+                 "active": lv.active.astype(np.uint8)}
+            if l > 0:
+                di, dj, cls, table, cdinv, cact = to_class_stencil(
+                    lv.A, lv.shape, self.periodic, lv.active)
+                d.update(di=di, dj=dj, cls=cls, table=table, class_dinv=cdinv,
+                         class_active=cact)
+            if l == len(self.levels) - 1:
+                d["coarse_inv"] = coarsest_pseudo_inverse(lv)
+            out.append(d)
+        self._device = out
+        return out
+
+    def stencil_sizes(self) -> list[tuple[int, int]]:
+        """(offsets, distinct rows) per coarse level."""
+        return [(int(d["di"].size), int(d["class_dinv"].size)) for d in self.device_levels()[1:]]

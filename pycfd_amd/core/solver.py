@@ -1,0 +1,256 @@
+"""Fractional-step solver: the reference's ``ProjectionSolver`` on the GPU.
+
+Same constructor, attributes and methods as ``pycfd/core/solver.py:82-756``.
+Setup (face ranges, body face masks, Poisson operator, multigrid hierarchy)
+happens once on the host; ``initialize``, ``project`` and ``step`` run
+entirely in HIP kernels on device-resident FP64 fields.  ``momentum_rhs``,
+``divergence`` and ``project`` also accept NumPy arrays, so tests written for
+the reference can call them unchanged.
+"""
+
+from __future__ import annotations
+
+import logging
+from collections.abc import Sequence
+
+import numpy as np
+
+from ..config import AdvectionScheme, BCKind, SimulationConfig, SolverType
+from .boundary import BoundaryManager
+from .engine import DeviceContext
+from .fields import FlowField, device_zeros, to_device, to_host
+from .mesh import NonUniformMeshError, StructuredMesh
+from .pressure import assemble_poisson_matrix, make_pressure_solver
+
+log = logging.getLogger(__name__)
+
+
+def _face_solid_mask(solid: np.ndarray, lo: int, hi: int, axis: int,
+                     periodic: bool) -> np.ndarray:
+    """Faces in ``[lo, hi)`` touching a solid cell; face m sits between cells
+    m-2 and m-1 (solver.py:62-79)."""
+    m = np.arange(lo, hi)
+    if periodic:
+        n = solid.shape[axis]
+        return np.take(solid, (m - 2) % n, axis=axis) | np.take(solid, (m - 1) % n, axis=axis)
+    pad = [(0, 0), (0, 0)]
+    pad[axis] = (1, 1)
+    padded = np.pad(solid, pad, constant_values=False)
+    return np.take(padded, m - 1, axis=axis) | np.take(padded, m, axis=axis)
+
+
+class ProjectionSolver:
+    """Chorin projection with SSP-RK3 on the staggered grid, on one MI355X."""
+
+    def __init__(self, config: SimulationConfig, mesh: StructuredMesh | None = None,
+                 obstacle: np.ndarray | None = None,
+                 bodies: Sequence[np.ndarray] | None = None) -> None:
+        if config.solver_type is SolverType.SIMPLE:
+            raise NotImplementedError(
+                "solver_type='simple' is reserved for a future SIMPLE "
+                "(pressure-correction) implementation; only 'projection' is "
+                "available. Set solver_type='projection'.")
+        self.config = config
+        self.mesh = mesh if mesh is not None else StructuredMesh.from_config(config)
+        if not self.mesh.is_uniform:
+            raise NonUniformMeshError(
+                "the GPU projection step is discretised for uniform spacing; "
+                "stretched meshes are not supported (stretch_x = stretch_y = 1.0)")
+        if config.use_les:
+            raise NotImplementedError("the Smagorinsky closure (use_les=True) is not "
+                                      "available on the GPU path")
+        self.nu = config.nu
+        self.boundaries = BoundaryManager(config.boundary_config, self.mesh)
+        self.periodic_x = self.boundaries.periodic_x
+        self.periodic_y = self.boundaries.periodic_y
+        nx, ny = self.mesh.shape
+        if obstacle is None:
+            self.solid = np.zeros((nx, ny), dtype=bool)
+        else:
+            self.solid = np.asarray(obstacle, dtype=bool)
+            if self.solid.shape != (nx, ny):
+                raise ValueError(
+                    f"obstacle mask must have shape {(nx, ny)}, got {self.solid.shape}")
+        self.has_obstacle = bool(self.solid.any())
+        self.body_masks: tuple[np.ndarray, ...] = ()
+        if bodies is not None and len(bodies) > 1:
+            self.body_masks = tuple(np.asarray(b, dtype=bool) for b in bodies)
+            for k, b in enumerate(self.body_masks):
+                if b.shape != (nx, ny):
+                    raise ValueError(f"body {k} mask must have shape {(nx, ny)}, got {b.shape}")
+        self._build_slices()
+        self.dirichlet_pressure = self.boundaries.dirichlet_pressure()
+        self.poisson = assemble_poisson_matrix(
+            self.mesh, self.periodic_x, self.periodic_y,
+            self.solid if self.has_obstacle else None,
+            dirichlet=self.dirichlet_pressure or None)
+        self.turbulence = None
+        self._use_kernel = True
+        self._blend = 0.0
+
+        walls = ("left", "right", "bottom", "top")
+        conds = self.boundaries.conditions
+        body_id = None
+        if self.body_masks:
+            body_id = np.full((nx, ny), -1, dtype=np.int8)
+            for k, b in enumerate(self.body_masks):
+                body_id[b] = k
+        self.ctx = DeviceContext(
+            nx, ny, self.mesh.dx, self.mesh.dy, self.nu, config.body_force,
+            config.time_scheme.value,
+            [conds[w].kind.value for w in walls],
+            [conds[w].spec.velocity for w in walls],
+            [conds[w].spec.p_ref for w in walls],
+            [conds[w].inlet_line(self.mesh) for w in walls],
+            self.solid if self.has_obstacle else None, body_id, len(self.body_masks))
+        self.pressure_solver = make_pressure_solver(
+            config.pressure_solver, self.poisson, tol=config.poisson_tol,
+            maxiter=config.poisson_maxiter, sor_omega=config.sor_omega,
+            mg_sweeps=config.mg_sweeps, context=self.ctx, precision=config.precision)
+        self.pressure_solver.configure()
+        self._has_guess = False
+        self._force = np.zeros(2 + 2 * len(self.body_masks))
+        self._force_stale = False
+
+    # ------------------------------------------------------------------ #
+    def _build_slices(self) -> None:
+        """Solvable face ranges and body face masks (solver.py:235-296)."""
+        nx, ny = self.mesh.shape
+        kinds = {w: bc.kind for w, bc in self.boundaries.conditions.items()}
+        outlet = BCKind.PRESSURE_OUTLET
+        u_lo = 1 if (self.periodic_x or kinds["left"] is outlet) else 2
+        u_hi = nx + 2 if (kinds["right"] is outlet and not self.periodic_x) else nx + 1
+        self.u_upd = (slice(u_lo, u_hi), slice(1, ny + 1))
+        self.u_rhs_sel = slice(u_lo - 1, u_hi - 1)
+        v_lo = 1 if (self.periodic_y or kinds["bottom"] is outlet) else 2
+        v_hi = ny + 2 if (kinds["top"] is outlet and not self.periodic_y) else ny + 1
+        self.v_upd = (slice(1, nx + 1), slice(v_lo, v_hi))
+        self.v_rhs_sel = slice(v_lo - 1, v_hi - 1)
+        if self.has_obstacle:
+            self.u_face_solid = _face_solid_mask(self.solid, u_lo, u_hi, 0, self.periodic_x)
+            self.v_face_solid = _face_solid_mask(self.solid, v_lo, v_hi, 1, self.periodic_y)
+        else:
+            self.u_face_solid = None
+            self.v_face_solid = None
+        self.u_face_body = tuple(_face_solid_mask(b, u_lo, u_hi, 0, self.periodic_x)
+                                 for b in self.body_masks)
+        self.v_face_body = tuple(_face_solid_mask(b, v_lo, v_hi, 1, self.periodic_y)
+                                 for b in self.body_masks)
+
+    # -- forces --------------------------------------------------------- #
+    @property
+    def body_force_reaction(self) -> tuple[float, float]:
+        """Reaction force of the last substep (solver.py:173-174)."""
+        f = self._current_force()
+        return float(f[0]), float(f[1])
+
+    @property
+    def body_force_reactions(self) -> tuple[tuple[float, float], ...]:
+        f = self._current_force()
+        return tuple((float(f[2 + 2 * k]), float(f[3 + 2 * k]))
+                     for k in range(len(self.body_masks)))
+
+    def _current_force(self) -> np.ndarray:
+        if self._force_stale:
+            self._force = self.ctx.force()
+            self._force_stale = False
+        return self._force
+
+    def set_force(self, fx: float, fy: float) -> None:
+        """Total force read back together with the step diagnostics."""
+        if self.body_masks:
+            self._force_stale = True
+        else:
+            self._force[0], self._force[1] = fx, fy
+            self._force_stale = False
+
+    # -- initialisation --------------------------------------------------- #
+    def initialize(self, u_init=None, v_init=None) -> FlowField:
+        """Initial state with BCs applied and projected (solver.py:301-329)."""
+        nx, ny = self.mesh.shape
+        u = np.zeros(self.mesh.u_shape)
+        v = np.zeros(self.mesh.v_shape)
+        if u_init is not None:
+            u[1:nx + 2, 1:ny + 1] = u_init
+        if v_init is not None:
+            v[1:nx + 1, 1:ny + 2] = v_init
+        fields = FlowField(self.mesh, to_device(u), to_device(v), device_zeros(self.mesh.p_shape))
+        info = self.ctx.initialize(fields.u_dev, fields.v_dev, fields.p_dev)
+        self.pressure_solver.record(info.iterations, info.residual, bool(info.converged))
+        self._has_guess = True
+        return fields
+
+    # -- spatial operators ------------------------------------------------ #
+    def momentum_rhs(self, u, v, blend: float = 0.0):
+        """``-A(u) + nu lap(u) + f`` on the GPU (solver.py:501-529)."""
+        import torch
+        nx, ny = self.mesh.shape
+        host = not isinstance(u, torch.Tensor)
+        ud, vd = to_device(u), to_device(v)
+        ru = torch.empty((nx + 1, ny), dtype=torch.float64, device="cuda")
+        rv = torch.empty((nx, ny + 1), dtype=torch.float64, device="cuda")
+        self.ctx.momentum_rhs(ud, vd, blend, ru, rv)
+        return (to_host(ru), to_host(rv)) if host else (ru, rv)
+
+    def divergence(self, u, v):
+        """Cell-centred divergence ``(nx, ny)`` (solver.py:532-543)."""
+        import torch
+        nx, ny = self.mesh.shape
+        host = not isinstance(u, torch.Tensor)
+        out = torch.empty((nx, ny), dtype=torch.float64, device="cuda")
+        self.ctx.divergence(to_device(u), to_device(v), out)
+        return to_host(out) if host else out
+
+    def max_divergence(self, fields: FlowField) -> float:
+        """Largest |div| over the fluid cells (solver.py:545-550)."""
+        d = fields.diagnostics
+        if "max_div" in d:
+            return d["max_div"]
+        out = self.ctx.diagnostics(fields.u_dev, fields.v_dev, None)
+        return float(out[5])
+
+    def project(self, u, v, dt: float):
+        """Remove the divergence in place; return the ghosted pressure (solver.py:643-683)."""
+        import torch
+        host = not isinstance(u, torch.Tensor)
+        ud, vd = to_device(u), to_device(v)
+        p = device_zeros(self.mesh.p_shape)
+        info = self.ctx.project(ud, vd, dt, p)
+        self.pressure_solver.record(info.iterations, info.residual, bool(info.converged))
+        self._has_guess = True
+        if host:
+            u[...] = to_host(ud)
+            v[...] = to_host(vd)
+            return to_host(p)
+        if ud.data_ptr() != u.data_ptr():
+            u.copy_(ud)
+            v.copy_(vd)
+        return p
+
+    # -- time step -------------------------------------------------------- #
+    def step(self, fields: FlowField, dt: float) -> FlowField:
+        """Advance by ``dt`` with the configured scheme (solver.py:686-731)."""
+        mesh = self.mesh
+        u, v, p = device_zeros(mesh.u_shape), device_zeros(mesh.v_shape), device_zeros(mesh.p_shape)
+        info = self.ctx.step(fields.u_dev, fields.v_dev, dt, self._blend, u, v, p)
+        self.pressure_solver.record(info.iterations, info.residual, bool(info.converged))
+        self._has_guess = True
+        self._force_stale = self.has_obstacle
+        return FlowField(mesh, u, v, p, fields.t + dt, fields.step + 1)
+
+    # ------------------------------------------------------------------ #
+    def set_upwind_blend(self, blend: float) -> None:
+        self._blend = float(np.clip(blend, 0.0, 1.0))
+
+    def upwind_blend_for(self, fields: FlowField, dt: float) -> float:
+        """solver.py:738-756."""
+        if self.config.advection_scheme is AdvectionScheme.CENTRAL:
+            return 0.0
+        if self.config.upwind_blend is not None:
+            return self.config.upwind_blend
+        umax, vmax = fields.max_velocity()
+        courant = max(umax * dt / self.mesh.dx, vmax * dt / self.mesh.dy)
+        return float(min(1.0, 1.2 * courant))
+
+
+__all__ = ["ProjectionSolver", "_face_solid_mask"]

@@ -1,0 +1,701 @@
+// C-ABI of libpycfd_hip.so: the simulation context and the time step.
+//
+// A context owns everything one ProjectionSolver (pycfd/core/solver.py:82)
+// needs on the device: the immersed-boundary face lists, the Poisson
+// operator flags, the pressure engine and the Runge-Kutta stage buffers.
+// Field arrays (u, v, p) belong to the caller and are passed per call.
+#include <cstdarg>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <vector>
+
+#include "../../include/pycfd_hip.h"
+#include "poisson.hpp"
+
+namespace pcfd {
+
+static thread_local char g_last_error[1024] = "";
+
+void set_last_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+  va_end(ap);
+}
+
+}  // namespace pcfd
+
+using namespace pcfd;
+
+struct pcfd_ctx {
+  int nx = 0, ny = 0;
+  int ldu = 0, ldv = 0, ldp = 0;
+  double dx = 0, dy = 0, nu = 0, fx = 0, fy = 0;
+  int scheme = PCFD_SCHEME_RK3;
+  int kind[4] = {0, 0, 0, 0};
+  int any_outlet = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  UpdRange upd{};
+  BCParams bc{};
+  MomParams mom{};
+  int n_bodies = 0;
+  // device buffers owned by the context
+  double* inlet[4] = {nullptr, nullptr, nullptr, nullptr};
+  int* uf = nullptr;
+  int* vf = nullptr;
+  int8_t* uf_body = nullptr;
+  int8_t* vf_body = nullptr;
+  double* force = nullptr;
+  uint8_t* flags = nullptr;
+  uint8_t* solid = nullptr;
+  double* stage_a[2] = {nullptr, nullptr};
+  double* stage_b[2] = {nullptr, nullptr};
+  double* diag_out = nullptr;
+  double* guess_backup = nullptr;
+  double* host_out = nullptr;  // pinned
+  PoissonEngine engine;
+  bool solver_set = false;
+
+  ~pcfd_ctx() {
+    for (auto* p : inlet) hipFree(p);
+    hipFree(uf);
+    hipFree(vf);
+    hipFree(uf_body);
+    hipFree(vf_body);
+    hipFree(force);
+    hipFree(flags);
+    hipFree(solid);
+    for (auto* p : stage_a) hipFree(p);
+    for (auto* p : stage_b) hipFree(p);
+    hipFree(diag_out);
+    hipFree(guess_backup);
+    if (host_out) hipHostFree(host_out);
+  }
+};
+
+namespace {
+
+BCOps ops_of(std::initializer_list<int> l) {
+  BCOps o{};
+  for (int x : l) o.op[o.n++] = x;
+  return o;
+}
+
+int boundary(pcfd_ctx* c, std::initializer_list<int> l, double dt, double* u, double* v) {
+  return launch_boundary_pass(c->stream, c->bc, ops_of(l), dt, u, v);
+}
+
+template <typename T>
+int upload(T** dst, const T* src, size_t n) {
+  PCFD_HIP_CHECK(hipMalloc(dst, std::max<size_t>(n, 1) * sizeof(T)));
+  if (n) PCFD_HIP_CHECK(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+  return kOk;
+}
+
+int check_ctx(pcfd_ctx* c) {
+  if (!c) {
+    set_last_error("null context");
+    return kErrBadArg;
+  }
+  return kOk;
+}
+
+int check_solver(pcfd_ctx* c) {
+  if (!c->solver_set) {
+    set_last_error("no pressure solver configured (pcfd_set_pressure_solver)");
+    return kErrBadArg;
+  }
+  return kOk;
+}
+
+void fill_info(pcfd_solve_info* info, const SolveInfo& s) {
+  if (!info) return;
+  info->iterations = s.iterations;
+  info->residual = s.residual;
+  info->converged = s.converged;
+  info->status = s.status;
+}
+
+// One projected Euler substep (solver.py:612-640).  `ue` non-null selects
+// the fused stage combination ca*u + cb*ue as the substep input.
+int substep(pcfd_ctx* c, const double* u, const double* v, const double* ue, const double* ve,
+            double ca, double cb, double dt, double* us, double* vs, double* p, SolveInfo* info,
+            bool sync) {
+  int rc;
+  if (ue) {
+    rc = launch_predictor_comb(c->stream, u, ue, v, ve, ca, cb, c->mom, c->upd, dt, us, vs);
+  } else {
+    rc = launch_predictor(c->stream, u, v, c->mom, c->upd, dt, us, vs);
+  }
+  if (rc) return rc;
+  if (c->any_outlet) {
+    rc = boundary(c, {BOP_VEL_PRED, BOP_MASS, BOP_MASK_RESET, BOP_WRAP_FACES}, dt, us, vs);
+  } else {
+    rc = boundary(c, {BOP_VEL_PRED, BOP_MASK_RESET, BOP_WRAP_FACES}, dt, us, vs);
+  }
+  if (rc) return rc;
+  if ((rc = c->engine.prepare_from_velocity(us, vs, c->ldu, c->ldv, c->dx, c->dy, dt))) return rc;
+  if ((rc = c->engine.solve(info, sync))) return rc;
+  if ((rc = c->engine.scatter_pressure(p, c->ldp))) return rc;
+  if ((rc = launch_pressure_ghosts(c->stream, c->bc, p, c->ldp))) return rc;
+  if ((rc = launch_correct(c->stream, p, c->ldp, c->nx, c->ny, c->ldu, c->ldv, c->dx, c->dy,
+                           c->upd, dt, us, vs)))
+    return rc;
+  return boundary(c, {BOP_MASK_ACC, BOP_WRAP_FACES, BOP_VEL_CORR}, dt, us, vs);
+}
+
+// Per-cell flag byte of the fine Poisson operator (pressure.py:209-251):
+// solvable bit plus one bit per face that carries flux.  Returns the number
+// of solvable cells.
+int poisson_flags(const pcfd_case_desc& d, const std::vector<uint8_t>& solid, uint8_t* flags) {
+  const int nx = d.nx, ny = d.ny;
+  const bool px = d.bc_kind[0] == BC_PERIODIC, py = d.bc_kind[2] == BC_PERIODIC;
+  bool dir[4];
+  for (int w = 0; w < 4; ++w) dir[w] = d.bc_kind[w] == BC_PRESSURE_OUTLET;
+  auto sol = [&](int i, int j) -> bool { return solid[(size_t)i * ny + j] != 0; };
+  int n_fluid = 0;
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j) {
+      flags[(size_t)i * ny + j] = 0;
+      if (sol(i, j)) continue;
+      uint8_t f = 0;
+      auto nbf = [&](int ii, int jj) -> bool {
+        if (px) ii = (ii + nx) % nx;
+        if (py) jj = (jj + ny) % ny;
+        if (ii < 0 || ii >= nx || jj < 0 || jj >= ny) return false;
+        return !sol(ii, jj);
+      };
+      if (nbf(i - 1, j)) f |= F_L;
+      if (nbf(i + 1, j)) f |= F_R;
+      if (nbf(i, j - 1)) f |= F_B;
+      if (nbf(i, j + 1)) f |= F_T;
+      const bool on_dir = (dir[0] && i == 0) || (dir[1] && i == nx - 1) ||
+                          (dir[2] && j == 0) || (dir[3] && j == ny - 1);
+      if (f != 0 || on_dir) {
+        f |= F_SOLV;
+        ++n_fluid;
+      }
+      flags[(size_t)i * ny + j] = f;
+    }
+  return n_fluid;
+}
+
+int project_impl(pcfd_ctx* c, double* u, double* v, double dt, double* p, SolveInfo* info) {
+  int rc;
+  if ((rc = c->engine.prepare_from_velocity(u, v, c->ldu, c->ldv, c->dx, c->dy, dt))) return rc;
+  if ((rc = c->engine.solve(info, true))) return rc;
+  if ((rc = c->engine.scatter_pressure(p, c->ldp))) return rc;
+  if ((rc = launch_pressure_ghosts(c->stream, c->bc, p, c->ldp))) return rc;
+  return launch_correct(c->stream, p, c->ldp, c->nx, c->ny, c->ldu, c->ldv, c->dx, c->dy,
+                        c->upd, dt, u, v);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pcfd_last_error(char* buf, size_t len) {
+  if (buf && len) {
+    std::strncpy(buf, g_last_error, len - 1);
+    buf[len - 1] = '\0';
+  }
+  return kOk;
+}
+
+int pcfd_device_count(int32_t* count) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) n = 0;
+  if (count) *count = n;
+  return n > 0 ? kOk : kErrNoDevice;
+}
+
+int pcfd_poisson_flags(const pcfd_case_desc* d, uint8_t* flags_out, int32_t* n_solvable) {
+  if (!d || !flags_out || d->nx < 1 || d->ny < 1) {
+    set_last_error("pcfd_poisson_flags: bad argument");
+    return kErrBadArg;
+  }
+  std::vector<uint8_t> solid((size_t)d->nx * d->ny, 0);
+  if (d->solid)
+    for (size_t k = 0; k < solid.size(); ++k) solid[k] = d->solid[k] ? 1 : 0;
+  const int n = poisson_flags(*d, solid, flags_out);
+  if (n_solvable) *n_solvable = n;
+  return kOk;
+}
+
+int pcfd_ctx_create(const pcfd_case_desc* d, void* hip_stream, pcfd_ctx** out) {
+  if (!d || !out) {
+    set_last_error("pcfd_ctx_create: null argument");
+    return kErrBadArg;
+  }
+  *out = nullptr;
+  const int nx = d->nx, ny = d->ny;
+  if (nx < 4 || ny < 4) {
+    set_last_error("grid must be at least 4x4, got %dx%d", nx, ny);
+    return kErrBadArg;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    set_last_error("no HIP device available");
+    return kErrNoDevice;
+  }
+  std::unique_ptr<pcfd_ctx> c(new (std::nothrow) pcfd_ctx());
+  if (!c) return kErrBadArg;
+  c->nx = nx;
+  c->ny = ny;
+  c->ldu = ny + 2;
+  c->ldv = ny + 3;
+  c->ldp = ny + 2;
+  c->dx = d->dx;
+  c->dy = d->dy;
+  c->nu = d->nu;
+  c->fx = d->body_fx;
+  c->fy = d->body_fy;
+  c->scheme = d->time_scheme;
+  for (int w = 0; w < 4; ++w) c->kind[w] = d->bc_kind[w];
+  if (hip_stream) {
+    c->stream = (hipStream_t)hip_stream;
+  } else {
+    PCFD_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->own_stream = true;
+  }
+  const bool px = d->bc_kind[0] == BC_PERIODIC;
+  const bool py = d->bc_kind[2] == BC_PERIODIC;
+  if (px != (d->bc_kind[1] == BC_PERIODIC) || py != (d->bc_kind[3] == BC_PERIODIC)) {
+    set_last_error("periodicity must be set on both walls of an axis");
+    return kErrBadArg;
+  }
+  const int PO = BC_PRESSURE_OUTLET;
+  // solvable face ranges (solver.py:251-263)
+  UpdRange R;
+  R.u_lo = (px || d->bc_kind[0] == PO) ? 1 : 2;
+  R.u_hi = (d->bc_kind[1] == PO && !px) ? nx + 2 : nx + 1;
+  R.v_lo = (py || d->bc_kind[2] == PO) ? 1 : 2;
+  R.v_hi = (d->bc_kind[3] == PO && !py) ? ny + 2 : ny + 1;
+  c->upd = R;
+
+  // immersed-boundary face lists (solver.py:62-79, 276-296)
+  std::vector<uint8_t> solid((size_t)nx * ny, 0);
+  std::vector<int8_t> body((size_t)nx * ny, -1);
+  bool has_obstacle = false;
+  if (d->solid) {
+    for (size_t k = 0; k < solid.size(); ++k) {
+      solid[k] = d->solid[k] ? 1 : 0;
+      has_obstacle |= solid[k] != 0;
+      if (d->body_id) body[k] = d->body_id[k];
+    }
+  }
+  auto sol = [&](int i, int j) -> bool { return solid[(size_t)i * ny + j] != 0; };
+  auto bod = [&](int i, int j) -> int8_t { return body[(size_t)i * ny + j]; };
+  std::vector<int> uf, vf;
+  std::vector<int8_t> ufb, vfb;
+  if (has_obstacle) {
+    for (int m = R.u_lo; m < R.u_hi; ++m)
+      for (int j = 0; j < ny; ++j) {
+        int a = m - 2, b = m - 1;
+        if (px) {
+          a = ((a % nx) + nx) % nx;
+          b = ((b % nx) + nx) % nx;
+        }
+        const bool sa = (a >= 0 && a < nx) && sol(a, j);
+        const bool sb = (b >= 0 && b < nx) && sol(b, j);
+        if (sa || sb) {
+          uf.push_back(m * c->ldu + (j + 1));
+          ufb.push_back(sa ? bod(a, j) : bod(b, j));
+        }
+      }
+    for (int i = 0; i < nx; ++i)
+      for (int n = R.v_lo; n < R.v_hi; ++n) {
+        int a = n - 2, b = n - 1;
+        if (py) {
+          a = ((a % ny) + ny) % ny;
+          b = ((b % ny) + ny) % ny;
+        }
+        const bool sa = (a >= 0 && a < ny) && sol(i, a);
+        const bool sb = (b >= 0 && b < ny) && sol(i, b);
+        if (sa || sb) {
+          vf.push_back((i + 1) * c->ldv + n);
+          vfb.push_back(sa ? bod(i, a) : bod(i, b));
+        }
+      }
+  }
+  int rc;
+  if ((rc = upload(&c->uf, uf.data(), uf.size()))) return rc;
+  if ((rc = upload(&c->vf, vf.data(), vf.size()))) return rc;
+  if ((rc = upload(&c->uf_body, ufb.data(), ufb.size()))) return rc;
+  if ((rc = upload(&c->vf_body, vfb.data(), vfb.size()))) return rc;
+  if ((rc = upload(&c->solid, solid.data(), solid.size()))) return rc;
+  c->n_bodies = (d->n_bodies > 1 && d->body_id) ? d->n_bodies : 0;
+  std::vector<double> zeros(2 + 2 * (size_t)std::max(c->n_bodies, 0), 0.0);
+  if ((rc = upload(&c->force, zeros.data(), zeros.size()))) return rc;
+
+  // inlet lines (boundary.py:205-216)
+  for (int w = 0; w < 4; ++w) {
+    if (d->bc_kind[w] == BC_INLET) {
+      if (!d->inlet_line[w]) {
+        set_last_error("INLET wall %d without an inlet line", w);
+        return kErrBadArg;
+      }
+      const size_t len = (w < 2) ? (size_t)ny + 2 : (size_t)nx + 2;
+      if ((rc = upload(&c->inlet[w], d->inlet_line[w], len))) return rc;
+    }
+    if (d->bc_kind[w] == BC_OUTLET) c->any_outlet = 1;
+  }
+
+  BCParams& B = c->bc;
+  B.nx = nx;
+  B.ny = ny;
+  B.ldu = c->ldu;
+  B.ldv = c->ldv;
+  for (int w = 0; w < 4; ++w) {
+    B.kind[w] = d->bc_kind[w];
+    B.vel[w] = d->bc_velocity[w];
+    B.pref[w] = d->bc_p_ref[w];
+    B.inlet[w] = c->inlet[w];
+  }
+  B.dx = d->dx;
+  B.dy = d->dy;
+  B.periodic_x = px;
+  B.periodic_y = py;
+  B.has_obstacle = has_obstacle;
+  B.n_uf = (int)uf.size();
+  B.n_vf = (int)vf.size();
+  B.uf = c->uf;
+  B.vf = c->vf;
+  B.uf_body = c->uf_body;
+  B.vf_body = c->vf_body;
+  B.n_bodies = c->n_bodies;
+  B.area = d->dx * d->dy;  // mesh.py:372 (_dx * _dy)
+  B.force = c->force;
+
+  MomParams& M = c->mom;
+  M.nx = nx;
+  M.ny = ny;
+  M.ldu = c->ldu;
+  M.ldv = c->ldv;
+  M.dx = d->dx;
+  M.dy = d->dy;
+  M.dx2 = d->dx * d->dx;
+  M.dy2 = d->dy * d->dy;
+  M.nu = d->nu;
+  M.fx = d->body_fx;
+  M.fy = d->body_fy;
+  M.blend = 0.0;
+
+  // Poisson operator flags (pressure.py:209-251)
+  bool dir[4];
+  for (int w = 0; w < 4; ++w) dir[w] = d->bc_kind[w] == PO;
+  std::vector<uint8_t> flags((size_t)nx * ny, 0);
+  const int n_fluid = poisson_flags(*d, solid, flags.data());
+  if ((rc = upload(&c->flags, flags.data(), flags.size()))) return rc;
+  const bool singular = !(dir[0] || dir[1] || dir[2] || dir[3]);
+
+  FineOp op{};
+  op.flags = c->flags;
+  op.nx = nx;
+  op.ny = ny;
+  op.cx = 1.0 / (d->dx * d->dx);
+  op.cy = 1.0 / (d->dy * d->dy);
+  op.px = px;
+  op.py = py;
+  for (int w = 0; w < 4; ++w) {
+    op.dir[w] = dir[w];
+    op.pref[w] = d->bc_p_ref[w];
+  }
+  if ((rc = c->engine.init(op, n_fluid, singular, c->stream))) return rc;
+
+  const size_t usz = (size_t)(nx + 3) * c->ldu, vsz = (size_t)(nx + 2) * c->ldv;
+  PCFD_HIP_CHECK(hipMalloc(&c->stage_a[0], usz * sizeof(double)));
+  PCFD_HIP_CHECK(hipMalloc(&c->stage_a[1], vsz * sizeof(double)));
+  PCFD_HIP_CHECK(hipMalloc(&c->stage_b[0], usz * sizeof(double)));
+  PCFD_HIP_CHECK(hipMalloc(&c->stage_b[1], vsz * sizeof(double)));
+  PCFD_HIP_CHECK(hipMalloc(&c->diag_out, 16 * sizeof(double)));
+  PCFD_HIP_CHECK(hipMalloc(&c->guess_backup, (size_t)nx * ny * sizeof(double)));
+  PCFD_HIP_CHECK(hipHostMalloc(&c->host_out, 64 * sizeof(double), hipHostMallocDefault));
+  PCFD_HIP_CHECK(hipStreamSynchronize(c->stream));
+  *out = c.release();
+  return kOk;
+}
+
+int pcfd_ctx_destroy(pcfd_ctx* c) {
+  if (!c) return kOk;
+  hipStreamSynchronize(c->stream);
+  const bool own = c->own_stream;
+  hipStream_t s = c->stream;
+  delete c;
+  if (own) hipStreamDestroy(s);
+  return kOk;
+}
+
+int pcfd_synchronize(pcfd_ctx* c) {
+  if (int rc = check_ctx(c)) return rc;
+  PCFD_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return kOk;
+}
+
+int pcfd_set_pressure_solver(pcfd_ctx* c, const pcfd_solver_desc* sd,
+                             const pcfd_mg_level_desc* levels, int32_t n_levels) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!sd) {
+    set_last_error("null solver description");
+    return kErrBadArg;
+  }
+  PoissonConfig cfg;
+  cfg.kind = sd->kind;
+  cfg.tol = sd->tol;
+  cfg.maxiter = sd->maxiter;
+  cfg.omega = sd->omega;
+  cfg.sweeps = sd->sweeps;
+  cfg.precision = sd->precision;
+  if (cfg.sweeps < 1) {
+    set_last_error("mg sweeps must be at least 1, got %d", cfg.sweeps);
+    return kErrBadArg;
+  }
+  std::vector<MGLevelHost> lv;
+  for (int l = 0; l < n_levels; ++l) {
+    const pcfd_mg_level_desc& L = levels[l];
+    MGLevelHost h;
+    h.nx = L.nx;
+    h.ny = L.ny;
+    h.fx = L.fx;
+    h.fy = L.fy;
+    h.weight = L.weight;
+    const size_t n = (size_t)L.nx * L.ny;
+    if (l == 0 && (L.nx != c->nx || L.ny != c->ny)) {
+      set_last_error("level 0 of the hierarchy is %dx%d, grid is %dx%d", L.nx, L.ny, c->nx,
+                     c->ny);
+      return kErrBadArg;
+    }
+    if (L.active) h.active.assign(L.active, L.active + n);
+    else h.active.assign(n, 1);
+    if (l > 0) {
+      if (!L.cls || !L.table || !L.class_dinv || !L.class_active || L.n_offsets <= 0 ||
+          L.n_classes <= 0 || !L.di || !L.dj) {
+        set_last_error("coarse level %d lacks its stencil dictionary", l);
+        return kErrBadArg;
+      }
+      h.di.assign(L.di, L.di + L.n_offsets);
+      h.dj.assign(L.dj, L.dj + L.n_offsets);
+      h.cls.assign(L.cls, L.cls + n);
+      for (size_t k = 0; k < n; ++k)
+        if (h.cls[k] >= (uint16_t)L.n_classes) {
+          set_last_error("level %d: class index out of range", l);
+          return kErrBadArg;
+        }
+      h.table.assign(L.table, L.table + (size_t)L.n_classes * L.n_offsets);
+      h.cdinv.assign(L.class_dinv, L.class_dinv + L.n_classes);
+      h.cactive.assign(L.class_active, L.class_active + L.n_classes);
+    }
+    if (l == n_levels - 1) {
+      if (!L.coarse_inv || L.coarse_n != (int)n) {
+        set_last_error("coarsest level needs its %zux%zu pseudo-inverse", n, n);
+        return kErrBadArg;
+      }
+      if (n > 8192) {
+        set_last_error("coarsest level has %zu unknowns (max 8192)", n);
+        return kErrUnsupported;
+      }
+      h.coarse_inv.assign(L.coarse_inv, L.coarse_inv + n * n);
+    }
+    lv.push_back(std::move(h));
+  }
+  int rc = c->engine.configure(cfg, lv);
+  if (rc) return rc;
+  c->solver_set = true;
+  return kOk;
+}
+
+int pcfd_momentum_rhs(pcfd_ctx* c, const double* u, const double* v, double blend,
+                      double* rhs_u, double* rhs_v) {
+  if (int rc = check_ctx(c)) return rc;
+  MomParams P = c->mom;
+  P.blend = blend;
+  return launch_momentum_rhs(c->stream, u, v, P, rhs_u, rhs_v);
+}
+
+int pcfd_divergence(pcfd_ctx* c, const double* u, const double* v, double* out) {
+  if (int rc = check_ctx(c)) return rc;
+  return launch_divergence(c->stream, u, v, c->nx, c->ny, c->ldu, c->ldv, c->dx, c->dy, out);
+}
+
+int pcfd_apply_velocity_bc(pcfd_ctx* c, double* u, double* v, int32_t predictor) {
+  if (int rc = check_ctx(c)) return rc;
+  return boundary(c, {predictor ? BOP_VEL_PRED : BOP_VEL_CORR}, 1.0, u, v);
+}
+
+int pcfd_mass_balance(pcfd_ctx* c, double* u, double* v) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!c->any_outlet) return kOk;
+  return boundary(c, {BOP_MASS}, 1.0, u, v);
+}
+
+int pcfd_mask_obstacle(pcfd_ctx* c, double* u, double* v, double dt, int32_t mode) {
+  if (int rc = check_ctx(c)) return rc;
+  const int op = mode == PCFD_MASK_RESET      ? BOP_MASK_RESET
+                 : mode == PCFD_MASK_ACCUMULATE ? BOP_MASK_ACC
+                                                : BOP_MASK_ZERO;
+  if (op != BOP_MASK_ZERO && !(dt > 0.0)) {
+    set_last_error("mask with force accumulation needs dt > 0");
+    return kErrBadArg;
+  }
+  return boundary(c, {op}, dt, u, v);
+}
+
+int pcfd_wrap_faces(pcfd_ctx* c, double* u, double* v) {
+  if (int rc = check_ctx(c)) return rc;
+  return boundary(c, {BOP_WRAP_FACES}, 1.0, u, v);
+}
+
+int pcfd_apply_pressure_bc(pcfd_ctx* c, double* p) {
+  if (int rc = check_ctx(c)) return rc;
+  return launch_pressure_ghosts(c->stream, c->bc, p, c->ldp);
+}
+
+int pcfd_pressure_solve(pcfd_ctx* c, const double* rhs, const double* p0, double* p_out,
+                        pcfd_solve_info* info) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_solver(c)) return rc;
+  const size_t bytes = (size_t)c->nx * c->ny * sizeof(double);
+  PoissonEngine& E = c->engine;
+  PCFD_HIP_CHECK(hipMemcpyAsync(c->guess_backup, E.guess(), bytes, hipMemcpyDeviceToDevice,
+                                c->stream));
+  int rc = p0 ? E.set_guess(p0) : E.reset_guess();
+  if (rc) return rc;
+  SolveInfo s{};
+  if ((rc = E.prepare_from_rhs(rhs))) return rc;
+  if ((rc = E.solve(&s, true))) return rc;
+  if ((rc = E.write_interior(p_out))) return rc;
+  PCFD_HIP_CHECK(hipMemcpyAsync(E.guess(), c->guess_backup, bytes, hipMemcpyDeviceToDevice,
+                                c->stream));
+  PCFD_HIP_CHECK(hipStreamSynchronize(c->stream));
+  fill_info(info, s);
+  return kOk;
+}
+
+int pcfd_project(pcfd_ctx* c, double* u, double* v, double dt, double* p, pcfd_solve_info* info) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_solver(c)) return rc;
+  SolveInfo s{};
+  int rc = project_impl(c, u, v, dt, p, &s);
+  if (rc) return rc;
+  PCFD_HIP_CHECK(hipStreamSynchronize(c->stream));
+  fill_info(info, s);
+  return kOk;
+}
+
+int pcfd_initialize(pcfd_ctx* c, double* u, double* v, double* p, pcfd_solve_info* info) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_solver(c)) return rc;
+  int rc;
+  if (c->any_outlet) {
+    rc = boundary(c, {BOP_VEL_PRED, BOP_MASS, BOP_MASK_ZERO, BOP_WRAP_FACES}, 1.0, u, v);
+  } else {
+    rc = boundary(c, {BOP_VEL_PRED, BOP_MASK_ZERO, BOP_WRAP_FACES}, 1.0, u, v);
+  }
+  if (rc) return rc;
+  SolveInfo s{};
+  if ((rc = project_impl(c, u, v, 1.0, p, &s))) return rc;
+  if ((rc = boundary(c, {BOP_MASK_ZERO, BOP_WRAP_FACES, BOP_VEL_CORR}, 1.0, u, v))) return rc;
+  PCFD_HIP_CHECK(hipStreamSynchronize(c->stream));
+  fill_info(info, s);
+  return kOk;
+}
+
+int pcfd_step(pcfd_ctx* c, const double* u0, const double* v0, double dt, double blend,
+              double* u, double* v, double* p, pcfd_solve_info* info) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_solver(c)) return rc;
+  c->mom.blend = blend;
+  SolveInfo s{};
+  int rc;
+  double* A[2] = {c->stage_a[0], c->stage_a[1]};
+  double* B[2] = {c->stage_b[0], c->stage_b[1]};
+  const bool want = info != nullptr;
+  switch (c->scheme) {
+    case PCFD_SCHEME_EULER:
+      rc = substep(c, u0, v0, nullptr, nullptr, 0, 0, dt, u, v, p, &s, want);
+      if (rc) return rc;
+      break;
+    case PCFD_SCHEME_RK2:  // solver.py:702-707
+      if ((rc = substep(c, u0, v0, nullptr, nullptr, 0, 0, dt, A[0], A[1], p, &s, false)))
+        return rc;
+      if ((rc = substep(c, A[0], A[1], nullptr, nullptr, 0, 0, dt, B[0], B[1], p, &s, want)))
+        return rc;
+      if ((rc = launch_combine(c->stream, u0, B[0], v0, B[1], 0.5, 0.5, c->nx, c->ny, c->ldu,
+                               c->ldv, u, v)))
+        return rc;
+      break;
+    default:  // SSP-RK3, solver.py:709-721
+      if ((rc = substep(c, u0, v0, nullptr, nullptr, 0, 0, dt, A[0], A[1], p, &s, false)))
+        return rc;
+      if ((rc = substep(c, A[0], A[1], nullptr, nullptr, 0, 0, dt, B[0], B[1], p, &s, false)))
+        return rc;
+      if ((rc = substep(c, u0, v0, B[0], B[1], 0.75, 0.25, dt, A[0], A[1], p, &s, want)))
+        return rc;
+      if ((rc = launch_combine(c->stream, u0, A[0], v0, A[1], 1.0 / 3.0, 2.0 / 3.0, c->nx, c->ny,
+                               c->ldu, c->ldv, u, v)))
+        return rc;
+      break;
+  }
+  if (c->scheme != PCFD_SCHEME_EULER) {
+    if ((rc = boundary(c, {BOP_VEL_CORR, BOP_WRAP_FACES}, dt, u, v))) return rc;
+  }
+  if (want) fill_info(info, s);
+  return kOk;
+}
+
+int pcfd_reset_guess(pcfd_ctx* c) {
+  if (int rc = check_ctx(c)) return rc;
+  return c->engine.reset_guess();
+}
+
+int pcfd_diagnostics(pcfd_ctx* c, const double* u, const double* v, const double* p,
+                     const double* u_prev, const double* v_prev, double* out_host) {
+  if (int rc = check_ctx(c)) return rc;
+  int rc = launch_diagnostics(c->stream, u, v, p, u_prev, v_prev, c->solid, c->nx, c->ny, c->ldu,
+                              c->ldv, c->ldp, c->dx, c->dy, c->engine.partial_,
+                              c->engine.counter_, c->diag_out);
+  if (rc) return rc;
+  PCFD_HIP_CHECK(
+      hipMemcpyAsync(c->host_out, c->diag_out, 7 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  PCFD_HIP_CHECK(hipMemcpyAsync(c->host_out + 7, c->force, 2 * sizeof(double),
+                                hipMemcpyDeviceToHost, c->stream));
+  PCFD_HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (out_host) std::memcpy(out_host, c->host_out, 9 * sizeof(double));
+  return kOk;
+}
+
+int pcfd_get_force(pcfd_ctx* c, double* out_host) {
+  if (int rc = check_ctx(c)) return rc;
+  const size_t n = 2 + 2 * (size_t)c->n_bodies;
+  PCFD_HIP_CHECK(
+      hipMemcpyAsync(c->host_out, c->force, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  PCFD_HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (out_host) std::memcpy(out_host, c->host_out, n * sizeof(double));
+  return kOk;
+}
+
+int pcfd_poisson_apply(pcfd_ctx* c, const double* x, double* y) {
+  if (int rc = check_ctx(c)) return rc;
+  return c->engine.apply_operator(x, y);
+}
+
+int pcfd_time_kernel(pcfd_ctx* c, int32_t which, int32_t reps, double* avg_ms,
+                     double* bytes_per_launch) {
+  if (int rc = check_ctx(c)) return rc;
+  if (reps < 1 || !avg_ms || !bytes_per_launch) {
+    set_last_error("pcfd_time_kernel: bad argument");
+    return kErrBadArg;
+  }
+  return c->engine.time_kernel(which, reps, avg_ms, bytes_per_launch);
+}
+
+int pcfd_mg_apply(pcfd_ctx* c, const double* r, double* z) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_solver(c)) return rc;
+  return c->engine.apply_preconditioner(r, z);
+}
+
+}  // extern "C"

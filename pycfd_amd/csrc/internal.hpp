@@ -1,0 +1,90 @@
+// Declarations shared by the translation units of libpycfd_hip.so.
+#pragma once
+
+#include "common.hpp"
+
+namespace pcfd {
+
+// Wall-condition kinds (pycfd/config.py:85-94), same numbering as the C-ABI.
+enum BCKind : int {
+  BC_NO_SLIP = 0,
+  BC_MOVING_WALL = 1,
+  BC_INLET = 2,
+  BC_OUTLET = 3,
+  BC_PRESSURE_OUTLET = 4,
+  BC_PERIODIC = 5,
+  BC_SYMMETRY = 6,
+};
+
+// Operations of a boundary pass, executed in order by one workgroup.
+enum BoundaryOp : int {
+  BOP_VEL_PRED = 1,   // apply_velocity(predictor=True)  (boundary.py:375-379)
+  BOP_VEL_CORR = 2,   // apply_velocity(predictor=False)
+  BOP_MASS = 3,       // enforce_global_mass_balance      (boundary.py:423-487)
+  BOP_MASK_RESET = 4, // _mask_obstacle(dt, reset=True)   (solver.py:553-599)
+  BOP_MASK_ACC = 5,   // _mask_obstacle(dt, reset=False)
+  BOP_MASK_ZERO = 6,  // _mask_obstacle(dt=None)
+  BOP_WRAP_FACES = 7, // _wrap_periodic_faces             (solver.py:601-607)
+};
+
+struct BCOps {
+  int n;
+  int op[8];
+};
+
+struct BCParams {
+  int nx, ny, ldu, ldv;
+  int kind[4];             // left, right, bottom, top
+  double vel[4];
+  double pref[4];
+  const double* inlet[4];  // device: full normal-velocity line of an INLET wall
+  double dx, dy;
+  int periodic_x, periodic_y;
+  int has_obstacle;
+  int n_uf, n_vf;          // immersed-boundary faces (flat indices into u / v)
+  const int* uf;
+  const int* vf;
+  const int8_t* uf_body;
+  const int8_t* vf_body;
+  int n_bodies;
+  double area;             // uniform cell area
+  double* force;           // device [fx, fy, (fx, fy) per body]
+};
+
+struct MomParams {
+  int nx, ny, ldu, ldv;
+  double dx, dy, dx2, dy2, nu, fx, fy, blend;
+};
+
+// Solvable face ranges u_upd / v_upd (solver.py:235-263): u faces i in
+// [u_lo, u_hi) x rows 1..ny; v faces columns 1..nx x j in [v_lo, v_hi).
+struct UpdRange {
+  int u_lo, u_hi, v_lo, v_hi;
+};
+
+// stencil.hip
+int launch_momentum_rhs(hipStream_t s, const double* u, const double* v, const MomParams& P,
+                        double* rhs_u, double* rhs_v);
+int launch_predictor(hipStream_t s, const double* u, const double* v, const MomParams& P,
+                     const UpdRange& R, double dt, double* us, double* vs);
+int launch_predictor_comb(hipStream_t s, const double* u0, const double* ue, const double* v0,
+                          const double* ve, double ca, double cb, const MomParams& P,
+                          const UpdRange& R, double dt, double* us, double* vs);
+int launch_combine(hipStream_t s, const double* u0, const double* ue, const double* v0,
+                   const double* ve, double ca, double cb, int nx, int ny, int ldu, int ldv,
+                   double* uo, double* vo);
+int launch_correct(hipStream_t s, const double* p, int ldp, int nx, int ny, int ldu, int ldv,
+                   double dx, double dy, const UpdRange& R, double dt, double* u, double* v);
+int launch_divergence(hipStream_t s, const double* u, const double* v, int nx, int ny, int ldu,
+                      int ldv, double dx, double dy, double* out);
+int launch_diagnostics(hipStream_t s, const double* u, const double* v, const double* p,
+                       const double* up, const double* vp, const uint8_t* solid, int nx, int ny,
+                       int ldu, int ldv, int ldp, double dx, double dy, double* partial,
+                       unsigned int* counter, double* out);
+
+// boundary.hip
+int launch_boundary_pass(hipStream_t s, const BCParams& P, const BCOps& ops, double dt, double* u,
+                         double* v);
+int launch_pressure_ghosts(hipStream_t s, const BCParams& P, double* p, int ldp);
+
+}  // namespace pcfd

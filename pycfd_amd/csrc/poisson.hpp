@@ -1,0 +1,187 @@
+// Pressure-Poisson engine: matrix-free fine operator, Krylov / relaxation
+// solvers and the aggregation-multigrid V-cycle, all resident on the GPU.
+#pragma once
+
+#include <vector>
+#include "internal.hpp"
+
+namespace pcfd {
+
+// Per-cell bits of the fine Poisson operator (pycfd/core/pressure.py:209-251).
+enum FineFlag : uint8_t {
+  F_SOLV = 1,   // solvable cell: fluid and not enclosed
+  F_L = 2,      // face to the x-1 neighbour carries flux
+  F_R = 4,      // face to the x+1 neighbour
+  F_B = 8,      // face to the y-1 neighbour
+  F_T = 16,     // face to the y+1 neighbour
+};
+
+// Matrix-free 5-point operator on the interior cells, k = i*ny + j.
+struct FineOp {
+  const uint8_t* flags;
+  int nx, ny;
+  double cx, cy;  // 1/dx^2, 1/dy^2
+  int px, py;     // periodic axes
+  int dir[4];     // Dirichlet (pressure-outlet) walls: left, right, bottom, top
+  double pref[4];
+};
+
+// A Galerkin coarse level on its own rectangular grid, stored as a stencil
+// dictionary: the level's operator rows take only a few hundred distinct
+// values (interior rows repeat by translation invariance; the rest sit near
+// walls and bodies), so each cell keeps a 2-byte class index and the class
+// table of coefficients (offset (di[s], dj[s]) is column s) stays resident
+// in L2.  Per class also the damped-Jacobi diagonal inverse and the active
+// flag (inactive rows are the identity).
+template <typename T>
+struct ClassLevel {
+  int nx, ny, n;
+  int px, py;
+  int S;                      // stencil offsets
+  const int8_t* di;
+  const int8_t* dj;
+  const uint16_t* cls;        // per cell
+  const T* table;             // n_classes x S
+  const T* cdinv;             // per class
+  const uint8_t* cactive;     // per class
+};
+
+// Device-resident scalars of one solve.  Kernels read them directly so the
+// host never has to synchronise inside an iteration.
+struct SolveState {
+  double mean_b;     // fluid mean of the assembled right-hand side
+  double bnorm;      // ||b||
+  double atol;       // CG stopping threshold on the recursive ||r||
+  double rr;         // ||r||^2 of the current iterate
+  double rho, rho_prev, beta, alpha;
+  double mean_r;     // mean of r over the active set
+  double mean_x;     // pending constant shift of x (subtracted on active cells)
+  double mean_z;     // active-set mean of the last V-cycle output
+  double sum_red;    // red-colour sum after a red SOR sweep
+  double res, res_prev;
+  double res_final;  // relative true residual of the returned iterate
+  double mean_final; // fluid mean removed by the normalisation
+  int iter;          // completed iterations / sweeps / cycles
+  int done;
+  int converged;
+  int stalled;
+  int zero_rhs;
+  int pad[3];
+};
+
+enum SolverKind : int {
+  SK_DIRECT = 0,
+  SK_CG = 1,
+  SK_JACOBI = 2,
+  SK_SOR = 3,
+  SK_MULTIGRID = 4,
+  SK_MGCG = 5,
+};
+
+// Host description of one multigrid level handed in by the setup code.
+struct MGLevelHost {
+  int nx, ny;
+  int fx, fy;                    // aggregation factors towards the next level
+  std::vector<int8_t> di, dj;    // stencil offsets (coarse levels)
+  std::vector<uint16_t> cls;     // per-cell class (coarse levels)
+  std::vector<double> table;     // n_classes x S coefficients
+  std::vector<double> cdinv;     // per-class 1/diag
+  std::vector<uint8_t> cactive;  // per-class active flag
+  std::vector<uint8_t> active;   // per-cell active flag (all levels)
+  double weight;
+  std::vector<double> coarse_inv;  // dense pseudo-inverse (coarsest level)
+};
+
+struct PoissonConfig {
+  int kind;
+  double tol;
+  int maxiter;
+  double omega;     // SOR over-relaxation / Jacobi damping
+  int sweeps;       // multigrid pre/post smoothing sweeps
+  int precision;    // 0 = FP64 throughout, 1 = FP32 V-cycle inside FP64 CG
+};
+
+struct SolveInfo {
+  int iterations;
+  double residual;
+  int converged;
+  int status;
+};
+
+class PoissonEngine {
+ public:
+  PoissonEngine() = default;
+  ~PoissonEngine();
+  int init(const FineOp& op, int n_fluid, int singular, hipStream_t stream);
+  int configure(const PoissonConfig& cfg, const std::vector<MGLevelHost>& levels);
+  // b_raw: assembled right-hand side (fluid rows, Dirichlet terms added, not
+  // yet mean-free).  The solution overwrites `x_` (the warm-start buffer).
+  int prepare_from_velocity(const double* u, const double* v, int ldu, int ldv, double dx,
+                            double dy, double dt);
+  int prepare_from_rhs(const double* rhs);
+  int solve(SolveInfo* info, bool sync_info);
+  int scatter_pressure(double* p, int ldp);   // ghosted p, ghost ring zeroed
+  int write_interior(double* out);            // normalised (nx, ny)
+  int reset_guess();
+  int set_guess(const double* x0);
+  int apply_operator(const double* x, double* y);
+  int apply_preconditioner(const double* r, double* z);
+  int read_state(SolveState* st);
+  // Times `reps` back-to-back launches of one hot kernel on this engine's
+  // buffers with HIP events; reports the mean duration and the kernel's
+  // algorithmic bytes per launch (arrays read + written once, at stored size).
+  int time_kernel(int which, int reps, double* avg_ms, double* bytes);
+  double* guess() { return x_; }
+  int last_iterations() const { return last_iters_; }
+  hipStream_t stream() const { return s_; }
+  int n() const { return n_; }
+  int levels() const { return (int)lv64_.size(); }
+
+  double *partial_ = nullptr;
+  unsigned int* counter_ = nullptr;
+
+ private:
+  int run_cg();
+  int run_mgcg();
+  int run_sor();
+  int run_jacobi();
+  int run_multigrid();
+  int finalize(bool recompute_residual);
+  int enqueue_vcycle(bool nonzero_start);
+  template <typename T>
+  int enqueue_vcycle_t(bool nonzero_start);
+  int poll(int* iter, int* done);
+  void free_levels();
+
+  FineOp op_{};
+  hipStream_t s_ = nullptr;
+  int n_ = 0, n_fluid_ = 0, singular_ = 0;
+  PoissonConfig cfg_{};
+  SolveState* st_ = nullptr;
+  SolveState* st_host_ = nullptr;  // pinned mirror for polling
+  hipEvent_t ev_ = nullptr;
+  double *b_ = nullptr, *x_ = nullptr, *r_ = nullptr, *pa_ = nullptr, *pb_ = nullptr,
+         *q_ = nullptr, *z_ = nullptr;
+  int last_iters_ = 0;
+  int predicted_ = 0;
+
+  // multigrid
+  struct LevelBuf {
+    int nx, ny, n, fx, fy;
+    void *b = nullptr, *x = nullptr, *y = nullptr, *r = nullptr;  // precision-typed vectors
+    int S = 0, ncls = 0;
+    int8_t *di = nullptr, *dj = nullptr;
+    uint16_t* cls = nullptr;
+    void *table = nullptr, *cdinv = nullptr;
+    uint8_t* cactive = nullptr;
+    double weight;
+    double* cinv = nullptr;  // coarsest dense pseudo-inverse (FP64)
+  };
+  std::vector<LevelBuf> lv64_;
+  std::vector<LevelBuf> lv32_;
+  uint8_t* fine_active_ = nullptr;
+  int n_active0_ = 0;
+  double weight0_ = 1.0;
+};
+
+}  // namespace pcfd

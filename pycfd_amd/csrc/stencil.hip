@@ -1,0 +1,287 @@
+// Field stencils of the projection step on the staggered MAC grid.
+//
+// Every expression below keeps the operation order of the reference's NumPy
+// array path (pycfd/core/solver.py:334-465, 532-543, 677-682 and the fused
+// kernel pycfd/core/kernels.py:39-110), and the library is compiled with
+// -ffp-contract=off, so each output is bit-identical to the reference for the
+// same inputs: no fused multiply-add, IEEE division by dx, dy, dx^2, dy^2.
+#include "internal.hpp"
+
+namespace pcfd {
+
+// ----------------------------------------------------------------------------
+// Momentum right-hand side, -A(u) + nu*lap(u) + f, evaluated at one face.
+// `U(i, j)` / `V(i, j)` return the ghosted velocity arrays' entries; they are
+// functors so a Runge-Kutta stage combination can be folded into the loads.
+// ----------------------------------------------------------------------------
+template <class FU, class FV>
+__device__ __forceinline__ double rhs_u_face(const FU& U, const FV& V, int m, int j,
+                                             const MomParams& P) {
+  const double u0 = U(m, j);
+  const double um = U(m - 1, j), up = U(m + 1, j);
+  const double ud = U(m, j - 1), uu = U(m, j + 1);
+  // cell-centred interpolants right/left of the face (solver.py:376, 382-383)
+  const double uu_r = 0.5 * (u0 + up);
+  const double uu_l = 0.5 * (um + u0);
+  double duudx = (uu_r * uu_r - uu_l * uu_l) / P.dx;
+  // corner products above/below the face (solver.py:351-357, 385)
+  const double u_top = 0.5 * (u0 + uu);
+  const double v_top = 0.5 * (V(m - 1, j + 1) + V(m, j + 1));
+  const double u_bot = 0.5 * (ud + u0);
+  const double v_bot = 0.5 * (V(m - 1, j) + V(m, j));
+  double duvdy = (u_top * v_top - u_bot * v_bot) / P.dy;
+  if (P.blend > 0.0) {  // donor-cell correction (solver.py:393-407)
+    duudx += P.blend / P.dx * (fabs(uu_r) * (u0 - up) / 2.0 - fabs(uu_l) * (um - u0) / 2.0);
+    duvdy += P.blend / P.dy * (fabs(v_top) * (u0 - uu) / 2.0 - fabs(v_bot) * (ud - u0) / 2.0);
+  }
+  const double d2u = (um - 2.0 * u0 + up) / P.dx2 + (ud - 2.0 * u0 + uu) / P.dy2;
+  return -(duudx + duvdy) + P.nu * d2u + P.fx;
+}
+
+template <class FU, class FV>
+__device__ __forceinline__ double rhs_v_face(const FU& U, const FV& V, int i, int n,
+                                             const MomParams& P) {
+  const double v0 = V(i, n);
+  const double vm = V(i, n - 1), vp = V(i, n + 1);
+  const double vl = V(i - 1, n), vr = V(i + 1, n);
+  const double vv_t = 0.5 * (v0 + vp);
+  const double vv_b = 0.5 * (vm + v0);
+  double dvvdy = (vv_t * vv_t - vv_b * vv_b) / P.dy;
+  const double u_right = 0.5 * (U(i + 1, n - 1) + U(i + 1, n));
+  const double v_right = 0.5 * (v0 + vr);
+  const double u_left = 0.5 * (U(i, n - 1) + U(i, n));
+  const double v_left = 0.5 * (vl + v0);
+  double duvdx = (u_right * v_right - u_left * v_left) / P.dx;
+  if (P.blend > 0.0) {  // solver.py:409-417
+    dvvdy += P.blend / P.dy * (fabs(vv_t) * (v0 - vp) / 2.0 - fabs(vv_b) * (vm - v0) / 2.0);
+    duvdx += P.blend / P.dx * (fabs(u_right) * (v0 - vr) / 2.0 - fabs(u_left) * (vl - v0) / 2.0);
+  }
+  const double d2v = (vl - 2.0 * v0 + vr) / P.dx2 + (vm - 2.0 * v0 + vp) / P.dy2;
+  return -(duvdx + dvvdy) + P.nu * d2v + P.fy;
+}
+
+struct PlainLoad {
+  const double* a;
+  int ld;
+  __device__ __forceinline__ double operator()(int i, int j) const { return a[(size_t)i * ld + j]; }
+};
+
+// ca*a + cb*b, the SSP-RK3 stage combination (solver.py:716-717), fused into
+// the loads of the next substep's stencil.
+struct CombLoad {
+  const double* a;
+  const double* b;
+  int ld;
+  double ca, cb;
+  __device__ __forceinline__ double operator()(int i, int j) const {
+    const size_t k = (size_t)i * ld + j;
+    return ca * a[k] + cb * b[k];
+  }
+};
+
+// rhs_u (nx+1, ny) and rhs_v (nx, ny+1), contiguous (solver.py:501-529).
+__global__ void __launch_bounds__(kBlock)
+k_momentum_rhs(const double* __restrict__ u, const double* __restrict__ v, MomParams P,
+               double* __restrict__ rhs_u, double* __restrict__ rhs_v) {
+  const Tile2D t = tile_coords(P.ny + 1);
+  const int i = t.i, j = t.j;  // i: 0..nx, j: 0..ny
+  PlainLoad U{u, P.ldu}, V{v, P.ldv};
+  if (i <= P.nx && j < P.ny) rhs_u[(size_t)i * P.ny + j] = rhs_u_face(U, V, i + 1, j + 1, P);
+  if (i < P.nx && j <= P.ny) rhs_v[(size_t)i * (P.ny + 1) + j] = rhs_v_face(U, V, i + 1, j + 1, P);
+}
+
+// Predictor us = u; us[u_upd] += dt*rhs_u (solver.py:617-622), both
+// components, over the full ghosted arrays.
+template <class FU, class FV>
+__device__ __forceinline__ void predictor_body(const FU& U, const FV& V, const MomParams& P,
+                                               const UpdRange& R, double dt,
+                                               double* __restrict__ us, double* __restrict__ vs) {
+  const Tile2D t = tile_coords(P.ny + 3);
+  const int i = t.i, j = t.j;
+  if (i <= P.nx + 2 && j <= P.ny + 1) {
+    double val = U(i, j);
+    if (i >= R.u_lo && i < R.u_hi && j >= 1 && j <= P.ny) val += dt * rhs_u_face(U, V, i, j, P);
+    us[(size_t)i * P.ldu + j] = val;
+  }
+  if (i <= P.nx + 1 && j <= P.ny + 2) {
+    double val = V(i, j);
+    if (i >= 1 && i <= P.nx && j >= R.v_lo && j < R.v_hi) val += dt * rhs_v_face(U, V, i, j, P);
+    vs[(size_t)i * P.ldv + j] = val;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_predictor(const double* __restrict__ u, const double* __restrict__ v, MomParams P,
+            UpdRange R, double dt, double* __restrict__ us, double* __restrict__ vs) {
+  predictor_body(PlainLoad{u, P.ldu}, PlainLoad{v, P.ldv}, P, R, dt, us, vs);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_predictor_comb(const double* __restrict__ u0, const double* __restrict__ ue,
+                 const double* __restrict__ v0, const double* __restrict__ ve, double ca,
+                 double cb, MomParams P, UpdRange R, double dt, double* __restrict__ us,
+                 double* __restrict__ vs) {
+  predictor_body(CombLoad{u0, ue, P.ldu, ca, cb}, CombLoad{v0, ve, P.ldv, ca, cb}, P, R, dt, us,
+                 vs);
+}
+
+// out = ca*a + cb*b over both ghosted velocity arrays (solver.py:706-721).
+__global__ void __launch_bounds__(kBlock)
+k_combine(const double* __restrict__ u0, const double* __restrict__ ue,
+          const double* __restrict__ v0, const double* __restrict__ ve, double ca, double cb,
+          int nx, int ny, int ldu, int ldv, double* __restrict__ uo, double* __restrict__ vo) {
+  const Tile2D t = tile_coords(ny + 3);
+  const int i = t.i, j = t.j;
+  if (i <= nx + 2 && j <= ny + 1) {
+    const size_t k = (size_t)i * ldu + j;
+    uo[k] = ca * u0[k] + cb * ue[k];
+  }
+  if (i <= nx + 1 && j <= ny + 2) {
+    const size_t k = (size_t)i * ldv + j;
+    vo[k] = ca * v0[k] + cb * ve[k];
+  }
+}
+
+// Gradient correction u[u_upd] -= dt*(p[i]-p[i-1])/dx, same for v
+// (solver.py:677-682), in place.
+__global__ void __launch_bounds__(kBlock)
+k_correct(const double* __restrict__ p, int ldp, int nx, int ny, int ldu, int ldv, double dx,
+          double dy, UpdRange R, double dt, double* __restrict__ u, double* __restrict__ v) {
+  const Tile2D t = tile_coords(ny + 3);
+  const int i = t.i, j = t.j;
+  if (i >= R.u_lo && i < R.u_hi && j >= 1 && j <= ny) {
+    const double g = dt * (p[(size_t)i * ldp + j] - p[(size_t)(i - 1) * ldp + j]) / dx;
+    u[(size_t)i * ldu + j] -= g;
+  }
+  if (i >= 1 && i <= nx && j >= R.v_lo && j < R.v_hi) {
+    const double g = dt * (p[(size_t)i * ldp + j] - p[(size_t)i * ldp + j - 1]) / dy;
+    v[(size_t)i * ldv + j] -= g;
+  }
+}
+
+// Raw divergence (nx, ny) (solver.py:532-543).
+__global__ void __launch_bounds__(kBlock)
+k_divergence(const double* __restrict__ u, const double* __restrict__ v, int nx, int ny, int ldu,
+             int ldv, double dx, double dy, double* __restrict__ out) {
+  const Tile2D t = tile_coords(ny);
+  const int i = t.i, j = t.j;
+  if (i < nx && j < ny) {
+    const int a = i + 1, b = j + 1;
+    out[(size_t)i * ny + j] = (u[(size_t)(a + 1) * ldu + b] - u[(size_t)a * ldu + b]) / dx +
+                              (v[(size_t)a * ldv + b + 1] - v[(size_t)a * ldv + b]) / dy;
+  }
+}
+
+// Per-step diagnostics in one pass (timestepper.py:94-142, 208-212, 251-266;
+// fields.py:70-96; solver.py:545-550):
+//   out[0] max|u_phys|  out[1] max|v_phys|  out[2] non-finite count (u,v,p)
+//   out[3] max|u-u_prev| out[4] max|v-v_prev| out[5] max|div| over fluid
+//   out[6] sum(uc^2+vc^2) over cells
+__global__ void __launch_bounds__(kBlock)
+k_diagnostics(const double* __restrict__ u, const double* __restrict__ v,
+              const double* __restrict__ p, const double* __restrict__ up,
+              const double* __restrict__ vp, const uint8_t* __restrict__ solid, int nx, int ny,
+              int ldu, int ldv, int ldp, double dx, double dy, double* partial,
+              unsigned int* counter, double* out) {
+  const Tile2D t = tile_coords(ny + 3);
+  const int i = t.i, j = t.j;
+  double r[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (i <= nx + 2 && j <= ny + 1) {
+    const size_t k = (size_t)i * ldu + j;
+    const double a = u[k];
+    if (!isfinite(a)) r[2] += 1.0;
+    if (i >= 1 && i <= nx + 1 && j >= 1 && j <= ny) r[0] = fabs(a);
+    if (up) r[3] = fabs(a - up[k]);
+  }
+  if (i <= nx + 1 && j <= ny + 2) {
+    const size_t k = (size_t)i * ldv + j;
+    const double a = v[k];
+    if (!isfinite(a)) r[2] += 1.0;
+    if (i >= 1 && i <= nx && j >= 1 && j <= ny + 1) r[1] = fabs(a);
+    if (vp) r[4] = fabs(a - vp[k]);
+  }
+  if (p && i <= nx + 1 && j <= ny + 1) {
+    if (!isfinite(p[(size_t)i * ldp + j])) r[2] += 1.0;
+  }
+  if (i >= 1 && i <= nx && j >= 1 && j <= ny) {
+    const double ua = u[(size_t)i * ldu + j], ub = u[(size_t)(i + 1) * ldu + j];
+    const double va = v[(size_t)i * ldv + j], vb = v[(size_t)i * ldv + j + 1];
+    if (!solid || !solid[(size_t)(i - 1) * ny + (j - 1)]) {
+      const double d = (ub - ua) / dx + (vb - va) / dy;
+      r[5] = fabs(d);
+    }
+    const double uc = 0.5 * (ua + ub), vc = 0.5 * (va + vb);
+    r[6] = uc * uc + vc * vc;
+  }
+  const int ops[7] = {kMax, kMax, kSum, kMax, kMax, kMax, kSum};
+  if (grid_reduce<7>(r, ops, partial, counter, t.tile, gridDim.x) && threadIdx.x == 0) {
+    for (int q = 0; q < 7; ++q) out[q] = r[q];
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Host launchers.
+// ----------------------------------------------------------------------------
+static inline dim3 grid_for(int nxr, int nyr) { return dim3(num_tiles(nxr, nyr)); }
+
+int launch_momentum_rhs(hipStream_t s, const double* u, const double* v, const MomParams& P,
+                        double* rhs_u, double* rhs_v) {
+  hipLaunchKernelGGL(k_momentum_rhs, grid_for(P.nx + 1, P.ny + 1), dim3(kBlock), 0, s, u, v, P,
+                     rhs_u, rhs_v);
+  PCFD_LAUNCH_CHECK();
+  return kOk;
+}
+
+int launch_predictor(hipStream_t s, const double* u, const double* v, const MomParams& P,
+                     const UpdRange& R, double dt, double* us, double* vs) {
+  hipLaunchKernelGGL(k_predictor, grid_for(P.nx + 3, P.ny + 3), dim3(kBlock), 0, s, u, v, P, R,
+                     dt, us, vs);
+  PCFD_LAUNCH_CHECK();
+  return kOk;
+}
+
+int launch_predictor_comb(hipStream_t s, const double* u0, const double* ue, const double* v0,
+                          const double* ve, double ca, double cb, const MomParams& P,
+                          const UpdRange& R, double dt, double* us, double* vs) {
+  hipLaunchKernelGGL(k_predictor_comb, grid_for(P.nx + 3, P.ny + 3), dim3(kBlock), 0, s, u0, ue,
+                     v0, ve, ca, cb, P, R, dt, us, vs);
+  PCFD_LAUNCH_CHECK();
+  return kOk;
+}
+
+int launch_combine(hipStream_t s, const double* u0, const double* ue, const double* v0,
+                   const double* ve, double ca, double cb, int nx, int ny, int ldu, int ldv,
+                   double* uo, double* vo) {
+  hipLaunchKernelGGL(k_combine, grid_for(nx + 3, ny + 3), dim3(kBlock), 0, s, u0, ue, v0, ve, ca,
+                     cb, nx, ny, ldu, ldv, uo, vo);
+  PCFD_LAUNCH_CHECK();
+  return kOk;
+}
+
+int launch_correct(hipStream_t s, const double* p, int ldp, int nx, int ny, int ldu, int ldv,
+                   double dx, double dy, const UpdRange& R, double dt, double* u, double* v) {
+  hipLaunchKernelGGL(k_correct, grid_for(nx + 3, ny + 3), dim3(kBlock), 0, s, p, ldp, nx, ny, ldu,
+                     ldv, dx, dy, R, dt, u, v);
+  PCFD_LAUNCH_CHECK();
+  return kOk;
+}
+
+int launch_divergence(hipStream_t s, const double* u, const double* v, int nx, int ny, int ldu,
+                      int ldv, double dx, double dy, double* out) {
+  hipLaunchKernelGGL(k_divergence, grid_for(nx, ny), dim3(kBlock), 0, s, u, v, nx, ny, ldu, ldv,
+                     dx, dy, out);
+  PCFD_LAUNCH_CHECK();
+  return kOk;
+}
+
+int launch_diagnostics(hipStream_t s, const double* u, const double* v, const double* p,
+                       const double* up, const double* vp, const uint8_t* solid, int nx, int ny,
+                       int ldu, int ldv, int ldp, double dx, double dy, double* partial,
+                       unsigned int* counter, double* out) {
+  hipLaunchKernelGGL(k_diagnostics, grid_for(nx + 3, ny + 3), dim3(kBlock), 0, s, u, v, p, up,
+                     vp, solid, nx, ny, ldu, ldv, ldp, dx, dy, partial, counter, out);
+  PCFD_LAUNCH_CHECK();
+  return kOk;
+}
+
+}  // namespace pcfd

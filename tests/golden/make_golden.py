@@ -1,0 +1,213 @@
+"""Write the golden fixtures for the projection-step parity tests.
+
+Runs the reference implementation (kiliancmd/pycfd, pure NumPy/SciPy) from
+/root/reference in the build container and records small input/output
+vectors for the functions nothing else pins (momentum RHS on random fields,
+short trajectories of whole steps, boundary passes, forces, multigrid
+hierarchy data and V-cycle applications, solver outputs).  The reference never
+travels with the repository; only these .npz files do.
+
+Usage:  python tests/golden/make_golden.py   (re-record deliberately)
+"""
+
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+
+REF = Path("/root/reference")
+OUT = Path(__file__).resolve().parent
+sys.path.insert(0, str(REF))
+
+from pycfd.config import BCKind, BCSpec, SimulationConfig  # noqa: E402
+from pycfd.core import pressure as ref_pressure  # noqa: E402
+from pycfd.core.solver import ProjectionSolver  # noqa: E402
+from pycfd.core.timestepper import TimeStepper  # noqa: E402
+from pycfd.geometry.obstacles import circle_mask  # noqa: E402
+from pycfd.physics.incompressible import Simulation  # noqa: E402
+
+
+def walls(**kinds):
+    out = {w: BCSpec(BCKind.NO_SLIP) for w in ("left", "right", "bottom", "top")}
+    out.update(kinds)
+    return out
+
+
+CASES = {
+    "cavity": dict(nx=24, ny=20, lx=1.0, ly=1.0, re=100.0, dt=0.01, cfl_max=0.5,
+                   boundary_config=walls(top=BCSpec(BCKind.MOVING_WALL, velocity=1.0))),
+    "periodic": dict(nx=24, ny=16, lx=2 * np.pi, ly=2 * np.pi, re=50.0, dt=2e-3,
+                     adaptive_dt=False,
+                     boundary_config={w: BCSpec(BCKind.PERIODIC)
+                                      for w in ("left", "right", "bottom", "top")}),
+    "channel": dict(nx=32, ny=16, lx=4.0, ly=1.0, re=50.0, dt=0.01, cfl_max=0.4,
+                    boundary_config={"left": BCSpec(BCKind.INLET, velocity=1.0,
+                                                    profile="parabolic"),
+                                     "right": BCSpec(BCKind.OUTLET),
+                                     "bottom": BCSpec(BCKind.NO_SLIP),
+                                     "top": BCSpec(BCKind.NO_SLIP)}),
+    "cyl_outlet": dict(nx=48, ny=24, lx=12.0, ly=6.0, re=100.0, dt=0.02, cfl_max=0.4,
+                       boundary_config={"left": BCSpec(BCKind.INLET, velocity=1.0),
+                                        "right": BCSpec(BCKind.OUTLET),
+                                        "bottom": BCSpec(BCKind.SYMMETRY),
+                                        "top": BCSpec(BCKind.SYMMETRY)}),
+    "cyl_pout": dict(nx=48, ny=24, lx=12.0, ly=6.0, re=100.0, dt=0.02, cfl_max=0.4,
+                     boundary_config={"left": BCSpec(BCKind.INLET, velocity=1.0),
+                                      "right": BCSpec(BCKind.PRESSURE_OUTLET, p_ref=0.0),
+                                      "bottom": BCSpec(BCKind.SYMMETRY),
+                                      "top": BCSpec(BCKind.SYMMETRY)}),
+}
+
+
+def config(name, **kw):
+    base = dict(CASES[name])
+    base.update(kw)
+    base.setdefault("name", name)
+    base.setdefault("t_end", 100.0)
+    return SimulationConfig(**base)
+
+
+def body_for(name, mesh):
+    if name.startswith("cyl"):
+        return circle_mask(mesh, (3.0, 3.0), 0.5)
+    return None
+
+
+def init_for(name, mesh):
+    if name.startswith("cyl") or name == "channel":
+        Xv, Yv = mesh.v_grid()
+        return 1.0 if name.startswith("cyl") else 0.0, 0.05 * np.exp(-((Xv - 4.0) ** 2 + (Yv - 3.0) ** 2))
+    if name == "periodic":
+        Xu, Yu = mesh.u_grid()
+        Xv, Yv = mesh.v_grid()
+        return np.sin(Xu) * np.cos(Yu), -np.cos(Xv) * np.sin(Yv)
+    return None, None
+
+
+def rhs_fixture():
+    rng = np.random.default_rng(1234)
+    out = {}
+    for name in ("cavity", "periodic"):
+        solver = ProjectionSolver(config(name, use_numba=False))
+        nx, ny = solver.mesh.shape
+        u = rng.standard_normal((nx + 3, ny + 2))
+        v = rng.standard_normal((nx + 2, ny + 3))
+        for blend in (0.0, 0.7):
+            ru, rv = solver.momentum_rhs(u, v, blend)
+            tag = f"{name}_b{int(blend * 10)}"
+            out[f"{tag}_u"], out[f"{tag}_v"] = u, v
+            out[f"{tag}_rhs_u"], out[f"{tag}_rhs_v"] = ru.copy(), rv.copy()
+        out[f"{name}_div"] = solver.divergence(u, v)
+    np.savez_compressed(OUT / "momentum_rhs.npz", **out)
+
+
+def boundary_fixture():
+    """apply_velocity / mass balance / mask on random fields, per case."""
+    rng = np.random.default_rng(99)
+    out = {}
+    for name in CASES:
+        cfg = config(name)
+        from pycfd.core.mesh import StructuredMesh
+        mesh = StructuredMesh.from_config(cfg)
+        body = body_for(name, mesh)
+        solver = ProjectionSolver(cfg, mesh, body.mask if body else None)
+        nx, ny = mesh.shape
+        from pycfd.core.fields import FlowField
+        u = rng.standard_normal((nx + 3, ny + 2)) + 0.5
+        v = rng.standard_normal((nx + 2, ny + 3))
+        out[f"{name}_u"], out[f"{name}_v"] = u.copy(), v.copy()
+        for pred in (True, False):
+            f = FlowField(mesh, u.copy(), v.copy(), mesh.zeros_p())
+            solver.boundaries.apply_velocity(f, predictor=pred)
+            out[f"{name}_bc{int(pred)}_u"], out[f"{name}_bc{int(pred)}_v"] = f.u, f.v
+        f = FlowField(mesh, u.copy(), v.copy(), mesh.zeros_p())
+        solver.boundaries.apply_velocity(f, predictor=True)
+        solver.boundaries.enforce_global_mass_balance(f)
+        solver._mask_obstacle(f.u, f.v, dt=0.01, reset=True)
+        solver._wrap_periodic_faces(f.u, f.v)
+        out[f"{name}_pass_u"], out[f"{name}_pass_v"] = f.u, f.v
+        out[f"{name}_pass_force"] = np.array(solver.body_force_reaction)
+        p = rng.standard_normal((nx + 2, ny + 2))
+        out[f"{name}_p_in"] = p.copy()
+        f = FlowField(mesh, u.copy(), v.copy(), p)
+        solver.boundaries.apply_pressure(f)
+        out[f"{name}_p_bc"] = f.p
+    np.savez_compressed(OUT / "boundary.npz", **out)
+
+
+def trajectory_fixture():
+    """Whole steps through Simulation.run with tight solves."""
+    out = {}
+    runs = [("cavity", "direct", 20), ("cavity", "cg", 5), ("periodic", "direct", 10),
+            ("channel", "direct", 10), ("cyl_outlet", "direct", 10),
+            ("cyl_pout", "direct", 10)]
+    for name, ps, steps in runs:
+        cfg = config(name, pressure_solver=ps, poisson_tol=1e-12, use_numba=False)
+        from pycfd.core.mesh import StructuredMesh
+        mesh = StructuredMesh.from_config(cfg)
+        body = body_for(name, mesh)
+        u0, v0 = init_for(name, mesh)
+        sim = Simulation(cfg, obstacle=body, u_init=u0, v_init=v0)
+        tag = f"{name}_{ps}"
+        out[f"{tag}_u0"], out[f"{tag}_v0"], out[f"{tag}_p0"] = (
+            sim.fields.u.copy(), sim.fields.v.copy(), sim.fields.p.copy())
+        res = sim.run(max_steps=steps)
+        h = res.history
+        out[f"{tag}_u"], out[f"{tag}_v"], out[f"{tag}_p"] = sim.fields.u, sim.fields.v, sim.fields.p
+        for k in ("dt", "max_div", "kinetic_energy", "fx", "fy"):
+            out[f"{tag}_{k}"] = np.asarray(h[k])
+        out[f"{tag}_t"] = np.array(sim.fields.t)
+    np.savez_compressed(OUT / "trajectory.npz", **out)
+
+
+def multigrid_fixture():
+    from pycfd.core.mesh import StructuredMesh
+    out = {}
+    rng = np.random.default_rng(7)
+    for name in ("cavity", "periodic", "cyl_outlet", "cyl_pout"):
+        for sweeps in (1, 2):
+            cfg = config(name, pressure_solver="mgcg", mg_sweeps=sweeps)
+            mesh = StructuredMesh.from_config(cfg)
+            body = body_for(name, mesh)
+            solver = ProjectionSolver(cfg, mesh, body.mask if body else None)
+            h = solver.pressure_solver.hierarchy
+            tag = f"{name}_s{sweeps}"
+            out[f"{tag}_unknowns"] = np.array(h.unknowns)
+            out[f"{tag}_weights"] = np.array([lv.weight for lv in h.levels])
+            out[f"{tag}_nnz"] = np.array([lv.A.nnz for lv in h.levels])
+            r = rng.standard_normal(h.levels[0].n)
+            out[f"{tag}_r"] = r
+            out[f"{tag}_z"] = h.apply(r)
+            rhs = rng.standard_normal(mesh.shape)
+            out[f"{tag}_rhs"] = rhs
+            for ps in ("direct", "cg", "sor", "multigrid", "mgcg", "jacobi"):
+                if ps == "jacobi" and name != "cavity":
+                    continue
+                s = ref_pressure.make_pressure_solver(
+                    ps, solver.poisson, tol=1e-10, maxiter=200000, mg_sweeps=sweeps)
+                p = s.solve(rhs)
+                out[f"{tag}_{ps}_p"] = p
+                out[f"{tag}_{ps}_it"] = np.array(s.last_iterations)
+                out[f"{tag}_{ps}_res"] = np.array(s.last_residual)
+    np.savez_compressed(OUT / "multigrid.npz", **out)
+
+
+def mask_fixture():
+    from pycfd.core.mesh import StructuredMesh
+    out = {}
+    for nx, ny, lx, ly, cx in ((2048, 1024, 32.0, 16.0, 4.0), (512, 256, 16.0, 8.0, 4.0),
+                                (48, 24, 12.0, 6.0, 3.0)):
+        mesh = StructuredMesh(nx, ny, lx, ly)
+        ob = circle_mask(mesh, (cx, ly / 2), 0.5)
+        out[f"m{nx}x{ny}"] = np.packbits(ob.mask.ravel())
+    np.savez_compressed(OUT / "circle_mask.npz", **out)
+
+
+if __name__ == "__main__":
+    which = sys.argv[1:] or ["rhs", "boundary", "trajectory", "multigrid", "mask"]
+    for w in which:
+        print("writing", w, flush=True)
+        {"rhs": rhs_fixture, "boundary": boundary_fixture, "trajectory": trajectory_fixture,
+         "multigrid": multigrid_fixture, "mask": mask_fixture}[w]()

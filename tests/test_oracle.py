@@ -1,0 +1,236 @@
+"""Pin the CPU oracle before trusting it as the checker of the HIP engine.
+
+Each oracle function is checked against the golden vectors recorded from the
+reference (tests/golden/make_golden.py) and, where the mathematics fixes the
+answer, against closed forms, dense linear algebra and invariants chosen so
+that a dropped term, a sign or index slip, or a transposed operand fails.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import oracle as O
+from tests.cases_def import CASES, circle, initial, oracle_case
+from tests.conftest import golden
+
+
+# ---------------------------------------------------------------- stencils
+@pytest.mark.parametrize("name", ["cavity", "periodic"])
+@pytest.mark.parametrize("blend", [0, 7])
+def test_momentum_rhs_matches_reference_bitwise(name, blend):
+    g = golden("momentum_rhs")
+    tag = f"{name}_b{blend}"
+    case = oracle_case(name)
+    ru, rv = O.momentum_rhs(case, g[f"{tag}_u"], g[f"{tag}_v"], blend / 10.0)
+    assert np.array_equal(ru, g[f"{tag}_rhs_u"])
+    assert np.array_equal(rv, g[f"{tag}_rhs_v"])
+
+
+@pytest.mark.parametrize("name", ["cavity", "periodic"])
+def test_divergence_matches_reference_bitwise(name):
+    g = golden("momentum_rhs")
+    case = oracle_case(name)
+    assert np.array_equal(O.divergence(case, g[f"{name}_b0_u"], g[f"{name}_b0_v"]),
+                          g[f"{name}_div"])
+
+
+def test_momentum_rhs_closed_forms():
+    """Uniform flow: no advection, no diffusion -> rhs = body force exactly;
+    a linear shear u = y is steady (lap = 0, d(uu)/dx = 0, d(uv)/dy = 0 with v = 0)."""
+    case = O.Case(nx=8, ny=6, lx=1.0, ly=1.0, body_force=(0.25, -0.5))
+    u = np.full((11, 8), 3.0)
+    v = np.full((10, 9), -2.0)
+    ru, rv = O.momentum_rhs(case, u, v)
+    assert np.all(ru == 0.25) and np.all(rv == -0.5)
+    case = O.Case(nx=8, ny=6, lx=1.0, ly=1.0)
+    y = (np.arange(8) - 0.5) / 6.0
+    u = np.tile(y, (11, 1))
+    ru, rv = O.momentum_rhs(case, u, np.zeros((10, 9)))
+    assert np.abs(ru).max() < 1e-12 and np.abs(rv).max() < 1e-12
+
+
+# ---------------------------------------------------------------- boundaries
+@pytest.mark.parametrize("name", list(CASES))
+def test_boundary_passes_match_reference(name):
+    g = golden("boundary")
+    case = oracle_case(name)
+    for pred in (1, 0):
+        u, v = g[f"{name}_u"].copy(), g[f"{name}_v"].copy()
+        O.apply_velocity(case, u, v, bool(pred))
+        assert np.array_equal(u, g[f"{name}_bc{pred}_u"])
+        assert np.array_equal(v, g[f"{name}_bc{pred}_v"])
+    u, v = g[f"{name}_u"].copy(), g[f"{name}_v"].copy()
+    st = O.State()
+    O.apply_velocity(case, u, v, True)
+    O.mass_balance(case, u, v)
+    O.mask(case, st, u, v, 0.01, reset=True)
+    O.wrap_faces(case, u, v)
+    np.testing.assert_allclose(u, g[f"{name}_pass_u"], rtol=1e-14, atol=1e-14)
+    np.testing.assert_allclose(v, g[f"{name}_pass_v"], rtol=1e-14, atol=1e-14)
+    np.testing.assert_allclose(st.force, g[f"{name}_pass_force"], rtol=1e-12, atol=1e-12)
+    p = g[f"{name}_p_in"].copy()
+    O.apply_pressure(case, p)
+    assert np.array_equal(p, g[f"{name}_p_bc"])
+
+
+def test_mass_balance_closes_the_flux():
+    """After the balance the net boundary flux of a random field vanishes."""
+    case = oracle_case("channel")
+    rng = np.random.default_rng(0)
+    u = rng.random((case.nx + 3, case.ny + 2))
+    v = rng.random((case.nx + 2, case.ny + 3))
+    O.apply_velocity(case, u, v, True)
+    O.mass_balance(case, u, v)
+    nx, ny = case.nx, case.ny
+    net = (u[nx + 1, 1:ny + 1].sum() - u[1, 1:ny + 1].sum()) * case.dy + \
+          (v[1:nx + 1, ny + 1].sum() - v[1:nx + 1, 1].sum()) * case.dx
+    assert abs(net) < 1e-12
+
+
+def test_circle_mask_matches_reference():
+    g = golden("circle_mask")
+    for nx, ny, lx, ly, cx in ((2048, 1024, 32.0, 16.0, 4.0), (512, 256, 16.0, 8.0, 4.0),
+                                (48, 24, 12.0, 6.0, 3.0)):
+        m = circle(nx, ny, lx, ly, cx, ly / 2, 0.5)
+        bits = np.unpackbits(g[f"m{nx}x{ny}"])[: nx * ny].astype(bool).reshape(nx, ny)
+        assert np.array_equal(m, bits)
+
+
+# ---------------------------------------------------------------- operator
+@pytest.mark.parametrize("name", list(CASES))
+def test_operator_invariants(name):
+    case = oracle_case(name)
+    S = O.assemble(case)
+    A = S.A
+    assert abs(A - A.T).max() == 0.0                       # symmetric
+    if S.singular:                                         # null space = constant
+        rs = np.asarray(A.sum(axis=1)).ravel()[S.fluid]
+        assert np.abs(rs).max() < 1e-9 * abs(A.diagonal()).max()
+    assert (A.getnnz(axis=1) <= 5).all()
+
+
+def test_operator_5point_coefficients():
+    case = O.Case(nx=6, ny=5, lx=3.0, ly=1.0)
+    A = O.assemble(case).A.toarray()
+    cx, cy = 1.0 / (case.dx * case.dx), 1.0 / (case.dy * case.dy)
+    k = 2 * 5 + 2                                          # interior cell (2, 2)
+    assert A[k, k] == pytest.approx(-(2 * cx + 2 * cy), rel=1e-15)
+    assert A[k, k - 5] == cx and A[k, k + 5] == cx and A[k, k - 1] == cy and A[k, k + 1] == cy
+    k = 0                                                  # corner: two faces
+    assert A[k, k] == pytest.approx(-(cx + cy), rel=1e-15)
+
+
+@pytest.mark.parametrize("kind", ["direct", "cg", "sor", "jacobi", "multigrid", "mgcg"])
+def test_solvers_manufactured_solution(kind):
+    """lap(p) = -2 pi^2 cos(pi x) cos(pi y) on the unit square, Neumann:
+    the discrete solution converges to cos(pi x)cos(pi y) (test_pressure.py:95-121)."""
+    errs = []
+    for n in (16, 32):
+        case = O.Case(nx=n, ny=n, pressure_solver=kind, tol=1e-11, maxiter=200000)
+        X, Y = np.meshgrid(case.xc, case.yc, indexing="ij")
+        exact = np.cos(np.pi * X) * np.cos(np.pi * Y)
+        p = O.solve(case, O.State(), -2 * np.pi ** 2 * exact)
+        errs.append(np.abs(p - (exact - exact.mean())).max())
+    assert errs[0] < 5e-3
+    assert np.log2(errs[0] / errs[1]) > 1.8
+
+
+@pytest.mark.parametrize("name", ["cavity", "periodic", "cyl_outlet", "cyl_pout"])
+@pytest.mark.parametrize("sweeps", [1, 2])
+def test_solvers_match_reference_outputs(name, sweeps):
+    g = golden("multigrid")
+    tag = f"{name}_s{sweeps}"
+    for kind in ("direct", "cg", "sor", "multigrid", "mgcg", "jacobi"):
+        key = f"{tag}_{kind}_p"
+        if key not in g:
+            continue
+        case = oracle_case(name, pressure_solver=kind, tol=1e-10, maxiter=200000,
+                           mg_sweeps=sweeps)
+        st = O.State()
+        p = O.solve(case, st, g[f"{tag}_rhs"])
+        scale = np.abs(g[key]).max()
+        tol = 1e-12 if kind == "direct" else 1e-7
+        assert np.abs(p - g[key]).max() <= tol * scale, kind
+        if kind in ("direct", "mgcg", "multigrid", "sor"):
+            assert st.last_iterations == int(g[f"{tag}_{kind}_it"]), kind
+
+
+@pytest.mark.parametrize("name", ["cavity", "periodic", "cyl_outlet", "cyl_pout"])
+def test_hierarchy_matches_reference(name):
+    g = golden("multigrid")
+    for sweeps in (1, 2):
+        tag = f"{name}_s{sweeps}"
+        case = oracle_case(name)
+        S = O.assemble(case)
+        H = O.Hierarchy(S.A, (case.nx, case.ny), S.fluid, S.singular, sweeps)
+        assert [lv["A"].shape[0] for lv in H.levels] == list(g[f"{tag}_unknowns"])
+        assert [lv["A"].nnz for lv in H.levels] == list(g[f"{tag}_nnz"])
+        np.testing.assert_allclose([lv["weight"] for lv in H.levels], g[f"{tag}_weights"],
+                                   rtol=1e-13)
+        z = H.apply(g[f"{tag}_r"])
+        np.testing.assert_allclose(z, g[f"{tag}_z"], rtol=1e-10,
+                                   atol=1e-11 * np.abs(g[f"{tag}_z"]).max())
+
+
+def test_vcycle_is_symmetric_dense():
+    """Materialise M column by column: symmetric to round-off (test_multigrid.py:339-359)."""
+    case = oracle_case("cyl_outlet", nx=24, ny=12, lx=6.0, ly=3.0)
+    case.solid = circle(24, 12, 6.0, 3.0, 3.0, 1.5, 0.5)
+    S = O.assemble(case)
+    H = O.Hierarchy(S.A, (24, 12), S.fluid, S.singular, 1)
+    n = S.A.shape[0]
+    M = np.column_stack([H.apply(np.eye(n)[:, k]) for k in range(n)])
+    assert np.abs(M - M.T).max() < 1e-10 * np.abs(M).max()
+
+
+def test_coarsest_is_pseudo_inverse():
+    """A tiny singular Neumann operator: one level, coarse solve = pseudo-inverse."""
+    case = O.Case(nx=8, ny=8)
+    S = O.assemble(case)
+    H = O.Hierarchy(S.A, (8, 8), S.fluid, S.singular, 1)
+    assert len(H.levels) == 1
+    b = np.random.default_rng(3).standard_normal(64)
+    b -= b.mean()
+    x = H.apply(b)
+    np.testing.assert_allclose(x, np.linalg.pinv(S.A.toarray()) @ b, atol=1e-12)
+
+
+# ---------------------------------------------------------------- time steps
+TRAJ = [("cavity", "direct", 20), ("cavity", "cg", 5), ("periodic", "direct", 10),
+        ("channel", "direct", 10), ("cyl_outlet", "direct", 10), ("cyl_pout", "direct", 10)]
+
+
+@pytest.mark.parametrize("name,ps,steps", TRAJ)
+def test_trajectory_matches_reference(name, ps, steps):
+    g = golden("trajectory")
+    tag = f"{name}_{ps}"
+    case = oracle_case(name, pressure_solver=ps, tol=1e-12)
+    u0, v0 = initial(name, case)
+    u, v, p, hist, t = O.run(case, steps, u0, v0)
+    scale = max(np.abs(g[f"{tag}_u"]).max(), 1.0)
+    assert np.abs(u - g[f"{tag}_u"]).max() <= 1e-10 * scale
+    assert np.abs(v - g[f"{tag}_v"]).max() <= 1e-10 * scale
+    pscale = max(np.abs(g[f"{tag}_p"]).max(), 1.0)
+    assert np.abs(p - g[f"{tag}_p"]).max() <= 1e-8 * pscale
+    np.testing.assert_allclose(hist["dt"], g[f"{tag}_dt"], rtol=1e-12)
+    np.testing.assert_allclose(hist["kinetic_energy"], g[f"{tag}_kinetic_energy"], rtol=1e-10)
+    np.testing.assert_allclose(hist["fx"], g[f"{tag}_fx"], rtol=1e-8, atol=1e-10)
+    assert max(hist["max_div"]) < 1e-9
+
+
+def test_taylor_green_decay_rate():
+    """Exact solution of the full equations: amplitude decays as exp(-2 nu t)."""
+    n, nu, dt, steps = 32, 0.05, 0.01, 10
+    case = O.Case(nx=n, ny=n, lx=2 * np.pi, ly=2 * np.pi, re=1 / nu, dt=dt, adaptive_dt=False,
+                  kinds=("periodic",) * 4, pressure_solver="direct")
+    u0, v0 = initial("periodic", case)
+    u, v, p, hist, t = O.run(case, steps, u0, v0)
+    Xu, Yu = np.meshgrid(case.xf, case.yc, indexing="ij")
+    exact = np.sin(Xu) * np.cos(Yu) * np.exp(-2 * nu * t)
+    err = np.abs(u[1:n + 2, 1:n + 1] - exact).max()
+    assert err < 5e-3                          # O(h^2) spatial error at 32^2
+    ke0 = 0.5 * (2 * np.pi) ** 2 * 0.5
+    assert abs(hist["kinetic_energy"][-1] / ke0 / np.exp(-4 * nu * t) - 1) < 2e-2
