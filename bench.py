@@ -42,7 +42,8 @@ def run_path(precision: str, steps: int, warmup: int, nx: int, ny: int):
     from pycfd_amd.physics.incompressible import Simulation
 
     cfg, body, u0, v0 = cases.cylinder(nx=nx, ny=ny, pressure_solver="mgcg",
-                                       poisson_tol=1e-10, precision=precision)
+                                       poisson_tol=1e-10, poisson_maxiter=300,
+                                       precision=precision)
     t0 = time.perf_counter()
     sim = Simulation(cfg, obstacle=body, u_init=u0, v_init=v0)
     log(f"[{precision}] setup {time.perf_counter() - t0:.1f} s "

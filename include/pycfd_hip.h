@@ -129,7 +129,8 @@ typedef struct pcfd_solve_info {
 
 /* --- lifetime --------------------------------------------------------- */
 /* ProjectionSolver.__init__ (solver.py:101-195): device state, IB face lists,
- * Poisson operator flags (pressure.py:110-270).  hip_stream may be NULL. */
+ * Poisson operator flags (pressure.py:110-270).  hip_stream NULL = the
+ * default stream. */
 int pcfd_ctx_create(const pcfd_case_desc* desc, void* hip_stream, pcfd_ctx** out);
 int pcfd_ctx_destroy(pcfd_ctx* ctx);
 int pcfd_last_error(char* buf, size_t len);

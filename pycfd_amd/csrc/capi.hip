@@ -255,12 +255,9 @@ int pcfd_ctx_create(const pcfd_case_desc* d, void* hip_stream, pcfd_ctx** out) {
   c->fy = d->body_fy;
   c->scheme = d->time_scheme;
   for (int w = 0; w < 4; ++w) c->kind[w] = d->bc_kind[w];
-  if (hip_stream) {
-    c->stream = (hipStream_t)hip_stream;
-  } else {
-    PCFD_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    c->own_stream = true;
-  }
+  // NULL selects the default stream, which orders this context's work with
+  // everything else the caller enqueues there (torch's fills and copies).
+  c->stream = (hipStream_t)hip_stream;
   const bool px = d->bc_kind[0] == BC_PERIODIC;
   const bool py = d->bc_kind[2] == BC_PERIODIC;
   if (px != (d->bc_kind[1] == BC_PERIODIC) || py != (d->bc_kind[3] == BC_PERIODIC)) {

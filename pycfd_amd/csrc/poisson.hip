@@ -915,6 +915,8 @@ int PoissonEngine::configure(const PoissonConfig& cfg, const std::vector<MGLevel
   }
   n_active0_ = 0;
   for (uint8_t a : levels[0].active) n_active0_ += a ? 1 : 0;
+  // multigrid.py:453-458: the mean is removed only when the operator is singular
+  n_proj_ = singular_ ? n_active0_ : 0;
   weight0_ = levels[0].weight;
   for (int pass = 0; pass < 2; ++pass) {
     const bool f32 = pass == 1;
@@ -1166,7 +1168,7 @@ int PoissonEngine::enqueue_vcycle_t(bool nonzero_start) {
     hipLaunchKernelGGL(k_coarse_solve<T>, dim3(1), dim3(kBlock), L.n * sizeof(double), s_,
                        (const double*)L.cinv, L.n, (const T*)L.b, (T*)L.x, st_);
     hipLaunchKernelGGL(k_cycle_reduce<T>, tiles_for(op_.nx, op_.ny), dim3(kBlock), 0, s_, op_,
-                       (const T*)L.b, (const T*)L.x, red, n_active0_, partial_, counter_, st_);
+                       (const T*)L.b, (const T*)L.x, red, n_proj_, partial_, counter_, st_);
     PCFD_LAUNCH_CHECK();
     return kOk;
   }
@@ -1180,7 +1182,7 @@ int PoissonEngine::enqueue_vcycle_t(bool nonzero_start) {
       const T* xin = (s == 0 && !nz) ? nullptr : (const T*)L.x;
       if (l == 0) {
         hipLaunchKernelGGL(k_smooth_fine<T>, g, dim3(kBlock), 0, s_, F, (const T*)L.b, xin,
-                           (T*)L.y, (s == 0 && nz) ? 1 : 0, (int)SR_NONE, n_active0_, partial_,
+                           (T*)L.y, (s == 0 && nz) ? 1 : 0, (int)SR_NONE, n_proj_, partial_,
                            counter_, st_);
       } else {
         hipLaunchKernelGGL(k_smooth_cls<T>, g, dim3(kBlock), 0, s_, dia(L), (T)L.weight,
@@ -1223,7 +1225,7 @@ int PoissonEngine::enqueue_vcycle_t(bool nonzero_start) {
       if (l == 0) {
         const int rmode = (s == nu - 1) ? red : (int)SR_NONE;
         hipLaunchKernelGGL(k_smooth_fine<T>, g, dim3(kBlock), 0, s_, F, (const T*)L.b,
-                           (const T*)L.x, (T*)L.y, 0, rmode, n_active0_, partial_, counter_,
+                           (const T*)L.x, (T*)L.y, 0, rmode, n_proj_, partial_, counter_,
                            st_);
       } else {
         hipLaunchKernelGGL(k_smooth_cls<T>, g, dim3(kBlock), 0, s_, dia(L), (T)L.weight,
@@ -1245,7 +1247,7 @@ int PoissonEngine::run_mgcg() {
   const dim3 g = tiles_for(op_.nx, op_.ny);
   hipLaunchKernelGGL(k_zero_nonfluid, g, dim3(kBlock), 0, s_, op_, x_);
   hipLaunchKernelGGL(k_init, g, dim3(kBlock), 0, s_, op_, b_, x_, r_, 0, (int)IM_MGCG, cfg_.tol,
-                     cfg_.maxiter, n_active0_, partial_, counter_, st_);
+                     cfg_.maxiter, n_proj_, partial_, counter_, st_);
   PCFD_LAUNCH_CHECK();
   double* p_old = pa_;
   double* p_new = pb_;
@@ -1268,7 +1270,7 @@ int PoissonEngine::run_mgcg() {
                          (const double*)lv64_[0].x, p_old, p_new, q_, partial_, counter_, st_);
     }
     hipLaunchKernelGGL(k_cg_update, g, dim3(kBlock), 0, s_, op_, x_, r_, p_new, q_, (int)IM_MGCG,
-                       cfg_.maxiter, n_active0_, partial_, counter_, st_);
+                       cfg_.maxiter, n_proj_, partial_, counter_, st_);
     PCFD_LAUNCH_CHECK();
     std::swap(p_old, p_new);
   });
@@ -1279,7 +1281,7 @@ int PoissonEngine::run_multigrid() {
   const dim3 g = tiles_for(op_.nx, op_.ny);
   hipLaunchKernelGGL(k_zero_nonfluid, g, dim3(kBlock), 0, s_, op_, x_);
   hipLaunchKernelGGL(k_init, g, dim3(kBlock), 0, s_, op_, b_, x_, (double*)nullptr, 0,
-                     (int)IM_MG, cfg_.tol, cfg_.maxiter, n_active0_, partial_, counter_, st_);
+                     (int)IM_MG, cfg_.tol, cfg_.maxiter, n_proj_, partial_, counter_, st_);
   PCFD_LAUNCH_CHECK();
   LevelBuf& L0 = lv64_[0];
   // the hierarchy's level-0 vectors alias the solver's: b and the iterate
@@ -1308,7 +1310,7 @@ int PoissonEngine::apply_preconditioner(const double* r, double* z) {
   const dim3 g = tiles_for(op_.nx, op_.ny);
   PCFD_HIP_CHECK(hipMemcpyAsync(r_, r, (size_t)n_ * sizeof(double), hipMemcpyDeviceToDevice, s_));
   hipLaunchKernelGGL(k_open_for_apply, g, dim3(kBlock), 0, s_, op_, (const double*)r_,
-                     n_active0_, partial_, counter_, st_);
+                     n_proj_, partial_, counter_, st_);
   const bool f32 = cfg_.precision == 1;
   if (f32) {
     hipLaunchKernelGGL(k_project_in<float>, g, dim3(kBlock), 0, s_, op_, r_, (float*)lv32_[0].b,

@@ -181,6 +181,7 @@ class PoissonEngine {
   std::vector<LevelBuf> lv32_;
   uint8_t* fine_active_ = nullptr;
   int n_active0_ = 0;
+  int n_proj_ = 0;  // cells of the constant mode removed by projections (0: non-singular)
   double weight0_ = 1.0;
 };
 
