@@ -143,43 +143,52 @@ __device__ __forceinline__ void block_reduce(double (&v)[K], const int (&ops)[K]
   __syncthreads();
 }
 
-// Grid-wide deterministic reduction: each block contributes K partials at
-// slot `tile`; returns true in the single block that finishes last, in which
-// case `v` (valid in thread 0) holds the totals folded in tile order.
+// Grid-wide deterministic reductions are split over two launches.  The
+// producing kernel folds its values inside each workgroup and writes one row
+// of K partials per tile with plain stores (block_partials); a one-workgroup
+// fold kernel launched right after it (k_fold) sums the rows in tile order
+// and runs the reduction's epilogue.  The kernel boundary makes the partials
+// visible across XCDs, so no workgroup pays for a device-scope fence.
 template <int K>
-__device__ bool grid_reduce(double (&v)[K], const int (&ops)[K], double* partial,
-                            unsigned int* counter, int tile, int ntiles) {
-  __shared__ bool am_last;
+__device__ __forceinline__ void block_partials(double (&v)[K], const int (&ops)[K],
+                                               double* partial, int tile) {
   block_reduce<K>(v, ops);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < K; ++k) partial[(size_t)tile * K + k] = v[k];
-    __threadfence();
-    const unsigned int ticket = atomicAdd(counter, 1u);
-    am_last = (ticket == (unsigned int)(ntiles - 1));
   }
-  __syncthreads();
-  if (!am_last) return false;
-  __threadfence();
-  // Fixed-order fold: thread t owns tiles t, t+256, ... (sequential), then the
-  // block tree combines threads in index order.
-  double acc[K];
+}
+
+// Fixed-order fold of `ntiles` rows of K partials in one workgroup; thread 0
+// receives the totals.
+template <int K>
+__device__ __forceinline__ void fold_partials(double (&v)[K], const int (&ops)[K],
+                                              const double* partial, int ntiles) {
 #pragma unroll
-  for (int k = 0; k < K; ++k) acc[k] = (ops[k] == kSum) ? 0.0 : -INFINITY;
+  for (int k = 0; k < K; ++k) v[k] = (ops[k] == kSum) ? 0.0 : -INFINITY;
   for (int t = threadIdx.x; t < ntiles; t += kBlock) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const double x = partial[(size_t)t * K + k];
-      acc[k] = (ops[k] == kSum) ? acc[k] + x : fmax(acc[k], x);
+      v[k] = (ops[k] == kSum) ? v[k] + x : fmax(v[k], x);
     }
   }
-  block_reduce<K>(acc, ops);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) v[k] = acc[k];
-    *counter = 0u;  // ready for the next launch on this slot
-  }
-  return true;
+  block_reduce<K>(v, ops);
+}
+
+// Epi provides: static constexpr int K; ops(int (&)[K]) const; bool active()
+// const (false skips the epilogue, e.g. once a solve has finished); and
+// operator()(const double (&)[K]) const, run by thread 0 with the totals.
+template <class Epi>
+__global__ void __launch_bounds__(kBlock) k_fold(const double* __restrict__ partial, int ntiles,
+                                                 Epi epi) {
+  if (!epi.active()) return;
+  constexpr int K = Epi::K;
+  int ops[K];
+  epi.ops(ops);
+  double v[K];
+  fold_partials<K>(v, ops, partial, ntiles);
+  if (threadIdx.x == 0) epi(v);
 }
 
 }  // namespace pcfd

@@ -26,24 +26,25 @@ struct FineOp {
   double pref[4];
 };
 
-// A Galerkin coarse level on its own rectangular grid, stored as a stencil
+// A Galerkin coarse level on its own rectangular grid, stored as a row
 // dictionary: the level's operator rows take only a few hundred distinct
 // values (interior rows repeat by translation invariance; the rest sit near
-// walls and bodies), so each cell keeps a 2-byte class index and the class
-// table of coefficients (offset (di[s], dj[s]) is column s) stays resident
-// in L2.  Per class also the damped-Jacobi diagonal inverse and the active
-// flag (inactive rows are the identity).
+// walls and bodies), so each cell keeps a 2-byte class index and each class
+// keeps only its nonzeros (neighbour offset, coefficient), resident in L2.
+// `agg[k]` is the coarse cell that aggregates cell k (-1 if k is inactive),
+// i.e. the tentative prolongator of the next level.
 template <typename T>
 struct ClassLevel {
   int nx, ny, n;
   int px, py;
-  int S;                      // stencil offsets
-  const int8_t* di;
-  const int8_t* dj;
-  const uint16_t* cls;        // per cell
-  const T* table;             // n_classes x S
-  const T* cdinv;             // per class
-  const uint8_t* cactive;     // per class
+  const uint16_t* cls;     // per cell
+  const int* row_ptr;      // per class, n_classes + 1
+  const int* ent_lin;      // per entry: di*ny + dj (non-periodic neighbour)
+  const int8_t* ent_di;    // per entry, for periodic wrap
+  const int8_t* ent_dj;
+  const T* ent_val;        // per entry coefficient
+  const T* cdinv;          // per class 1/diag
+  const uint8_t* cactive;  // per class
 };
 
 // Device-resident scalars of one solve.  Kernels read them directly so the
@@ -84,7 +85,7 @@ struct MGLevelHost {
   int fx, fy;                    // aggregation factors towards the next level
   std::vector<int8_t> di, dj;    // stencil offsets (coarse levels)
   std::vector<uint16_t> cls;     // per-cell class (coarse levels)
-  std::vector<double> table;     // n_classes x S coefficients
+  std::vector<double> table;     // n_classes x S coefficients (dense rows)
   std::vector<double> cdinv;     // per-class 1/diag
   std::vector<uint8_t> cactive;  // per-class active flag
   std::vector<uint8_t> active;   // per-cell active flag (all levels)
@@ -106,6 +107,20 @@ struct SolveInfo {
   double residual;
   int converged;
   int status;
+};
+
+// Device buffers of one multigrid level (vectors in the cycle's precision).
+struct LevelBuf {
+  int nx = 0, ny = 0, n = 0, fx = 1, fy = 1;
+  void *b = nullptr, *x = nullptr, *y = nullptr, *r = nullptr, *t = nullptr;
+  int ncls = 0, nnz = 0, max_row = 0, lanes = 1;
+  uint16_t* cls = nullptr;
+  int *row_ptr = nullptr, *ent_lin = nullptr, *agg = nullptr;
+  int8_t *ent_di = nullptr, *ent_dj = nullptr;
+  void *ent_val = nullptr, *cdinv = nullptr;
+  uint8_t* cactive = nullptr;
+  double weight = 1.0;
+  double* cinv = nullptr;  // coarsest dense pseudo-inverse (FP64)
 };
 
 class PoissonEngine {
@@ -151,6 +166,7 @@ class PoissonEngine {
   template <typename T>
   int enqueue_vcycle_t(bool nonzero_start);
   int poll(int* iter, int* done);
+  int start(int mode, double* r);
   void free_levels();
 
   FineOp op_{};
@@ -166,17 +182,6 @@ class PoissonEngine {
   int predicted_ = 0;
 
   // multigrid
-  struct LevelBuf {
-    int nx, ny, n, fx, fy;
-    void *b = nullptr, *x = nullptr, *y = nullptr, *r = nullptr;  // precision-typed vectors
-    int S = 0, ncls = 0;
-    int8_t *di = nullptr, *dj = nullptr;
-    uint16_t* cls = nullptr;
-    void *table = nullptr, *cdinv = nullptr;
-    uint8_t* cactive = nullptr;
-    double weight;
-    double* cinv = nullptr;  // coarsest dense pseudo-inverse (FP64)
-  };
   std::vector<LevelBuf> lv64_;
   std::vector<LevelBuf> lv32_;
   uint8_t* fine_active_ = nullptr;

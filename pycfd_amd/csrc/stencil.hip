@@ -181,8 +181,7 @@ __global__ void __launch_bounds__(kBlock)
 k_diagnostics(const double* __restrict__ u, const double* __restrict__ v,
               const double* __restrict__ p, const double* __restrict__ up,
               const double* __restrict__ vp, const uint8_t* __restrict__ solid, int nx, int ny,
-              int ldu, int ldv, int ldp, double dx, double dy, double* partial,
-              unsigned int* counter, double* out) {
+              int ldu, int ldv, int ldp, double dx, double dy, double* partial) {
   const Tile2D t = tile_coords(ny + 3);
   const int i = t.i, j = t.j;
   double r[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -214,10 +213,20 @@ k_diagnostics(const double* __restrict__ u, const double* __restrict__ v,
     r[6] = uc * uc + vc * vc;
   }
   const int ops[7] = {kMax, kMax, kSum, kMax, kMax, kMax, kSum};
-  if (grid_reduce<7>(r, ops, partial, counter, t.tile, gridDim.x) && threadIdx.x == 0) {
-    for (int q = 0; q < 7; ++q) out[q] = r[q];
-  }
+  block_partials<7>(r, ops, partial, t.tile);
 }
+
+struct EpiDiag {
+  static constexpr int K = 7;
+  double* out;
+  __device__ void ops(int (&o)[K]) const {
+    o[0] = kMax; o[1] = kMax; o[2] = kSum; o[3] = kMax; o[4] = kMax; o[5] = kMax; o[6] = kSum;
+  }
+  __device__ bool active() const { return true; }
+  __device__ void operator()(const double (&v)[K]) const {
+    for (int q = 0; q < K; ++q) out[q] = v[q];
+  }
+};
 
 // ----------------------------------------------------------------------------
 // Host launchers.
@@ -278,8 +287,12 @@ int launch_diagnostics(hipStream_t s, const double* u, const double* v, const do
                        const double* up, const double* vp, const uint8_t* solid, int nx, int ny,
                        int ldu, int ldv, int ldp, double dx, double dy, double* partial,
                        unsigned int* counter, double* out) {
+  (void)counter;
   hipLaunchKernelGGL(k_diagnostics, grid_for(nx + 3, ny + 3), dim3(kBlock), 0, s, u, v, p, up,
-                     vp, solid, nx, ny, ldu, ldv, ldp, dx, dy, partial, counter, out);
+                     vp, solid, nx, ny, ldu, ldv, ldp, dx, dy, partial);
+  PCFD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_fold<EpiDiag>, dim3(1), dim3(kBlock), 0, s, (const double*)partial,
+                     num_tiles(nx + 3, ny + 3), EpiDiag{out});
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
