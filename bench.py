@@ -167,7 +167,7 @@ def main() -> int:
         "config": {"workload": f"cylinder Re=100 {args.nx}x{args.ny} (32x16 D), OUTLET, "
                                "mgcg tol 1e-10, SSP-RK3",
                    "path": main_path, "nx": args.nx, "ny": args.ny},
-        "paths": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv)
+        "paths": {k: {kk: (float(f"{vv:.5g}") if isinstance(vv, float) else vv)
                       for kk, vv in v.items()} for k, v in results.items()},
         "roofline": dominant,
         "kernels": roof,
