@@ -26,8 +26,8 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-KERNELS = {0: "k_cg_update", 1: "k_mgcg_dir<double>", 2: "k_smooth_fine<double>",
-           3: "k_residual_fine<double>"}
+KERNELS = {0: "k_cg_update", 1: "k_mgcg_dir<double>", 2: "k_fine_down<double>",
+           3: "k_fine_up<double>"}
 
 
 def log(msg: str) -> None:
@@ -48,6 +48,10 @@ def run_path(precision: str, steps: int, warmup: int, nx: int, ny: int):
     sim = Simulation(cfg, obstacle=body, u_init=u0, v_init=v0)
     log(f"[{precision}] setup {time.perf_counter() - t0:.1f} s "
         f"({sim.solver.pressure_solver.hierarchy})")
+    plan = sim.solver.ctx.mg_plan()
+    log(f"[{precision}] V-cycle plan: " + ", ".join(
+        f"{p['nx']}x{p['ny']}:" + (f"fused {p['tile'][0]}x{p['tile'][1]}" if p["fused"] else "passes")
+        for p in plan))
     if warmup:
         sim.run(max_steps=warmup)
     torch.cuda.synchronize()
@@ -62,6 +66,7 @@ def run_path(precision: str, steps: int, warmup: int, nx: int, ny: int):
         "max_div": float(max(hist["max_div"])),
         "iterations_last_solve": int(sim.solver.pressure_solver.last_iterations),
         "cd_last": float(sim.force_coefficients()[0]),
+        "mg_plan": [[p["nx"], p["ny"], int(p["fused"]), *p["tile"]] for p in plan],
         "t": float(sim.fields.t),
     }
     log(f"[{precision}] {steps} steps in {wall:.3f} s -> {out['mlups']:.1f} MLUPS, "

@@ -93,6 +93,12 @@ typedef struct pcfd_solver_desc {
   double omega;       /* SOR over-relaxation (config.py:219) or Jacobi damping */
   int32_t sweeps;     /* V-cycle pre/post sweeps (config.py:220) */
   int32_t precision;  /* 0: FP64 throughout; 1: FP32 V-cycle, FP64 Krylov */
+  /* Multigrid level kernels (no counterpart in the reference: its V-cycle is
+   * SciPy sparse products).  fused_levels = 0 runs every stored level pass by
+   * pass; tile_tx / tile_ty fix the level-0 tile (0: cost model).  A zeroed
+   * tail keeps the defaults. */
+  int32_t fused_levels;
+  int32_t tile_tx, tile_ty;
 } pcfd_solver_desc;
 
 /* One level of the aggregation hierarchy (multigrid.py:293-322).  Level 0 is
@@ -204,11 +210,19 @@ int pcfd_mg_apply(pcfd_ctx* ctx, const double* r, double* z);
 /* --- measurement -------------------------------------------------------
  * Times `reps` back-to-back launches of one hot fine-level kernel on the
  * context's stream with HIP events (which: 0 CG update, 1 multigrid-CG
- * direction + SpMV, 2 damped-Jacobi sweep, 3 residual) and reports the mean
- * duration and the kernel's algorithmic bytes per launch.  No reference
- * counterpart: measurement only. */
+ * direction + SpMV, 2 level-0 V-cycle down leg, 3 level-0 up leg) and reports
+ * the mean duration and the kernel's algorithmic bytes per launch.  No
+ * reference counterpart: measurement only. */
 int pcfd_time_kernel(pcfd_ctx* ctx, int32_t which, int32_t reps, double* avg_ms,
                      double* bytes_per_launch);
+
+/* The multigrid plan of the configured solver, for reporting: per level (all
+ * levels of the cycle precision, finest first) five integers -- nx, ny,
+ * fused (1: one fused launch per leg, 0: pass by pass; the coarsest level is
+ * the dense solve and reports 0), tile x, tile y -- written to out (capacity
+ * `cap` integers).  Returns the number of levels, or a negative status.  No
+ * reference counterpart. */
+int pcfd_mg_plan(pcfd_ctx* ctx, int32_t* out, int32_t cap);
 
 #ifdef __cplusplus
 }

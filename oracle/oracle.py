@@ -408,7 +408,11 @@ def residual_norm(sysm: System, p, b) -> float:
 # Aggregation multigrid: multigrid.py:129-543
 # --------------------------------------------------------------------------
 class Hierarchy:
-    def __init__(self, A, shape, active, singular, sweeps=1):
+    """multigrid.py:324-543.  ``coarse_max`` moves the exact (pinned LU + mean
+    removal) coarsest solve up to the first level with at most that many
+    unknowns, dropping the levels below it; 96 is the reference hierarchy."""
+
+    def __init__(self, A, shape, active, singular, sweeps=1, coarse_max=96):
         self.sweeps = sweeps
         self.levels = []
         A = sp.csr_matrix(A)
@@ -434,6 +438,10 @@ class Hierarchy:
             Ac.sum_duplicates()
             lev = self._level(Ac, (ncx, ncy), cact, singular, P)
             self.levels.append(lev)
+        for m, lev in enumerate(self.levels):
+            if lev["A"].shape[0] <= coarse_max:
+                del self.levels[m + 1:]
+                break
         last = self.levels[-1]
         last["pin"] = int(np.flatnonzero(last["active"])[0]) if last["active"].any() else 0
         M = last["A"]

@@ -58,6 +58,7 @@ class SolverDesc(C.Structure):
     _fields_ = [
         ("kind", C.c_int32), ("tol", C.c_double), ("maxiter", C.c_int32),
         ("omega", C.c_double), ("sweeps", C.c_int32), ("precision", C.c_int32),
+        ("fused_levels", C.c_int32), ("tile_tx", C.c_int32), ("tile_ty", C.c_int32),
     ]
 
 
@@ -109,6 +110,7 @@ _SIGNATURES = {
     "pcfd_poisson_apply": [C.c_void_p, C.c_void_p, C.c_void_p],
     "pcfd_mg_apply": [C.c_void_p, C.c_void_p, C.c_void_p],
     "pcfd_time_kernel": [C.c_void_p, C.c_int32, C.c_int32, _dp, _dp],
+    "pcfd_mg_plan": [C.c_void_p, C.POINTER(C.c_int32), C.c_int32],
 }
 
 EXPORTED = tuple(_SIGNATURES)

@@ -90,6 +90,7 @@ DEFAULT_POISSON_TOL = 1.0e-10
 DEFAULT_POISSON_MAXITER = 20_000
 DEFAULT_SOR_OMEGA = 1.8
 DEFAULT_MG_SWEEPS = 1
+DEFAULT_MG_COARSE_MAX = 2048
 
 _CLUSTER_MODES = ("low", "walls", "centre")
 
@@ -107,9 +108,12 @@ def _normalise_cluster(mode: str) -> str:
 class SimulationConfig:
     """Complete description of one run (config.py:179-405).
 
-    ``precision`` is the one field the reference does not have: ``"fp64"``
-    runs every kernel in FP64 (the reference's arithmetic); ``"mixed"`` runs
-    the multigrid V-cycle in FP32 inside the FP64 Krylov iteration.
+    Two fields the reference does not have.  ``precision``: ``"fp64"`` runs
+    every kernel in FP64 (the reference's arithmetic); ``"mixed"`` runs the
+    multigrid V-cycle in FP32 inside the FP64 Krylov iteration.
+    ``mg_coarse_max``: the device V-cycle solves the first multigrid level with
+    at most this many unknowns exactly (dense pseudo-inverse) instead of
+    recursing further; 96 keeps the reference's hierarchy whole.
     """
 
     nx: int = 128
@@ -154,6 +158,7 @@ class SimulationConfig:
     name: str = "simulation"
 
     precision: str = "fp64"
+    mg_coarse_max: int = DEFAULT_MG_COARSE_MAX
 
     def __post_init__(self) -> None:
         self.solver_type = SolverType(self.solver_type)
@@ -218,6 +223,8 @@ class SimulationConfig:
             raise ValueError("max_steps must be >= 1 when given")
         if self.precision not in ("fp64", "mixed"):
             raise ValueError(f"precision must be 'fp64' or 'mixed', got {self.precision!r}")
+        if not 1 <= self.mg_coarse_max <= 8192:
+            raise ValueError(f"mg_coarse_max must lie in [1, 8192], got {self.mg_coarse_max}")
         missing = set(WALLS) - set(self.boundary_config)
         if missing:
             raise ValueError(f"boundary_config is missing walls: {sorted(missing)}")

@@ -85,9 +85,11 @@ class DeviceContext:
 
     # -- configuration ---------------------------------------------------
     def set_solver(self, kind: str, tol: float, maxiter: int, omega: float, sweeps: int,
-                   precision: int, levels: list[dict] | None) -> None:
+                   precision: int, levels: list[dict] | None, *, fused_levels: bool = True,
+                   tile: tuple[int, int] = (0, 0)) -> None:
         sd = _lib.SolverDesc(_lib.SOLVER_CODES[kind], float(tol), int(maxiter), float(omega),
-                             int(sweeps), int(precision))
+                             int(sweeps), int(precision), int(bool(fused_levels)),
+                             int(tile[0]), int(tile[1]))
         keep = []
         if levels:
             arr = (_lib.MGLevelDesc * len(levels))()
@@ -192,6 +194,16 @@ class DeviceContext:
         ms, b = C.c_double(0), C.c_double(0)
         self._call("pcfd_time_kernel", int(which), int(reps), C.byref(ms), C.byref(b))
         return float(ms.value), float(b.value)
+
+    def mg_plan(self) -> list[dict]:
+        """Per multigrid level: grid, whether its legs run fused, and the tile."""
+        buf = (C.c_int32 * (5 * 32))()
+        n = int(self._lib.pcfd_mg_plan(self.handle, buf, len(buf)))
+        if n < 0:
+            _lib.check(-n, "pcfd_mg_plan")
+        vals = list(buf)
+        return [{"nx": vals[5 * k], "ny": vals[5 * k + 1], "fused": bool(vals[5 * k + 2]),
+                 "tile": (vals[5 * k + 3], vals[5 * k + 4])} for k in range(min(n, 32))]
 
     def synchronize(self) -> None:
         self._call("pcfd_synchronize")

@@ -9,11 +9,13 @@ dropped aggregates (:276-287, 380-383), damping weights from a seeded
 exactly as a pinned solve followed by mean removal (:514-543).
 
 Because every level is a structured grid, each coarse operator is handed to
-the GPU as a structured stencil (one coefficient plane per stencil offset,
-coalesced along y) rather than as CSR, and the prolongator is never stored:
-the device applies ``P = (I - w D^-1 A) P_tent`` and ``P^T`` on the fly from
-the level's own operator.  The coarsest level becomes a dense FP64
-pseudo-inverse applied by one workgroup.
+the GPU as a row dictionary: a per-cell class index into the level's distinct
+operator rows (interior rows repeat by translation invariance), and the
+prolongator is never stored: the device applies ``P = (I - w D^-1 A) P_tent``
+and ``P^T`` on the fly from the level's own operator.  The device's coarsest
+level is the first one with at most ``coarse_max`` unknowns, solved by its
+dense FP64 pseudo-inverse; with ``coarse_max`` at the reference's 96 this is
+the reference's own coarsest level.
 """
 
 from __future__ import annotations
@@ -168,7 +170,13 @@ def to_class_stencil(A: sp.csr_matrix, shape: tuple[int, int], periodic: tuple[b
 
 
 def coarsest_pseudo_inverse(level: Level) -> np.ndarray:
-    """Dense operator of the coarsest solve, projections included (multigrid.py:514-543)."""
+    """Dense operator of the coarsest solve, projections included (multigrid.py:514-543).
+
+    Singular case: ``Pi inv(M) Z Pi`` with M the operator pinned at ``pin``,
+    Z zeroing the pinned right-hand side entry and Pi the mean removal on the
+    active set (zero on inactive cells); the two projections are applied as
+    column and row mean subtractions.  Otherwise inactive rows are zeroed.
+    """
     n = level.n
     A = level.A.toarray()
     act = level.active
@@ -176,13 +184,16 @@ def coarsest_pseudo_inverse(level: Level) -> np.ndarray:
         M = A.copy()
         M[level.pin, :] = 0.0
         M[level.pin, level.pin] = 1.0
-        na = max(int(act.sum()), 1)
-        proj = np.zeros((n, n))
+        Y = np.linalg.inv(M)
+        Y[:, level.pin] = 0.0
         ia = np.flatnonzero(act)
-        proj[np.ix_(ia, ia)] = np.eye(ia.size) - 1.0 / na
-        pin_zero = np.eye(n)
-        pin_zero[level.pin, level.pin] = 0.0
-        return proj @ np.linalg.inv(M) @ pin_zero @ proj
+        out = np.zeros((n, n))
+        if ia.size:
+            Yaa = Y[np.ix_(ia, ia)]
+            Yaa -= Yaa.mean(axis=0, keepdims=True)
+            Yaa -= Yaa.mean(axis=1, keepdims=True)
+            out[np.ix_(ia, ia)] = Yaa
+        return out
     zero_inactive = np.diag(act.astype(float))
     return zero_inactive @ np.linalg.inv(A)
 
@@ -208,7 +219,7 @@ class MultigridHierarchy:
         self.levels: list[Level] = []
         self._build(sp.csr_matrix(A), tuple(shape), np.asarray(active, dtype=bool).ravel(),
                     bool(singular))
-        self._device = None
+        self._device: dict[int, list[dict]] = {}
 
     def _make_level(self, A, shape, active, singular) -> Level:
         diag = A.diagonal()
@@ -256,24 +267,41 @@ class MultigridHierarchy:
         return (f"MultigridHierarchy({self.n_levels} levels, {self.unknowns[0]} -> "
                 f"{self.unknowns[-1]} unknowns, complexity {self.operator_complexity:.2f})")
 
-    def device_levels(self) -> list[dict]:
-        """The hierarchy in the form ``pcfd_set_pressure_solver`` takes."""
-        if self._device is not None:
-            return self._device
-        out = []
+    def device_levels(self, coarse_max: int = COARSEST_UNKNOWNS) -> list[dict]:
+        """The hierarchy in the form ``pcfd_set_pressure_solver`` takes.
+
+        The device solves the first level with at most ``coarse_max`` unknowns
+        exactly (its dense pseudo-inverse) and drops the levels below it;
+        ``coarse_max = COARSEST_UNKNOWNS`` keeps the reference hierarchy whole.
+        """
+        m = len(self.levels) - 1
         for l, lv in enumerate(self.levels):
-            d = {"nx": lv.shape[0], "ny": lv.shape[1], "fx": lv.factors[0],
-                 "fy": lv.factors[1], "weight": lv.weight,
-                 "active": lv.active.astype(np.uint8)}
+            if lv.n <= coarse_max:
+                m = l
+                break
+        if m in self._device:
+            return self._device[m]
+        out = []
+        for l, lv in enumerate(self.levels[:m + 1]):
+            last = l == m
+            fx, fy = (1, 1) if last else lv.factors
+            d = {"nx": lv.shape[0], "ny": lv.shape[1], "fx": fx, "fy": fy,
+                 "weight": lv.weight, "active": lv.active.astype(np.uint8)}
             if l > 0:
                 di, dj, cls, table, cdinv, cact = to_class_stencil(
                     lv.A, lv.shape, self.periodic, lv.active)
                 d.update(di=di, dj=dj, cls=cls, table=table, class_dinv=cdinv,
                          class_active=cact)
-            if l == len(self.levels) - 1:
-                d["coarse_inv"] = coarsest_pseudo_inverse(lv)
+            if last:
+                pin_level = lv
+                if lv is not self.levels[-1]:
+                    pin_level = Level(A=lv.A, dinv=lv.dinv, weight=lv.weight, active=lv.active,
+                                      shape=lv.shape, singular=lv.singular,
+                                      pin=int(np.flatnonzero(lv.active)[0])
+                                      if lv.active.any() else 0)
+                d["coarse_inv"] = coarsest_pseudo_inverse(pin_level)
             out.append(d)
-        self._device = out
+        self._device[m] = out
         return out
 
     def stencil_sizes(self) -> list[tuple[int, int]]:

@@ -17,8 +17,8 @@ from dataclasses import dataclass, field
 import numpy as np
 import scipy.sparse as sp
 
-from ..config import (DEFAULT_MG_SWEEPS, DEFAULT_POISSON_MAXITER, DEFAULT_POISSON_TOL,
-                      DEFAULT_SOR_OMEGA, PressureSolver)
+from ..config import (DEFAULT_MG_COARSE_MAX, DEFAULT_MG_SWEEPS, DEFAULT_POISSON_MAXITER,
+                      DEFAULT_POISSON_TOL, DEFAULT_SOR_OMEGA, PressureSolver)
 from .mesh import StructuredMesh
 
 _ZERO_RHS_NORM = 1.0e-300
@@ -143,11 +143,15 @@ class PoissonSolverBase:
 
     def __init__(self, system: PoissonSystem, tol: float = DEFAULT_POISSON_TOL,
                  maxiter: int = DEFAULT_POISSON_MAXITER, *, context=None,
-                 precision: str = "fp64") -> None:
+                 precision: str = "fp64", coarse_max: int = DEFAULT_MG_COARSE_MAX,
+                 fused_levels: bool = True, tile: tuple[int, int] = (0, 0)) -> None:
         self.system = system
         self.tol = tol
         self.maxiter = maxiter
         self.precision = precision
+        self.coarse_max = int(coarse_max)
+        self.fused_levels = bool(fused_levels)
+        self.tile = (int(tile[0]), int(tile[1]))
         self.last_iterations = 0
         self.last_residual = 0.0
         self._ctx = context
@@ -190,7 +194,8 @@ class PoissonSolverBase:
     def configure(self) -> None:
         tol, maxiter = self._tol_maxiter()
         self.context.set_solver(self._device_kind, tol, maxiter, self._omega(), self._sweeps(),
-                                1 if self.precision == "mixed" else 0, self._levels())
+                                1 if self.precision == "mixed" else 0, self._levels(),
+                                fused_levels=self.fused_levels, tile=self.tile)
         self._configured = True
 
     def record(self, iterations: int, residual: float, converged: bool) -> None:
@@ -322,7 +327,7 @@ class MultigridSolverBase(PoissonSolverBase):
                                             periodic=(system.periodic_x, system.periodic_y))
 
     def _levels(self):
-        return self.hierarchy.device_levels()
+        return self.hierarchy.device_levels(self.coarse_max)
 
     def _sweeps(self) -> int:
         return self.smooth_sweeps
@@ -394,11 +399,12 @@ def make_pressure_solver(kind: PressureSolver, system: PoissonSystem,
                          maxiter: int = DEFAULT_POISSON_MAXITER,
                          sor_omega: float = DEFAULT_SOR_OMEGA,
                          mg_sweeps: int = DEFAULT_MG_SWEEPS, *, context=None,
-                         precision: str = "fp64") -> PoissonSolverBase:
+                         precision: str = "fp64",
+                         mg_coarse_max: int = DEFAULT_MG_COARSE_MAX) -> PoissonSolverBase:
     """Instantiate the requested solver (pressure.py:645-660)."""
     kind = PressureSolver(kind)
     cls = _SOLVERS[kind]
-    kw = {"context": context, "precision": precision}
+    kw = {"context": context, "precision": precision, "coarse_max": mg_coarse_max}
     if cls is SORSolver:
         return SORSolver(system, tol, maxiter, omega=sor_omega, **kw)
     if issubclass(cls, MultigridSolverBase):

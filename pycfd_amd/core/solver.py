@@ -106,7 +106,8 @@ class ProjectionSolver:
         self.pressure_solver = make_pressure_solver(
             config.pressure_solver, self.poisson, tol=config.poisson_tol,
             maxiter=config.poisson_maxiter, sor_omega=config.sor_omega,
-            mg_sweeps=config.mg_sweeps, context=self.ctx, precision=config.precision)
+            mg_sweeps=config.mg_sweeps, context=self.ctx, precision=config.precision,
+            mg_coarse_max=config.mg_coarse_max)
         self.pressure_solver.configure()
         self._has_guess = False
         self._force = np.zeros(2 + 2 * len(self.body_masks))

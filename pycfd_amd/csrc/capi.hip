@@ -446,6 +446,13 @@ int pcfd_set_pressure_solver(pcfd_ctx* c, const pcfd_solver_desc* sd,
   cfg.omega = sd->omega;
   cfg.sweeps = sd->sweeps;
   cfg.precision = sd->precision;
+  cfg.fused = sd->fused_levels;
+  cfg.tile_tx = sd->tile_tx;
+  cfg.tile_ty = sd->tile_ty;
+  if (cfg.tile_tx < 0 || cfg.tile_ty < 0) {
+    set_last_error("tile sizes must be >= 0, got %dx%d", cfg.tile_tx, cfg.tile_ty);
+    return kErrBadArg;
+  }
   if (cfg.sweeps < 1) {
     set_last_error("mg sweeps must be at least 1, got %d", cfg.sweeps);
     return kErrBadArg;
@@ -687,6 +694,15 @@ int pcfd_time_kernel(pcfd_ctx* c, int32_t which, int32_t reps, double* avg_ms,
     return kErrBadArg;
   }
   return c->engine.time_kernel(which, reps, avg_ms, bytes_per_launch);
+}
+
+int pcfd_mg_plan(pcfd_ctx* c, int32_t* out, int32_t cap) {
+  if (int rc = check_ctx(c)) return -rc;
+  if (!out || cap < 0) {
+    set_last_error("pcfd_mg_plan: bad argument");
+    return -kErrBadArg;
+  }
+  return c->engine.plan(out, cap);
 }
 
 int pcfd_mg_apply(pcfd_ctx* c, const double* r, double* z) {

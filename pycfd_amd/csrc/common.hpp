@@ -11,9 +11,8 @@
 //     along y (one wavefront per row segment, fully coalesced FP64 loads)
 //     by 4 rows along x.
 //   * Reductions are deterministic: every block writes its partials to a
-//     slot indexed by its tile number and the last block to finish (atomic
-//     ticket) folds the partials in a fixed order.  Nothing that is
-//     parity-checked uses floating-point atomics.
+//     slot indexed by its tile number and a one-workgroup fold launch sums
+//     the slots in a fixed order.  No floating-point atomics are used.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -164,13 +163,23 @@ __device__ __forceinline__ void block_partials(double (&v)[K], const int (&ops)[
 template <int K>
 __device__ __forceinline__ void fold_partials(double (&v)[K], const int (&ops)[K],
                                               const double* partial, int ntiles) {
+  constexpr int kBatch = 8;  // rows loaded before any is combined: loads overlap
 #pragma unroll
   for (int k = 0; k < K; ++k) v[k] = (ops[k] == kSum) ? 0.0 : -INFINITY;
-  for (int t = threadIdx.x; t < ntiles; t += kBlock) {
+  for (int t0 = threadIdx.x; t0 < ntiles; t0 += kBlock * kBatch) {
+    double x[kBatch][K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const double x = partial[(size_t)t * K + k];
-      v[k] = (ops[k] == kSum) ? v[k] + x : fmax(v[k], x);
+    for (int u = 0; u < kBatch; ++u) {
+      const int t = t0 + u * kBlock;
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        x[u][k] = (t < ntiles) ? partial[(size_t)t * K + k] : ((ops[k] == kSum) ? 0.0 : -INFINITY);
+    }
+    // Same order as a plain strided loop: rows t0, t0 + kBlock, ...
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) v[k] = (ops[k] == kSum) ? v[k] + x[u][k] : fmax(v[k], x[u][k]);
     }
   }
   block_reduce<K>(v, ops);

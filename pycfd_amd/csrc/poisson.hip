@@ -20,79 +20,23 @@
 // kernels turn into no-ops once `done` is set, so iterations are enqueued
 // without waiting for the host.
 #include "poisson.hpp"
+#include "poisson_dev.hpp"
+#include "vcycle.hpp"
 
 #include <algorithm>
+#include <array>
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 namespace pcfd {
-
-// ----------------------------------------------------------------------------
-// Fine operator helpers
-// ----------------------------------------------------------------------------
-__device__ __forceinline__ double fine_diag(const FineOp& A, uint8_t f, int i, int j) {
-  if (!(f & F_SOLV)) return 1.0;
-  // pressure.py:219-238: one -coeff per active face, -2*coeff per Dirichlet
-  // wall face, accumulated left, right, bottom, top.
-  double d = 0.0;
-  if (f & F_L) d -= A.cx;
-  if (A.dir[0] && i == 0) d -= 2.0 * A.cx;
-  if (f & F_R) d -= A.cx;
-  if (A.dir[1] && i == A.nx - 1) d -= 2.0 * A.cx;
-  if (f & F_B) d -= A.cy;
-  if (A.dir[2] && j == 0) d -= 2.0 * A.cy;
-  if (f & F_T) d -= A.cy;
-  if (A.dir[3] && j == A.ny - 1) d -= 2.0 * A.cy;
-  return d;
-}
-
-struct Nb {
-  int kL, kR, kB, kT;
-};
-
-__device__ __forceinline__ Nb fine_neighbours(const FineOp& A, int i, int j) {
-  const int ny = A.ny, nx = A.nx, k = i * ny + j;
-  Nb n;
-  n.kL = (i > 0) ? k - ny : (A.px ? k + (nx - 1) * ny : k);
-  n.kR = (i < nx - 1) ? k + ny : (A.px ? k - (nx - 1) * ny : k);
-  n.kB = (j > 0) ? k - 1 : (A.py ? k + ny - 1 : k);
-  n.kT = (j < ny - 1) ? k + 1 : (A.py ? k - (ny - 1) : k);
-  return n;
-}
-
-// y = (A x)[k], terms in CSR column order for a non-wrapping row.
-template <typename T, class X>
-__device__ __forceinline__ T fine_apply(const FineOp& A, uint8_t f, const Nb& nb, int k, T d,
-                                        const X& x) {
-  T y = (T)0;
-  if (f & F_L) y += (T)A.cx * x(nb.kL);
-  if (f & F_B) y += (T)A.cy * x(nb.kB);
-  y += d * x(k);
-  if (f & F_T) y += (T)A.cy * x(nb.kT);
-  if (f & F_R) y += (T)A.cx * x(nb.kR);
-  return y;
-}
-
-__device__ __forceinline__ bool cell_of_tile(int n_x, int n_y, Tile2D& t) {
-  t = tile_coords(n_y);
-  return t.i < n_x && t.j < n_y;
-}
-
-__device__ __forceinline__ double dirichlet_rhs(const FineOp& A, int i, int j) {
-  // pressure.py:233-238: bc -= 2*coeff*p_ref on the faces of a Dirichlet wall.
-  double bc = 0.0;
-  if (A.dir[0] && i == 0) bc -= 2.0 * A.cx * A.pref[0];
-  if (A.dir[1] && i == A.nx - 1) bc -= 2.0 * A.cx * A.pref[1];
-  if (A.dir[2] && j == 0) bc -= 2.0 * A.cy * A.pref[2];
-  if (A.dir[3] && j == A.ny - 1) bc -= 2.0 * A.cy * A.pref[3];
-  return bc;
-}
 
 // ----------------------------------------------------------------------------
 // Reduction epilogues (run by k_fold in one thread after the partials are in)
 // ----------------------------------------------------------------------------
 enum InitMode : int { IM_JPCG = 0, IM_MGCG = 1, IM_RELAX = 2, IM_MG = 3 };
 enum CheckMode : int { CM_RELAX = 0, CM_MG = 1, CM_FINAL = 2 };
-enum SmoothReduce : int { SR_NONE = 0, SR_MG = 1, SR_MGCG = 2 };
 
 struct EpiPrepare {  // fluid mean of the right-hand side (pressure.py:306-307)
   static constexpr int K = 1;
@@ -572,59 +516,8 @@ k_scatter(FineOp A, double* __restrict__ x, const SolveState* st, double* __rest
 }
 
 // ----------------------------------------------------------------------------
-// Multigrid, level 0: the matrix-free fine operator.  T is the cycle's
-// storage and arithmetic precision.
+// Multigrid helpers (the level kernels themselves are in vcycle.hip)
 // ----------------------------------------------------------------------------
-template <typename T>
-struct FineLevelArgs {
-  FineOp A;
-  T weight;
-};
-
-template <typename T>
-__device__ __forceinline__ T fine_dinv(const FineOp& A, int k) {
-  const int i = k / A.ny, j = k - i * A.ny;
-  return (T)(1.0 / fine_diag(A, A.flags[k], i, j));
-}
-
-// Damped Jacobi sweep (multigrid.py:424-428); x_in == nullptr means a zero
-// start.  A pending mean shift may be applied to x_in, and the last sweep of
-// a cycle may take the sums sum(x), sum(b*x), sum(b) over the active set.
-template <typename T>
-__global__ void __launch_bounds__(kBlock)
-k_smooth_fine(FineLevelArgs<T> L, const T* __restrict__ b, const T* __restrict__ x_in,
-              T* __restrict__ x_out, int shift_in, int reduce, double* partial,
-              const SolveState* st) {
-  if (st->done) return;
-  const FineOp& A = L.A;
-  Tile2D t;
-  double s[3] = {0.0, 0.0, 0.0};
-  const T m = shift_in ? (T)st->mean_x : (T)0;
-  if (cell_of_tile(A.nx, A.ny, t)) {
-    const int i = t.i, j = t.j, k = i * A.ny + j;
-    const uint8_t f = A.flags[k];
-    const T d = (T)fine_diag(A, f, i, j);
-    const T wd = L.weight * ((T)1 / d);
-    T val;
-    if (x_in) {
-      const Nb nb = fine_neighbours(A, i, j);
-      auto xv = [&](int q) -> T { return (A.flags[q] & F_SOLV) ? x_in[q] - m : x_in[q]; };
-      val = xv(k) + wd * (b[k] - fine_apply<T>(A, f, nb, k, d, xv));
-    } else {
-      val = wd * b[k];
-    }
-    x_out[k] = val;
-    if (reduce != SR_NONE && (f & F_SOLV)) {
-      s[0] = (double)val;
-      s[1] = (double)b[k] * (double)val;
-      s[2] = (double)b[k];
-    }
-  }
-  if (reduce == SR_NONE) return;
-  const int ops[3] = {kSum, kSum, kSum};
-  block_partials<3>(s, ops, partial, t.tile);
-}
-
 // Projection sums of a single-level hierarchy (the coarsest solve is the
 // whole cycle).
 template <typename T>
@@ -646,72 +539,6 @@ k_cycle_sums(FineOp A, const T* __restrict__ b, const T* __restrict__ z, double*
   block_partials<3>(s, ops, partial, t.tile);
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kBlock)
-k_residual_fine(FineOp A, const T* __restrict__ b, const T* __restrict__ x, T* __restrict__ r,
-                const SolveState* st) {
-  if (st->done) return;
-  Tile2D t;
-  if (!cell_of_tile(A.nx, A.ny, t)) return;
-  const int i = t.i, j = t.j, k = i * A.ny + j;
-  const uint8_t f = A.flags[k];
-  const Nb nb = fine_neighbours(A, i, j);
-  auto xv = [&](int q) -> T { return x[q]; };
-  r[k] = b[k] - fine_apply<T>(A, f, nb, k, (T)fine_diag(A, f, i, j), xv);
-}
-
-// Restriction b_c = P^T r with P = (I - w D^-1 A) P_tent
-// (multigrid.py:248-273): b_c[I] = sum over active a in block I of
-// (r - w A D^-1 r)[a].  One thread per coarse cell.
-template <typename T>
-__global__ void __launch_bounds__(kBlock)
-k_restrict_fine(FineLevelArgs<T> L, int fx, int fy, int cnx, int cny, const T* __restrict__ r,
-                T* __restrict__ bc, const SolveState* st) {
-  if (st->done) return;
-  const FineOp& A = L.A;
-  Tile2D t;
-  if (!cell_of_tile(cnx, cny, t)) return;
-  T acc = (T)0;
-  for (int a = 0; a < fx; ++a) {
-    const int i = t.i * fx + a;
-    if (i >= A.nx) break;
-    for (int c = 0; c < fy; ++c) {
-      const int j = t.j * fy + c;
-      if (j >= A.ny) break;
-      const int k = i * A.ny + j;
-      const uint8_t f = A.flags[k];
-      if (!(f & F_SOLV)) continue;
-      const Nb nb = fine_neighbours(A, i, j);
-      auto tv = [&](int q) -> T { return r[q] * fine_dinv<T>(A, q); };
-      const T atr = fine_apply<T>(A, f, nb, k, (T)fine_diag(A, f, i, j), tv);
-      acc += r[k] - L.weight * atr;
-    }
-  }
-  bc[t.i * cny + t.j] = acc;
-}
-
-// Prolongation x += P x_c = e - w D^-1 A e with e = P_tent x_c.
-template <typename T>
-__global__ void __launch_bounds__(kBlock)
-k_prolong_fine(FineLevelArgs<T> L, int fx, int fy, int cny, const T* __restrict__ xc,
-               T* __restrict__ x, const SolveState* st) {
-  if (st->done) return;
-  const FineOp& A = L.A;
-  Tile2D t;
-  if (!cell_of_tile(A.nx, A.ny, t)) return;
-  const int i = t.i, j = t.j, k = i * A.ny + j;
-  const uint8_t f = A.flags[k];
-  const Nb nb = fine_neighbours(A, i, j);
-  auto ev = [&](int q) -> T {
-    if (!(A.flags[q] & F_SOLV)) return (T)0;
-    const int qi = q / A.ny, qj = q - qi * A.ny;
-    return xc[(qi / fx) * cny + (qj / fy)];
-  };
-  const T d = (T)fine_diag(A, f, i, j);
-  const T ae = fine_apply<T>(A, f, nb, k, d, ev);
-  x[k] += ev(k) - L.weight * ((T)1 / d) * ae;
-}
-
 // ----------------------------------------------------------------------------
 // Multigrid, levels >= 1: stored Galerkin operators (row dictionary), one
 // row per LPR lanes; the lanes split the row's nonzeros and combine with
@@ -721,7 +548,6 @@ enum RowOp : int {
   RO_SMOOTH0_RES = 0,  // x = w D^-1 b (zero start), r = b - A x, t = D^-1 r
   RO_SMOOTH = 1,       // x_out = x + w D^-1 (b - A x)
   RO_RES = 2,          // r = b - A x, t = D^-1 r
-  RO_RESTRICT_Y = 3,   // y = r - w A t on active rows, 0 elsewhere
   RO_PROLONG = 4,      // x += e - w D^-1 A e, e = P_tent x_c
 };
 
@@ -750,14 +576,26 @@ k_cls_rows(ClassLevel<T> L, T w, const T* __restrict__ b, const T* __restrict__ 
   if (valid) {
     c = L.cls[row];
     const int e1 = L.row_ptr[c + 1];
-    for (int e = L.row_ptr[c] + lane; e < e1; e += LPR) {
-      const int q = cls_nb(L, row, e);
-      T xv;
-      if (OP == RO_SMOOTH0_RES) xv = w * L.cdinv[L.cls[q]] * b[q];
-      else if (OP == RO_PROLONG) xv = (agg[q] >= 0) ? xc[agg[q]] : (T)0;
-      else xv = xin[q];
-      acc += L.ent_val[e] * xv;
+    auto xval = [&](int q) -> T {
+      if (OP == RO_SMOOTH0_RES) return w * L.cdinv[L.cls[q]] * b[q];
+      if (OP == RO_PROLONG) return (agg[q] >= 0) ? xc[agg[q]] : (T)0;
+      return xin[q];
+    };
+    int e = L.row_ptr[c] + lane;
+    // Four entries per trip with their loads issued together; the sum keeps
+    // the entry order of the plain loop.
+    for (; e + 3 * LPR < e1; e += 4 * LPR) {
+      const int q0 = cls_nb(L, row, e), q1 = cls_nb(L, row, e + LPR);
+      const int q2 = cls_nb(L, row, e + 2 * LPR), q3 = cls_nb(L, row, e + 3 * LPR);
+      const T a0 = L.ent_val[e], a1 = L.ent_val[e + LPR];
+      const T a2 = L.ent_val[e + 2 * LPR], a3 = L.ent_val[e + 3 * LPR];
+      const T x0 = xval(q0), x1 = xval(q1), x2 = xval(q2), x3 = xval(q3);
+      acc += a0 * x0;
+      acc += a1 * x1;
+      acc += a2 * x2;
+      acc += a3 * x3;
     }
+    for (; e < e1; e += LPR) acc += L.ent_val[e] * xval(cls_nb(L, row, e));
   }
 #pragma unroll
   for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
@@ -775,51 +613,57 @@ k_cls_rows(ClassLevel<T> L, T w, const T* __restrict__ b, const T* __restrict__ 
     const T rk = b[row] - acc;
     o2[row] = rk;
     o3[row] = dinv * rk;
-  } else if (OP == RO_RESTRICT_Y) {
-    o1[row] = L.cactive[c] ? b[row] - w * acc : (T)0;
   } else {  // RO_PROLONG
     const T e = (agg[row] >= 0) ? xc[agg[row]] : (T)0;
     o1[row] = o1[row] + e - w * dinv * acc;
   }
 }
 
-// b_c[I] = sum of y over the cells of block I (P_tent^T y).
-template <typename T>
+// Restriction in one pass: b_c[I] = sum over the cells a of block I of
+// (r - w A t)[a] on active rows (t = D^-1 r), LPR lanes per fine row; the
+// block's cells are summed in x-major order (P_tent^T y with y = P-smoothed r).
+template <typename T, int LPR>
 __global__ void __launch_bounds__(kBlock)
-k_block_sum(const T* __restrict__ y, int nx, int ny, int fx, int fy, int cnx, int cny,
-            T* __restrict__ bc, const SolveState* st) {
+k_cls_restrict(ClassLevel<T> L, T w, const T* __restrict__ r, const T* __restrict__ t, int fx,
+               int fy, int cnx, int cny, T* __restrict__ bc, const SolveState* st) {
   if (st->done) return;
-  Tile2D t;
-  if (!cell_of_tile(cnx, cny, t)) return;
-  T acc = (T)0;
-  for (int a = 0; a < fx; ++a) {
-    const int i = t.i * fx + a;
-    if (i >= nx) break;
-    for (int c = 0; c < fy; ++c) {
-      const int j = t.j * fy + c;
-      if (j >= ny) break;
-      acc += y[i * ny + j];
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int crow = gid / LPR;
+  const int lane = gid % LPR;
+  const bool valid = crow < cnx * cny;
+  const int I = valid ? crow / cny : 0, J = valid ? crow - I * cny : 0;
+  T sum = (T)0;
+  for (int a2 = 0; a2 < fx; ++a2) {
+    const int i = I * fx + a2;
+    for (int c2 = 0; c2 < fy; ++c2) {
+      const int j = J * fy + c2;
+      const bool in = valid && i < L.nx && j < L.ny;
+      const int row = in ? i * L.ny + j : 0;
+      const int c = in ? (int)L.cls[row] : 0;
+      const bool act = in && L.cactive[c];
+      T acc = (T)0;
+      if (act) {
+        const int e1 = L.row_ptr[c + 1];
+        int e = L.row_ptr[c] + lane;
+        for (; e + 3 * LPR < e1; e += 4 * LPR) {
+          const int q0 = cls_nb(L, row, e), q1 = cls_nb(L, row, e + LPR);
+          const int q2 = cls_nb(L, row, e + 2 * LPR), q3 = cls_nb(L, row, e + 3 * LPR);
+          const T a0 = L.ent_val[e], a1 = L.ent_val[e + LPR];
+          const T a2v = L.ent_val[e + 2 * LPR], a3 = L.ent_val[e + 3 * LPR];
+          const T x0 = t[q0], x1 = t[q1], x2 = t[q2], x3 = t[q3];
+          acc += a0 * x0;
+          acc += a1 * x1;
+          acc += a2v * x2;
+          acc += a3 * x3;
+        }
+        for (; e < e1; e += LPR) acc += L.ent_val[e] * t[cls_nb(L, row, e)];
+      }
+#pragma unroll
+      for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
+      if (act) sum += r[row] - w * acc;
     }
   }
-  bc[t.i * cny + t.j] = acc;
-}
-
-// Coarsest level: x = C b with C the dense FP64 pseudo-inverse
-// (multigrid.py:514-543).  One workgroup.
-template <typename T>
-__global__ void __launch_bounds__(kBlock)
-k_coarse_solve(const double* __restrict__ C, int n, const T* __restrict__ b, T* __restrict__ x,
-               const SolveState* st) {
-  if (st->done) return;
-  extern __shared__ double sb[];
-  for (int k = threadIdx.x; k < n; k += blockDim.x) sb[k] = (double)b[k];
-  __syncthreads();
-  for (int row = threadIdx.x; row < n; row += blockDim.x) {
-    double acc = 0.0;
-    const double* c = C + (size_t)row * n;
-    for (int k = 0; k < n; ++k) acc += c[k] * sb[k];
-    x[row] = (T)acc;
-  }
+  if (valid && lane == 0) bc[crow] = sum;
 }
 
 // ----------------------------------------------------------------------------
@@ -893,6 +737,7 @@ void PoissonEngine::free_levels() {
       hipFree(L.ent_dj);
       hipFree(L.cactive);
       hipFree(L.cinv);
+      hipFree(L.ent_off);
     }
     v->clear();
   }
@@ -910,7 +755,8 @@ int PoissonEngine::init(const FineOp& op, int n_fluid, int singular, hipStream_t
     PCFD_HIP_CHECK(hipMemsetAsync(*v, 0, bytes, s_));
   }
   const size_t nt = (size_t)num_tiles(op.nx + 3, op.ny + 3) + 64;
-  PCFD_HIP_CHECK(hipMalloc(&partial_, nt * 8 * sizeof(double)));
+  partial_cap_ = nt * 8;
+  PCFD_HIP_CHECK(hipMalloc(&partial_, partial_cap_ * sizeof(double)));
   PCFD_HIP_CHECK(hipMalloc(&counter_, 64 * sizeof(unsigned int)));
   PCFD_HIP_CHECK(hipMemsetAsync(counter_, 0, 64 * sizeof(unsigned int), s_));
   PCFD_HIP_CHECK(hipMalloc(&st_, sizeof(SolveState)));
@@ -923,9 +769,14 @@ int PoissonEngine::init(const FineOp& op, int n_fluid, int singular, hipStream_t
 // Uploads one level in precision T: vectors, and for coarse levels the row
 // dictionary built from the dense class table (zeros dropped) plus the
 // aggregation map towards the next level.
+// LDS budget of one fused level workgroup (the dynamic allocation a launch
+// may take without raising the kernel's attribute).
+constexpr unsigned kLdsCap = 64u * 1024u;
+constexpr int kFuseMaxRadius = 3;
+
 template <typename T>
-static int upload_level(const MGLevelHost& h, const MGLevelHost* next, bool fine, LevelBuf& L,
-                        hipStream_t s) {
+static int upload_level(const MGLevelHost& h, const MGLevelHost* next, bool fine, int nu,
+                        bool fused_ok, LevelBuf& L, hipStream_t s) {
   const size_t n = (size_t)h.nx * h.ny;
   for (void** v : {&L.b, &L.x, &L.y, &L.r, &L.t}) {
     PCFD_HIP_CHECK(hipMalloc(v, n * sizeof(T)));
@@ -951,6 +802,22 @@ static int upload_level(const MGLevelHost& h, const MGLevelHost* next, bool fine
     }
     std::vector<T> cd(nc);
     for (size_t k = 0; k < nc; ++k) cd[k] = (T)h.cdinv[k];
+    L.rxs = 0;
+    L.rys = 0;
+    for (size_t e = 0; e < edi.size(); ++e) {
+      L.rxs = std::max(L.rxs, std::abs((int)edi[e]));
+      L.rys = std::max(L.rys, std::abs((int)edj[e]));
+    }
+    L.rxs = std::max(L.rxs, 1);
+    L.rys = std::max(L.rys, 1);
+    // Stored levels with a coarser one below and a narrow stencil may run
+    // fused (the halo recomputation grows with the radius squared times the
+    // row length; past radius 3 it costs more than the launches it saves).
+    // The tile plan keeps the entry offsets to place them in its LDS region.
+    if (next && fused_ok && std::max(L.rxs, L.rys) <= kFuseMaxRadius) {
+      L.hdi = edi;
+      L.hdj = edj;
+    }
     const size_t nnz = std::max<size_t>(val.size(), 1);
     PCFD_HIP_CHECK(hipMalloc(&L.cls, n * sizeof(uint16_t)));
     PCFD_HIP_CHECK(hipMalloc(&L.row_ptr, (nc + 1) * sizeof(int)));
@@ -983,6 +850,7 @@ static int upload_level(const MGLevelHost& h, const MGLevelHost* next, bool fine
     }
   }
   if (!h.coarse_inv.empty()) {
+    L.coarse_n = (int)std::lround(std::sqrt((double)h.coarse_inv.size()));
     PCFD_HIP_CHECK(hipMalloc(&L.cinv, h.coarse_inv.size() * sizeof(double)));
     PCFD_HIP_CHECK(hipMemcpy(L.cinv, h.coarse_inv.data(), h.coarse_inv.size() * sizeof(double),
                              hipMemcpyHostToDevice));
@@ -1020,14 +888,204 @@ int PoissonEngine::configure(const PoissonConfig& cfg, const std::vector<MGLevel
       L.fx = h.fx;
       L.fy = h.fy;
       L.weight = h.weight;
-      int rc = f32 ? upload_level<float>(h, next, l == 0, L, s_)
-                   : upload_level<double>(h, next, l == 0, L, s_);
+      int rc = f32 ? upload_level<float>(h, next, l == 0, cfg.sweeps, cfg.fused != 0, L, s_)
+                   : upload_level<double>(h, next, l == 0, cfg.sweeps, cfg.fused != 0, L, s_);
       if (rc) return rc;
       L.lanes = L.max_row <= 16 ? 1 : L.max_row <= 64 ? 4 : L.max_row <= 256 ? 16 : 64;
       dst.push_back(L);
     }
+    const int rc = f32 ? plan_level_tiles<float>(dst) : plan_level_tiles<double>(dst);
+    if (rc) return rc;
+  }
+  const LevelBuf& last = lv64_.back();
+  if (last.cinv == nullptr || last.coarse_n != last.n) {
+    set_last_error("the coarsest level needs an n x n dense inverse (n = %d)", last.n);
+    return kErrBadArg;
+  }
+  if ((size_t)last.n * sizeof(double) > kLdsCap) {
+    set_last_error("coarsest level of %d unknowns exceeds the dense solve's %u-byte LDS stage",
+                   last.n, kLdsCap);
+    return kErrBadArg;
   }
   PCFD_HIP_CHECK(hipStreamSynchronize(s_));
+  return kOk;
+}
+
+int PoissonEngine::ensure_partials(size_t n) {
+  if (n <= partial_cap_) return kOk;
+  PCFD_HIP_CHECK(hipStreamSynchronize(s_));
+  PCFD_HIP_CHECK(hipFree(partial_));
+  partial_ = nullptr;
+  partial_cap_ = 0;
+  PCFD_HIP_CHECK(hipMalloc(&partial_, n * sizeof(double)));
+  partial_cap_ = n;
+  return kOk;
+}
+
+// Measured tile choices, kept for the life of the process: a level of the same
+// shape, stencil radius, sweeps and precision gets the same plan again.
+namespace {
+using TuneKey = std::array<int, 13>;
+struct TunePick {
+  int fused, tx, ty;
+};
+std::map<TuneKey, TunePick> g_tune_cache;
+std::mutex g_tune_mutex;
+
+bool autotune_enabled() {
+  const char* e = std::getenv("PCFD_AUTOTUNE");
+  return !(e && e[0] == '0');
+}
+}  // namespace
+
+// Tiles of the level-l legs.  Every tiling gives the same arithmetic (each
+// cell's passes read the same inputs); only speed differs.  Candidates whose
+// LDS region fits are timed on this device (down + up leg, HIP events), and
+// for a stored level the pass-by-pass path competes too.  PCFD_AUTOTUNE=0
+// uses the static cost estimate instead.
+template <typename T>
+int PoissonEngine::tune_level(std::vector<LevelBuf>& lv, int l, SolveState* tst) {
+  LevelBuf& L = lv[l];
+  LevelBuf& C = lv[l + 1];
+  const int nu = cfg_.sweeps;
+  const bool fine = l == 0;
+  L.fused = 0;
+  if (!fine && L.hdi.empty()) return kOk;
+  const unsigned per_cell = fine ? 3u * sizeof(T) + 1u : 3u * sizeof(T) + 2u;
+  const unsigned fixed = fine ? 64u * sizeof(T) : 0u;
+  const int ftx = fine ? cfg_.tile_tx : 0, fty = fine ? cfg_.tile_ty : 0;
+  std::vector<FusedTile> cands = tile_candidates(L.nx, L.ny, L.rxs, L.rys, nu + 1, L.fx, L.fy,
+                                                 per_cell, fixed, kLdsCap, ftx, fty);
+  if (fine) {  // the standalone solver's nonzero start needs one more ring
+    std::vector<FusedTile> keep;
+    for (const FusedTile& t : cands) {
+      FusedTile t1;
+      if (choose_tile(L.nx, L.ny, 1, 1, nu + 2, L.fx, L.fy, per_cell, fixed, kLdsCap, t.tx, t.ty,
+                      &t1))
+        keep.push_back(t);
+    }
+    cands.swap(keep);
+    if (cands.empty()) {
+      set_last_error("no fused tiling of the %dx%d level fits (tile %dx%d, %d sweeps)", L.nx,
+                     L.ny, cfg_.tile_tx, cfg_.tile_ty, nu);
+      return kErrBadArg;
+    }
+    size_t most = 0;
+    for (const FusedTile& t : cands) most = std::max(most, (size_t)t.ntiles() * 3);
+    if (int rc = ensure_partials(most)) return rc;
+  }
+  if (cands.empty()) return kOk;
+
+  auto upload_offsets = [&](const FusedTile& t) -> int {
+    std::vector<int> off(std::max<size_t>(L.hdi.size(), 1), 0);
+    for (size_t e = 0; e < L.hdi.size(); ++e) off[e] = (int)L.hdi[e] * t.ry + (int)L.hdj[e];
+    if (!L.ent_off) PCFD_HIP_CHECK(hipMalloc(&L.ent_off, off.size() * sizeof(int)));
+    PCFD_HIP_CHECK(hipMemcpyAsync(L.ent_off, off.data(), off.size() * sizeof(int),
+                                  hipMemcpyHostToDevice, s_));
+    PCFD_HIP_CHECK(hipStreamSynchronize(s_));
+    return kOk;
+  };
+  auto apply = [&](int fused, const FusedTile& t) -> int {
+    L.fused = fused;
+    if (!fused) return kOk;
+    L.down0 = t;
+    L.up = t;
+    if (fine) {
+      if (!choose_tile(L.nx, L.ny, 1, 1, nu + 2, L.fx, L.fy, per_cell, fixed, kLdsCap, t.tx, t.ty,
+                       &L.down1))
+        return kErrBadArg;
+      return kOk;
+    }
+    return upload_offsets(t);
+  };
+  auto legs = [&]() -> int {
+    if (fine) {
+      if (int rc = launch_fine_down<T>(s_, op_, L.down0, (T)L.weight, nu, 0, r_, nullptr, nullptr,
+                                       0, (T*)L.b, (T*)L.t, (T*)C.b, L.fx, L.fy, C.nx, C.ny, tst))
+        return rc;
+      return launch_fine_up<T>(s_, op_, L.up, (T)L.weight, nu, (const T*)L.t, (const T*)L.b,
+                               (const T*)C.y, L.fx, L.fy, C.ny, (T*)L.y, (int)SR_MGCG, partial_,
+                               tst);
+    }
+    if (int rc = level_down<T>(L, C, tst)) return rc;
+    return level_up<T>(L, C, tst);
+  };
+
+  const TuneKey key = {fine ? 1 : 0, L.nx, L.ny, L.rxs, L.rys, L.fx, L.fy, nu, (int)sizeof(T),
+                       op_.px, op_.py, ftx, fty};
+  {
+    std::lock_guard<std::mutex> g(g_tune_mutex);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      const TunePick pk = it->second;
+      for (const FusedTile& t : cands)
+        if (t.tx == pk.tx && t.ty == pk.ty) return apply(pk.fused, t);
+      if (!pk.fused && !fine) return apply(0, cands[0]);
+    }
+  }
+  int best_fused = 1;
+  FusedTile best = *std::min_element(cands.begin(), cands.end(),
+                                     [](const FusedTile& a, const FusedTile& b) {
+                                       return tile_cost(a) < tile_cost(b);
+                                     });
+  if (autotune_enabled() && (cands.size() > 1 || !fine)) {
+    hipEvent_t e0, e1;
+    PCFD_HIP_CHECK(hipEventCreate(&e0));
+    PCFD_HIP_CHECK(hipEventCreate(&e1));
+    auto time_legs = [&](float* ms) -> int {
+      if (int rc = legs()) return rc;  // warm-up
+      PCFD_HIP_CHECK(hipEventRecord(e0, s_));
+      for (int k = 0; k < 3; ++k)
+        if (int rc = legs()) return rc;
+      PCFD_HIP_CHECK(hipEventRecord(e1, s_));
+      PCFD_HIP_CHECK(hipEventSynchronize(e1));
+      PCFD_HIP_CHECK(hipEventElapsedTime(ms, e0, e1));
+      return kOk;
+    };
+    float best_ms = 0.f;
+    bool have = false;
+    if (!fine) {  // the pass-by-pass path as the first contender
+      if (int rc = apply(0, cands[0])) return rc;
+      if (int rc = time_legs(&best_ms)) return rc;
+      best_fused = 0;
+      have = true;
+    }
+    for (const FusedTile& t : cands) {
+      if (int rc = apply(1, t)) return rc;
+      float ms = 0.f;
+      if (int rc = time_legs(&ms)) return rc;
+      if (!have || ms < best_ms) {
+        have = true;
+        best_ms = ms;
+        best = t;
+        best_fused = 1;
+      }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+  }
+  {
+    std::lock_guard<std::mutex> g(g_tune_mutex);
+    g_tune_cache[key] = TunePick{best_fused, best.tx, best.ty};
+  }
+  return apply(best_fused, best);
+}
+
+template <typename T>
+int PoissonEngine::plan_level_tiles(std::vector<LevelBuf>& lv) {
+  if (lv.size() < 2) return kOk;
+  SolveState* tst = nullptr;
+  PCFD_HIP_CHECK(hipMalloc(&tst, sizeof(SolveState)));
+  PCFD_HIP_CHECK(hipMemsetAsync(tst, 0, sizeof(SolveState), s_));
+  int rc = kOk;
+  for (int l = 0; l + 1 < (int)lv.size() && rc == kOk; ++l) rc = tune_level<T>(lv, l, tst);
+  const hipError_t e = hipStreamSynchronize(s_);
+  hipFree(tst);
+  if (rc) return rc;
+  if (e != hipSuccess) {
+    set_last_error("tile planning failed: %s", hipGetErrorString(e));
+    return kErrHip;
+  }
   return kOk;
 }
 
@@ -1281,107 +1339,169 @@ static void cls_rows(hipStream_t s, int lanes, const ClassLevel<T>& L, T w, cons
   }
 }
 
-// One V-cycle (multigrid.py:430-451) over the levels in precision T.  Level
-// 0's right-hand side is lv[0].b; the result is left in lv[0].x (unprojected;
-// the last sweep's sums give its active-set mean).
 template <typename T>
-int PoissonEngine::enqueue_vcycle_t(bool nonzero_start) {
-  auto& lv = (sizeof(T) == 4) ? lv32_ : lv64_;
-  const int nl = (int)lv.size();
+static ClassRows<T> class_rows(const LevelBuf& L, int px, int py) {
+  ClassRows<T> D;
+  D.nx = L.nx;
+  D.ny = L.ny;
+  D.px = px;
+  D.py = py;
+  D.cls = L.cls;
+  D.row_ptr = L.row_ptr;
+  D.ent_off = L.ent_off;
+  D.ent_val = (const T*)L.ent_val;
+  D.cdinv = (const T*)L.cdinv;
+  D.cactive = L.cactive;
+  return D;
+}
+
+template <typename T>
+static void cls_restrict(hipStream_t s, int lanes, const ClassLevel<T>& L, T w, const T* r,
+                         const T* t, int fx, int fy, int cnx, int cny, T* bc,
+                         const SolveState* st) {
+  const long threads = (long)cnx * cny * lanes;
+  const dim3 grid((unsigned)((threads + kBlock - 1) / kBlock));
+  switch (lanes) {
+    case 1:
+      hipLaunchKernelGGL((k_cls_restrict<T, 1>), grid, dim3(kBlock), 0, s, L, w, r, t, fx, fy,
+                         cnx, cny, bc, st);
+      break;
+    case 4:
+      hipLaunchKernelGGL((k_cls_restrict<T, 4>), grid, dim3(kBlock), 0, s, L, w, r, t, fx, fy,
+                         cnx, cny, bc, st);
+      break;
+    case 16:
+      hipLaunchKernelGGL((k_cls_restrict<T, 16>), grid, dim3(kBlock), 0, s, L, w, r, t, fx, fy,
+                         cnx, cny, bc, st);
+      break;
+    default:
+      hipLaunchKernelGGL((k_cls_restrict<T, 64>), grid, dim3(kBlock), 0, s, L, w, r, t, fx, fy,
+                         cnx, cny, bc, st);
+      break;
+  }
+}
+
+// Down leg of a stored level L (right-hand side L.b) towards C: pre-smoothed
+// iterate into L.x, restricted residual into C.b.  Fused when the level's
+// tile plan says so, else pass by pass.
+template <typename T>
+int PoissonEngine::level_down(LevelBuf& L, LevelBuf& C, const SolveState* cst) {
   const int nu = cfg_.sweeps;
-  const int red = (cfg_.kind == SK_MULTIGRID) ? SR_MG : SR_MGCG;
-  const int nt0 = ntiles(op_.nx, op_.ny);
-  const SolveState* cst = st_;
-  FineLevelArgs<T> F{op_, (T)lv[0].weight};
-  if (nl == 1) {
-    LevelBuf& L = lv[0];
-    hipLaunchKernelGGL(k_coarse_solve<T>, dim3(1), dim3(kBlock), L.n * sizeof(double), s_,
-                       (const double*)L.cinv, L.n, (const T*)L.b, (T*)L.x, cst);
-    hipLaunchKernelGGL(k_cycle_sums<T>, tiles_for(op_.nx, op_.ny), dim3(kBlock), 0, s_, op_,
-                       (const T*)L.b, (const T*)L.x, partial_, cst);
-    PCFD_LAUNCH_CHECK();
-    return fold(s_, partial_, nt0, EpiCycle{st_, red, n_proj_});
+  const T w = (T)L.weight;
+  if (L.fused) {
+    const ClassRows<T> D = class_rows<T>(L, op_.px, op_.py);
+    return launch_class_down<T>(s_, D, L.down0, L.rxs, L.rys, w, nu, (const T*)L.b, (T*)L.x,
+                                (T*)C.b, L.fx, L.fy, C.nx, C.ny, cst);
   }
-  // Downward leg.
-  for (int l = 0; l < nl - 1; ++l) {
-    LevelBuf& L = lv[l];
-    LevelBuf& C = lv[l + 1];
-    const bool nz = (l == 0) && nonzero_start;
-    const dim3 g = tiles_for(L.nx, L.ny);
-    if (l == 0) {
-      for (int s = 0; s < nu; ++s) {
-        const T* xin = (s == 0 && !nz) ? nullptr : (const T*)L.x;
-        hipLaunchKernelGGL(k_smooth_fine<T>, g, dim3(kBlock), 0, s_, F, (const T*)L.b, xin,
-                           (T*)L.y, (s == 0 && nz) ? 1 : 0, (int)SR_NONE, partial_, cst);
-        std::swap(L.x, L.y);
-      }
-      hipLaunchKernelGGL(k_residual_fine<T>, g, dim3(kBlock), 0, s_, op_, (const T*)L.b,
-                         (const T*)L.x, (T*)L.r, cst);
-      hipLaunchKernelGGL(k_restrict_fine<T>, tiles_for(C.nx, C.ny), dim3(kBlock), 0, s_, F, L.fx,
-                         L.fy, C.nx, C.ny, (const T*)L.r, (T*)C.b, cst);
-    } else {
-      const ClassLevel<T> D = class_level<T>(L, op_.px, op_.py);
-      const T w = (T)L.weight;
-      cls_rows<T, RO_SMOOTH0_RES>(s_, L.lanes, D, w, (const T*)L.b, nullptr, nullptr, nullptr,
-                                  (T*)L.x, (T*)L.r, (T*)L.t, cst);
-      if (nu > 1) {
-        for (int s = 1; s < nu; ++s) {
-          cls_rows<T, RO_SMOOTH>(s_, L.lanes, D, w, (const T*)L.b, (const T*)L.x, nullptr,
-                                 nullptr, (T*)L.y, nullptr, nullptr, cst);
-          std::swap(L.x, L.y);
-        }
-        cls_rows<T, RO_RES>(s_, L.lanes, D, w, (const T*)L.b, (const T*)L.x, nullptr, nullptr,
-                            nullptr, (T*)L.r, (T*)L.t, cst);
-      }
-      // y = r - w A t into L.y, then the block sums into the next level's b
-      cls_rows<T, RO_RESTRICT_Y>(s_, L.lanes, D, w, (const T*)L.r, (const T*)L.t, nullptr,
-                                 nullptr, (T*)L.y, nullptr, nullptr, cst);
-      hipLaunchKernelGGL(k_block_sum<T>, tiles_for(C.nx, C.ny), dim3(kBlock), 0, s_,
-                         (const T*)L.y, L.nx, L.ny, L.fx, L.fy, C.nx, C.ny, (T*)C.b, cst);
+  const ClassLevel<T> D = class_level<T>(L, op_.px, op_.py);
+  cls_rows<T, RO_SMOOTH0_RES>(s_, L.lanes, D, w, (const T*)L.b, nullptr, nullptr, nullptr,
+                              (T*)L.x, (T*)L.r, (T*)L.t, cst);
+  if (nu > 1) {
+    for (int sw = 1; sw < nu; ++sw) {
+      cls_rows<T, RO_SMOOTH>(s_, L.lanes, D, w, (const T*)L.b, (const T*)L.x, nullptr, nullptr,
+                             (T*)L.y, nullptr, nullptr, cst);
+      std::swap(L.x, L.y);
     }
-    PCFD_LAUNCH_CHECK();
+    cls_rows<T, RO_RES>(s_, L.lanes, D, w, (const T*)L.b, (const T*)L.x, nullptr, nullptr,
+                        nullptr, (T*)L.r, (T*)L.t, cst);
   }
-  {
-    LevelBuf& L = lv[nl - 1];
-    hipLaunchKernelGGL(k_coarse_solve<T>, dim3(1), dim3(kBlock), L.n * sizeof(double), s_,
-                       (const double*)L.cinv, L.n, (const T*)L.b, (T*)L.x, cst);
-    PCFD_LAUNCH_CHECK();
-  }
-  // Upward leg.
-  for (int l = nl - 2; l >= 0; --l) {
-    LevelBuf& L = lv[l];
-    LevelBuf& C = lv[l + 1];
-    const dim3 g = tiles_for(L.nx, L.ny);
-    if (l == 0) {
-      hipLaunchKernelGGL(k_prolong_fine<T>, g, dim3(kBlock), 0, s_, F, L.fx, L.fy, C.ny,
-                         (const T*)C.x, (T*)L.x, cst);
-      for (int s = 0; s < nu; ++s) {
-        const int rmode = (s == nu - 1) ? red : (int)SR_NONE;
-        hipLaunchKernelGGL(k_smooth_fine<T>, g, dim3(kBlock), 0, s_, F, (const T*)L.b,
-                           (const T*)L.x, (T*)L.y, 0, rmode, partial_, cst);
-        std::swap(L.x, L.y);
-      }
-      PCFD_LAUNCH_CHECK();
-      if (int rc = fold(s_, partial_, nt0, EpiCycle{st_, red, n_proj_})) return rc;
-    } else {
-      const ClassLevel<T> D = class_level<T>(L, op_.px, op_.py);
-      const T w = (T)L.weight;
-      cls_rows<T, RO_PROLONG>(s_, L.lanes, D, w, nullptr, nullptr, (const T*)C.x, L.agg,
-                              (T*)L.x, nullptr, nullptr, cst);
-      for (int s = 0; s < nu; ++s) {
-        cls_rows<T, RO_SMOOTH>(s_, L.lanes, D, w, (const T*)L.b, (const T*)L.x, nullptr, nullptr,
-                               (T*)L.y, nullptr, nullptr, cst);
-        std::swap(L.x, L.y);
-      }
-      PCFD_LAUNCH_CHECK();
-    }
-  }
+  // b_c = P^T r in one pass over the coarse cells
+  cls_restrict<T>(s_, L.lanes, D, w, (const T*)L.r, (const T*)L.t, L.fx, L.fy, C.nx, C.ny,
+                  (T*)C.b, cst);
+  PCFD_LAUNCH_CHECK();
   return kOk;
 }
 
-int PoissonEngine::enqueue_vcycle(bool nonzero_start) {
-  if (cfg_.precision == 1) return enqueue_vcycle_t<float>(nonzero_start);
-  return enqueue_vcycle_t<double>(nonzero_start);
+// Up leg of a stored level: L.y = S^nu (L.x + P C.y).
+template <typename T>
+int PoissonEngine::level_up(LevelBuf& L, LevelBuf& C, const SolveState* cst) {
+  const int nu = cfg_.sweeps;
+  const T w = (T)L.weight;
+  if (L.fused) {
+    const ClassRows<T> D = class_rows<T>(L, op_.px, op_.py);
+    return launch_class_up<T>(s_, D, L.up, L.rxs, L.rys, w, nu, (const T*)L.x, (const T*)L.b,
+                              (const T*)C.y, L.fx, L.fy, C.ny, (T*)L.y, cst);
+  }
+  const ClassLevel<T> D = class_level<T>(L, op_.px, op_.py);
+  cls_rows<T, RO_PROLONG>(s_, L.lanes, D, w, nullptr, nullptr, (const T*)C.y, L.agg, (T*)L.x,
+                          nullptr, nullptr, cst);
+  for (int sw = 0; sw < nu; ++sw) {
+    cls_rows<T, RO_SMOOTH>(s_, L.lanes, D, w, (const T*)L.b, (const T*)L.x, nullptr, nullptr,
+                           (T*)L.y, nullptr, nullptr, cst);
+    std::swap(L.x, L.y);
+  }
+  std::swap(L.x, L.y);  // the corrected iterate is named y
+  PCFD_LAUNCH_CHECK();
+  return kOk;
+}
+
+// One V-cycle (multigrid.py:430-451) over the levels in precision T.
+//
+// Buffer contract per level: the down leg reads the level's right-hand side
+// b, leaves the pre-smoothed iterate (level 0: t, coarser levels: x) and
+// writes the next level's b; the up leg reads those and the next level's
+// corrected iterate y and writes its own corrected iterate to y.  The
+// coarsest level is the dense pseudo-inverse b -> y.  Level 0's right-hand
+// side comes from r_ (Krylov mode; b0 = r - mean_r is kept in lv[0].b) or is
+// lv[0].b itself with the iterate lv[0].x (standalone mode).  The cycle
+// output lands in lv[0].y (Krylov) or back in lv[0].x (standalone; the up leg
+// reads the pre-smoothed copy in lv[0].t, so nothing reads x while it is
+// written) with the partial sums `reduce` asks for.
+template <typename T>
+int PoissonEngine::enqueue_vcycle_t(int mode, int reduce) {
+  auto& lv = (sizeof(T) == 4) ? lv32_ : lv64_;
+  const int nl = (int)lv.size();
+  const int nu = cfg_.sweeps;
+  const SolveState* cst = st_;
+  const bool krylov = mode == VC_KRYLOV;
+  LevelBuf& L0 = lv[0];
+  (void)nu;
+  T* out = krylov ? (T*)L0.y : (T*)L0.x;
+  const int nt0 = ntiles(op_.nx, op_.ny);
+  if (nl == 1) {
+    if (krylov) {
+      hipLaunchKernelGGL(k_project_in<T>, tiles_for(op_.nx, op_.ny), dim3(kBlock), 0, s_, op_,
+                         (const double*)r_, (T*)L0.b, cst);
+      PCFD_LAUNCH_CHECK();
+    }
+    if (int rc = launch_dense_apply<T>(s_, L0.cinv, L0.n, (const T*)L0.b, out, cst)) return rc;
+    hipLaunchKernelGGL(k_cycle_sums<T>, tiles_for(op_.nx, op_.ny), dim3(kBlock), 0, s_, op_,
+                       (const T*)L0.b, (const T*)out, partial_, cst);
+    PCFD_LAUNCH_CHECK();
+    return fold(s_, partial_, nt0, EpiCycle{st_, reduce, n_proj_});
+  }
+  // Downward leg.
+  {
+    LevelBuf& C = lv[1];
+    const int rc = launch_fine_down<T>(
+        s_, op_, krylov ? L0.down0 : L0.down1, (T)L0.weight, nu, krylov ? 0 : 1,
+        krylov ? (const double*)r_ : nullptr, krylov ? nullptr : (const T*)L0.b,
+        krylov ? nullptr : (const T*)L0.x, krylov ? 0 : 1, krylov ? (T*)L0.b : nullptr,
+        (T*)L0.t, (T*)C.b, L0.fx, L0.fy, C.nx, C.ny, cst);
+    if (rc) return rc;
+  }
+  for (int l = 1; l < nl - 1; ++l)
+    if (int rc = level_down<T>(lv[l], lv[l + 1], cst)) return rc;
+  {
+    LevelBuf& L = lv[nl - 1];
+    if (int rc = launch_dense_apply<T>(s_, L.cinv, L.n, (const T*)L.b, (T*)L.y, cst)) return rc;
+  }
+  // Upward leg.
+  for (int l = nl - 2; l >= 1; --l)
+    if (int rc = level_up<T>(lv[l], lv[l + 1], cst)) return rc;
+  {
+    LevelBuf& C = lv[1];
+    if (int rc = launch_fine_up<T>(s_, op_, L0.up, (T)L0.weight, nu, (const T*)L0.t,
+                                   (const T*)L0.b, (const T*)C.y, L0.fx, L0.fy, C.ny, out,
+                                   reduce, partial_, cst))
+      return rc;
+  }
+  return fold(s_, partial_, L0.up.ntiles(), EpiCycle{st_, reduce, n_proj_});
+}
+
+int PoissonEngine::enqueue_vcycle(int mode, int reduce) {
+  if (cfg_.precision == 1) return enqueue_vcycle_t<float>(mode, reduce);
+  return enqueue_vcycle_t<double>(mode, reduce);
 }
 
 int PoissonEngine::run_mgcg() {
@@ -1392,21 +1512,13 @@ int PoissonEngine::run_mgcg() {
   double* p_new = pb_;
   const bool f32 = cfg_.precision == 1;
   PCFD_ITERATE(1, cfg_.maxiter, {
+    if (int rc = enqueue_vcycle(VC_KRYLOV, SR_MGCG)) return rc;
     if (f32) {
-      hipLaunchKernelGGL(k_project_in<float>, g, dim3(kBlock), 0, s_, op_, r_, (float*)lv32_[0].b,
-                         (const SolveState*)st_);
-    } else {
-      hipLaunchKernelGGL(k_project_in<double>, g, dim3(kBlock), 0, s_, op_, r_,
-                         (double*)lv64_[0].b, (const SolveState*)st_);
-    }
-    if (int rc = enqueue_vcycle(false)) return rc;
-    if (f32) {
-      hipLaunchKernelGGL(k_mgcg_dir<float>, g, dim3(kBlock), 0, s_, op_,
-                         (const float*)lv32_[0].x, p_old, p_new, q_, partial_,
-                         (const SolveState*)st_);
+      hipLaunchKernelGGL(k_mgcg_dir<float>, g, dim3(kBlock), 0, s_, op_, (const float*)lv32_[0].y,
+                         p_old, p_new, q_, partial_, (const SolveState*)st_);
     } else {
       hipLaunchKernelGGL(k_mgcg_dir<double>, g, dim3(kBlock), 0, s_, op_,
-                         (const double*)lv64_[0].x, p_old, p_new, q_, partial_,
+                         (const double*)lv64_[0].y, p_old, p_new, q_, partial_,
                          (const SolveState*)st_);
     }
     if (int rc = fold(s_, partial_, nt, EpiAlpha{st_})) return rc;
@@ -1429,13 +1541,14 @@ int PoissonEngine::run_multigrid() {
   PCFD_HIP_CHECK(hipMemcpyAsync(L0.b, b_, bytes, hipMemcpyDeviceToDevice, s_));
   PCFD_HIP_CHECK(hipMemcpyAsync(L0.x, x_, bytes, hipMemcpyDeviceToDevice, s_));
   PCFD_ITERATE(1, cfg_.maxiter, {
-    if (int rc = enqueue_vcycle_t<double>(true)) return rc;
-    hipLaunchKernelGGL(k_check, g, dim3(kBlock), 0, s_, op_, (const double*)L0.x, b_,
-                       (int)CM_MG, partial_, (const SolveState*)st_);
+    if (int rc = enqueue_vcycle_t<double>(VC_STANDALONE, SR_MG)) return rc;
+    hipLaunchKernelGGL(k_check, g, dim3(kBlock), 0, s_, op_, (const double*)L0.x, b_, (int)CM_MG,
+                       partial_, (const SolveState*)st_);
     if (int rc = fold(s_, partial_, nt,
                       EpiCheck{st_, (int)CM_MG, cfg_.tol, cfg_.maxiter, singular_, n_fluid_}))
       return rc;
   });
+  // The pending mean shift is applied by the final residual check and the scatter.
   PCFD_HIP_CHECK(hipMemcpyAsync(x_, L0.x, bytes, hipMemcpyDeviceToDevice, s_));
   return kOk;
 }
@@ -1452,21 +1565,13 @@ int PoissonEngine::apply_preconditioner(const double* r, double* z) {
   hipLaunchKernelGGL(k_active_sum, g, dim3(kBlock), 0, s_, op_, (const double*)r_, partial_);
   PCFD_LAUNCH_CHECK();
   if (int rc = fold(s_, partial_, ntiles(op_.nx, op_.ny), EpiOpen{st_, n_proj_})) return rc;
-  const bool f32 = cfg_.precision == 1;
-  if (f32) {
-    hipLaunchKernelGGL(k_project_in<float>, g, dim3(kBlock), 0, s_, op_, r_, (float*)lv32_[0].b,
-                       (const SolveState*)st_);
-  } else {
-    hipLaunchKernelGGL(k_project_in<double>, g, dim3(kBlock), 0, s_, op_, r_, (double*)lv64_[0].b,
-                       (const SolveState*)st_);
-  }
-  if (int rc = enqueue_vcycle(false)) return rc;
-  if (f32) {
-    hipLaunchKernelGGL(k_cast_out<float>, g, dim3(kBlock), 0, s_, op_, (const float*)lv32_[0].x,
+  if (int rc = enqueue_vcycle(VC_KRYLOV, SR_MGCG)) return rc;
+  if (cfg_.precision == 1) {
+    hipLaunchKernelGGL(k_cast_out<float>, g, dim3(kBlock), 0, s_, op_, (const float*)lv32_[0].y,
                        (const SolveState*)st_, z);
   } else {
     hipLaunchKernelGGL(k_cast_out<double>, g, dim3(kBlock), 0, s_, op_,
-                       (const double*)lv64_[0].x, (const SolveState*)st_, z);
+                       (const double*)lv64_[0].y, (const SolveState*)st_, z);
   }
   PCFD_LAUNCH_CHECK();
   PCFD_HIP_CHECK(hipStreamSynchronize(s_));
@@ -1476,13 +1581,17 @@ int PoissonEngine::apply_preconditioner(const double* r, double* z) {
 int PoissonEngine::time_kernel(int which, int reps, double* avg_ms, double* bytes) {
   const dim3 g = tiles_for(op_.nx, op_.ny);
   const double n = (double)n_;
+  const bool have_mg = lv64_.size() >= 2;
+  if ((which == 2 || which == 3) && !have_mg) {
+    set_last_error("kernel %d needs a multigrid hierarchy of at least two levels", which);
+    return kErrBadArg;
+  }
   SolveState* st = nullptr;
   PCFD_HIP_CHECK(hipMalloc(&st, sizeof(SolveState)));
   PCFD_HIP_CHECK(hipMemsetAsync(st, 0, sizeof(SolveState), s_));
   hipEvent_t e0, e1;
   PCFD_HIP_CHECK(hipEventCreate(&e0));
   PCFD_HIP_CHECK(hipEventCreate(&e1));
-  const bool have_mg = !lv64_.empty();
   double B = 0.0;
   auto launch = [&]() -> int {
     switch (which) {
@@ -1496,21 +1605,29 @@ int PoissonEngine::time_kernel(int which, int reps, double* avg_ms, double* byte
                            pa_, pb_, q_, partial_, (const SolveState*)st);
         B = n * (4 * 8 + 1);
         break;
-      case 2: {  // fine damped-Jacobi sweep: read b, x + flags, write x'
-        if (!have_mg) return kErrBadArg;
-        FineLevelArgs<double> F{op_, lv64_[0].weight};
-        hipLaunchKernelGGL(k_smooth_fine<double>, g, dim3(kBlock), 0, s_, F,
-                           (const double*)b_, (const double*)x_, z_, 0, (int)SR_NONE, partial_,
-                           (const SolveState*)st);
-        B = n * (3 * 8 + 1);
+      case 2: {  // level-0 down leg: read r + flags, write b0, x, and b_c (n/4)
+        LevelBuf& L = lv64_[0];
+        LevelBuf& C = lv64_[1];
+        if (int rc = launch_fine_down<double>(s_, op_, L.down0, L.weight, cfg_.sweeps, 0, r_,
+                                              nullptr, nullptr, 0, (double*)L.b, (double*)L.t,
+                                              (double*)C.b, L.fx, L.fy, C.nx, C.ny, st))
+          return rc;
+        B = n * (8 + 1 + 8 + 8) + (double)C.n * 8;
         break;
       }
-      case 3:  // fine residual: read b, x + flags, write r
-        hipLaunchKernelGGL(k_residual_fine<double>, g, dim3(kBlock), 0, s_, op_,
-                           (const double*)b_, (const double*)x_, z_, (const SolveState*)st);
-        B = n * (3 * 8 + 1);
+      case 3: {  // level-0 up leg: read x, b0, flags, x_c (n/4), write z
+        LevelBuf& L = lv64_[0];
+        LevelBuf& C = lv64_[1];
+        if (int rc = launch_fine_up<double>(s_, op_, L.up, L.weight, cfg_.sweeps,
+                                            (const double*)L.t, (const double*)L.b,
+                                            (const double*)C.y, L.fx, L.fy, C.ny, (double*)L.y,
+                                            (int)SR_MGCG, partial_, st))
+          return rc;
+        B = n * (8 + 8 + 1 + 8) + (double)C.n * 8;
         break;
+      }
       default:
+        set_last_error("unknown kernel %d", which);
         return kErrBadArg;
     }
     return kOk;

@@ -4,6 +4,7 @@
 
 #include <vector>
 #include "internal.hpp"
+#include "vcycle_tile.hpp"
 
 namespace pcfd {
 
@@ -70,6 +71,11 @@ struct SolveState {
   int pad[3];
 };
 
+// Partial sums taken over the output of a V-cycle: none, sum(z) for the
+// standalone solver's mean removal, or sum(z), sum(b*z), sum(b) for the
+// preconditioned-CG inner product.
+enum SmoothReduce : int { SR_NONE = 0, SR_MG = 1, SR_MGCG = 2 };
+
 enum SolverKind : int {
   SK_DIRECT = 0,
   SK_CG = 1,
@@ -100,6 +106,9 @@ struct PoissonConfig {
   double omega;     // SOR over-relaxation / Jacobi damping
   int sweeps;       // multigrid pre/post smoothing sweeps
   int precision;    // 0 = FP64 throughout, 1 = FP32 V-cycle inside FP64 CG
+  int fused = 1;    // stored levels through the fused kernels when they fit
+  int tile_tx = 0;  // level-0 fused tile (0 = chosen by the cost model)
+  int tile_ty = 0;
 };
 
 struct SolveInfo {
@@ -121,6 +130,15 @@ struct LevelBuf {
   uint8_t* cactive = nullptr;
   double weight = 1.0;
   double* cinv = nullptr;  // coarsest dense pseudo-inverse (FP64)
+  int coarse_n = 0;
+  // Fused level kernels (vcycle.hip): tilings and the stencil radius; level 0
+  // has one down tiling per start (zero / nonzero iterate).  `fused` is 0 for
+  // a stored level whose LDS region does not fit (it then runs pass by pass).
+  int fused = 0;
+  int rxs = 1, rys = 1;
+  FusedTile down0, down1, up;
+  int* ent_off = nullptr;
+  std::vector<int8_t> hdi, hdj;  // host copy of the entry offsets (tile planning)
 };
 
 class PoissonEngine {
@@ -146,13 +164,25 @@ class PoissonEngine {
   // buffers with HIP events; reports the mean duration and the kernel's
   // algorithmic bytes per launch (arrays read + written once, at stored size).
   int time_kernel(int which, int reps, double* avg_ms, double* bytes);
+  // Per level: nx, ny, fused, tile x, tile y (see pcfd_mg_plan).
+  int plan(int32_t* out, int cap) const {
+    const auto& lv = (cfg_.precision == 1) ? lv32_ : lv64_;
+    int k = 0;
+    for (const LevelBuf& L : lv) {
+      const int v[5] = {L.nx, L.ny, L.fused, L.fused ? L.down0.tx : 0, L.fused ? L.down0.ty : 0};
+      for (int q = 0; q < 5; ++q)
+        if (k < cap) out[k++] = v[q];
+    }
+    return (int)lv.size();
+  }
   double* guess() { return x_; }
   int last_iterations() const { return last_iters_; }
   hipStream_t stream() const { return s_; }
   int n() const { return n_; }
   int levels() const { return (int)lv64_.size(); }
 
-  double *partial_ = nullptr;
+  double *partial_ = nullptr;  // per-tile partial sums of the two-launch reductions
+  size_t partial_cap_ = 0;     // capacity in doubles
   unsigned int* counter_ = nullptr;
 
  private:
@@ -162,9 +192,21 @@ class PoissonEngine {
   int run_jacobi();
   int run_multigrid();
   int finalize(bool recompute_residual);
-  int enqueue_vcycle(bool nonzero_start);
+  // V-cycle entry modes: from the Krylov residual r_ (zero start, output
+  // projected by the caller's sums) or from lv[0].b / lv[0].x (standalone).
+  enum CycleMode : int { VC_KRYLOV = 0, VC_STANDALONE = 1 };
+  int enqueue_vcycle(int mode, int reduce);
   template <typename T>
-  int enqueue_vcycle_t(bool nonzero_start);
+  int enqueue_vcycle_t(int mode, int reduce);
+  template <typename T>
+  int plan_level_tiles(std::vector<LevelBuf>& lv);
+  template <typename T>
+  int level_down(LevelBuf& L, LevelBuf& C, const SolveState* st);
+  template <typename T>
+  int level_up(LevelBuf& L, LevelBuf& C, const SolveState* st);
+  template <typename T>
+  int tune_level(std::vector<LevelBuf>& lv, int l, SolveState* tst);
+  int ensure_partials(size_t n);
   int poll(int* iter, int* done);
   int start(int mode, double* r);
   void free_levels();

@@ -92,19 +92,62 @@ def test_operator_apply(gpu_available, name):
     assert np.abs(y.cpu().numpy() - ref).max() <= 1e-14 * np.abs(ref).max()
 
 
+def _mg_solver(name, fused=True, **kw):
+    s = _solver(name, pressure_solver="mgcg", **kw)
+    if not fused:
+        s.pressure_solver.fused_levels = False
+        s.pressure_solver.configure()
+    return s
+
+
 @pytest.mark.parametrize("name", ["cavity", "periodic", "cyl_outlet", "cyl_pout"])
 @pytest.mark.parametrize("sweeps", [1, 2])
-def test_vcycle_matches_reference(gpu_available, name, sweeps):
-    """z = M r: one V-cycle with projections, FP64, vs the reference's, 1e-10."""
+@pytest.mark.parametrize("fused", [True, False])
+def test_vcycle_matches_reference(gpu_available, name, sweeps, fused):
+    """z = M r: one V-cycle of the reference hierarchy (coarsest level at 96
+    unknowns) with projections, FP64, vs the reference's, 1e-10."""
     import torch
     g = golden("multigrid")
     tag = f"{name}_s{sweeps}"
-    s = _solver(name, pressure_solver="mgcg", mg_sweeps=sweeps)
+    s = _mg_solver(name, fused, mg_sweeps=sweeps, mg_coarse_max=96)
     r = torch.as_tensor(g[f"{tag}_r"]).cuda()
     z = torch.empty_like(r)
     s.ctx.mg_apply(r, z)
     ref = g[f"{tag}_z"]
     assert np.abs(z.cpu().numpy() - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+VCYCLE_GRIDS = [("cavity", {}), ("periodic", {}), ("cyl_outlet", {}), ("cyl_pout", {}),
+                ("cyl_outlet", dict(nx=256, ny=128)), ("cyl_pout", dict(nx=160, ny=72)),
+                ("periodic", dict(nx=128, ny=96))]
+
+
+@pytest.mark.parametrize("name,grid", VCYCLE_GRIDS,
+                         ids=[f"{n}-{g.get('nx', 'g')}" for n, g in VCYCLE_GRIDS])
+@pytest.mark.parametrize("coarse_max", [96, 2048])
+@pytest.mark.parametrize("sweeps", [1, 2])
+@pytest.mark.parametrize("fused", [True, False])
+def test_vcycle_matches_oracle(gpu_available, name, grid, coarse_max, sweeps, fused):
+    """One V-cycle with the coarsest solve at the first level of at most
+    coarse_max unknowns, vs the oracle's cycle on the same hierarchy:
+    FP64 to 1e-10, the FP32 cycle to 1e-4 (relative to max|z|)."""
+    import torch
+    case = oracle_case(name, **grid)
+    S = O.assemble(case)
+    H = O.Hierarchy(S.A, (case.nx, case.ny), S.fluid, S.singular, sweeps,
+                    coarse_max=coarse_max)
+    rng = np.random.default_rng(21)
+    r = rng.standard_normal(S.A.shape[0])
+    ref = H.apply(r)
+    for precision, tol in (("fp64", 1e-10), ("mixed", 1e-4)):
+        s = _mg_solver(name, fused, mg_sweeps=sweeps, mg_coarse_max=coarse_max,
+                       precision=precision, **grid)
+        z = torch.empty(r.size, dtype=torch.float64, device="cuda")
+        s.ctx.mg_apply(torch.as_tensor(r).cuda(), z)
+        err = np.abs(z.cpu().numpy() - ref).max() / np.abs(ref).max()
+        print(f"{name} {grid} coarse_max={coarse_max} sweeps={sweeps} fused={fused} "
+              f"{precision}: {err:.2e}")
+        assert err <= tol, precision
 
 
 @pytest.mark.parametrize("name", ["cavity", "periodic", "cyl_outlet", "cyl_pout"])
@@ -165,14 +208,18 @@ def test_trajectory_matches_reference(gpu_available, name, ps, steps):
     assert max(res.history["max_div"]) <= 1e-12
 
 
-@pytest.mark.parametrize("ps", ["sor", "multigrid", "mgcg", "jacobi"])
+@pytest.mark.parametrize("ps", ["sor", "multigrid", "mgcg", "jacobi", "multigrid-96",
+                                "mgcg-96", "mgcg-40"])
 def test_every_solver_gives_the_same_step(gpu_available, ps):
     """test_solver.py:106-119: every pressure solver at tol 1e-13 agrees with
-    the direct one to 1e-9 after a few steps."""
+    the direct one to 1e-9 after a few steps (suffix: mg_coarse_max)."""
+    ps, _, cm = ps.partition("-")
     if ps == "jacobi":
         kw = dict(poisson_tol=1e-11, poisson_maxiter=400000)
     else:
         kw = dict(poisson_tol=1e-13, poisson_maxiter=200000)
+    if cm:
+        kw["mg_coarse_max"] = int(cm)
     a = _sim("cavity", pressure_solver="direct")
     b = _sim("cavity", pressure_solver=ps, **kw)
     a.run(max_steps=3)
@@ -206,13 +253,14 @@ def test_run_twice_bitwise(gpu_available):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_oracle_parity_mgcg(gpu_available, precision):
+@pytest.mark.parametrize("coarse_max", [96, 2048])
+def test_oracle_parity_mgcg(gpu_available, precision, coarse_max):
     """mgcg at tol 1e-12 vs the oracle's direct solve, cylinder with OUTLET."""
     case = oracle_case("cyl_outlet", pressure_solver="direct")
     u0, v0 = initial("cyl_outlet", case)
     u, v, p, hist, t = O.run(case, 6, u0, v0)
     sim = _sim("cyl_outlet", u0, v0, pressure_solver="mgcg", poisson_tol=1e-12,
-               poisson_maxiter=300, precision=precision)
+               poisson_maxiter=300, precision=precision, mg_coarse_max=coarse_max)
     sim.run(max_steps=6)
     assert np.abs(sim.fields.u - u).max() <= 1e-9 * np.abs(u).max()
     assert np.abs(sim.fields.v - v).max() <= 1e-9 * np.abs(u).max()

@@ -95,6 +95,28 @@ def test_coarsest_pseudo_inverse_equals_pinned_solve(name):
                                    atol=1e-12 * np.abs(C).max() * np.abs(b).max())
 
 
+@pytest.mark.parametrize("name", ["cavity", "periodic", "cyl_outlet", "cyl_pout"])
+@pytest.mark.parametrize("coarse_max", [40, 150, 2048])
+def test_truncated_device_levels_match_oracle(name, coarse_max):
+    """device_levels(coarse_max) stops at the first level with at most
+    coarse_max unknowns; its dense operator equals the oracle's exact solve
+    on that level."""
+    S = product_system(name)
+    H = MultigridHierarchy(S.A, S.shape, S.fluid, S.singular, 1, 1,
+                           (S.periodic_x, S.periodic_y))
+    R = O.Hierarchy(O.assemble(oracle_case(name)).A, S.shape, S.fluid, S.singular, 1,
+                    coarse_max=coarse_max)
+    d = H.device_levels(coarse_max)
+    assert [L["nx"] * L["ny"] for L in d] == [lv["A"].shape[0] for lv in R.levels]
+    assert (d[-1]["fx"], d[-1]["fy"]) == (1, 1)
+    C = d[-1]["coarse_inv"]
+    rng = np.random.default_rng(12)
+    for _ in range(3):
+        b = rng.standard_normal(C.shape[0])
+        np.testing.assert_allclose(C @ b, R._coarsest(R.levels[-1], b), rtol=1e-10,
+                                   atol=1e-12 * np.abs(C).max() * np.abs(b).max())
+
+
 def test_inlet_lines_match_oracle():
     cfg, _ = product_config("channel")
     mesh = StructuredMesh.from_config(cfg)

@@ -198,6 +198,61 @@ def test_coarsest_is_pseudo_inverse():
     np.testing.assert_allclose(x, np.linalg.pinv(S.A.toarray()) @ b, atol=1e-12)
 
 
+@pytest.mark.parametrize("name", ["cavity", "periodic", "cyl_pout"])
+def test_truncated_hierarchy_whole_level_is_pseudo_inverse(name):
+    """coarse_max >= n: the cycle is the exact solve, A z = b on the active
+    set with z mean-free (singular) -- checked against a dense lstsq."""
+    case = oracle_case(name)
+    S = O.assemble(case)
+    n = S.A.shape[0]
+    H = O.Hierarchy(S.A, (case.nx, case.ny), S.fluid, S.singular, 1, coarse_max=n)
+    assert len(H.levels) == 1
+    b = np.random.default_rng(4).standard_normal(n)
+    z = H.apply(b)
+    act = S.fluid
+    bp = b.copy()
+    bp[~act] = 0.0
+    if S.singular:
+        bp[act] -= bp[act].mean()
+    Aa = S.A.toarray()[np.ix_(act, act)]
+    ref = np.zeros(n)
+    ref[act] = np.linalg.lstsq(Aa, bp[act], rcond=None)[0]
+    if S.singular:
+        ref[act] -= ref[act].mean()
+    np.testing.assert_allclose(z, ref, atol=1e-10 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("coarse_max", [96, 200, 400])
+def test_truncated_hierarchy_two_grid_properties(coarse_max):
+    """Moving the exact solve up a level keeps the preconditioner symmetric,
+    and an exact coarse solve contracts at least as well as recursing."""
+    case = oracle_case("cyl_outlet", nx=32, ny=24, lx=8.0, ly=6.0)
+    case.solid = circle(32, 24, 8.0, 6.0, 3.0, 3.0, 0.5)
+    S = O.assemble(case)
+    n = S.A.shape[0]
+    full = O.Hierarchy(S.A, (32, 24), S.fluid, S.singular, 1)
+    H = O.Hierarchy(S.A, (32, 24), S.fluid, S.singular, 1, coarse_max=coarse_max)
+    assert [lv["A"].shape[0] for lv in H.levels] == \
+        [lv["A"].shape[0] for lv in full.levels][:len(H.levels)]
+    assert H.levels[-1]["A"].shape[0] <= max(coarse_max, full.levels[-1]["A"].shape[0])
+    I = np.eye(n)
+    M = np.column_stack([H.apply(I[:, k]) for k in range(n)])
+    assert np.abs(M - M.T).max() < 1e-10 * np.abs(M).max()
+    A = S.A.toarray()
+
+    def factor(Mx):
+        act = S.fluid
+        E = (I - Mx @ A)[np.ix_(act, act)]
+        if S.singular:
+            Pm = np.eye(act.sum()) - 1.0 / act.sum()
+            E = Pm @ E @ Pm
+        return np.abs(np.linalg.eigvals(E)).max()
+
+    Mf = np.column_stack([full.apply(I[:, k]) for k in range(n)])
+    assert factor(M) <= factor(Mf) + 1e-12
+    assert factor(M) < 0.5
+
+
 # ---------------------------------------------------------------- time steps
 TRAJ = [("cavity", "direct", 20), ("cavity", "cg", 5), ("periodic", "direct", 10),
         ("channel", "direct", 10), ("cyl_outlet", "direct", 10), ("cyl_pout", "direct", 10)]
