@@ -357,38 +357,6 @@ k_cg_update(FineOp A, double* __restrict__ x, double* __restrict__ r,
   block_partials<2>(s, ops, partial, t.tile);
 }
 
-// Multigrid-CG direction: p = zp + beta*p_old with zp the projected V-cycle
-// output (z - mean_z on the active set), q = A p, p.q.
-template <typename T>
-__global__ void __launch_bounds__(kBlock)
-k_mgcg_dir(FineOp A, const T* __restrict__ z, const double* __restrict__ p_old,
-           double* __restrict__ p_new, double* __restrict__ q, double* partial,
-           const SolveState* st) {
-  if (st->done) return;
-  Tile2D t;
-  double s[1] = {0.0};
-  const bool first = st->iter == 0;
-  const double beta = st->beta;
-  const double mz = st->mean_z;
-  if (cell_of_tile(A.nx, A.ny, t)) {
-    const int i = t.i, j = t.j, k = i * A.ny + j;
-    const uint8_t f = A.flags[k];
-    const Nb nb = fine_neighbours(A, i, j);
-    auto pv = [&](int m) -> double {
-      const double zp = (A.flags[m] & F_SOLV) ? (double)z[m] - mz : 0.0;
-      return first ? zp : p_old[m] * beta + zp;
-    };
-    const double d = fine_diag(A, f, i, j);
-    const double pk = pv(k);
-    const double qk = fine_apply<double>(A, f, nb, k, d, pv);
-    p_new[k] = pk;
-    q[k] = qk;
-    s[0] = pk * qk;
-  }
-  const int ops[1] = {kSum};
-  block_partials<1>(s, ops, partial, t.tile);
-}
-
 // V-cycle input: b0 = r - mean(r) on the active set, 0 elsewhere
 // (MultigridHierarchy._project, multigrid.py:453-458).
 template <typename T>
@@ -620,12 +588,63 @@ k_cls_rows(ClassLevel<T> L, T w, const T* __restrict__ b, const T* __restrict__ 
 }
 
 // Restriction in one pass: b_c[I] = sum over the cells a of block I of
-// (r - w A t)[a] on active rows (t = D^-1 r), LPR lanes per fine row; the
-// block's cells are summed in x-major order (P_tent^T y with y = P-smoothed r).
+// (r - w A t)[a] on active rows (t = D^-1 r); the block's cells are summed in
+// x-major order (P_tent^T y with y the P-smoothed residual).  A group of
+// 4 * LPR lanes serves one coarse cell, LPR lanes per fine row of its block
+// (up to 2 x 2 cells), so the rows of a block are evaluated side by side.
 template <typename T, int LPR>
 __global__ void __launch_bounds__(kBlock)
 k_cls_restrict(ClassLevel<T> L, T w, const T* __restrict__ r, const T* __restrict__ t, int fx,
                int fy, int cnx, int cny, T* __restrict__ bc, const SolveState* st) {
+  static_assert(4 * LPR <= kWave, "a coarse cell's lanes must share a wavefront");
+  if (st->done) return;
+  constexpr int G = 4 * LPR;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int crow = gid / G;
+  const int sub = (gid % G) / LPR;
+  const int lane = gid % LPR;
+  const bool valid = crow < cnx * cny;
+  const int I = valid ? crow / cny : 0, J = valid ? crow - I * cny : 0;
+  const int a2 = sub / fy, c2 = sub - a2 * fy;
+  const int i = I * fx + a2, j = J * fy + c2;
+  const bool in = valid && sub < fx * fy && i < L.nx && j < L.ny;
+  const int row = in ? i * L.ny + j : 0;
+  const int c = in ? (int)L.cls[row] : 0;
+  const bool act = in && L.cactive[c];
+  T acc = (T)0;
+  if (act) {
+    const int e1 = L.row_ptr[c + 1];
+    int e = L.row_ptr[c] + lane;
+    for (; e + 3 * LPR < e1; e += 4 * LPR) {
+      const int q0 = cls_nb(L, row, e), q1 = cls_nb(L, row, e + LPR);
+      const int q2 = cls_nb(L, row, e + 2 * LPR), q3 = cls_nb(L, row, e + 3 * LPR);
+      const T v0 = L.ent_val[e], v1 = L.ent_val[e + LPR];
+      const T v2 = L.ent_val[e + 2 * LPR], v3 = L.ent_val[e + 3 * LPR];
+      const T x0 = t[q0], x1 = t[q1], x2 = t[q2], x3 = t[q3];
+      acc += v0 * x0;
+      acc += v1 * x1;
+      acc += v2 * x2;
+      acc += v3 * x3;
+    }
+    for (; e < e1; e += LPR) acc += L.ent_val[e] * t[cls_nb(L, row, e)];
+  }
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
+  const T y = act ? r[row] - w * acc : (T)0;
+  // Sum the block's rows in x-major order (slot order).
+  const int g0 = (int)(threadIdx.x % kWave) - (int)(threadIdx.x % G);
+  T sum = (T)0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) sum += __shfl(y, g0 + q * LPR, kWave);
+  if (valid && sub == 0 && lane == 0) bc[crow] = sum;
+}
+
+// The same for rows too long for four of them to share a wavefront: the
+// block's rows one after another, LPR lanes each.
+template <typename T, int LPR>
+__global__ void __launch_bounds__(kBlock)
+k_cls_restrict_serial(ClassLevel<T> L, T w, const T* __restrict__ r, const T* __restrict__ t,
+                      int fx, int fy, int cnx, int cny, T* __restrict__ bc, const SolveState* st) {
   if (st->done) return;
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const int crow = gid / LPR;
@@ -644,19 +663,8 @@ k_cls_restrict(ClassLevel<T> L, T w, const T* __restrict__ r, const T* __restric
       T acc = (T)0;
       if (act) {
         const int e1 = L.row_ptr[c + 1];
-        int e = L.row_ptr[c] + lane;
-        for (; e + 3 * LPR < e1; e += 4 * LPR) {
-          const int q0 = cls_nb(L, row, e), q1 = cls_nb(L, row, e + LPR);
-          const int q2 = cls_nb(L, row, e + 2 * LPR), q3 = cls_nb(L, row, e + 3 * LPR);
-          const T a0 = L.ent_val[e], a1 = L.ent_val[e + LPR];
-          const T a2v = L.ent_val[e + 2 * LPR], a3 = L.ent_val[e + 3 * LPR];
-          const T x0 = t[q0], x1 = t[q1], x2 = t[q2], x3 = t[q3];
-          acc += a0 * x0;
-          acc += a1 * x1;
-          acc += a2v * x2;
-          acc += a3 * x3;
-        }
-        for (; e < e1; e += LPR) acc += L.ent_val[e] * t[cls_nb(L, row, e)];
+        for (int e = L.row_ptr[c] + lane; e < e1; e += LPR)
+          acc += L.ent_val[e] * t[cls_nb(L, row, e)];
       }
 #pragma unroll
       for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
@@ -861,6 +869,7 @@ static int upload_level(const MGLevelHost& h, const MGLevelHost* next, bool fine
 int PoissonEngine::configure(const PoissonConfig& cfg, const std::vector<MGLevelHost>& levels) {
   cfg_ = cfg;
   free_levels();
+  dir_tile_ = FusedTile{};
   last_iters_ = 0;
   predicted_ = 0;
   const bool need_mg = cfg.kind == SK_MULTIGRID || cfg.kind == SK_MGCG || cfg.kind == SK_DIRECT;
@@ -868,6 +877,11 @@ int PoissonEngine::configure(const PoissonConfig& cfg, const std::vector<MGLevel
   if (levels.empty()) {
     set_last_error("multigrid solver configured without a hierarchy");
     return kErrBadArg;
+  }
+  if ((op_.nx == 1 && op_.dir[0] && op_.dir[1]) || (op_.ny == 1 && op_.dir[2] && op_.dir[3])) {
+    set_last_error("multigrid on a grid one cell wide between two pressure outlets is not "
+                   "supported; use pressure_solver='cg'");
+    return kErrUnsupported;
   }
   n_active0_ = 0;
   for (uint8_t a : levels[0].active) n_active0_ += a ? 1 : 0;
@@ -952,7 +966,7 @@ int PoissonEngine::tune_level(std::vector<LevelBuf>& lv, int l, SolveState* tst)
   L.fused = 0;
   if (!fine && L.hdi.empty()) return kOk;
   const unsigned per_cell = fine ? 3u * sizeof(T) + 1u : 3u * sizeof(T) + 2u;
-  const unsigned fixed = fine ? 64u * sizeof(T) : 0u;
+  const unsigned fixed = fine ? 256u * sizeof(T) : 0u;
   const int ftx = fine ? cfg_.tile_tx : 0, fty = fine ? cfg_.tile_ty : 0;
   std::vector<FusedTile> cands = tile_candidates(L.nx, L.ny, L.rxs, L.rys, nu + 1, L.fx, L.fy,
                                                  per_cell, fixed, kLdsCap, ftx, fty);
@@ -1071,14 +1085,78 @@ int PoissonEngine::tune_level(std::vector<LevelBuf>& lv, int l, SolveState* tst)
   return apply(best_fused, best);
 }
 
+// Tile of the level-0 multigrid-CG direction kernel (halo 1), measured like
+// the level legs.
+template <typename T>
+int PoissonEngine::tune_dir(SolveState* tst) {
+  const unsigned per_cell = 8u + 1u, fixed = 256u * 8u;
+  std::vector<FusedTile> cands =
+      tile_candidates(op_.nx, op_.ny, 1, 1, 1, 1, 1, per_cell, fixed, kLdsCap, 0, 0);
+  if (cands.empty()) {
+    set_last_error("no tiling of the %dx%d grid fits the direction kernel", op_.nx, op_.ny);
+    return kErrBadArg;
+  }
+  size_t most = 0;
+  for (const FusedTile& t : cands) most = std::max(most, (size_t)t.ntiles());
+  if (int rc = ensure_partials(most)) return rc;
+  const TuneKey key = {2, op_.nx, op_.ny, 1, 1, 1, 1, 0, (int)sizeof(T), op_.px, op_.py, 0, 0};
+  {
+    std::lock_guard<std::mutex> g(g_tune_mutex);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end())
+      for (const FusedTile& t : cands)
+        if (t.tx == it->second.tx && t.ty == it->second.ty) {
+          dir_tile_ = t;
+          return kOk;
+        }
+  }
+  FusedTile best = *std::min_element(cands.begin(), cands.end(),
+                                     [](const FusedTile& a, const FusedTile& b) {
+                                       return tile_cost(a) < tile_cost(b);
+                                     });
+  if (autotune_enabled() && cands.size() > 1) {
+    hipEvent_t e0, e1;
+    PCFD_HIP_CHECK(hipEventCreate(&e0));
+    PCFD_HIP_CHECK(hipEventCreate(&e1));
+    const T* z = (const T*)((sizeof(T) == 4) ? lv32_[0].y : lv64_[0].y);
+    float best_ms = -1.f;
+    for (const FusedTile& t : cands) {
+      for (int k = 0; k < 4; ++k) {
+        if (k == 1) PCFD_HIP_CHECK(hipEventRecord(e0, s_));
+        if (int rc = launch_fine_dir<T>(s_, op_, t, z, pa_, pb_, q_, partial_, tst)) return rc;
+      }
+      PCFD_HIP_CHECK(hipEventRecord(e1, s_));
+      PCFD_HIP_CHECK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      PCFD_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      if (best_ms < 0.f || ms < best_ms) {
+        best_ms = ms;
+        best = t;
+      }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+  }
+  {
+    std::lock_guard<std::mutex> g(g_tune_mutex);
+    g_tune_cache[key] = TunePick{1, best.tx, best.ty};
+  }
+  dir_tile_ = best;
+  return kOk;
+}
+
 template <typename T>
 int PoissonEngine::plan_level_tiles(std::vector<LevelBuf>& lv) {
-  if (lv.size() < 2) return kOk;
+  if (lv.empty()) return kOk;
   SolveState* tst = nullptr;
   PCFD_HIP_CHECK(hipMalloc(&tst, sizeof(SolveState)));
   PCFD_HIP_CHECK(hipMemsetAsync(tst, 0, sizeof(SolveState), s_));
   int rc = kOk;
   for (int l = 0; l + 1 < (int)lv.size() && rc == kOk; ++l) rc = tune_level<T>(lv, l, tst);
+  // the Krylov direction kernel reads the cycle output in this precision
+  if (rc == kOk && (cfg_.kind == SK_MGCG || cfg_.kind == SK_DIRECT) &&
+      (cfg_.precision == 1) == (sizeof(T) == 4))
+    rc = tune_dir<T>(tst);
   const hipError_t e = hipStreamSynchronize(s_);
   hipFree(tst);
   if (rc) return rc;
@@ -1359,24 +1437,24 @@ template <typename T>
 static void cls_restrict(hipStream_t s, int lanes, const ClassLevel<T>& L, T w, const T* r,
                          const T* t, int fx, int fy, int cnx, int cny, T* bc,
                          const SolveState* st) {
-  const long threads = (long)cnx * cny * lanes;
-  const dim3 grid((unsigned)((threads + kBlock - 1) / kBlock));
+  const long cells = (long)cnx * cny;
+  auto grid = [&](int per) { return dim3((unsigned)((cells * per + kBlock - 1) / kBlock)); };
   switch (lanes) {
     case 1:
-      hipLaunchKernelGGL((k_cls_restrict<T, 1>), grid, dim3(kBlock), 0, s, L, w, r, t, fx, fy,
+      hipLaunchKernelGGL((k_cls_restrict<T, 1>), grid(4), dim3(kBlock), 0, s, L, w, r, t, fx, fy,
                          cnx, cny, bc, st);
       break;
     case 4:
-      hipLaunchKernelGGL((k_cls_restrict<T, 4>), grid, dim3(kBlock), 0, s, L, w, r, t, fx, fy,
+      hipLaunchKernelGGL((k_cls_restrict<T, 4>), grid(16), dim3(kBlock), 0, s, L, w, r, t, fx, fy,
                          cnx, cny, bc, st);
       break;
     case 16:
-      hipLaunchKernelGGL((k_cls_restrict<T, 16>), grid, dim3(kBlock), 0, s, L, w, r, t, fx, fy,
-                         cnx, cny, bc, st);
+      hipLaunchKernelGGL((k_cls_restrict<T, 16>), grid(64), dim3(kBlock), 0, s, L, w, r, t, fx,
+                         fy, cnx, cny, bc, st);
       break;
     default:
-      hipLaunchKernelGGL((k_cls_restrict<T, 64>), grid, dim3(kBlock), 0, s, L, w, r, t, fx, fy,
-                         cnx, cny, bc, st);
+      hipLaunchKernelGGL((k_cls_restrict_serial<T, 64>), grid(64), dim3(kBlock), 0, s, L, w, r, t,
+                         fx, fy, cnx, cny, bc, st);
       break;
   }
 }
@@ -1513,15 +1591,12 @@ int PoissonEngine::run_mgcg() {
   const bool f32 = cfg_.precision == 1;
   PCFD_ITERATE(1, cfg_.maxiter, {
     if (int rc = enqueue_vcycle(VC_KRYLOV, SR_MGCG)) return rc;
-    if (f32) {
-      hipLaunchKernelGGL(k_mgcg_dir<float>, g, dim3(kBlock), 0, s_, op_, (const float*)lv32_[0].y,
-                         p_old, p_new, q_, partial_, (const SolveState*)st_);
-    } else {
-      hipLaunchKernelGGL(k_mgcg_dir<double>, g, dim3(kBlock), 0, s_, op_,
-                         (const double*)lv64_[0].y, p_old, p_new, q_, partial_,
-                         (const SolveState*)st_);
-    }
-    if (int rc = fold(s_, partial_, nt, EpiAlpha{st_})) return rc;
+    const int rc = f32 ? launch_fine_dir<float>(s_, op_, dir_tile_, (const float*)lv32_[0].y,
+                                                p_old, p_new, q_, partial_, st_)
+                       : launch_fine_dir<double>(s_, op_, dir_tile_, (const double*)lv64_[0].y,
+                                                 p_old, p_new, q_, partial_, st_);
+    if (rc) return rc;
+    if (int rc2 = fold(s_, partial_, dir_tile_.ntiles(), EpiAlpha{st_})) return rc2;
     hipLaunchKernelGGL(k_cg_update, g, dim3(kBlock), 0, s_, op_, x_, r_, p_new, q_, (int)IM_MGCG,
                        partial_, (const SolveState*)st_);
     if (int rc = fold(s_, partial_, nt, EpiUpdate{st_, (int)IM_MGCG, cfg_.maxiter, n_proj_}))
@@ -1582,6 +1657,10 @@ int PoissonEngine::time_kernel(int which, int reps, double* avg_ms, double* byte
   const dim3 g = tiles_for(op_.nx, op_.ny);
   const double n = (double)n_;
   const bool have_mg = lv64_.size() >= 2;
+  if (which == 1 && dir_tile_.ntiles() == 0) {
+    set_last_error("kernel 1 needs a multigrid-CG solver configured");
+    return kErrBadArg;
+  }
   if ((which == 2 || which == 3) && !have_mg) {
     set_last_error("kernel %d needs a multigrid hierarchy of at least two levels", which);
     return kErrBadArg;
@@ -1601,8 +1680,9 @@ int PoissonEngine::time_kernel(int which, int reps, double* avg_ms, double* byte
         B = n * (6 * 8 + 1);
         break;
       case 1:  // multigrid-CG direction + SpMV: read z, p_old + flags, write p, q
-        hipLaunchKernelGGL(k_mgcg_dir<double>, g, dim3(kBlock), 0, s_, op_, (const double*)z_,
-                           pa_, pb_, q_, partial_, (const SolveState*)st);
+        if (int rc = launch_fine_dir<double>(s_, op_, dir_tile_, (const double*)z_, pa_, pb_, q_,
+                                             partial_, st))
+          return rc;
         B = n * (4 * 8 + 1);
         break;
       case 2: {  // level-0 down leg: read r + flags, write b0, x, and b_c (n/4)

@@ -207,6 +207,8 @@ class PoissonEngine {
   template <typename T>
   int tune_level(std::vector<LevelBuf>& lv, int l, SolveState* tst);
   int ensure_partials(size_t n);
+  template <typename T>
+  int tune_dir(SolveState* tst);
   int poll(int* iter, int* done);
   int start(int mode, double* r);
   void free_levels();
@@ -229,6 +231,7 @@ class PoissonEngine {
   uint8_t* fine_active_ = nullptr;
   int n_active0_ = 0;
   int n_proj_ = 0;  // cells of the constant mode removed by projections (0: non-singular)
+  FusedTile dir_tile_;  // level-0 tiling of the multigrid-CG direction kernel
   double weight0_ = 1.0;
 };
 

@@ -17,16 +17,21 @@ namespace {
 constexpr uint8_t F_DIRB = 0x20;  // region flag: solvable cell on a Dirichlet wall
 constexpr uint16_t kOutside = 0xFFFF;
 
+// Periodic image of i in [0, n).  Halos span a few cells, so the loops run
+// once at most except on grids smaller than the halo.
 __device__ __forceinline__ int wrap_index(int i, int n) {
-  const int m = i % n;
-  return m < 0 ? m + n : m;
+  while (i < 0) i += n;
+  while (i >= n) i -= n;
+  return i;
 }
 
 // Global cell of a region cell; false when it lies outside a non-periodic grid.
 __device__ __forceinline__ bool resolve(int gi, int gj, int nx, int ny, int px, int py, int& i,
                                         int& j) {
-  i = px ? wrap_index(gi, nx) : gi;
-  j = py ? wrap_index(gj, ny) : gj;
+  i = gi;
+  j = gj;
+  if (px) i = wrap_index(gi, nx);
+  if (py) j = wrap_index(gj, ny);
   return i >= 0 && i < nx && j >= 0 && j < ny;
 }
 
@@ -78,23 +83,26 @@ __device__ __forceinline__ void for_ring(const Ring& g, Fn&& fn) {
 // ---------------------------------------------------------------------------
 // Level 0: matrix-free 5-point operator from the flag bytes.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool on_dirichlet(const FineOp& A, int i, int j) {
-  return (A.dir[0] && i == 0) || (A.dir[1] && i == A.nx - 1) || (A.dir[2] && j == 0) ||
-         (A.dir[3] && j == A.ny - 1);
-}
 
-// Diagonal and its inverse for the 32 face-flag combinations of cells away
-// from Dirichlet walls (pycfd/core/pressure.py:219-232 order: x faces, then y).
+// Region cell code: the flag byte's low five bits (solvable, four flux faces)
+// plus bit 5 for a solvable cell on a Dirichlet x wall (i = 0 or nx - 1) and
+// bit 6 for one on a Dirichlet y wall.  The diagonal and its inverse are a
+// 128-entry table of the code, in pycfd/core/pressure.py:219-238's
+// accumulation order (x faces and wall, then y faces and wall).
+constexpr uint8_t C_XWALL = 0x20, C_YWALL = 0x40;
+
 template <typename T>
 __device__ __forceinline__ void fine_tables(const FineOp& A, T* td, T* ti) {
   const int f = threadIdx.x;
-  if (f < 32) {
+  if (f < 128) {
     double d = 1.0;
     if (f & F_SOLV) {
       d = 0.0;
       if (f & F_L) d -= A.cx;
+      if (f & C_XWALL) d -= 2.0 * A.cx;
       if (f & F_R) d -= A.cx;
       if (f & F_B) d -= A.cy;
+      if (f & C_YWALL) d -= 2.0 * A.cy;
       if (f & F_T) d -= A.cy;
     }
     td[f] = (T)d;
@@ -102,14 +110,13 @@ __device__ __forceinline__ void fine_tables(const FineOp& A, T* td, T* ti) {
   }
 }
 
-template <typename T>
-__device__ __forceinline__ T rdiag(const FineOp& A, const T* td, uint8_t f, int gi, int gj) {
-  return (f & F_DIRB) ? (T)fine_diag(A, f & 31, gi, gj) : td[f & 31];
-}
-
-template <typename T>
-__device__ __forceinline__ T rdinv(const FineOp& A, const T* ti, uint8_t f, int gi, int gj) {
-  return (f & F_DIRB) ? (T)(1.0 / fine_diag(A, f & 31, gi, gj)) : ti[f & 31];
+__device__ __forceinline__ uint8_t cell_code(const FineOp& A, uint8_t f, int i, int j) {
+  uint8_t code = f & 31;
+  if (f & F_SOLV) {
+    if ((A.dir[0] && i == 0) || (A.dir[1] && i == A.nx - 1)) code |= C_XWALL;
+    if ((A.dir[2] && j == 0) || (A.dir[3] && j == A.ny - 1)) code |= C_YWALL;
+  }
+  return code;
 }
 
 // (A x) at region cell idx, terms in CSR column order.
@@ -124,14 +131,15 @@ __device__ __forceinline__ T rapply(const FineOp& A, uint8_t f, const T* X, int 
   return y;
 }
 
+// LDS of a level-0 tile: the diagonal tables, three vectors and the cell codes.
 template <typename T>
 struct FineLds {
   T *td, *ti, *P, *Q, *S;
   uint8_t* F;
   __device__ FineLds(unsigned char* smem, int rn) {
     td = (T*)smem;
-    ti = td + 32;
-    P = ti + 32;
+    ti = td + 128;
+    P = ti + 128;
     Q = P + rn;
     S = Q + rn;
     F = (uint8_t*)(S + rn);
@@ -151,29 +159,34 @@ k_fine_down(FineOp A, FusedTile ft, T w, int nu, int nonzero, const double* __re
   T* B = M.P;
   T* X = M.Q;
   T* Y = M.S;
-  uint8_t* F = M.F;
+  const T* td = M.td;
+  const T* ti = M.ti;
+  const uint8_t* F = M.F;
   const TileOrigin o = tile_origin(ft);
   fine_tables<T>(A, M.td, M.ti);
   const double mr = r ? st->mean_r : 0.0;
   const T m = shift_in ? (T)st->mean_x : (T)0;
+  __syncthreads();
 
+  // Load; a zero start also forms x = 0 + w D^-1 (b - A 0) here.
   for_ring(Ring{0, ft.rx, 0, ry}, [&](int a, int c) {
     int i, j;
     uint8_t f = 0;
     T bv = (T)0, xv = (T)0;
     if (resolve(o.gi0 + a, o.gj0 + c, A.nx, A.ny, A.px, A.py, i, j)) {
       const int k = i * A.ny + j;
-      f = A.flags[k];
+      f = cell_code(A, A.flags[k], i, j);
       if (r) bv = (f & F_SOLV) ? (T)(r[k] - mr) : (T)0;
       else bv = b_in[k];
       if (nonzero) {
         const T x0 = x_in[k];
         xv = (f & F_SOLV) ? x0 - m : x0;
+      } else {
+        xv = w * ti[f] * bv;
       }
-      if ((f & F_SOLV) && on_dirichlet(A, i, j)) f |= F_DIRB;
     }
     const int idx = a * ry + c;
-    F[idx] = f;
+    M.F[idx] = f;
     B[idx] = bv;
     X[idx] = xv;
   });
@@ -181,23 +194,12 @@ k_fine_down(FineOp A, FusedTile ft, T w, int nu, int nonzero, const double* __re
 
   // Pre-smoothing; v is the number of stencil rings still valid.
   int v = ft.hx;
-  int s0 = 0;
-  if (!nonzero) {  // x = 0 + w D^-1 (b - A 0)
-    for_ring(Ring{0, ft.rx, 0, ry}, [&](int a, int c) {
-      const int idx = a * ry + c;
-      X[idx] = w * rdinv(A, M.ti, F[idx], o.gi0 + a, o.gj0 + c) * B[idx];
-    });
-    __syncthreads();
-    s0 = 1;
-  }
-  for (int s = s0; s < nu; ++s) {
+  for (int s = nonzero ? 0 : 1; s < nu; ++s) {
     --v;
     for_ring(ring(ft, v, 1, 1), [&](int a, int c) {
       const int idx = a * ry + c;
       const uint8_t f = F[idx];
-      const int gi = o.gi0 + a, gj = o.gj0 + c;
-      const T d = rdiag(A, M.td, f, gi, gj);
-      Y[idx] = X[idx] + (w * rdinv(A, M.ti, f, gi, gj)) * (B[idx] - rapply(A, f, X, idx, ry, d));
+      Y[idx] = X[idx] + (w * ti[f]) * (B[idx] - rapply(A, f, X, idx, ry, td[f]));
     });
     __syncthreads();
     T* t = X;
@@ -209,7 +211,7 @@ k_fine_down(FineOp A, FusedTile ft, T w, int nu, int nonzero, const double* __re
   for_ring(ring(ft, 1, 1, 1), [&](int a, int c) {
     const int idx = a * ry + c;
     const uint8_t f = F[idx];
-    Y[idx] = B[idx] - rapply(A, f, X, idx, ry, rdiag(A, M.td, f, o.gi0 + a, o.gj0 + c));
+    Y[idx] = B[idx] - rapply(A, f, X, idx, ry, td[f]);
   });
   __syncthreads();
   // Hand the tile's iterate (and right-hand side) over, then X <- D^-1 r.
@@ -224,13 +226,14 @@ k_fine_down(FineOp A, FusedTile ft, T w, int nu, int nonzero, const double* __re
         if (b_out) b_out[k] = B[idx];
       }
     }
-    X[idx] = rdinv(A, M.ti, F[idx], o.gi0 + a, o.gj0 + c) * Y[idx];
+    X[idx] = ti[F[idx]] * Y[idx];
   });
   __syncthreads();
-  // Restriction: one thread per coarse cell of the tile.
+  // Restriction: one thread per coarse cell of the tile (cty is a power of two).
   const int ctx = ft.tx / fx, cty = ft.ty / fy;
+  const int sh = __ffs(cty) - 1;
   for (int q = threadIdx.x; q < ctx * cty; q += kBlock) {
-    const int ca = q / cty, cc = q - ca * cty;
+    const int ca = q >> sh, cc = q & (cty - 1);
     const int I = o.ti * ctx + ca, J = o.tj * cty + cc;
     if (I >= cnx || J >= cny) continue;
     T acc = (T)0;
@@ -240,12 +243,10 @@ k_fine_down(FineOp A, FusedTile ft, T w, int nu, int nonzero, const double* __re
       for (int c2 = 0; c2 < fy; ++c2) {
         const int j = J * fy + c2;
         if (j >= A.ny) break;
-        const int a = ft.hx + ca * fx + a2, c = ft.hy + cc * fy + c2;
-        const int idx = a * ry + c;
+        const int idx = (ft.hx + ca * fx + a2) * ry + (ft.hy + cc * fy + c2);
         const uint8_t f = F[idx];
         if (!(f & F_SOLV)) continue;
-        const T atr = rapply(A, f, X, idx, ry, rdiag(A, M.td, f, o.gi0 + a, o.gj0 + c));
-        acc += Y[idx] - w * atr;
+        acc += Y[idx] - w * rapply(A, f, X, idx, ry, td[f]);
       }
     }
     bc[I * cny + J] = acc;
@@ -264,9 +265,12 @@ k_fine_up(FineOp A, FusedTile ft, T w, int nu, const T* __restrict__ x_pre,
   T* E = M.P;
   T* X = M.Q;
   T* B = M.S;
-  uint8_t* F = M.F;
+  const T* td = M.td;
+  const T* ti = M.ti;
+  const uint8_t* F = M.F;
   const TileOrigin o = tile_origin(ft);
   fine_tables<T>(A, M.td, M.ti);
+  __syncthreads();
 
   for_ring(Ring{0, ft.rx, 0, ry}, [&](int a, int c) {
     int i, j;
@@ -274,14 +278,13 @@ k_fine_up(FineOp A, FusedTile ft, T w, int nu, const T* __restrict__ x_pre,
     T xv = (T)0, bv = (T)0, ev = (T)0;
     if (resolve(o.gi0 + a, o.gj0 + c, A.nx, A.ny, A.px, A.py, i, j)) {
       const int k = i * A.ny + j;
-      f = A.flags[k];
+      f = cell_code(A, A.flags[k], i, j);
       xv = x_pre[k];
       bv = b0[k];
       if (f & F_SOLV) ev = xc[(i / fx) * cny + (j / fy)];
-      if ((f & F_SOLV) && on_dirichlet(A, i, j)) f |= F_DIRB;
     }
     const int idx = a * ry + c;
-    F[idx] = f;
+    M.F[idx] = f;
     X[idx] = xv;
     B[idx] = bv;
     E[idx] = ev;
@@ -291,10 +294,8 @@ k_fine_up(FineOp A, FusedTile ft, T w, int nu, const T* __restrict__ x_pre,
   for_ring(ring(ft, nu, 1, 1), [&](int a, int c) {
     const int idx = a * ry + c;
     const uint8_t f = F[idx];
-    const int gi = o.gi0 + a, gj = o.gj0 + c;
-    const T d = rdiag(A, M.td, f, gi, gj);
-    const T ae = rapply(A, f, E, idx, ry, d);
-    X[idx] += E[idx] - w * rdinv(A, M.ti, f, gi, gj) * ae;
+    const T ae = rapply(A, f, E, idx, ry, td[f]);
+    X[idx] += E[idx] - w * ti[f] * ae;
   });
   __syncthreads();
   // Post-smoothing; the last sweep writes the tile straight to z.
@@ -305,14 +306,12 @@ k_fine_up(FineOp A, FusedTile ft, T w, int nu, const T* __restrict__ x_pre,
     for_ring(ring(ft, nu - sw, 1, 1), [&](int a, int c) {
       const int idx = a * ry + c;
       const uint8_t f = F[idx];
-      const int gi = o.gi0 + a, gj = o.gj0 + c;
-      const T d = rdiag(A, M.td, f, gi, gj);
-      const T val =
-          X[idx] + (w * rdinv(A, M.ti, f, gi, gj)) * (B[idx] - rapply(A, f, X, idx, ry, d));
+      const T val = X[idx] + (w * ti[f]) * (B[idx] - rapply(A, f, X, idx, ry, td[f]));
       if (!last) {
         Y[idx] = val;
         return;
       }
+      const int gi = o.gi0 + a, gj = o.gj0 + c;
       if (gi >= A.nx || gj >= A.ny) return;  // tile cells past the grid's end
       z[gi * A.ny + gj] = val;
       if (reduce != SR_NONE && (f & F_SOLV)) {
@@ -334,14 +333,81 @@ k_fine_up(FineOp A, FusedTile ft, T w, int nu, const T* __restrict__ x_pre,
   block_partials<3>(s, ops, partial, o.t);
 }
 
+// Multigrid-CG direction on level 0 (pressure.py:577-632 through scipy's cg):
+// p = zp + beta p_old with zp = z - mean_z on solvable cells (0 elsewhere),
+// q = A p, and the tile's partial of p.q.  p is formed once per region cell
+// in LDS (halo 1), then q on the tile.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_fine_dir(FineOp A, FusedTile ft, const T* __restrict__ z, const double* __restrict__ p_old,
+           double* __restrict__ p_new, double* __restrict__ q, double* partial,
+           const SolveState* __restrict__ st) {
+  if (st->done) return;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int ry = ft.ry, rn = ft.rx * ry;
+  double* td = (double*)smem;
+  double* ti = td + 128;
+  double* P = ti + 128;
+  uint8_t* F = (uint8_t*)(P + rn);
+  const TileOrigin o = tile_origin(ft);
+  fine_tables<double>(A, td, ti);
+  const bool first = st->iter == 0;
+  const double beta = st->beta;
+  const double mz = st->mean_z;
+  for_ring(Ring{0, ft.rx, 0, ry}, [&](int a, int c) {
+    int i, j;
+    uint8_t f = 0;
+    double pv = 0.0;
+    if (resolve(o.gi0 + a, o.gj0 + c, A.nx, A.ny, A.px, A.py, i, j)) {
+      const int k = i * A.ny + j;
+      f = cell_code(A, A.flags[k], i, j);
+      const double zp = (f & F_SOLV) ? (double)z[k] - mz : 0.0;
+      pv = first ? zp : p_old[k] * beta + zp;
+    }
+    const int idx = a * ry + c;
+    F[idx] = f;
+    P[idx] = pv;
+  });
+  __syncthreads();
+  double s[1] = {0.0};
+  for_ring(ring(ft, 0, 1, 1), [&](int a, int c) {
+    const int gi = o.gi0 + a, gj = o.gj0 + c;
+    if (gi >= A.nx || gj >= A.ny) return;
+    const int idx = a * ry + c;
+    const uint8_t f = F[idx];
+    const double pk = P[idx];
+    const double qk = rapply(A, f, P, idx, ry, td[f]);
+    const int k = gi * A.ny + gj;
+    p_new[k] = pk;
+    q[k] = qk;
+    s[0] += pk * qk;
+  });
+  const int ops[1] = {kSum};
+  block_partials<1>(s, ops, partial, o.t);
+}
+
 // ---------------------------------------------------------------------------
 // Levels >= 1: stored Galerkin rows (class dictionary).
 // ---------------------------------------------------------------------------
+// Row c of the level applied at region cell idx; four entries per trip with
+// their loads issued together, summed in entry order.
 template <typename T>
 __device__ __forceinline__ T class_row(const ClassRows<T>& L, int c, const T* X, int idx) {
   T acc = (T)0;
   const int e1 = L.row_ptr[c + 1];
-  for (int e = L.row_ptr[c]; e < e1; ++e) acc += L.ent_val[e] * X[idx + L.ent_off[e]];
+  int e = L.row_ptr[c];
+  for (; e + 3 < e1; e += 4) {
+    const int o0 = L.ent_off[e], o1 = L.ent_off[e + 1], o2 = L.ent_off[e + 2],
+              o3 = L.ent_off[e + 3];
+    const T a0 = L.ent_val[e], a1 = L.ent_val[e + 1], a2 = L.ent_val[e + 2],
+            a3 = L.ent_val[e + 3];
+    const T x0 = X[idx + o0], x1 = X[idx + o1], x2 = X[idx + o2], x3 = X[idx + o3];
+    acc += a0 * x0;
+    acc += a1 * x1;
+    acc += a2 * x2;
+    acc += a3 * x3;
+  }
+  for (; e < e1; ++e) acc += L.ent_val[e] * X[idx + L.ent_off[e]];
   return acc;
 }
 
@@ -578,6 +644,16 @@ int launch_class_up(hipStream_t s, const ClassRows<T>& L, const FusedTile& ft, i
 }
 
 template <typename T>
+int launch_fine_dir(hipStream_t s, const FineOp& A, const FusedTile& ft, const T* z,
+                    const double* p_old, double* p_new, double* q, double* partial,
+                    const SolveState* st) {
+  hipLaunchKernelGGL(k_fine_dir<T>, dim3(ft.ntiles()), dim3(kBlock), ft.lds, s, A, ft, z, p_old,
+                     p_new, q, partial, st);
+  PCFD_LAUNCH_CHECK();
+  return kOk;
+}
+
+template <typename T>
 int launch_dense_apply(hipStream_t s, const double* C, int n, const T* b, T* x,
                        const SolveState* st) {
   constexpr int kWaves = kBlock / kWave;
@@ -602,6 +678,8 @@ std::vector<FusedTile> tile_candidates(int nx, int ny, int rxs, int rys, int hal
       if (force_ty > 0 && ty != force_ty) continue;
       if (ty % fy) continue;
       if (force_ty <= 0 && ty > 2 * ny && ty > fy) continue;
+      // tiny tiles only where the level itself is tiny
+      if (force_tx <= 0 && force_ty <= 0 && tx * ty < 32 && (long)nx * ny >= 4096) continue;
       FusedTile t;
       t.tx = tx;
       t.ty = ty;
@@ -655,7 +733,9 @@ bool choose_tile(int nx, int ny, int rxs, int rys, int halo_mult, int fx, int fy
                                   T, int, const T*, const T*, const T*, int, int, int, T*,    \
                                   const SolveState*);                                         \
   template int launch_dense_apply<T>(hipStream_t, const double*, int, const T*, T*,            \
-                                     const SolveState*);
+                                     const SolveState*);                                      \
+  template int launch_fine_dir<T>(hipStream_t, const FineOp&, const FusedTile&, const T*,      \
+                                  const double*, double*, double*, double*, const SolveState*);
 PCFD_VCYCLE_INST(float)
 PCFD_VCYCLE_INST(double)
 #undef PCFD_VCYCLE_INST

@@ -59,6 +59,13 @@ int launch_class_up(hipStream_t s, const ClassRows<T>& L, const FusedTile& ft, i
                     T w, int nu, const T* x_pre, const T* b, const T* xc, int fx, int fy, int cny,
                     T* x_out, const SolveState* st);
 
+// Multigrid-CG direction and operator product on level 0 (halo-1 tiling):
+// p_new = (z - mean_z) + beta p_old, q = A p_new, per-tile partials of p.q.
+template <typename T>
+int launch_fine_dir(hipStream_t s, const FineOp& A, const FusedTile& ft, const T* z,
+                    const double* p_old, double* p_new, double* q, double* partial,
+                    const SolveState* st);
+
 // x = C b with C an n x n dense FP64 matrix, spread over many workgroups.
 template <typename T>
 int launch_dense_apply(hipStream_t s, const double* C, int n, const T* b, T* x,
