@@ -188,6 +188,11 @@ int pcfd_step(pcfd_ctx* ctx, const double* u0, const double* v0, double dt, doub
               double* u, double* v, double* p, pcfd_solve_info* info);
 /* Forget the warm-start pressure (ProjectionSolver._p_guess = None). */
 int pcfd_reset_guess(pcfd_ctx* ctx);
+/* The warm-start pressure (ProjectionSolver._p_guess, solver.py:171,662-663)
+ * as an (nx, ny) device array: store = 0 copies it out to buf, store = 1
+ * replaces it with buf.  The reference keeps no such state in a checkpoint;
+ * persisting it makes an iterative-solver resume continue bit for bit. */
+int pcfd_pressure_guess(pcfd_ctx* ctx, double* buf, int32_t store);
 
 /* --- reductions for TimeStepper (timestepper.py:94-142, 208-266) ----------
  * out[0] max|u_phys| out[1] max|v_phys| out[2] non-finite count in u, v, p

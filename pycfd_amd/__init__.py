@@ -11,6 +11,8 @@ host side that keeps the reference's API:
     result = sim.run(max_steps=100)
 """
 
+__version__ = "0.1.0"
+
 from .config import (AdvectionScheme, BCKind, BCSpec, PressureSolver, SimulationConfig,
                      SolverType, TimeScheme)
 

@@ -111,6 +111,7 @@ _SIGNATURES = {
     "pcfd_mg_apply": [C.c_void_p, C.c_void_p, C.c_void_p],
     "pcfd_time_kernel": [C.c_void_p, C.c_int32, C.c_int32, _dp, _dp],
     "pcfd_mg_plan": [C.c_void_p, C.POINTER(C.c_int32), C.c_int32],
+    "pcfd_pressure_guess": [C.c_void_p, C.c_void_p, C.c_int32],
 }
 
 EXPORTED = tuple(_SIGNATURES)

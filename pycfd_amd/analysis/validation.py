@@ -94,10 +94,13 @@ def linf_error(numerical, reference, relative: bool = False) -> float:
     return err
 
 
-def ghia_errors(fields) -> dict[str, float]:
-    """Centre-line L2 errors against Ghia et al. (lid_driven_cavity.py:88-117)."""
+def ghia_errors(fields, re: float = 100.0) -> dict[str, float]:
+    """Centre-line L2 and L-infinity errors against Ghia et al. at ``re``
+    (lid_driven_cavity.py:88-117)."""
     from .postprocess import centerline_profiles
     y, u_line, x, v_line = centerline_profiles(fields)
-    ref = ghia_reference(100)
-    return {"ghia_u_L2": l2_error(np.interp(ref["y"], y, u_line), ref["u"]),
-            "ghia_v_L2": l2_error(np.interp(ref["x"], x, v_line), ref["v"])}
+    ref = ghia_reference(int(round(re)))
+    u_at = np.interp(ref["y"], y, u_line)
+    v_at = np.interp(ref["x"], x, v_line)
+    return {"ghia_u_L2": l2_error(u_at, ref["u"]), "ghia_u_Linf": linf_error(u_at, ref["u"]),
+            "ghia_v_L2": l2_error(v_at, ref["v"]), "ghia_v_Linf": linf_error(v_at, ref["v"])}

@@ -172,6 +172,10 @@ class DeviceContext:
     def reset_guess(self) -> None:
         self._call("pcfd_reset_guess")
 
+    def pressure_guess(self, buf, store: bool) -> None:
+        """Copy the warm-start pressure out to (store=False) or in from ``buf``."""
+        self._call("pcfd_pressure_guess", _ptr(buf), int(bool(store)))
+
     def diagnostics(self, u, v, p, u_prev=None, v_prev=None) -> np.ndarray:
         out = np.zeros(16)
         self._call("pcfd_diagnostics", _ptr(u), _ptr(v), _ptr(p), _ptr(u_prev), _ptr(v_prev),

@@ -41,21 +41,53 @@ def to_host(t) -> np.ndarray:
 
 @dataclass
 class FlowField:
-    """The staggered state plus simulation clock (fields.py:18-39)."""
+    """The staggered state plus simulation clock (fields.py:18-39).
+
+    ``u``, ``v``, ``p`` may be given as device tensors or as host arrays (a
+    restored checkpoint); host arrays are uploaded on first device use.
+    """
 
     mesh: StructuredMesh
-    u_dev: object
-    v_dev: object
-    p_dev: object
+    u_src: object
+    v_src: object
+    p_src: object
     t: float = 0.0
     step: int = 0
     diagnostics: dict = field(default_factory=dict)
     _host: dict = field(default_factory=dict, repr=False)
 
+    def __post_init__(self) -> None:
+        for name in ("u", "v", "p"):
+            src = getattr(self, name + "_src")
+            if isinstance(src, np.ndarray):
+                arr = np.array(src, dtype=np.float64, order="C")
+                arr.flags.writeable = False
+                self._host[name] = arr
+                setattr(self, name + "_src", None)
+
     @classmethod
     def zeros(cls, mesh: StructuredMesh) -> "FlowField":
         return cls(mesh, device_zeros(mesh.u_shape), device_zeros(mesh.v_shape),
                    device_zeros(mesh.p_shape))
+
+    def _dev(self, name: str):
+        src = getattr(self, name + "_src")
+        if src is None:
+            src = to_device(self._host[name])
+            setattr(self, name + "_src", src)
+        return src
+
+    @property
+    def u_dev(self):
+        return self._dev("u")
+
+    @property
+    def v_dev(self):
+        return self._dev("v")
+
+    @property
+    def p_dev(self):
+        return self._dev("p")
 
     def copy(self) -> "FlowField":
         return FlowField(self.mesh, self.u_dev.clone(), self.v_dev.clone(), self.p_dev.clone(),
@@ -63,12 +95,14 @@ class FlowField:
 
     def invalidate(self) -> None:
         """Drop cached host copies after the device arrays were changed in place."""
+        for name in ("u", "v", "p"):
+            self._dev(name)
         self._host.clear()
 
     def _view(self, name: str) -> np.ndarray:
         arr = self._host.get(name)
         if arr is None:
-            arr = to_host(getattr(self, name + "_dev")).copy()
+            arr = to_host(getattr(self, name + "_src")).copy()
             arr.flags.writeable = False
             self._host[name] = arr
         return arr

@@ -239,6 +239,29 @@ class ProjectionSolver:
         self._force_stale = self.has_obstacle
         return FlowField(mesh, u, v, p, fields.t + dt, fields.step + 1)
 
+    # -- warm start (solver.py:171, 662-663) ------------------------------ #
+    def pressure_guess(self) -> np.ndarray | None:
+        """The normalised pressure the next solve starts from, ``(nx, ny)``,
+        or ``None`` before the first projection."""
+        if not self._has_guess:
+            return None
+        import torch
+        buf = torch.empty(self.mesh.shape, dtype=torch.float64, device="cuda")
+        self.ctx.pressure_guess(buf, store=False)
+        return to_host(buf)
+
+    def set_pressure_guess(self, p: np.ndarray | None) -> None:
+        """Replace the warm-start pressure (``None`` forgets it)."""
+        if p is None:
+            self.ctx.reset_guess()
+            self._has_guess = False
+            return
+        arr = np.asarray(p, dtype=np.float64)
+        if arr.shape != tuple(self.mesh.shape):
+            raise ValueError(f"pressure guess has shape {arr.shape}, expected {self.mesh.shape}")
+        self.ctx.pressure_guess(to_device(arr), store=True)
+        self._has_guess = True
+
     # ------------------------------------------------------------------ #
     def set_upwind_blend(self, blend: float) -> None:
         self._blend = float(np.clip(blend, 0.0, 1.0))

@@ -655,6 +655,24 @@ int pcfd_reset_guess(pcfd_ctx* c) {
   return c->engine.reset_guess();
 }
 
+int pcfd_pressure_guess(pcfd_ctx* c, double* buf, int32_t store) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!buf) {
+    set_last_error("pcfd_pressure_guess: null buffer");
+    return kErrBadArg;
+  }
+  const size_t bytes = (size_t)c->nx * c->ny * sizeof(double);
+  if (store) {
+    PCFD_HIP_CHECK(
+        hipMemcpyAsync(c->engine.guess(), buf, bytes, hipMemcpyDeviceToDevice, c->stream));
+  } else {
+    PCFD_HIP_CHECK(
+        hipMemcpyAsync(buf, c->engine.guess(), bytes, hipMemcpyDeviceToDevice, c->stream));
+  }
+  PCFD_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return kOk;
+}
+
 int pcfd_diagnostics(pcfd_ctx* c, const double* u, const double* v, const double* p,
                      const double* u_prev, const double* v_prev, double* out_host) {
   if (int rc = check_ctx(c)) return rc;

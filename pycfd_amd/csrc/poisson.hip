@@ -780,7 +780,7 @@ int PoissonEngine::init(const FineOp& op, int n_fluid, int singular, hipStream_t
 // LDS budget of one fused level workgroup (the dynamic allocation a launch
 // may take without raising the kernel's attribute).
 constexpr unsigned kLdsCap = 64u * 1024u;
-constexpr int kFuseMaxRadius = 3;
+constexpr int kFuseMaxRadius = 8;
 
 template <typename T>
 static int upload_level(const MGLevelHost& h, const MGLevelHost* next, bool fine, int nu,
@@ -825,6 +825,14 @@ static int upload_level(const MGLevelHost& h, const MGLevelHost* next, bool fine
     if (next && fused_ok && std::max(L.rxs, L.rys) <= kFuseMaxRadius) {
       L.hdi = edi;
       L.hdj = edj;
+      std::vector<int> count(nc, 0);
+      for (uint16_t c : h.cls) ++count[c];
+      const int hot = (int)(std::max_element(count.begin(), count.end()) - count.begin());
+      const int hot_n = rp[hot + 1] - rp[hot];
+      if (hot_n <= kHotMax) {
+        L.hot = hot;
+        L.hot_n = hot_n;
+      }
     }
     const size_t nnz = std::max<size_t>(val.size(), 1);
     PCFD_HIP_CHECK(hipMalloc(&L.cls, n * sizeof(uint16_t)));
@@ -966,7 +974,8 @@ int PoissonEngine::tune_level(std::vector<LevelBuf>& lv, int l, SolveState* tst)
   L.fused = 0;
   if (!fine && L.hdi.empty()) return kOk;
   const unsigned per_cell = fine ? 3u * sizeof(T) + 1u : 3u * sizeof(T) + 2u;
-  const unsigned fixed = fine ? 256u * sizeof(T) : 0u;
+  const unsigned fixed = fine ? 256u * sizeof(T)
+                              : (L.hot >= 0 ? (unsigned)L.hot_n * (unsigned)(sizeof(T) + 4) : 0u);
   const int ftx = fine ? cfg_.tile_tx : 0, fty = fine ? cfg_.tile_ty : 0;
   std::vector<FusedTile> cands = tile_candidates(L.nx, L.ny, L.rxs, L.rys, nu + 1, L.fx, L.fy,
                                                  per_cell, fixed, kLdsCap, ftx, fty);
@@ -1430,6 +1439,8 @@ static ClassRows<T> class_rows(const LevelBuf& L, int px, int py) {
   D.ent_val = (const T*)L.ent_val;
   D.cdinv = (const T*)L.cdinv;
   D.cactive = L.cactive;
+  D.hot = L.hot;
+  D.hot_n = L.hot_n;
   return D;
 }
 

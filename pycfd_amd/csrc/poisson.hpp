@@ -139,6 +139,7 @@ struct LevelBuf {
   FusedTile down0, down1, up;
   int* ent_off = nullptr;
   std::vector<int8_t> hdi, hdj;  // host copy of the entry offsets (tile planning)
+  int hot = -1, hot_n = 0;       // most frequent class and its entry count
 };
 
 class PoissonEngine {

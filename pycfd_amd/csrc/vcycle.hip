@@ -411,17 +411,53 @@ __device__ __forceinline__ T class_row(const ClassRows<T>& L, int c, const T* X,
   return acc;
 }
 
+// LDS of a stored-level tile: the hot row's coefficients and offsets, three
+// vectors and the row classes.
 template <typename T>
 struct ClassLds {
-  T *P, *Q, *S;
+  T *HV, *P, *Q, *S;
+  int* HO;
   uint16_t* C;
-  __device__ ClassLds(unsigned char* smem, int rn) {
-    P = (T*)smem;
+  __device__ ClassLds(unsigned char* smem, int rn, int hot_n) {
+    HV = (T*)smem;
+    P = HV + hot_n;
     Q = P + rn;
     S = Q + rn;
-    C = (uint16_t*)(S + rn);
+    HO = (int*)(S + rn);
+    C = (uint16_t*)(HO + hot_n);
   }
 };
+
+// Row c at region cell idx, the hot row from LDS (the common case: all lanes
+// of an interior wavefront share it), any other row from the global dictionary.
+template <typename T>
+__device__ __forceinline__ T class_row_lds(const ClassRows<T>& L, const ClassLds<T>& M, int c,
+                                           const T* X, int idx) {
+  if (c != L.hot) return class_row(L, c, X, idx);
+  T acc = (T)0;
+  const int n = L.hot_n;
+  int e = 0;
+  for (; e + 3 < n; e += 4) {
+    const T x0 = X[idx + M.HO[e]], x1 = X[idx + M.HO[e + 1]], x2 = X[idx + M.HO[e + 2]],
+            x3 = X[idx + M.HO[e + 3]];
+    acc += M.HV[e] * x0;
+    acc += M.HV[e + 1] * x1;
+    acc += M.HV[e + 2] * x2;
+    acc += M.HV[e + 3] * x3;
+  }
+  for (; e < n; ++e) acc += M.HV[e] * X[idx + M.HO[e]];
+  return acc;
+}
+
+template <typename T>
+__device__ __forceinline__ void stage_hot_row(const ClassRows<T>& L, const ClassLds<T>& M) {
+  if (L.hot < 0) return;
+  const int e0 = L.row_ptr[L.hot];
+  for (int e = threadIdx.x; e < L.hot_n; e += kBlock) {
+    M.HV[e] = L.ent_val[e0 + e];
+    M.HO[e] = L.ent_off[e0 + e];
+  }
+}
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
@@ -431,12 +467,13 @@ k_class_down(ClassRows<T> L, FusedTile ft, int rxs, int rys, T w, int nu, const 
   if (st->done) return;
   extern __shared__ __align__(16) unsigned char smem[];
   const int ry = ft.ry;
-  ClassLds<T> M(smem, ft.rx * ry);
+  ClassLds<T> M(smem, ft.rx * ry, L.hot < 0 ? 0 : L.hot_n);
   T* B = M.P;
   T* X = M.Q;
   T* Y = M.S;
   uint16_t* C = M.C;
   const TileOrigin o = tile_origin(ft);
+  stage_hot_row(L, M);
 
   // Load b and the row classes, and start from zero: x = w D^-1 b.
   for_ring(Ring{0, ft.rx, 0, ry}, [&](int a, int c) {
@@ -463,7 +500,7 @@ k_class_down(ClassRows<T> L, FusedTile ft, int rxs, int rys, T w, int nu, const 
         Y[idx] = (T)0;
         return;
       }
-      Y[idx] = X[idx] + w * L.cdinv[cl] * (B[idx] - class_row(L, cl, X, idx));
+      Y[idx] = X[idx] + w * L.cdinv[cl] * (B[idx] - class_row_lds(L, M, cl, X, idx));
     });
     __syncthreads();
     T* t = X;
@@ -474,7 +511,7 @@ k_class_down(ClassRows<T> L, FusedTile ft, int rxs, int rys, T w, int nu, const 
   for_ring(ring(ft, 1, rxs, rys), [&](int a, int c) {
     const int idx = a * ry + c;
     const uint16_t cl = C[idx];
-    Y[idx] = (cl == kOutside) ? (T)0 : B[idx] - class_row(L, cl, X, idx);
+    Y[idx] = (cl == kOutside) ? (T)0 : B[idx] - class_row_lds(L, M, cl, X, idx);
   });
   __syncthreads();
   for_ring(ring(ft, 1, rxs, rys), [&](int a, int c) {
@@ -502,7 +539,7 @@ k_class_down(ClassRows<T> L, FusedTile ft, int rxs, int rys, T w, int nu, const 
         if (j >= L.ny) break;
         const int idx = (ft.hx + ca * fx + a2) * ry + (ft.hy + cc * fy + c2);
         const uint16_t cl = C[idx];
-        acc += L.cactive[cl] ? Y[idx] - w * class_row(L, cl, X, idx) : (T)0;
+        acc += L.cactive[cl] ? Y[idx] - w * class_row_lds(L, M, cl, X, idx) : (T)0;
       }
     }
     bc[I * cny + J] = acc;
@@ -517,12 +554,13 @@ k_class_up(ClassRows<T> L, FusedTile ft, int rxs, int rys, T w, int nu, const T*
   if (st->done) return;
   extern __shared__ __align__(16) unsigned char smem[];
   const int ry = ft.ry;
-  ClassLds<T> M(smem, ft.rx * ry);
+  ClassLds<T> M(smem, ft.rx * ry, L.hot < 0 ? 0 : L.hot_n);
   T* E = M.P;
   T* X = M.Q;
   T* B = M.S;
   uint16_t* C = M.C;
   const TileOrigin o = tile_origin(ft);
+  stage_hot_row(L, M);
 
   for_ring(Ring{0, ft.rx, 0, ry}, [&](int a, int c) {
     int i, j;
@@ -546,7 +584,7 @@ k_class_up(ClassRows<T> L, FusedTile ft, int rxs, int rys, T w, int nu, const T*
     const int idx = a * ry + c;
     const uint16_t cl = C[idx];
     if (cl == kOutside) return;
-    X[idx] = X[idx] + E[idx] - w * L.cdinv[cl] * class_row(L, cl, E, idx);
+    X[idx] = X[idx] + E[idx] - w * L.cdinv[cl] * class_row_lds(L, M, cl, E, idx);
   });
   __syncthreads();
   T* Y = E;
@@ -556,7 +594,7 @@ k_class_up(ClassRows<T> L, FusedTile ft, int rxs, int rys, T w, int nu, const T*
       const int idx = a * ry + c;
       const uint16_t cl = C[idx];
       T val = (T)0;
-      if (cl != kOutside) val = X[idx] + w * L.cdinv[cl] * (B[idx] - class_row(L, cl, X, idx));
+      if (cl != kOutside) val = X[idx] + w * L.cdinv[cl] * (B[idx] - class_row_lds(L, M, cl, X, idx));
       if (!last) {
         Y[idx] = val;
         return;

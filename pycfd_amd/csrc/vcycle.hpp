@@ -30,7 +30,12 @@ struct ClassRows {
   const T* ent_val;
   const T* cdinv;
   const uint8_t* cactive;
+  int hot;    // the level's most frequent class (interior rows), staged in LDS; -1: none
+  int hot_n;  // its entry count
 };
+
+// Largest row the fused class kernels stage in LDS as the level's hot class.
+constexpr int kHotMax = 256;
 
 // Level-0 (matrix-free 5-point) down leg.  Right-hand side either from the
 // Krylov residual r (b0 = r - mean_r on solvable cells, written to b_out) or

@@ -45,6 +45,27 @@ class Simulation:
         self.result: SimulationResult | None = None
         log.info("simulation ready\n%s", config.summary())
 
+    @classmethod
+    def from_checkpoint(cls, path,
+                        obstacle: Obstacle | ObstacleGroup | Sequence[Obstacle] | None = None
+                        ) -> "Simulation":
+        """Rebuild a simulation from a checkpoint and restore its state
+        (incompressible.py:74-84); a stored warm-start pressure is restored too."""
+        from ..analysis.export import load_checkpoint_with_guess
+        fields, config, guess = load_checkpoint_with_guess(path)
+        sim = cls(config, obstacle)
+        sim.fields = fields
+        if guess is not None:
+            sim.solver.set_pressure_guess(guess)
+        log.info("resumed from %s at t=%.6g (step %d)", path, fields.t, fields.step)
+        return sim
+
+    def save_checkpoint(self, path):
+        """Checkpoint the current state, the configuration and the warm-start
+        pressure (what the reference's CLI writes with ``--checkpoint``)."""
+        from ..analysis.export import save_checkpoint
+        return save_checkpoint(self.fields, self.config, path, solver=self.solver)
+
     def add_probe(self, x: float, y: float, name: str = "probe") -> Probe:
         probe = Probe(x, y, name)
         self.probes.append(probe)

@@ -264,3 +264,27 @@ def test_oracle_parity_mgcg(gpu_available, precision, coarse_max):
     sim.run(max_steps=6)
     assert np.abs(sim.fields.u - u).max() <= 1e-9 * np.abs(u).max()
     assert np.abs(sim.fields.v - v).max() <= 1e-9 * np.abs(u).max()
+
+
+# ------------------------------------------------------------ checkpoints
+@pytest.mark.parametrize("ps", ["mgcg", "sor"])
+def test_resume_from_checkpoint_is_bitwise(gpu_available, tmp_path, ps):
+    """Six steps straight through == three steps, checkpoint, resume, three
+    steps: bit for bit, because the warm-start pressure travels with the state
+    (the reference's resume restarts its iterative solvers cold)."""
+    from pycfd_amd.physics.incompressible import Simulation
+    kw = dict(pressure_solver=ps, poisson_tol=1e-10, mg_coarse_max=96)
+    case = oracle_case("cyl_outlet")
+    u0, v0 = initial("cyl_outlet", case)
+    a = _sim("cyl_outlet", u0, v0, **kw)
+    a.run(max_steps=6)
+    b = _sim("cyl_outlet", u0, v0, **kw)
+    b.run(max_steps=3)
+    path = b.save_checkpoint(tmp_path / "state")
+    c = Simulation.from_checkpoint(path, obstacle=b.obstacle)
+    assert c.fields.step == 3 and c.fields.t == b.fields.t
+    c.run(max_steps=3)
+    assert c.fields.step == 6
+    assert np.array_equal(c.fields.u, a.fields.u)
+    assert np.array_equal(c.fields.v, a.fields.v)
+    assert np.array_equal(c.fields.p, a.fields.p)

@@ -148,3 +148,51 @@ def test_sor_rejects_odd_periodic_colouring():
     S = assemble_poisson_matrix(mesh, True, False)
     with pytest.raises(ValueError, match="red/black"):
         SORSolver(S)
+
+
+# ---------------------------------------------------------------- checkpoints
+def test_checkpoint_round_trip_is_exact(tmp_path):
+    """test_validation.py:321-336 on host arrays: fields, clock and config come
+    back exactly; the archive keeps the reference's keys."""
+    from pycfd_amd.analysis.export import (checkpoint_provenance, load_checkpoint,
+                                           load_checkpoint_with_guess, save_checkpoint)
+    from pycfd_amd.config import BCKind, BCSpec, SimulationConfig
+    from pycfd_amd.core.fields import FlowField
+    from pycfd_amd.core.mesh import StructuredMesh
+    bcs = {w: BCSpec(BCKind.NO_SLIP) for w in ("left", "right", "bottom")}
+    bcs["top"] = BCSpec(BCKind.MOVING_WALL, velocity=1.0)
+    cfg = SimulationConfig(nx=12, ny=10, re=250, dt=3e-3, boundary_config=bcs)
+    mesh = StructuredMesh.from_config(cfg)
+    rng = np.random.default_rng(0)
+    fields = FlowField(mesh, rng.standard_normal(mesh.u_shape), rng.standard_normal(mesh.v_shape),
+                       rng.standard_normal(mesh.p_shape), 0.123, 7)
+    path = save_checkpoint(fields, cfg, tmp_path / "ck")
+    assert path.suffix == ".npz"
+    with np.load(path, allow_pickle=False) as npz:
+        assert {"u", "v", "p", "t", "step", "config", "provenance"} <= set(npz.files)
+        assert "p_guess" not in npz.files
+    restored, restored_cfg = load_checkpoint(path)
+    assert np.array_equal(restored.u, fields.u)
+    assert np.array_equal(restored.v, fields.v)
+    assert np.array_equal(restored.p, fields.p)
+    assert restored.t == fields.t and restored.step == fields.step
+    assert restored_cfg == cfg
+    assert restored_cfg.boundary_config["top"].velocity == 1.0
+    assert checkpoint_provenance(path)["generator"] == "pycfd_amd"
+    assert load_checkpoint_with_guess(path)[2] is None
+
+
+def test_checkpoint_rejects_mismatched_shapes_and_missing_files(tmp_path):
+    from pycfd_amd.analysis.export import load_checkpoint, save_checkpoint
+    from pycfd_amd.config import SimulationConfig
+    from pycfd_amd.core.fields import FlowField
+    from pycfd_amd.core.mesh import StructuredMesh
+    with pytest.raises(FileNotFoundError):
+        load_checkpoint(tmp_path / "nope.npz")
+    cfg = SimulationConfig(nx=8, ny=6)
+    mesh = StructuredMesh.from_config(cfg)
+    fields = FlowField(mesh, np.zeros(mesh.u_shape), np.zeros(mesh.v_shape),
+                       np.zeros(mesh.p_shape))
+    path = save_checkpoint(fields, cfg.replace(nx=9), tmp_path / "bad.npz")
+    with pytest.raises(ValueError, match="does not match"):
+        load_checkpoint(path)
