@@ -84,6 +84,8 @@ typedef struct pcfd_case_desc {
   const uint8_t* solid;         /* (nx*ny) 1 inside a body, or NULL */
   const int8_t* body_id;        /* (nx*ny) body of each solid cell, or NULL */
   int32_t n_bodies;             /* > 1 enables per-body forces (solver.py:136-146) */
+  int32_t use_les;              /* Smagorinsky closure (config.py use_les, solver.py:164-167) */
+  double les_coeff;             /* (Cs * sqrt(dx dy))^2 (turbulence.py:45-52) */
 } pcfd_case_desc;
 
 typedef struct pcfd_solver_desc {
@@ -158,6 +160,16 @@ int pcfd_set_pressure_solver(pcfd_ctx* ctx, const pcfd_solver_desc* desc,
  * rhs_v (nx, ny+1), contiguous. */
 int pcfd_momentum_rhs(pcfd_ctx* ctx, const double* u, const double* v, double blend,
                       double* rhs_u, double* rhs_v);
+/* SmagorinskyModel.eddy_viscosity (turbulence.py:101-113) of a context
+ * created with use_les: nu_c (nx+2, ny+2) and, unless NULL, nu_corner
+ * (nx+1, ny+1). */
+int pcfd_eddy_viscosity(pcfd_ctx* ctx, const double* u, const double* v, double* nu_c,
+                        double* nu_corner);
+/* The momentum RHS with the stress-divergence viscous term of a given
+ * cell-centred viscosity nu_c (nx+2, ny+2) (ProjectionSolver._diffusion_variable,
+ * solver.py:467-499, inside momentum_rhs, solver.py:523-529). */
+int pcfd_momentum_rhs_variable(pcfd_ctx* ctx, const double* u, const double* v, double blend,
+                               const double* nu_c, double* rhs_u, double* rhs_v);
 /* ProjectionSolver.divergence (solver.py:532-543): out (nx, ny). */
 int pcfd_divergence(pcfd_ctx* ctx, const double* u, const double* v, double* out);
 /* BoundaryManager.apply_velocity (boundary.py:375-379). */

@@ -64,6 +64,8 @@ class Case:
     cfl_max: float = 0.5
     advection: str = "central"
     upwind_blend: float | None = None
+    use_les: bool = False
+    smagorinsky_cs: float = 0.17
 
     def __post_init__(self):
         self.dx = float(np.full(self.nx, self.lx / self.nx)[0])
@@ -132,6 +134,11 @@ def momentum_rhs(case: Case, u, v, blend: float = 0.0):
         vl, vr = v[0:nx, 1:ny + 2], v[2:nx + 2, 1:ny + 2]
         a_vu += blend / dx * (np.abs(uc[1:nx + 1, 0:ny + 1]) * (v0 - vr) / 2.0
                               - np.abs(uc[0:nx, 0:ny + 1]) * (vl - v0) / 2.0)
+    fx, fy = case.body_force
+    if case.use_les:  # solver.py:523-528
+        nu_c, nu_k = eddy_viscosity(case, u, v)
+        d_u, d_v = diffusion_variable(case, u, v, nu_c, nu_k)
+        return -(a_uu + a_uv) + d_u + fx, -(a_vu + a_vv) + d_v + fy
     dx2, dy2 = dx ** 2, dy ** 2
     u0 = u[1:nx + 2, 1:ny + 1]
     lap_u = ((u[0:nx + 1, 1:ny + 1] - 2.0 * u0 + u[2:nx + 3, 1:ny + 1]) / dx2
@@ -139,9 +146,50 @@ def momentum_rhs(case: Case, u, v, blend: float = 0.0):
     v0 = v[1:nx + 1, 1:ny + 2]
     lap_v = ((v[0:nx, 1:ny + 2] - 2.0 * v0 + v[2:nx + 2, 1:ny + 2]) / dx2
              + (v[1:nx + 1, 0:ny + 1] - 2.0 * v0 + v[1:nx + 1, 2:ny + 3]) / dy2)
-    fx, fy = case.body_force
     return (-(a_uu + a_uv) + case.nu * lap_u + fx,
             -(a_vu + a_vv) + case.nu * lap_v + fy)
+
+
+def eddy_viscosity(case: Case, u, v):
+    """Smagorinsky nu + (Cs Delta)^2 |S| at cell centres (ghosted) and its
+    four-cell average at corners (turbulence.py:63-113, uniform mesh:
+    Delta = sqrt(dx dy)); |S| = sqrt(2 S11^2 + 2 S22^2 + 4 S12^2) with S12
+    averaged from the corners and copied into the ghost ring."""
+    nx, ny, dx, dy = case.nx, case.ny, case.dx, case.dy
+    s11 = (u[1:nx + 3, :] - u[0:nx + 2, :]) / dx
+    s22 = (v[:, 1:ny + 3] - v[:, 0:ny + 2]) / dy
+    dudy = (u[1:nx + 2, 1:ny + 2] - u[1:nx + 2, 0:ny + 1]) / dy
+    dvdx = (v[1:nx + 2, 1:ny + 2] - v[0:nx + 1, 1:ny + 2]) / dx
+    s12 = 0.5 * (dudy + dvdx)
+    s12c = np.zeros_like(s11)
+    s12c[1:nx + 1, 1:ny + 1] = 0.25 * (s12[0:nx, 0:ny] + s12[1:nx + 1, 0:ny]
+                                       + s12[0:nx, 1:ny + 1] + s12[1:nx + 1, 1:ny + 1])
+    s12c[0, :], s12c[nx + 1, :] = s12c[1, :], s12c[nx, :]
+    s12c[:, 0], s12c[:, ny + 1] = s12c[:, 1], s12c[:, ny]
+    s_mag = np.sqrt(2.0 * s11 ** 2 + 2.0 * s22 ** 2 + 4.0 * s12c ** 2)
+    coeff = (case.smagorinsky_cs * float(np.sqrt(dx * dy))) ** 2
+    nu_c = case.nu + coeff * s_mag
+    nu_k = 0.25 * (nu_c[0:nx + 1, 0:ny + 1] + nu_c[1:nx + 2, 0:ny + 1]
+                   + nu_c[0:nx + 1, 1:ny + 2] + nu_c[1:nx + 2, 1:ny + 2])
+    return nu_c, nu_k
+
+
+def diffusion_variable(case: Case, u, v, nu_c, nu_k):
+    """Stress-divergence viscous terms for a variable viscosity (solver.py:467-499):
+    corner shear stress nu_k (du/dy + dv/dx), centre normal stresses 2 nu_c du/dx."""
+    nx, ny, dx, dy = case.nx, case.ny, case.dx, case.dy
+    dudy = (u[1:nx + 2, 1:ny + 2] - u[1:nx + 2, 0:ny + 1]) / dy
+    dvdx = (v[1:nx + 2, 1:ny + 2] - v[0:nx + 1, 1:ny + 2]) / dx
+    tau = nu_k * (dudy + dvdx)
+    dudx = (u[1:nx + 3, :] - u[0:nx + 2, :]) / dx
+    dvdy = (v[:, 1:ny + 3] - v[:, 0:ny + 2]) / dy
+    d_u = (2.0 * (nu_c[1:nx + 2, 1:ny + 1] * dudx[1:nx + 2, 1:ny + 1]
+                  - nu_c[0:nx + 1, 1:ny + 1] * dudx[0:nx + 1, 1:ny + 1]) / dx
+           + (tau[0:nx + 1, 1:ny + 1] - tau[0:nx + 1, 0:ny]) / dy)
+    d_v = ((tau[1:nx + 1, 0:ny + 1] - tau[0:nx, 0:ny + 1]) / dx
+           + 2.0 * (nu_c[1:nx + 1, 1:ny + 2] * dvdy[1:nx + 1, 1:ny + 2]
+                    - nu_c[1:nx + 1, 0:ny + 1] * dvdy[1:nx + 1, 0:ny + 1]) / dy)
+    return d_u, d_v
 
 
 def divergence(case: Case, u, v):

@@ -51,6 +51,8 @@ class CaseDesc(C.Structure):
         ("solid", _u8p),
         ("body_id", _i8p),
         ("n_bodies", C.c_int32),
+        ("use_les", C.c_int32),
+        ("les_coeff", C.c_double),
     ]
 
 
@@ -112,6 +114,9 @@ _SIGNATURES = {
     "pcfd_time_kernel": [C.c_void_p, C.c_int32, C.c_int32, _dp, _dp],
     "pcfd_mg_plan": [C.c_void_p, C.POINTER(C.c_int32), C.c_int32],
     "pcfd_pressure_guess": [C.c_void_p, C.c_void_p, C.c_int32],
+    "pcfd_eddy_viscosity": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+    "pcfd_momentum_rhs_variable": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,
+                                   C.c_void_p, C.c_void_p],
 }
 
 EXPORTED = tuple(_SIGNATURES)

@@ -58,6 +58,23 @@ _GHIA = {
 GHIA_REYNOLDS = tuple(sorted(_GHIA))
 
 
+def poiseuille_profile(y, height: float, u_max: float) -> np.ndarray:
+    """Plane Poiseuille profile ``u_max 4 y (h - y) / h^2`` (validation.py:26-31)."""
+    y = np.asarray(y, dtype=float)
+    return u_max * 4.0 * y * (height - y) / height ** 2
+
+
+def convergence_orders(resolutions, errors) -> list[float]:
+    """``log(e1 / e2) / log(N2 / N1)`` per refinement pair (validation.py:223-244)."""
+    out = []
+    for k in range(len(resolutions) - 1):
+        e1, e2 = float(errors[k]), float(errors[k + 1])
+        n1, n2 = int(resolutions[k]), int(resolutions[k + 1])
+        out.append(float(np.log(e1 / e2) / np.log(n2 / n1)) if e1 > 0 and e2 > 0
+                   else float("nan"))
+    return out
+
+
 def ghia_reference(re: int) -> dict[str, np.ndarray]:
     re = int(re)
     if re not in _GHIA:

@@ -32,7 +32,8 @@ class DeviceContext:
                  body_force=(0.0, 0.0), scheme: str = "rk3",
                  bc_kinds=("no_slip",) * 4, bc_velocity=(0.0,) * 4, bc_p_ref=(0.0,) * 4,
                  inlet_lines=(None,) * 4, solid: np.ndarray | None = None,
-                 body_id: np.ndarray | None = None, n_bodies: int = 0) -> None:
+                 body_id: np.ndarray | None = None, n_bodies: int = 0,
+                 les_coeff: float | None = None) -> None:
         require_device()
         import torch
         self._lib = _lib.load()
@@ -62,6 +63,8 @@ class DeviceContext:
             d.body_id = barr.ctypes.data_as(C.POINTER(C.c_int8))
         d.n_bodies = int(n_bodies)
         self.n_bodies = int(n_bodies) if n_bodies > 1 else 0
+        d.use_les = int(les_coeff is not None)
+        d.les_coeff = float(les_coeff or 0.0)
         self.stream = torch.cuda.current_stream()
         handle = C.c_void_p(0)
         _lib.check(self._lib.pcfd_ctx_create(C.byref(d), C.c_void_p(self.stream.cuda_stream),
@@ -129,6 +132,14 @@ class DeviceContext:
     def momentum_rhs(self, u, v, blend: float, rhs_u, rhs_v) -> None:
         self._call("pcfd_momentum_rhs", _ptr(u), _ptr(v), float(blend), _ptr(rhs_u),
                    _ptr(rhs_v))
+
+    def eddy_viscosity(self, u, v, nu_c, nu_corner=None) -> None:
+        self._call("pcfd_eddy_viscosity", _ptr(u), _ptr(v), _ptr(nu_c),
+                   _ptr(nu_corner) if nu_corner is not None else None)
+
+    def momentum_rhs_variable(self, u, v, blend: float, nu_c, rhs_u, rhs_v) -> None:
+        self._call("pcfd_momentum_rhs_variable", _ptr(u), _ptr(v), float(blend), _ptr(nu_c),
+                   _ptr(rhs_u), _ptr(rhs_v))
 
     def divergence(self, u, v, out) -> None:
         self._call("pcfd_divergence", _ptr(u), _ptr(v), _ptr(out))

@@ -56,9 +56,6 @@ class ProjectionSolver:
             raise NonUniformMeshError(
                 "the GPU projection step is discretised for uniform spacing; "
                 "stretched meshes are not supported (stretch_x = stretch_y = 1.0)")
-        if config.use_les:
-            raise NotImplementedError("the Smagorinsky closure (use_les=True) is not "
-                                      "available on the GPU path")
         self.nu = config.nu
         self.boundaries = BoundaryManager(config.boundary_config, self.mesh)
         self.periodic_x = self.boundaries.periodic_x
@@ -85,7 +82,10 @@ class ProjectionSolver:
             self.solid if self.has_obstacle else None,
             dirichlet=self.dirichlet_pressure or None)
         self.turbulence = None
-        self._use_kernel = True
+        if config.use_les:  # solver.py:164-167
+            from ..physics.turbulence import SmagorinskyModel
+            self.turbulence = SmagorinskyModel(self.mesh, config.smagorinsky_cs)
+        self._use_kernel = self.turbulence is None
         self._blend = 0.0
 
         walls = ("left", "right", "bottom", "top")
@@ -102,7 +102,10 @@ class ProjectionSolver:
             [conds[w].spec.velocity for w in walls],
             [conds[w].spec.p_ref for w in walls],
             [conds[w].inlet_line(self.mesh) for w in walls],
-            self.solid if self.has_obstacle else None, body_id, len(self.body_masks))
+            self.solid if self.has_obstacle else None, body_id, len(self.body_masks),
+            les_coeff=self.turbulence.coeff if self.turbulence is not None else None)
+        if self.turbulence is not None:
+            self.turbulence.bind(self.ctx, self.nu)
         self.pressure_solver = make_pressure_solver(
             config.pressure_solver, self.poisson, tol=config.poisson_tol,
             maxiter=config.poisson_maxiter, sor_omega=config.sor_omega,
@@ -191,6 +194,17 @@ class ProjectionSolver:
         ru = torch.empty((nx + 1, ny), dtype=torch.float64, device="cuda")
         rv = torch.empty((nx, ny + 1), dtype=torch.float64, device="cuda")
         self.ctx.momentum_rhs(ud, vd, blend, ru, rv)
+        return (to_host(ru), to_host(rv)) if host else (ru, rv)
+
+    def momentum_rhs_variable(self, u, v, nu_c, blend: float = 0.0):
+        """The RHS with the stress-divergence viscous term of a given
+        cell-centred viscosity (solver.py:467-499 inside momentum_rhs)."""
+        import torch
+        nx, ny = self.mesh.shape
+        host = not isinstance(u, torch.Tensor)
+        ru = torch.empty((nx + 1, ny), dtype=torch.float64, device="cuda")
+        rv = torch.empty((nx, ny + 1), dtype=torch.float64, device="cuda")
+        self.ctx.momentum_rhs_variable(to_device(u), to_device(v), blend, to_device(nu_c), ru, rv)
         return (to_host(ru), to_host(rv)) if host else (ru, rv)
 
     def divergence(self, u, v):

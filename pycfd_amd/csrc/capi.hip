@@ -48,6 +48,9 @@ struct pcfd_ctx {
   int8_t* uf_body = nullptr;
   int8_t* vf_body = nullptr;
   double* force = nullptr;
+  double* nuc = nullptr;  // Smagorinsky nu + nu_t, ghosted cell layout (LES only)
+  LesParams les{};
+  bool use_les = false;
   uint8_t* flags = nullptr;
   uint8_t* solid = nullptr;
   double* stage_a[2] = {nullptr, nullptr};
@@ -71,6 +74,7 @@ struct pcfd_ctx {
     for (auto* p : stage_b) hipFree(p);
     hipFree(diag_out);
     hipFree(guess_backup);
+    hipFree(nuc);
     if (host_out) hipHostFree(host_out);
   }
 };
@@ -124,6 +128,9 @@ int substep(pcfd_ctx* c, const double* u, const double* v, const double* ue, con
             double ca, double cb, double dt, double* us, double* vs, double* p, SolveInfo* info,
             bool sync) {
   int rc;
+  if (c->use_les &&
+      (rc = launch_eddy_viscosity(c->stream, c->les, u, ue, v, ve, ca, cb, c->nuc, nullptr)))
+    return rc;
   if (ue) {
     rc = launch_predictor_comb(c->stream, u, ue, v, ve, ca, cb, c->mom, c->upd, dt, us, vs);
   } else {
@@ -380,6 +387,19 @@ int pcfd_ctx_create(const pcfd_case_desc* d, void* hip_stream, pcfd_ctx** out) {
   M.fx = d->body_fx;
   M.fy = d->body_fy;
   M.blend = 0.0;
+  M.nuc = nullptr;
+  if (d->use_les) {
+    if (!(d->les_coeff >= 0.0)) {
+      set_last_error("les_coeff must be >= 0, got %g", d->les_coeff);
+      return kErrBadArg;
+    }
+    c->use_les = true;
+    c->les = LesParams{nx, ny, c->ldu, c->ldv, d->dx, d->dy, d->nu, d->les_coeff};
+    PCFD_HIP_CHECK(hipMalloc(&c->nuc, (size_t)(nx + 2) * (ny + 2) * sizeof(double)));
+    PCFD_HIP_CHECK(hipMemsetAsync(c->nuc, 0, (size_t)(nx + 2) * (ny + 2) * sizeof(double),
+                                  c->stream));
+    M.nuc = c->nuc;
+  }
 
   // Poisson operator flags (pressure.py:209-251)
   bool dir[4];
@@ -516,6 +536,39 @@ int pcfd_momentum_rhs(pcfd_ctx* c, const double* u, const double* v, double blen
   if (int rc = check_ctx(c)) return rc;
   MomParams P = c->mom;
   P.blend = blend;
+  if (c->use_les) {
+    if (int rc = launch_eddy_viscosity(c->stream, c->les, u, nullptr, v, nullptr, 1.0, 0.0, c->nuc,
+                                       nullptr))
+      return rc;
+  }
+  return launch_momentum_rhs(c->stream, u, v, P, rhs_u, rhs_v);
+}
+
+int pcfd_eddy_viscosity(pcfd_ctx* c, const double* u, const double* v, double* nu_c,
+                        double* nu_corner) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!c->use_les) {
+    set_last_error("pcfd_eddy_viscosity: the context was created without use_les");
+    return kErrBadArg;
+  }
+  if (!nu_c) {
+    set_last_error("pcfd_eddy_viscosity: null nu_c");
+    return kErrBadArg;
+  }
+  return launch_eddy_viscosity(c->stream, c->les, u, nullptr, v, nullptr, 1.0, 0.0, nu_c,
+                               nu_corner);
+}
+
+int pcfd_momentum_rhs_variable(pcfd_ctx* c, const double* u, const double* v, double blend,
+                               const double* nu_c, double* rhs_u, double* rhs_v) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!nu_c) {
+    set_last_error("pcfd_momentum_rhs_variable: null nu_c");
+    return kErrBadArg;
+  }
+  MomParams P = c->mom;
+  P.blend = blend;
+  P.nuc = nu_c;
   return launch_momentum_rhs(c->stream, u, v, P, rhs_u, rhs_v);
 }
 

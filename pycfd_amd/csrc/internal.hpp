@@ -54,6 +54,15 @@ struct BCParams {
 struct MomParams {
   int nx, ny, ldu, ldv;
   double dx, dy, dx2, dy2, nu, fx, fy, blend;
+  // Variable viscosity (Smagorinsky LES): cell-centred nu + nu_t on the
+  // ghosted (nx+2, ny+2) layout, row pitch ny + 2; nullptr: constant nu.
+  const double* nuc;
+};
+
+// Smagorinsky constant term (Cs * sqrt(dx dy))^2 of the eddy viscosity.
+struct LesParams {
+  int nx, ny, ldu, ldv;
+  double dx, dy, nu, coeff;
 };
 
 // Solvable face ranges u_upd / v_upd (solver.py:235-263): u faces i in
@@ -63,6 +72,12 @@ struct UpdRange {
 };
 
 // stencil.hip
+// nu_c = nu + coeff |S| on the ghosted cell layout (turbulence.py:63-113),
+// from (u, v) or from the stage combination ca*(u0, v0) + cb*(ue, ve) when ue
+// is given; optionally the corner average nu_corner (nx+1, ny+1).
+int launch_eddy_viscosity(hipStream_t s, const LesParams& P, const double* u0, const double* ue,
+                          const double* v0, const double* ve, double ca, double cb, double* nuc,
+                          double* nu_corner);
 int launch_momentum_rhs(hipStream_t s, const double* u, const double* v, const MomParams& P,
                         double* rhs_u, double* rhs_v);
 int launch_predictor(hipStream_t s, const double* u, const double* v, const MomParams& P,

@@ -10,6 +10,52 @@
 namespace pcfd {
 
 // ----------------------------------------------------------------------------
+// Variable-viscosity (stress-divergence) viscous terms, solver.py:467-499 on a
+// uniform mesh.  nu_c is cell-centred on the ghosted layout; the corner value
+// is the four-cell average of turbulence.py:107-110; tau = nu_k (du/dy +
+// dv/dx) at corners; the normal stresses 2 nu_c du/dx, 2 nu_c dv/dy at centres.
+// Index conventions below are the ghosted arrays': u face m, v face n, cell
+// (i, j) of nu_c, corner (a, b) between cells a, a+1 and b, b+1.
+// ----------------------------------------------------------------------------
+template <class FU, class FV>
+__device__ __forceinline__ double tau_corner(const FU& U, const FV& V, int a, int b,
+                                             const MomParams& P) {
+  const double* nc = P.nuc;
+  const int ld = P.ny + 2;
+  const double nuk = 0.25 * (nc[(size_t)a * ld + b] + nc[(size_t)(a + 1) * ld + b] +
+                             nc[(size_t)a * ld + b + 1] + nc[(size_t)(a + 1) * ld + b + 1]);
+  const double dudy = (U(a + 1, b + 1) - U(a + 1, b)) / P.dy;
+  const double dvdx = (V(a + 1, b + 1) - V(a, b + 1)) / P.dx;
+  return nuk * (dudy + dvdx);
+}
+
+template <class FU, class FV>
+__device__ __forceinline__ double visc_u_face(const FU& U, const FV& V, int m, int j,
+                                              const MomParams& P) {
+  const double* nc = P.nuc;
+  const int ld = P.ny + 2;
+  const double gx_hi = (U(m + 1, j) - U(m, j)) / P.dx;   // du/dx in cell m
+  const double gx_lo = (U(m, j) - U(m - 1, j)) / P.dx;   // du/dx in cell m-1
+  const double sxx = 2.0 * (nc[(size_t)m * ld + j] * gx_hi - nc[(size_t)(m - 1) * ld + j] * gx_lo) /
+                     P.dx;
+  const double sxy = (tau_corner(U, V, m - 1, j, P) - tau_corner(U, V, m - 1, j - 1, P)) / P.dy;
+  return sxx + sxy;
+}
+
+template <class FU, class FV>
+__device__ __forceinline__ double visc_v_face(const FU& U, const FV& V, int i, int n,
+                                              const MomParams& P) {
+  const double* nc = P.nuc;
+  const int ld = P.ny + 2;
+  const double syx = (tau_corner(U, V, i, n - 1, P) - tau_corner(U, V, i - 1, n - 1, P)) / P.dx;
+  const double gy_hi = (V(i, n + 1) - V(i, n)) / P.dy;
+  const double gy_lo = (V(i, n) - V(i, n - 1)) / P.dy;
+  const double syy = 2.0 * (nc[(size_t)i * ld + n] * gy_hi - nc[(size_t)i * ld + n - 1] * gy_lo) /
+                     P.dy;
+  return syx + syy;
+}
+
+// ----------------------------------------------------------------------------
 // Momentum right-hand side, -A(u) + nu*lap(u) + f, evaluated at one face.
 // `U(i, j)` / `V(i, j)` return the ghosted velocity arrays' entries; they are
 // functors so a Runge-Kutta stage combination can be folded into the loads.
@@ -34,6 +80,7 @@ __device__ __forceinline__ double rhs_u_face(const FU& U, const FV& V, int m, in
     duudx += P.blend / P.dx * (fabs(uu_r) * (u0 - up) / 2.0 - fabs(uu_l) * (um - u0) / 2.0);
     duvdy += P.blend / P.dy * (fabs(v_top) * (u0 - uu) / 2.0 - fabs(v_bot) * (ud - u0) / 2.0);
   }
+  if (P.nuc) return -(duudx + duvdy) + visc_u_face(U, V, m, j, P) + P.fx;
   const double d2u = (um - 2.0 * u0 + up) / P.dx2 + (ud - 2.0 * u0 + uu) / P.dy2;
   return -(duudx + duvdy) + P.nu * d2u + P.fx;
 }
@@ -56,6 +103,7 @@ __device__ __forceinline__ double rhs_v_face(const FU& U, const FV& V, int i, in
     dvvdy += P.blend / P.dy * (fabs(vv_t) * (v0 - vp) / 2.0 - fabs(vv_b) * (vm - v0) / 2.0);
     duvdx += P.blend / P.dx * (fabs(u_right) * (v0 - vr) / 2.0 - fabs(u_left) * (vl - v0) / 2.0);
   }
+  if (P.nuc) return -(duvdx + dvvdy) + visc_v_face(U, V, i, n, P) + P.fy;
   const double d2v = (vl - 2.0 * v0 + vr) / P.dx2 + (vm - 2.0 * v0 + vp) / P.dy2;
   return -(duvdx + dvvdy) + P.nu * d2v + P.fy;
 }
@@ -78,6 +126,53 @@ struct CombLoad {
     return ca * a[k] + cb * b[k];
   }
 };
+
+// Smagorinsky eddy viscosity (turbulence.py:63-113, uniform mesh), one cell of
+// the ghosted (nx+2, ny+2) layout per thread.  The corner shear rate is
+// averaged to interior cells; ghost cells take the nearest interior value
+// (the reference copies rows, then columns, into the ghost ring).
+template <class FU, class FV>
+__device__ __forceinline__ double s12_corner(const FU& U, const FV& V, int a, int b,
+                                             const LesParams& P) {
+  const double dudy = (U(a + 1, b + 1) - U(a + 1, b)) / P.dy;
+  const double dvdx = (V(a + 1, b + 1) - V(a, b + 1)) / P.dx;
+  return 0.5 * (dudy + dvdx);
+}
+
+template <class FU, class FV>
+__device__ __forceinline__ void eddy_viscosity_body(const FU& U, const FV& V, const LesParams& P,
+                                                    double* __restrict__ nuc,
+                                                    double* __restrict__ nuk) {
+  const Tile2D t = tile_coords(P.ny + 2);
+  const int i = t.i, j = t.j;
+  if (i > P.nx + 1 || j > P.ny + 1) return;
+  const double s11 = (U(i + 1, j) - U(i, j)) / P.dx;
+  const double s22 = (V(i, j + 1) - V(i, j)) / P.dy;
+  const int ic = min(max(i, 1), P.nx), jc = min(max(j, 1), P.ny);
+  const double s12 = 0.25 * (s12_corner(U, V, ic - 1, jc - 1, P) + s12_corner(U, V, ic, jc - 1, P) +
+                             s12_corner(U, V, ic - 1, jc, P) + s12_corner(U, V, ic, jc, P));
+  const double s_mag = sqrt(2.0 * (s11 * s11) + 2.0 * (s22 * s22) + 4.0 * (s12 * s12));
+  nuc[(size_t)i * (P.ny + 2) + j] = P.nu + P.coeff * s_mag;
+  (void)nuk;
+}
+
+template <class FU, class FV>
+__global__ void __launch_bounds__(kBlock)
+k_eddy_viscosity(FU U, FV V, LesParams P, double* __restrict__ nuc, double* __restrict__ nuk) {
+  eddy_viscosity_body(U, V, P, nuc, nuk);
+}
+
+// Corner average of nu_c (turbulence.py:107-110), (nx+1, ny+1).
+__global__ void __launch_bounds__(kBlock)
+k_corner_viscosity(const double* __restrict__ nuc, int nx, int ny, double* __restrict__ nuk) {
+  const Tile2D t = tile_coords(ny + 1);
+  const int a = t.i, b = t.j;
+  if (a > nx || b > ny) return;
+  const int ld = ny + 2;
+  nuk[(size_t)a * (ny + 1) + b] =
+      0.25 * (nuc[(size_t)a * ld + b] + nuc[(size_t)(a + 1) * ld + b] + nuc[(size_t)a * ld + b + 1] +
+              nuc[(size_t)(a + 1) * ld + b + 1]);
+}
 
 // rhs_u (nx+1, ny) and rhs_v (nx, ny+1), contiguous (solver.py:501-529).
 __global__ void __launch_bounds__(kBlock)
@@ -232,6 +327,27 @@ struct EpiDiag {
 // Host launchers.
 // ----------------------------------------------------------------------------
 static inline dim3 grid_for(int nxr, int nyr) { return dim3(num_tiles(nxr, nyr)); }
+
+int launch_eddy_viscosity(hipStream_t s, const LesParams& P, const double* u0, const double* ue,
+                          const double* v0, const double* ve, double ca, double cb, double* nuc,
+                          double* nu_corner) {
+  const dim3 g = grid_for(P.nx + 2, P.ny + 2);
+  if (ue) {
+    hipLaunchKernelGGL((k_eddy_viscosity<CombLoad, CombLoad>), g, dim3(kBlock), 0, s,
+                       CombLoad{u0, ue, P.ldu, ca, cb}, CombLoad{v0, ve, P.ldv, ca, cb}, P, nuc,
+                       nu_corner);
+  } else {
+    hipLaunchKernelGGL((k_eddy_viscosity<PlainLoad, PlainLoad>), g, dim3(kBlock), 0, s,
+                       PlainLoad{u0, P.ldu}, PlainLoad{v0, P.ldv}, P, nuc, nu_corner);
+  }
+  PCFD_LAUNCH_CHECK();
+  if (nu_corner) {
+    hipLaunchKernelGGL(k_corner_viscosity, grid_for(P.nx + 1, P.ny + 1), dim3(kBlock), 0, s, nuc,
+                       P.nx, P.ny, nu_corner);
+    PCFD_LAUNCH_CHECK();
+  }
+  return kOk;
+}
 
 int launch_momentum_rhs(hipStream_t s, const double* u, const double* v, const MomParams& P,
                         double* rhs_u, double* rhs_v) {

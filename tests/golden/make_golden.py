@@ -194,6 +194,37 @@ def multigrid_fixture():
     np.savez_compressed(OUT / "multigrid.npz", **out)
 
 
+def les_fixture():
+    """Smagorinsky closure: eddy viscosity and the stress-form momentum RHS on
+    random fields, and a short LES trajectory (turbulence.py:63-113,
+    solver.py:467-529)."""
+    from pycfd.core.mesh import StructuredMesh
+    rng = np.random.default_rng(4321)
+    out = {}
+    for name in ("cavity", "periodic"):
+        solver = ProjectionSolver(config(name, use_les=True, smagorinsky_cs=0.17))
+        nx, ny = solver.mesh.shape
+        u = rng.standard_normal((nx + 3, ny + 2))
+        v = rng.standard_normal((nx + 2, ny + 3))
+        nu_c, nu_k = solver.turbulence.eddy_viscosity(u, v, solver.nu)
+        out[f"{name}_u"], out[f"{name}_v"] = u, v
+        out[f"{name}_nu_c"], out[f"{name}_nu_corner"] = nu_c, nu_k
+        for blend in (0.0, 0.7):
+            ru, rv = solver.momentum_rhs(u, v, blend)
+            tag = f"{name}_b{int(blend * 10)}"
+            out[f"{tag}_rhs_u"], out[f"{tag}_rhs_v"] = ru.copy(), rv.copy()
+    for name, steps in (("cavity", 10), ("cyl_outlet", 6)):
+        cfg = config(name, pressure_solver="direct", use_les=True, smagorinsky_cs=0.17)
+        mesh = StructuredMesh.from_config(cfg)
+        body = body_for(name, mesh)
+        u0, v0 = init_for(name, mesh)
+        sim = Simulation(cfg, obstacle=body, u_init=u0, v_init=v0)
+        sim.run(max_steps=steps)
+        out[f"traj_{name}_u"], out[f"traj_{name}_v"] = sim.fields.u, sim.fields.v
+        out[f"traj_{name}_p"] = sim.fields.p
+    np.savez_compressed(OUT / "les.npz", **out)
+
+
 def mask_fixture():
     from pycfd.core.mesh import StructuredMesh
     out = {}
@@ -206,8 +237,8 @@ def mask_fixture():
 
 
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["rhs", "boundary", "trajectory", "multigrid", "mask"]
+    which = sys.argv[1:] or ["rhs", "boundary", "trajectory", "multigrid", "mask", "les"]
     for w in which:
         print("writing", w, flush=True)
         {"rhs": rhs_fixture, "boundary": boundary_fixture, "trajectory": trajectory_fixture,
-         "multigrid": multigrid_fixture, "mask": mask_fixture}[w]()
+         "multigrid": multigrid_fixture, "mask": mask_fixture, "les": les_fixture}[w]()

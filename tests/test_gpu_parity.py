@@ -288,3 +288,52 @@ def test_resume_from_checkpoint_is_bitwise(gpu_available, tmp_path, ps):
     assert np.array_equal(c.fields.u, a.fields.u)
     assert np.array_equal(c.fields.v, a.fields.v)
     assert np.array_equal(c.fields.p, a.fields.p)
+
+
+# ------------------------------------------------------------ Smagorinsky LES
+@pytest.mark.parametrize("name", ["cavity", "periodic"])
+def test_les_viscosity_and_rhs_bitwise(gpu_available, name):
+    """Eddy viscosity (turbulence.py:63-113) and the stress-form momentum RHS
+    (solver.py:467-529) on random fields: bit for bit against the reference."""
+    g = golden("les")
+    s = _solver(name, use_les=True, smagorinsky_cs=0.17)
+    nu_c, nu_k = s.turbulence.eddy_viscosity(g[f"{name}_u"], g[f"{name}_v"], s.nu)
+    assert np.array_equal(nu_c, g[f"{name}_nu_c"])
+    assert np.array_equal(nu_k, g[f"{name}_nu_corner"])
+    for b in (0, 7):
+        ru, rv = s.momentum_rhs(g[f"{name}_u"], g[f"{name}_v"], b / 10)
+        assert np.array_equal(ru, g[f"{name}_b{b}_rhs_u"])
+        assert np.array_equal(rv, g[f"{name}_b{b}_rhs_v"])
+
+
+def test_les_closed_forms(gpu_available):
+    """test_solver.py:262-289: a uniform periodic flow has nu_c = nu; with a
+    constant viscosity the stress form equals nu * lap on a solenoidal field."""
+    s = _solver("periodic", use_les=True)
+    f = s.initialize(1.0, -0.5)
+    nu_c, _ = s.turbulence.eddy_viscosity(f.u, f.v, s.nu)
+    assert np.abs(nu_c - s.nu).max() < 1e-14
+    lam = _solver("periodic")
+    Xu, Yu = lam.mesh.u_grid()
+    Xv, Yv = lam.mesh.v_grid()
+    f = lam.initialize(-np.cos(Xu) * np.sin(Yu), np.sin(Xv) * np.cos(Yv))
+    nx, ny = lam.mesh.shape
+    ru, rv = lam.momentum_rhs(f.u, f.v)
+    rvu, rvv = lam.momentum_rhs_variable(f.u, f.v, np.full((nx + 2, ny + 2), lam.nu))
+    assert np.abs(ru - rvu).max() < 1e-14
+    assert np.abs(rv - rvv).max() < 1e-14
+
+
+@pytest.mark.parametrize("name,steps", [("cavity", 10), ("cyl_outlet", 6)])
+def test_les_trajectory_matches_reference(gpu_available, name, steps):
+    """Whole LES steps through Simulation.run vs the reference, velocity to
+    1e-10 relative (direct solve at 1e-13)."""
+    g = golden("les")
+    case = oracle_case(name)
+    u0, v0 = initial(name, case)
+    sim = _sim(name, u0, v0, pressure_solver="direct", use_les=True, smagorinsky_cs=0.17)
+    sim.run(max_steps=steps)
+    scale = max(np.abs(g[f"traj_{name}_u"]).max(), 1.0)
+    assert np.abs(sim.fields.u - g[f"traj_{name}_u"]).max() <= 1e-10 * scale
+    assert np.abs(sim.fields.v - g[f"traj_{name}_v"]).max() <= 1e-10 * scale
+    assert max(sim.result.history["max_div"]) <= 1e-11

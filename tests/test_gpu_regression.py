@@ -100,3 +100,90 @@ def test_cylinder_re20_matches_baseline(gpu_available):
     assert cl_rms < 1e-8
     assert body.characteristic_length == pytest.approx(1.0)
     assert sim.solver.max_divergence(sim.fields) < 1e-10
+
+
+def test_taylor_green_convergence_matches_baseline(gpu_available):
+    """baselines.json fast tier, taylor_green_convergence: taylor_green.build
+    (nu = 0.05, t_end = 0.5, dt = 0.4 * 2 pi / N, fixed dt, direct solver) at
+    N = 16, 32, 64, 128; errors of the face velocities against the exact vortex
+    (taylor_green.py:118-132) and the observed orders."""
+    from pycfd_amd.analysis.validation import convergence_orders, taylor_green
+    from pycfd_amd.config import BCKind, BCSpec, SimulationConfig
+    from pycfd_amd.core.mesh import StructuredMesh
+    from pycfd_amd.physics.incompressible import Simulation
+    nu, L = 0.05, 2.0 * np.pi
+    ns, l2, linf = (16, 32, 64, 128), [], []
+    for n in ns:
+        cfg = SimulationConfig(nx=n, ny=n, lx=L, ly=L, re=1.0 / nu, u_ref=1.0, l_ref=1.0,
+                               dt=0.4 * L / n, t_end=0.5, adaptive_dt=False,
+                               boundary_config={w: BCSpec(BCKind.PERIODIC)
+                                                for w in ("left", "right", "bottom", "top")})
+        mesh = StructuredMesh.from_config(cfg)
+        Xu, Yu = mesh.u_grid()
+        Xv, Yv = mesh.v_grid()
+        sim = Simulation(cfg, u_init=taylor_green(Xu, Yu, 0.0, nu, "u"),
+                         v_init=taylor_green(Xv, Yv, 0.0, nu, "v"))
+        sim.run()
+        t = sim.fields.t
+        du = sim.fields.u_phys[:-1, :] - taylor_green(Xu, Yu, t, nu, "u")[:-1, :]
+        dv = sim.fields.v_phys[:, :-1] - taylor_green(Xv, Yv, t, nu, "v")[:, :-1]
+        l2.append(float(np.sqrt((np.sum(du ** 2) + np.sum(dv ** 2)) / (du.size + dv.size))))
+        linf.append(float(max(np.abs(du).max(), np.abs(dv).max())))
+        assert sim.solver.max_divergence(sim.fields) < 1e-10
+    o2, oi = convergence_orders(ns, l2)[-1], convergence_orders(ns, linf)[-1]
+    print(l2, linf, o2, oi)
+    assert o2 == pytest.approx(1.999669973940691, rel=RTOL)
+    assert oi == pytest.approx(1.9965084715978074, rel=LINF_RTOL)
+    assert l2[-1] == pytest.approx(4.77475461891742e-06, rel=RTOL)
+    assert linf[-1] == pytest.approx(9.558930812225697e-06, rel=LINF_RTOL)
+
+
+@pytest.mark.parametrize("mode", ["periodic", "developing"])
+def test_channel_matches_baseline(gpu_available, mode):
+    """baselines.json fast tier, channel_periodic (16 x 32, body force) and
+    channel_developing (48 x 24, uniform inlet, zero-gradient outlet):
+    channel_flow.build at Re 10 to steady_tol 1e-9, profile against the
+    parabola of equal flux (channel_flow.py:162-208)."""
+    from pycfd_amd.analysis.validation import l2_error, poiseuille_profile
+    from pycfd_amd.config import BCKind, BCSpec, SimulationConfig
+    from pycfd_amd.physics.incompressible import Simulation
+    re, nu = 10.0, 0.1
+    if mode == "periodic":
+        nx, ny, length, force, u0 = 16, 32, 2.0, (8.0 * nu, 0.0), 0.0
+        bcs = {"left": BCSpec(BCKind.PERIODIC), "right": BCSpec(BCKind.PERIODIC),
+               "bottom": BCSpec(BCKind.NO_SLIP), "top": BCSpec(BCKind.NO_SLIP)}
+        ref = dict(centerline_error_pct=1.0090861479739033e-07,
+                   profile_L2_relative=0.0009104384738361293,
+                   u_max_numerical=0.9999999989909139, mean_velocity=0.6679687493565628,
+                   steps=5722, final_time=21.036764705883428)
+    else:
+        nx, ny, length, force, u0 = 48, 24, 6.0, (0.0, 0.0), 1.0
+        bcs = {"left": BCSpec(BCKind.INLET, velocity=1.0), "right": BCSpec(BCKind.OUTLET),
+               "bottom": BCSpec(BCKind.NO_SLIP), "top": BCSpec(BCKind.NO_SLIP)}
+        ref = dict(centerline_error_pct=0.002499413032156876,
+                   profile_L2_relative=0.0023089094005366083,
+                   u_max_numerical=1.4999625088045176, mean_velocity=0.9999999999999997,
+                   steps=447, final_time=2.793750000000023)
+    cfg = SimulationConfig(nx=nx, ny=ny, lx=length, ly=1.0, re=re, u_ref=1.0, l_ref=1.0,
+                           dt=0.01, t_end=80.0, cfl_max=0.4, body_force=force,
+                           boundary_config=bcs, steady_tol=1.0e-9)
+    sim = Simulation(cfg, u_init=u0)
+    res = sim.run()
+    uc, _ = sim.fields.cell_velocities()
+    profile = uc[nx // 2 if mode == "periodic" else nx - 1, :]
+    mean = float(profile.mean())
+    peak = 1.0 if mode == "periodic" else 1.5
+    got = dict(centerline_error_pct=abs(profile.max() - peak) / peak * 100.0,
+               profile_L2_relative=l2_error(profile, poiseuille_profile(sim.mesh.yc, 1.0,
+                                                                        1.5 * mean),
+                                            relative=True),
+               u_max_numerical=float(profile.max()), mean_velocity=mean,
+               steps=res.steps, final_time=res.time)
+    print(mode, got)
+    assert res.converged
+    assert got["steps"] == ref["steps"]
+    for k in ("profile_L2_relative", "u_max_numerical", "mean_velocity", "final_time"):
+        assert got[k] == pytest.approx(ref[k], rel=RTOL), k
+    assert got["centerline_error_pct"] == pytest.approx(ref["centerline_error_pct"],
+                                                        rel=1e-3, abs=1e-9)
+    assert sim.solver.max_divergence(sim.fields) < 1e-10

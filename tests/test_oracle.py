@@ -289,3 +289,55 @@ def test_taylor_green_decay_rate():
     assert err < 5e-3                          # O(h^2) spatial error at 32^2
     ke0 = 0.5 * (2 * np.pi) ** 2 * 0.5
     assert abs(hist["kinetic_energy"][-1] / ke0 / np.exp(-4 * nu * t) - 1) < 2e-2
+
+
+# ---------------------------------------------------------------- LES closure
+@pytest.mark.parametrize("name", ["cavity", "periodic"])
+def test_les_viscosity_and_rhs_match_reference_bitwise(name):
+    """turbulence.py:63-113 and the stress-form RHS (solver.py:467-529) on
+    random fields, against vectors recorded from the reference."""
+    g = golden("les")
+    case = oracle_case(name, use_les=True)
+    nu_c, nu_k = O.eddy_viscosity(case, g[f"{name}_u"], g[f"{name}_v"])
+    assert np.array_equal(nu_c, g[f"{name}_nu_c"])
+    assert np.array_equal(nu_k, g[f"{name}_nu_corner"])
+    for b in (0, 7):
+        ru, rv = O.momentum_rhs(case, g[f"{name}_u"], g[f"{name}_v"], b / 10)
+        assert np.array_equal(ru, g[f"{name}_b{b}_rhs_u"])
+        assert np.array_equal(rv, g[f"{name}_b{b}_rhs_v"])
+
+
+def test_les_closed_forms():
+    """test_solver.py:257-289: a uniform periodic flow has zero strain (nu_c =
+    nu); with constant viscosity arrays the stress form is nu * lap on a
+    solenoidal field."""
+    case = oracle_case("periodic", use_les=True)
+    nx, ny = case.nx, case.ny
+    st = O.State()
+    u, v, _ = O.initialize(case, st, 1.0, -0.5)
+    nu_c, _ = O.eddy_viscosity(case, u, v)
+    assert np.abs(nu_c - case.nu).max() < 1e-14
+    lam = oracle_case("periodic")
+    xs = (np.arange(nx + 1) * case.dx)[:, None]
+    yc = ((np.arange(ny) + 0.5) * case.dy)[None, :]
+    xc = ((np.arange(nx) + 0.5) * case.dx)[:, None]
+    ys = (np.arange(ny + 1) * case.dy)[None, :]
+    u, v, _ = O.initialize(lam, O.State(), -np.cos(xs) * np.sin(yc), np.sin(xc) * np.cos(ys))
+    nuc = np.full((nx + 2, ny + 2), lam.nu)
+    nuk = np.full((nx + 1, ny + 1), lam.nu)
+    d_u, d_v = O.diffusion_variable(lam, u, v, nuc, nuk)
+    ru, rv = O.momentum_rhs(lam, u, v)
+    ru_adv, rv_adv = O.momentum_rhs(oracle_case("periodic", re=1e300), u, v)
+    np.testing.assert_allclose(d_u, ru - ru_adv, atol=1e-12)
+    np.testing.assert_allclose(d_v, rv - rv_adv, atol=1e-12)
+
+
+@pytest.mark.parametrize("name,steps", [("cavity", 10), ("cyl_outlet", 6)])
+def test_les_trajectory_matches_reference(name, steps):
+    g = golden("les")
+    case = oracle_case(name, pressure_solver="direct", use_les=True)
+    u0, v0 = initial(name, case)
+    u, v, p, hist, t = O.run(case, steps, u0, v0)
+    scale = max(np.abs(g[f"traj_{name}_u"]).max(), 1.0)
+    assert np.abs(u - g[f"traj_{name}_u"]).max() <= 1e-10 * scale
+    assert np.abs(v - g[f"traj_{name}_v"]).max() <= 1e-10 * scale
