@@ -50,7 +50,8 @@ def run_path(precision: str, steps: int, warmup: int, nx: int, ny: int):
         f"({sim.solver.pressure_solver.hierarchy})")
     plan = sim.solver.ctx.mg_plan()
     log(f"[{precision}] V-cycle plan: " + ", ".join(
-        f"{p['nx']}x{p['ny']}:" + (f"fused {p['tile'][0]}x{p['tile'][1]}" if p["fused"] else "passes")
+        f"{p['nx']}x{p['ny']}:" + (f"fused {p['tile'][0]}x{p['tile'][1]}" if p["fused"]
+                                   else p["mode"])
         for p in plan))
     if warmup:
         sim.run(max_steps=warmup)
@@ -66,7 +67,7 @@ def run_path(precision: str, steps: int, warmup: int, nx: int, ny: int):
         "max_div": float(max(hist["max_div"])),
         "iterations_last_solve": int(sim.solver.pressure_solver.last_iterations),
         "cd_last": float(sim.force_coefficients()[0]),
-        "mg_plan": [[p["nx"], p["ny"], int(p["fused"]), *p["tile"]] for p in plan],
+        "mg_plan": [[p["nx"], p["ny"], p["mode"], *p["tile"]] for p in plan],
         "t": float(sim.fields.t),
     }
     log(f"[{precision}] {steps} steps in {wall:.3f} s -> {out['mlups']:.1f} MLUPS, "

@@ -97,10 +97,13 @@ typedef struct pcfd_solver_desc {
   int32_t precision;  /* 0: FP64 throughout; 1: FP32 V-cycle, FP64 Krylov */
   /* Multigrid level kernels (no counterpart in the reference: its V-cycle is
    * SciPy sparse products).  fused_levels = 0 runs every stored level pass by
-   * pass; tile_tx / tile_ty fix the level-0 tile (0: cost model).  A zeroed
-   * tail keeps the defaults. */
+   * pass; tile_tx / tile_ty fix the level-0 tile (0: measured choice);
+   * coarse_chain picks the cooperative launch of the deepest levels (0:
+   * measured choice, -1: never, k >= 1: from level k down).  A zeroed tail
+   * keeps the defaults. */
   int32_t fused_levels;
   int32_t tile_tx, tile_ty;
+  int32_t coarse_chain;
 } pcfd_solver_desc;
 
 /* One level of the aggregation hierarchy (multigrid.py:293-322).  Level 0 is
@@ -235,8 +238,9 @@ int pcfd_time_kernel(pcfd_ctx* ctx, int32_t which, int32_t reps, double* avg_ms,
 
 /* The multigrid plan of the configured solver, for reporting: per level (all
  * levels of the cycle precision, finest first) five integers -- nx, ny,
- * fused (1: one fused launch per leg, 0: pass by pass; the coarsest level is
- * the dense solve and reports 0), tile x, tile y -- written to out (capacity
+ * mode (1: one fused launch per leg, 0: pass by pass, 2: inside the
+ * cooperative launch of the deepest levels; the coarsest level is the dense
+ * solve), tile x, tile y -- written to out (capacity
  * `cap` integers).  Returns the number of levels, or a negative status.  No
  * reference counterpart. */
 int pcfd_mg_plan(pcfd_ctx* ctx, int32_t* out, int32_t cap);

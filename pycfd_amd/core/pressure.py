@@ -152,6 +152,10 @@ class PoissonSolverBase:
         self.coarse_max = int(coarse_max)
         self.fused_levels = bool(fused_levels)
         self.tile = (int(tile[0]), int(tile[1]))
+        # cooperative launch of the deepest multigrid levels: -1 never (the
+        # default: on MI355X a grid-wide barrier costs more than the launch it
+        # replaces), 0 measured choice, k >= 1 from level k down
+        self.coarse_chain = -1
         self.last_iterations = 0
         self.last_residual = 0.0
         self._ctx = context
@@ -195,7 +199,8 @@ class PoissonSolverBase:
         tol, maxiter = self._tol_maxiter()
         self.context.set_solver(self._device_kind, tol, maxiter, self._omega(), self._sweeps(),
                                 1 if self.precision == "mixed" else 0, self._levels(),
-                                fused_levels=self.fused_levels, tile=self.tile)
+                                fused_levels=self.fused_levels, tile=self.tile,
+                                coarse_chain=self.coarse_chain)
         self._configured = True
 
     def record(self, iterations: int, residual: float, converged: bool) -> None:

@@ -469,6 +469,7 @@ int pcfd_set_pressure_solver(pcfd_ctx* c, const pcfd_solver_desc* sd,
   cfg.fused = sd->fused_levels;
   cfg.tile_tx = sd->tile_tx;
   cfg.tile_ty = sd->tile_ty;
+  cfg.chain = sd->coarse_chain;
   if (cfg.tile_tx < 0 || cfg.tile_ty < 0) {
     set_last_error("tile sizes must be >= 0, got %dx%d", cfg.tile_tx, cfg.tile_ty);
     return kErrBadArg;

@@ -30,6 +30,8 @@
 #include <map>
 #include <mutex>
 
+#include <hip/hip_cooperative_groups.h>
+
 namespace pcfd {
 
 // ----------------------------------------------------------------------------
@@ -529,13 +531,12 @@ __device__ __forceinline__ int cls_nb(const ClassLevel<T>& L, int row, int e) {
   return ii * L.ny + jj;
 }
 
+// One row operation at global lane gid (LPR lanes per row).
 template <typename T, int LPR, int OP>
-__global__ void __launch_bounds__(kBlock)
-k_cls_rows(ClassLevel<T> L, T w, const T* __restrict__ b, const T* __restrict__ xin,
-           const T* __restrict__ xc, const int* __restrict__ agg, T* __restrict__ o1,
-           T* __restrict__ o2, T* __restrict__ o3, const SolveState* st) {
-  if (st->done) return;
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void cls_rows_at(const ClassLevel<T>& L, T w, const T* __restrict__ b,
+                                            const T* __restrict__ xin, const T* __restrict__ xc,
+                                            const int* __restrict__ agg, T* __restrict__ o1,
+                                            T* __restrict__ o2, T* __restrict__ o3, int gid) {
   const int row = gid / LPR;
   const int lane = gid % LPR;
   const bool valid = row < L.n;
@@ -587,19 +588,27 @@ k_cls_rows(ClassLevel<T> L, T w, const T* __restrict__ b, const T* __restrict__ 
   }
 }
 
+template <typename T, int LPR, int OP>
+__global__ void __launch_bounds__(kBlock)
+k_cls_rows(ClassLevel<T> L, T w, const T* __restrict__ b, const T* __restrict__ xin,
+           const T* __restrict__ xc, const int* __restrict__ agg, T* __restrict__ o1,
+           T* __restrict__ o2, T* __restrict__ o3, const SolveState* st) {
+  if (st->done) return;
+  cls_rows_at<T, LPR, OP>(L, w, b, xin, xc, agg, o1, o2, o3, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // Restriction in one pass: b_c[I] = sum over the cells a of block I of
 // (r - w A t)[a] on active rows (t = D^-1 r); the block's cells are summed in
 // x-major order (P_tent^T y with y the P-smoothed residual).  A group of
 // 4 * LPR lanes serves one coarse cell, LPR lanes per fine row of its block
 // (up to 2 x 2 cells), so the rows of a block are evaluated side by side.
 template <typename T, int LPR>
-__global__ void __launch_bounds__(kBlock)
-k_cls_restrict(ClassLevel<T> L, T w, const T* __restrict__ r, const T* __restrict__ t, int fx,
-               int fy, int cnx, int cny, T* __restrict__ bc, const SolveState* st) {
+__device__ __forceinline__ void cls_restrict_at(const ClassLevel<T>& L, T w,
+                                                const T* __restrict__ r, const T* __restrict__ t,
+                                                int fx, int fy, int cnx, int cny,
+                                                T* __restrict__ bc, int gid) {
   static_assert(4 * LPR <= kWave, "a coarse cell's lanes must share a wavefront");
-  if (st->done) return;
   constexpr int G = 4 * LPR;
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const int crow = gid / G;
   const int sub = (gid % G) / LPR;
   const int lane = gid % LPR;
@@ -639,14 +648,22 @@ k_cls_restrict(ClassLevel<T> L, T w, const T* __restrict__ r, const T* __restric
   if (valid && sub == 0 && lane == 0) bc[crow] = sum;
 }
 
+template <typename T, int LPR>
+__global__ void __launch_bounds__(kBlock)
+k_cls_restrict(ClassLevel<T> L, T w, const T* __restrict__ r, const T* __restrict__ t, int fx,
+               int fy, int cnx, int cny, T* __restrict__ bc, const SolveState* st) {
+  if (st->done) return;
+  cls_restrict_at<T, LPR>(L, w, r, t, fx, fy, cnx, cny, bc, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // The same for rows too long for four of them to share a wavefront: the
 // block's rows one after another, LPR lanes each.
 template <typename T, int LPR>
-__global__ void __launch_bounds__(kBlock)
-k_cls_restrict_serial(ClassLevel<T> L, T w, const T* __restrict__ r, const T* __restrict__ t,
-                      int fx, int fy, int cnx, int cny, T* __restrict__ bc, const SolveState* st) {
-  if (st->done) return;
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void cls_restrict_serial_at(const ClassLevel<T>& L, T w,
+                                                       const T* __restrict__ r,
+                                                       const T* __restrict__ t, int fx, int fy,
+                                                       int cnx, int cny, T* __restrict__ bc,
+                                                       int gid) {
   const int crow = gid / LPR;
   const int lane = gid % LPR;
   const bool valid = crow < cnx * cny;
@@ -672,6 +689,166 @@ k_cls_restrict_serial(ClassLevel<T> L, T w, const T* __restrict__ r, const T* __
     }
   }
   if (valid && lane == 0) bc[crow] = sum;
+}
+
+template <typename T, int LPR>
+__global__ void __launch_bounds__(kBlock)
+k_cls_restrict_serial(ClassLevel<T> L, T w, const T* __restrict__ r, const T* __restrict__ t,
+                      int fx, int fy, int cnx, int cny, T* __restrict__ bc, const SolveState* st) {
+  if (st->done) return;
+  cls_restrict_serial_at<T, LPR>(L, w, r, t, fx, fy, cnx, cny, bc,
+                                 blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// ----------------------------------------------------------------------------
+// The deepest stored levels and the dense coarsest solve in one cooperative
+// launch.  These levels are small (tens of thousands of rows at most) and their
+// passes are bound by launch latency, so the down leg, the coarsest solve and
+// the up leg run as phases of one grid, separated by grid-wide barriers,
+// instead of as a dozen dependent launches.  Every phase is the same per-row
+// arithmetic as the pass-by-pass kernels, spread over the grid by striding.
+// ----------------------------------------------------------------------------
+constexpr int kMaxChain = 8;
+
+template <typename T>
+struct ChainLevel {
+  ClassLevel<T> L;
+  T w;
+  int lanes, fx, fy;
+  T *b, *x, *y, *r, *t;
+  const int* agg;
+};
+
+template <typename T>
+struct Chain {
+  int n;  // stored levels in the chain, finest first
+  int nu;
+  ChainLevel<T> lv[kMaxChain];
+  int cn;             // coarsest unknowns
+  const double* cinv; // dense pseudo-inverse, cn x cn
+  T* cb;              // coarsest right-hand side (written by the last restriction)
+  T* cy;              // coarsest solution
+  int cnx, cny;       // coarsest grid
+};
+
+template <typename T, int OP>
+__device__ __forceinline__ void chain_rows(const ChainLevel<T>& c, const T* b, const T* xin,
+                                           const T* xc, T* o1, T* o2, T* o3, int tid,
+                                           int nth) {
+  const int n = c.L.n;
+  switch (c.lanes) {
+    case 1:
+      for (int g = tid; g < n; g += nth)
+        cls_rows_at<T, 1, OP>(c.L, c.w, b, xin, xc, c.agg, o1, o2, o3, g);
+      break;
+    case 4:
+      for (int g = tid; g < n * 4; g += nth)
+        cls_rows_at<T, 4, OP>(c.L, c.w, b, xin, xc, c.agg, o1, o2, o3, g);
+      break;
+    case 16:
+      for (int g = tid; g < n * 16; g += nth)
+        cls_rows_at<T, 16, OP>(c.L, c.w, b, xin, xc, c.agg, o1, o2, o3, g);
+      break;
+    default:
+      for (int g = tid; g < n * 64; g += nth)
+        cls_rows_at<T, 64, OP>(c.L, c.w, b, xin, xc, c.agg, o1, o2, o3, g);
+      break;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void chain_restrict(const ChainLevel<T>& c, int cnx, int cny, T* bc,
+                                               int tid, int nth) {
+  const int cells = cnx * cny;
+  switch (c.lanes) {
+    case 1:
+      for (int g = tid; g < cells * 4; g += nth)
+        cls_restrict_at<T, 1>(c.L, c.w, c.r, c.t, c.fx, c.fy, cnx, cny, bc, g);
+      break;
+    case 4:
+      for (int g = tid; g < cells * 16; g += nth)
+        cls_restrict_at<T, 4>(c.L, c.w, c.r, c.t, c.fx, c.fy, cnx, cny, bc, g);
+      break;
+    case 16:
+      for (int g = tid; g < cells * 64; g += nth)
+        cls_restrict_at<T, 16>(c.L, c.w, c.r, c.t, c.fx, c.fy, cnx, cny, bc, g);
+      break;
+    default:
+      for (int g = tid; g < cells * 64; g += nth)
+        cls_restrict_serial_at<T, 64>(c.L, c.w, c.r, c.t, c.fx, c.fy, cnx, cny, bc, g);
+      break;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_chain(Chain<T> C, const SolveState* st) {
+  // `done` is read once by every workgroup before any barrier: uniform.
+  if (st->done) return;
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  extern __shared__ double cshared[];
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nth = gridDim.x * blockDim.x;
+  const int nu = C.nu;
+  bool pre_in_y[kMaxChain];  // where each level's pre-smoothed iterate sits
+  // Down leg.
+  for (int l = 0; l < C.n; ++l) {
+    const ChainLevel<T>& c = C.lv[l];
+    const bool last = l + 1 == C.n;
+    T* bnext = last ? C.cb : C.lv[l + 1].b;
+    const int cnx = last ? C.cnx : C.lv[l + 1].L.nx;
+    const int cny = last ? C.cny : C.lv[l + 1].L.ny;
+    chain_rows<T, RO_SMOOTH0_RES>(c, c.b, nullptr, nullptr, c.x, c.r, c.t, tid, nth);
+    grid.sync();
+    T* cur = c.x;
+    T* oth = c.y;
+    if (nu > 1) {
+      for (int sw = 1; sw < nu; ++sw) {
+        chain_rows<T, RO_SMOOTH>(c, c.b, cur, nullptr, oth, nullptr, nullptr, tid, nth);
+        grid.sync();
+        T* tmp = cur;
+        cur = oth;
+        oth = tmp;
+      }
+      chain_rows<T, RO_RES>(c, c.b, cur, nullptr, nullptr, c.r, c.t, tid, nth);
+      grid.sync();
+    }
+    pre_in_y[l] = cur == c.y;
+    chain_restrict<T>(c, cnx, cny, bnext, tid, nth);
+    grid.sync();
+  }
+  // Coarsest level: y = C b, one wavefront per row, b staged in LDS.
+  for (int k = threadIdx.x; k < C.cn; k += blockDim.x) cshared[k] = (double)C.cb[k];
+  __syncthreads();
+  {
+    const int lane = threadIdx.x % kWave;
+    const int wv = tid / kWave, nwv = nth / kWave;
+    for (int row = wv; row < C.cn; row += nwv) {
+      const double* crow = C.cinv + (size_t)row * C.cn;
+      double acc = 0.0;
+#pragma unroll 8
+      for (int k = lane; k < C.cn; k += kWave) acc += crow[k] * cshared[k];
+      acc = wave_sum(acc);
+      if (lane == 0) C.cy[row] = (T)acc;
+    }
+  }
+  grid.sync();
+  // Up leg: prolongate into the pre-smoothed iterate, then smooth; the
+  // alternation leaves the corrected iterate in y (see the down leg).
+  for (int l = C.n - 1; l >= 0; --l) {
+    const ChainLevel<T>& c = C.lv[l];
+    const T* xc = (l + 1 == C.n) ? C.cy : C.lv[l + 1].y;
+    T* cur = pre_in_y[l] ? c.y : c.x;
+    T* oth = pre_in_y[l] ? c.x : c.y;
+    chain_rows<T, RO_PROLONG>(c, nullptr, nullptr, xc, cur, nullptr, nullptr, tid, nth);
+    grid.sync();
+    for (int sw = 0; sw < nu; ++sw) {
+      chain_rows<T, RO_SMOOTH>(c, c.b, cur, nullptr, oth, nullptr, nullptr, tid, nth);
+      grid.sync();
+      T* tmp = cur;
+      cur = oth;
+      oth = tmp;
+    }
+  }
 }
 
 // ----------------------------------------------------------------------------
@@ -878,6 +1055,7 @@ int PoissonEngine::configure(const PoissonConfig& cfg, const std::vector<MGLevel
   cfg_ = cfg;
   free_levels();
   dir_tile_ = FusedTile{};
+  chain_first_ = -1;
   last_iters_ = 0;
   predicted_ = 0;
   const bool need_mg = cfg.kind == SK_MULTIGRID || cfg.kind == SK_MGCG || cfg.kind == SK_DIRECT;
@@ -1154,6 +1332,72 @@ int PoissonEngine::tune_dir(SolveState* tst) {
   return kOk;
 }
 
+// Whether the deepest levels run as one cooperative launch, and from which
+// level: the per-level legs and the chain starting at each of the last few
+// levels are timed on this device (HIP events over the coarse legs) and the
+// fastest kept.  The chain needs every workgroup of its grid resident at once,
+// which the cooperative launch guarantees or refuses.
+template <typename T>
+int PoissonEngine::tune_chain(std::vector<LevelBuf>& lv, SolveState* tst) {
+  chain_first_ = -1;
+  const int nl = (int)lv.size();
+  if (nl < 3 || cfg_.chain < 0) return kOk;
+  if (cfg_.chain == 0 && !autotune_enabled()) return kOk;
+  int dev = 0, cus = 0, coop = 0, per = 0;
+  PCFD_HIP_CHECK(hipGetDevice(&dev));
+  PCFD_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  PCFD_HIP_CHECK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+  const size_t lds = (size_t)lv[nl - 1].n * sizeof(double);
+  if (!coop || hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                   &per, reinterpret_cast<const void*>(&k_chain<T>), kBlock, lds) != hipSuccess ||
+      per < 1) {
+    (void)hipGetLastError();
+    return kOk;
+  }
+  chain_blocks_ = cus * std::min(per, 2);
+  if (cfg_.chain > 0) {  // pinned by the caller
+    chain_first_ = std::min(std::max(cfg_.chain, nl - 1 - kMaxChain), nl - 2);
+    chain_first_ = std::max(chain_first_, 1);
+    return kOk;
+  }
+  hipEvent_t e0, e1;
+  PCFD_HIP_CHECK(hipEventCreate(&e0));
+  PCFD_HIP_CHECK(hipEventCreate(&e1));
+  auto time_legs = [&](float* ms) -> int {
+    for (int k = 0; k < 4; ++k) {
+      if (k == 1) PCFD_HIP_CHECK(hipEventRecord(e0, s_));
+      if (int rc = coarse_legs<T>(lv, 1, tst)) return rc;
+    }
+    PCFD_HIP_CHECK(hipEventRecord(e1, s_));
+    PCFD_HIP_CHECK(hipEventSynchronize(e1));
+    PCFD_HIP_CHECK(hipEventElapsedTime(ms, e0, e1));
+    return kOk;
+  };
+  float best = 0.f;
+  int rc = time_legs(&best);
+  int pick = -1;
+  // candidate starts: the deepest stored level up to three levels higher
+  for (int f = nl - 2; f >= std::max(1, nl - 5) && rc == kOk; --f) {
+    if (nl - 1 - f > kMaxChain) break;
+    chain_first_ = f;
+    float ms = 0.f;
+    const int r2 = time_legs(&ms);
+    if (r2 != kOk) {  // a refused cooperative launch just leaves the chain out
+      (void)hipGetLastError();
+      (void)hipStreamSynchronize(s_);
+      break;
+    }
+    if (ms < best) {
+      best = ms;
+      pick = f;
+    }
+  }
+  chain_first_ = pick;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return rc;
+}
+
 template <typename T>
 int PoissonEngine::plan_level_tiles(std::vector<LevelBuf>& lv) {
   if (lv.empty()) return kOk;
@@ -1162,6 +1406,8 @@ int PoissonEngine::plan_level_tiles(std::vector<LevelBuf>& lv) {
   PCFD_HIP_CHECK(hipMemsetAsync(tst, 0, sizeof(SolveState), s_));
   int rc = kOk;
   for (int l = 0; l + 1 < (int)lv.size() && rc == kOk; ++l) rc = tune_level<T>(lv, l, tst);
+  // the chain serves the cycle precision's hierarchy
+  if (rc == kOk && (cfg_.precision == 1) == (sizeof(T) == 4)) rc = tune_chain<T>(lv, tst);
   // the Krylov direction kernel reads the cycle output in this precision
   if (rc == kOk && (cfg_.kind == SK_MGCG || cfg_.kind == SK_DIRECT) &&
       (cfg_.precision == 1) == (sizeof(T) == 4))
@@ -1524,6 +1770,69 @@ int PoissonEngine::level_up(LevelBuf& L, LevelBuf& C, const SolveState* cst) {
   return kOk;
 }
 
+// The stored levels from `first` down, the coarsest solve and the way back
+// up: per-level legs above the cooperative chain (when the plan has one), the
+// chain itself, then the per-level up legs.
+template <typename T>
+int PoissonEngine::coarse_legs(std::vector<LevelBuf>& lv, int first, const SolveState* cst) {
+  const int nl = (int)lv.size();
+  const bool cycle_precision = (cfg_.precision == 1) == (sizeof(T) == 4);
+  const int cf = (cycle_precision && chain_first_ >= first && chain_first_ < nl) ? chain_first_
+                                                                                : nl - 1;
+  for (int l = first; l < cf; ++l)
+    if (int rc = level_down<T>(lv[l], lv[l + 1], cst)) return rc;
+  if (cf < nl - 1) {
+    if (int rc = launch_chain<T>(lv, cf, cst)) return rc;
+  } else {
+    for (int l = cf; l < nl - 1; ++l)
+      if (int rc = level_down<T>(lv[l], lv[l + 1], cst)) return rc;
+    LevelBuf& L = lv[nl - 1];
+    if (int rc = launch_dense_apply<T>(s_, L.cinv, L.n, (const T*)L.b, (T*)L.y, cst)) return rc;
+  }
+  for (int l = cf - 1; l >= first; --l)
+    if (int rc = level_up<T>(lv[l], lv[l + 1], cst)) return rc;
+  return kOk;
+}
+
+template <typename T>
+int PoissonEngine::launch_chain(std::vector<LevelBuf>& lv, int first, const SolveState* cst) {
+  const int nl = (int)lv.size();
+  Chain<T> C{};
+  C.n = nl - 1 - first;
+  C.nu = cfg_.sweeps;
+  if (C.n < 1 || C.n > kMaxChain) {
+    set_last_error("cooperative chain of %d levels is out of range", C.n);
+    return kErrBadArg;
+  }
+  for (int k = 0; k < C.n; ++k) {
+    LevelBuf& L = lv[first + k];
+    ChainLevel<T>& c = C.lv[k];
+    c.L = class_level<T>(L, op_.px, op_.py);
+    c.w = (T)L.weight;
+    c.lanes = L.lanes;
+    c.fx = L.fx;
+    c.fy = L.fy;
+    c.b = (T*)L.b;
+    c.x = (T*)L.x;
+    c.y = (T*)L.y;
+    c.r = (T*)L.r;
+    c.t = (T*)L.t;
+    c.agg = L.agg;
+  }
+  LevelBuf& Lc = lv[nl - 1];
+  C.cn = Lc.n;
+  C.cinv = Lc.cinv;
+  C.cb = (T*)Lc.b;
+  C.cy = (T*)Lc.y;
+  C.cnx = Lc.nx;
+  C.cny = Lc.ny;
+  void* args[] = {(void*)&C, (void*)&cst};
+  PCFD_HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_chain<T>),
+                                            dim3(chain_blocks_), dim3(kBlock), args,
+                                            (unsigned)(Lc.n * sizeof(double)), s_));
+  return kOk;
+}
+
 // One V-cycle (multigrid.py:430-451) over the levels in precision T.
 //
 // Buffer contract per level: the down leg reads the level's right-hand side
@@ -1569,15 +1878,7 @@ int PoissonEngine::enqueue_vcycle_t(int mode, int reduce) {
         (T*)L0.t, (T*)C.b, L0.fx, L0.fy, C.nx, C.ny, cst);
     if (rc) return rc;
   }
-  for (int l = 1; l < nl - 1; ++l)
-    if (int rc = level_down<T>(lv[l], lv[l + 1], cst)) return rc;
-  {
-    LevelBuf& L = lv[nl - 1];
-    if (int rc = launch_dense_apply<T>(s_, L.cinv, L.n, (const T*)L.b, (T*)L.y, cst)) return rc;
-  }
-  // Upward leg.
-  for (int l = nl - 2; l >= 1; --l)
-    if (int rc = level_up<T>(lv[l], lv[l + 1], cst)) return rc;
+  if (int rc = coarse_legs<T>(lv, 1, cst)) return rc;
   {
     LevelBuf& C = lv[1];
     if (int rc = launch_fine_up<T>(s_, op_, L0.up, (T)L0.weight, nu, (const T*)L0.t,

@@ -109,6 +109,7 @@ struct PoissonConfig {
   int fused = 1;    // stored levels through the fused kernels when they fit
   int tile_tx = 0;  // level-0 fused tile (0 = chosen by the cost model)
   int tile_ty = 0;
+  int chain = 0;    // cooperative coarse chain: 0 measured, -1 never, k >= 1 from level k
 };
 
 struct SolveInfo {
@@ -168,9 +169,12 @@ class PoissonEngine {
   // Per level: nx, ny, fused, tile x, tile y (see pcfd_mg_plan).
   int plan(int32_t* out, int cap) const {
     const auto& lv = (cfg_.precision == 1) ? lv32_ : lv64_;
-    int k = 0;
+    int k = 0, l = 0;
     for (const LevelBuf& L : lv) {
-      const int v[5] = {L.nx, L.ny, L.fused, L.fused ? L.down0.tx : 0, L.fused ? L.down0.ty : 0};
+      const int mode = (chain_first_ >= 0 && l >= chain_first_) ? 2 : L.fused;
+      ++l;
+      const int v[5] = {L.nx, L.ny, mode, mode == 1 ? L.down0.tx : 0,
+                        mode == 1 ? L.down0.ty : 0};
       for (int q = 0; q < 5; ++q)
         if (k < cap) out[k++] = v[q];
     }
@@ -210,6 +214,12 @@ class PoissonEngine {
   int ensure_partials(size_t n);
   template <typename T>
   int tune_dir(SolveState* tst);
+  template <typename T>
+  int coarse_legs(std::vector<LevelBuf>& lv, int first, const SolveState* st);
+  template <typename T>
+  int launch_chain(std::vector<LevelBuf>& lv, int first, const SolveState* st);
+  template <typename T>
+  int tune_chain(std::vector<LevelBuf>& lv, SolveState* tst);
   int poll(int* iter, int* done);
   int start(int mode, double* r);
   void free_levels();
@@ -233,6 +243,8 @@ class PoissonEngine {
   int n_active0_ = 0;
   int n_proj_ = 0;  // cells of the constant mode removed by projections (0: non-singular)
   FusedTile dir_tile_;  // level-0 tiling of the multigrid-CG direction kernel
+  int chain_first_ = -1;  // first level of the cooperative coarse chain (-1: none)
+  int chain_blocks_ = 0;  // its grid size (co-resident workgroups)
   double weight0_ = 1.0;
 };
 

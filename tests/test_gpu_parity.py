@@ -92,11 +92,15 @@ def test_operator_apply(gpu_available, name):
     assert np.abs(y.cpu().numpy() - ref).max() <= 1e-14 * np.abs(ref).max()
 
 
-def _mg_solver(name, fused=True, **kw):
-    s = _solver(name, pressure_solver="mgcg", **kw)
-    if not fused:
-        s.pressure_solver.fused_levels = False
-        s.pressure_solver.configure()
+def _mg_solver(name, fused=True, chain=-1, **kw):
+    """fused: stored levels through the fused kernels where they fit; chain:
+    the cooperative launch of the deepest levels (-1 never, k from level k)."""
+    s = _solver(name, pressure_solver=kw.pop("pressure_solver", "mgcg"), **kw)
+    ps = s.pressure_solver
+    if not fused or chain != 0:
+        ps.fused_levels = bool(fused)
+        ps.coarse_chain = chain
+        ps.configure()
     return s
 
 
@@ -109,7 +113,7 @@ def test_vcycle_matches_reference(gpu_available, name, sweeps, fused):
     import torch
     g = golden("multigrid")
     tag = f"{name}_s{sweeps}"
-    s = _mg_solver(name, fused, mg_sweeps=sweeps, mg_coarse_max=96)
+    s = _mg_solver(name, fused, -1, mg_sweeps=sweeps, mg_coarse_max=96)
     r = torch.as_tensor(g[f"{tag}_r"]).cuda()
     z = torch.empty_like(r)
     s.ctx.mg_apply(r, z)
@@ -122,15 +126,21 @@ VCYCLE_GRIDS = [("cavity", {}), ("periodic", {}), ("cyl_outlet", {}), ("cyl_pout
                 ("periodic", dict(nx=128, ny=96))]
 
 
+CYCLE_MODES = {"fused": (True, -1), "passes": (False, -1), "chain1": (True, 1),
+               "chain2": (True, 2)}
+
+
 @pytest.mark.parametrize("name,grid", VCYCLE_GRIDS,
                          ids=[f"{n}-{g.get('nx', 'g')}" for n, g in VCYCLE_GRIDS])
 @pytest.mark.parametrize("coarse_max", [96, 2048])
 @pytest.mark.parametrize("sweeps", [1, 2])
-@pytest.mark.parametrize("fused", [True, False])
-def test_vcycle_matches_oracle(gpu_available, name, grid, coarse_max, sweeps, fused):
+@pytest.mark.parametrize("mode", list(CYCLE_MODES))
+def test_vcycle_matches_oracle(gpu_available, name, grid, coarse_max, sweeps, mode):
     """One V-cycle with the coarsest solve at the first level of at most
     coarse_max unknowns, vs the oracle's cycle on the same hierarchy:
-    FP64 to 1e-10, the FP32 cycle to 1e-4 (relative to max|z|)."""
+    FP64 to 1e-10, the FP32 cycle to 1e-4 (relative to max|z|).  Modes: fused
+    level kernels, pass-by-pass levels, the cooperative chain from level 1 or 2."""
+    fused, chain = CYCLE_MODES[mode]
     import torch
     case = oracle_case(name, **grid)
     S = O.assemble(case)
@@ -140,13 +150,13 @@ def test_vcycle_matches_oracle(gpu_available, name, grid, coarse_max, sweeps, fu
     r = rng.standard_normal(S.A.shape[0])
     ref = H.apply(r)
     for precision, tol in (("fp64", 1e-10), ("mixed", 1e-4)):
-        s = _mg_solver(name, fused, mg_sweeps=sweeps, mg_coarse_max=coarse_max,
+        s = _mg_solver(name, fused, chain, mg_sweeps=sweeps, mg_coarse_max=coarse_max,
                        precision=precision, **grid)
         z = torch.empty(r.size, dtype=torch.float64, device="cuda")
         s.ctx.mg_apply(torch.as_tensor(r).cuda(), z)
         err = np.abs(z.cpu().numpy() - ref).max() / np.abs(ref).max()
-        print(f"{name} {grid} coarse_max={coarse_max} sweeps={sweeps} fused={fused} "
-              f"{precision}: {err:.2e}")
+        print(f"{name} {grid} coarse_max={coarse_max} sweeps={sweeps} {mode} "
+              f"{precision}: {err:.2e} plan={s.ctx.mg_plan()}")
         assert err <= tol, precision
 
 
@@ -209,7 +219,7 @@ def test_trajectory_matches_reference(gpu_available, name, ps, steps):
 
 
 @pytest.mark.parametrize("ps", ["sor", "multigrid", "mgcg", "jacobi", "multigrid-96",
-                                "mgcg-96", "mgcg-40"])
+                                "mgcg-96", "mgcg-40", "multigrid-chain", "mgcg-chain"])
 def test_every_solver_gives_the_same_step(gpu_available, ps):
     """test_solver.py:106-119: every pressure solver at tol 1e-13 agrees with
     the direct one to 1e-9 after a few steps (suffix: mg_coarse_max)."""
@@ -218,10 +228,15 @@ def test_every_solver_gives_the_same_step(gpu_available, ps):
         kw = dict(poisson_tol=1e-11, poisson_maxiter=400000)
     else:
         kw = dict(poisson_tol=1e-13, poisson_maxiter=200000)
+    chain = cm == "chain"
     if cm:
-        kw["mg_coarse_max"] = int(cm)
+        kw["mg_coarse_max"] = 40 if chain else int(cm)
     a = _sim("cavity", pressure_solver="direct")
     b = _sim("cavity", pressure_solver=ps, **kw)
+    if chain:
+        b.solver.pressure_solver.coarse_chain = 1
+        b.solver.pressure_solver.configure()
+        assert any(p["mode"] == "chain" for p in b.solver.ctx.mg_plan())
     a.run(max_steps=3)
     b.run(max_steps=3)
     assert np.abs(a.fields.u - b.fields.u).max() < 1e-9
