@@ -66,11 +66,38 @@ class Case:
     upwind_blend: float | None = None
     use_les: bool = False
     smagorinsky_cs: float = 0.17
+    stretch_x: float = 1.0
+    stretch_y: float = 1.0
+    cluster_x: str = "low"
+    cluster_y: str = "low"
 
     def __post_init__(self):
-        self.dx = float(np.full(self.nx, self.lx / self.nx)[0])
-        self.dy = float(np.full(self.ny, self.ly / self.ny)[0])
+        self.dx_cells = geometric_widths(self.lx, self.nx, self.stretch_x, self.cluster_x)
+        self.dy_cells = geometric_widths(self.ly, self.ny, self.stretch_y, self.cluster_y)
+        self.uniform = bool(np.allclose(self.dx_cells, self.dx_cells[0], rtol=1e-12)
+                            and np.allclose(self.dy_cells, self.dy_cells[0], rtol=1e-12))
+        self.dx = float(self.dx_cells[0])
+        self.dy = float(self.dy_cells[0])
         self.nu = self.u_ref * self.l_ref / self.re
+        # spacings the operators divide by (mesh.py:122-233): scalars on a
+        # uniform mesh, broadcastable arrays otherwise
+        hx_ext = np.concatenate(([self.dx_cells[0]], self.dx_cells, [self.dx_cells[-1]]))
+        hy_ext = np.concatenate(([self.dy_cells[0]], self.dy_cells, [self.dy_cells[-1]]))
+        hxu = 0.5 * (hx_ext[:-1] + hx_ext[1:])
+        hyv = 0.5 * (hy_ext[:-1] + hy_ext[1:])
+        self.cell_areas = self.dx_cells[:, None] * self.dy_cells[None, :]
+        self.min_hx = float(self.dx_cells.min())
+        self.min_hy = float(self.dy_cells.min())
+        if self.uniform:
+            self.hx = self.hxu = self.hx_ext = self.dx
+            self.hy = self.hyv = self.hy_ext = self.dy
+            self.wx, self.wy = 0.5, 0.5
+        else:
+            self.hx, self.hy = self.dx_cells[:, None], self.dy_cells[None, :]
+            self.hxu, self.hyv = hxu[:, None], hyv[None, :]
+            self.hx_ext, self.hy_ext = hx_ext[:, None], hy_ext[None, :]
+            self.wx = (0.5 * hx_ext[:-1] / hxu)[:, None]
+            self.wy = (0.5 * hy_ext[:-1] / hyv)[None, :]
         self.px = self.kinds[0] == "periodic"
         self.py = self.kinds[2] == "periodic"
         if self.solid is not None:
@@ -78,13 +105,44 @@ class Case:
             if not self.solid.any():
                 self.solid = None
         # coordinates, mesh.py:353-375
-        dxc = np.full(self.nx, self.lx / self.nx)
-        dyc = np.full(self.ny, self.ly / self.ny)
-        self.xf = np.concatenate(([0.0], np.cumsum(dxc)))
-        self.yf = np.concatenate(([0.0], np.cumsum(dyc)))
+        self.xf = np.concatenate(([0.0], np.cumsum(self.dx_cells)))
+        self.yf = np.concatenate(([0.0], np.cumsum(self.dy_cells)))
         self.xf[-1], self.yf[-1] = self.lx, self.ly
         self.xc = 0.5 * (self.xf[:-1] + self.xf[1:])
         self.yc = 0.5 * (self.yf[:-1] + self.yf[1:])
+
+
+def _sl(value, lo, hi, axis):
+    """value[lo:hi] along axis, or a scalar unchanged (mesh.py:223-233)."""
+    if np.isscalar(value):
+        return value
+    return value[lo:hi, :] if axis == 0 else value[:, lo:hi]
+
+
+def geometric_widths(length: float, n: int, ratio: float, cluster: str = "low"):
+    """n widths growing by `ratio` per cell, summing to `length` (mesh.py:267-310):
+    smallest at the low end ("low"), at both ends ("walls") or in the middle
+    ("centre"); an odd count keeps one unpaired middle cell."""
+    if abs(ratio - 1.0) < 1.0e-14:
+        return np.full(n, length / n, dtype=float)
+    if cluster == "low":
+        w0 = length * (ratio - 1.0) / (ratio ** n - 1.0)
+        return w0 * ratio ** np.arange(n, dtype=float)
+    half, odd = divmod(n, 2)
+    if cluster == "walls":
+        run = ratio ** np.arange(half, dtype=float)
+        w0 = length / (2.0 * run.sum() + odd * ratio ** half)
+        run = w0 * run
+        middle = [w0 * ratio ** half] if odd else []
+        return np.concatenate([run, middle, run[::-1]])
+    if odd:
+        out = ratio ** np.arange(1, half + 1, dtype=float)
+        w0 = length / (1.0 + 2.0 * out.sum())
+        out = w0 * out
+        return np.concatenate([out[::-1], [w0], out])
+    run = ratio ** np.arange(half, dtype=float)
+    run = (length / (2.0 * run.sum())) * run
+    return np.concatenate([run[::-1], run])
 
 
 def wall_index(case: Case, w: int):
@@ -110,7 +168,60 @@ def _put(a, axis, idx, val):
 # --------------------------------------------------------------------------
 # Momentum right-hand side: solver.py:334-465, 501-529 (array path, uniform)
 # --------------------------------------------------------------------------
+def _momentum_rhs_stretched(case: Case, u, v, blend: float):
+    """solver.py:334-465 with the stretched metrics: weighted corner
+    interpolation, fluxes differenced over each face's own control volume,
+    and the viscous term as a difference of face gradients."""
+    nx, ny = case.nx, case.ny
+    wx, wy = case.wx, case.wy
+    u_lo, u_hi = u[1:nx + 2, 0:ny + 1], u[1:nx + 2, 1:ny + 2]
+    v_lo, v_hi = v[0:nx + 1, 1:ny + 2], v[1:nx + 2, 1:ny + 2]
+    u_c, v_c = u_lo + wy * (u_hi - u_lo), v_lo + wx * (v_hi - v_lo)
+    uv = u_c * v_c
+    u_cc = 0.5 * (u[0:nx + 2, :] + u[1:nx + 3, :])
+    v_cc = 0.5 * (v[:, 0:ny + 2] + v[:, 1:ny + 3])
+    uu_r, uu_l = u_cc[1:nx + 2, 1:ny + 1], u_cc[0:nx + 1, 1:ny + 1]
+    duudx = (uu_r ** 2 - uu_l ** 2) / case.hxu
+    duvdy = (uv[0:nx + 1, 1:ny + 1] - uv[0:nx + 1, 0:ny]) / case.hy
+    vv_t, vv_b = v_cc[1:nx + 1, 1:ny + 2], v_cc[1:nx + 1, 0:ny + 1]
+    dvvdy = (vv_t ** 2 - vv_b ** 2) / case.hyv
+    duvdx = (uv[1:nx + 1, 0:ny + 1] - uv[0:nx, 0:ny + 1]) / case.hx
+    if blend > 0.0:
+        um, u0, up = u[0:nx + 1, 1:ny + 1], u[1:nx + 2, 1:ny + 1], u[2:nx + 3, 1:ny + 1]
+        duudx += blend / case.hxu * (np.abs(uu_r) * (u0 - up) / 2.0
+                                     - np.abs(uu_l) * (um - u0) / 2.0)
+        ud, uc0, uu_ = u[1:nx + 2, 0:ny], u[1:nx + 2, 1:ny + 1], u[1:nx + 2, 2:ny + 2]
+        duvdy += blend / case.hy * (np.abs(v_c[0:nx + 1, 1:ny + 1]) * (uc0 - uu_) / 2.0
+                                    - np.abs(v_c[0:nx + 1, 0:ny]) * (ud - uc0) / 2.0)
+        vm, v0, vp = v[1:nx + 1, 0:ny + 1], v[1:nx + 1, 1:ny + 2], v[1:nx + 1, 2:ny + 3]
+        dvvdy += blend / case.hyv * (np.abs(vv_t) * (v0 - vp) / 2.0
+                                     - np.abs(vv_b) * (vm - v0) / 2.0)
+        vl, vc0, vr = v[0:nx, 1:ny + 2], v[1:nx + 1, 1:ny + 2], v[2:nx + 2, 1:ny + 2]
+        duvdx += blend / case.hx * (np.abs(u_c[1:nx + 1, 0:ny + 1]) * (vc0 - vr) / 2.0
+                                    - np.abs(u_c[0:nx, 0:ny + 1]) * (vl - vc0) / 2.0)
+    # viscous term, solver.py:446-465
+    u0 = u[1:nx + 2, 1:ny + 1]
+    gx_hi = (u[2:nx + 3, 1:ny + 1] - u0) / _sl(case.hx_ext, 1, nx + 2, 0)
+    gx_lo = (u0 - u[0:nx + 1, 1:ny + 1]) / _sl(case.hx_ext, 0, nx + 1, 0)
+    gy_hi = (u[1:nx + 2, 2:ny + 2] - u0) / _sl(case.hyv, 1, ny + 1, 1)
+    gy_lo = (u0 - u[1:nx + 2, 0:ny]) / _sl(case.hyv, 0, ny, 1)
+    d2u = (gx_hi - gx_lo) / case.hxu + (gy_hi - gy_lo) / case.hy
+    v0 = v[1:nx + 1, 1:ny + 2]
+    hx_hi = (v[2:nx + 2, 1:ny + 2] - v0) / _sl(case.hxu, 1, nx + 1, 0)
+    hx_lo = (v0 - v[0:nx, 1:ny + 2]) / _sl(case.hxu, 0, nx, 0)
+    hy_hi = (v[1:nx + 1, 2:ny + 3] - v0) / _sl(case.hy_ext, 1, ny + 2, 1)
+    hy_lo = (v0 - v[1:nx + 1, 0:ny + 1]) / _sl(case.hy_ext, 0, ny + 1, 1)
+    d2v = (hx_hi - hx_lo) / case.hx + (hy_hi - hy_lo) / case.hyv
+    fx, fy = case.body_force
+    return (-(duudx + duvdy) + case.nu * d2u + fx,
+            -(duvdx + dvvdy) + case.nu * d2v + fy)
+
+
 def momentum_rhs(case: Case, u, v, blend: float = 0.0):
+    if not case.uniform:
+        if case.use_les:
+            raise NotImplementedError("the LES closure is restated for uniform meshes")
+        return _momentum_rhs_stretched(case, u, v, blend)
     nx, ny, dx, dy = case.nx, case.ny, case.dx, case.dy
     uc = 0.5 * (u[1:nx + 2, 0:ny + 1] + u[1:nx + 2, 1:ny + 2])   # corner u
     vc = 0.5 * (v[0:nx + 1, 1:ny + 2] + v[1:nx + 2, 1:ny + 2])   # corner v
@@ -195,8 +306,8 @@ def diffusion_variable(case: Case, u, v, nu_c, nu_k):
 def divergence(case: Case, u, v):
     """solver.py:532-543."""
     nx, ny = case.nx, case.ny
-    return ((u[2:nx + 2, 1:ny + 1] - u[1:nx + 1, 1:ny + 1]) / case.dx
-            + (v[1:nx + 1, 2:ny + 2] - v[1:nx + 1, 1:ny + 1]) / case.dy)
+    return ((u[2:nx + 2, 1:ny + 1] - u[1:nx + 1, 1:ny + 1]) / case.hx
+            + (v[1:nx + 1, 2:ny + 2] - v[1:nx + 1, 1:ny + 1]) / case.hy)
 
 
 # --------------------------------------------------------------------------
@@ -272,8 +383,12 @@ def mass_balance(case: Case, u, v) -> None:
             continue
         axis, nf, *_, outward = wall_index(case, w)
         face = (u[nf, :][1:ny + 1]) if axis == 0 else (v[:, nf][1:nx + 1])
-        width = case.dy if axis == 0 else case.dx
-        flux = outward * (float(face.sum()) * width)
+        if case.uniform:
+            width = case.dy if axis == 0 else case.dx
+            flux = outward * (float(face.sum()) * width)
+        else:  # boundary.py:438-459: height-weighted face sums
+            width = case.dy_cells if axis == 0 else case.dx_cells
+            flux = outward * float((face * width).sum())
         if kind == "outlet":
             outflux += flux
             faces.append((face, width, outward))
@@ -283,7 +398,8 @@ def mass_balance(case: Case, u, v) -> None:
         for face, _, _ in faces:
             face *= influx / outflux
     else:
-        area = sum(face.size * width for face, width, _ in faces)
+        area = sum(face.size * width if case.uniform else float(width.sum())
+                   for face, width, _ in faces)
         for face, _, outward in faces:
             face += outward * (influx - outflux) / area
 
@@ -342,9 +458,15 @@ def mask(case: Case, st: State, u, v, dt, reset: bool = False) -> None:
     uf = u[u_lo:u_hi, 1:case.ny + 1]
     vf = v[1:case.nx + 1, v_lo:v_hi]
     if dt is not None:
-        area = case.dx * case.dy
-        fx = float(uf[um].sum()) * area / dt
-        fy = float(vf[vm].sum()) * area / dt
+        if case.uniform:
+            area = case.dx * case.dy
+            fx = float(uf[um].sum()) * area / dt
+            fy = float(vf[vm].sum()) * area / dt
+        else:  # solver.py:197-207, 271-274: control-volume weighted sums
+            ua = _sl(case.hxu, u_lo - 1, u_hi - 1, 0) * case.hy
+            va = case.hx * _sl(case.hyv, v_lo - 1, v_hi - 1, 1)
+            fx = float((uf * ua)[um].sum()) / dt
+            fy = float((vf * va)[vm].sum()) / dt
         st.force = (fx, fy) if reset else (st.force[0] + fx, st.force[1] + fy)
     uf[um] = 0.0
     vf[vm] = 0.0
@@ -387,6 +509,7 @@ class System:
     singular: bool
     pin: int
     bc_rhs: np.ndarray
+    row_scale: object = 1.0
 
 
 def assemble(case: Case) -> System:
@@ -394,14 +517,31 @@ def assemble(case: Case) -> System:
     n = nx * ny
     solid = np.zeros((nx, ny), bool) if case.solid is None else case.solid
     fluid = ~solid
-    cx, cy = 1.0 / (case.dx * case.dx), 1.0 / (case.dy * case.dy)
     dirichlet = {w: case.p_ref[w] for w in range(4) if case.kinds[w] == "pressure_outlet"}
+    if case.uniform:
+        cx, cy = 1.0 / (case.dx * case.dx), 1.0 / (case.dy * case.dy)
+        coeffs = (cx, cx, cy, cy)
+        row_scale = 1.0
+    else:
+        # pressure.py:193-207: face area / centre distance / mean cell area,
+        # rows scaled by cell area / mean cell area (symmetric operator)
+        vbar = float(case.cell_areas.mean())
+        hx, hy = case.dx_cells[:, None], case.dy_cells[None, :]
+        hxu = 0.5 * (np.concatenate(([case.dx_cells[0]], case.dx_cells))
+                     + np.concatenate((case.dx_cells, [case.dx_cells[-1]])))
+        hyv = 0.5 * (np.concatenate(([case.dy_cells[0]], case.dy_cells))
+                     + np.concatenate((case.dy_cells, [case.dy_cells[-1]])))
+        cxf = hy / (hxu[:, None] * vbar)
+        cyf = hx / (hyv[None, :] * vbar)
+        coeffs = (cxf[0:nx, :], cxf[1:nx + 1, :], cyf[:, 0:ny], cyf[:, 1:ny + 1])
+        row_scale = (case.cell_areas / vbar).ravel()
     ii, jj = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
     diag = np.zeros((nx, ny))
     bc = np.zeros((nx, ny))
     rows, cols, vals = [], [], []
-    for w, (di, dj, c, per) in enumerate(((-1, 0, cx, case.px), (1, 0, cx, case.px),
-                                          (0, -1, cy, case.py), (0, 1, cy, case.py))):
+    for w, (di, dj, per) in enumerate(((-1, 0, case.px), (1, 0, case.px),
+                                       (0, -1, case.py), (0, 1, case.py))):
+        c = coeffs[w]
         ni, nj = ii + di, jj + dj
         if per:
             ni, nj = ni % nx, nj % ny
@@ -410,7 +550,7 @@ def assemble(case: Case) -> System:
         act = fluid & inside & fluid[nic, njc]
         rows.append((ii * ny + jj)[act])
         cols.append((nic * ny + njc)[act])
-        vals.append(np.full(int(act.sum()), c))
+        vals.append(np.full(int(act.sum()), c) if np.isscalar(c) else c[act])
         diag -= c * act
         if w in dirichlet:
             on = fluid & ~inside
@@ -427,12 +567,12 @@ def assemble(case: Case) -> System:
     A.sum_duplicates()
     f = solv.ravel()
     pin = int(np.flatnonzero(f)[0]) if f.any() else 0
-    return System(A, f, not dirichlet, pin, bc.ravel())
+    return System(A, f, not dirichlet, pin, bc.ravel(), row_scale)
 
 
 def prepare_rhs(sysm: System, rhs):
     """pressure.py:300-308."""
-    b = np.asarray(rhs, float).ravel() * 1.0 + sysm.bc_rhs
+    b = np.asarray(rhs, float).ravel() * sysm.row_scale + sysm.bc_rhs
     b[~sysm.fluid] = 0.0
     if sysm.singular:
         b[sysm.fluid] -= b[sysm.fluid].mean()
@@ -685,8 +825,10 @@ def project(case: Case, st: State, u, v, dt):
     p[1:nx + 1, 1:ny + 1] = p_int
     apply_pressure(case, p)
     u_lo, u_hi, v_lo, v_hi = upd_ranges(case)
-    u[u_lo:u_hi, 1:ny + 1] -= dt * (p[u_lo:u_hi, 1:ny + 1] - p[u_lo - 1:u_hi - 1, 1:ny + 1]) / case.dx
-    v[1:nx + 1, v_lo:v_hi] -= dt * (p[1:nx + 1, v_lo:v_hi] - p[1:nx + 1, v_lo - 1:v_hi - 1]) / case.dy
+    u[u_lo:u_hi, 1:ny + 1] -= dt * (p[u_lo:u_hi, 1:ny + 1] - p[u_lo - 1:u_hi - 1, 1:ny + 1]) \
+        / _sl(case.hxu, u_lo - 1, u_hi - 1, 0)
+    v[1:nx + 1, v_lo:v_hi] -= dt * (p[1:nx + 1, v_lo:v_hi] - p[1:nx + 1, v_lo - 1:v_hi - 1]) \
+        / _sl(case.hyv, v_lo - 1, v_hi - 1, 1)
     return p
 
 
@@ -755,9 +897,9 @@ def compute_dt(case: Case, u, v) -> float:
     nx, ny = case.nx, case.ny
     umax = float(np.abs(u[1:nx + 2, 1:ny + 1]).max())
     vmax = float(np.abs(v[1:nx + 1, 1:ny + 2]).max())
-    scale = umax / case.dx + vmax / case.dy
+    scale = umax / case.min_hx + vmax / case.min_hy
     dt_conv = case.cfl_max / scale if scale > 1e-12 else float("inf")
-    visc = 0.8 / (2.0 * case.nu * (1.0 / case.dx ** 2 + 1.0 / case.dy ** 2))
+    visc = 0.8 / (2.0 * case.nu * (1.0 / case.min_hx ** 2 + 1.0 / case.min_hy ** 2))
     return float(min(dt_conv, visc, case.dt))
 
 
@@ -778,7 +920,8 @@ def run(case: Case, steps: int, u_init=None, v_init=None, st: State | None = Non
             else:
                 umax = float(np.abs(u[1:nx + 2, 1:ny + 1]).max())
                 vmax = float(np.abs(v[1:nx + 1, 1:ny + 2]).max())
-                blend = float(min(1.0, 1.2 * max(umax * dt / case.dx, vmax * dt / case.dy)))
+                blend = float(min(1.0, 1.2 * max(umax * dt / case.min_hx,
+                                                 vmax * dt / case.min_hy)))
         u, v, p = step(case, st, u, v, dt, blend)
         t += dt
         d = divergence(case, u, v)
@@ -788,7 +931,9 @@ def run(case: Case, steps: int, u_init=None, v_init=None, st: State | None = Non
         vc = 0.5 * (v[1:nx + 1, 1:ny + 1] + v[1:nx + 1, 2:ny + 2])
         hist["dt"].append(dt)
         hist["max_div"].append(float(np.abs(d).max()))
-        hist["kinetic_energy"].append(0.5 * (float(np.sum(uc ** 2 + vc ** 2)) * (case.dx * case.dy)))
+        ke2 = (float(np.sum(uc ** 2 + vc ** 2)) * (case.dx * case.dy) if case.uniform
+               else float(np.sum((uc ** 2 + vc ** 2) * case.cell_areas)))
+        hist["kinetic_energy"].append(0.5 * ke2)
         hist["fx"].append(st.force[0])
         hist["fy"].append(st.force[1])
     return u, v, p, hist, t

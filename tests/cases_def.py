@@ -28,31 +28,57 @@ CASES = {
 }
 
 
-def circle(nx, ny, lx, ly, cx, cy, r, sub=8):
+# Stretched variants of the cases above: base case plus stretching
+# (tests/golden/make_golden.py records the same four).
+STRETCHED = {
+    "s_cavity": dict(base="cavity", stretch_x=1.06, stretch_y=1.04, cluster_x="walls",
+                     cluster_y="walls"),
+    "s_channel": dict(base="channel", stretch_y=1.08, cluster_y="walls"),
+    "s_cyl": dict(base="cyl_outlet", stretch_x=1.03, stretch_y=1.05, cluster_x="low",
+                  cluster_y="centre"),
+    "s_cylp": dict(base="cyl_pout", stretch_y=1.05, cluster_y="centre"),
+}
+
+
+def _split(name):
+    if name in STRETCHED:
+        spec = dict(STRETCHED[name])
+        return spec.pop("base"), spec
+    return name, {}
+
+
+def circle(nx, ny, lx, ly, cx, cy, r, sub=8, dx_cells=None, dy_cells=None):
     """Solid iff more than half of 8x8 mid-point samples lie in the circle
     (obstacles.py:87-135)."""
     offs = (np.arange(sub) + 0.5) / sub
-    dx, dy = lx / nx, ly / ny
-    xf = np.concatenate(([0.0], np.cumsum(np.full(nx, dx))))
-    yf = np.concatenate(([0.0], np.cumsum(np.full(ny, dy))))
+    dxc = np.full(nx, lx / nx) if dx_cells is None else np.asarray(dx_cells)
+    dyc = np.full(ny, ly / ny) if dy_cells is None else np.asarray(dy_cells)
+    xf = np.concatenate(([0.0], np.cumsum(dxc)))
+    yf = np.concatenate(([0.0], np.cumsum(dyc)))
     xf[-1], yf[-1] = lx, ly
-    xs = (xf[:-1, None] + offs[None, :] * np.full(nx, dx)[:, None]).ravel()
-    ys = (yf[:-1, None] + offs[None, :] * np.full(ny, dy)[:, None]).ravel()
+    xs = (xf[:-1, None] + offs[None, :] * dxc[:, None]).ravel()
+    ys = (yf[:-1, None] + offs[None, :] * dyc[:, None]).ravel()
     X, Y = np.meshgrid(xs, ys, indexing="ij")
     hits = ((X - cx) ** 2 + (Y - cy) ** 2 <= r ** 2).astype(float)
     return hits.reshape(nx, sub, ny, sub).mean(axis=(1, 3)) > 0.5
 
 
 def oracle_case(name: str, **kw) -> Case:
-    args = dict(CASES[name])
+    base, stretch = _split(name)
+    args = dict(CASES[base])
+    args.update(stretch)
     args.update(kw)
-    if name.startswith("cyl"):
-        args["solid"] = circle(args["nx"], args["ny"], args["lx"], args["ly"], 3.0, 3.0, 0.5)
-    return Case(**args)
+    case = Case(**args)
+    if base.startswith("cyl"):
+        case = Case(**args, solid=circle(args["nx"], args["ny"], args["lx"], args["ly"], 3.0,
+                                         3.0, 0.5, dx_cells=case.dx_cells,
+                                         dy_cells=case.dy_cells))
+    return case
 
 
 def initial(name: str, case: Case):
     """Initial faces used by make_golden.py for the trajectory runs."""
+    name = _split(name)[0]
     if name.startswith("cyl") or name == "channel":
         Xv, Yv = np.meshgrid(case.xc, case.yf, indexing="ij")
         v0 = 0.05 * np.exp(-((Xv - 4.0) ** 2 + (Yv - 3.0) ** 2))
@@ -69,7 +95,8 @@ def product_config(name: str, **kw):
     from pycfd_amd.config import BCKind, BCSpec, SimulationConfig
     from pycfd_amd.core.mesh import StructuredMesh
     from pycfd_amd.geometry.obstacles import circle_mask
-    c = CASES[name]
+    base, stretch = _split(name)
+    c = CASES[base]
     walls = ("left", "right", "bottom", "top")
     prof = c.get("profile", ("uniform",) * 4)
     vel = c.get("velocity", (0.0,) * 4)
@@ -78,9 +105,10 @@ def product_config(name: str, **kw):
     args = dict(nx=c["nx"], ny=c["ny"], lx=c["lx"], ly=c["ly"], re=c["re"], dt=c["dt"],
                 cfl_max=c.get("cfl_max", 0.5), adaptive_dt=c.get("adaptive_dt", True),
                 t_end=100.0, boundary_config=bcs, name=name)
+    args.update(stretch)
     args.update(kw)
     cfg = SimulationConfig(**args)
     body = None
-    if name.startswith("cyl"):
+    if base.startswith("cyl"):
         body = circle_mask(StructuredMesh.from_config(cfg), (3.0, 3.0), 0.5)
     return cfg, body

@@ -225,6 +225,71 @@ def les_fixture():
     np.savez_compressed(OUT / "les.npz", **out)
 
 
+STRETCHED = {
+    "s_cavity": dict(base="cavity", stretch_x=1.06, stretch_y=1.04, cluster_x="walls",
+                     cluster_y="walls"),
+    "s_channel": dict(base="channel", stretch_y=1.08, cluster_y="walls"),
+    "s_cyl": dict(base="cyl_outlet", stretch_x=1.03, stretch_y=1.05, cluster_x="low",
+                  cluster_y="centre"),
+    "s_cylp": dict(base="cyl_pout", stretch_y=1.05, cluster_y="centre"),
+}
+
+
+def stretched_config(name, **kw):
+    spec = dict(STRETCHED[name])
+    base = spec.pop("base")
+    spec.update(kw)
+    return config(base, **spec), base
+
+
+def stretched_fixture():
+    """Stretched meshes (mesh.py:122-310): momentum RHS and divergence on
+    random fields, the predictor boundary pass with its force, the Poisson
+    operator, and short trajectories (solver.py:334-683, pressure.py:193-207)."""
+    from pycfd.core.mesh import StructuredMesh
+    rng = np.random.default_rng(2024)
+    out = {}
+    for name in STRETCHED:
+        cfg, base = stretched_config(name, use_numba=False)
+        mesh = StructuredMesh.from_config(cfg)
+        body = body_for(base, mesh)
+        solver = ProjectionSolver(cfg, mesh, body.mask if body else None)
+        nx, ny = mesh.shape
+        u = rng.standard_normal((nx + 3, ny + 2))
+        v = rng.standard_normal((nx + 2, ny + 3))
+        out[f"{name}_u"], out[f"{name}_v"] = u, v
+        out[f"{name}_dx_cells"], out[f"{name}_dy_cells"] = mesh.dx_cells, mesh.dy_cells
+        for blend in (0.0, 0.7):
+            ru, rv = solver.momentum_rhs(u, v, blend)
+            out[f"{name}_b{int(blend * 10)}_rhs_u"] = ru.copy()
+            out[f"{name}_b{int(blend * 10)}_rhs_v"] = rv.copy()
+        out[f"{name}_div"] = solver.divergence(u, v)
+        from pycfd.core.fields import FlowField
+        us, vs = u.copy(), v.copy()
+        fld = FlowField(mesh, us, vs, mesh.zeros_p())
+        solver.boundaries.apply_velocity(fld, predictor=True)
+        solver.boundaries.enforce_global_mass_balance(fld)
+        solver._mask_obstacle(us, vs, 0.01, reset=True)
+        out[f"{name}_pass_u"], out[f"{name}_pass_v"] = us, vs
+        out[f"{name}_pass_force"] = np.array(solver.body_force_reaction)
+        A = solver.poisson.A
+        x = rng.standard_normal(A.shape[0])
+        out[f"{name}_Ax_x"], out[f"{name}_Ax"] = x, A @ x
+        out[f"{name}_row_scale"] = np.asarray(solver.poisson.row_scale, dtype=float)
+        out[f"{name}_bc_rhs"] = solver.poisson.bc_rhs
+        steps = 8
+        cfg2, _ = stretched_config(name, pressure_solver="direct", poisson_tol=1e-12,
+                                   use_numba=False)
+        u0, v0 = init_for(base, mesh)
+        sim = Simulation(cfg2, obstacle=body, u_init=u0, v_init=v0)
+        res = sim.run(max_steps=steps)
+        out[f"traj_{name}_u"], out[f"traj_{name}_v"] = sim.fields.u, sim.fields.v
+        out[f"traj_{name}_p"] = sim.fields.p
+        for k in ("dt", "max_div", "kinetic_energy", "fx", "fy"):
+            out[f"traj_{name}_{k}"] = np.asarray(res.history[k])
+    np.savez_compressed(OUT / "stretched.npz", **out)
+
+
 def mask_fixture():
     from pycfd.core.mesh import StructuredMesh
     out = {}
@@ -237,8 +302,10 @@ def mask_fixture():
 
 
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["rhs", "boundary", "trajectory", "multigrid", "mask", "les"]
+    which = sys.argv[1:] or ["rhs", "boundary", "trajectory", "multigrid", "mask", "les",
+                             "stretched"]
     for w in which:
         print("writing", w, flush=True)
         {"rhs": rhs_fixture, "boundary": boundary_fixture, "trajectory": trajectory_fixture,
-         "multigrid": multigrid_fixture, "mask": mask_fixture, "les": les_fixture}[w]()
+         "multigrid": multigrid_fixture, "mask": mask_fixture, "les": les_fixture,
+         "stretched": stretched_fixture}[w]()
