@@ -86,6 +86,16 @@ typedef struct pcfd_case_desc {
   int32_t n_bodies;             /* > 1 enables per-body forces (solver.py:136-146) */
   int32_t use_les;              /* Smagorinsky closure (config.py use_les, solver.py:164-167) */
   double les_coeff;             /* (Cs * sqrt(dx dy))^2 (turbulence.py:45-52) */
+  /* Stretched mesh (mesh.py:122-233, 269-311): the nx cell widths and ny
+     cell heights, both NULL for the uniform mesh of dx, dy.  With them every
+     operator takes the reference's array-metric form.  area_mean is the mean
+     cell area as numpy computes it (pressure.py:197); les_cs the Smagorinsky
+     constant, whose filter width is then per cell (turbulence.py:47-58).  A
+     stretched axis may not be periodic (solver.py:209-230). */
+  const double* dx_cells;
+  const double* dy_cells;
+  double area_mean;
+  double les_cs;
 } pcfd_case_desc;
 
 typedef struct pcfd_solver_desc {
@@ -129,6 +139,8 @@ typedef struct pcfd_mg_level_desc {
   double weight;               /* damped-Jacobi weight (multigrid.py:354-362) */
   int32_t coarse_n;
   const double* coarse_inv;
+  const uint32_t* cls32;       /* (nx*ny) in place of cls when n_classes > 65535
+                                  (stretched meshes); NULL otherwise */
 } pcfd_mg_level_desc;
 
 typedef struct pcfd_solve_info {

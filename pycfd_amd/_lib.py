@@ -53,6 +53,9 @@ class CaseDesc(C.Structure):
         ("n_bodies", C.c_int32),
         ("use_les", C.c_int32),
         ("les_coeff", C.c_double),
+        ("dx_cells", _dp), ("dy_cells", _dp),
+        ("area_mean", C.c_double),
+        ("les_cs", C.c_double),
     ]
 
 
@@ -72,6 +75,7 @@ class MGLevelDesc(C.Structure):
         ("n_classes", C.c_int32), ("cls", C.POINTER(C.c_uint16)), ("table", _dp),
         ("class_dinv", _dp), ("class_active", _u8p), ("active", _u8p),
         ("weight", C.c_double), ("coarse_n", C.c_int32), ("coarse_inv", _dp),
+        ("cls32", C.POINTER(C.c_uint32)),
     ]
 
 

@@ -14,11 +14,12 @@ import numpy as np
 
 
 def vorticity_corner(fields) -> np.ndarray:
+    """``dv/dx - du/dy`` at the corners, each derivative over the centre-to-centre span."""
     nx, ny = fields.mesh.shape
-    dx, dy = fields.mesh.dx, fields.mesh.dy
+    m = fields.mesh.metrics
     u, v = fields.u, fields.v
-    dvdx = (v[1:nx + 2, 1:ny + 2] - v[0:nx + 1, 1:ny + 2]) / dx
-    dudy = (u[1:nx + 2, 1:ny + 2] - u[1:nx + 2, 0:ny + 1]) / dy
+    dvdx = (v[1:nx + 2, 1:ny + 2] - v[0:nx + 1, 1:ny + 2]) / m.hxu
+    dudy = (u[1:nx + 2, 1:ny + 2] - u[1:nx + 2, 0:ny + 1]) / m.hyv
     return dvdx - dudy
 
 

@@ -33,7 +33,9 @@ class DeviceContext:
                  bc_kinds=("no_slip",) * 4, bc_velocity=(0.0,) * 4, bc_p_ref=(0.0,) * 4,
                  inlet_lines=(None,) * 4, solid: np.ndarray | None = None,
                  body_id: np.ndarray | None = None, n_bodies: int = 0,
-                 les_coeff: float | None = None) -> None:
+                 les_coeff: float | None = None, *, les_cs: float = 0.0,
+                 dx_cells: np.ndarray | None = None, dy_cells: np.ndarray | None = None,
+                 area_mean: float | None = None) -> None:
         require_device()
         import torch
         self._lib = _lib.load()
@@ -65,6 +67,19 @@ class DeviceContext:
         self.n_bodies = int(n_bodies) if n_bodies > 1 else 0
         d.use_les = int(les_coeff is not None)
         d.les_coeff = float(les_coeff or 0.0)
+        d.les_cs = float(les_cs)
+        self.stretched = dx_cells is not None
+        if self.stretched:
+            if dy_cells is None or area_mean is None:
+                raise ValueError("a stretched mesh needs dx_cells, dy_cells and area_mean")
+            wx = np.ascontiguousarray(dx_cells, dtype=np.float64)
+            wy = np.ascontiguousarray(dy_cells, dtype=np.float64)
+            if wx.shape != (self.nx,) or wy.shape != (self.ny,):
+                raise ValueError(f"cell widths must have shapes ({self.nx},), ({self.ny},)")
+            self._keep += [wx, wy]
+            d.dx_cells = wx.ctypes.data_as(C.POINTER(C.c_double))
+            d.dy_cells = wy.ctypes.data_as(C.POINTER(C.c_double))
+            d.area_mean = float(area_mean)
         self.stream = torch.cuda.current_stream()
         handle = C.c_void_p(0)
         _lib.check(self._lib.pcfd_ctx_create(C.byref(d), C.c_void_p(self.stream.cuda_stream),
@@ -106,7 +121,8 @@ class DeviceContext:
                 if L.get("table") is not None:
                     di = np.ascontiguousarray(L["di"], dtype=np.int8)
                     dj = np.ascontiguousarray(L["dj"], dtype=np.int8)
-                    cls = np.ascontiguousarray(L["cls"], dtype=np.uint16)
+                    wide = np.asarray(L["cls"]).dtype == np.uint32
+                    cls = np.ascontiguousarray(L["cls"], dtype=np.uint32 if wide else np.uint16)
                     tab = np.ascontiguousarray(L["table"], dtype=np.float64)
                     cdv = np.ascontiguousarray(L["class_dinv"], dtype=np.float64)
                     cac = np.ascontiguousarray(L["class_active"], dtype=np.uint8)
@@ -115,7 +131,10 @@ class DeviceContext:
                     e.di = di.ctypes.data_as(C.POINTER(C.c_int8))
                     e.dj = dj.ctypes.data_as(C.POINTER(C.c_int8))
                     e.n_classes = cdv.size
-                    e.cls = cls.ctypes.data_as(C.POINTER(C.c_uint16))
+                    if wide:
+                        e.cls32 = cls.ctypes.data_as(C.POINTER(C.c_uint32))
+                    else:
+                        e.cls = cls.ctypes.data_as(C.POINTER(C.c_uint16))
                     e.table = tab.ctypes.data_as(C.POINTER(C.c_double))
                     e.class_dinv = cdv.ctypes.data_as(C.POINTER(C.c_double))
                     e.class_active = cac.ctypes.data_as(C.POINTER(C.c_uint8))

@@ -3,7 +3,9 @@
 Index convention and coordinate arithmetic follow ``pycfd/core/mesh.py``
 (layout :14-31, widths :269-311, faces and centres :353-375, shapes
 :483-508) so that initial conditions and body masks evaluated on this mesh are
-bit-identical to the reference's.  Device kernels only need ``dx``, ``dy``.
+bit-identical to the reference's.  A uniform mesh hands the device its scalar
+``dx``, ``dy``; a stretched one its cell widths, from which the library forms
+the :class:`MeshMetrics` arrays with the same elementwise arithmetic.
 """
 
 from __future__ import annotations
@@ -46,6 +48,55 @@ def _geometric_widths(length: float, n: int, ratio: float, cluster: str = "low")
     return np.concatenate([run[::-1], run])
 
 
+class MeshMetrics:
+    """The spacings the staggered operators divide by (mesh.py:122-233).
+
+    Plain floats on a uniform mesh, broadcastable arrays (``(n, 1)`` for x,
+    ``(1, n)`` for y) on a stretched one: ``hx, hy`` cell widths, ``hxu, hyv``
+    centre-to-centre spans across each face (ghost cells mirror their
+    neighbour, so ``hxu[0] == hx[0]``), ``hx_ext, hy_ext`` widths with those
+    ghosts, ``wx_corner, wy_corner`` weights of the upper node at a corner,
+    ``cell_areas``, ``min_hx, min_hy``.
+    """
+
+    __slots__ = ("hx", "hy", "hxu", "hyv", "hx_ext", "hy_ext", "wx_corner", "wy_corner",
+                 "cell_areas", "min_hx", "min_hy", "uniform_area")
+
+    def __init__(self, mesh: "StructuredMesh") -> None:
+        dxc, dyc = mesh.dx_cells, mesh.dy_cells
+        hx_ext = np.concatenate(([dxc[0]], dxc, [dxc[-1]]))
+        hy_ext = np.concatenate(([dyc[0]], dyc, [dyc[-1]]))
+        hxu = 0.5 * (hx_ext[:-1] + hx_ext[1:])
+        hyv = 0.5 * (hy_ext[:-1] + hy_ext[1:])
+        self.cell_areas = dxc[:, None] * dyc[None, :]
+        self.min_hx = float(dxc.min())
+        self.min_hy = float(dyc.min())
+        self.uniform_area = float(dxc[0] * dyc[0]) if mesh.is_uniform else None
+        if mesh.is_uniform:
+            self.hx = self.hxu = self.hx_ext = float(dxc[0])
+            self.hy = self.hyv = self.hy_ext = float(dyc[0])
+            self.wx_corner = self.wy_corner = 0.5
+        else:
+            self.hx, self.hy = dxc[:, None], dyc[None, :]
+            self.hxu, self.hyv = hxu[:, None], hyv[None, :]
+            self.hx_ext, self.hy_ext = hx_ext[:, None], hy_ext[None, :]
+            self.wx_corner = (0.5 * hx_ext[:-1] / hxu)[:, None]
+            self.wy_corner = (0.5 * hy_ext[:-1] / hyv)[None, :]
+
+    def integrate(self, field: np.ndarray) -> float:
+        """Domain integral of a cell-centred ``(nx, ny)`` field (mesh.py:213-223)."""
+        if self.uniform_area is not None:
+            return float(np.sum(field)) * self.uniform_area
+        return float(np.sum(field * self.cell_areas))
+
+    @staticmethod
+    def _slice(value, lo: int, hi: int, axis: int):
+        """``value[lo:hi]`` along ``axis``, or the scalar unchanged (mesh.py:225-233)."""
+        if np.isscalar(value):
+            return value
+        return value[lo:hi, :] if axis == 0 else value[:, lo:hi]
+
+
 class StructuredMesh:
     """Cartesian mesh on ``[0, lx] x [0, ly]`` with ``nx x ny`` cells."""
 
@@ -76,6 +127,7 @@ class StructuredMesh:
         self._is_uniform = not (self.stretched_x or self.stretched_y)
         self._dx = float(self.dx_cells[0])
         self._dy = float(self.dy_cells[0])
+        self._metrics: MeshMetrics | None = None
 
     @classmethod
     def from_config(cls, cfg) -> "StructuredMesh":
@@ -85,6 +137,17 @@ class StructuredMesh:
     @property
     def is_uniform(self) -> bool:
         return self._is_uniform
+
+    @property
+    def metrics(self) -> MeshMetrics:
+        if self._metrics is None:
+            self._metrics = MeshMetrics(self)
+        return self._metrics
+
+    @property
+    def area_mean(self) -> float:
+        """Mean cell area, the Poisson scaling of a stretched mesh (pressure.py:197)."""
+        return float(self.metrics.cell_areas.mean())
 
     def require_uniform(self, who: str = "operator") -> None:
         if not self._is_uniform:
@@ -160,9 +223,14 @@ class StructuredMesh:
         return np.zeros(self.p_shape)
 
     def integrate(self, field: np.ndarray) -> float:
-        """Domain integral of a cell-centred field (mesh.py:213-223, uniform)."""
-        return float(np.sum(field)) * (self._dx * self._dy)
+        """Domain integral of a cell-centred field (mesh.py:213-223)."""
+        return self.metrics.integrate(field)
 
     def __repr__(self) -> str:
+        if not self._is_uniform:
+            return (f"StructuredMesh({self.nx}x{self.ny} cells, domain "
+                    f"[0,{self.lx:g}]x[0,{self.ly:g}], stretched: dx in "
+                    f"[{self.dx_cells.min():.6g}, {self.dx_cells.max():.6g}], dy in "
+                    f"[{self.dy_cells.min():.6g}, {self.dy_cells.max():.6g}])")
         return (f"StructuredMesh({self.nx}x{self.ny} cells, domain "
                 f"[0,{self.lx:g}]x[0,{self.ly:g}], dx={self._dx:.6g}, dy={self._dy:.6g})")

@@ -129,12 +129,14 @@ class Level:
 
 
 def to_class_stencil(A: sp.csr_matrix, shape: tuple[int, int], periodic: tuple[bool, bool],
-                     active: np.ndarray):
+                     active: np.ndarray, wide: bool = False):
     """CSR operator on a rectangular grid -> stencil dictionary.
 
     Returns offsets ``(di, dj)``, the per-cell class index, the class table
     (one row of coefficients per distinct operator row) and, per class, the
-    diagonal inverse and the active flag.
+    diagonal inverse and the active flag.  The class index is ``uint16``, or
+    ``uint32`` past 65535 classes (a stretched mesh, whose rows all differ)
+    or when ``wide`` asks for it.
     """
     nx, ny = shape
     coo = A.tocoo()
@@ -159,13 +161,15 @@ def to_class_stencil(A: sp.csr_matrix, shape: tuple[int, int], periodic: tuple[b
     rows[r, inv] = v
     rows[:, S] = np.asarray(active, dtype=float).ravel()
     table, cls = np.unique(rows, axis=0, return_inverse=True)
-    if table.shape[0] > 65535:
+    if table.shape[0] >= 2 ** 32:
         raise ValueError(f"{table.shape[0]} distinct operator rows exceed the class index range")
+    wide = wide or table.shape[0] > 65535
     centre = int(np.flatnonzero((udi == 0) & (udj == 0))[0])
     diag = table[:, centre]
     cdinv = 1.0 / np.where(diag == 0.0, 1.0, diag)
     cactive = table[:, S] != 0.0
-    return (udi.astype(np.int8), udj.astype(np.int8), cls.ravel().astype(np.uint16),
+    return (udi.astype(np.int8), udj.astype(np.int8),
+            cls.ravel().astype(np.uint32 if wide else np.uint16),
             np.ascontiguousarray(table[:, :S]), cdinv, cactive.astype(np.uint8))
 
 
@@ -219,7 +223,7 @@ class MultigridHierarchy:
         self.levels: list[Level] = []
         self._build(sp.csr_matrix(A), tuple(shape), np.asarray(active, dtype=bool).ravel(),
                     bool(singular))
-        self._device: dict[int, list[dict]] = {}
+        self._device: dict[tuple[int, bool], list[dict]] = {}
 
     def _make_level(self, A, shape, active, singular) -> Level:
         diag = A.diagonal()
@@ -267,20 +271,24 @@ class MultigridHierarchy:
         return (f"MultigridHierarchy({self.n_levels} levels, {self.unknowns[0]} -> "
                 f"{self.unknowns[-1]} unknowns, complexity {self.operator_complexity:.2f})")
 
-    def device_levels(self, coarse_max: int = COARSEST_UNKNOWNS) -> list[dict]:
+    def device_levels(self, coarse_max: int = COARSEST_UNKNOWNS,
+                      wide_classes: bool = False) -> list[dict]:
         """The hierarchy in the form ``pcfd_set_pressure_solver`` takes.
 
         The device solves the first level with at most ``coarse_max`` unknowns
         exactly (its dense pseudo-inverse) and drops the levels below it;
         ``coarse_max = COARSEST_UNKNOWNS`` keeps the reference hierarchy whole.
+        ``wide_classes`` hands every stored level over with the 32-bit class
+        index that levels past 65535 distinct rows need anyway.
         """
         m = len(self.levels) - 1
         for l, lv in enumerate(self.levels):
             if lv.n <= coarse_max:
                 m = l
                 break
-        if m in self._device:
-            return self._device[m]
+        key = (m, bool(wide_classes))
+        if key in self._device:
+            return self._device[key]
         out = []
         for l, lv in enumerate(self.levels[:m + 1]):
             last = l == m
@@ -289,7 +297,7 @@ class MultigridHierarchy:
                  "weight": lv.weight, "active": lv.active.astype(np.uint8)}
             if l > 0:
                 di, dj, cls, table, cdinv, cact = to_class_stencil(
-                    lv.A, lv.shape, self.periodic, lv.active)
+                    lv.A, lv.shape, self.periodic, lv.active, wide=wide_classes)
                 d.update(di=di, dj=dj, cls=cls, table=table, class_dinv=cdinv,
                          class_active=cact)
             if last:
@@ -301,7 +309,7 @@ class MultigridHierarchy:
                                       if lv.active.any() else 0)
                 d["coarse_inv"] = coarsest_pseudo_inverse(pin_level)
             out.append(d)
-        self._device[m] = out
+        self._device[key] = out
         return out
 
     def stencil_sizes(self) -> list[tuple[int, int]]:

@@ -44,13 +44,17 @@ class PoissonSystem:
     singular: bool
     pin: int
     bc_rhs: np.ndarray = None  # type: ignore[assignment]
-    row_scale: float = 1.0
+    row_scale: float | np.ndarray = 1.0
     dx: float = 1.0
     dy: float = 1.0
     periodic_x: bool = False
     periodic_y: bool = False
     solid: np.ndarray | None = None
     dirichlet: dict = field(default_factory=dict)
+    # stretched mesh: cell widths and mean cell area (None on a uniform mesh)
+    dx_cells: np.ndarray | None = None
+    dy_cells: np.ndarray | None = None
+    area_mean: float | None = None
 
     def __post_init__(self) -> None:
         if self.bc_rhs is None:
@@ -64,8 +68,12 @@ class PoissonSystem:
 def assemble_poisson_matrix(mesh: StructuredMesh, periodic_x: bool = False,
                             periodic_y: bool = False, solid_mask: np.ndarray | None = None,
                             dirichlet: dict[str, float] | None = None) -> PoissonSystem:
-    """5-point ``div(grad)`` over the correctable faces (pressure.py:110-270)."""
-    mesh.require_uniform("assemble_poisson_matrix")
+    """5-point ``div(grad)`` over the correctable faces (pressure.py:110-270).
+
+    On a stretched mesh each face carries face area / centre distance / mean
+    cell area and each row is scaled by its cell area / mean cell area
+    (pressure.py:193-207), which keeps the operator symmetric.
+    """
     nx, ny = mesh.shape
     n = nx * ny
     solid = (np.zeros((nx, ny), dtype=bool) if solid_mask is None
@@ -74,16 +82,28 @@ def assemble_poisson_matrix(mesh: StructuredMesh, periodic_x: bool = False,
         raise ValueError(f"solid_mask must have shape {(nx, ny)}, got {solid.shape}")
     fluid = ~solid
     dirichlet = dict(dirichlet or {})
-    dx, dy = mesh.dx, mesh.dy
-    cx, cy = 1.0 / (dx * dx), 1.0 / (dy * dy)
+    stretched = not mesh.is_uniform
+    if stretched:
+        m = mesh.metrics
+        vbar = mesh.area_mean
+        cxf = m.hy / (m.hxu * vbar)   # (nx+1, ny), by x face
+        cyf = m.hx / (m.hyv * vbar)   # (nx, ny+1), by y face
+        coeffs = (cxf[0:nx, :], cxf[1:nx + 1, :], cyf[:, 0:ny], cyf[:, 1:ny + 1])
+        row_scale = (m.cell_areas / vbar).ravel()
+        dx, dy = float(mesh.dx_cells[0]), float(mesh.dy_cells[0])
+    else:
+        dx, dy = mesh.dx, mesh.dy
+        cx, cy = 1.0 / (dx * dx), 1.0 / (dy * dy)
+        coeffs = (cx, cx, cy, cy)
+        row_scale = 1.0
     idx = np.arange(n, dtype=np.int64).reshape(nx, ny)
     diag = np.zeros((nx, ny))
     bc = np.zeros((nx, ny))
     rows, cols, vals = [], [], []
-    for axis, shift, coeff, periodic, wall in ((0, -1, cx, periodic_x, "left"),
-                                              (0, +1, cx, periodic_x, "right"),
-                                              (1, -1, cy, periodic_y, "bottom"),
-                                              (1, +1, cy, periodic_y, "top")):
+    for axis, shift, coeff, periodic, wall in ((0, -1, coeffs[0], periodic_x, "left"),
+                                              (0, +1, coeffs[1], periodic_x, "right"),
+                                              (1, -1, coeffs[2], periodic_y, "bottom"),
+                                              (1, +1, coeffs[3], periodic_y, "top")):
         nb = np.roll(idx, -shift, axis=axis)
         inside = np.ones((nx, ny), dtype=bool)
         if not periodic:
@@ -93,7 +113,7 @@ def assemble_poisson_matrix(mesh: StructuredMesh, periodic_x: bool = False,
         active = fluid & inside & fluid.ravel()[nb]
         rows.append(idx[active])
         cols.append(nb[active])
-        vals.append(np.full(int(active.sum()), coeff))
+        vals.append(np.full(int(active.sum()), coeff) if np.isscalar(coeff) else coeff[active])
         diag -= coeff * active
         if wall in dirichlet:
             on_wall = fluid & ~inside
@@ -110,9 +130,12 @@ def assemble_poisson_matrix(mesh: StructuredMesh, periodic_x: bool = False,
     A.sum_duplicates()
     flat = solvable.ravel()
     pin = int(np.flatnonzero(flat)[0]) if flat.any() else 0
-    return PoissonSystem(A, (nx, ny), flat.copy(), not dirichlet, pin, bc.ravel().copy(), 1.0,
-                         dx, dy, bool(periodic_x), bool(periodic_y),
-                         solid if solid.any() else None, dirichlet)
+    return PoissonSystem(A, (nx, ny), flat.copy(), not dirichlet, pin, bc.ravel().copy(),
+                         row_scale, dx, dy, bool(periodic_x), bool(periodic_y),
+                         solid if solid.any() else None, dirichlet,
+                         mesh.dx_cells.copy() if stretched else None,
+                         mesh.dy_cells.copy() if stretched else None,
+                         vbar if stretched else None)
 
 
 def _standalone_context(system: PoissonSystem):
@@ -131,7 +154,8 @@ def _standalone_context(system: PoissonSystem):
         prefs.append(system.dirichlet.get(wall, 0.0))
     nx, ny = system.shape
     return DeviceContext(nx, ny, system.dx, system.dy, 1.0, bc_kinds=kinds, bc_p_ref=prefs,
-                         solid=system.solid)
+                         solid=system.solid, dx_cells=system.dx_cells, dy_cells=system.dy_cells,
+                         area_mean=system.area_mean)
 
 
 class PoissonSolverBase:
@@ -156,6 +180,9 @@ class PoissonSolverBase:
         # default: on MI355X a grid-wide barrier costs more than the launch it
         # replaces), 0 measured choice, k >= 1 from level k down
         self.coarse_chain = -1
+        # hand the stored multigrid levels over with 32-bit class indices even
+        # when 16 bits would do (they are chosen by size otherwise)
+        self.wide_classes = False
         self.last_iterations = 0
         self.last_residual = 0.0
         self._ctx = context
@@ -332,7 +359,7 @@ class MultigridSolverBase(PoissonSolverBase):
                                             periodic=(system.periodic_x, system.periodic_y))
 
     def _levels(self):
-        return self.hierarchy.device_levels(self.coarse_max)
+        return self.hierarchy.device_levels(self.coarse_max, self.wide_classes)
 
     def _sweeps(self) -> int:
         return self.smooth_sweeps

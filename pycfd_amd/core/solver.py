@@ -19,7 +19,7 @@ from ..config import AdvectionScheme, BCKind, SimulationConfig, SolverType
 from .boundary import BoundaryManager
 from .engine import DeviceContext
 from .fields import FlowField, device_zeros, to_device, to_host
-from .mesh import NonUniformMeshError, StructuredMesh
+from .mesh import StructuredMesh
 from .pressure import assemble_poisson_matrix, make_pressure_solver
 
 log = logging.getLogger(__name__)
@@ -52,14 +52,12 @@ class ProjectionSolver:
                 "available. Set solver_type='projection'.")
         self.config = config
         self.mesh = mesh if mesh is not None else StructuredMesh.from_config(config)
-        if not self.mesh.is_uniform:
-            raise NonUniformMeshError(
-                "the GPU projection step is discretised for uniform spacing; "
-                "stretched meshes are not supported (stretch_x = stretch_y = 1.0)")
+        self.metrics = self.mesh.metrics
         self.nu = config.nu
         self.boundaries = BoundaryManager(config.boundary_config, self.mesh)
         self.periodic_x = self.boundaries.periodic_x
         self.periodic_y = self.boundaries.periodic_y
+        self._reject_stretched_periodic()
         nx, ny = self.mesh.shape
         if obstacle is None:
             self.solid = np.zeros((nx, ny), dtype=bool)
@@ -95,15 +93,21 @@ class ProjectionSolver:
             body_id = np.full((nx, ny), -1, dtype=np.int8)
             for k, b in enumerate(self.body_masks):
                 body_id[b] = k
+        stretched = not self.mesh.is_uniform
         self.ctx = DeviceContext(
-            nx, ny, self.mesh.dx, self.mesh.dy, self.nu, config.body_force,
+            nx, ny, float(self.mesh.dx_cells[0]), float(self.mesh.dy_cells[0]), self.nu,
+            config.body_force,
             config.time_scheme.value,
             [conds[w].kind.value for w in walls],
             [conds[w].spec.velocity for w in walls],
             [conds[w].spec.p_ref for w in walls],
             [conds[w].inlet_line(self.mesh) for w in walls],
             self.solid if self.has_obstacle else None, body_id, len(self.body_masks),
-            les_coeff=self.turbulence.coeff if self.turbulence is not None else None)
+            les_coeff=self.turbulence.device_coeff if self.turbulence is not None else None,
+            les_cs=self.turbulence.cs if self.turbulence is not None else 0.0,
+            dx_cells=self.mesh.dx_cells if stretched else None,
+            dy_cells=self.mesh.dy_cells if stretched else None,
+            area_mean=self.mesh.area_mean if stretched else None)
         if self.turbulence is not None:
             self.turbulence.bind(self.ctx, self.nu)
         self.pressure_solver = make_pressure_solver(
@@ -281,14 +285,28 @@ class ProjectionSolver:
         self._blend = float(np.clip(blend, 0.0, 1.0))
 
     def upwind_blend_for(self, fields: FlowField, dt: float) -> float:
-        """solver.py:738-756."""
+        """solver.py:738-756 (the worst cell: smallest widths)."""
         if self.config.advection_scheme is AdvectionScheme.CENTRAL:
             return 0.0
         if self.config.upwind_blend is not None:
             return self.config.upwind_blend
         umax, vmax = fields.max_velocity()
-        courant = max(umax * dt / self.mesh.dx, vmax * dt / self.mesh.dy)
+        m = self.metrics
+        courant = max(umax * dt / m.min_hx, vmax * dt / m.min_hy)
         return float(min(1.0, 1.2 * courant))
+
+    def _reject_stretched_periodic(self) -> None:
+        """solver.py:209-230: a periodic axis must keep equal first and last cells."""
+        mesh = self.mesh
+        for axis, stretched, periodic, ratio in (
+                ("x", mesh.stretched_x, self.periodic_x, mesh.stretch_x),
+                ("y", mesh.stretched_y, self.periodic_y, mesh.stretch_y)):
+            if stretched and periodic:
+                raise ValueError(
+                    f"the {axis} axis is periodic and stretched (stretch_{axis}={ratio}): "
+                    "the first and last cells differ in width, so the wrap would join "
+                    f"mismatched cells. Use stretch_{axis}=1.0 on a periodic axis, or a "
+                    "non-periodic boundary on that axis.")
 
 
 __all__ = ["ProjectionSolver", "_face_solid_mask"]

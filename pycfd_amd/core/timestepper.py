@@ -55,7 +55,8 @@ class TimeStepper:
         nu = self.solver.nu
         if nu <= 0:
             return float("inf")
-        dx, dy = self.mesh.dx, self.mesh.dy
+        m = self.mesh.metrics
+        dx, dy = m.min_hx, m.min_hy
         return VISCOUS_SAFETY_FACTOR / (2.0 * nu * (1.0 / dx ** 2 + 1.0 / dy ** 2))
 
     def measure(self, fields: FlowField, previous: FlowField | None = None) -> dict:
@@ -69,7 +70,11 @@ class TimeStepper:
         d["nonfinite"] = int(out[2])
         d["du"] = max(float(out[3]), float(out[4]))
         d["max_div"] = float(out[5])
-        d["kinetic_energy"] = 0.5 * (float(out[6]) * (self.mesh.dx * self.mesh.dy))
+        # mesh.py:213-223: one area after the sum on a uniform mesh; on a
+        # stretched one the device already weighted each cell by its area
+        area = self.mesh.metrics.uniform_area
+        d["kinetic_energy"] = 0.5 * (float(out[6]) * area if area is not None
+                                     else float(out[6]))
         self.solver.set_force(float(out[7]), float(out[8]))
         return d
 
@@ -78,13 +83,15 @@ class TimeStepper:
         if not self.config.adaptive_dt:
             return self.config.dt
         umax, vmax = fields.max_velocity()
-        scale = umax / self.mesh.dx + vmax / self.mesh.dy
+        m = self.mesh.metrics
+        scale = umax / m.min_hx + vmax / m.min_hy
         dt_conv = self.config.cfl_max / scale if scale > QUIESCENT_VELOCITY_FLOOR else float("inf")
         return float(min(dt_conv, self.viscous_dt_limit, self.config.dt))
 
     def cfl_number(self, fields: FlowField, dt: float) -> float:
         umax, vmax = fields.max_velocity()
-        return float(dt * (umax / self.mesh.dx + vmax / self.mesh.dy))
+        m = self.mesh.metrics
+        return float(dt * (umax / m.min_hx + vmax / m.min_hy))
 
     def _check_health(self, fields: FlowField, dt: float) -> None:
         """timestepper.py:122-142."""

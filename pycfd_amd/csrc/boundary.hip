@@ -154,10 +154,16 @@ __device__ void mass_balance(const BCParams& P, double* u, double* v) {
                             : line_of(v, P.ldv, P.nx + 2, P.ny + 3, 1, wi.n_face);
     const int n = (wi.axis == 0) ? P.ny : P.nx;
     const double width = (wi.axis == 0) ? P.dy : P.dx;
+    // stretched mesh: each face weighted by its own height / width
+    const double* wk = (wi.axis == 0) ? P.hy : P.hx;
     double part = 0.0;
-    for (int k = threadIdx.x; k < n; k += blockDim.x) part += f[1 + k];
+    if (wk) {
+      for (int k = threadIdx.x; k < n; k += blockDim.x) part += f[1 + k] * wk[k];
+    } else {
+      for (int k = threadIdx.x; k < n; k += blockDim.x) part += f[1 + k];
+    }
     const double s = block_sum(part);
-    const double flux_out = (double)wi.outward * (s * width);
+    const double flux_out = (double)wi.outward * (wk ? s : s * width);
     if (kind == BC_OUTLET) {
       outflux += flux_out;
       ++n_out;
@@ -182,7 +188,14 @@ __device__ void mass_balance(const BCParams& P, double* u, double* v) {
       const WallIdx wi = wall_index(w, P.nx, P.ny);
       const int n = (wi.axis == 0) ? P.ny : P.nx;
       const double width = (wi.axis == 0) ? P.dy : P.dx;
-      total_area = total_area + (double)n * width;
+      const double* wk = (wi.axis == 0) ? P.hy : P.hx;
+      if (wk) {
+        double part = 0.0;
+        for (int k = threadIdx.x; k < n; k += blockDim.x) part += wk[k];
+        total_area = total_area + block_sum(part);
+      } else {
+        total_area = total_area + (double)n * width;
+      }
     }
     const double deficit = influx - outflux;
     for (int w = 0; w < 4; ++w) {
@@ -203,11 +216,18 @@ __device__ void mass_balance(const BCParams& P, double* u, double* v) {
 __device__ void mask_obstacle(const BCParams& P, int mode, double dt, double* u, double* v) {
   if (!P.has_obstacle) return;
   if (mode != BOP_MASK_ZERO) {
+    // Uniform mesh: one area after the sum; stretched: each face weighted
+    // by its control volume before it (solver.py:197-207).
+    auto ua = [&](int k) -> double { return P.uf_area ? u[P.uf[k]] * P.uf_area[k] : u[P.uf[k]]; };
+    auto va = [&](int k) -> double { return P.vf_area ? v[P.vf[k]] * P.vf_area[k] : v[P.vf[k]]; };
+    auto scaled = [&](double sum, bool weighted) -> double {
+      return weighted ? sum / dt : sum * P.area / dt;
+    };
     double su = 0.0, sv = 0.0;
-    for (int k = threadIdx.x; k < P.n_uf; k += blockDim.x) su += u[P.uf[k]];
-    for (int k = threadIdx.x; k < P.n_vf; k += blockDim.x) sv += v[P.vf[k]];
-    const double fx = block_sum(su) * P.area / dt;
-    const double fy = block_sum(sv) * P.area / dt;
+    for (int k = threadIdx.x; k < P.n_uf; k += blockDim.x) su += ua(k);
+    for (int k = threadIdx.x; k < P.n_vf; k += blockDim.x) sv += va(k);
+    const double fx = scaled(block_sum(su), P.uf_area != nullptr);
+    const double fy = scaled(block_sum(sv), P.vf_area != nullptr);
     if (threadIdx.x == 0) {
       if (mode == BOP_MASK_RESET) {
         P.force[0] = fx;
@@ -220,11 +240,11 @@ __device__ void mask_obstacle(const BCParams& P, int mode, double dt, double* u,
     for (int b = 0; b < P.n_bodies; ++b) {
       double bu = 0.0, bv = 0.0;
       for (int k = threadIdx.x; k < P.n_uf; k += blockDim.x)
-        if (P.uf_body[k] == b) bu += u[P.uf[k]];
+        if (P.uf_body[k] == b) bu += ua(k);
       for (int k = threadIdx.x; k < P.n_vf; k += blockDim.x)
-        if (P.vf_body[k] == b) bv += v[P.vf[k]];
-      const double bx = block_sum(bu) * P.area / dt;
-      const double by = block_sum(bv) * P.area / dt;
+        if (P.vf_body[k] == b) bv += va(k);
+      const double bx = scaled(block_sum(bu), P.uf_area != nullptr);
+      const double by = scaled(block_sum(bv), P.vf_area != nullptr);
       if (threadIdx.x == 0) {
         if (mode == BOP_MASK_RESET) {
           P.force[2 + 2 * b] = bx;

@@ -49,6 +49,12 @@ struct pcfd_ctx {
   int8_t* vf_body = nullptr;
   double* force = nullptr;
   double* nuc = nullptr;  // Smagorinsky nu + nu_t, ghosted cell layout (LES only)
+  // Stretched mesh: one allocation holding the metric arrays, the Poisson
+  // face denominators and the immersed-boundary face areas.
+  double* metric = nullptr;
+  Metrics g{};
+  const double* poisson_hxv = nullptr;
+  const double* poisson_hyw = nullptr;
   LesParams les{};
   bool use_les = false;
   uint8_t* flags = nullptr;
@@ -75,6 +81,7 @@ struct pcfd_ctx {
     hipFree(diag_out);
     hipFree(guess_backup);
     hipFree(nuc);
+    hipFree(metric);
     if (host_out) hipHostFree(host_out);
   }
 };
@@ -148,7 +155,7 @@ int substep(pcfd_ctx* c, const double* u, const double* v, const double* ue, con
   if ((rc = c->engine.scatter_pressure(p, c->ldp))) return rc;
   if ((rc = launch_pressure_ghosts(c->stream, c->bc, p, c->ldp))) return rc;
   if ((rc = launch_correct(c->stream, p, c->ldp, c->nx, c->ny, c->ldu, c->ldv, c->dx, c->dy,
-                           c->upd, dt, us, vs)))
+                           c->g, c->upd, dt, us, vs)))
     return rc;
   return boundary(c, {BOP_MASK_ACC, BOP_WRAP_FACES, BOP_VEL_CORR}, dt, us, vs);
 }
@@ -196,7 +203,7 @@ int project_impl(pcfd_ctx* c, double* u, double* v, double dt, double* p, SolveI
   if ((rc = c->engine.scatter_pressure(p, c->ldp))) return rc;
   if ((rc = launch_pressure_ghosts(c->stream, c->bc, p, c->ldp))) return rc;
   return launch_correct(c->stream, p, c->ldp, c->nx, c->ny, c->ldu, c->ldv, c->dx, c->dy,
-                        c->upd, dt, u, v);
+                        c->g, c->upd, dt, u, v);
 }
 
 }  // namespace
@@ -271,6 +278,38 @@ int pcfd_ctx_create(const pcfd_case_desc* d, void* hip_stream, pcfd_ctx** out) {
     set_last_error("periodicity must be set on both walls of an axis");
     return kErrBadArg;
   }
+  const bool stretched = d->dx_cells || d->dy_cells;
+  if (stretched) {
+    if (!d->dx_cells || !d->dy_cells) {
+      set_last_error("a stretched mesh needs both dx_cells and dy_cells");
+      return kErrBadArg;
+    }
+    if (!(d->area_mean > 0.0)) {
+      set_last_error("a stretched mesh needs area_mean > 0, got %g", d->area_mean);
+      return kErrBadArg;
+    }
+    for (int i = 0; i < nx; ++i)
+      if (!(d->dx_cells[i] > 0.0)) {
+        set_last_error("dx_cells[%d] = %g is not positive", i, d->dx_cells[i]);
+        return kErrBadArg;
+      }
+    for (int j = 0; j < ny; ++j)
+      if (!(d->dy_cells[j] > 0.0)) {
+        set_last_error("dy_cells[%d] = %g is not positive", j, d->dy_cells[j]);
+        return kErrBadArg;
+      }
+    // solver.py:209-230: the first and last cells of a periodic axis must match
+    for (int i = 1; i < nx && px; ++i)
+      if (d->dx_cells[i] != d->dx_cells[0]) {
+        set_last_error("the x axis is periodic and stretched; use a uniform periodic axis");
+        return kErrBadArg;
+      }
+    for (int j = 1; j < ny && py; ++j)
+      if (d->dy_cells[j] != d->dy_cells[0]) {
+        set_last_error("the y axis is periodic and stretched; use a uniform periodic axis");
+        return kErrBadArg;
+      }
+  }
   const int PO = BC_PRESSURE_OUTLET;
   // solvable face ranges (solver.py:251-263)
   UpdRange R;
@@ -335,6 +374,71 @@ int pcfd_ctx_create(const pcfd_case_desc* d, void* hip_stream, pcfd_ctx** out) {
   std::vector<double> zeros(2 + 2 * (size_t)std::max(c->n_bodies, 0), 0.0);
   if ((rc = upload(&c->force, zeros.data(), zeros.size()))) return rc;
 
+  // Stretched-mesh arrays (mesh.py:160-211, pressure.py:193-207,
+  // solver.py:197-207, 271-274), each computed with the reference's
+  // elementwise arithmetic.
+  if (stretched) {
+    const double* dxc = d->dx_cells;
+    const double* dyc = d->dy_cells;
+    std::vector<double> hxe(nx + 2), hye(ny + 2), hxu(nx + 1), hyv(ny + 1), wx(nx + 1),
+        wy(ny + 1), hxv(nx + 1), hyw(ny + 1);
+    hxe[0] = dxc[0];
+    hxe[nx + 1] = dxc[nx - 1];
+    for (int i = 0; i < nx; ++i) hxe[i + 1] = dxc[i];
+    hye[0] = dyc[0];
+    hye[ny + 1] = dyc[ny - 1];
+    for (int j = 0; j < ny; ++j) hye[j + 1] = dyc[j];
+    for (int m = 0; m <= nx; ++m) {
+      hxu[m] = 0.5 * (hxe[m] + hxe[m + 1]);
+      wx[m] = 0.5 * hxe[m] / hxu[m];
+      hxv[m] = hxu[m] * d->area_mean;
+    }
+    for (int n = 0; n <= ny; ++n) {
+      hyv[n] = 0.5 * (hye[n] + hye[n + 1]);
+      wy[n] = 0.5 * hye[n] / hyv[n];
+      hyw[n] = hyv[n] * d->area_mean;
+    }
+    // face control volumes: u face m, row j -> hxu[m-1] * hy[j-1]; v face
+    // (i, n) -> hx[i-1] * hyv[n-1]
+    std::vector<double> ufa(uf.size()), vfa(vf.size());
+    for (size_t k = 0; k < uf.size(); ++k) {
+      const int m = uf[k] / c->ldu, j = uf[k] % c->ldu;
+      ufa[k] = hxu[m - 1] * dyc[j - 1];
+    }
+    for (size_t k = 0; k < vf.size(); ++k) {
+      const int i = vf[k] / c->ldv, n = vf[k] % c->ldv;
+      vfa[k] = dxc[i - 1] * hyv[n - 1];
+    }
+    std::vector<double> all;
+    std::vector<size_t> at;
+    for (const std::vector<double>* a :
+         {&hxe, &hye, &hxu, &hyv, &wx, &wy, &hxv, &hyw, &ufa, &vfa}) {
+      at.push_back(all.size());
+      all.insert(all.end(), a->begin(), a->end());
+    }
+    at.push_back(all.size());
+    all.insert(all.end(), dxc, dxc + nx);
+    at.push_back(all.size());
+    all.insert(all.end(), dyc, dyc + ny);
+    if ((rc = upload(&c->metric, all.data(), all.size()))) return rc;
+    const double* base = c->metric;
+    Metrics& g = c->g;
+    g.hxe = base + at[0];
+    g.hye = base + at[1];
+    g.hxu = base + at[2];
+    g.hyv = base + at[3];
+    g.wx = base + at[4];
+    g.wy = base + at[5];
+    g.hx = base + at[10];
+    g.hy = base + at[11];
+    c->bc.uf_area = uf.empty() ? nullptr : base + at[8];
+    c->bc.vf_area = vf.empty() ? nullptr : base + at[9];
+    c->bc.hx = g.hx;
+    c->bc.hy = g.hy;
+    c->poisson_hxv = base + at[6];
+    c->poisson_hyw = base + at[7];
+  }
+
   // inlet lines (boundary.py:205-216)
   for (int w = 0; w < 4; ++w) {
     if (d->bc_kind[w] == BC_INLET) {
@@ -388,13 +492,14 @@ int pcfd_ctx_create(const pcfd_case_desc* d, void* hip_stream, pcfd_ctx** out) {
   M.fy = d->body_fy;
   M.blend = 0.0;
   M.nuc = nullptr;
+  M.g = c->g;
   if (d->use_les) {
-    if (!(d->les_coeff >= 0.0)) {
-      set_last_error("les_coeff must be >= 0, got %g", d->les_coeff);
+    if (!(d->les_coeff >= 0.0) || (stretched && !(d->les_cs >= 0.0))) {
+      set_last_error("les_coeff and les_cs must be >= 0, got %g, %g", d->les_coeff, d->les_cs);
       return kErrBadArg;
     }
     c->use_les = true;
-    c->les = LesParams{nx, ny, c->ldu, c->ldv, d->dx, d->dy, d->nu, d->les_coeff};
+    c->les = LesParams{nx, ny, c->ldu, c->ldv, d->dx, d->dy, d->nu, d->les_coeff, d->les_cs, c->g};
     PCFD_HIP_CHECK(hipMalloc(&c->nuc, (size_t)(nx + 2) * (ny + 2) * sizeof(double)));
     PCFD_HIP_CHECK(hipMemsetAsync(c->nuc, 0, (size_t)(nx + 2) * (ny + 2) * sizeof(double),
                                   c->stream));
@@ -420,6 +525,13 @@ int pcfd_ctx_create(const pcfd_case_desc* d, void* hip_stream, pcfd_ctx** out) {
   for (int w = 0; w < 4; ++w) {
     op.dir[w] = dir[w];
     op.pref[w] = d->bc_p_ref[w];
+  }
+  if (stretched) {
+    op.hxc = c->g.hx;
+    op.hyc = c->g.hy;
+    op.hxv = c->poisson_hxv;
+    op.hyv = c->poisson_hyw;
+    op.vbar = d->area_mean;
   }
   if ((rc = c->engine.init(op, n_fluid, singular, c->stream))) return rc;
 
@@ -496,19 +608,29 @@ int pcfd_set_pressure_solver(pcfd_ctx* c, const pcfd_solver_desc* sd,
     if (L.active) h.active.assign(L.active, L.active + n);
     else h.active.assign(n, 1);
     if (l > 0) {
-      if (!L.cls || !L.table || !L.class_dinv || !L.class_active || L.n_offsets <= 0 ||
-          L.n_classes <= 0 || !L.di || !L.dj) {
+      if ((!L.cls && !L.cls32) || !L.table || !L.class_dinv || !L.class_active ||
+          L.n_offsets <= 0 || L.n_classes <= 0 || !L.di || !L.dj) {
         set_last_error("coarse level %d lacks its stencil dictionary", l);
+        return kErrBadArg;
+      }
+      if (!L.cls32 && L.n_classes > 65535) {
+        set_last_error("level %d: %d classes need the 32-bit class index", l, L.n_classes);
         return kErrBadArg;
       }
       h.di.assign(L.di, L.di + L.n_offsets);
       h.dj.assign(L.dj, L.dj + L.n_offsets);
-      h.cls.assign(L.cls, L.cls + n);
-      for (size_t k = 0; k < n; ++k)
-        if (h.cls[k] >= (uint16_t)L.n_classes) {
-          set_last_error("level %d: class index out of range", l);
-          return kErrBadArg;
-        }
+      bool in_range = true;
+      if (L.cls32) {
+        h.cls32.assign(L.cls32, L.cls32 + n);
+        for (size_t k = 0; k < n; ++k) in_range &= h.cls32[k] < (uint32_t)L.n_classes;
+      } else {
+        h.cls.assign(L.cls, L.cls + n);
+        for (size_t k = 0; k < n; ++k) in_range &= h.cls[k] < (uint32_t)L.n_classes;
+      }
+      if (!in_range) {
+        set_last_error("level %d: class index out of range", l);
+        return kErrBadArg;
+      }
       h.table.assign(L.table, L.table + (size_t)L.n_classes * L.n_offsets);
       h.cdinv.assign(L.class_dinv, L.class_dinv + L.n_classes);
       h.cactive.assign(L.class_active, L.class_active + L.n_classes);
@@ -575,7 +697,8 @@ int pcfd_momentum_rhs_variable(pcfd_ctx* c, const double* u, const double* v, do
 
 int pcfd_divergence(pcfd_ctx* c, const double* u, const double* v, double* out) {
   if (int rc = check_ctx(c)) return rc;
-  return launch_divergence(c->stream, u, v, c->nx, c->ny, c->ldu, c->ldv, c->dx, c->dy, out);
+  return launch_divergence(c->stream, u, v, c->nx, c->ny, c->ldu, c->ldv, c->dx, c->dy, c->g,
+                           out);
 }
 
 int pcfd_apply_velocity_bc(pcfd_ctx* c, double* u, double* v, int32_t predictor) {
@@ -731,7 +854,7 @@ int pcfd_diagnostics(pcfd_ctx* c, const double* u, const double* v, const double
                      const double* u_prev, const double* v_prev, double* out_host) {
   if (int rc = check_ctx(c)) return rc;
   int rc = launch_diagnostics(c->stream, u, v, p, u_prev, v_prev, c->solid, c->nx, c->ny, c->ldu,
-                              c->ldv, c->ldp, c->dx, c->dy, c->engine.partial_,
+                              c->ldv, c->ldp, c->dx, c->dy, c->g, c->engine.partial_,
                               c->engine.counter_, c->diag_out);
   if (rc) return rc;
   PCFD_HIP_CHECK(

@@ -32,6 +32,22 @@ struct BCOps {
   int op[8];
 };
 
+// Spacings of a stretched mesh (pycfd/core/mesh.py:122-233) as device
+// arrays.  All null on a uniform mesh, whose kernels divide by the scalars
+// dx, dy with the reference's uniform arithmetic instead.  The reference
+// switches every operator to the array form as soon as either axis is
+// stretched, so a stretched mesh uses these on both axes.
+struct Metrics {
+  const double* hx;   // nx    cell widths
+  const double* hy;   // ny    cell heights
+  const double* hxu;  // nx+1  centre-to-centre distance across x face m
+  const double* hyv;  // ny+1
+  const double* hxe;  // nx+2  widths with one mirrored ghost at each end
+  const double* hye;  // ny+2
+  const double* wx;   // nx+1  corner weight of the upper node
+  const double* wy;   // ny+1
+};
+
 struct BCParams {
   int nx, ny, ldu, ldv;
   int kind[4];             // left, right, bottom, top
@@ -49,6 +65,13 @@ struct BCParams {
   int n_bodies;
   double area;             // uniform cell area
   double* force;           // device [fx, fy, (fx, fy) per body]
+  // Stretched mesh: control-volume area of each immersed-boundary face
+  // (solver.py:197-207, 271-274) and the wall widths weighting the outlet
+  // fluxes (boundary.py:438-459); null on a uniform mesh.
+  const double* uf_area;
+  const double* vf_area;
+  const double* hx;        // nx cell widths
+  const double* hy;        // ny cell heights
 };
 
 struct MomParams {
@@ -57,12 +80,17 @@ struct MomParams {
   // Variable viscosity (Smagorinsky LES): cell-centred nu + nu_t on the
   // ghosted (nx+2, ny+2) layout, row pitch ny + 2; nullptr: constant nu.
   const double* nuc;
+  Metrics g;  // stretched mesh, or all null
 };
 
-// Smagorinsky constant term (Cs * sqrt(dx dy))^2 of the eddy viscosity.
+// Smagorinsky constant term (Cs * sqrt(dx dy))^2 of the eddy viscosity; on a
+// stretched mesh the filter width is per cell (turbulence.py:47-58) and the
+// kernel forms (cs * sqrt(hx hy))^2 itself.
 struct LesParams {
   int nx, ny, ldu, ldv;
   double dx, dy, nu, coeff;
+  double cs;
+  Metrics g;
 };
 
 // Solvable face ranges u_upd / v_upd (solver.py:235-263): u faces i in
@@ -88,14 +116,17 @@ int launch_predictor_comb(hipStream_t s, const double* u0, const double* ue, con
 int launch_combine(hipStream_t s, const double* u0, const double* ue, const double* v0,
                    const double* ve, double ca, double cb, int nx, int ny, int ldu, int ldv,
                    double* uo, double* vo);
+// The launchers below take the mesh's Metrics: null arrays select the
+// uniform arithmetic on dx, dy.
 int launch_correct(hipStream_t s, const double* p, int ldp, int nx, int ny, int ldu, int ldv,
-                   double dx, double dy, const UpdRange& R, double dt, double* u, double* v);
+                   double dx, double dy, const Metrics& g, const UpdRange& R, double dt, double* u,
+                   double* v);
 int launch_divergence(hipStream_t s, const double* u, const double* v, int nx, int ny, int ldu,
-                      int ldv, double dx, double dy, double* out);
+                      int ldv, double dx, double dy, const Metrics& g, double* out);
 int launch_diagnostics(hipStream_t s, const double* u, const double* v, const double* p,
                        const double* up, const double* vp, const uint8_t* solid, int nx, int ny,
-                       int ldu, int ldv, int ldp, double dx, double dy, double* partial,
-                       unsigned int* counter, double* out);
+                       int ldu, int ldv, int ldp, double dx, double dy, const Metrics& g,
+                       double* partial, unsigned int* counter, double* out);
 
 // boundary.hip
 int launch_boundary_pass(hipStream_t s, const BCParams& P, const BCOps& ops, double dt, double* u,

@@ -252,11 +252,13 @@ k_prepare(FineOp A, const double* __restrict__ u, const double* __restrict__ v, 
       val = rhs[k];
     } else {
       const int a = i + 1, c = j + 1;
-      const double div = (u[(size_t)(a + 1) * ldu + c] - u[(size_t)a * ldu + c]) / dx +
-                         (v[(size_t)a * ldv + c + 1] - v[(size_t)a * ldv + c]) / dy;
+      const double hx = A.hxc ? A.hxc[i] : dx, hy = A.hyc ? A.hyc[j] : dy;
+      const double div = (u[(size_t)(a + 1) * ldu + c] - u[(size_t)a * ldu + c]) / hx +
+                         (v[(size_t)a * ldv + c + 1] - v[(size_t)a * ldv + c]) / hy;
       val = div / dt;
     }
-    val = val * 1.0 + ((f & F_SOLV) ? dirichlet_rhs(A, i, j) : 0.0);
+    val = val * fine_row_scale(A, i, j) +
+          ((f & F_SOLV) ? dirichlet_rhs(A, fine_faces(A, i, j), i, j) : 0.0);
     if (!(f & F_SOLV)) val = 0.0;
     b[k] = val;
     s[0] = val;
@@ -280,8 +282,9 @@ k_init(FineOp A, double* __restrict__ b, const double* __restrict__ x, double* _
     const Nb nb = fine_neighbours(A, i, j);
     const double bk = (f & F_SOLV) ? b[k] - mb : 0.0;
     auto xv = [&](int q) -> double { return x[q]; };
-    const double d = fine_diag(A, f, i, j);
-    const double rk = bk - fine_apply<double>(A, f, nb, k, d, xv);
+    const FaceC fc = fine_faces(A, i, j);
+    const double d = fine_diag(A, fc, f, i, j);
+    const double rk = bk - fine_apply<double>(fc, f, nb, k, d, xv);
     s[0] = bk * bk;
     s[1] = rk * rk;
     if (mode == IM_JPCG) s[2] = rk * (rk / d);
@@ -324,9 +327,10 @@ k_jpcg_dir(FineOp A, const double* __restrict__ r, const double* __restrict__ p_
       const double z = r[m] / fine_diag(A, A.flags[m], mi, mj);
       return first ? z : p_old[m] * beta + z;
     };
-    const double d = fine_diag(A, f, i, j);
+    const FaceC fc = fine_faces(A, i, j);
+    const double d = fine_diag(A, fc, f, i, j);
     const double pk = pv(k);
-    const double qk = fine_apply<double>(A, f, nb, k, d, pv);
+    const double qk = fine_apply<double>(fc, f, nb, k, d, pv);
     p_new[k] = pk;
     q[k] = qk;
     s[0] = pk * qk;
@@ -396,12 +400,13 @@ k_sor_colour(FineOp A, double* __restrict__ p, const double* __restrict__ b, int
         const double sh = (colour == 0 || same) ? m : 0.0;
         return (A.flags[q] & F_SOLV) ? p[q] - sh : p[q];
       };
-      const double d = fine_diag(A, f, i, j);
+      const FaceC fc = fine_faces(A, i, j);
+    const double d = fine_diag(A, fc, f, i, j);
       double off = 0.0;  // A_rb p_b in CSR column order
-      if (f & F_L) off += A.cx * pv(nb.kL);
-      if (f & F_B) off += A.cy * pv(nb.kB);
-      if (f & F_T) off += A.cy * pv(nb.kT);
-      if (f & F_R) off += A.cx * pv(nb.kR);
+      if (f & F_L) off += fc.l * pv(nb.kL);
+      if (f & F_B) off += fc.b * pv(nb.kB);
+      if (f & F_T) off += fc.t * pv(nb.kT);
+      if (f & F_R) off += fc.r * pv(nb.kR);
       const double pk = pv(k);
       const double val = (1.0 - omega) * pk + omega * (b[k] - off) / d;
       p[k] = val;
@@ -426,8 +431,9 @@ k_jacobi_sweep(FineOp A, const double* __restrict__ p, double* __restrict__ pn,
     const uint8_t f = A.flags[k];
     const Nb nb = fine_neighbours(A, i, j);
     auto pv = [&](int q) -> double { return (A.flags[q] & F_SOLV) ? p[q] - m : p[q]; };
-    const double d = fine_diag(A, f, i, j);
-    const double ap = fine_apply<double>(A, f, nb, k, d, pv);
+    const FaceC fc = fine_faces(A, i, j);
+    const double d = fine_diag(A, fc, f, i, j);
+    const double ap = fine_apply<double>(fc, f, nb, k, d, pv);
     const double val = pv(k) + omega * (1.0 / d) * (b[k] - ap);
     pn[k] = val;
     if (f & F_SOLV) s[0] = val;
@@ -453,8 +459,9 @@ k_check(FineOp A, const double* __restrict__ x, const double* __restrict__ b, in
       if (zero) return 0.0;
       return (A.flags[q] & F_SOLV) ? x[q] - m : x[q];
     };
-    const double d = fine_diag(A, f, i, j);
-    const double rk = b[k] - fine_apply<double>(A, f, nb, k, d, xv);
+    const FaceC fc = fine_faces(A, i, j);
+    const double d = fine_diag(A, fc, f, i, j);
+    const double rk = b[k] - fine_apply<double>(fc, f, nb, k, d, xv);
     s[0] = rk * rk;
     if (f & F_SOLV) s[1] = xv(k);
   }
@@ -522,6 +529,11 @@ enum RowOp : int {
 };
 
 template <typename T>
+__device__ __forceinline__ int cls_of(const ClassLevel<T>& L, int row) {
+  return L.cls32 ? (int)L.cls32[row] : (int)L.cls[row];
+}
+
+template <typename T>
 __device__ __forceinline__ int cls_nb(const ClassLevel<T>& L, int row, int e) {
   if (!(L.px | L.py)) return row + L.ent_lin[e];
   const int i = row / L.ny, j = row - i * L.ny;
@@ -543,10 +555,10 @@ __device__ __forceinline__ void cls_rows_at(const ClassLevel<T>& L, T w, const T
   T acc = (T)0;
   int c = 0;
   if (valid) {
-    c = L.cls[row];
+    c = cls_of(L, row);
     const int e1 = L.row_ptr[c + 1];
     auto xval = [&](int q) -> T {
-      if (OP == RO_SMOOTH0_RES) return w * L.cdinv[L.cls[q]] * b[q];
+      if (OP == RO_SMOOTH0_RES) return w * L.cdinv[cls_of(L, q)] * b[q];
       if (OP == RO_PROLONG) return (agg[q] >= 0) ? xc[agg[q]] : (T)0;
       return xin[q];
     };
@@ -618,7 +630,7 @@ __device__ __forceinline__ void cls_restrict_at(const ClassLevel<T>& L, T w,
   const int i = I * fx + a2, j = J * fy + c2;
   const bool in = valid && sub < fx * fy && i < L.nx && j < L.ny;
   const int row = in ? i * L.ny + j : 0;
-  const int c = in ? (int)L.cls[row] : 0;
+  const int c = in ? cls_of(L, row) : 0;
   const bool act = in && L.cactive[c];
   T acc = (T)0;
   if (act) {
@@ -675,7 +687,7 @@ __device__ __forceinline__ void cls_restrict_serial_at(const ClassLevel<T>& L, T
       const int j = J * fy + c2;
       const bool in = valid && i < L.nx && j < L.ny;
       const int row = in ? i * L.ny + j : 0;
-      const int c = in ? (int)L.cls[row] : 0;
+      const int c = in ? cls_of(L, row) : 0;
       const bool act = in && L.cactive[c];
       T acc = (T)0;
       if (act) {
@@ -862,7 +874,8 @@ k_apply_fine(FineOp A, const double* __restrict__ x, double* __restrict__ y) {
   const uint8_t f = A.flags[k];
   const Nb nb = fine_neighbours(A, i, j);
   auto xv = [&](int q) -> double { return x[q]; };
-  y[k] = fine_apply<double>(A, f, nb, k, fine_diag(A, f, i, j), xv);
+  const FaceC fc = fine_faces(A, i, j);
+  y[k] = fine_apply<double>(fc, f, nb, k, fine_diag(A, fc, f, i, j), xv);
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -915,6 +928,7 @@ void PoissonEngine::free_levels() {
     for (auto& L : *v) {
       for (void* p : {L.b, L.x, L.y, L.r, L.t, L.ent_val, L.cdinv}) hipFree(p);
       hipFree(L.cls);
+      hipFree(L.cls32);
       hipFree(L.row_ptr);
       hipFree(L.ent_lin);
       hipFree(L.agg);
@@ -999,7 +1013,8 @@ static int upload_level(const MGLevelHost& h, const MGLevelHost* next, bool fine
     // fused (the halo recomputation grows with the radius squared times the
     // row length; past radius 3 it costs more than the launches it saves).
     // The tile plan keeps the entry offsets to place them in its LDS region.
-    if (next && fused_ok && std::max(L.rxs, L.rys) <= kFuseMaxRadius) {
+    const bool wide = !h.cls32.empty();
+    if (next && fused_ok && !wide && std::max(L.rxs, L.rys) <= kFuseMaxRadius) {
       L.hdi = edi;
       L.hdj = edj;
       std::vector<int> count(nc, 0);
@@ -1012,7 +1027,14 @@ static int upload_level(const MGLevelHost& h, const MGLevelHost* next, bool fine
       }
     }
     const size_t nnz = std::max<size_t>(val.size(), 1);
-    PCFD_HIP_CHECK(hipMalloc(&L.cls, n * sizeof(uint16_t)));
+    if (wide) {
+      PCFD_HIP_CHECK(hipMalloc(&L.cls32, n * sizeof(uint32_t)));
+      PCFD_HIP_CHECK(
+          hipMemcpy(L.cls32, h.cls32.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    } else {
+      PCFD_HIP_CHECK(hipMalloc(&L.cls, n * sizeof(uint16_t)));
+      PCFD_HIP_CHECK(hipMemcpy(L.cls, h.cls.data(), n * sizeof(uint16_t), hipMemcpyHostToDevice));
+    }
     PCFD_HIP_CHECK(hipMalloc(&L.row_ptr, (nc + 1) * sizeof(int)));
     PCFD_HIP_CHECK(hipMalloc(&L.ent_lin, nnz * sizeof(int)));
     PCFD_HIP_CHECK(hipMalloc(&L.ent_di, nnz));
@@ -1020,7 +1042,6 @@ static int upload_level(const MGLevelHost& h, const MGLevelHost* next, bool fine
     PCFD_HIP_CHECK(hipMalloc(&L.ent_val, nnz * sizeof(T)));
     PCFD_HIP_CHECK(hipMalloc(&L.cdinv, nc * sizeof(T)));
     PCFD_HIP_CHECK(hipMalloc(&L.cactive, nc));
-    PCFD_HIP_CHECK(hipMemcpy(L.cls, h.cls.data(), n * sizeof(uint16_t), hipMemcpyHostToDevice));
     PCFD_HIP_CHECK(hipMemcpy(L.row_ptr, rp.data(), (nc + 1) * sizeof(int), hipMemcpyHostToDevice));
     PCFD_HIP_CHECK(hipMemcpy(L.ent_lin, lin.data(), val.size() * sizeof(int), hipMemcpyHostToDevice));
     PCFD_HIP_CHECK(hipMemcpy(L.ent_di, edi.data(), val.size(), hipMemcpyHostToDevice));
@@ -1125,7 +1146,7 @@ int PoissonEngine::ensure_partials(size_t n) {
 // Measured tile choices, kept for the life of the process: a level of the same
 // shape, stencil radius, sweeps and precision gets the same plan again.
 namespace {
-using TuneKey = std::array<int, 13>;
+using TuneKey = std::array<int, 14>;
 struct TunePick {
   int fused, tx, ty;
 };
@@ -1213,7 +1234,7 @@ int PoissonEngine::tune_level(std::vector<LevelBuf>& lv, int l, SolveState* tst)
   };
 
   const TuneKey key = {fine ? 1 : 0, L.nx, L.ny, L.rxs, L.rys, L.fx, L.fy, nu, (int)sizeof(T),
-                       op_.px, op_.py, ftx, fty};
+                       op_.px, op_.py, ftx, fty, op_.hxc != nullptr};
   {
     std::lock_guard<std::mutex> g(g_tune_mutex);
     auto it = g_tune_cache.find(key);
@@ -1286,7 +1307,8 @@ int PoissonEngine::tune_dir(SolveState* tst) {
   size_t most = 0;
   for (const FusedTile& t : cands) most = std::max(most, (size_t)t.ntiles());
   if (int rc = ensure_partials(most)) return rc;
-  const TuneKey key = {2, op_.nx, op_.ny, 1, 1, 1, 1, 0, (int)sizeof(T), op_.px, op_.py, 0, 0};
+  const TuneKey key = {2, op_.nx, op_.ny, 1, 1, 1, 1, 0, (int)sizeof(T), op_.px, op_.py, 0, 0,
+                       op_.hxc != nullptr};
   {
     std::lock_guard<std::mutex> g(g_tune_mutex);
     auto it = g_tune_cache.find(key);
@@ -1634,6 +1656,7 @@ static ClassLevel<T> class_level(const LevelBuf& L, int px, int py) {
   D.px = px;
   D.py = py;
   D.cls = L.cls;
+  D.cls32 = L.cls32;
   D.row_ptr = L.row_ptr;
   D.ent_lin = L.ent_lin;
   D.ent_di = L.ent_di;

@@ -25,6 +25,16 @@ struct FineOp {
   int px, py;     // periodic axes
   int dir[4];     // Dirichlet (pressure-outlet) walls: left, right, bottom, top
   double pref[4];
+  // Stretched mesh (pressure.py:193-207), null on a uniform one: the face
+  // coefficient of x face m (0..nx) in row j is hyc[j] / hxv[m], of y face n
+  // (0..ny) in column i is hxc[i] / hyv[n], with hxv = hxu * vbar and
+  // hyv = hyv_centre * vbar (vbar the mean cell area); row i, j is scaled by
+  // hxc[i] * hyc[j] / vbar.
+  const double* hxc;  // nx cell widths
+  const double* hyc;  // ny cell heights
+  const double* hxv;  // nx+1
+  const double* hyv;  // ny+1
+  double vbar;
 };
 
 // A Galerkin coarse level on its own rectangular grid, stored as a row
@@ -39,6 +49,7 @@ struct ClassLevel {
   int nx, ny, n;
   int px, py;
   const uint16_t* cls;     // per cell
+  const uint32_t* cls32;   // per cell, in place of cls past 65535 classes
   const int* row_ptr;      // per class, n_classes + 1
   const int* ent_lin;      // per entry: di*ny + dj (non-periodic neighbour)
   const int8_t* ent_di;    // per entry, for periodic wrap
@@ -91,6 +102,7 @@ struct MGLevelHost {
   int fx, fy;                    // aggregation factors towards the next level
   std::vector<int8_t> di, dj;    // stencil offsets (coarse levels)
   std::vector<uint16_t> cls;     // per-cell class (coarse levels)
+  std::vector<uint32_t> cls32;   // the same past 65535 classes (stretched meshes)
   std::vector<double> table;     // n_classes x S coefficients (dense rows)
   std::vector<double> cdinv;     // per-class 1/diag
   std::vector<uint8_t> cactive;  // per-class active flag
@@ -125,6 +137,7 @@ struct LevelBuf {
   void *b = nullptr, *x = nullptr, *y = nullptr, *r = nullptr, *t = nullptr;
   int ncls = 0, nnz = 0, max_row = 0, lanes = 1;
   uint16_t* cls = nullptr;
+  uint32_t* cls32 = nullptr;  // wide class index (never run through the fused kernels)
   int *row_ptr = nullptr, *ent_lin = nullptr, *agg = nullptr;
   int8_t *ent_di = nullptr, *ent_dj = nullptr;
   void *ent_val = nullptr, *cdinv = nullptr;

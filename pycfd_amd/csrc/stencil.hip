@@ -5,6 +5,8 @@
 // kernel pycfd/core/kernels.py:39-110), and the library is compiled with
 // -ffp-contract=off, so each output is bit-identical to the reference for the
 // same inputs: no fused multiply-add, IEEE division by dx, dy, dx^2, dy^2.
+// A stretched mesh has its own kernel instances (template flag S) with the
+// reference's array-metric arithmetic; the uniform ones are unchanged.
 #include "internal.hpp"
 
 namespace pcfd {
@@ -17,41 +19,67 @@ namespace pcfd {
 // Index conventions below are the ghosted arrays': u face m, v face n, cell
 // (i, j) of nu_c, corner (a, b) between cells a, a+1 and b, b+1.
 // ----------------------------------------------------------------------------
-template <class FU, class FV>
+// Spacings by ghosted index (mesh.py:122-233): the scalar dx / dy on a
+// uniform mesh (S = false), the metric arrays on a stretched one.  HX(i):
+// width of ghosted cell i (1..nx); HXU(m): span across ghosted u face m
+// (1..nx+1); HXE(q): width of ghosted cell q including the ghosts (0..nx+1).
+template <bool S> struct Sp;
+template <> struct Sp<false> {
+  template <class P> static __device__ __forceinline__ double HX(const P& p, int) { return p.dx; }
+  template <class P> static __device__ __forceinline__ double HY(const P& p, int) { return p.dy; }
+  template <class P> static __device__ __forceinline__ double HXU(const P& p, int) { return p.dx; }
+  template <class P> static __device__ __forceinline__ double HYV(const P& p, int) { return p.dy; }
+  template <class P> static __device__ __forceinline__ double HXE(const P& p, int) { return p.dx; }
+  template <class P> static __device__ __forceinline__ double HYE(const P& p, int) { return p.dy; }
+};
+template <> struct Sp<true> {
+  template <class P> static __device__ __forceinline__ double HX(const P& p, int i) { return p.g.hx[i - 1]; }
+  template <class P> static __device__ __forceinline__ double HY(const P& p, int j) { return p.g.hy[j - 1]; }
+  template <class P> static __device__ __forceinline__ double HXU(const P& p, int m) { return p.g.hxu[m - 1]; }
+  template <class P> static __device__ __forceinline__ double HYV(const P& p, int n) { return p.g.hyv[n - 1]; }
+  template <class P> static __device__ __forceinline__ double HXE(const P& p, int q) { return p.g.hxe[q]; }
+  template <class P> static __device__ __forceinline__ double HYE(const P& p, int q) { return p.g.hye[q]; }
+};
+
+// tau at corner (a, b): the shear derivatives span a face, hyv[b] and
+// hxu[a] on a stretched mesh (solver.py:482-486).
+template <bool S, class FU, class FV>
 __device__ __forceinline__ double tau_corner(const FU& U, const FV& V, int a, int b,
                                              const MomParams& P) {
   const double* nc = P.nuc;
   const int ld = P.ny + 2;
   const double nuk = 0.25 * (nc[(size_t)a * ld + b] + nc[(size_t)(a + 1) * ld + b] +
                              nc[(size_t)a * ld + b + 1] + nc[(size_t)(a + 1) * ld + b + 1]);
-  const double dudy = (U(a + 1, b + 1) - U(a + 1, b)) / P.dy;
-  const double dvdx = (V(a + 1, b + 1) - V(a, b + 1)) / P.dx;
+  const double dudy = (U(a + 1, b + 1) - U(a + 1, b)) / Sp<S>::HYV(P, b + 1);
+  const double dvdx = (V(a + 1, b + 1) - V(a, b + 1)) / Sp<S>::HXU(P, a + 1);
   return nuk * (dudy + dvdx);
 }
 
-template <class FU, class FV>
+template <bool S, class FU, class FV>
 __device__ __forceinline__ double visc_u_face(const FU& U, const FV& V, int m, int j,
                                               const MomParams& P) {
   const double* nc = P.nuc;
   const int ld = P.ny + 2;
-  const double gx_hi = (U(m + 1, j) - U(m, j)) / P.dx;   // du/dx in cell m
-  const double gx_lo = (U(m, j) - U(m - 1, j)) / P.dx;   // du/dx in cell m-1
+  const double gx_hi = (U(m + 1, j) - U(m, j)) / Sp<S>::HXE(P, m);      // du/dx in cell m
+  const double gx_lo = (U(m, j) - U(m - 1, j)) / Sp<S>::HXE(P, m - 1);  // du/dx in cell m-1
   const double sxx = 2.0 * (nc[(size_t)m * ld + j] * gx_hi - nc[(size_t)(m - 1) * ld + j] * gx_lo) /
-                     P.dx;
-  const double sxy = (tau_corner(U, V, m - 1, j, P) - tau_corner(U, V, m - 1, j - 1, P)) / P.dy;
+                     Sp<S>::HXU(P, m);
+  const double sxy = (tau_corner<S>(U, V, m - 1, j, P) - tau_corner<S>(U, V, m - 1, j - 1, P)) /
+                     Sp<S>::HY(P, j);
   return sxx + sxy;
 }
 
-template <class FU, class FV>
+template <bool S, class FU, class FV>
 __device__ __forceinline__ double visc_v_face(const FU& U, const FV& V, int i, int n,
                                               const MomParams& P) {
   const double* nc = P.nuc;
   const int ld = P.ny + 2;
-  const double syx = (tau_corner(U, V, i, n - 1, P) - tau_corner(U, V, i - 1, n - 1, P)) / P.dx;
-  const double gy_hi = (V(i, n + 1) - V(i, n)) / P.dy;
-  const double gy_lo = (V(i, n) - V(i, n - 1)) / P.dy;
+  const double syx = (tau_corner<S>(U, V, i, n - 1, P) - tau_corner<S>(U, V, i - 1, n - 1, P)) /
+                     Sp<S>::HX(P, i);
+  const double gy_hi = (V(i, n + 1) - V(i, n)) / Sp<S>::HYE(P, n);
+  const double gy_lo = (V(i, n) - V(i, n - 1)) / Sp<S>::HYE(P, n - 1);
   const double syy = 2.0 * (nc[(size_t)i * ld + n] * gy_hi - nc[(size_t)i * ld + n - 1] * gy_lo) /
-                     P.dy;
+                     Sp<S>::HYV(P, n);
   return syx + syy;
 }
 
@@ -61,8 +89,8 @@ __device__ __forceinline__ double visc_v_face(const FU& U, const FV& V, int i, i
 // functors so a Runge-Kutta stage combination can be folded into the loads.
 // ----------------------------------------------------------------------------
 template <class FU, class FV>
-__device__ __forceinline__ double rhs_u_face(const FU& U, const FV& V, int m, int j,
-                                             const MomParams& P) {
+__device__ __forceinline__ double rhs_u_uniform(const FU& U, const FV& V, int m, int j,
+                                                const MomParams& P) {
   const double u0 = U(m, j);
   const double um = U(m - 1, j), up = U(m + 1, j);
   const double ud = U(m, j - 1), uu = U(m, j + 1);
@@ -80,14 +108,14 @@ __device__ __forceinline__ double rhs_u_face(const FU& U, const FV& V, int m, in
     duudx += P.blend / P.dx * (fabs(uu_r) * (u0 - up) / 2.0 - fabs(uu_l) * (um - u0) / 2.0);
     duvdy += P.blend / P.dy * (fabs(v_top) * (u0 - uu) / 2.0 - fabs(v_bot) * (ud - u0) / 2.0);
   }
-  if (P.nuc) return -(duudx + duvdy) + visc_u_face(U, V, m, j, P) + P.fx;
+  if (P.nuc) return -(duudx + duvdy) + visc_u_face<false>(U, V, m, j, P) + P.fx;
   const double d2u = (um - 2.0 * u0 + up) / P.dx2 + (ud - 2.0 * u0 + uu) / P.dy2;
   return -(duudx + duvdy) + P.nu * d2u + P.fx;
 }
 
 template <class FU, class FV>
-__device__ __forceinline__ double rhs_v_face(const FU& U, const FV& V, int i, int n,
-                                             const MomParams& P) {
+__device__ __forceinline__ double rhs_v_uniform(const FU& U, const FV& V, int i, int n,
+                                                const MomParams& P) {
   const double v0 = V(i, n);
   const double vm = V(i, n - 1), vp = V(i, n + 1);
   const double vl = V(i - 1, n), vr = V(i + 1, n);
@@ -103,9 +131,94 @@ __device__ __forceinline__ double rhs_v_face(const FU& U, const FV& V, int i, in
     dvvdy += P.blend / P.dy * (fabs(vv_t) * (v0 - vp) / 2.0 - fabs(vv_b) * (vm - v0) / 2.0);
     duvdx += P.blend / P.dx * (fabs(u_right) * (v0 - vr) / 2.0 - fabs(u_left) * (vl - v0) / 2.0);
   }
-  if (P.nuc) return -(duvdx + dvvdy) + visc_v_face(U, V, i, n, P) + P.fy;
+  if (P.nuc) return -(duvdx + dvvdy) + visc_v_face<false>(U, V, i, n, P) + P.fy;
   const double d2v = (vl - 2.0 * v0 + vr) / P.dx2 + (vm - 2.0 * v0 + vp) / P.dy2;
   return -(duvdx + dvvdy) + P.nu * d2v + P.fy;
+}
+
+// The same on a stretched mesh (solver.py:334-465 with array metrics):
+// corner values interpolated with the upper-node weights wy / wx, each flux
+// differenced over its face's own control volume (hxu, hy for u; hx, hyv for
+// v), the viscous term as a difference of face gradients, each over the
+// distance it spans.  u face m, row j: rhs row m-1, column j-1.
+template <class FU, class FV>
+__device__ __forceinline__ double rhs_u_stretched(const FU& U, const FV& V, int m, int j,
+                                                  const MomParams& P) {
+  const Metrics& g = P.g;
+  const double u0 = U(m, j);
+  const double um = U(m - 1, j), up = U(m + 1, j);
+  const double ud = U(m, j - 1), uu = U(m, j + 1);
+  const double uu_r = 0.5 * (u0 + up);
+  const double uu_l = 0.5 * (um + u0);
+  const double hxu = g.hxu[m - 1], hy = g.hy[j - 1];
+  double duudx = (uu_r * uu_r - uu_l * uu_l) / hxu;
+  // corners (m-1, j) above and (m-1, j-1) below: u_c = u_lo + wy (u_hi - u_lo),
+  // v_c = v_lo + wx (v_hi - v_lo) (solver.py:353-358)
+  const double u_top = u0 + g.wy[j] * (uu - u0);
+  const double v_lo_t = V(m - 1, j + 1);
+  const double v_top = v_lo_t + g.wx[m - 1] * (V(m, j + 1) - v_lo_t);
+  const double u_bot = ud + g.wy[j - 1] * (u0 - ud);
+  const double v_lo_b = V(m - 1, j);
+  const double v_bot = v_lo_b + g.wx[m - 1] * (V(m, j) - v_lo_b);
+  double duvdy = (u_top * v_top - u_bot * v_bot) / hy;
+  if (P.blend > 0.0) {
+    duudx += P.blend / hxu * (fabs(uu_r) * (u0 - up) / 2.0 - fabs(uu_l) * (um - u0) / 2.0);
+    duvdy += P.blend / hy * (fabs(v_top) * (u0 - uu) / 2.0 - fabs(v_bot) * (ud - u0) / 2.0);
+  }
+  if (P.nuc) return -(duudx + duvdy) + visc_u_face<true>(U, V, m, j, P) + P.fx;
+  const double gx_hi = (up - u0) / g.hxe[m];
+  const double gx_lo = (u0 - um) / g.hxe[m - 1];
+  const double gy_hi = (uu - u0) / g.hyv[j];
+  const double gy_lo = (u0 - ud) / g.hyv[j - 1];
+  const double d2u = (gx_hi - gx_lo) / hxu + (gy_hi - gy_lo) / hy;
+  return -(duudx + duvdy) + P.nu * d2u + P.fx;
+}
+
+// v face (i, n): rhs row i-1, column n-1.
+template <class FU, class FV>
+__device__ __forceinline__ double rhs_v_stretched(const FU& U, const FV& V, int i, int n,
+                                                  const MomParams& P) {
+  const Metrics& g = P.g;
+  const double v0 = V(i, n);
+  const double vm = V(i, n - 1), vp = V(i, n + 1);
+  const double vl = V(i - 1, n), vr = V(i + 1, n);
+  const double vv_t = 0.5 * (v0 + vp);
+  const double vv_b = 0.5 * (vm + v0);
+  const double hx = g.hx[i - 1], hyv = g.hyv[n - 1];
+  double dvvdy = (vv_t * vv_t - vv_b * vv_b) / hyv;
+  // corners (i, n-1) right and (i-1, n-1) left
+  const double u_lo_r = U(i + 1, n - 1);
+  const double u_right = u_lo_r + g.wy[n - 1] * (U(i + 1, n) - u_lo_r);
+  const double v_right = v0 + g.wx[i] * (vr - v0);
+  const double u_lo_l = U(i, n - 1);
+  const double u_left = u_lo_l + g.wy[n - 1] * (U(i, n) - u_lo_l);
+  const double v_left = vl + g.wx[i - 1] * (v0 - vl);
+  double duvdx = (u_right * v_right - u_left * v_left) / hx;
+  if (P.blend > 0.0) {
+    dvvdy += P.blend / hyv * (fabs(vv_t) * (v0 - vp) / 2.0 - fabs(vv_b) * (vm - v0) / 2.0);
+    duvdx += P.blend / hx * (fabs(u_right) * (v0 - vr) / 2.0 - fabs(u_left) * (vl - v0) / 2.0);
+  }
+  if (P.nuc) return -(duvdx + dvvdy) + visc_v_face<true>(U, V, i, n, P) + P.fy;
+  const double hx_hi = (vr - v0) / g.hxu[i];
+  const double hx_lo = (v0 - vl) / g.hxu[i - 1];
+  const double hy_hi = (vp - v0) / g.hye[n];
+  const double hy_lo = (v0 - vm) / g.hye[n - 1];
+  const double d2v = (hx_hi - hx_lo) / hx + (hy_hi - hy_lo) / hyv;
+  return -(duvdx + dvvdy) + P.nu * d2v + P.fy;
+}
+
+template <bool S, class FU, class FV>
+__device__ __forceinline__ double rhs_u_face(const FU& U, const FV& V, int m, int j,
+                                             const MomParams& P) {
+  if constexpr (S) return rhs_u_stretched(U, V, m, j, P);
+  else return rhs_u_uniform(U, V, m, j, P);
+}
+
+template <bool S, class FU, class FV>
+__device__ __forceinline__ double rhs_v_face(const FU& U, const FV& V, int i, int n,
+                                             const MomParams& P) {
+  if constexpr (S) return rhs_v_stretched(U, V, i, n, P);
+  else return rhs_v_uniform(U, V, i, n, P);
 }
 
 struct PlainLoad {
@@ -131,35 +244,41 @@ struct CombLoad {
 // the ghosted (nx+2, ny+2) layout per thread.  The corner shear rate is
 // averaged to interior cells; ghost cells take the nearest interior value
 // (the reference copies rows, then columns, into the ghost ring).
-template <class FU, class FV>
+template <bool S, class FU, class FV>
 __device__ __forceinline__ double s12_corner(const FU& U, const FV& V, int a, int b,
                                              const LesParams& P) {
-  const double dudy = (U(a + 1, b + 1) - U(a + 1, b)) / P.dy;
-  const double dvdx = (V(a + 1, b + 1) - V(a, b + 1)) / P.dx;
+  const double dudy = (U(a + 1, b + 1) - U(a + 1, b)) / Sp<S>::HYV(P, b + 1);
+  const double dvdx = (V(a + 1, b + 1) - V(a, b + 1)) / Sp<S>::HXU(P, a + 1);
   return 0.5 * (dudy + dvdx);
 }
 
-template <class FU, class FV>
+template <bool S, class FU, class FV>
 __device__ __forceinline__ void eddy_viscosity_body(const FU& U, const FV& V, const LesParams& P,
                                                     double* __restrict__ nuc,
                                                     double* __restrict__ nuk) {
   const Tile2D t = tile_coords(P.ny + 2);
   const int i = t.i, j = t.j;
   if (i > P.nx + 1 || j > P.ny + 1) return;
-  const double s11 = (U(i + 1, j) - U(i, j)) / P.dx;
-  const double s22 = (V(i, j + 1) - V(i, j)) / P.dy;
+  const double s11 = (U(i + 1, j) - U(i, j)) / Sp<S>::HXE(P, i);
+  const double s22 = (V(i, j + 1) - V(i, j)) / Sp<S>::HYE(P, j);
   const int ic = min(max(i, 1), P.nx), jc = min(max(j, 1), P.ny);
-  const double s12 = 0.25 * (s12_corner(U, V, ic - 1, jc - 1, P) + s12_corner(U, V, ic, jc - 1, P) +
-                             s12_corner(U, V, ic - 1, jc, P) + s12_corner(U, V, ic, jc, P));
+  const double s12 = 0.25 * (s12_corner<S>(U, V, ic - 1, jc - 1, P) +
+                             s12_corner<S>(U, V, ic, jc - 1, P) +
+                             s12_corner<S>(U, V, ic - 1, jc, P) + s12_corner<S>(U, V, ic, jc, P));
   const double s_mag = sqrt(2.0 * (s11 * s11) + 2.0 * (s22 * s22) + 4.0 * (s12 * s12));
-  nuc[(size_t)i * (P.ny + 2) + j] = P.nu + P.coeff * s_mag;
+  double coeff = P.coeff;
+  if constexpr (S) {  // (cs * sqrt(hx hy))^2, ghost ring repeating its neighbour
+    const double cd = P.cs * sqrt(P.g.hx[ic - 1] * P.g.hy[jc - 1]);
+    coeff = cd * cd;
+  }
+  nuc[(size_t)i * (P.ny + 2) + j] = P.nu + coeff * s_mag;
   (void)nuk;
 }
 
-template <class FU, class FV>
+template <bool S, class FU, class FV>
 __global__ void __launch_bounds__(kBlock)
 k_eddy_viscosity(FU U, FV V, LesParams P, double* __restrict__ nuc, double* __restrict__ nuk) {
-  eddy_viscosity_body(U, V, P, nuc, nuk);
+  eddy_viscosity_body<S>(U, V, P, nuc, nuk);
 }
 
 // Corner average of nu_c (turbulence.py:107-110), (nx+1, ny+1).
@@ -175,19 +294,21 @@ k_corner_viscosity(const double* __restrict__ nuc, int nx, int ny, double* __res
 }
 
 // rhs_u (nx+1, ny) and rhs_v (nx, ny+1), contiguous (solver.py:501-529).
+template <bool S>
 __global__ void __launch_bounds__(kBlock)
 k_momentum_rhs(const double* __restrict__ u, const double* __restrict__ v, MomParams P,
                double* __restrict__ rhs_u, double* __restrict__ rhs_v) {
   const Tile2D t = tile_coords(P.ny + 1);
   const int i = t.i, j = t.j;  // i: 0..nx, j: 0..ny
   PlainLoad U{u, P.ldu}, V{v, P.ldv};
-  if (i <= P.nx && j < P.ny) rhs_u[(size_t)i * P.ny + j] = rhs_u_face(U, V, i + 1, j + 1, P);
-  if (i < P.nx && j <= P.ny) rhs_v[(size_t)i * (P.ny + 1) + j] = rhs_v_face(U, V, i + 1, j + 1, P);
+  if (i <= P.nx && j < P.ny) rhs_u[(size_t)i * P.ny + j] = rhs_u_face<S>(U, V, i + 1, j + 1, P);
+  if (i < P.nx && j <= P.ny)
+    rhs_v[(size_t)i * (P.ny + 1) + j] = rhs_v_face<S>(U, V, i + 1, j + 1, P);
 }
 
 // Predictor us = u; us[u_upd] += dt*rhs_u (solver.py:617-622), both
 // components, over the full ghosted arrays.
-template <class FU, class FV>
+template <bool S, class FU, class FV>
 __device__ __forceinline__ void predictor_body(const FU& U, const FV& V, const MomParams& P,
                                                const UpdRange& R, double dt,
                                                double* __restrict__ us, double* __restrict__ vs) {
@@ -195,29 +316,31 @@ __device__ __forceinline__ void predictor_body(const FU& U, const FV& V, const M
   const int i = t.i, j = t.j;
   if (i <= P.nx + 2 && j <= P.ny + 1) {
     double val = U(i, j);
-    if (i >= R.u_lo && i < R.u_hi && j >= 1 && j <= P.ny) val += dt * rhs_u_face(U, V, i, j, P);
+    if (i >= R.u_lo && i < R.u_hi && j >= 1 && j <= P.ny) val += dt * rhs_u_face<S>(U, V, i, j, P);
     us[(size_t)i * P.ldu + j] = val;
   }
   if (i <= P.nx + 1 && j <= P.ny + 2) {
     double val = V(i, j);
-    if (i >= 1 && i <= P.nx && j >= R.v_lo && j < R.v_hi) val += dt * rhs_v_face(U, V, i, j, P);
+    if (i >= 1 && i <= P.nx && j >= R.v_lo && j < R.v_hi) val += dt * rhs_v_face<S>(U, V, i, j, P);
     vs[(size_t)i * P.ldv + j] = val;
   }
 }
 
+template <bool S>
 __global__ void __launch_bounds__(kBlock)
 k_predictor(const double* __restrict__ u, const double* __restrict__ v, MomParams P,
             UpdRange R, double dt, double* __restrict__ us, double* __restrict__ vs) {
-  predictor_body(PlainLoad{u, P.ldu}, PlainLoad{v, P.ldv}, P, R, dt, us, vs);
+  predictor_body<S>(PlainLoad{u, P.ldu}, PlainLoad{v, P.ldv}, P, R, dt, us, vs);
 }
 
+template <bool S>
 __global__ void __launch_bounds__(kBlock)
 k_predictor_comb(const double* __restrict__ u0, const double* __restrict__ ue,
                  const double* __restrict__ v0, const double* __restrict__ ve, double ca,
                  double cb, MomParams P, UpdRange R, double dt, double* __restrict__ us,
                  double* __restrict__ vs) {
-  predictor_body(CombLoad{u0, ue, P.ldu, ca, cb}, CombLoad{v0, ve, P.ldv, ca, cb}, P, R, dt, us,
-                 vs);
+  predictor_body<S>(CombLoad{u0, ue, P.ldu, ca, cb}, CombLoad{v0, ve, P.ldv, ca, cb}, P, R, dt,
+                    us, vs);
 }
 
 // out = ca*a + cb*b over both ghosted velocity arrays (solver.py:706-721).
@@ -238,32 +361,38 @@ k_combine(const double* __restrict__ u0, const double* __restrict__ ue,
 }
 
 // Gradient correction u[u_upd] -= dt*(p[i]-p[i-1])/dx, same for v
-// (solver.py:677-682), in place.
+// (solver.py:677-682), in place; divided by hxu / hyv of the face on a
+// stretched mesh.
 __global__ void __launch_bounds__(kBlock)
 k_correct(const double* __restrict__ p, int ldp, int nx, int ny, int ldu, int ldv, double dx,
-          double dy, UpdRange R, double dt, double* __restrict__ u, double* __restrict__ v) {
+          double dy, Metrics M, UpdRange R, double dt, double* __restrict__ u,
+          double* __restrict__ v) {
   const Tile2D t = tile_coords(ny + 3);
   const int i = t.i, j = t.j;
   if (i >= R.u_lo && i < R.u_hi && j >= 1 && j <= ny) {
-    const double g = dt * (p[(size_t)i * ldp + j] - p[(size_t)(i - 1) * ldp + j]) / dx;
+    const double h = M.hxu ? M.hxu[i - 1] : dx;
+    const double g = dt * (p[(size_t)i * ldp + j] - p[(size_t)(i - 1) * ldp + j]) / h;
     u[(size_t)i * ldu + j] -= g;
   }
   if (i >= 1 && i <= nx && j >= R.v_lo && j < R.v_hi) {
-    const double g = dt * (p[(size_t)i * ldp + j] - p[(size_t)i * ldp + j - 1]) / dy;
+    const double h = M.hyv ? M.hyv[j - 1] : dy;
+    const double g = dt * (p[(size_t)i * ldp + j] - p[(size_t)i * ldp + j - 1]) / h;
     v[(size_t)i * ldv + j] -= g;
   }
 }
 
-// Raw divergence (nx, ny) (solver.py:532-543).
+// Raw divergence (nx, ny) (solver.py:532-543), over each cell's own width
+// and height on a stretched mesh.
 __global__ void __launch_bounds__(kBlock)
 k_divergence(const double* __restrict__ u, const double* __restrict__ v, int nx, int ny, int ldu,
-             int ldv, double dx, double dy, double* __restrict__ out) {
+             int ldv, double dx, double dy, Metrics M, double* __restrict__ out) {
   const Tile2D t = tile_coords(ny);
   const int i = t.i, j = t.j;
   if (i < nx && j < ny) {
     const int a = i + 1, b = j + 1;
-    out[(size_t)i * ny + j] = (u[(size_t)(a + 1) * ldu + b] - u[(size_t)a * ldu + b]) / dx +
-                              (v[(size_t)a * ldv + b + 1] - v[(size_t)a * ldv + b]) / dy;
+    const double hx = M.hx ? M.hx[i] : dx, hy = M.hy ? M.hy[j] : dy;
+    out[(size_t)i * ny + j] = (u[(size_t)(a + 1) * ldu + b] - u[(size_t)a * ldu + b]) / hx +
+                              (v[(size_t)a * ldv + b + 1] - v[(size_t)a * ldv + b]) / hy;
   }
 }
 
@@ -271,12 +400,13 @@ k_divergence(const double* __restrict__ u, const double* __restrict__ v, int nx,
 // fields.py:70-96; solver.py:545-550):
 //   out[0] max|u_phys|  out[1] max|v_phys|  out[2] non-finite count (u,v,p)
 //   out[3] max|u-u_prev| out[4] max|v-v_prev| out[5] max|div| over fluid
-//   out[6] sum(uc^2+vc^2) over cells
+//   out[6] sum(uc^2+vc^2) over cells, each weighted by its area on a
+//          stretched mesh (mesh.py:213-223)
 __global__ void __launch_bounds__(kBlock)
 k_diagnostics(const double* __restrict__ u, const double* __restrict__ v,
               const double* __restrict__ p, const double* __restrict__ up,
               const double* __restrict__ vp, const uint8_t* __restrict__ solid, int nx, int ny,
-              int ldu, int ldv, int ldp, double dx, double dy, double* partial) {
+              int ldu, int ldv, int ldp, double dx, double dy, Metrics M, double* partial) {
   const Tile2D t = tile_coords(ny + 3);
   const int i = t.i, j = t.j;
   double r[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -300,12 +430,14 @@ k_diagnostics(const double* __restrict__ u, const double* __restrict__ v,
   if (i >= 1 && i <= nx && j >= 1 && j <= ny) {
     const double ua = u[(size_t)i * ldu + j], ub = u[(size_t)(i + 1) * ldu + j];
     const double va = v[(size_t)i * ldv + j], vb = v[(size_t)i * ldv + j + 1];
+    const double hx = M.hx ? M.hx[i - 1] : dx, hy = M.hy ? M.hy[j - 1] : dy;
     if (!solid || !solid[(size_t)(i - 1) * ny + (j - 1)]) {
-      const double d = (ub - ua) / dx + (vb - va) / dy;
+      const double d = (ub - ua) / hx + (vb - va) / hy;
       r[5] = fabs(d);
     }
     const double uc = 0.5 * (ua + ub), vc = 0.5 * (va + vb);
     r[6] = uc * uc + vc * vc;
+    if (M.hx) r[6] = r[6] * (hx * hy);
   }
   const int ops[7] = {kMax, kMax, kSum, kMax, kMax, kMax, kSum};
   block_partials<7>(r, ops, partial, t.tile);
@@ -332,13 +464,23 @@ int launch_eddy_viscosity(hipStream_t s, const LesParams& P, const double* u0, c
                           const double* v0, const double* ve, double ca, double cb, double* nuc,
                           double* nu_corner) {
   const dim3 g = grid_for(P.nx + 2, P.ny + 2);
+  const bool st = P.g.hx != nullptr;
   if (ue) {
-    hipLaunchKernelGGL((k_eddy_viscosity<CombLoad, CombLoad>), g, dim3(kBlock), 0, s,
-                       CombLoad{u0, ue, P.ldu, ca, cb}, CombLoad{v0, ve, P.ldv, ca, cb}, P, nuc,
-                       nu_corner);
+    const CombLoad U{u0, ue, P.ldu, ca, cb}, V{v0, ve, P.ldv, ca, cb};
+    if (st)
+      hipLaunchKernelGGL((k_eddy_viscosity<true, CombLoad, CombLoad>), g, dim3(kBlock), 0, s, U,
+                         V, P, nuc, nu_corner);
+    else
+      hipLaunchKernelGGL((k_eddy_viscosity<false, CombLoad, CombLoad>), g, dim3(kBlock), 0, s, U,
+                         V, P, nuc, nu_corner);
   } else {
-    hipLaunchKernelGGL((k_eddy_viscosity<PlainLoad, PlainLoad>), g, dim3(kBlock), 0, s,
-                       PlainLoad{u0, P.ldu}, PlainLoad{v0, P.ldv}, P, nuc, nu_corner);
+    const PlainLoad U{u0, P.ldu}, V{v0, P.ldv};
+    if (st)
+      hipLaunchKernelGGL((k_eddy_viscosity<true, PlainLoad, PlainLoad>), g, dim3(kBlock), 0, s,
+                         U, V, P, nuc, nu_corner);
+    else
+      hipLaunchKernelGGL((k_eddy_viscosity<false, PlainLoad, PlainLoad>), g, dim3(kBlock), 0, s,
+                         U, V, P, nuc, nu_corner);
   }
   PCFD_LAUNCH_CHECK();
   if (nu_corner) {
@@ -351,16 +493,22 @@ int launch_eddy_viscosity(hipStream_t s, const LesParams& P, const double* u0, c
 
 int launch_momentum_rhs(hipStream_t s, const double* u, const double* v, const MomParams& P,
                         double* rhs_u, double* rhs_v) {
-  hipLaunchKernelGGL(k_momentum_rhs, grid_for(P.nx + 1, P.ny + 1), dim3(kBlock), 0, s, u, v, P,
-                     rhs_u, rhs_v);
+  const dim3 g = grid_for(P.nx + 1, P.ny + 1);
+  if (P.g.hx)
+    hipLaunchKernelGGL(k_momentum_rhs<true>, g, dim3(kBlock), 0, s, u, v, P, rhs_u, rhs_v);
+  else
+    hipLaunchKernelGGL(k_momentum_rhs<false>, g, dim3(kBlock), 0, s, u, v, P, rhs_u, rhs_v);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
 
 int launch_predictor(hipStream_t s, const double* u, const double* v, const MomParams& P,
                      const UpdRange& R, double dt, double* us, double* vs) {
-  hipLaunchKernelGGL(k_predictor, grid_for(P.nx + 3, P.ny + 3), dim3(kBlock), 0, s, u, v, P, R,
-                     dt, us, vs);
+  const dim3 g = grid_for(P.nx + 3, P.ny + 3);
+  if (P.g.hx)
+    hipLaunchKernelGGL(k_predictor<true>, g, dim3(kBlock), 0, s, u, v, P, R, dt, us, vs);
+  else
+    hipLaunchKernelGGL(k_predictor<false>, g, dim3(kBlock), 0, s, u, v, P, R, dt, us, vs);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
@@ -368,8 +516,13 @@ int launch_predictor(hipStream_t s, const double* u, const double* v, const MomP
 int launch_predictor_comb(hipStream_t s, const double* u0, const double* ue, const double* v0,
                           const double* ve, double ca, double cb, const MomParams& P,
                           const UpdRange& R, double dt, double* us, double* vs) {
-  hipLaunchKernelGGL(k_predictor_comb, grid_for(P.nx + 3, P.ny + 3), dim3(kBlock), 0, s, u0, ue,
-                     v0, ve, ca, cb, P, R, dt, us, vs);
+  const dim3 g = grid_for(P.nx + 3, P.ny + 3);
+  if (P.g.hx)
+    hipLaunchKernelGGL(k_predictor_comb<true>, g, dim3(kBlock), 0, s, u0, ue, v0, ve, ca, cb, P,
+                       R, dt, us, vs);
+  else
+    hipLaunchKernelGGL(k_predictor_comb<false>, g, dim3(kBlock), 0, s, u0, ue, v0, ve, ca, cb, P,
+                       R, dt, us, vs);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
@@ -384,28 +537,29 @@ int launch_combine(hipStream_t s, const double* u0, const double* ue, const doub
 }
 
 int launch_correct(hipStream_t s, const double* p, int ldp, int nx, int ny, int ldu, int ldv,
-                   double dx, double dy, const UpdRange& R, double dt, double* u, double* v) {
+                   double dx, double dy, const Metrics& g, const UpdRange& R, double dt, double* u,
+                   double* v) {
   hipLaunchKernelGGL(k_correct, grid_for(nx + 3, ny + 3), dim3(kBlock), 0, s, p, ldp, nx, ny, ldu,
-                     ldv, dx, dy, R, dt, u, v);
+                     ldv, dx, dy, g, R, dt, u, v);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
 
 int launch_divergence(hipStream_t s, const double* u, const double* v, int nx, int ny, int ldu,
-                      int ldv, double dx, double dy, double* out) {
+                      int ldv, double dx, double dy, const Metrics& g, double* out) {
   hipLaunchKernelGGL(k_divergence, grid_for(nx, ny), dim3(kBlock), 0, s, u, v, nx, ny, ldu, ldv,
-                     dx, dy, out);
+                     dx, dy, g, out);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
 
 int launch_diagnostics(hipStream_t s, const double* u, const double* v, const double* p,
                        const double* up, const double* vp, const uint8_t* solid, int nx, int ny,
-                       int ldu, int ldv, int ldp, double dx, double dy, double* partial,
-                       unsigned int* counter, double* out) {
+                       int ldu, int ldv, int ldp, double dx, double dy, const Metrics& g,
+                       double* partial, unsigned int* counter, double* out) {
   (void)counter;
   hipLaunchKernelGGL(k_diagnostics, grid_for(nx + 3, ny + 3), dim3(kBlock), 0, s, u, v, p, up,
-                     vp, solid, nx, ny, ldu, ldv, ldp, dx, dy, partial);
+                     vp, solid, nx, ny, ldu, ldv, ldp, dx, dy, g, partial);
   PCFD_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_fold<EpiDiag>, dim3(1), dim3(kBlock), 0, s, (const double*)partial,
                      num_tiles(nx + 3, ny + 3), EpiDiag{out});

@@ -119,15 +119,42 @@ __device__ __forceinline__ uint8_t cell_code(const FineOp& A, uint8_t f, int i, 
   return code;
 }
 
+// Row coefficients of a region cell: its four face coefficients, diagonal
+// and diagonal inverse.  Uniform mesh (S = false): the constants and the
+// code-indexed tables.  Stretched mesh (S = true): formed from the metric
+// arrays at the cell's grid position (gi, gj unwrapped; a stretched axis is
+// never periodic and a periodic one is uniform, so wrapping only picks an
+// equal value), in the accumulation order of fine_diag.  A cell outside the
+// grid or not solvable has code 0: an identity row with no faces.
+template <typename T>
+struct RCoef {
+  T l, r, b, t, d, di;
+};
+
+template <typename T, bool S>
+__device__ __forceinline__ RCoef<T> rcoef(const FineOp& A, uint8_t f, int gi, int gj, const T* td,
+                                          const T* ti) {
+  if constexpr (!S) {
+    return {(T)A.cx, (T)A.cx, (T)A.cy, (T)A.cy, td[f], ti[f]};
+  } else {
+    if (!(f & F_SOLV)) return {(T)0, (T)0, (T)0, (T)0, (T)1, (T)1};
+    const int i = A.px ? wrap_index(gi, A.nx) : gi;
+    const int j = A.py ? wrap_index(gj, A.ny) : gj;
+    const FaceC c = fine_faces(A, i, j);
+    const double d = fine_diag(A, c, f, i, j);
+    return {(T)c.l, (T)c.r, (T)c.b, (T)c.t, (T)d, (T)(1.0 / d)};
+  }
+}
+
 // (A x) at region cell idx, terms in CSR column order.
 template <typename T>
-__device__ __forceinline__ T rapply(const FineOp& A, uint8_t f, const T* X, int idx, int ry, T d) {
+__device__ __forceinline__ T rapply(const RCoef<T>& c, uint8_t f, const T* X, int idx, int ry) {
   T y = (T)0;
-  if (f & F_L) y += (T)A.cx * X[idx - ry];
-  if (f & F_B) y += (T)A.cy * X[idx - 1];
-  y += d * X[idx];
-  if (f & F_T) y += (T)A.cy * X[idx + 1];
-  if (f & F_R) y += (T)A.cx * X[idx + ry];
+  if (f & F_L) y += c.l * X[idx - ry];
+  if (f & F_B) y += c.b * X[idx - 1];
+  y += c.d * X[idx];
+  if (f & F_T) y += c.t * X[idx + 1];
+  if (f & F_R) y += c.r * X[idx + ry];
   return y;
 }
 
@@ -146,7 +173,7 @@ struct FineLds {
   }
 };
 
-template <typename T>
+template <typename T, bool S>
 __global__ void __launch_bounds__(kBlock)
 k_fine_down(FineOp A, FusedTile ft, T w, int nu, int nonzero, const double* __restrict__ r,
             const T* __restrict__ b_in, const T* __restrict__ x_in, int shift_in,
@@ -182,7 +209,7 @@ k_fine_down(FineOp A, FusedTile ft, T w, int nu, int nonzero, const double* __re
         const T x0 = x_in[k];
         xv = (f & F_SOLV) ? x0 - m : x0;
       } else {
-        xv = w * ti[f] * bv;
+        xv = w * rcoef<T, S>(A, f, i, j, td, ti).di * bv;
       }
     }
     const int idx = a * ry + c;
@@ -199,7 +226,8 @@ k_fine_down(FineOp A, FusedTile ft, T w, int nu, int nonzero, const double* __re
     for_ring(ring(ft, v, 1, 1), [&](int a, int c) {
       const int idx = a * ry + c;
       const uint8_t f = F[idx];
-      Y[idx] = X[idx] + (w * ti[f]) * (B[idx] - rapply(A, f, X, idx, ry, td[f]));
+      const RCoef<T> cf = rcoef<T, S>(A, f, o.gi0 + a, o.gj0 + c, td, ti);
+      Y[idx] = X[idx] + (w * cf.di) * (B[idx] - rapply(cf, f, X, idx, ry));
     });
     __syncthreads();
     T* t = X;
@@ -211,7 +239,7 @@ k_fine_down(FineOp A, FusedTile ft, T w, int nu, int nonzero, const double* __re
   for_ring(ring(ft, 1, 1, 1), [&](int a, int c) {
     const int idx = a * ry + c;
     const uint8_t f = F[idx];
-    Y[idx] = B[idx] - rapply(A, f, X, idx, ry, td[f]);
+    Y[idx] = B[idx] - rapply(rcoef<T, S>(A, f, o.gi0 + a, o.gj0 + c, td, ti), f, X, idx, ry);
   });
   __syncthreads();
   // Hand the tile's iterate (and right-hand side) over, then X <- D^-1 r.
@@ -226,7 +254,7 @@ k_fine_down(FineOp A, FusedTile ft, T w, int nu, int nonzero, const double* __re
         if (b_out) b_out[k] = B[idx];
       }
     }
-    X[idx] = ti[F[idx]] * Y[idx];
+    X[idx] = rcoef<T, S>(A, F[idx], o.gi0 + a, o.gj0 + c, td, ti).di * Y[idx];
   });
   __syncthreads();
   // Restriction: one thread per coarse cell of the tile (cty is a power of two).
@@ -246,14 +274,14 @@ k_fine_down(FineOp A, FusedTile ft, T w, int nu, int nonzero, const double* __re
         const int idx = (ft.hx + ca * fx + a2) * ry + (ft.hy + cc * fy + c2);
         const uint8_t f = F[idx];
         if (!(f & F_SOLV)) continue;
-        acc += Y[idx] - w * rapply(A, f, X, idx, ry, td[f]);
+        acc += Y[idx] - w * rapply(rcoef<T, S>(A, f, i, j, td, ti), f, X, idx, ry);
       }
     }
     bc[I * cny + J] = acc;
   }
 }
 
-template <typename T>
+template <typename T, bool S>
 __global__ void __launch_bounds__(kBlock)
 k_fine_up(FineOp A, FusedTile ft, T w, int nu, const T* __restrict__ x_pre,
           const T* __restrict__ b0, const T* __restrict__ xc, int fx, int fy, int cny,
@@ -294,8 +322,9 @@ k_fine_up(FineOp A, FusedTile ft, T w, int nu, const T* __restrict__ x_pre,
   for_ring(ring(ft, nu, 1, 1), [&](int a, int c) {
     const int idx = a * ry + c;
     const uint8_t f = F[idx];
-    const T ae = rapply(A, f, E, idx, ry, td[f]);
-    X[idx] += E[idx] - w * ti[f] * ae;
+    const RCoef<T> cf = rcoef<T, S>(A, f, o.gi0 + a, o.gj0 + c, td, ti);
+    const T ae = rapply(cf, f, E, idx, ry);
+    X[idx] += E[idx] - w * cf.di * ae;
   });
   __syncthreads();
   // Post-smoothing; the last sweep writes the tile straight to z.
@@ -306,7 +335,8 @@ k_fine_up(FineOp A, FusedTile ft, T w, int nu, const T* __restrict__ x_pre,
     for_ring(ring(ft, nu - sw, 1, 1), [&](int a, int c) {
       const int idx = a * ry + c;
       const uint8_t f = F[idx];
-      const T val = X[idx] + (w * ti[f]) * (B[idx] - rapply(A, f, X, idx, ry, td[f]));
+      const RCoef<T> cf = rcoef<T, S>(A, f, o.gi0 + a, o.gj0 + c, td, ti);
+      const T val = X[idx] + (w * cf.di) * (B[idx] - rapply(cf, f, X, idx, ry));
       if (!last) {
         Y[idx] = val;
         return;
@@ -337,7 +367,7 @@ k_fine_up(FineOp A, FusedTile ft, T w, int nu, const T* __restrict__ x_pre,
 // p = zp + beta p_old with zp = z - mean_z on solvable cells (0 elsewhere),
 // q = A p, and the tile's partial of p.q.  p is formed once per region cell
 // in LDS (halo 1), then q on the tile.
-template <typename T>
+template <typename T, bool S>
 __global__ void __launch_bounds__(kBlock)
 k_fine_dir(FineOp A, FusedTile ft, const T* __restrict__ z, const double* __restrict__ p_old,
            double* __restrict__ p_new, double* __restrict__ q, double* partial,
@@ -376,7 +406,7 @@ k_fine_dir(FineOp A, FusedTile ft, const T* __restrict__ z, const double* __rest
     const int idx = a * ry + c;
     const uint8_t f = F[idx];
     const double pk = P[idx];
-    const double qk = rapply(A, f, P, idx, ry, td[f]);
+    const double qk = rapply(rcoef<double, S>(A, f, gi, gj, td, ti), f, P, idx, ry);
     const int k = gi * A.ny + gj;
     p_new[k] = pk;
     q[k] = qk;
@@ -645,8 +675,14 @@ int launch_fine_down(hipStream_t s, const FineOp& A, const FusedTile& ft, T w, i
                      int nonzero, const double* r, const T* b_in, const T* x_in, int shift_in,
                      T* b_out, T* x_out, T* bc, int fx, int fy, int cnx, int cny,
                      const SolveState* st) {
-  hipLaunchKernelGGL(k_fine_down<T>, dim3(ft.ntiles()), dim3(kBlock), ft.lds, s, A, ft, w, nu,
-                     nonzero, r, b_in, x_in, shift_in, b_out, x_out, bc, fx, fy, cnx, cny, st);
+  if (A.hxc)
+    hipLaunchKernelGGL((k_fine_down<T, true>), dim3(ft.ntiles()), dim3(kBlock), ft.lds, s, A, ft,
+                       w, nu, nonzero, r, b_in, x_in, shift_in, b_out, x_out, bc, fx, fy, cnx,
+                       cny, st);
+  else
+    hipLaunchKernelGGL((k_fine_down<T, false>), dim3(ft.ntiles()), dim3(kBlock), ft.lds, s, A, ft,
+                       w, nu, nonzero, r, b_in, x_in, shift_in, b_out, x_out, bc, fx, fy, cnx,
+                       cny, st);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
@@ -655,8 +691,12 @@ template <typename T>
 int launch_fine_up(hipStream_t s, const FineOp& A, const FusedTile& ft, T w, int nu,
                    const T* x_pre, const T* b0, const T* xc, int fx, int fy, int cny, T* z,
                    int reduce, double* partial, const SolveState* st) {
-  hipLaunchKernelGGL(k_fine_up<T>, dim3(ft.ntiles()), dim3(kBlock), ft.lds, s, A, ft, w, nu, x_pre,
-                     b0, xc, fx, fy, cny, z, reduce, partial, st);
+  if (A.hxc)
+    hipLaunchKernelGGL((k_fine_up<T, true>), dim3(ft.ntiles()), dim3(kBlock), ft.lds, s, A, ft, w,
+                       nu, x_pre, b0, xc, fx, fy, cny, z, reduce, partial, st);
+  else
+    hipLaunchKernelGGL((k_fine_up<T, false>), dim3(ft.ntiles()), dim3(kBlock), ft.lds, s, A, ft, w,
+                       nu, x_pre, b0, xc, fx, fy, cny, z, reduce, partial, st);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
@@ -685,8 +725,12 @@ template <typename T>
 int launch_fine_dir(hipStream_t s, const FineOp& A, const FusedTile& ft, const T* z,
                     const double* p_old, double* p_new, double* q, double* partial,
                     const SolveState* st) {
-  hipLaunchKernelGGL(k_fine_dir<T>, dim3(ft.ntiles()), dim3(kBlock), ft.lds, s, A, ft, z, p_old,
-                     p_new, q, partial, st);
+  if (A.hxc)
+    hipLaunchKernelGGL((k_fine_dir<T, true>), dim3(ft.ntiles()), dim3(kBlock), ft.lds, s, A, ft, z,
+                       p_old, p_new, q, partial, st);
+  else
+    hipLaunchKernelGGL((k_fine_dir<T, false>), dim3(ft.ntiles()), dim3(kBlock), ft.lds, s, A, ft,
+                       z, p_old, p_new, q, partial, st);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }

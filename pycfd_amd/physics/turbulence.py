@@ -12,29 +12,47 @@ from __future__ import annotations
 
 import numpy as np
 
-from ..core.mesh import NonUniformMeshError, StructuredMesh
+from ..core.mesh import StructuredMesh
 
 
 class SmagorinskyModel:
-    """turbulence.py:25-113 on a uniform mesh, evaluated on the GPU."""
+    """turbulence.py:25-113, evaluated on the GPU.
+
+    On a stretched mesh the filter width is per cell, ``sqrt(hx hy)``, and
+    ``coeff`` is the ghosted ``(nx+2, ny+2)`` array the reference keeps
+    (turbulence.py:47-58); the device forms the same values itself.
+    """
 
     def __init__(self, mesh: StructuredMesh, cs: float = 0.17) -> None:
         if cs < 0:
             raise ValueError(f"Smagorinsky constant must be non-negative, got {cs}")
-        if not mesh.is_uniform:
-            raise NonUniformMeshError("the GPU Smagorinsky closure needs a uniform mesh")
         self.mesh = mesh
         self.cs = float(cs)
-        self.delta = float(np.sqrt(mesh.dx * mesh.dy))
-        self._coeff = (self.cs * self.delta) ** 2
+        if mesh.is_uniform:
+            self.delta: float | np.ndarray = float(np.sqrt(mesh.dx * mesh.dy))
+            self._coeff: float | np.ndarray = (self.cs * self.delta) ** 2
+        else:
+            delta = np.sqrt(mesh.dx_cells[:, None] * mesh.dy_cells[None, :])
+            self.delta = delta
+            nx, ny = mesh.shape
+            coeff = np.empty((nx + 2, ny + 2))
+            coeff[1:nx + 1, 1:ny + 1] = (self.cs * delta) ** 2
+            coeff[0, :], coeff[nx + 1, :] = coeff[1, :], coeff[nx, :]
+            coeff[:, 0], coeff[:, ny + 1] = coeff[:, 1], coeff[:, ny]
+            self._coeff = coeff
         self.last_nu_t_max = 0.0
         self._ctx = None
         self._nu = None
 
     @property
-    def coeff(self) -> float:
-        """``(Cs Delta)^2``."""
+    def coeff(self):
+        """``(Cs Delta)^2``: a float, or the ghosted per-cell array on a stretched mesh."""
         return self._coeff
+
+    @property
+    def device_coeff(self) -> float:
+        """The scalar the device takes (unused on a stretched mesh, which sends ``cs``)."""
+        return float(self._coeff) if np.isscalar(self._coeff) else 0.0
 
     def bind(self, ctx, nu: float) -> None:
         """Attach the device context whose kernels evaluate the closure."""
@@ -60,7 +78,10 @@ class SmagorinskyModel:
         return nu_c_h, nu_k_h
 
     def __repr__(self) -> str:
-        return f"SmagorinskyModel(cs={self.cs:g}, delta={self.delta:.4g})"
+        if np.isscalar(self.delta):
+            return f"SmagorinskyModel(cs={self.cs:g}, delta={self.delta:.4g})"
+        return (f"SmagorinskyModel(cs={self.cs:g}, delta in "
+                f"[{self.delta.min():.4g}, {self.delta.max():.4g}])")
 
 
 __all__ = ["SmagorinskyModel"]

@@ -27,6 +27,18 @@ CASES = {
                      velocity=(1.0, 0.0, 0.0, 0.0)),
 }
 
+# Cases compared with the oracle only (the reference recorded nothing for them).
+ORACLE_CASES = {
+    # body-force channel, periodic in x (channel_flow.py periodic mode)
+    "pchannel": dict(nx=16, ny=24, lx=2.0, ly=1.0, re=10.0, dt=0.01, cfl_max=0.4,
+                     kinds=("periodic", "periodic", "no_slip", "no_slip"),
+                     body_force=(0.8, 0.0)),
+}
+
+
+def _base(base):
+    return CASES[base] if base in CASES else ORACLE_CASES[base]
+
 
 # Stretched variants of the cases above: base case plus stretching
 # (tests/golden/make_golden.py records the same four).
@@ -39,11 +51,18 @@ STRETCHED = {
     "s_cylp": dict(base="cyl_pout", stretch_y=1.05, cluster_y="centre"),
 }
 
+# Stretched cases checked against the oracle only (no reference recording):
+# a uniform periodic axis beside a stretched one.
+STRETCHED_ORACLE = {
+    "s_pchannel": dict(base="pchannel", stretch_y=1.08, cluster_y="walls"),
+}
+
 
 def _split(name):
-    if name in STRETCHED:
-        spec = dict(STRETCHED[name])
-        return spec.pop("base"), spec
+    for table in (STRETCHED, STRETCHED_ORACLE):
+        if name in table:
+            spec = dict(table[name])
+            return spec.pop("base"), spec
     return name, {}
 
 
@@ -65,7 +84,7 @@ def circle(nx, ny, lx, ly, cx, cy, r, sub=8, dx_cells=None, dy_cells=None):
 
 def oracle_case(name: str, **kw) -> Case:
     base, stretch = _split(name)
-    args = dict(CASES[base])
+    args = dict(_base(base))
     args.update(stretch)
     args.update(kw)
     case = Case(**args)
@@ -87,6 +106,9 @@ def initial(name: str, case: Case):
         Xu, Yu = np.meshgrid(case.xf, case.yc, indexing="ij")
         Xv, Yv = np.meshgrid(case.xc, case.yf, indexing="ij")
         return np.sin(Xu) * np.cos(Yu), -np.cos(Xv) * np.sin(Yv)
+    if name == "pchannel":
+        Xv, Yv = np.meshgrid(case.xc, case.yf, indexing="ij")
+        return None, 0.05 * np.sin(np.pi * Xv) * np.sin(np.pi * Yv)
     return None, None
 
 
@@ -96,7 +118,7 @@ def product_config(name: str, **kw):
     from pycfd_amd.core.mesh import StructuredMesh
     from pycfd_amd.geometry.obstacles import circle_mask
     base, stretch = _split(name)
-    c = CASES[base]
+    c = _base(base)
     walls = ("left", "right", "bottom", "top")
     prof = c.get("profile", ("uniform",) * 4)
     vel = c.get("velocity", (0.0,) * 4)
@@ -104,7 +126,8 @@ def product_config(name: str, **kw):
            for k, w in enumerate(walls)}
     args = dict(nx=c["nx"], ny=c["ny"], lx=c["lx"], ly=c["ly"], re=c["re"], dt=c["dt"],
                 cfl_max=c.get("cfl_max", 0.5), adaptive_dt=c.get("adaptive_dt", True),
-                t_end=100.0, boundary_config=bcs, name=name)
+                t_end=100.0, boundary_config=bcs, name=name,
+                body_force=tuple(c.get("body_force", (0.0, 0.0))))
     args.update(stretch)
     args.update(kw)
     cfg = SimulationConfig(**args)

@@ -51,13 +51,18 @@ def test_hierarchy_equals_reference(name, sweeps):
     np.testing.assert_allclose([lv.weight for lv in H.levels], g[f"{tag}_weights"], rtol=1e-13)
 
 
-@pytest.mark.parametrize("name", ["cavity", "periodic", "cyl_outlet", "cyl_pout"])
-def test_stencil_dictionary_rebuilds_operator(name):
-    """cls/table/offsets reproduce every coarse operator exactly."""
+@pytest.mark.parametrize("name,wide", [("cavity", False), ("periodic", False),
+                                       ("cyl_outlet", False), ("cyl_pout", False),
+                                       ("s_cyl", False), ("s_pchannel", False),
+                                       ("s_cyl", True)])
+def test_stencil_dictionary_rebuilds_operator(name, wide):
+    """cls/table/offsets reproduce every coarse operator exactly, with the
+    16-bit class index or the 32-bit one of wide levels."""
     S = product_system(name)
     H = MultigridHierarchy(S.A, S.shape, S.fluid, S.singular, 1, 1,
                            (S.periodic_x, S.periodic_y))
-    for lv, d in zip(H.levels[1:], H.device_levels()[1:]):
+    for lv, d in zip(H.levels[1:], H.device_levels(wide_classes=wide)[1:]):
+        assert d["cls"].dtype == (np.uint32 if wide else np.uint16)
         nx, ny = lv.shape
         n = nx * ny
         rows, cols, vals = [], [], []
@@ -196,3 +201,40 @@ def test_checkpoint_rejects_mismatched_shapes_and_missing_files(tmp_path):
     path = save_checkpoint(fields, cfg.replace(nx=9), tmp_path / "bad.npz")
     with pytest.raises(ValueError, match="does not match"):
         load_checkpoint(path)
+
+
+# ---------------------------------------------------------------- stretched meshes
+@pytest.mark.parametrize("name", ["s_cavity", "s_channel", "s_cyl", "s_cylp", "s_pchannel"])
+def test_stretched_assembly_equals_oracle(name):
+    """pressure.py:193-207: per-face coefficients, area row scale and
+    Dirichlet terms, bit for bit with the oracle (itself pinned to the
+    reference's vectors); the operator is symmetric."""
+    S = product_system(name)
+    R = O.assemble(oracle_case(name))
+    assert abs(S.A - R.A).max() == 0.0
+    assert np.array_equal(S.fluid, R.fluid)
+    assert np.array_equal(S.bc_rhs, R.bc_rhs)
+    assert np.array_equal(np.asarray(S.row_scale), np.asarray(R.row_scale))
+    assert S.singular == R.singular and S.pin == R.pin
+    assert abs(S.A - S.A.T).max() <= 1e-12 * abs(S.A).max()
+
+
+@pytest.mark.parametrize("name", ["s_cavity", "s_cyl"])
+def test_stretched_metrics_match_oracle(name):
+    """mesh.py:160-211 metric arrays, bit for bit."""
+    cfg, _ = product_config(name)
+    m = StructuredMesh.from_config(cfg).metrics
+    c = oracle_case(name)
+    for a, b in ((m.hx, c.hx), (m.hy, c.hy), (m.hxu, c.hxu), (m.hyv, c.hyv),
+                 (m.hx_ext, c.hx_ext), (m.hy_ext, c.hy_ext), (m.wx_corner, c.wx),
+                 (m.wy_corner, c.wy), (m.cell_areas, c.cell_areas)):
+        assert np.array_equal(a, b)
+    assert (m.min_hx, m.min_hy) == (c.min_hx, c.min_hy)
+
+
+def test_stretched_periodic_axis_is_refused():
+    """solver.py:209-230: a stretched axis may not be periodic."""
+    from pycfd_amd.core.solver import ProjectionSolver
+    cfg, _ = product_config("pchannel", stretch_x=1.05)
+    with pytest.raises(ValueError, match="periodic and stretched"):
+        ProjectionSolver(cfg)
