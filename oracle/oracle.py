@@ -213,14 +213,16 @@ def _momentum_rhs_stretched(case: Case, u, v, blend: float):
     hy_lo = (v0 - v[1:nx + 1, 0:ny + 1]) / _sl(case.hy_ext, 0, ny + 1, 1)
     d2v = (hx_hi - hx_lo) / case.hx + (hy_hi - hy_lo) / case.hyv
     fx, fy = case.body_force
+    if case.use_les:  # solver.py:523-528
+        nu_c, nu_k = eddy_viscosity(case, u, v)
+        d_u, d_v = diffusion_variable(case, u, v, nu_c, nu_k)
+        return -(duudx + duvdy) + d_u + fx, -(duvdx + dvvdy) + d_v + fy
     return (-(duudx + duvdy) + case.nu * d2u + fx,
             -(duvdx + dvvdy) + case.nu * d2v + fy)
 
 
 def momentum_rhs(case: Case, u, v, blend: float = 0.0):
     if not case.uniform:
-        if case.use_les:
-            raise NotImplementedError("the LES closure is restated for uniform meshes")
         return _momentum_rhs_stretched(case, u, v, blend)
     nx, ny, dx, dy = case.nx, case.ny, case.dx, case.dy
     uc = 0.5 * (u[1:nx + 2, 0:ny + 1] + u[1:nx + 2, 1:ny + 2])   # corner u
@@ -263,14 +265,16 @@ def momentum_rhs(case: Case, u, v, blend: float = 0.0):
 
 def eddy_viscosity(case: Case, u, v):
     """Smagorinsky nu + (Cs Delta)^2 |S| at cell centres (ghosted) and its
-    four-cell average at corners (turbulence.py:63-113, uniform mesh:
-    Delta = sqrt(dx dy)); |S| = sqrt(2 S11^2 + 2 S22^2 + 4 S12^2) with S12
-    averaged from the corners and copied into the ghost ring."""
+    four-cell average at corners (turbulence.py:40-113: Delta = sqrt(dx dy),
+    per cell on a stretched mesh with the ghost ring repeating its neighbour);
+    |S| = sqrt(2 S11^2 + 2 S22^2 + 4 S12^2) with S12 averaged from the corners
+    and copied into the ghost ring.  Normal rates are differenced across a
+    cell (ghost-extended widths), shear rates across a face."""
     nx, ny, dx, dy = case.nx, case.ny, case.dx, case.dy
-    s11 = (u[1:nx + 3, :] - u[0:nx + 2, :]) / dx
-    s22 = (v[:, 1:ny + 3] - v[:, 0:ny + 2]) / dy
-    dudy = (u[1:nx + 2, 1:ny + 2] - u[1:nx + 2, 0:ny + 1]) / dy
-    dvdx = (v[1:nx + 2, 1:ny + 2] - v[0:nx + 1, 1:ny + 2]) / dx
+    s11 = (u[1:nx + 3, :] - u[0:nx + 2, :]) / case.hx_ext
+    s22 = (v[:, 1:ny + 3] - v[:, 0:ny + 2]) / case.hy_ext
+    dudy = (u[1:nx + 2, 1:ny + 2] - u[1:nx + 2, 0:ny + 1]) / case.hyv
+    dvdx = (v[1:nx + 2, 1:ny + 2] - v[0:nx + 1, 1:ny + 2]) / case.hxu
     s12 = 0.5 * (dudy + dvdx)
     s12c = np.zeros_like(s11)
     s12c[1:nx + 1, 1:ny + 1] = 0.25 * (s12[0:nx, 0:ny] + s12[1:nx + 1, 0:ny]
@@ -278,7 +282,14 @@ def eddy_viscosity(case: Case, u, v):
     s12c[0, :], s12c[nx + 1, :] = s12c[1, :], s12c[nx, :]
     s12c[:, 0], s12c[:, ny + 1] = s12c[:, 1], s12c[:, ny]
     s_mag = np.sqrt(2.0 * s11 ** 2 + 2.0 * s22 ** 2 + 4.0 * s12c ** 2)
-    coeff = (case.smagorinsky_cs * float(np.sqrt(dx * dy))) ** 2
+    if case.uniform:
+        coeff = (case.smagorinsky_cs * float(np.sqrt(dx * dy))) ** 2
+    else:
+        delta = np.sqrt(case.dx_cells[:, None] * case.dy_cells[None, :])
+        coeff = np.empty((nx + 2, ny + 2))
+        coeff[1:nx + 1, 1:ny + 1] = (case.smagorinsky_cs * delta) ** 2
+        coeff[0, :], coeff[nx + 1, :] = coeff[1, :], coeff[nx, :]
+        coeff[:, 0], coeff[:, ny + 1] = coeff[:, 1], coeff[:, ny]
     nu_c = case.nu + coeff * s_mag
     nu_k = 0.25 * (nu_c[0:nx + 1, 0:ny + 1] + nu_c[1:nx + 2, 0:ny + 1]
                    + nu_c[0:nx + 1, 1:ny + 2] + nu_c[1:nx + 2, 1:ny + 2])
@@ -287,19 +298,20 @@ def eddy_viscosity(case: Case, u, v):
 
 def diffusion_variable(case: Case, u, v, nu_c, nu_k):
     """Stress-divergence viscous terms for a variable viscosity (solver.py:467-499):
-    corner shear stress nu_k (du/dy + dv/dx), centre normal stresses 2 nu_c du/dx."""
-    nx, ny, dx, dy = case.nx, case.ny, case.dx, case.dy
-    dudy = (u[1:nx + 2, 1:ny + 2] - u[1:nx + 2, 0:ny + 1]) / dy
-    dvdx = (v[1:nx + 2, 1:ny + 2] - v[0:nx + 1, 1:ny + 2]) / dx
+    corner shear stress nu_k (du/dy + dv/dx), centre normal stresses 2 nu_c du/dx;
+    each difference over the span it crosses (scalars on a uniform mesh)."""
+    nx, ny = case.nx, case.ny
+    dudy = (u[1:nx + 2, 1:ny + 2] - u[1:nx + 2, 0:ny + 1]) / case.hyv
+    dvdx = (v[1:nx + 2, 1:ny + 2] - v[0:nx + 1, 1:ny + 2]) / case.hxu
     tau = nu_k * (dudy + dvdx)
-    dudx = (u[1:nx + 3, :] - u[0:nx + 2, :]) / dx
-    dvdy = (v[:, 1:ny + 3] - v[:, 0:ny + 2]) / dy
+    dudx = (u[1:nx + 3, :] - u[0:nx + 2, :]) / case.hx_ext
+    dvdy = (v[:, 1:ny + 3] - v[:, 0:ny + 2]) / case.hy_ext
     d_u = (2.0 * (nu_c[1:nx + 2, 1:ny + 1] * dudx[1:nx + 2, 1:ny + 1]
-                  - nu_c[0:nx + 1, 1:ny + 1] * dudx[0:nx + 1, 1:ny + 1]) / dx
-           + (tau[0:nx + 1, 1:ny + 1] - tau[0:nx + 1, 0:ny]) / dy)
-    d_v = ((tau[1:nx + 1, 0:ny + 1] - tau[0:nx, 0:ny + 1]) / dx
+                  - nu_c[0:nx + 1, 1:ny + 1] * dudx[0:nx + 1, 1:ny + 1]) / case.hxu
+           + (tau[0:nx + 1, 1:ny + 1] - tau[0:nx + 1, 0:ny]) / case.hy)
+    d_v = ((tau[1:nx + 1, 0:ny + 1] - tau[0:nx, 0:ny + 1]) / case.hx
            + 2.0 * (nu_c[1:nx + 1, 1:ny + 2] * dvdy[1:nx + 1, 1:ny + 2]
-                    - nu_c[1:nx + 1, 0:ny + 1] * dvdy[1:nx + 1, 0:ny + 1]) / dy)
+                    - nu_c[1:nx + 1, 0:ny + 1] * dvdy[1:nx + 1, 0:ny + 1]) / case.hyv)
     return d_u, d_v
 
 

@@ -290,6 +290,40 @@ def stretched_fixture():
     np.savez_compressed(OUT / "stretched.npz", **out)
 
 
+def les_stretched_fixture():
+    """Smagorinsky closure on stretched meshes: per-cell filter width and the
+    array-metric strain rates and stress form (turbulence.py:40-113,
+    solver.py:467-529) on random fields, and short LES trajectories."""
+    from pycfd.core.mesh import StructuredMesh
+    rng = np.random.default_rng(777)
+    out = {}
+    for name in ("s_cavity", "s_cyl"):
+        cfg, base = stretched_config(name, use_les=True, smagorinsky_cs=0.17, use_numba=False)
+        mesh = StructuredMesh.from_config(cfg)
+        body = body_for(base, mesh)
+        solver = ProjectionSolver(cfg, mesh, body.mask if body else None)
+        nx, ny = mesh.shape
+        u = rng.standard_normal((nx + 3, ny + 2))
+        v = rng.standard_normal((nx + 2, ny + 3))
+        nu_c, nu_k = solver.turbulence.eddy_viscosity(u, v, solver.nu)
+        out[f"{name}_u"], out[f"{name}_v"] = u, v
+        out[f"{name}_nu_c"], out[f"{name}_nu_corner"] = nu_c, nu_k
+        for blend in (0.0, 0.7):
+            ru, rv = solver.momentum_rhs(u, v, blend)
+            tag = f"{name}_b{int(blend * 10)}"
+            out[f"{tag}_rhs_u"], out[f"{tag}_rhs_v"] = ru.copy(), rv.copy()
+        cfg2, _ = stretched_config(name, pressure_solver="direct", poisson_tol=1e-12,
+                                   use_les=True, smagorinsky_cs=0.17, use_numba=False)
+        u0, v0 = init_for(base, mesh)
+        sim = Simulation(cfg2, obstacle=body, u_init=u0, v_init=v0)
+        res = sim.run(max_steps=8)
+        out[f"traj_{name}_u"], out[f"traj_{name}_v"] = sim.fields.u, sim.fields.v
+        out[f"traj_{name}_p"] = sim.fields.p
+        for k in ("dt", "kinetic_energy", "fx"):
+            out[f"traj_{name}_{k}"] = np.asarray(res.history[k])
+    np.savez_compressed(OUT / "les_stretched.npz", **out)
+
+
 def mask_fixture():
     from pycfd.core.mesh import StructuredMesh
     out = {}
@@ -303,9 +337,9 @@ def mask_fixture():
 
 if __name__ == "__main__":
     which = sys.argv[1:] or ["rhs", "boundary", "trajectory", "multigrid", "mask", "les",
-                             "stretched"]
+                             "stretched", "les_stretched"]
     for w in which:
         print("writing", w, flush=True)
         {"rhs": rhs_fixture, "boundary": boundary_fixture, "trajectory": trajectory_fixture,
          "multigrid": multigrid_fixture, "mask": mask_fixture, "les": les_fixture,
-         "stretched": stretched_fixture}[w]()
+         "stretched": stretched_fixture, "les_stretched": les_stretched_fixture}[w]()
