@@ -3,12 +3,22 @@
 Workload: flow past a cylinder, 2048 x 1024 channel (32 x 16 D), Re = 100,
 uniform inflow, zero-gradient OUTLET, slip walls, D = 1 body at 4 D, v seeded
 with a 1e-3 Gaussian blob; SSP-RK3, adaptive dt (ceiling 0.02, CFL 0.4);
-pressure by multigrid-preconditioned CG to relative residual 1e-10.  Both
-precision paths run: plain FP64, and mixed (FP32 V-cycle inside FP64 CG).
+pressure by multigrid-preconditioned CG.  Both precision paths run: plain
+FP64, and mixed (FP32 V-cycle inside FP64 CG), at the same fixed divergence
+target: the relative residual tolerance (2.5e-11 by default) is chosen so
+that max|div u| <= 1e-10 after every step on both paths, and the line says
+whether it held.
 
 One "step" is one full SSP-RK3 time step (three projections) through
 Simulation.run, per-step diagnostics included, with fields resident in HBM.
 MLUPS = nx * ny * steps / wall seconds / 1e6.
+
+Kernel rooflines come from a second window of the same length right after
+the timed one, with the engine's live kernel clock on: every launch of the
+four fine-level kernels inside the real solves records start/stop events on
+the solver's stream (hipExtLaunchKernelGGL).  The clock is off in the MLUPS
+window.  HBM traffic per launch is read from the committed rocprofv3 counter
+summary when one exists for the kernel (profiles/pmc_traffic_*.md).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -26,15 +36,38 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-KERNELS = {0: "k_cg_update", 1: "k_fine_dir<double>", 2: "k_fine_down<double>",
-           3: "k_fine_up<double>"}
+DIV_TARGET = 1.0e-10
+KERNEL_NAMES = ("k_cg_update", "k_fine_dir<{t}>", "k_fine_down<{t}>", "k_fine_up<{t}>")
+TRAFFIC_FILE = ROOT / "profiles" / "pmc_traffic_cylinder2048x1024.md"
 
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_path(precision: str, steps: int, warmup: int, nx: int, ny: int):
+def kernel_names(precision: str) -> list[str]:
+    t = "float" if precision == "mixed" else "double"
+    return [k.format(t=t) for k in KERNEL_NAMES]
+
+
+def load_traffic() -> dict:
+    """Measured HBM bytes per launch by kernel, from the fenced JSON block of
+    the committed counter summary ({} when absent)."""
+    if not TRAFFIC_FILE.exists():
+        return {}
+    text = TRAFFIC_FILE.read_text()
+    start = text.find("```json")
+    if start < 0:
+        return {}
+    end = text.find("```", start + 7)
+    try:
+        return json.loads(text[start + 7:end])
+    except ValueError:
+        return {}
+
+
+def run_path(precision: str, steps: int, warmup: int, nx: int, ny: int, tol: float,
+             traffic: dict):
     import numpy as np
     import torch
 
@@ -42,7 +75,7 @@ def run_path(precision: str, steps: int, warmup: int, nx: int, ny: int):
     from pycfd_amd.physics.incompressible import Simulation
 
     cfg, body, u0, v0 = cases.cylinder(nx=nx, ny=ny, pressure_solver="mgcg",
-                                       poisson_tol=1e-10, poisson_maxiter=300,
+                                       poisson_tol=tol, poisson_maxiter=300,
                                        precision=precision)
     t0 = time.perf_counter()
     sim = Simulation(cfg, obstacle=body, u_init=u0, v_init=v0)
@@ -61,10 +94,12 @@ def run_path(precision: str, steps: int, warmup: int, nx: int, ny: int):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     hist = res.history
+    max_div = float(max(hist["max_div"]))
     out = {
         "mlups": nx * ny * steps / wall / 1e6,
         "ms_per_step": 1e3 * wall / steps,
-        "max_div": float(max(hist["max_div"])),
+        "max_div": max_div,
+        "div_ok": max_div <= DIV_TARGET,
         "iterations_last_solve": int(sim.solver.pressure_solver.last_iterations),
         "cd_last": float(sim.force_coefficients()[0]),
         "mg_plan": [[p["nx"], p["ny"], p["mode"], *p["tile"]] for p in plan],
@@ -72,15 +107,29 @@ def run_path(precision: str, steps: int, warmup: int, nx: int, ny: int):
     }
     log(f"[{precision}] {steps} steps in {wall:.3f} s -> {out['mlups']:.1f} MLUPS, "
         f"max|div| {out['max_div']:.2e}, {out['iterations_last_solve']} it/solve")
-    return sim, out
-
-
-def kernel_roofline(sim, which: int, reps: int = 50):
-    ms, nbytes = sim.solver.ctx.time_kernel(which, reps)
-    achieved = nbytes / (ms * 1e-3) / 1e9
-    return {"kernel": KERNELS[which], "bound": "hbm", "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None, "avg_us": round(ms * 1e3, 3), "bytes_per_launch": nbytes}
+    # second window, live kernel clock on
+    ctx = sim.solver.ctx
+    ctx.kernel_clock_start(max_launches=steps * 3 * 40)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.run(max_steps=steps)
+    torch.cuda.synchronize()
+    clocked = time.perf_counter() - t0
+    kernels = []
+    for name, (n, ms, nbytes) in zip(kernel_names(precision), ctx.kernel_clock_read()):
+        if n == 0:
+            continue
+        achieved = nbytes / (ms * 1e-3) / 1e9
+        kernels.append({"kernel": name, "bound": "hbm", "achieved": round(achieved, 1),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "traffic": traffic.get(name), "avg_us": round(ms * 1e3, 3),
+                        "launches": n, "bytes_per_launch": nbytes,
+                        "share_of_window": round(n * ms * 1e-3 / clocked, 4)})
+    out["clocked_ms_per_step"] = 1e3 * clocked / steps
+    log(f"[{precision}] clocked window {1e3 * clocked / steps:.3f} ms/step; " + "; ".join(
+        f"{k['kernel']} {k['avg_us']} us {k['frac']}" for k in kernels))
+    return sim, out, kernels
 
 
 def copy_calibration(nbytes: int = 1 << 26) -> None:
@@ -94,7 +143,7 @@ def copy_calibration(nbytes: int = 1 << 26) -> None:
     torch.cuda.synchronize()
 
 
-def cpu_baseline(nx: int, ny: int, steps: int):
+def cpu_baseline(nx: int, ny: int, steps: int, tol: float):
     """The CPU oracle (NumPy/SciPy FP64 restatement of the reference) on the
     same workload, timed over `steps` steps after its own setup."""
     import numpy as np
@@ -115,7 +164,7 @@ def cpu_baseline(nx: int, ny: int, steps: int):
     case = O.Case(nx=nx, ny=ny, lx=Lx, ly=Ly, re=100.0, dt=0.02, cfl_max=0.4,
                   kinds=("inlet", "outlet", "symmetry", "symmetry"),
                   velocity=(1.0, 0.0, 0.0, 0.0), solid=solid, pressure_solver="mgcg",
-                  tol=1e-10)
+                  tol=tol)
     Xv, Yv = np.meshgrid(case.xc, case.yf, indexing="ij")
     v0 = 1e-3 * np.exp(-(((Xv - 5.0) / 1.0) ** 2 + ((Yv - 8.0) / 0.5) ** 2))
     st = O.State()
@@ -127,7 +176,7 @@ def cpu_baseline(nx: int, ny: int, steps: int):
     wall = time.perf_counter() - t0
     return {"value": round(nx * ny * steps / wall / 1e6, 4), "unit": "MLUPS", "cores": 1,
             "kind": "port",
-            "sample": f"{steps} RK3 steps of the {nx}x{ny} cylinder (mgcg, tol 1e-10) after "
+            "sample": f"{steps} RK3 steps of the {nx}x{ny} cylinder (mgcg, tol {tol:g}) after "
                       f"setup and the initial projection; {wall:.1f} s; NumPy/SciPy FP64, "
                       "single-threaded sparse kernels"}
 
@@ -142,6 +191,7 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--paths", default="fp64,mixed")
+    ap.add_argument("--tol", type=float, default=2.5e-11)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
@@ -156,18 +206,19 @@ def main() -> int:
     _lib.load()
 
     copy_calibration()
-    results = {}
-    sims = {}
+    traffic = load_traffic()
+    results, kernels = {}, {}
     for prec in args.paths.split(","):
-        sims[prec], results[prec] = run_path(prec, args.steps, args.warmup, args.nx, args.ny)
+        _, results[prec], kernels[prec] = run_path(prec, args.steps, args.warmup, args.nx,
+                                                   args.ny, args.tol, traffic)
     main_path = "mixed" if "mixed" in results else "fp64"
     r = results[main_path]
-    roof = [kernel_roofline(sims["fp64" if "fp64" in sims else main_path], w) for w in KERNELS]
-    dominant = max(roof, key=lambda d: d["avg_us"] * (2 if d["kernel"] != "k_cg_update" else 1))
+    # the dominant kernel of the reported path: largest share of its window
+    dominant = dict(max(kernels[main_path], key=lambda d: d["share_of_window"]))
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle) ...")
-        cpu = cpu_baseline(args.nx, args.ny, args.cpu_steps)
+        cpu = cpu_baseline(args.nx, args.ny, args.cpu_steps, args.tol)
         log(f"cpu baseline {cpu['value']} MLUPS")
     line = {
         "metric": "MLUPS per projection step @2048x1024, mixed vs FP64 at fixed div-u; HBM GB/s",
@@ -183,12 +234,13 @@ def main() -> int:
         "dtype": "f64" if main_path == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": f"cylinder Re=100 {args.nx}x{args.ny} (32x16 D), OUTLET, "
-                               "mgcg tol 1e-10, SSP-RK3",
+                               f"mgcg to max|div u| <= {DIV_TARGET:g} (tol {args.tol:g}), "
+                               "SSP-RK3",
                    "path": main_path, "nx": args.nx, "ny": args.ny},
         "paths": {k: {kk: (float(f"{vv:.5g}") if isinstance(vv, float) else vv)
                       for kk, vv in v.items()} for k, v in results.items()},
         "roofline": dominant,
-        "kernels": roof,
+        "kernels": kernels,
         "cpu_baseline": cpu,
     }
     if "fp64" in results and "mixed" in results:

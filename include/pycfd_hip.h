@@ -247,6 +247,16 @@ int pcfd_mg_apply(pcfd_ctx* ctx, const double* r, double* z);
  * reference counterpart: measurement only. */
 int pcfd_time_kernel(pcfd_ctx* ctx, int32_t which, int32_t reps, double* avg_ms,
                      double* bytes_per_launch);
+/* The same four kernels timed live inside the solves that follow: every
+ * launch records events at the kernel's own start and end
+ * (hipExtLaunchKernelGGL), up to max_launches per kernel; launches of
+ * iterations enqueued past convergence (no-ops) are dropped.  _read stops the
+ * clock and fills, per kernel (arrays of 4): launches timed, mean duration in
+ * ms, algorithmic bytes per launch.  No reference counterpart: measurement
+ * only. */
+int pcfd_kernel_clock_start(pcfd_ctx* ctx, int32_t max_launches);
+int pcfd_kernel_clock_read(pcfd_ctx* ctx, int32_t* launches, double* avg_ms,
+                           double* bytes_per_launch);
 
 /* The multigrid plan of the configured solver, for reporting: per level (all
  * levels of the cycle precision, finest first) five integers -- nx, ny,

@@ -117,6 +117,8 @@ _SIGNATURES = {
     "pcfd_poisson_apply": [C.c_void_p, C.c_void_p, C.c_void_p],
     "pcfd_mg_apply": [C.c_void_p, C.c_void_p, C.c_void_p],
     "pcfd_time_kernel": [C.c_void_p, C.c_int32, C.c_int32, _dp, _dp],
+    "pcfd_kernel_clock_start": [C.c_void_p, C.c_int32],
+    "pcfd_kernel_clock_read": [C.c_void_p, C.POINTER(C.c_int32), _dp, _dp],
     "pcfd_mg_plan": [C.c_void_p, C.POINTER(C.c_int32), C.c_int32],
     "pcfd_pressure_guess": [C.c_void_p, C.c_void_p, C.c_int32],
     "pcfd_eddy_viscosity": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],

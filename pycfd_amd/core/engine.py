@@ -229,6 +229,17 @@ class DeviceContext:
         self._call("pcfd_time_kernel", int(which), int(reps), C.byref(ms), C.byref(b))
         return float(ms.value), float(b.value)
 
+    def kernel_clock_start(self, max_launches: int) -> None:
+        """Time kernels 0-3 live inside the solves that follow."""
+        self._call("pcfd_kernel_clock_start", int(max_launches))
+
+    def kernel_clock_read(self) -> list[tuple[int, float, float]]:
+        """Per kernel 0-3: (launches, mean ms, algorithmic bytes per launch)."""
+        n = (C.c_int32 * 4)()
+        ms, b = (C.c_double * 4)(), (C.c_double * 4)()
+        self._call("pcfd_kernel_clock_read", n, ms, b)
+        return [(int(n[k]), float(ms[k]), float(b[k])) for k in range(4)]
+
     def mg_plan(self) -> list[dict]:
         """Per multigrid level: grid, whether its legs run fused, and the tile."""
         buf = (C.c_int32 * (5 * 32))()

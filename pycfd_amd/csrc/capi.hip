@@ -891,6 +891,21 @@ int pcfd_time_kernel(pcfd_ctx* c, int32_t which, int32_t reps, double* avg_ms,
   return c->engine.time_kernel(which, reps, avg_ms, bytes_per_launch);
 }
 
+int pcfd_kernel_clock_start(pcfd_ctx* c, int32_t max_launches) {
+  if (int rc = check_ctx(c)) return rc;
+  return c->engine.clock_start(max_launches);
+}
+
+int pcfd_kernel_clock_read(pcfd_ctx* c, int32_t* launches, double* avg_ms,
+                           double* bytes_per_launch) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!launches || !avg_ms || !bytes_per_launch) {
+    set_last_error("pcfd_kernel_clock_read: null output");
+    return kErrBadArg;
+  }
+  return c->engine.clock_read(launches, avg_ms, bytes_per_launch);
+}
+
 int pcfd_mg_plan(pcfd_ctx* c, int32_t* out, int32_t cap) {
   if (int rc = check_ctx(c)) return -rc;
   if (!out || cap < 0) {

@@ -15,6 +15,7 @@
 //     the slots in a fixed order.  No floating-point atomics are used.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstdio>
@@ -60,6 +61,23 @@ void set_last_error(const char* fmt, ...);
                              hipGetErrorString(_e), __FILE__, __LINE__);      \
       return ::pcfd::kErrHip;                                                 \
     }                                                                         \
+  } while (0)
+
+// Start/stop events around one kernel launch, for timing a kernel inside a
+// real solve; both null for a plain launch.
+struct LaunchClock {
+  hipEvent_t start = nullptr, stop = nullptr;
+};
+
+// hipLaunchKernelGGL, or hipExtLaunchKernelGGL recording the clock's events
+// at the kernel's own start and end when the clock is set.
+#define PCFD_LAUNCH_TIMED(clk, kernel, grid, block, shm, stream, ...)                     \
+  do {                                                                                   \
+    if ((clk).start)                                                                     \
+      hipExtLaunchKernelGGL(kernel, grid, block, shm, stream, (clk).start, (clk).stop, 0, \
+                            __VA_ARGS__);                                                \
+    else                                                                                 \
+      hipLaunchKernelGGL(kernel, grid, block, shm, stream, __VA_ARGS__);                 \
   } while (0)
 
 // ---------------------------------------------------------------------------

@@ -921,6 +921,60 @@ PoissonEngine::~PoissonEngine() {
   hipFree(fine_active_);
   if (st_host_) hipHostFree(st_host_);
   if (ev_) hipEventDestroy(ev_);
+  clock_free();
+}
+
+void PoissonEngine::clock_free() {
+  for (KernelClock& k : clock_) {
+    for (hipEvent_t e : k.ev) hipEventDestroy(e);
+    k.ev.clear();
+    k.used = 0;
+    k.bytes = 0.0;
+  }
+  clock_on_ = false;
+}
+
+int PoissonEngine::clock_start(int max_launches) {
+  if (max_launches < 1) {
+    set_last_error("clock_start: max_launches must be >= 1, got %d", max_launches);
+    return kErrBadArg;
+  }
+  PCFD_HIP_CHECK(hipStreamSynchronize(s_));
+  clock_free();
+  for (KernelClock& k : clock_) {
+    k.ev.resize(2 * (size_t)max_launches, nullptr);
+    for (hipEvent_t& e : k.ev) PCFD_HIP_CHECK(hipEventCreate(&e));
+  }
+  clock_on_ = true;
+  return kOk;
+}
+
+LaunchClock PoissonEngine::clock_next(int which, double bytes) {
+  if (!clock_on_) return LaunchClock{};
+  KernelClock& k = clock_[which];
+  if (2 * (k.used + 1) > k.ev.size()) return LaunchClock{};
+  LaunchClock c{k.ev[2 * k.used], k.ev[2 * k.used + 1]};
+  ++k.used;
+  k.bytes = bytes;
+  return c;
+}
+
+int PoissonEngine::clock_read(int32_t* launches, double* avg_ms, double* bytes) {
+  PCFD_HIP_CHECK(hipStreamSynchronize(s_));
+  for (int w = 0; w < kClockKernels; ++w) {
+    KernelClock& k = clock_[w];
+    double total = 0.0;
+    for (size_t i = 0; i < k.used; ++i) {
+      float ms = 0.f;
+      PCFD_HIP_CHECK(hipEventElapsedTime(&ms, k.ev[2 * i], k.ev[2 * i + 1]));
+      total += ms;
+    }
+    launches[w] = (int32_t)k.used;
+    avg_ms[w] = k.used ? total / (double)k.used : 0.0;
+    bytes[w] = k.bytes;
+  }
+  clock_free();
+  return kOk;
 }
 
 void PoissonEngine::free_levels() {
@@ -1891,22 +1945,26 @@ int PoissonEngine::enqueue_vcycle_t(int mode, int reduce) {
     PCFD_LAUNCH_CHECK();
     return fold(s_, partial_, nt0, EpiCycle{st_, reduce, n_proj_});
   }
-  // Downward leg.
+  // Downward leg.  Algorithmic bytes for the clock: r (FP64) or b_in and
+  // x_in, flags, the hand-over vectors and the restricted residual.
+  const double n0 = (double)L0.n, ts = (double)sizeof(T);
   {
     LevelBuf& C = lv[1];
+    const double bytes = (krylov ? n0 * (8 + 1 + 2 * ts) : n0 * (3 * ts + 1)) + C.n * ts;
     const int rc = launch_fine_down<T>(
         s_, op_, krylov ? L0.down0 : L0.down1, (T)L0.weight, nu, krylov ? 0 : 1,
         krylov ? (const double*)r_ : nullptr, krylov ? nullptr : (const T*)L0.b,
         krylov ? nullptr : (const T*)L0.x, krylov ? 0 : 1, krylov ? (T*)L0.b : nullptr,
-        (T*)L0.t, (T*)C.b, L0.fx, L0.fy, C.nx, C.ny, cst);
+        (T*)L0.t, (T*)C.b, L0.fx, L0.fy, C.nx, C.ny, cst, clock_next(2, bytes));
     if (rc) return rc;
   }
   if (int rc = coarse_legs<T>(lv, 1, cst)) return rc;
   {
     LevelBuf& C = lv[1];
+    const double bytes = n0 * (3 * ts + 1) + C.n * ts;  // x_pre, b0, flags, z; x_c
     if (int rc = launch_fine_up<T>(s_, op_, L0.up, (T)L0.weight, nu, (const T*)L0.t,
                                    (const T*)L0.b, (const T*)C.y, L0.fx, L0.fy, C.ny, out,
-                                   reduce, partial_, cst))
+                                   reduce, partial_, cst, clock_next(3, bytes)))
       return rc;
   }
   return fold(s_, partial_, L0.up.ntiles(), EpiCycle{st_, reduce, n_proj_});
@@ -1924,20 +1982,26 @@ int PoissonEngine::run_mgcg() {
   double* p_old = pa_;
   double* p_new = pb_;
   const bool f32 = cfg_.precision == 1;
+  size_t mark[kClockKernels];
+  clock_mark(mark);
   PCFD_ITERATE(1, cfg_.maxiter, {
     if (int rc = enqueue_vcycle(VC_KRYLOV, SR_MGCG)) return rc;
+    // direction: z (cycle precision), p_old, flags in; p_new, q out
+    const LaunchClock cd = clock_next(1, (double)n_ * ((f32 ? 4 : 8) + 3 * 8 + 1));
     const int rc = f32 ? launch_fine_dir<float>(s_, op_, dir_tile_, (const float*)lv32_[0].y,
-                                                p_old, p_new, q_, partial_, st_)
+                                                p_old, p_new, q_, partial_, st_, cd)
                        : launch_fine_dir<double>(s_, op_, dir_tile_, (const double*)lv64_[0].y,
-                                                 p_old, p_new, q_, partial_, st_);
+                                                 p_old, p_new, q_, partial_, st_, cd);
     if (rc) return rc;
     if (int rc2 = fold(s_, partial_, dir_tile_.ntiles(), EpiAlpha{st_})) return rc2;
-    hipLaunchKernelGGL(k_cg_update, g, dim3(kBlock), 0, s_, op_, x_, r_, p_new, q_, (int)IM_MGCG,
-                       partial_, (const SolveState*)st_);
+    const LaunchClock cu = clock_next(0, (double)n_ * (6 * 8 + 1));  // x, p, r, q, flags; x, r
+    PCFD_LAUNCH_TIMED(cu, k_cg_update, g, dim3(kBlock), 0, s_, op_, x_, r_, p_new, q_,
+                      (int)IM_MGCG, partial_, (const SolveState*)st_);
     if (int rc = fold(s_, partial_, nt, EpiUpdate{st_, (int)IM_MGCG, cfg_.maxiter, n_proj_}))
       return rc;
     std::swap(p_old, p_new);
   });
+  clock_trim(mark, predicted_);
   return kOk;
 }
 
@@ -1950,6 +2014,8 @@ int PoissonEngine::run_multigrid() {
   const size_t bytes = (size_t)n_ * sizeof(double);
   PCFD_HIP_CHECK(hipMemcpyAsync(L0.b, b_, bytes, hipMemcpyDeviceToDevice, s_));
   PCFD_HIP_CHECK(hipMemcpyAsync(L0.x, x_, bytes, hipMemcpyDeviceToDevice, s_));
+  size_t mark[kClockKernels];
+  clock_mark(mark);
   PCFD_ITERATE(1, cfg_.maxiter, {
     if (int rc = enqueue_vcycle_t<double>(VC_STANDALONE, SR_MG)) return rc;
     hipLaunchKernelGGL(k_check, g, dim3(kBlock), 0, s_, op_, (const double*)L0.x, b_, (int)CM_MG,
@@ -1958,6 +2024,7 @@ int PoissonEngine::run_multigrid() {
                       EpiCheck{st_, (int)CM_MG, cfg_.tol, cfg_.maxiter, singular_, n_fluid_}))
       return rc;
   });
+  clock_trim(mark, predicted_);
   // The pending mean shift is applied by the final residual check and the scatter.
   PCFD_HIP_CHECK(hipMemcpyAsync(x_, L0.x, bytes, hipMemcpyDeviceToDevice, s_));
   return kOk;

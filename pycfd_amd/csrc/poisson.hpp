@@ -179,6 +179,13 @@ class PoissonEngine {
   // buffers with HIP events; reports the mean duration and the kernel's
   // algorithmic bytes per launch (arrays read + written once, at stored size).
   int time_kernel(int which, int reps, double* avg_ms, double* bytes);
+  // Live clock: every launch of kernels 0-3 (time_kernel's numbering) inside
+  // the solves that follow records start/stop events, up to `max_launches`
+  // per kernel; clock_read stops it and reports launches, mean duration and
+  // algorithmic bytes per launch for each kernel.
+  int clock_start(int max_launches);
+  int clock_read(int32_t* launches, double* avg_ms, double* bytes);
+  static constexpr int kClockKernels = 4;
   // Per level: nx, ny, fused, tile x, tile y (see pcfd_mg_plan).
   int plan(int32_t* out, int cap) const {
     const auto& lv = (cfg_.precision == 1) ? lv32_ : lv64_;
@@ -244,6 +251,25 @@ class PoissonEngine {
   SolveState* st_ = nullptr;
   SolveState* st_host_ = nullptr;  // pinned mirror for polling
   hipEvent_t ev_ = nullptr;
+  // live kernel clock
+  struct KernelClock {
+    std::vector<hipEvent_t> ev;  // start/stop pairs
+    size_t used = 0;
+    double bytes = 0.0;
+  };
+  bool clock_on_ = false;
+  KernelClock clock_[kClockKernels];
+  LaunchClock clock_next(int which, double bytes);
+  void clock_free();
+  // Iterations enqueued past convergence run as no-ops; keep only the first
+  // `iters` launches of each kernel since `mark` (one launch per iteration).
+  void clock_mark(size_t* mark) const {
+    for (int w = 0; w < kClockKernels; ++w) mark[w] = clock_[w].used;
+  }
+  void clock_trim(const size_t* mark, int iters) {
+    for (int w = 0; w < kClockKernels; ++w)
+      if (clock_[w].used > mark[w] + (size_t)iters) clock_[w].used = mark[w] + (size_t)iters;
+  }
   double *b_ = nullptr, *x_ = nullptr, *r_ = nullptr, *pa_ = nullptr, *pb_ = nullptr,
          *q_ = nullptr, *z_ = nullptr;
   int last_iters_ = 0;

@@ -674,15 +674,15 @@ template <typename T>
 int launch_fine_down(hipStream_t s, const FineOp& A, const FusedTile& ft, T w, int nu,
                      int nonzero, const double* r, const T* b_in, const T* x_in, int shift_in,
                      T* b_out, T* x_out, T* bc, int fx, int fy, int cnx, int cny,
-                     const SolveState* st) {
+                     const SolveState* st, const LaunchClock& clk) {
   if (A.hxc)
-    hipLaunchKernelGGL((k_fine_down<T, true>), dim3(ft.ntiles()), dim3(kBlock), ft.lds, s, A, ft,
-                       w, nu, nonzero, r, b_in, x_in, shift_in, b_out, x_out, bc, fx, fy, cnx,
-                       cny, st);
+    PCFD_LAUNCH_TIMED(clk, (k_fine_down<T, true>), dim3(ft.ntiles()), dim3(kBlock), ft.lds, s, A,
+                      ft, w, nu, nonzero, r, b_in, x_in, shift_in, b_out, x_out, bc, fx, fy, cnx,
+                      cny, st);
   else
-    hipLaunchKernelGGL((k_fine_down<T, false>), dim3(ft.ntiles()), dim3(kBlock), ft.lds, s, A, ft,
-                       w, nu, nonzero, r, b_in, x_in, shift_in, b_out, x_out, bc, fx, fy, cnx,
-                       cny, st);
+    PCFD_LAUNCH_TIMED(clk, (k_fine_down<T, false>), dim3(ft.ntiles()), dim3(kBlock), ft.lds, s, A,
+                      ft, w, nu, nonzero, r, b_in, x_in, shift_in, b_out, x_out, bc, fx, fy, cnx,
+                      cny, st);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
@@ -690,13 +690,13 @@ int launch_fine_down(hipStream_t s, const FineOp& A, const FusedTile& ft, T w, i
 template <typename T>
 int launch_fine_up(hipStream_t s, const FineOp& A, const FusedTile& ft, T w, int nu,
                    const T* x_pre, const T* b0, const T* xc, int fx, int fy, int cny, T* z,
-                   int reduce, double* partial, const SolveState* st) {
+                   int reduce, double* partial, const SolveState* st, const LaunchClock& clk) {
   if (A.hxc)
-    hipLaunchKernelGGL((k_fine_up<T, true>), dim3(ft.ntiles()), dim3(kBlock), ft.lds, s, A, ft, w,
-                       nu, x_pre, b0, xc, fx, fy, cny, z, reduce, partial, st);
+    PCFD_LAUNCH_TIMED(clk, (k_fine_up<T, true>), dim3(ft.ntiles()), dim3(kBlock), ft.lds, s, A,
+                      ft, w, nu, x_pre, b0, xc, fx, fy, cny, z, reduce, partial, st);
   else
-    hipLaunchKernelGGL((k_fine_up<T, false>), dim3(ft.ntiles()), dim3(kBlock), ft.lds, s, A, ft, w,
-                       nu, x_pre, b0, xc, fx, fy, cny, z, reduce, partial, st);
+    PCFD_LAUNCH_TIMED(clk, (k_fine_up<T, false>), dim3(ft.ntiles()), dim3(kBlock), ft.lds, s, A,
+                      ft, w, nu, x_pre, b0, xc, fx, fy, cny, z, reduce, partial, st);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
@@ -724,13 +724,13 @@ int launch_class_up(hipStream_t s, const ClassRows<T>& L, const FusedTile& ft, i
 template <typename T>
 int launch_fine_dir(hipStream_t s, const FineOp& A, const FusedTile& ft, const T* z,
                     const double* p_old, double* p_new, double* q, double* partial,
-                    const SolveState* st) {
+                    const SolveState* st, const LaunchClock& clk) {
   if (A.hxc)
-    hipLaunchKernelGGL((k_fine_dir<T, true>), dim3(ft.ntiles()), dim3(kBlock), ft.lds, s, A, ft, z,
-                       p_old, p_new, q, partial, st);
+    PCFD_LAUNCH_TIMED(clk, (k_fine_dir<T, true>), dim3(ft.ntiles()), dim3(kBlock), ft.lds, s, A,
+                      ft, z, p_old, p_new, q, partial, st);
   else
-    hipLaunchKernelGGL((k_fine_dir<T, false>), dim3(ft.ntiles()), dim3(kBlock), ft.lds, s, A, ft,
-                       z, p_old, p_new, q, partial, st);
+    PCFD_LAUNCH_TIMED(clk, (k_fine_dir<T, false>), dim3(ft.ntiles()), dim3(kBlock), ft.lds, s, A,
+                      ft, z, p_old, p_new, q, partial, st);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
@@ -804,10 +804,10 @@ bool choose_tile(int nx, int ny, int rxs, int rys, int halo_mult, int fx, int fy
 #define PCFD_VCYCLE_INST(T)                                                                    \
   template int launch_fine_down<T>(hipStream_t, const FineOp&, const FusedTile&, T, int, int,   \
                                    const double*, const T*, const T*, int, T*, T*, T*, int,   \
-                                   int, int, int, const SolveState*);                         \
+                                   int, int, int, const SolveState*, const LaunchClock&);     \
   template int launch_fine_up<T>(hipStream_t, const FineOp&, const FusedTile&, T, int,         \
                                  const T*, const T*, const T*, int, int, int, T*, int, double*, \
-                                 const SolveState*);                                          \
+                                 const SolveState*, const LaunchClock&);                      \
   template int launch_class_down<T>(hipStream_t, const ClassRows<T>&, const FusedTile&, int,   \
                                     int, T, int, const T*, T*, T*, int, int, int, int,        \
                                     const SolveState*);                                       \
@@ -817,7 +817,8 @@ bool choose_tile(int nx, int ny, int rxs, int rys, int halo_mult, int fx, int fy
   template int launch_dense_apply<T>(hipStream_t, const double*, int, const T*, T*,            \
                                      const SolveState*);                                      \
   template int launch_fine_dir<T>(hipStream_t, const FineOp&, const FusedTile&, const T*,      \
-                                  const double*, double*, double*, double*, const SolveState*);
+                                  const double*, double*, double*, double*, const SolveState*, \
+                                  const LaunchClock&);
 PCFD_VCYCLE_INST(float)
 PCFD_VCYCLE_INST(double)
 #undef PCFD_VCYCLE_INST

@@ -45,14 +45,15 @@ template <typename T>
 int launch_fine_down(hipStream_t s, const FineOp& A, const FusedTile& ft, T w, int nu,
                      int nonzero, const double* r, const T* b_in, const T* x_in, int shift_in,
                      T* b_out, T* x_out, T* bc, int fx, int fy, int cnx, int cny,
-                     const SolveState* st);
+                     const SolveState* st, const LaunchClock& clk = LaunchClock{});
 
 // Level-0 up leg: z = S^nu (x_pre + P x_c); `reduce` selects the partial sums
 // of the cycle output written per tile (SmoothReduce).
 template <typename T>
 int launch_fine_up(hipStream_t s, const FineOp& A, const FusedTile& ft, T w, int nu,
                    const T* x_pre, const T* b0, const T* xc, int fx, int fy, int cny, T* z,
-                   int reduce, double* partial, const SolveState* st);
+                   int reduce, double* partial, const SolveState* st,
+                   const LaunchClock& clk = LaunchClock{});
 
 template <typename T>
 int launch_class_down(hipStream_t s, const ClassRows<T>& L, const FusedTile& ft, int rxs,
@@ -69,7 +70,7 @@ int launch_class_up(hipStream_t s, const ClassRows<T>& L, const FusedTile& ft, i
 template <typename T>
 int launch_fine_dir(hipStream_t s, const FineOp& A, const FusedTile& ft, const T* z,
                     const double* p_old, double* p_new, double* q, double* partial,
-                    const SolveState* st);
+                    const SolveState* st, const LaunchClock& clk = LaunchClock{});
 
 // x = C b with C an n x n dense FP64 matrix, spread over many workgroups.
 template <typename T>
