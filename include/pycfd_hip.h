@@ -114,6 +114,10 @@ typedef struct pcfd_solver_desc {
   int32_t fused_levels;
   int32_t tile_tx, tile_ty;
   int32_t coarse_chain;
+  /* Level 0's Krylov-start legs in streaming form (uniform mesh, one sweep):
+   * 0 measured choice, -1 never, k > 0 always, k rows per wavefront (a
+   * multiple of 4). */
+  int32_t stream_rows;
 } pcfd_solver_desc;
 
 /* One level of the aggregation hierarchy (multigrid.py:293-322).  Level 0 is
@@ -261,8 +265,9 @@ int pcfd_kernel_clock_read(pcfd_ctx* ctx, int32_t* launches, double* avg_ms,
 /* The multigrid plan of the configured solver, for reporting: per level (all
  * levels of the cycle precision, finest first) five integers -- nx, ny,
  * mode (1: one fused launch per leg, 0: pass by pass, 2: inside the
- * cooperative launch of the deepest levels; the coarsest level is the dense
- * solve), tile x, tile y -- written to out (capacity
+ * cooperative launch of the deepest levels, 3: level 0's Krylov-start legs in
+ * streaming form, tile = rows per wavefront x 60 columns; the coarsest level
+ * is the dense solve), tile x, tile y -- written to out (capacity
  * `cap` integers).  Returns the number of levels, or a negative status.  No
  * reference counterpart. */
 int pcfd_mg_plan(pcfd_ctx* ctx, int32_t* out, int32_t cap);

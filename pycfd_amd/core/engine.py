@@ -104,10 +104,11 @@ class DeviceContext:
     # -- configuration ---------------------------------------------------
     def set_solver(self, kind: str, tol: float, maxiter: int, omega: float, sweeps: int,
                    precision: int, levels: list[dict] | None, *, fused_levels: bool = True,
-                   tile: tuple[int, int] = (0, 0), coarse_chain: int = 0) -> None:
+                   tile: tuple[int, int] = (0, 0), coarse_chain: int = 0,
+                   stream_rows: int = 0) -> None:
         sd = _lib.SolverDesc(_lib.SOLVER_CODES[kind], float(tol), int(maxiter), float(omega),
                              int(sweeps), int(precision), int(bool(fused_levels)),
-                             int(tile[0]), int(tile[1]), int(coarse_chain))
+                             int(tile[0]), int(tile[1]), int(coarse_chain), int(stream_rows))
         keep = []
         if levels:
             arr = (_lib.MGLevelDesc * len(levels))()
@@ -247,9 +248,10 @@ class DeviceContext:
         if n < 0:
             _lib.check(-n, "pcfd_mg_plan")
         vals = list(buf)
-        modes = {0: "passes", 1: "fused", 2: "chain"}
+        modes = {0: "passes", 1: "fused", 2: "chain", 3: "stream"}
         return [{"nx": vals[5 * k], "ny": vals[5 * k + 1], "mode": modes[vals[5 * k + 2]],
-                 "fused": vals[5 * k + 2] == 1, "tile": (vals[5 * k + 3], vals[5 * k + 4])}
+                 "fused": vals[5 * k + 2] in (1, 3),
+                 "tile": (vals[5 * k + 3], vals[5 * k + 4])}
                 for k in range(min(n, 32))]
 
     def synchronize(self) -> None:

@@ -180,6 +180,9 @@ class PoissonSolverBase:
         # default: on MI355X a grid-wide barrier costs more than the launch it
         # replaces), 0 measured choice, k >= 1 from level k down
         self.coarse_chain = -1
+        # level 0's Krylov-start legs in streaming form: 0 measured choice,
+        # -1 never, k > 0 always with k rows per wavefront
+        self.stream_rows = 0
         # hand the stored multigrid levels over with 32-bit class indices even
         # when 16 bits would do (they are chosen by size otherwise)
         self.wide_classes = False
@@ -227,7 +230,7 @@ class PoissonSolverBase:
         self.context.set_solver(self._device_kind, tol, maxiter, self._omega(), self._sweeps(),
                                 1 if self.precision == "mixed" else 0, self._levels(),
                                 fused_levels=self.fused_levels, tile=self.tile,
-                                coarse_chain=self.coarse_chain)
+                                coarse_chain=self.coarse_chain, stream_rows=self.stream_rows)
         self._configured = True
 
     def record(self, iterations: int, residual: float, converged: bool) -> None:

@@ -582,6 +582,11 @@ int pcfd_set_pressure_solver(pcfd_ctx* c, const pcfd_solver_desc* sd,
   cfg.tile_tx = sd->tile_tx;
   cfg.tile_ty = sd->tile_ty;
   cfg.chain = sd->coarse_chain;
+  cfg.stream = sd->stream_rows;
+  if (cfg.stream > 0 && cfg.stream % 4) {
+    set_last_error("stream_rows must be a multiple of 4, got %d", cfg.stream);
+    return kErrBadArg;
+  }
   if (cfg.tile_tx < 0 || cfg.tile_ty < 0) {
     set_last_error("tile sizes must be >= 0, got %dx%d", cfg.tile_tx, cfg.tile_ty);
     return kErrBadArg;

@@ -1203,6 +1203,7 @@ namespace {
 using TuneKey = std::array<int, 14>;
 struct TunePick {
   int fused, tx, ty;
+  int stream = 0;  // level 0: streaming legs with this many rows per wavefront
 };
 std::map<TuneKey, TunePick> g_tune_cache;
 std::mutex g_tune_mutex;
@@ -1261,8 +1262,13 @@ int PoissonEngine::tune_level(std::vector<LevelBuf>& lv, int l, SolveState* tst)
     PCFD_HIP_CHECK(hipStreamSynchronize(s_));
     return kOk;
   };
-  auto apply = [&](int fused, const FusedTile& t) -> int {
+  auto apply = [&](int fused, const FusedTile& t, int stream = 0) -> int {
     L.fused = fused;
+    L.stream_rows = stream;
+    L.stream_nb = stream ? stream_blocks(L.nx, L.ny, stream) : 0;
+    if (stream) {
+      if (int rc = ensure_partials((size_t)L.stream_nb * 3)) return rc;
+    }
     if (!fused) return kOk;
     L.down0 = t;
     L.up = t;
@@ -1275,6 +1281,14 @@ int PoissonEngine::tune_level(std::vector<LevelBuf>& lv, int l, SolveState* tst)
     return upload_offsets(t);
   };
   auto legs = [&]() -> int {
+    if (fine && L.stream_rows) {
+      if (int rc = launch_fine_down_stream<T>(s_, op_, L.stream_rows, (T)L.weight, r_, (T*)L.b,
+                                              (T*)L.t, (T*)C.b, L.fx, L.fy, C.nx, C.ny, tst))
+        return rc;
+      return launch_fine_up_stream<T>(s_, op_, L.stream_rows, (T)L.weight, (const T*)L.t,
+                                      (const T*)L.b, (const T*)C.y, L.fx, L.fy, C.ny, (T*)L.y,
+                                      (int)SR_MGCG, partial_, tst);
+    }
     if (fine) {
       if (int rc = launch_fine_down<T>(s_, op_, L.down0, (T)L.weight, nu, 0, r_, nullptr, nullptr,
                                        0, (T*)L.b, (T*)L.t, (T*)C.b, L.fx, L.fy, C.nx, C.ny, tst))
@@ -1294,12 +1308,14 @@ int PoissonEngine::tune_level(std::vector<LevelBuf>& lv, int l, SolveState* tst)
     auto it = g_tune_cache.find(key);
     if (it != g_tune_cache.end()) {
       const TunePick pk = it->second;
-      for (const FusedTile& t : cands)
-        if (t.tx == pk.tx && t.ty == pk.ty) return apply(pk.fused, t);
+      // a forced or disabled streaming choice re-plans level 0 instead
+      if (cfg_.stream == 0 || !fine)
+        for (const FusedTile& t : cands)
+          if (t.tx == pk.tx && t.ty == pk.ty) return apply(pk.fused, t, pk.stream);
       if (!pk.fused && !fine) return apply(0, cands[0]);
     }
   }
-  int best_fused = 1;
+  int best_fused = 1, best_stream = 0;
   FusedTile best = *std::min_element(cands.begin(), cands.end(),
                                      [](const FusedTile& a, const FusedTile& b) {
                                        return tile_cost(a) < tile_cost(b);
@@ -1337,14 +1353,31 @@ int PoissonEngine::tune_level(std::vector<LevelBuf>& lv, int l, SolveState* tst)
         best_fused = 1;
       }
     }
+    // level 0: the streaming Krylov-start legs as further contenders (the
+    // tiled tiling stays planned for the nonzero-start legs)
+    if (fine && cfg_.stream == 0 && stream_supported(op_, nu, L.fx, L.fy)) {
+      for (int rows : {4, 8, 16, 32}) {
+        if (int rc = apply(1, best, rows)) return rc;
+        float ms = 0.f;
+        if (int rc = time_legs(&ms)) return rc;
+        if (ms < best_ms) {
+          best_ms = ms;
+          best_stream = rows;
+        }
+      }
+    }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
   }
   {
     std::lock_guard<std::mutex> g(g_tune_mutex);
-    g_tune_cache[key] = TunePick{best_fused, best.tx, best.ty};
+    TunePick pk{best_fused, best.tx, best.ty};
+    pk.stream = best_stream;
+    if (cfg_.stream == 0 || !fine) g_tune_cache[key] = pk;
   }
-  return apply(best_fused, best);
+  if (fine && cfg_.stream > 0 && stream_supported(op_, nu, L.fx, L.fy))
+    best_stream = cfg_.stream;
+  return apply(best_fused, best, best_stream);
 }
 
 // Tile of the level-0 multigrid-CG direction kernel (halo 1), measured like
@@ -1948,6 +1981,23 @@ int PoissonEngine::enqueue_vcycle_t(int mode, int reduce) {
   // Downward leg.  Algorithmic bytes for the clock: r (FP64) or b_in and
   // x_in, flags, the hand-over vectors and the restricted residual.
   const double n0 = (double)L0.n, ts = (double)sizeof(T);
+  if (krylov && L0.stream_rows) {
+    LevelBuf& C = lv[1];
+    const double bytes_down = n0 * (8 + 1 + 2 * ts) + C.n * ts;
+    if (int rc = launch_fine_down_stream<T>(s_, op_, L0.stream_rows, (T)L0.weight,
+                                            (const double*)r_, (T*)L0.b, (T*)L0.t, (T*)C.b,
+                                            L0.fx, L0.fy, C.nx, C.ny, cst,
+                                            clock_next(2, bytes_down)))
+      return rc;
+    if (int rc = coarse_legs<T>(lv, 1, cst)) return rc;
+    const double bytes_up = n0 * (3 * ts + 1) + C.n * ts;
+    if (int rc = launch_fine_up_stream<T>(s_, op_, L0.stream_rows, (T)L0.weight,
+                                          (const T*)L0.t, (const T*)L0.b, (const T*)C.y, L0.fx,
+                                          L0.fy, C.ny, out, reduce, partial_, cst,
+                                          clock_next(3, bytes_up)))
+      return rc;
+    return fold(s_, partial_, L0.stream_nb, EpiCycle{st_, reduce, n_proj_});
+  }
   {
     LevelBuf& C = lv[1];
     const double bytes = (krylov ? n0 * (8 + 1 + 2 * ts) : n0 * (3 * ts + 1)) + C.n * ts;
@@ -2090,21 +2140,31 @@ int PoissonEngine::time_kernel(int which, int reps, double* avg_ms, double* byte
       case 2: {  // level-0 down leg: read r + flags, write b0, x, and b_c (n/4)
         LevelBuf& L = lv64_[0];
         LevelBuf& C = lv64_[1];
-        if (int rc = launch_fine_down<double>(s_, op_, L.down0, L.weight, cfg_.sweeps, 0, r_,
-                                              nullptr, nullptr, 0, (double*)L.b, (double*)L.t,
-                                              (double*)C.b, L.fx, L.fy, C.nx, C.ny, st))
-          return rc;
+        const int rc =
+            L.stream_rows
+                ? launch_fine_down_stream<double>(s_, op_, L.stream_rows, L.weight, r_,
+                                                  (double*)L.b, (double*)L.t, (double*)C.b, L.fx,
+                                                  L.fy, C.nx, C.ny, st)
+                : launch_fine_down<double>(s_, op_, L.down0, L.weight, cfg_.sweeps, 0, r_,
+                                           nullptr, nullptr, 0, (double*)L.b, (double*)L.t,
+                                           (double*)C.b, L.fx, L.fy, C.nx, C.ny, st);
+        if (rc) return rc;
         B = n * (8 + 1 + 8 + 8) + (double)C.n * 8;
         break;
       }
       case 3: {  // level-0 up leg: read x, b0, flags, x_c (n/4), write z
         LevelBuf& L = lv64_[0];
         LevelBuf& C = lv64_[1];
-        if (int rc = launch_fine_up<double>(s_, op_, L.up, L.weight, cfg_.sweeps,
-                                            (const double*)L.t, (const double*)L.b,
-                                            (const double*)C.y, L.fx, L.fy, C.ny, (double*)L.y,
-                                            (int)SR_MGCG, partial_, st))
-          return rc;
+        const int rc =
+            L.stream_rows
+                ? launch_fine_up_stream<double>(s_, op_, L.stream_rows, L.weight,
+                                                (const double*)L.t, (const double*)L.b,
+                                                (const double*)C.y, L.fx, L.fy, C.ny,
+                                                (double*)L.y, (int)SR_MGCG, partial_, st)
+                : launch_fine_up<double>(s_, op_, L.up, L.weight, cfg_.sweeps, (const double*)L.t,
+                                         (const double*)L.b, (const double*)C.y, L.fx, L.fy,
+                                         C.ny, (double*)L.y, (int)SR_MGCG, partial_, st);
+        if (rc) return rc;
         B = n * (8 + 8 + 1 + 8) + (double)C.n * 8;
         break;
       }

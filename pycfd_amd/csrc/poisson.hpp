@@ -122,6 +122,7 @@ struct PoissonConfig {
   int tile_tx = 0;  // level-0 fused tile (0 = chosen by the cost model)
   int tile_ty = 0;
   int chain = 0;    // cooperative coarse chain: 0 measured, -1 never, k >= 1 from level k
+  int stream = 0;   // level-0 streaming legs: 0 measured, -1 never, k > 0 with k rows
 };
 
 struct SolveInfo {
@@ -151,6 +152,9 @@ struct LevelBuf {
   int fused = 0;
   int rxs = 1, rys = 1;
   FusedTile down0, down1, up;
+  // Level 0 only: rows per wavefront of the streaming Krylov-start legs (0:
+  // the tiled kernels run them) and the up leg's workgroup count.
+  int stream_rows = 0, stream_nb = 0;
   int* ent_off = nullptr;
   std::vector<int8_t> hdi, hdj;  // host copy of the entry offsets (tile planning)
   int hot = -1, hot_n = 0;       // most frequent class and its entry count
@@ -191,10 +195,12 @@ class PoissonEngine {
     const auto& lv = (cfg_.precision == 1) ? lv32_ : lv64_;
     int k = 0, l = 0;
     for (const LevelBuf& L : lv) {
-      const int mode = (chain_first_ >= 0 && l >= chain_first_) ? 2 : L.fused;
+      int mode = (chain_first_ >= 0 && l >= chain_first_) ? 2 : L.fused;
+      if (l == 0 && L.stream_rows) mode = 3;
       ++l;
-      const int v[5] = {L.nx, L.ny, mode, mode == 1 ? L.down0.tx : 0,
-                        mode == 1 ? L.down0.ty : 0};
+      const int v[5] = {L.nx, L.ny, mode,
+                        mode == 1 ? L.down0.tx : (mode == 3 ? L.stream_rows : 0),
+                        mode == 1 ? L.down0.ty : (mode == 3 ? 60 : 0)};
       for (int q = 0; q < 5; ++q)
         if (k < cap) out[k++] = v[q];
     }

@@ -80,6 +80,51 @@ __device__ __forceinline__ void for_ring(const Ring& g, Fn&& fn) {
   }
 }
 
+// Visits every cell of a whole region (ring 0) in batches of KB cells per
+// thread: `ld(a, c)` issues a cell's global loads and returns them, and all
+// KB loads of a batch are in flight before `st(a, c, idx, v)` consumes them
+// (one load round trip per batch instead of one per cell).
+template <int KB, class Ld, class St>
+__device__ __forceinline__ void for_region_batched(int rx, int ry, Ld&& ld, St&& st) {
+  using V = decltype(ld(0, 0));
+  const int n = rx * ry;
+  const float inv = 1.0f / (float)ry;
+  auto split = [&](int q, int& a, int& c) {
+    a = (int)(((float)q + 0.5f) * inv);
+    c = q - a * ry;
+    if (c < 0) {
+      --a;
+      c += ry;
+    } else if (c >= ry) {
+      ++a;
+      c -= ry;
+    }
+  };
+  for (int base = threadIdx.x; base < n; base += KB * kBlock) {
+    V vals[KB];
+#pragma unroll
+    for (int u = 0; u < KB; ++u) {
+      const int q = base + u * kBlock;
+      if (q < n) {
+        int a, c;
+        split(q, a, c);
+        vals[u] = ld(a, c);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < KB; ++u) {
+      const int q = base + u * kBlock;
+      if (q < n) {
+        int a, c;
+        split(q, a, c);
+        st(a, c, q, vals[u]);
+      }
+    }
+  }
+}
+
+constexpr int kLoadBatch = 4;
+
 // ---------------------------------------------------------------------------
 // Level 0: matrix-free 5-point operator from the flag bytes.
 // ---------------------------------------------------------------------------
@@ -196,27 +241,48 @@ k_fine_down(FineOp A, FusedTile ft, T w, int nu, int nonzero, const double* __re
   __syncthreads();
 
   // Load; a zero start also forms x = 0 + w D^-1 (b - A 0) here.
-  for_ring(Ring{0, ft.rx, 0, ry}, [&](int a, int c) {
+  struct DownLd {
     int i, j;
-    uint8_t f = 0;
-    T bv = (T)0, xv = (T)0;
-    if (resolve(o.gi0 + a, o.gj0 + c, A.nx, A.ny, A.px, A.py, i, j)) {
-      const int k = i * A.ny + j;
-      f = cell_code(A, A.flags[k], i, j);
-      if (r) bv = (f & F_SOLV) ? (T)(r[k] - mr) : (T)0;
-      else bv = b_in[k];
-      if (nonzero) {
-        const T x0 = x_in[k];
-        xv = (f & F_SOLV) ? x0 - m : x0;
-      } else {
-        xv = w * rcoef<T, S>(A, f, i, j, td, ti).di * bv;
-      }
-    }
-    const int idx = a * ry + c;
-    M.F[idx] = f;
-    B[idx] = bv;
-    X[idx] = xv;
-  });
+    bool ok;
+    uint8_t fl;
+    double rv;
+    T bi, xi;
+  };
+  for_region_batched<kLoadBatch>(
+      ft.rx, ry,
+      [&](int a, int c) {
+        DownLd v;
+        v.ok = resolve(o.gi0 + a, o.gj0 + c, A.nx, A.ny, A.px, A.py, v.i, v.j);
+        v.fl = 0;
+        v.rv = 0.0;
+        v.bi = (T)0;
+        v.xi = (T)0;
+        if (v.ok) {
+          const int k = v.i * A.ny + v.j;
+          v.fl = A.flags[k];
+          if (r) v.rv = r[k];
+          else v.bi = b_in[k];
+          if (nonzero) v.xi = x_in[k];
+        }
+        return v;
+      },
+      [&](int a, int c, int idx, const DownLd& v) {
+        uint8_t f = 0;
+        T bv = (T)0, xv = (T)0;
+        if (v.ok) {
+          f = cell_code(A, v.fl, v.i, v.j);
+          if (r) bv = (f & F_SOLV) ? (T)(v.rv - mr) : (T)0;
+          else bv = v.bi;
+          if (nonzero) {
+            xv = (f & F_SOLV) ? v.xi - m : v.xi;
+          } else {
+            xv = w * rcoef<T, S>(A, f, v.i, v.j, td, ti).di * bv;
+          }
+        }
+        M.F[idx] = f;
+        B[idx] = bv;
+        X[idx] = xv;
+      });
   __syncthreads();
 
   // Pre-smoothing; v is the number of stencil rings still valid.
@@ -300,23 +366,38 @@ k_fine_up(FineOp A, FusedTile ft, T w, int nu, const T* __restrict__ x_pre,
   fine_tables<T>(A, M.td, M.ti);
   __syncthreads();
 
-  for_ring(Ring{0, ft.rx, 0, ry}, [&](int a, int c) {
+  struct UpLd {
     int i, j;
-    uint8_t f = 0;
-    T xv = (T)0, bv = (T)0, ev = (T)0;
-    if (resolve(o.gi0 + a, o.gj0 + c, A.nx, A.ny, A.px, A.py, i, j)) {
-      const int k = i * A.ny + j;
-      f = cell_code(A, A.flags[k], i, j);
-      xv = x_pre[k];
-      bv = b0[k];
-      if (f & F_SOLV) ev = xc[(i / fx) * cny + (j / fy)];
-    }
-    const int idx = a * ry + c;
-    M.F[idx] = f;
-    X[idx] = xv;
-    B[idx] = bv;
-    E[idx] = ev;
-  });
+    bool ok;
+    uint8_t fl;
+    T xv, bv, ev;
+  };
+  for_region_batched<kLoadBatch>(
+      ft.rx, ry,
+      [&](int a, int c) {
+        UpLd v;
+        v.ok = resolve(o.gi0 + a, o.gj0 + c, A.nx, A.ny, A.px, A.py, v.i, v.j);
+        v.fl = 0;
+        v.xv = v.bv = v.ev = (T)0;
+        if (v.ok) {
+          const int k = v.i * A.ny + v.j;
+          v.fl = A.flags[k];
+          v.xv = x_pre[k];
+          v.bv = b0[k];
+          // the coarse value is loaded for every grid cell (always in range)
+          // and discarded below on unsolvable ones, so no load waits on flags
+          v.ev = xc[(v.i / fx) * cny + (v.j / fy)];
+        }
+        return v;
+      },
+      [&](int a, int c, int idx, const UpLd& v) {
+        uint8_t f = 0;
+        if (v.ok) f = cell_code(A, v.fl, v.i, v.j);
+        M.F[idx] = f;
+        X[idx] = v.xv;
+        B[idx] = v.bv;
+        E[idx] = (f & F_SOLV) ? v.ev : (T)0;
+      });
   __syncthreads();
   // x += e - w D^-1 A e on ring nu.
   for_ring(ring(ft, nu, 1, 1), [&](int a, int c) {
@@ -384,20 +465,40 @@ k_fine_dir(FineOp A, FusedTile ft, const T* __restrict__ z, const double* __rest
   const bool first = st->iter == 0;
   const double beta = st->beta;
   const double mz = st->mean_z;
-  for_ring(Ring{0, ft.rx, 0, ry}, [&](int a, int c) {
+  struct DirLd {
     int i, j;
-    uint8_t f = 0;
-    double pv = 0.0;
-    if (resolve(o.gi0 + a, o.gj0 + c, A.nx, A.ny, A.px, A.py, i, j)) {
-      const int k = i * A.ny + j;
-      f = cell_code(A, A.flags[k], i, j);
-      const double zp = (f & F_SOLV) ? (double)z[k] - mz : 0.0;
-      pv = first ? zp : p_old[k] * beta + zp;
-    }
-    const int idx = a * ry + c;
-    F[idx] = f;
-    P[idx] = pv;
-  });
+    bool ok;
+    uint8_t fl;
+    T zv;
+    double po;
+  };
+  for_region_batched<kLoadBatch>(
+      ft.rx, ry,
+      [&](int a, int c) {
+        DirLd v;
+        v.ok = resolve(o.gi0 + a, o.gj0 + c, A.nx, A.ny, A.px, A.py, v.i, v.j);
+        v.fl = 0;
+        v.zv = (T)0;
+        v.po = 0.0;
+        if (v.ok) {
+          const int k = v.i * A.ny + v.j;
+          v.fl = A.flags[k];
+          v.zv = z[k];
+          if (!first) v.po = p_old[k];
+        }
+        return v;
+      },
+      [&](int a, int c, int idx, const DirLd& v) {
+        uint8_t f = 0;
+        double pv = 0.0;
+        if (v.ok) {
+          f = cell_code(A, v.fl, v.i, v.j);
+          const double zp = (f & F_SOLV) ? (double)v.zv - mz : 0.0;
+          pv = first ? zp : v.po * beta + zp;
+        }
+        F[idx] = f;
+        P[idx] = pv;
+      });
   __syncthreads();
   double s[1] = {0.0};
   for_ring(ring(ft, 0, 1, 1), [&](int a, int c) {
@@ -414,6 +515,245 @@ k_fine_dir(FineOp A, FusedTile ft, const T* __restrict__ z, const double* __rest
   });
   const int ops[1] = {kSum};
   block_partials<1>(s, ops, partial, o.t);
+}
+
+// ---------------------------------------------------------------------------
+// Level 0, streaming form: uniform mesh, one sweep each way, zero start (the
+// multigrid-CG preconditioner's cycle).  One wavefront owns a strip of 64
+// columns (the middle 60 produce output; two halo lanes each side) and walks
+// down L rows plus a four-row halo, keeping the last rows of every
+// intermediate in registers: a row's x-neighbours are the rows before and
+// after it in the walk, its y-neighbours the adjacent lanes (cross-lane
+// shuffles).  No LDS tile and no barriers.  Cells that are not solvable
+// hold zero at every stage, so a solvable row of the operator is
+// d u + cx (u_l + u_r) + cy (u_b + u_t) without face tests: a face carries
+// flux exactly when the neighbour is solvable (two fluid cells always share
+// one), and every other neighbour contributes zero.  Global loads run
+// kStreamPrefetch rows ahead of their use.
+// ---------------------------------------------------------------------------
+constexpr int kStreamHalo = 2;
+constexpr int kStreamOut = kWave - 2 * kStreamHalo;
+constexpr int kStreamPrefetch = 4;
+
+struct StreamPos {
+  int lane, i0, j, jr;
+  bool jin, out_col, active;
+};
+
+__device__ __forceinline__ StreamPos stream_pos(const FineOp& A, int strips, int L) {
+  const int wave = (int)(blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave);
+  StreamPos p;
+  p.lane = (int)(threadIdx.x % kWave);
+  const int strip = wave % strips;
+  p.i0 = (wave / strips) * L;
+  p.active = p.i0 < A.nx;
+  p.j = strip * kStreamOut - kStreamHalo + p.lane;
+  p.jr = p.j;
+  p.jin = true;
+  if (A.py) p.jr = wrap_index(p.j, A.ny);
+  else p.jin = p.j >= 0 && p.j < A.ny;
+  p.out_col = p.lane >= kStreamHalo && p.lane < kStreamHalo + kStreamOut && p.j < A.ny;
+  return p;
+}
+
+// Grid row of walk row p: wrapped on a periodic axis; false outside the grid.
+__device__ __forceinline__ bool stream_row(const FineOp& A, int p, int& ir) {
+  ir = p;
+  if (A.px) {
+    ir = wrap_index(p, A.nx);
+    return true;
+  }
+  return p >= 0 && p < A.nx;
+}
+
+template <typename T>
+__device__ __forceinline__ T lane_up(T v) {  // value of lane - 1
+  return __shfl_up(v, 1, kWave);
+}
+template <typename T>
+__device__ __forceinline__ T lane_down(T v) {  // value of lane + 1
+  return __shfl_down(v, 1, kWave);
+}
+
+// Down leg: b0 = r - mean_r on solvable cells, x = w D^-1 b0 (the zero-start
+// sweep), res = b0 - A x, t = D^-1 res, and the restriction
+// b_c = P_tent^T (res - w A t) over FX x FY blocks; writes b0, x and b_c.
+template <typename T, int FX, int FY>
+__global__ void __launch_bounds__(kBlock)
+k_fine_down_stream(FineOp A, int strips, int L, T w, const double* __restrict__ r,
+                   T* __restrict__ b_out, T* __restrict__ x_out, T* __restrict__ bc, int cnx,
+                   int cny, const SolveState* __restrict__ st) {
+  if (st->done) return;
+  __shared__ T td[128], ti[128];
+  fine_tables<T>(A, td, ti);
+  __syncthreads();
+  const StreamPos P = stream_pos(A, strips, L);
+  if (!P.active) return;  // whole wavefront: no shuffles are left half-done
+  const double mr = st->mean_r;
+  const T cx = (T)A.cx, cy = (T)A.cy;
+  const int p_begin = P.i0 - 2, p_end = P.i0 + L + 2;  // L % 4 == 0: a whole number of batches
+  // prefetch slots
+  uint8_t pf_f[kStreamPrefetch];
+  double pf_r[kStreamPrefetch];
+  int pf_i[kStreamPrefetch];
+  auto issue = [&](int slot, int p) {
+    int ir = 0;
+    pf_f[slot] = 0;
+    pf_r[slot] = 0.0;
+    pf_i[slot] = 0;
+    if (p < p_end && stream_row(A, p, ir) && P.jin) {
+      const int k = ir * A.ny + P.jr;
+      pf_f[slot] = A.flags[k];
+      pf_r[slot] = r[k];
+      pf_i[slot] = ir;
+    }
+  };
+#pragma unroll
+  for (int u = 0; u < kStreamPrefetch; ++u) issue(u, p_begin + u);
+  // pipeline registers: rows p-3 .. p
+  T x0 = 0, x1 = 0, b1 = 0, d0 = 1, d1 = 1, di1 = 1, res0 = 0, t0 = 0, t1 = 0, acc = 0;
+  bool s0 = false, s1 = false;
+  for (int pb = p_begin; pb < p_end; pb += kStreamPrefetch) {
+#pragma unroll
+    for (int u = 0; u < kStreamPrefetch; ++u) {
+      const int p = pb + u;
+      // stage A, row p
+      uint8_t f = 0;
+      if (pf_f[u]) f = cell_code(A, pf_f[u], pf_i[u], P.jr);
+      const double rv = pf_r[u];
+      issue(u, p + kStreamPrefetch);
+      const bool s2 = (f & F_SOLV) != 0;
+      const T b2 = s2 ? (T)(rv - mr) : (T)0;
+      const T d2 = td[f], di2 = ti[f];
+      const T x2 = w * di2 * b2;
+      if (p >= P.i0 && p < P.i0 + L && p < A.nx && P.out_col) {
+        const size_t k = (size_t)p * A.ny + P.j;
+        x_out[k] = x2;
+        b_out[k] = b2;
+      }
+      // stage B, row p-1: residual and t = D^-1 res
+      const T xl = lane_up(x1), xr = lane_down(x1);
+      const T res1 = s1 ? b1 - (d1 * x1 + cx * (x0 + x2) + cy * (xl + xr)) : (T)0;
+      const T t2 = di1 * res1;
+      // stage C, row p-2: y = res - w A t, restricted
+      const T tl = lane_up(t1), tr = lane_down(t1);
+      const int sr = p - 2;
+      const bool own = sr >= P.i0 && sr < A.nx && P.out_col;
+      T y = (s0 && own) ? res0 - w * (d0 * t1 + cx * (t0 + t2) + cy * (tl + tr)) : (T)0;
+      if (FY == 2) y += lane_down(y);
+      if (sr >= P.i0) {
+        const bool first_row = FX == 1 || ((sr - P.i0) & 1) == 0;
+        acc = first_row ? y : acc + y;
+        const bool last_row = FX == 1 || ((sr - P.i0) & 1) == 1;
+        const bool lane_ok = FY == 1 || ((P.lane - kStreamHalo) & 1) == 0;
+        if (last_row && lane_ok && P.lane >= kStreamHalo && P.lane < kStreamHalo + kStreamOut) {
+          const int I = sr / FX, J = P.j / FY;
+          if (I < cnx && J < cny && P.j < A.ny) bc[(size_t)I * cny + J] = acc;
+        }
+      }
+      // rotate the pipeline
+      x0 = x1;
+      x1 = x2;
+      b1 = b2;
+      d0 = d1;
+      s0 = s1;
+      d1 = d2;
+      di1 = di2;
+      s1 = s2;
+      res0 = res1;
+      t0 = t1;
+      t1 = t2;
+    }
+  }
+}
+
+// Up leg: e = P_tent x_c on solvable cells, xs = x_pre + e - w D^-1 A e, then
+// one post-sweep z = xs + w D^-1 (b0 - A xs); writes z and, for the
+// preconditioned CG, the block partials of sum z, sum b0 z, sum b0.
+template <typename T, int FX, int FY>
+__global__ void __launch_bounds__(kBlock)
+k_fine_up_stream(FineOp A, int strips, int L, T w, const T* __restrict__ x_pre,
+                 const T* __restrict__ b0, const T* __restrict__ xc, int cny, T* __restrict__ z,
+                 int reduce, double* partial, const SolveState* __restrict__ st) {
+  if (st->done) return;
+  __shared__ T td[128], ti[128];
+  fine_tables<T>(A, td, ti);
+  __syncthreads();
+  const StreamPos P = stream_pos(A, strips, L);
+  double sums[3] = {0.0, 0.0, 0.0};
+  if (P.active) {
+    const T cx = (T)A.cx, cy = (T)A.cy;
+    const int p_begin = P.i0 - 2, p_end = P.i0 + L + 2;
+    uint8_t pf_f[kStreamPrefetch];
+    int pf_i[kStreamPrefetch];
+    T pf_x[kStreamPrefetch], pf_b[kStreamPrefetch], pf_e[kStreamPrefetch];
+    auto issue = [&](int slot, int p) {
+      int ir = 0;
+      pf_f[slot] = 0;
+      pf_i[slot] = 0;
+      pf_x[slot] = pf_b[slot] = pf_e[slot] = (T)0;
+      if (p < p_end && stream_row(A, p, ir) && P.jin) {
+        const int k = ir * A.ny + P.jr;
+        pf_f[slot] = A.flags[k];
+        pf_i[slot] = ir;
+        pf_x[slot] = x_pre[k];
+        pf_b[slot] = b0[k];
+        pf_e[slot] = xc[(ir / FX) * cny + P.jr / FY];  // in range for every grid cell
+      }
+    };
+#pragma unroll
+    for (int u = 0; u < kStreamPrefetch; ++u) issue(u, p_begin + u);
+    // rows p-3 .. p
+    T e0 = 0, e1 = 0, x1 = 0, b0r = 0, b1 = 0, d0 = 1, d1 = 1, di0 = 1, di1 = 1;
+    T xs0 = 0, xs1 = 0;
+    bool s0 = false, s1 = false;
+    for (int pb = p_begin; pb < p_end; pb += kStreamPrefetch) {
+#pragma unroll
+      for (int u = 0; u < kStreamPrefetch; ++u) {
+        const int p = pb + u;
+        uint8_t f = 0;
+        if (pf_f[u]) f = cell_code(A, pf_f[u], pf_i[u], P.jr);
+        const bool s2 = (f & F_SOLV) != 0;
+        const T x2 = s2 ? pf_x[u] : (T)0, b2 = s2 ? pf_b[u] : (T)0, e2 = s2 ? pf_e[u] : (T)0;
+        issue(u, p + kStreamPrefetch);
+        const T d2 = td[f], di2 = ti[f];
+        // stage B, row p-1: prolongation with the smoothed prolongator
+        const T el = lane_up(e1), er = lane_down(e1);
+        const T xs2 =
+            s1 ? x1 + e1 - w * di1 * (d1 * e1 + cx * (e0 + e2) + cy * (el + er)) : (T)0;
+        // stage C, row p-2: post-sweep
+        const T ql = lane_up(xs1), qr = lane_down(xs1);
+        const int sr = p - 2;
+        if (sr >= P.i0 && sr < A.nx && P.out_col) {
+          const T zv =
+              s0 ? xs1 + w * di0 * (b0r - (d0 * xs1 + cx * (xs0 + xs2) + cy * (ql + qr))) : (T)0;
+          z[(size_t)sr * A.ny + P.j] = zv;
+          if (reduce != SR_NONE && s0) {
+            const double bd = (double)b0r;
+            sums[0] += (double)zv;
+            sums[1] += bd * (double)zv;
+            sums[2] += bd;
+          }
+        }
+        e0 = e1;
+        e1 = e2;
+        x1 = x2;
+        b0r = b1;
+        b1 = b2;
+        d0 = d1;
+        di0 = di1;
+        s0 = s1;
+        d1 = d2;
+        di1 = di2;
+        s1 = s2;
+        xs0 = xs1;
+        xs1 = xs2;
+      }
+    }
+  }
+  if (reduce == SR_NONE) return;
+  const int ops[3] = {kSum, kSum, kSum};
+  block_partials<3>(sums, ops, partial, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -506,21 +846,31 @@ k_class_down(ClassRows<T> L, FusedTile ft, int rxs, int rys, T w, int nu, const 
   stage_hot_row(L, M);
 
   // Load b and the row classes, and start from zero: x = w D^-1 b.
-  for_ring(Ring{0, ft.rx, 0, ry}, [&](int a, int c) {
-    int i, j;
-    uint16_t cl = kOutside;
-    T bv = (T)0, xv = (T)0;
-    if (resolve(o.gi0 + a, o.gj0 + c, L.nx, L.ny, L.px, L.py, i, j)) {
-      const int k = i * L.ny + j;
-      cl = L.cls[k];
-      bv = b[k];
-      xv = w * L.cdinv[cl] * bv;
-    }
-    const int idx = a * ry + c;
-    C[idx] = cl;
-    B[idx] = bv;
-    X[idx] = xv;
-  });
+  struct ClsLd {
+    bool ok;
+    uint16_t cl;
+    T bv;
+  };
+  for_region_batched<kLoadBatch>(
+      ft.rx, ry,
+      [&](int a, int c) {
+        ClsLd v;
+        int i, j;
+        v.ok = resolve(o.gi0 + a, o.gj0 + c, L.nx, L.ny, L.px, L.py, i, j);
+        v.cl = kOutside;
+        v.bv = (T)0;
+        if (v.ok) {
+          const int k = i * L.ny + j;
+          v.cl = L.cls[k];
+          v.bv = b[k];
+        }
+        return v;
+      },
+      [&](int a, int c, int idx, const ClsLd& v) {
+        C[idx] = v.cl;
+        B[idx] = v.bv;
+        X[idx] = v.ok ? w * L.cdinv[v.cl] * v.bv : (T)0;
+      });
   __syncthreads();
   for (int s = 1; s < nu; ++s) {
     for_ring(ring(ft, nu + 1 - s, rxs, rys), [&](int a, int c) {
@@ -592,23 +942,34 @@ k_class_up(ClassRows<T> L, FusedTile ft, int rxs, int rys, T w, int nu, const T*
   const TileOrigin o = tile_origin(ft);
   stage_hot_row(L, M);
 
-  for_ring(Ring{0, ft.rx, 0, ry}, [&](int a, int c) {
-    int i, j;
-    uint16_t cl = kOutside;
-    T xv = (T)0, bv = (T)0, ev = (T)0;
-    if (resolve(o.gi0 + a, o.gj0 + c, L.nx, L.ny, L.px, L.py, i, j)) {
-      const int k = i * L.ny + j;
-      cl = L.cls[k];
-      xv = x_pre[k];
-      bv = b[k];
-      if (L.cactive[cl]) ev = xc[(i / fx) * cny + (j / fy)];
-    }
-    const int idx = a * ry + c;
-    C[idx] = cl;
-    X[idx] = xv;
-    B[idx] = bv;
-    E[idx] = ev;
-  });
+  struct ClsUpLd {
+    bool ok;
+    uint16_t cl;
+    T xv, bv, ev;
+  };
+  for_region_batched<kLoadBatch>(
+      ft.rx, ry,
+      [&](int a, int c) {
+        ClsUpLd v;
+        int i, j;
+        v.ok = resolve(o.gi0 + a, o.gj0 + c, L.nx, L.ny, L.px, L.py, i, j);
+        v.cl = kOutside;
+        v.xv = v.bv = v.ev = (T)0;
+        if (v.ok) {
+          const int k = i * L.ny + j;
+          v.cl = L.cls[k];
+          v.xv = x_pre[k];
+          v.bv = b[k];
+          v.ev = xc[(i / fx) * cny + (j / fy)];  // in range for every grid cell
+        }
+        return v;
+      },
+      [&](int a, int c, int idx, const ClsUpLd& v) {
+        C[idx] = v.cl;
+        X[idx] = v.xv;
+        B[idx] = v.bv;
+        E[idx] = (v.ok && L.cactive[v.cl]) ? v.ev : (T)0;
+      });
   __syncthreads();
   for_ring(ring(ft, nu, rxs, rys), [&](int a, int c) {
     const int idx = a * ry + c;
@@ -700,6 +1061,61 @@ int launch_fine_up(hipStream_t s, const FineOp& A, const FusedTile& ft, T w, int
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
+
+int stream_strips(int ny) { return (ny + kStreamOut - 1) / kStreamOut; }
+
+int stream_blocks(int nx, int ny, int rows) {
+  const long waves = (long)stream_strips(ny) * ((nx + rows - 1) / rows);
+  return (int)((waves + kBlock / kWave - 1) / (kBlock / kWave));
+}
+
+bool stream_supported(const FineOp& A, int nu, int fx, int fy) {
+  return !A.hxc && nu == 1 && (fx == 1 || fx == 2) && (fy == 1 || fy == 2);
+}
+
+#define PCFD_STREAM_DISPATCH(KERNEL, ...)                                                    \
+  do {                                                                                     \
+    if (fx == 2 && fy == 2)                                                                \
+      PCFD_LAUNCH_TIMED(clk, (KERNEL<T, 2, 2>), grid, dim3(kBlock), 0, s, __VA_ARGS__);    \
+    else if (fx == 2)                                                                      \
+      PCFD_LAUNCH_TIMED(clk, (KERNEL<T, 2, 1>), grid, dim3(kBlock), 0, s, __VA_ARGS__);    \
+    else if (fy == 2)                                                                      \
+      PCFD_LAUNCH_TIMED(clk, (KERNEL<T, 1, 2>), grid, dim3(kBlock), 0, s, __VA_ARGS__);    \
+    else                                                                                   \
+      PCFD_LAUNCH_TIMED(clk, (KERNEL<T, 1, 1>), grid, dim3(kBlock), 0, s, __VA_ARGS__);    \
+  } while (0)
+
+template <typename T>
+int launch_fine_down_stream(hipStream_t s, const FineOp& A, int rows, T w, const double* r,
+                            T* b_out, T* x_out, T* bc, int fx, int fy, int cnx, int cny,
+                            const SolveState* st, const LaunchClock& clk) {
+  if (rows < 4 || rows % 4 || (fx == 2 && rows % 2)) {
+    set_last_error("streaming down leg: %d rows per wavefront (need a multiple of 4)", rows);
+    return kErrBadArg;
+  }
+  const dim3 grid(stream_blocks(A.nx, A.ny, rows));
+  const int strips = stream_strips(A.ny);
+  PCFD_STREAM_DISPATCH(k_fine_down_stream, A, strips, rows, w, r, b_out, x_out, bc, cnx, cny, st);
+  PCFD_LAUNCH_CHECK();
+  return kOk;
+}
+
+template <typename T>
+int launch_fine_up_stream(hipStream_t s, const FineOp& A, int rows, T w, const T* x_pre,
+                          const T* b0, const T* xc, int fx, int fy, int cny, T* z, int reduce,
+                          double* partial, const SolveState* st, const LaunchClock& clk) {
+  if (rows < 4 || rows % 4) {
+    set_last_error("streaming up leg: %d rows per wavefront (need a multiple of 4)", rows);
+    return kErrBadArg;
+  }
+  const dim3 grid(stream_blocks(A.nx, A.ny, rows));
+  const int strips = stream_strips(A.ny);
+  PCFD_STREAM_DISPATCH(k_fine_up_stream, A, strips, rows, w, x_pre, b0, xc, cny, z, reduce,
+                       partial, st);
+  PCFD_LAUNCH_CHECK();
+  return kOk;
+}
+#undef PCFD_STREAM_DISPATCH
 
 template <typename T>
 int launch_class_down(hipStream_t s, const ClassRows<T>& L, const FusedTile& ft, int rxs,
@@ -816,6 +1232,12 @@ bool choose_tile(int nx, int ny, int rxs, int rys, int halo_mult, int fx, int fy
                                   const SolveState*);                                         \
   template int launch_dense_apply<T>(hipStream_t, const double*, int, const T*, T*,            \
                                      const SolveState*);                                      \
+  template int launch_fine_down_stream<T>(hipStream_t, const FineOp&, int, T, const double*,   \
+                                          T*, T*, T*, int, int, int, int, const SolveState*,  \
+                                          const LaunchClock&);                                \
+  template int launch_fine_up_stream<T>(hipStream_t, const FineOp&, int, T, const T*, const T*, \
+                                        const T*, int, int, int, T*, int, double*,            \
+                                        const SolveState*, const LaunchClock&);               \
   template int launch_fine_dir<T>(hipStream_t, const FineOp&, const FusedTile&, const T*,      \
                                   const double*, double*, double*, double*, const SolveState*, \
                                   const LaunchClock&);

@@ -55,6 +55,21 @@ int launch_fine_up(hipStream_t s, const FineOp& A, const FusedTile& ft, T w, int
                    int reduce, double* partial, const SolveState* st,
                    const LaunchClock& clk = LaunchClock{});
 
+// Level-0 legs in streaming form (uniform mesh, one sweep, zero start): one
+// wavefront per strip of 60 output columns and `rows` rows (a multiple of 4);
+// the up leg writes one row of partials per workgroup (stream_blocks).
+bool stream_supported(const FineOp& A, int nu, int fx, int fy);
+int stream_blocks(int nx, int ny, int rows);
+template <typename T>
+int launch_fine_down_stream(hipStream_t s, const FineOp& A, int rows, T w, const double* r,
+                            T* b_out, T* x_out, T* bc, int fx, int fy, int cnx, int cny,
+                            const SolveState* st, const LaunchClock& clk = LaunchClock{});
+template <typename T>
+int launch_fine_up_stream(hipStream_t s, const FineOp& A, int rows, T w, const T* x_pre,
+                          const T* b0, const T* xc, int fx, int fy, int cny, T* z, int reduce,
+                          double* partial, const SolveState* st,
+                          const LaunchClock& clk = LaunchClock{});
+
 template <typename T>
 int launch_class_down(hipStream_t s, const ClassRows<T>& L, const FusedTile& ft, int rxs,
                       int rys, T w, int nu, const T* b, T* x_out, T* bc, int fx, int fy, int cnx,

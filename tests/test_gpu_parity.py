@@ -92,14 +92,17 @@ def test_operator_apply(gpu_available, name):
     assert np.abs(y.cpu().numpy() - ref).max() <= 1e-14 * np.abs(ref).max()
 
 
-def _mg_solver(name, fused=True, chain=-1, **kw):
+def _mg_solver(name, fused=True, chain=-1, stream=0, **kw):
     """fused: stored levels through the fused kernels where they fit; chain:
-    the cooperative launch of the deepest levels (-1 never, k from level k)."""
+    the cooperative launch of the deepest levels (-1 never, k from level k);
+    stream: level 0's Krylov-start legs in streaming form (0 measured, -1
+    never, k rows per wavefront)."""
     s = _solver(name, pressure_solver=kw.pop("pressure_solver", "mgcg"), **kw)
     ps = s.pressure_solver
-    if not fused or chain != 0:
+    if not fused or chain != 0 or stream != 0:
         ps.fused_levels = bool(fused)
         ps.coarse_chain = chain
+        ps.stream_rows = stream
         ps.configure()
     return s
 
@@ -123,11 +126,11 @@ def test_vcycle_matches_reference(gpu_available, name, sweeps, fused):
 
 VCYCLE_GRIDS = [("cavity", {}), ("periodic", {}), ("cyl_outlet", {}), ("cyl_pout", {}),
                 ("cyl_outlet", dict(nx=256, ny=128)), ("cyl_pout", dict(nx=160, ny=72)),
-                ("periodic", dict(nx=128, ny=96))]
+                ("periodic", dict(nx=128, ny=96)), ("cavity", dict(nx=37, ny=23))]
 
 
-CYCLE_MODES = {"fused": (True, -1), "passes": (False, -1), "chain1": (True, 1),
-               "chain2": (True, 2)}
+CYCLE_MODES = {"fused": (True, -1, -1), "passes": (False, -1, -1), "chain1": (True, 1, -1),
+               "chain2": (True, 2, -1), "stream8": (True, -1, 8), "stream16": (True, -1, 16)}
 
 
 @pytest.mark.parametrize("name,grid", VCYCLE_GRIDS,
@@ -139,8 +142,11 @@ def test_vcycle_matches_oracle(gpu_available, name, grid, coarse_max, sweeps, mo
     """One V-cycle with the coarsest solve at the first level of at most
     coarse_max unknowns, vs the oracle's cycle on the same hierarchy:
     FP64 to 1e-10, the FP32 cycle to 1e-4 (relative to max|z|).  Modes: fused
-    level kernels, pass-by-pass levels, the cooperative chain from level 1 or 2."""
-    fused, chain = CYCLE_MODES[mode]
+    level kernels, pass-by-pass levels, the cooperative chain from level 1 or 2,
+    level 0's streaming legs (8 or 16 rows per wavefront; one sweep only)."""
+    fused, chain, stream = CYCLE_MODES[mode]
+    if stream > 0 and sweeps != 1:
+        pytest.skip("the streaming legs serve one sweep each way")
     import torch
     case = oracle_case(name, **grid)
     S = O.assemble(case)
@@ -150,8 +156,11 @@ def test_vcycle_matches_oracle(gpu_available, name, grid, coarse_max, sweeps, mo
     r = rng.standard_normal(S.A.shape[0])
     ref = H.apply(r)
     for precision, tol in (("fp64", 1e-10), ("mixed", 1e-4)):
-        s = _mg_solver(name, fused, chain, mg_sweeps=sweeps, mg_coarse_max=coarse_max,
+        s = _mg_solver(name, fused, chain, stream, mg_sweeps=sweeps, mg_coarse_max=coarse_max,
                        precision=precision, **grid)
+        plan = s.ctx.mg_plan()
+        if stream > 0 and len(plan) > 1:  # a one-level hierarchy is the dense solve alone
+            assert plan[0]["mode"] == "stream"
         z = torch.empty(r.size, dtype=torch.float64, device="cuda")
         s.ctx.mg_apply(torch.as_tensor(r).cuda(), z)
         err = np.abs(z.cpu().numpy() - ref).max() / np.abs(ref).max()
@@ -269,13 +278,18 @@ def test_run_twice_bitwise(gpu_available):
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("coarse_max", [96, 2048])
-def test_oracle_parity_mgcg(gpu_available, precision, coarse_max):
-    """mgcg at tol 1e-12 vs the oracle's direct solve, cylinder with OUTLET."""
+@pytest.mark.parametrize("stream", [-1, 8])
+def test_oracle_parity_mgcg(gpu_available, precision, coarse_max, stream):
+    """mgcg at tol 1e-12 vs the oracle's direct solve, cylinder with OUTLET,
+    with level 0's tiled or streaming legs."""
     case = oracle_case("cyl_outlet", pressure_solver="direct")
     u0, v0 = initial("cyl_outlet", case)
     u, v, p, hist, t = O.run(case, 6, u0, v0)
     sim = _sim("cyl_outlet", u0, v0, pressure_solver="mgcg", poisson_tol=1e-12,
                poisson_maxiter=300, precision=precision, mg_coarse_max=coarse_max)
+    ps = sim.solver.pressure_solver
+    ps.stream_rows = stream
+    ps.configure()
     sim.run(max_steps=6)
     assert np.abs(sim.fields.u - u).max() <= 1e-9 * np.abs(u).max()
     assert np.abs(sim.fields.v - v).max() <= 1e-9 * np.abs(u).max()
