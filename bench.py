@@ -13,10 +13,11 @@ One "step" is one full SSP-RK3 time step (three projections) through
 Simulation.run, per-step diagnostics included, with fields resident in HBM.
 MLUPS = nx * ny * steps / wall seconds / 1e6.
 
-Kernel rooflines come from a second window of the same length right after
-the timed one, with the engine's live kernel clock on: every launch of the
-four fine-level kernels inside the real solves records start/stop events on
-the solver's stream (hipExtLaunchKernelGGL).  The clock is off in the MLUPS
+Kernel rooflines come from a second window (up to 10 steps) right after
+the timed one, with the engine's live kernel clock on: every kernel launch of
+the real pressure solves records start/stop events on the solver's stream
+(hipExtLaunchKernelGGL), filed by kernel and multigrid level.  `roofline` is
+the one with the largest share of that window.  The clock is off in the MLUPS
 window.  HBM traffic per launch is read from the committed rocprofv3 counter
 summary when one exists for the kernel (profiles/pmc_traffic_*.md).
 
@@ -37,7 +38,6 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 DIV_TARGET = 1.0e-10
-KERNEL_NAMES = ("k_cg_update", "k_fine_dir<{t}>", "k_fine_down<{t}>", "k_fine_up<{t}>")
 TRAFFIC_FILE = ROOT / "profiles" / "pmc_traffic_cylinder2048x1024.md"
 
 
@@ -45,9 +45,21 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def kernel_names(precision: str) -> list[str]:
+def kernel_name(entry: dict, precision: str, plan: list[dict]) -> str:
+    """Name of a clocked (kind, level) slot: template type, and the level's
+    grid for the stored levels; level 0's legs carry `_stream` when the
+    streaming form runs them."""
     t = "float" if precision == "mixed" else "double"
-    return [k.format(t=t) for k in KERNEL_NAMES]
+    kind, level = entry["kind"], entry["level"]
+    if kind in ("k_cg_update", "k_fold"):
+        return kind if kind == "k_cg_update" else f"k_fold[{('cycle', 'alpha', 'update')[level]}]"
+    if kind in ("k_fine_down", "k_fine_up"):
+        stream = plan and plan[0]["mode"] == "stream"
+        return f"{kind}{'_stream' if stream else ''}<{t}>"
+    if kind == "k_fine_dir":
+        return f"{kind}<{t}>"
+    grid = f"{plan[level]['nx']}x{plan[level]['ny']}" if level < len(plan) else "?"
+    return f"{kind}<{t}> L{level} {grid}"
 
 
 def load_traffic() -> dict:
@@ -109,16 +121,16 @@ def run_path(precision: str, steps: int, warmup: int, nx: int, ny: int, tol: flo
         f"max|div| {out['max_div']:.2e}, {out['iterations_last_solve']} it/solve")
     # second window, live kernel clock on
     ctx = sim.solver.ctx
-    ctx.kernel_clock_start(max_launches=steps * 3 * 40)
+    csteps = min(steps, 10)  # ~32 launches per iteration, ~16 iterations per solve
+    ctx.kernel_clock_start(max_launches=csteps * 3 * 16 * 32)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    sim.run(max_steps=steps)
+    sim.run(max_steps=csteps)
     torch.cuda.synchronize()
     clocked = time.perf_counter() - t0
     kernels = []
-    for name, (n, ms, nbytes) in zip(kernel_names(precision), ctx.kernel_clock_read()):
-        if n == 0:
-            continue
+    for e in ctx.kernel_clock_read():
+        name, n, ms, nbytes = kernel_name(e, precision, plan), e["launches"], e["ms"], e["bytes"]
         achieved = nbytes / (ms * 1e-3) / 1e9
         kernels.append({"kernel": name, "bound": "hbm", "achieved": round(achieved, 1),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -126,9 +138,13 @@ def run_path(precision: str, steps: int, warmup: int, nx: int, ny: int, tol: flo
                         "traffic": traffic.get(name), "avg_us": round(ms * 1e3, 3),
                         "launches": n, "bytes_per_launch": nbytes,
                         "share_of_window": round(n * ms * 1e-3 / clocked, 4)})
-    out["clocked_ms_per_step"] = 1e3 * clocked / steps
-    log(f"[{precision}] clocked window {1e3 * clocked / steps:.3f} ms/step; " + "; ".join(
-        f"{k['kernel']} {k['avg_us']} us {k['frac']}" for k in kernels))
+    kernels.sort(key=lambda k: -k["share_of_window"])
+    out["clocked_ms_per_step"] = 1e3 * clocked / csteps
+    out["solver_kernel_share"] = round(sum(k["share_of_window"] for k in kernels), 4)
+    log(f"[{precision}] clocked window {1e3 * clocked / csteps:.3f} ms/step, solver kernels "
+        f"{100 * out['solver_kernel_share']:.1f}% of it; top: " + "; ".join(
+            f"{k['kernel']} {k['avg_us']} us x{k['launches']} frac {k['frac']} "
+            f"share {k['share_of_window']}" for k in kernels[:8]))
     return sim, out, kernels
 
 

@@ -251,13 +251,21 @@ int pcfd_mg_apply(pcfd_ctx* ctx, const double* r, double* z);
  * reference counterpart: measurement only. */
 int pcfd_time_kernel(pcfd_ctx* ctx, int32_t which, int32_t reps, double* avg_ms,
                      double* bytes_per_launch);
-/* The same four kernels timed live inside the solves that follow: every
- * launch records events at the kernel's own start and end
- * (hipExtLaunchKernelGGL), up to max_launches per kernel; launches of
- * iterations enqueued past convergence (no-ops) are dropped.  _read stops the
- * clock and fills, per kernel (arrays of 4): launches timed, mean duration in
- * ms, algorithmic bytes per launch.  No reference counterpart: measurement
- * only. */
+/* Every kernel of the pressure solves that follow, timed live: each launch
+ * records events at the kernel's own start and end (hipExtLaunchKernelGGL),
+ * up to max_launches launches in all; launches of iterations enqueued past
+ * convergence (no-ops) are dropped.  _read stops the clock and fills, per
+ * slot (arrays of PCFD_CLOCK_SLOTS), launches timed, mean duration in ms and
+ * algorithmic bytes per launch.  slot = kind * PCFD_CLOCK_LEVELS + level,
+ * kinds: 0 CG update, 1 multigrid-CG direction, 2 / 3 level-0 down / up leg,
+ * 4 / 5 fused stored-level down / up leg, 6-9 stored-level row passes
+ * (zero-start smooth + residual, smooth, residual, prolongation), 10
+ * restriction, 11 dense coarsest solve, 12 cooperative chain, 13 reductions
+ * (level 0: cycle, 1: alpha, 2: update).  No reference counterpart:
+ * measurement only. */
+#define PCFD_CLOCK_KINDS 14
+#define PCFD_CLOCK_LEVELS 16
+#define PCFD_CLOCK_SLOTS (PCFD_CLOCK_KINDS * PCFD_CLOCK_LEVELS)
 int pcfd_kernel_clock_start(pcfd_ctx* ctx, int32_t max_launches);
 int pcfd_kernel_clock_read(pcfd_ctx* ctx, int32_t* launches, double* avg_ms,
                            double* bytes_per_launch);
@@ -265,9 +273,9 @@ int pcfd_kernel_clock_read(pcfd_ctx* ctx, int32_t* launches, double* avg_ms,
 /* The multigrid plan of the configured solver, for reporting: per level (all
  * levels of the cycle precision, finest first) five integers -- nx, ny,
  * mode (1: one fused launch per leg, 0: pass by pass, 2: inside the
- * cooperative launch of the deepest levels, 3: level 0's Krylov-start legs in
- * streaming form, tile = rows per wavefront x 60 columns; the coarsest level
- * is the dense solve), tile x, tile y -- written to out (capacity
+ * cooperative launch of the deepest levels, 3: streaming legs, tile = rows
+ * per wavefront x output columns per wavefront; the coarsest level is the
+ * dense solve), tile x, tile y -- written to out (capacity
  * `cap` integers).  Returns the number of levels, or a negative status.  No
  * reference counterpart. */
 int pcfd_mg_plan(pcfd_ctx* ctx, int32_t* out, int32_t cap);

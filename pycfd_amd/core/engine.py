@@ -230,16 +230,29 @@ class DeviceContext:
         self._call("pcfd_time_kernel", int(which), int(reps), C.byref(ms), C.byref(b))
         return float(ms.value), float(b.value)
 
+    CLOCK_KINDS = ("k_cg_update", "k_fine_dir", "k_fine_down", "k_fine_up", "k_class_down",
+                   "k_class_up", "k_cls_rows[smooth0+res]", "k_cls_rows[smooth]",
+                   "k_cls_rows[res]", "k_cls_rows[prolong]", "k_cls_restrict", "k_dense_apply",
+                   "k_chain", "k_fold")
+    CLOCK_LEVELS = 16
+
     def kernel_clock_start(self, max_launches: int) -> None:
-        """Time kernels 0-3 live inside the solves that follow."""
+        """Time every kernel of the pressure solves that follow, live."""
         self._call("pcfd_kernel_clock_start", int(max_launches))
 
-    def kernel_clock_read(self) -> list[tuple[int, float, float]]:
-        """Per kernel 0-3: (launches, mean ms, algorithmic bytes per launch)."""
-        n = (C.c_int32 * 4)()
-        ms, b = (C.c_double * 4)(), (C.c_double * 4)()
+    def kernel_clock_read(self) -> list[dict]:
+        """Per clocked (kind, level): launches, mean ms, algorithmic bytes per launch."""
+        slots = len(self.CLOCK_KINDS) * self.CLOCK_LEVELS
+        n = (C.c_int32 * slots)()
+        ms, b = (C.c_double * slots)(), (C.c_double * slots)()
         self._call("pcfd_kernel_clock_read", n, ms, b)
-        return [(int(n[k]), float(ms[k]), float(b[k])) for k in range(4)]
+        out = []
+        for q in range(slots):
+            if n[q]:
+                kind, level = divmod(q, self.CLOCK_LEVELS)
+                out.append({"kind": self.CLOCK_KINDS[kind], "level": level, "launches": int(n[q]),
+                            "ms": float(ms[q]), "bytes": float(b[q])})
+        return out
 
     def mg_plan(self) -> list[dict]:
         """Per multigrid level: grid, whether its legs run fused, and the tile."""

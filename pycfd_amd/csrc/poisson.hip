@@ -906,8 +906,9 @@ static inline dim3 tiles_for(int nx, int ny) { return dim3(num_tiles(nx, ny)); }
 static inline int ntiles(int nx, int ny) { return num_tiles(nx, ny); }
 
 template <class Epi>
-static int fold(hipStream_t s, double* partial, int n, const Epi& e) {
-  hipLaunchKernelGGL(k_fold<Epi>, dim3(1), dim3(kBlock), 0, s, (const double*)partial, n, e);
+static int fold(hipStream_t s, double* partial, int n, const Epi& e,
+                const LaunchClock& clk = LaunchClock{}) {
+  PCFD_LAUNCH_TIMED(clk, k_fold<Epi>, dim3(1), dim3(kBlock), 0, s, (const double*)partial, n, e);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
@@ -925,12 +926,10 @@ PoissonEngine::~PoissonEngine() {
 }
 
 void PoissonEngine::clock_free() {
-  for (KernelClock& k : clock_) {
-    for (hipEvent_t e : k.ev) hipEventDestroy(e);
-    k.ev.clear();
-    k.used = 0;
-    k.bytes = 0.0;
-  }
+  for (hipEvent_t e : clock_pool_) hipEventDestroy(e);
+  clock_pool_.clear();
+  clock_pool_used_ = 0;
+  clock_log_.clear();
   clock_on_ = false;
 }
 
@@ -941,38 +940,38 @@ int PoissonEngine::clock_start(int max_launches) {
   }
   PCFD_HIP_CHECK(hipStreamSynchronize(s_));
   clock_free();
-  for (KernelClock& k : clock_) {
-    k.ev.resize(2 * (size_t)max_launches, nullptr);
-    for (hipEvent_t& e : k.ev) PCFD_HIP_CHECK(hipEventCreate(&e));
-  }
+  clock_pool_.resize(2 * (size_t)max_launches, nullptr);
+  for (hipEvent_t& e : clock_pool_) PCFD_HIP_CHECK(hipEventCreate(&e));
+  clock_log_.reserve((size_t)max_launches);
   clock_on_ = true;
   return kOk;
 }
 
-LaunchClock PoissonEngine::clock_next(int which, double bytes) {
-  if (!clock_on_) return LaunchClock{};
-  KernelClock& k = clock_[which];
-  if (2 * (k.used + 1) > k.ev.size()) return LaunchClock{};
-  LaunchClock c{k.ev[2 * k.used], k.ev[2 * k.used + 1]};
-  ++k.used;
-  k.bytes = bytes;
+LaunchClock PoissonEngine::clock_next(int kind, int level, double bytes) {
+  if (!clock_on_ || clock_pool_used_ + 2 > clock_pool_.size()) return LaunchClock{};
+  const int lev = level < kClockLevels ? level : kClockLevels - 1;
+  LaunchClock c{clock_pool_[clock_pool_used_], clock_pool_[clock_pool_used_ + 1]};
+  clock_pool_used_ += 2;
+  clock_log_.push_back(ClockEntry{kind * kClockLevels + lev, clock_iter_, c.start, c.stop, bytes});
   return c;
 }
 
 int PoissonEngine::clock_read(int32_t* launches, double* avg_ms, double* bytes) {
   PCFD_HIP_CHECK(hipStreamSynchronize(s_));
-  for (int w = 0; w < kClockKernels; ++w) {
-    KernelClock& k = clock_[w];
-    double total = 0.0;
-    for (size_t i = 0; i < k.used; ++i) {
-      float ms = 0.f;
-      PCFD_HIP_CHECK(hipEventElapsedTime(&ms, k.ev[2 * i], k.ev[2 * i + 1]));
-      total += ms;
-    }
-    launches[w] = (int32_t)k.used;
-    avg_ms[w] = k.used ? total / (double)k.used : 0.0;
-    bytes[w] = k.bytes;
+  std::vector<double> total(kClockSlots, 0.0);
+  for (int q = 0; q < kClockSlots; ++q) {
+    launches[q] = 0;
+    bytes[q] = 0.0;
   }
+  for (const ClockEntry& e : clock_log_) {
+    if (e.slot < 0) continue;
+    float ms = 0.f;
+    PCFD_HIP_CHECK(hipEventElapsedTime(&ms, e.a, e.b));
+    total[e.slot] += ms;
+    ++launches[e.slot];
+    bytes[e.slot] = e.bytes;
+  }
+  for (int q = 0; q < kClockSlots; ++q) avg_ms[q] = launches[q] ? total[q] / launches[q] : 0.0;
   clock_free();
   return kOk;
 }
@@ -991,6 +990,9 @@ void PoissonEngine::free_levels() {
       hipFree(L.cactive);
       hipFree(L.cinv);
       hipFree(L.ent_off);
+      hipFree(L.box);
+      hipFree(L.box_dinv);
+      hipFree(L.box_act);
     }
     v->clear();
   }
@@ -1106,6 +1108,27 @@ static int upload_level(const MGLevelHost& h, const MGLevelHost* next, bool fine
     L.ncls = (int)nc;
     L.nnz = (int)val.size();
     L.max_row = max_row;
+    // box dictionary for the streaming legs: class rows as dense (2R+1)^2
+    // boxes, then one all-zero class for cells outside the grid
+    const int R = std::max(L.rxs, L.rys);
+    if (next && !wide && R <= kBoxMaxRadius) {
+      const int W = 2 * R + 1;
+      std::vector<T> box((nc + 1) * (size_t)W * W, (T)0), bd(nc + 1, (T)0);
+      std::vector<uint8_t> ba(nc + 1, 0);
+      for (size_t c = 0; c < nc; ++c) {
+        for (size_t q = 0; q < S; ++q)
+          box[c * W * W + (size_t)(h.di[q] + R) * W + (h.dj[q] + R)] = (T)h.table[c * S + q];
+        bd[c] = cd[c];
+        ba[c] = h.cactive[c];
+      }
+      PCFD_HIP_CHECK(hipMalloc(&L.box, box.size() * sizeof(T)));
+      PCFD_HIP_CHECK(hipMalloc(&L.box_dinv, bd.size() * sizeof(T)));
+      PCFD_HIP_CHECK(hipMalloc(&L.box_act, ba.size()));
+      PCFD_HIP_CHECK(hipMemcpy(L.box, box.data(), box.size() * sizeof(T), hipMemcpyHostToDevice));
+      PCFD_HIP_CHECK(hipMemcpy(L.box_dinv, bd.data(), bd.size() * sizeof(T), hipMemcpyHostToDevice));
+      PCFD_HIP_CHECK(hipMemcpy(L.box_act, ba.data(), ba.size(), hipMemcpyHostToDevice));
+      L.box_radius = R;
+    }
     if (next) {
       std::vector<int> agg(n, -1);
       for (int i = 0; i < h.nx; ++i)
@@ -1226,13 +1249,21 @@ int PoissonEngine::tune_level(std::vector<LevelBuf>& lv, int l, SolveState* tst)
   const int nu = cfg_.sweeps;
   const bool fine = l == 0;
   L.fused = 0;
-  if (!fine && L.hdi.empty()) return kOk;
+  L.box_rows = 0;
+  // streaming legs: level 0 (uniform mesh) or a stored level with a box
+  // dictionary; one sweep each way
+  const bool stream_ok = cfg_.stream >= 0 &&
+                         (fine ? stream_supported(op_, nu, L.fx, L.fy)
+                               : (L.box_radius > 0 && nu == 1));
+  if (!fine && L.hdi.empty() && !stream_ok) return kOk;
   const unsigned per_cell = fine ? 3u * sizeof(T) + 1u : 3u * sizeof(T) + 2u;
   const unsigned fixed = fine ? 256u * sizeof(T)
                               : (L.hot >= 0 ? (unsigned)L.hot_n * (unsigned)(sizeof(T) + 4) : 0u);
   const int ftx = fine ? cfg_.tile_tx : 0, fty = fine ? cfg_.tile_ty : 0;
-  std::vector<FusedTile> cands = tile_candidates(L.nx, L.ny, L.rxs, L.rys, nu + 1, L.fx, L.fy,
-                                                 per_cell, fixed, kLdsCap, ftx, fty);
+  std::vector<FusedTile> cands;
+  if (fine || !L.hdi.empty())
+    cands = tile_candidates(L.nx, L.ny, L.rxs, L.rys, nu + 1, L.fx, L.fy, per_cell, fixed,
+                            kLdsCap, ftx, fty);
   if (fine) {  // the standalone solver's nonzero start needs one more ring
     std::vector<FusedTile> keep;
     for (const FusedTile& t : cands) {
@@ -1251,7 +1282,7 @@ int PoissonEngine::tune_level(std::vector<LevelBuf>& lv, int l, SolveState* tst)
     for (const FusedTile& t : cands) most = std::max(most, (size_t)t.ntiles() * 3);
     if (int rc = ensure_partials(most)) return rc;
   }
-  if (cands.empty()) return kOk;
+  if (cands.empty() && !stream_ok) return kOk;
 
   auto upload_offsets = [&](const FusedTile& t) -> int {
     std::vector<int> off(std::max<size_t>(L.hdi.size(), 1), 0);
@@ -1262,12 +1293,19 @@ int PoissonEngine::tune_level(std::vector<LevelBuf>& lv, int l, SolveState* tst)
     PCFD_HIP_CHECK(hipStreamSynchronize(s_));
     return kOk;
   };
-  auto apply = [&](int fused, const FusedTile& t, int stream = 0) -> int {
+  // fused: tiled kernels with tile t (0: pass by pass); stream > 0: the
+  // streaming legs with that many rows per wavefront (level 0 keeps the tile
+  // for its nonzero-start legs)
+  auto apply = [&](int fused, const FusedTile& t, int stream) -> int {
     L.fused = fused;
-    L.stream_rows = stream;
-    L.stream_nb = stream ? stream_blocks(L.nx, L.ny, stream) : 0;
-    if (stream) {
-      if (int rc = ensure_partials((size_t)L.stream_nb * 3)) return rc;
+    if (fine) {
+      L.stream_rows = stream;
+      L.stream_nb = stream ? stream_blocks(L.nx, L.ny, stream) : 0;
+      if (stream) {
+        if (int rc = ensure_partials((size_t)L.stream_nb * 3)) return rc;
+      }
+    } else {
+      L.box_rows = stream;
     }
     if (!fused) return kOk;
     L.down0 = t;
@@ -1297,9 +1335,20 @@ int PoissonEngine::tune_level(std::vector<LevelBuf>& lv, int l, SolveState* tst)
                                (const T*)C.y, L.fx, L.fy, C.ny, (T*)L.y, (int)SR_MGCG, partial_,
                                tst);
     }
-    if (int rc = level_down<T>(L, C, tst)) return rc;
-    return level_up<T>(L, C, tst);
+    if (int rc = level_down<T>(L, C, tst, l)) return rc;
+    return level_up<T>(L, C, tst, l);
   };
+  const FusedTile none{};
+  // a forced streaming choice (cfg_.stream > 0) needs no measurement
+  if (cfg_.stream > 0 && stream_ok) {
+    const FusedTile t =
+        cands.empty() ? none
+                      : *std::min_element(cands.begin(), cands.end(),
+                                          [](const FusedTile& a, const FusedTile& b) {
+                                            return tile_cost(a) < tile_cost(b);
+                                          });
+    return apply(fine ? 1 : 0, t, cfg_.stream);
+  }
 
   const TuneKey key = {fine ? 1 : 0, L.nx, L.ny, L.rxs, L.rys, L.fx, L.fy, nu, (int)sizeof(T),
                        op_.px, op_.py, ftx, fty, op_.hxc != nullptr};
@@ -1308,19 +1357,20 @@ int PoissonEngine::tune_level(std::vector<LevelBuf>& lv, int l, SolveState* tst)
     auto it = g_tune_cache.find(key);
     if (it != g_tune_cache.end()) {
       const TunePick pk = it->second;
-      // a forced or disabled streaming choice re-plans level 0 instead
-      if (cfg_.stream == 0 || !fine)
-        for (const FusedTile& t : cands)
-          if (t.tx == pk.tx && t.ty == pk.ty) return apply(pk.fused, t, pk.stream);
-      if (!pk.fused && !fine) return apply(0, cands[0]);
+      const int st = stream_ok ? pk.stream : 0;
+      if (!pk.fused && !fine) return apply(0, none, st);
+      for (const FusedTile& t : cands)
+        if (t.tx == pk.tx && t.ty == pk.ty) return apply(pk.fused, t, st);
     }
   }
-  int best_fused = 1, best_stream = 0;
-  FusedTile best = *std::min_element(cands.begin(), cands.end(),
-                                     [](const FusedTile& a, const FusedTile& b) {
-                                       return tile_cost(a) < tile_cost(b);
-                                     });
-  if (autotune_enabled() && (cands.size() > 1 || !fine)) {
+  int best_fused = fine ? 1 : 0, best_stream = 0;
+  FusedTile best = cands.empty() ? none
+                                 : *std::min_element(cands.begin(), cands.end(),
+                                                     [](const FusedTile& a, const FusedTile& b) {
+                                                       return tile_cost(a) < tile_cost(b);
+                                                     });
+  if (!cands.empty() && !fine) best_fused = 1;
+  if (autotune_enabled() && (cands.size() > 1 || !fine || stream_ok)) {
     hipEvent_t e0, e1;
     PCFD_HIP_CHECK(hipEventCreate(&e0));
     PCFD_HIP_CHECK(hipEventCreate(&e1));
@@ -1337,13 +1387,14 @@ int PoissonEngine::tune_level(std::vector<LevelBuf>& lv, int l, SolveState* tst)
     float best_ms = 0.f;
     bool have = false;
     if (!fine) {  // the pass-by-pass path as the first contender
-      if (int rc = apply(0, cands[0])) return rc;
+      if (int rc = apply(0, none, 0)) return rc;
       if (int rc = time_legs(&best_ms)) return rc;
       best_fused = 0;
+      best = none;
       have = true;
     }
     for (const FusedTile& t : cands) {
-      if (int rc = apply(1, t)) return rc;
+      if (int rc = apply(1, t, 0)) return rc;
       float ms = 0.f;
       if (int rc = time_legs(&ms)) return rc;
       if (!have || ms < best_ms) {
@@ -1353,16 +1404,21 @@ int PoissonEngine::tune_level(std::vector<LevelBuf>& lv, int l, SolveState* tst)
         best_fused = 1;
       }
     }
-    // level 0: the streaming Krylov-start legs as further contenders (the
-    // tiled tiling stays planned for the nonzero-start legs)
-    if (fine && cfg_.stream == 0 && stream_supported(op_, nu, L.fx, L.fy)) {
+    // the streaming legs as further contenders (level 0 keeps its best tile
+    // for the nonzero-start legs)
+    if (stream_ok) {
       for (int rows : {4, 8, 16, 32}) {
-        if (int rc = apply(1, best, rows)) return rc;
+        if (int rc = apply(fine ? 1 : 0, fine ? best : none, rows)) return rc;
         float ms = 0.f;
         if (int rc = time_legs(&ms)) return rc;
-        if (ms < best_ms) {
+        if (!have || ms < best_ms) {
+          have = true;
           best_ms = ms;
           best_stream = rows;
+          if (!fine) {
+            best_fused = 0;
+            best = none;
+          }
         }
       }
     }
@@ -1373,10 +1429,8 @@ int PoissonEngine::tune_level(std::vector<LevelBuf>& lv, int l, SolveState* tst)
     std::lock_guard<std::mutex> g(g_tune_mutex);
     TunePick pk{best_fused, best.tx, best.ty};
     pk.stream = best_stream;
-    if (cfg_.stream == 0 || !fine) g_tune_cache[key] = pk;
+    if (cfg_.stream == 0) g_tune_cache[key] = pk;
   }
-  if (fine && cfg_.stream > 0 && stream_supported(op_, nu, L.fx, L.fy))
-    best_stream = cfg_.stream;
   return apply(best_fused, best, best_stream);
 }
 
@@ -1759,25 +1813,25 @@ static ClassLevel<T> class_level(const LevelBuf& L, int px, int py) {
 template <typename T, int OP>
 static void cls_rows(hipStream_t s, int lanes, const ClassLevel<T>& L, T w, const T* b,
                      const T* xin, const T* xc, const int* agg, T* o1, T* o2, T* o3,
-                     const SolveState* st) {
+                     const SolveState* st, const LaunchClock& clk = LaunchClock{}) {
   const long threads = (long)L.n * lanes;
   const dim3 grid((unsigned)((threads + kBlock - 1) / kBlock));
   switch (lanes) {
     case 1:
-      hipLaunchKernelGGL((k_cls_rows<T, 1, OP>), grid, dim3(kBlock), 0, s, L, w, b, xin, xc,
-                         agg, o1, o2, o3, st);
+      PCFD_LAUNCH_TIMED(clk, (k_cls_rows<T, 1, OP>), grid, dim3(kBlock), 0, s, L, w, b, xin, xc,
+                        agg, o1, o2, o3, st);
       break;
     case 4:
-      hipLaunchKernelGGL((k_cls_rows<T, 4, OP>), grid, dim3(kBlock), 0, s, L, w, b, xin, xc,
-                         agg, o1, o2, o3, st);
+      PCFD_LAUNCH_TIMED(clk, (k_cls_rows<T, 4, OP>), grid, dim3(kBlock), 0, s, L, w, b, xin, xc,
+                        agg, o1, o2, o3, st);
       break;
     case 16:
-      hipLaunchKernelGGL((k_cls_rows<T, 16, OP>), grid, dim3(kBlock), 0, s, L, w, b, xin, xc,
-                         agg, o1, o2, o3, st);
+      PCFD_LAUNCH_TIMED(clk, (k_cls_rows<T, 16, OP>), grid, dim3(kBlock), 0, s, L, w, b, xin, xc,
+                        agg, o1, o2, o3, st);
       break;
     default:
-      hipLaunchKernelGGL((k_cls_rows<T, 64, OP>), grid, dim3(kBlock), 0, s, L, w, b, xin, xc,
-                         agg, o1, o2, o3, st);
+      PCFD_LAUNCH_TIMED(clk, (k_cls_rows<T, 64, OP>), grid, dim3(kBlock), 0, s, L, w, b, xin, xc,
+                        agg, o1, o2, o3, st);
       break;
   }
 }
@@ -1801,27 +1855,42 @@ static ClassRows<T> class_rows(const LevelBuf& L, int px, int py) {
 }
 
 template <typename T>
+static BoxRows<T> box_rows(const LevelBuf& L, int px, int py) {
+  BoxRows<T> D;
+  D.nx = L.nx;
+  D.ny = L.ny;
+  D.px = px;
+  D.py = py;
+  D.cls = L.cls;
+  D.box = (const T*)L.box;
+  D.cdinv = (const T*)L.box_dinv;
+  D.cactive = L.box_act;
+  D.zero_class = L.ncls;
+  return D;
+}
+
+template <typename T>
 static void cls_restrict(hipStream_t s, int lanes, const ClassLevel<T>& L, T w, const T* r,
                          const T* t, int fx, int fy, int cnx, int cny, T* bc,
-                         const SolveState* st) {
+                         const SolveState* st, const LaunchClock& clk = LaunchClock{}) {
   const long cells = (long)cnx * cny;
   auto grid = [&](int per) { return dim3((unsigned)((cells * per + kBlock - 1) / kBlock)); };
   switch (lanes) {
     case 1:
-      hipLaunchKernelGGL((k_cls_restrict<T, 1>), grid(4), dim3(kBlock), 0, s, L, w, r, t, fx, fy,
-                         cnx, cny, bc, st);
+      PCFD_LAUNCH_TIMED(clk, (k_cls_restrict<T, 1>), grid(4), dim3(kBlock), 0, s, L, w, r, t, fx,
+                        fy, cnx, cny, bc, st);
       break;
     case 4:
-      hipLaunchKernelGGL((k_cls_restrict<T, 4>), grid(16), dim3(kBlock), 0, s, L, w, r, t, fx, fy,
-                         cnx, cny, bc, st);
+      PCFD_LAUNCH_TIMED(clk, (k_cls_restrict<T, 4>), grid(16), dim3(kBlock), 0, s, L, w, r, t, fx,
+                        fy, cnx, cny, bc, st);
       break;
     case 16:
-      hipLaunchKernelGGL((k_cls_restrict<T, 16>), grid(64), dim3(kBlock), 0, s, L, w, r, t, fx,
-                         fy, cnx, cny, bc, st);
+      PCFD_LAUNCH_TIMED(clk, (k_cls_restrict<T, 16>), grid(64), dim3(kBlock), 0, s, L, w, r, t,
+                        fx, fy, cnx, cny, bc, st);
       break;
     default:
-      hipLaunchKernelGGL((k_cls_restrict_serial<T, 64>), grid(64), dim3(kBlock), 0, s, L, w, r, t,
-                         fx, fy, cnx, cny, bc, st);
+      PCFD_LAUNCH_TIMED(clk, (k_cls_restrict_serial<T, 64>), grid(64), dim3(kBlock), 0, s, L, w, r,
+                        t, fx, fy, cnx, cny, bc, st);
       break;
   }
 }
@@ -1830,49 +1899,74 @@ static void cls_restrict(hipStream_t s, int lanes, const ClassLevel<T>& L, T w, 
 // iterate into L.x, restricted residual into C.b.  Fused when the level's
 // tile plan says so, else pass by pass.
 template <typename T>
-int PoissonEngine::level_down(LevelBuf& L, LevelBuf& C, const SolveState* cst) {
+int PoissonEngine::level_down(LevelBuf& L, LevelBuf& C, const SolveState* cst, int lev) {
   const int nu = cfg_.sweeps;
   const T w = (T)L.weight;
+  // algorithmic bytes for the clock: vectors at T, the class index, the
+  // coarse vector; the L2-resident row dictionary counts zero
+  const double n = (double)L.n, nc = (double)C.n, ts = (double)sizeof(T);
+  const double cb = L.cls32 ? 4.0 : 2.0;
+  if (L.box_rows) {
+    return launch_class_down_stream<T>(s_, box_rows<T>(L, op_.px, op_.py), L.box_radius,
+                                       L.box_rows, w, (const T*)L.b, (T*)L.x, (T*)C.b, L.fx,
+                                       L.fy, C.nx, C.ny, cst,
+                                       clock_next(CK_CLASS_DOWN, lev, n * (2 * ts + cb) + nc * ts));
+  }
   if (L.fused) {
     const ClassRows<T> D = class_rows<T>(L, op_.px, op_.py);
     return launch_class_down<T>(s_, D, L.down0, L.rxs, L.rys, w, nu, (const T*)L.b, (T*)L.x,
-                                (T*)C.b, L.fx, L.fy, C.nx, C.ny, cst);
+                                (T*)C.b, L.fx, L.fy, C.nx, C.ny, cst,
+                                clock_next(CK_CLASS_DOWN, lev, n * (2 * ts + cb) + nc * ts));
   }
   const ClassLevel<T> D = class_level<T>(L, op_.px, op_.py);
   cls_rows<T, RO_SMOOTH0_RES>(s_, L.lanes, D, w, (const T*)L.b, nullptr, nullptr, nullptr,
-                              (T*)L.x, (T*)L.r, (T*)L.t, cst);
+                              (T*)L.x, (T*)L.r, (T*)L.t, cst,
+                              clock_next(CK_ROWS_SMOOTH0, lev, n * (4 * ts + cb)));
   if (nu > 1) {
     for (int sw = 1; sw < nu; ++sw) {
       cls_rows<T, RO_SMOOTH>(s_, L.lanes, D, w, (const T*)L.b, (const T*)L.x, nullptr, nullptr,
-                             (T*)L.y, nullptr, nullptr, cst);
+                             (T*)L.y, nullptr, nullptr, cst,
+                             clock_next(CK_ROWS_SMOOTH, lev, n * (3 * ts + cb)));
       std::swap(L.x, L.y);
     }
     cls_rows<T, RO_RES>(s_, L.lanes, D, w, (const T*)L.b, (const T*)L.x, nullptr, nullptr,
-                        nullptr, (T*)L.r, (T*)L.t, cst);
+                        nullptr, (T*)L.r, (T*)L.t, cst,
+                        clock_next(CK_ROWS_RES, lev, n * (4 * ts + cb)));
   }
   // b_c = P^T r in one pass over the coarse cells
   cls_restrict<T>(s_, L.lanes, D, w, (const T*)L.r, (const T*)L.t, L.fx, L.fy, C.nx, C.ny,
-                  (T*)C.b, cst);
+                  (T*)C.b, cst, clock_next(CK_RESTRICT, lev, n * (2 * ts + cb) + nc * ts));
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
 
 // Up leg of a stored level: L.y = S^nu (L.x + P C.y).
 template <typename T>
-int PoissonEngine::level_up(LevelBuf& L, LevelBuf& C, const SolveState* cst) {
+int PoissonEngine::level_up(LevelBuf& L, LevelBuf& C, const SolveState* cst, int lev) {
   const int nu = cfg_.sweeps;
   const T w = (T)L.weight;
+  const double n = (double)L.n, nc = (double)C.n, ts = (double)sizeof(T);
+  const double cb = L.cls32 ? 4.0 : 2.0;
+  if (L.box_rows) {
+    return launch_class_up_stream<T>(s_, box_rows<T>(L, op_.px, op_.py), L.box_radius,
+                                     L.box_rows, w, (const T*)L.x, (const T*)L.b, (const T*)C.y,
+                                     L.fx, L.fy, C.ny, (T*)L.y, cst,
+                                     clock_next(CK_CLASS_UP, lev, n * (3 * ts + cb) + nc * ts));
+  }
   if (L.fused) {
     const ClassRows<T> D = class_rows<T>(L, op_.px, op_.py);
     return launch_class_up<T>(s_, D, L.up, L.rxs, L.rys, w, nu, (const T*)L.x, (const T*)L.b,
-                              (const T*)C.y, L.fx, L.fy, C.ny, (T*)L.y, cst);
+                              (const T*)C.y, L.fx, L.fy, C.ny, (T*)L.y, cst,
+                              clock_next(CK_CLASS_UP, lev, n * (3 * ts + cb) + nc * ts));
   }
   const ClassLevel<T> D = class_level<T>(L, op_.px, op_.py);
   cls_rows<T, RO_PROLONG>(s_, L.lanes, D, w, nullptr, nullptr, (const T*)C.y, L.agg, (T*)L.x,
-                          nullptr, nullptr, cst);
+                          nullptr, nullptr, cst,
+                          clock_next(CK_ROWS_PROLONG, lev, n * (2 * ts + 4 + cb) + nc * ts));
   for (int sw = 0; sw < nu; ++sw) {
     cls_rows<T, RO_SMOOTH>(s_, L.lanes, D, w, (const T*)L.b, (const T*)L.x, nullptr, nullptr,
-                           (T*)L.y, nullptr, nullptr, cst);
+                           (T*)L.y, nullptr, nullptr, cst,
+                           clock_next(CK_ROWS_SMOOTH, lev, n * (3 * ts + cb)));
     std::swap(L.x, L.y);
   }
   std::swap(L.x, L.y);  // the corrected iterate is named y
@@ -1890,17 +1984,21 @@ int PoissonEngine::coarse_legs(std::vector<LevelBuf>& lv, int first, const Solve
   const int cf = (cycle_precision && chain_first_ >= first && chain_first_ < nl) ? chain_first_
                                                                                 : nl - 1;
   for (int l = first; l < cf; ++l)
-    if (int rc = level_down<T>(lv[l], lv[l + 1], cst)) return rc;
+    if (int rc = level_down<T>(lv[l], lv[l + 1], cst, l)) return rc;
   if (cf < nl - 1) {
     if (int rc = launch_chain<T>(lv, cf, cst)) return rc;
   } else {
     for (int l = cf; l < nl - 1; ++l)
-      if (int rc = level_down<T>(lv[l], lv[l + 1], cst)) return rc;
+      if (int rc = level_down<T>(lv[l], lv[l + 1], cst, l)) return rc;
     LevelBuf& L = lv[nl - 1];
-    if (int rc = launch_dense_apply<T>(s_, L.cinv, L.n, (const T*)L.b, (T*)L.y, cst)) return rc;
+    const double n = (double)L.n;
+    if (int rc = launch_dense_apply<T>(s_, L.cinv, L.n, (const T*)L.b, (T*)L.y, cst,
+                                       clock_next(CK_DENSE, nl - 1,
+                                                  n * n * 8 + 2 * n * sizeof(T))))
+      return rc;
   }
   for (int l = cf - 1; l >= first; --l)
-    if (int rc = level_up<T>(lv[l], lv[l + 1], cst)) return rc;
+    if (int rc = level_up<T>(lv[l], lv[l + 1], cst, l)) return rc;
   return kOk;
 }
 
@@ -1987,16 +2085,17 @@ int PoissonEngine::enqueue_vcycle_t(int mode, int reduce) {
     if (int rc = launch_fine_down_stream<T>(s_, op_, L0.stream_rows, (T)L0.weight,
                                             (const double*)r_, (T*)L0.b, (T*)L0.t, (T*)C.b,
                                             L0.fx, L0.fy, C.nx, C.ny, cst,
-                                            clock_next(2, bytes_down)))
+                                            clock_next(CK_FINE_DOWN, 0, bytes_down)))
       return rc;
     if (int rc = coarse_legs<T>(lv, 1, cst)) return rc;
     const double bytes_up = n0 * (3 * ts + 1) + C.n * ts;
     if (int rc = launch_fine_up_stream<T>(s_, op_, L0.stream_rows, (T)L0.weight,
                                           (const T*)L0.t, (const T*)L0.b, (const T*)C.y, L0.fx,
                                           L0.fy, C.ny, out, reduce, partial_, cst,
-                                          clock_next(3, bytes_up)))
+                                          clock_next(CK_FINE_UP, 0, bytes_up)))
       return rc;
-    return fold(s_, partial_, L0.stream_nb, EpiCycle{st_, reduce, n_proj_});
+    return fold(s_, partial_, L0.stream_nb, EpiCycle{st_, reduce, n_proj_},
+                clock_next(CK_FOLD, 0, L0.stream_nb * 24.0));
   }
   {
     LevelBuf& C = lv[1];
@@ -2005,7 +2104,7 @@ int PoissonEngine::enqueue_vcycle_t(int mode, int reduce) {
         s_, op_, krylov ? L0.down0 : L0.down1, (T)L0.weight, nu, krylov ? 0 : 1,
         krylov ? (const double*)r_ : nullptr, krylov ? nullptr : (const T*)L0.b,
         krylov ? nullptr : (const T*)L0.x, krylov ? 0 : 1, krylov ? (T*)L0.b : nullptr,
-        (T*)L0.t, (T*)C.b, L0.fx, L0.fy, C.nx, C.ny, cst, clock_next(2, bytes));
+        (T*)L0.t, (T*)C.b, L0.fx, L0.fy, C.nx, C.ny, cst, clock_next(CK_FINE_DOWN, 0, bytes));
     if (rc) return rc;
   }
   if (int rc = coarse_legs<T>(lv, 1, cst)) return rc;
@@ -2014,10 +2113,11 @@ int PoissonEngine::enqueue_vcycle_t(int mode, int reduce) {
     const double bytes = n0 * (3 * ts + 1) + C.n * ts;  // x_pre, b0, flags, z; x_c
     if (int rc = launch_fine_up<T>(s_, op_, L0.up, (T)L0.weight, nu, (const T*)L0.t,
                                    (const T*)L0.b, (const T*)C.y, L0.fx, L0.fy, C.ny, out,
-                                   reduce, partial_, cst, clock_next(3, bytes)))
+                                   reduce, partial_, cst, clock_next(CK_FINE_UP, 0, bytes)))
       return rc;
   }
-  return fold(s_, partial_, L0.up.ntiles(), EpiCycle{st_, reduce, n_proj_});
+  return fold(s_, partial_, L0.up.ntiles(), EpiCycle{st_, reduce, n_proj_},
+              clock_next(CK_FOLD, 0, L0.up.ntiles() * 24.0));
 }
 
 int PoissonEngine::enqueue_vcycle(int mode, int reduce) {
@@ -2032,26 +2132,29 @@ int PoissonEngine::run_mgcg() {
   double* p_old = pa_;
   double* p_new = pb_;
   const bool f32 = cfg_.precision == 1;
-  size_t mark[kClockKernels];
-  clock_mark(mark);
+  const int it0 = clock_iter_;
   PCFD_ITERATE(1, cfg_.maxiter, {
+    ++clock_iter_;
     if (int rc = enqueue_vcycle(VC_KRYLOV, SR_MGCG)) return rc;
     // direction: z (cycle precision), p_old, flags in; p_new, q out
-    const LaunchClock cd = clock_next(1, (double)n_ * ((f32 ? 4 : 8) + 3 * 8 + 1));
+    const LaunchClock cd = clock_next(CK_DIR, 0, (double)n_ * ((f32 ? 4 : 8) + 3 * 8 + 1));
     const int rc = f32 ? launch_fine_dir<float>(s_, op_, dir_tile_, (const float*)lv32_[0].y,
                                                 p_old, p_new, q_, partial_, st_, cd)
                        : launch_fine_dir<double>(s_, op_, dir_tile_, (const double*)lv64_[0].y,
                                                  p_old, p_new, q_, partial_, st_, cd);
     if (rc) return rc;
-    if (int rc2 = fold(s_, partial_, dir_tile_.ntiles(), EpiAlpha{st_})) return rc2;
-    const LaunchClock cu = clock_next(0, (double)n_ * (6 * 8 + 1));  // x, p, r, q, flags; x, r
+    if (int rc2 = fold(s_, partial_, dir_tile_.ntiles(), EpiAlpha{st_},
+                       clock_next(CK_FOLD, 1, dir_tile_.ntiles() * 8.0)))
+      return rc2;
+    const LaunchClock cu = clock_next(CK_UPDATE, 0, (double)n_ * (6 * 8 + 1));  // x, p, r, q, flags; x, r
     PCFD_LAUNCH_TIMED(cu, k_cg_update, g, dim3(kBlock), 0, s_, op_, x_, r_, p_new, q_,
                       (int)IM_MGCG, partial_, (const SolveState*)st_);
-    if (int rc = fold(s_, partial_, nt, EpiUpdate{st_, (int)IM_MGCG, cfg_.maxiter, n_proj_}))
+    if (int rc = fold(s_, partial_, nt, EpiUpdate{st_, (int)IM_MGCG, cfg_.maxiter, n_proj_},
+                      clock_next(CK_FOLD, 2, nt * 16.0)))
       return rc;
     std::swap(p_old, p_new);
   });
-  clock_trim(mark, predicted_);
+  clock_trim(it0, predicted_);
   return kOk;
 }
 
@@ -2064,9 +2167,9 @@ int PoissonEngine::run_multigrid() {
   const size_t bytes = (size_t)n_ * sizeof(double);
   PCFD_HIP_CHECK(hipMemcpyAsync(L0.b, b_, bytes, hipMemcpyDeviceToDevice, s_));
   PCFD_HIP_CHECK(hipMemcpyAsync(L0.x, x_, bytes, hipMemcpyDeviceToDevice, s_));
-  size_t mark[kClockKernels];
-  clock_mark(mark);
+  const int it0 = clock_iter_;
   PCFD_ITERATE(1, cfg_.maxiter, {
+    ++clock_iter_;
     if (int rc = enqueue_vcycle_t<double>(VC_STANDALONE, SR_MG)) return rc;
     hipLaunchKernelGGL(k_check, g, dim3(kBlock), 0, s_, op_, (const double*)L0.x, b_, (int)CM_MG,
                        partial_, (const SolveState*)st_);
@@ -2074,7 +2177,7 @@ int PoissonEngine::run_multigrid() {
                       EpiCheck{st_, (int)CM_MG, cfg_.tol, cfg_.maxiter, singular_, n_fluid_}))
       return rc;
   });
-  clock_trim(mark, predicted_);
+  clock_trim(it0, predicted_);
   // The pending mean shift is applied by the final residual check and the scatter.
   PCFD_HIP_CHECK(hipMemcpyAsync(x_, L0.x, bytes, hipMemcpyDeviceToDevice, s_));
   return kOk;

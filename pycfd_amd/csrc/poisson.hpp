@@ -155,6 +155,13 @@ struct LevelBuf {
   // Level 0 only: rows per wavefront of the streaming Krylov-start legs (0:
   // the tiled kernels run them) and the up leg's workgroup count.
   int stream_rows = 0, stream_nb = 0;
+  // Stored levels of radius <= 3: the box dictionary of the streaming legs
+  // (classes + one zero class) and the rows per wavefront when they run the
+  // level (0: fused tiles or pass by pass).
+  void* box = nullptr;
+  void* box_dinv = nullptr;
+  uint8_t* box_act = nullptr;
+  int box_radius = 0, box_rows = 0;
   int* ent_off = nullptr;
   std::vector<int8_t> hdi, hdj;  // host copy of the entry offsets (tile planning)
   int hot = -1, hot_n = 0;       // most frequent class and its entry count
@@ -183,24 +190,38 @@ class PoissonEngine {
   // buffers with HIP events; reports the mean duration and the kernel's
   // algorithmic bytes per launch (arrays read + written once, at stored size).
   int time_kernel(int which, int reps, double* avg_ms, double* bytes);
-  // Live clock: every launch of kernels 0-3 (time_kernel's numbering) inside
-  // the solves that follow records start/stop events, up to `max_launches`
-  // per kernel; clock_read stops it and reports launches, mean duration and
-  // algorithmic bytes per launch for each kernel.
+  // Live clock: every kernel launch of the solves that follow records
+  // start/stop events (up to `max_launches` launches in all), filed under a
+  // slot = kind * kClockLevels + level (ClockKind); clock_read stops it and
+  // reports per slot the launches, mean duration and algorithmic bytes per
+  // launch.  Iterations enqueued past convergence (no-ops) are dropped.
+  enum ClockKind : int {
+    CK_UPDATE = 0, CK_DIR, CK_FINE_DOWN, CK_FINE_UP, CK_CLASS_DOWN, CK_CLASS_UP,
+    CK_ROWS_SMOOTH0, CK_ROWS_SMOOTH, CK_ROWS_RES, CK_ROWS_PROLONG, CK_RESTRICT, CK_DENSE,
+    CK_CHAIN, CK_FOLD, kClockKinds
+  };
+  static constexpr int kClockLevels = 16, kClockSlots = kClockKinds * kClockLevels;
   int clock_start(int max_launches);
   int clock_read(int32_t* launches, double* avg_ms, double* bytes);
-  static constexpr int kClockKernels = 4;
   // Per level: nx, ny, fused, tile x, tile y (see pcfd_mg_plan).
   int plan(int32_t* out, int cap) const {
     const auto& lv = (cfg_.precision == 1) ? lv32_ : lv64_;
     int k = 0, l = 0;
     for (const LevelBuf& L : lv) {
       int mode = (chain_first_ >= 0 && l >= chain_first_) ? 2 : L.fused;
-      if (l == 0 && L.stream_rows) mode = 3;
+      int sx = 0, sy = 0;  // streaming: rows per wavefront x output columns
+      if (l == 0 && L.stream_rows) {
+        mode = 3;
+        sx = L.stream_rows;
+        sy = 60;
+      } else if (mode != 2 && L.box_rows) {
+        mode = 3;
+        sx = L.box_rows;
+        sy = 64 - 4 * L.box_radius;
+      }
       ++l;
-      const int v[5] = {L.nx, L.ny, mode,
-                        mode == 1 ? L.down0.tx : (mode == 3 ? L.stream_rows : 0),
-                        mode == 1 ? L.down0.ty : (mode == 3 ? 60 : 0)};
+      const int v[5] = {L.nx, L.ny, mode, mode == 1 ? L.down0.tx : sx,
+                        mode == 1 ? L.down0.ty : sy};
       for (int q = 0; q < 5; ++q)
         if (k < cap) out[k++] = v[q];
     }
@@ -232,9 +253,9 @@ class PoissonEngine {
   template <typename T>
   int plan_level_tiles(std::vector<LevelBuf>& lv);
   template <typename T>
-  int level_down(LevelBuf& L, LevelBuf& C, const SolveState* st);
+  int level_down(LevelBuf& L, LevelBuf& C, const SolveState* st, int lev = 0);
   template <typename T>
-  int level_up(LevelBuf& L, LevelBuf& C, const SolveState* st);
+  int level_up(LevelBuf& L, LevelBuf& C, const SolveState* st, int lev = 0);
   template <typename T>
   int tune_level(std::vector<LevelBuf>& lv, int l, SolveState* tst);
   int ensure_partials(size_t n);
@@ -258,23 +279,23 @@ class PoissonEngine {
   SolveState* st_host_ = nullptr;  // pinned mirror for polling
   hipEvent_t ev_ = nullptr;
   // live kernel clock
-  struct KernelClock {
-    std::vector<hipEvent_t> ev;  // start/stop pairs
-    size_t used = 0;
-    double bytes = 0.0;
+  struct ClockEntry {
+    int slot, iter;
+    hipEvent_t a, b;
+    double bytes;
   };
   bool clock_on_ = false;
-  KernelClock clock_[kClockKernels];
-  LaunchClock clock_next(int which, double bytes);
+  std::vector<hipEvent_t> clock_pool_;
+  size_t clock_pool_used_ = 0;
+  std::vector<ClockEntry> clock_log_;
+  int clock_iter_ = 0;  // iteration tag of the launches being enqueued
+  LaunchClock clock_next(int kind, int level, double bytes);
   void clock_free();
-  // Iterations enqueued past convergence run as no-ops; keep only the first
-  // `iters` launches of each kernel since `mark` (one launch per iteration).
-  void clock_mark(size_t* mark) const {
-    for (int w = 0; w < kClockKernels; ++w) mark[w] = clock_[w].used;
-  }
-  void clock_trim(const size_t* mark, int iters) {
-    for (int w = 0; w < kClockKernels; ++w)
-      if (clock_[w].used > mark[w] + (size_t)iters) clock_[w].used = mark[w] + (size_t)iters;
+  // Launches of iterations past the solve's `iters` (tagged after `it0`) ran
+  // as no-ops: drop them.
+  void clock_trim(int it0, int iters) {
+    for (ClockEntry& e : clock_log_)
+      if (e.iter > it0 + iters) e.slot = -1;
   }
   double *b_ = nullptr, *x_ = nullptr, *r_ = nullptr, *pa_ = nullptr, *pb_ = nullptr,
          *q_ = nullptr, *z_ = nullptr;

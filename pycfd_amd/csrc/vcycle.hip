@@ -1062,6 +1062,229 @@ int launch_fine_up(hipStream_t s, const FineOp& A, const FusedTile& ft, T w, int
   return kOk;
 }
 
+// ---------------------------------------------------------------------------
+// Stored levels, streaming form.  The same walk as level 0's streaming legs
+// with a (2R+1)-row register window and lane shifts up to R: one sweep,
+// zero start, rows from the box dictionary (a cell's class selects a dense
+// (2R+1)^2 row; cells outside the grid take the all-zero class, so every
+// value outside stays zero without tests).  Valid lanes shrink by R per
+// operator application, hence 2R halo lanes and rows each side.
+// ---------------------------------------------------------------------------
+template <typename T, int R>
+__device__ __forceinline__ T box_apply(const BoxRows<T>& L, int c, const T (&win)[2 * R + 1],
+                                       int lane) {
+  constexpr int W = 2 * R + 1;
+  const T* row = L.box + (size_t)c * (W * W);
+  T acc = (T)0;
+#pragma unroll
+  for (int a = 0; a < W; ++a) {
+#pragma unroll
+    for (int d = -R; d <= R; ++d) {
+      const T v = d == 0 ? win[a] : __shfl(win[a], lane + d, kWave);
+      acc += row[a * W + d + R] * v;
+    }
+  }
+  return acc;
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void win_push(T (&w)[N], T v) {
+#pragma unroll
+  for (int k = 0; k + 1 < N; ++k) w[k] = w[k + 1];
+  w[N - 1] = v;
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(kBlock)
+k_class_down_stream(BoxRows<T> L, int strips, int rows, T w, const T* __restrict__ b,
+                    T* __restrict__ x_out, T* __restrict__ bc, int fx, int fy, int cnx, int cny,
+                    const SolveState* __restrict__ st) {
+  if (st->done) return;
+  constexpr int H = 2 * R, OUT = kWave - 2 * H, W = 2 * R + 1;
+  const int wave = (int)(blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave);
+  const int lane = (int)(threadIdx.x % kWave);
+  const int strip = wave % strips;
+  const int i0 = (wave / strips) * rows;
+  if (i0 >= L.nx) return;  // whole wavefront
+  const int j = strip * OUT - H + lane;
+  int jr = j;
+  bool jin = true;
+  if (L.py) jr = wrap_index(j, L.ny);
+  else jin = j >= 0 && j < L.ny;
+  const bool out_lane = lane >= H && lane < H + OUT;
+  const bool out_col = out_lane && j < L.ny;
+  T xw[W], tw[W], rw[R + 1], bw[R + 1], dw[W];
+  int cw[W];
+#pragma unroll
+  for (int k = 0; k < W; ++k) {
+    xw[k] = tw[k] = dw[k] = (T)0;
+    cw[k] = L.zero_class;
+  }
+#pragma unroll
+  for (int k = 0; k <= R; ++k) rw[k] = bw[k] = (T)0;
+  T acc = (T)0;
+  for (int p = i0 - H; p < i0 + rows + H; ++p) {
+    // stage A, row p: class, b, x = w D^-1 b
+    int ir = p;
+    bool in = jin;
+    if (L.px) ir = wrap_index(p, L.nx);
+    else in = in && p >= 0 && p < L.nx;
+    int c = L.zero_class;
+    T bv = (T)0;
+    if (in) {
+      const size_t k = (size_t)ir * L.ny + jr;
+      c = L.cls[k];
+      bv = b[k];
+    }
+    const T dinv = L.cdinv[c];
+    const T xv = w * dinv * bv;
+    if (p >= i0 && p < i0 + rows && p < L.nx && out_col) x_out[(size_t)p * L.ny + j] = xv;
+    win_push(xw, xv);
+    win_push(cw, c);
+    win_push(dw, dinv);
+    win_push(bw, bv);
+    // stage B, row p - R: res = b - A x, t = D^-1 res
+    const int cq = cw[R];
+    const T res = bw[0] - box_apply<T, R>(L, cq, xw, lane);
+    win_push(rw, res);
+    win_push(tw, dw[R] * res);
+    // stage C, row p - 2R: y = res - w A t on active rows, restricted
+    const int s = p - H;
+    const int cs = cw[0];
+    const T at = box_apply<T, R>(L, cs, tw, lane);
+    const bool own = s >= i0 && s < L.nx && out_col && L.cactive[cs];
+    T y = own ? rw[0] - w * at : (T)0;
+    if (fy == 2) y += __shfl_down(y, 1, kWave);
+    if (s >= i0) {
+      const bool first = fx == 1 || ((s - i0) & 1) == 0;
+      const bool last = fx == 1 || ((s - i0) & 1) == 1;
+      acc = first ? y : acc + y;
+      const bool lane_ok = out_lane && (fy == 1 || ((lane - H) & 1) == 0);
+      if (last && lane_ok && j < L.ny) {
+        const int I = s / fx, J = j / fy;
+        if (I < cnx && J < cny) bc[(size_t)I * cny + J] = acc;
+      }
+    }
+  }
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(kBlock)
+k_class_up_stream(BoxRows<T> L, int strips, int rows, T w, const T* __restrict__ x_pre,
+                  const T* __restrict__ b, const T* __restrict__ xc, int fx, int fy, int cny,
+                  T* __restrict__ x_out, const SolveState* __restrict__ st) {
+  if (st->done) return;
+  constexpr int H = 2 * R, OUT = kWave - 2 * H, W = 2 * R + 1;
+  const int wave = (int)(blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave);
+  const int lane = (int)(threadIdx.x % kWave);
+  const int strip = wave % strips;
+  const int i0 = (wave / strips) * rows;
+  if (i0 >= L.nx) return;
+  const int j = strip * OUT - H + lane;
+  int jr = j;
+  bool jin = true;
+  if (L.py) jr = wrap_index(j, L.ny);
+  else jin = j >= 0 && j < L.ny;
+  const bool out_col = lane >= H && lane < H + OUT && j < L.ny;
+  T ew[W], qw[W], dw[W], bw[W], xw[R + 1];
+  int cw[W];
+#pragma unroll
+  for (int k = 0; k < W; ++k) {
+    ew[k] = qw[k] = dw[k] = bw[k] = (T)0;
+    cw[k] = L.zero_class;
+  }
+#pragma unroll
+  for (int k = 0; k <= R; ++k) xw[k] = (T)0;
+  for (int p = i0 - H; p < i0 + rows + H; ++p) {
+    // stage A, row p: class, x_pre, b, e = P_tent x_c on active rows
+    int ir = p;
+    bool in = jin;
+    if (L.px) ir = wrap_index(p, L.nx);
+    else in = in && p >= 0 && p < L.nx;
+    int c = L.zero_class;
+    T xv = (T)0, bv = (T)0, ev = (T)0;
+    if (in) {
+      const size_t k = (size_t)ir * L.ny + jr;
+      c = L.cls[k];
+      xv = x_pre[k];
+      bv = b[k];
+      ev = xc[(size_t)(ir / fx) * cny + jr / fy];  // in range for every grid cell
+    }
+    if (!L.cactive[c]) ev = (T)0;
+    win_push(ew, ev);
+    win_push(cw, c);
+    win_push(dw, L.cdinv[c]);
+    win_push(bw, bv);
+    win_push(xw, xv);
+    // stage B, row p - R: xs = x + e - w D^-1 A e
+    const T xs = xw[0] + ew[R] - w * dw[R] * box_apply<T, R>(L, cw[R], ew, lane);
+    win_push(qw, xs);
+    // stage C, row p - 2R: z = xs + w D^-1 (b - A xs)
+    const int s = p - H;
+    const T az = box_apply<T, R>(L, cw[0], qw, lane);
+    if (s >= i0 && s < L.nx && out_col)
+      x_out[(size_t)s * L.ny + j] = qw[R] + w * dw[0] * (bw[0] - az);
+  }
+}
+
+int box_strips(int ny, int radius) {
+  const int out = kWave - 4 * radius;
+  return (ny + out - 1) / out;
+}
+
+int box_blocks(int nx, int ny, int radius, int rows) {
+  const long waves = (long)box_strips(ny, radius) * ((nx + rows - 1) / rows);
+  return (int)((waves + kBlock / kWave - 1) / (kBlock / kWave));
+}
+
+#define PCFD_BOX_DISPATCH(KERNEL, ...)                                                       \
+  do {                                                                                     \
+    switch (radius) {                                                                      \
+      case 1:                                                                              \
+        PCFD_LAUNCH_TIMED(clk, (KERNEL<T, 1>), grid, dim3(kBlock), 0, s, __VA_ARGS__);     \
+        break;                                                                             \
+      case 2:                                                                              \
+        PCFD_LAUNCH_TIMED(clk, (KERNEL<T, 2>), grid, dim3(kBlock), 0, s, __VA_ARGS__);     \
+        break;                                                                             \
+      default:                                                                             \
+        PCFD_LAUNCH_TIMED(clk, (KERNEL<T, 3>), grid, dim3(kBlock), 0, s, __VA_ARGS__);     \
+        break;                                                                             \
+    }                                                                                      \
+  } while (0)
+
+template <typename T>
+int launch_class_down_stream(hipStream_t s, const BoxRows<T>& L, int radius, int rows, T w,
+                             const T* b, T* x_out, T* bc, int fx, int fy, int cnx, int cny,
+                             const SolveState* st, const LaunchClock& clk) {
+  if (radius < 1 || radius > kBoxMaxRadius || rows < 2 || rows % 2) {
+    set_last_error("streaming stored level: radius %d, %d rows (radius 1-%d, even rows)", radius,
+                   rows, kBoxMaxRadius);
+    return kErrBadArg;
+  }
+  const dim3 grid(box_blocks(L.nx, L.ny, radius, rows));
+  const int strips = box_strips(L.ny, radius);
+  PCFD_BOX_DISPATCH(k_class_down_stream, L, strips, rows, w, b, x_out, bc, fx, fy, cnx, cny, st);
+  PCFD_LAUNCH_CHECK();
+  return kOk;
+}
+
+template <typename T>
+int launch_class_up_stream(hipStream_t s, const BoxRows<T>& L, int radius, int rows, T w,
+                           const T* x_pre, const T* b, const T* xc, int fx, int fy, int cny,
+                           T* x_out, const SolveState* st, const LaunchClock& clk) {
+  if (radius < 1 || radius > kBoxMaxRadius || rows < 2 || rows % 2) {
+    set_last_error("streaming stored level: radius %d, %d rows (radius 1-%d, even rows)", radius,
+                   rows, kBoxMaxRadius);
+    return kErrBadArg;
+  }
+  const dim3 grid(box_blocks(L.nx, L.ny, radius, rows));
+  const int strips = box_strips(L.ny, radius);
+  PCFD_BOX_DISPATCH(k_class_up_stream, L, strips, rows, w, x_pre, b, xc, fx, fy, cny, x_out, st);
+  PCFD_LAUNCH_CHECK();
+  return kOk;
+}
+#undef PCFD_BOX_DISPATCH
+
 int stream_strips(int ny) { return (ny + kStreamOut - 1) / kStreamOut; }
 
 int stream_blocks(int nx, int ny, int rows) {
@@ -1120,9 +1343,9 @@ int launch_fine_up_stream(hipStream_t s, const FineOp& A, int rows, T w, const T
 template <typename T>
 int launch_class_down(hipStream_t s, const ClassRows<T>& L, const FusedTile& ft, int rxs,
                       int rys, T w, int nu, const T* b, T* x_out, T* bc, int fx, int fy, int cnx,
-                      int cny, const SolveState* st) {
-  hipLaunchKernelGGL(k_class_down<T>, dim3(ft.ntiles()), dim3(kBlock), ft.lds, s, L, ft, rxs, rys,
-                     w, nu, b, x_out, bc, fx, fy, cnx, cny, st);
+                      int cny, const SolveState* st, const LaunchClock& clk) {
+  PCFD_LAUNCH_TIMED(clk, k_class_down<T>, dim3(ft.ntiles()), dim3(kBlock), ft.lds, s, L, ft, rxs,
+                    rys, w, nu, b, x_out, bc, fx, fy, cnx, cny, st);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
@@ -1130,9 +1353,9 @@ int launch_class_down(hipStream_t s, const ClassRows<T>& L, const FusedTile& ft,
 template <typename T>
 int launch_class_up(hipStream_t s, const ClassRows<T>& L, const FusedTile& ft, int rxs, int rys,
                     T w, int nu, const T* x_pre, const T* b, const T* xc, int fx, int fy, int cny,
-                    T* x_out, const SolveState* st) {
-  hipLaunchKernelGGL(k_class_up<T>, dim3(ft.ntiles()), dim3(kBlock), ft.lds, s, L, ft, rxs, rys, w,
-                     nu, x_pre, b, xc, fx, fy, cny, x_out, st);
+                    T* x_out, const SolveState* st, const LaunchClock& clk) {
+  PCFD_LAUNCH_TIMED(clk, k_class_up<T>, dim3(ft.ntiles()), dim3(kBlock), ft.lds, s, L, ft, rxs,
+                    rys, w, nu, x_pre, b, xc, fx, fy, cny, x_out, st);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
@@ -1153,11 +1376,11 @@ int launch_fine_dir(hipStream_t s, const FineOp& A, const FusedTile& ft, const T
 
 template <typename T>
 int launch_dense_apply(hipStream_t s, const double* C, int n, const T* b, T* x,
-                       const SolveState* st) {
+                       const SolveState* st, const LaunchClock& clk) {
   constexpr int kWaves = kBlock / kWave;
   const int blocks = std::max(1, std::min((n + kWaves - 1) / kWaves, 1024));
-  hipLaunchKernelGGL(k_dense_apply<T>, dim3(blocks), dim3(kBlock), (size_t)n * sizeof(double), s,
-                     C, n, b, x, st);
+  PCFD_LAUNCH_TIMED(clk, k_dense_apply<T>, dim3(blocks), dim3(kBlock), (size_t)n * sizeof(double),
+                    s, C, n, b, x, st);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
@@ -1226,18 +1449,24 @@ bool choose_tile(int nx, int ny, int rxs, int rys, int halo_mult, int fx, int fy
                                  const SolveState*, const LaunchClock&);                      \
   template int launch_class_down<T>(hipStream_t, const ClassRows<T>&, const FusedTile&, int,   \
                                     int, T, int, const T*, T*, T*, int, int, int, int,        \
-                                    const SolveState*);                                       \
+                                    const SolveState*, const LaunchClock&);                   \
   template int launch_class_up<T>(hipStream_t, const ClassRows<T>&, const FusedTile&, int, int, \
                                   T, int, const T*, const T*, const T*, int, int, int, T*,    \
-                                  const SolveState*);                                         \
+                                  const SolveState*, const LaunchClock&);                     \
   template int launch_dense_apply<T>(hipStream_t, const double*, int, const T*, T*,            \
-                                     const SolveState*);                                      \
+                                     const SolveState*, const LaunchClock&);                  \
   template int launch_fine_down_stream<T>(hipStream_t, const FineOp&, int, T, const double*,   \
                                           T*, T*, T*, int, int, int, int, const SolveState*,  \
                                           const LaunchClock&);                                \
   template int launch_fine_up_stream<T>(hipStream_t, const FineOp&, int, T, const T*, const T*, \
                                         const T*, int, int, int, T*, int, double*,            \
                                         const SolveState*, const LaunchClock&);               \
+  template int launch_class_down_stream<T>(hipStream_t, const BoxRows<T>&, int, int, T,        \
+                                           const T*, T*, T*, int, int, int, int,              \
+                                           const SolveState*, const LaunchClock&);            \
+  template int launch_class_up_stream<T>(hipStream_t, const BoxRows<T>&, int, int, T, const T*, \
+                                         const T*, const T*, int, int, int, T*,               \
+                                         const SolveState*, const LaunchClock&);              \
   template int launch_fine_dir<T>(hipStream_t, const FineOp&, const FusedTile&, const T*,      \
                                   const double*, double*, double*, double*, const SolveState*, \
                                   const LaunchClock&);

@@ -37,6 +37,23 @@ struct ClassRows {
 // Largest row the fused class kernels stage in LDS as the level's hot class.
 constexpr int kHotMax = 256;
 
+// A stored level in streaming form: the row dictionary with every class's
+// row laid out as a dense (2R+1) x (2R+1) box around the diagonal (zeros
+// where the row has no entry), plus one all-zero class `zero_class` for
+// cells outside the grid.
+template <typename T>
+struct BoxRows {
+  int nx, ny, px, py;
+  const uint16_t* cls;
+  const T* box;
+  const T* cdinv;  // zero_class + 1 entries
+  const uint8_t* cactive;
+  int zero_class;
+};
+
+// Largest stencil radius the streaming stored-level legs take.
+constexpr int kBoxMaxRadius = 3;
+
 // Level-0 (matrix-free 5-point) down leg.  Right-hand side either from the
 // Krylov residual r (b0 = r - mean_r on solvable cells, written to b_out) or
 // given in b_in; iterate from zero, or from x_in minus a pending mean shift.
@@ -70,15 +87,29 @@ int launch_fine_up_stream(hipStream_t s, const FineOp& A, int rows, T w, const T
                           double* partial, const SolveState* st,
                           const LaunchClock& clk = LaunchClock{});
 
+// Stored-level legs in streaming form (one sweep each way, zero start), for
+// stencils of radius R <= kBoxMaxRadius: one wavefront per strip of
+// 64 - 4R output columns and `rows` rows.
+int box_strips(int ny, int radius);
+int box_blocks(int nx, int ny, int radius, int rows);
+template <typename T>
+int launch_class_down_stream(hipStream_t s, const BoxRows<T>& L, int radius, int rows, T w,
+                             const T* b, T* x_out, T* bc, int fx, int fy, int cnx, int cny,
+                             const SolveState* st, const LaunchClock& clk = LaunchClock{});
+template <typename T>
+int launch_class_up_stream(hipStream_t s, const BoxRows<T>& L, int radius, int rows, T w,
+                           const T* x_pre, const T* b, const T* xc, int fx, int fy, int cny,
+                           T* x_out, const SolveState* st, const LaunchClock& clk = LaunchClock{});
+
 template <typename T>
 int launch_class_down(hipStream_t s, const ClassRows<T>& L, const FusedTile& ft, int rxs,
                       int rys, T w, int nu, const T* b, T* x_out, T* bc, int fx, int fy, int cnx,
-                      int cny, const SolveState* st);
+                      int cny, const SolveState* st, const LaunchClock& clk = LaunchClock{});
 
 template <typename T>
 int launch_class_up(hipStream_t s, const ClassRows<T>& L, const FusedTile& ft, int rxs, int rys,
                     T w, int nu, const T* x_pre, const T* b, const T* xc, int fx, int fy, int cny,
-                    T* x_out, const SolveState* st);
+                    T* x_out, const SolveState* st, const LaunchClock& clk = LaunchClock{});
 
 // Multigrid-CG direction and operator product on level 0 (halo-1 tiling):
 // p_new = (z - mean_z) + beta p_old, q = A p_new, per-tile partials of p.q.
@@ -90,7 +121,7 @@ int launch_fine_dir(hipStream_t s, const FineOp& A, const FusedTile& ft, const T
 // x = C b with C an n x n dense FP64 matrix, spread over many workgroups.
 template <typename T>
 int launch_dense_apply(hipStream_t s, const double* C, int n, const T* b, T* x,
-                       const SolveState* st);
+                       const SolveState* st, const LaunchClock& clk = LaunchClock{});
 
 // Tiles (power-of-two sides, multiples of the aggregation factors) whose LDS
 // region fits `lds_cap` bytes; `force_*` > 0 pins a side.
