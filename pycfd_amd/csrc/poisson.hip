@@ -529,11 +529,6 @@ enum RowOp : int {
 };
 
 template <typename T>
-__device__ __forceinline__ int cls_of(const ClassLevel<T>& L, int row) {
-  return L.cls32 ? (int)L.cls32[row] : (int)L.cls[row];
-}
-
-template <typename T>
 __device__ __forceinline__ int cls_nb(const ClassLevel<T>& L, int row, int e) {
   if (!(L.px | L.py)) return row + L.ent_lin[e];
   const int i = row / L.ny, j = row - i * L.ny;
@@ -543,45 +538,56 @@ __device__ __forceinline__ int cls_nb(const ClassLevel<T>& L, int row, int e) {
   return ii * L.ny + jj;
 }
 
-// One row operation at global lane gid (LPR lanes per row).
+// One row operation at global lane gid (LPR lanes per row).  The row's entry
+// range, 1/diag and active flag come from per-cell arrays, so the only
+// dependent loads are entry -> neighbour value.
 template <typename T, int LPR, int OP>
 __device__ __forceinline__ void cls_rows_at(const ClassLevel<T>& L, T w, const T* __restrict__ b,
                                             const T* __restrict__ xin, const T* __restrict__ xc,
                                             const int* __restrict__ agg, T* __restrict__ o1,
                                             T* __restrict__ o2, T* __restrict__ o3, int gid) {
+  (void)agg;
   const int row = gid / LPR;
   const int lane = gid % LPR;
   const bool valid = row < L.n;
   T acc = (T)0;
-  int c = 0;
+  const int i = valid ? row / L.ny : 0, j = valid ? row - i * L.ny : 0;
+  // P_tent x_c at cell (qi, qj): its aggregate's value on active cells
+  auto prolong_at = [&](int qi, int qj) -> T {
+    const T ev = xc[(qi / L.fx) * L.cny + qj / L.fy];
+    return L.ract[qi * L.ny + qj] ? ev : (T)0;
+  };
   if (valid) {
-    c = cls_of(L, row);
-    const int e1 = L.row_ptr[c + 1];
-    auto xval = [&](int q) -> T {
-      if (OP == RO_SMOOTH0_RES) return w * L.cdinv[cls_of(L, q)] * b[q];
-      if (OP == RO_PROLONG) return (agg[q] >= 0) ? xc[agg[q]] : (T)0;
+    const int2 rr = L.rr[row];
+    auto xval = [&](int e) -> T {
+      if (OP == RO_PROLONG) {
+        int qi = i + L.ent_di[e], qj = j + L.ent_dj[e];
+        if (L.px) qi = (qi % L.nx + L.nx) % L.nx;
+        if (L.py) qj = (qj % L.ny + L.ny) % L.ny;
+        return prolong_at(qi, qj);
+      }
+      const int q = cls_nb(L, row, e);
+      if (OP == RO_SMOOTH0_RES) return w * L.rdinv[q] * b[q];
       return xin[q];
     };
-    int e = L.row_ptr[c] + lane;
+    int e = rr.x + lane;
     // Four entries per trip with their loads issued together; the sum keeps
     // the entry order of the plain loop.
-    for (; e + 3 * LPR < e1; e += 4 * LPR) {
-      const int q0 = cls_nb(L, row, e), q1 = cls_nb(L, row, e + LPR);
-      const int q2 = cls_nb(L, row, e + 2 * LPR), q3 = cls_nb(L, row, e + 3 * LPR);
+    for (; e + 3 * LPR < rr.y; e += 4 * LPR) {
       const T a0 = L.ent_val[e], a1 = L.ent_val[e + LPR];
       const T a2 = L.ent_val[e + 2 * LPR], a3 = L.ent_val[e + 3 * LPR];
-      const T x0 = xval(q0), x1 = xval(q1), x2 = xval(q2), x3 = xval(q3);
+      const T x0 = xval(e), x1 = xval(e + LPR), x2 = xval(e + 2 * LPR), x3 = xval(e + 3 * LPR);
       acc += a0 * x0;
       acc += a1 * x1;
       acc += a2 * x2;
       acc += a3 * x3;
     }
-    for (; e < e1; e += LPR) acc += L.ent_val[e] * xval(cls_nb(L, row, e));
+    for (; e < rr.y; e += LPR) acc += L.ent_val[e] * xval(e);
   }
 #pragma unroll
   for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
   if (!valid || lane != 0) return;
-  const T dinv = L.cdinv[c];
+  const T dinv = L.rdinv[row];
   if (OP == RO_SMOOTH0_RES) {
     const T bk = b[row];
     const T rk = bk - acc;
@@ -595,8 +601,7 @@ __device__ __forceinline__ void cls_rows_at(const ClassLevel<T>& L, T w, const T
     o2[row] = rk;
     o3[row] = dinv * rk;
   } else {  // RO_PROLONG
-    const T e = (agg[row] >= 0) ? xc[agg[row]] : (T)0;
-    o1[row] = o1[row] + e - w * dinv * acc;
+    o1[row] = o1[row] + prolong_at(i, j) - w * dinv * acc;
   }
 }
 
@@ -630,12 +635,12 @@ __device__ __forceinline__ void cls_restrict_at(const ClassLevel<T>& L, T w,
   const int i = I * fx + a2, j = J * fy + c2;
   const bool in = valid && sub < fx * fy && i < L.nx && j < L.ny;
   const int row = in ? i * L.ny + j : 0;
-  const int c = in ? cls_of(L, row) : 0;
-  const bool act = in && L.cactive[c];
+  const bool act = in && L.ract[row];
   T acc = (T)0;
   if (act) {
-    const int e1 = L.row_ptr[c + 1];
-    int e = L.row_ptr[c] + lane;
+    const int2 rr = L.rr[row];
+    const int e1 = rr.y;
+    int e = rr.x + lane;
     for (; e + 3 * LPR < e1; e += 4 * LPR) {
       const int q0 = cls_nb(L, row, e), q1 = cls_nb(L, row, e + LPR);
       const int q2 = cls_nb(L, row, e + 2 * LPR), q3 = cls_nb(L, row, e + 3 * LPR);
@@ -687,13 +692,11 @@ __device__ __forceinline__ void cls_restrict_serial_at(const ClassLevel<T>& L, T
       const int j = J * fy + c2;
       const bool in = valid && i < L.nx && j < L.ny;
       const int row = in ? i * L.ny + j : 0;
-      const int c = in ? cls_of(L, row) : 0;
-      const bool act = in && L.cactive[c];
+      const bool act = in && L.ract[row];
       T acc = (T)0;
       if (act) {
-        const int e1 = L.row_ptr[c + 1];
-        for (int e = L.row_ptr[c] + lane; e < e1; e += LPR)
-          acc += L.ent_val[e] * t[cls_nb(L, row, e)];
+        const int2 rr = L.rr[row];
+        for (int e = rr.x + lane; e < rr.y; e += LPR) acc += L.ent_val[e] * t[cls_nb(L, row, e)];
       }
 #pragma unroll
       for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
@@ -993,6 +996,9 @@ void PoissonEngine::free_levels() {
       hipFree(L.box);
       hipFree(L.box_dinv);
       hipFree(L.box_act);
+      hipFree(L.rr);
+      hipFree(L.rdinv);
+      hipFree(L.ract);
     }
     v->clear();
   }
@@ -1108,6 +1114,23 @@ static int upload_level(const MGLevelHost& h, const MGLevelHost* next, bool fine
     L.ncls = (int)nc;
     L.nnz = (int)val.size();
     L.max_row = max_row;
+    {  // per-cell resolution of the class dictionary
+      std::vector<int2> rr(n);
+      std::vector<T> rd(n);
+      std::vector<uint8_t> ra(n);
+      for (size_t k = 0; k < n; ++k) {
+        const size_t c = wide ? (size_t)h.cls32[k] : (size_t)h.cls[k];
+        rr[k] = make_int2(rp[c], rp[c + 1]);
+        rd[k] = cd[c];
+        ra[k] = h.cactive[c];
+      }
+      PCFD_HIP_CHECK(hipMalloc(&L.rr, n * sizeof(int2)));
+      PCFD_HIP_CHECK(hipMalloc(&L.rdinv, n * sizeof(T)));
+      PCFD_HIP_CHECK(hipMalloc(&L.ract, n));
+      PCFD_HIP_CHECK(hipMemcpy(L.rr, rr.data(), n * sizeof(int2), hipMemcpyHostToDevice));
+      PCFD_HIP_CHECK(hipMemcpy(L.rdinv, rd.data(), n * sizeof(T), hipMemcpyHostToDevice));
+      PCFD_HIP_CHECK(hipMemcpy(L.ract, ra.data(), n, hipMemcpyHostToDevice));
+    }
     // box dictionary for the streaming legs: class rows as dense (2R+1)^2
     // boxes, then one all-zero class for cells outside the grid
     const int R = std::max(L.rxs, L.rys);
@@ -1805,6 +1828,12 @@ static ClassLevel<T> class_level(const LevelBuf& L, int px, int py) {
   D.ent_val = (const T*)L.ent_val;
   D.cdinv = (const T*)L.cdinv;
   D.cactive = L.cactive;
+  D.rr = L.rr;
+  D.rdinv = (const T*)L.rdinv;
+  D.ract = L.ract;
+  D.fx = L.fx;
+  D.fy = L.fy;
+  D.cny = (L.ny + L.fy - 1) / L.fy;
   return D;
 }
 

@@ -57,6 +57,12 @@ struct ClassLevel {
   const T* ent_val;        // per entry coefficient
   const T* cdinv;          // per class 1/diag
   const uint8_t* cactive;  // per class
+  // The same resolved per cell, so a row needs no class lookup before its
+  // entries: first / one-past-last entry, 1/diag and active flag.
+  const int2* rr;
+  const T* rdinv;
+  const uint8_t* ract;
+  int fx, fy, cny;  // aggregation towards the next level (prolongation index)
 };
 
 // Device-resident scalars of one solve.  Kernels read them directly so the
@@ -139,6 +145,9 @@ struct LevelBuf {
   int ncls = 0, nnz = 0, max_row = 0, lanes = 1;
   uint16_t* cls = nullptr;
   uint32_t* cls32 = nullptr;  // wide class index (never run through the fused kernels)
+  int2* rr = nullptr;         // per cell: entry range of its class row
+  void* rdinv = nullptr;      // per cell 1/diag
+  uint8_t* ract = nullptr;    // per cell active flag
   int *row_ptr = nullptr, *ent_lin = nullptr, *agg = nullptr;
   int8_t *ent_di = nullptr, *ent_dj = nullptr;
   void *ent_val = nullptr, *cdinv = nullptr;
