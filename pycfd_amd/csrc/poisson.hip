@@ -1086,6 +1086,7 @@ static int upload_level(const MGLevelHost& h, const MGLevelHost* next, bool fine
       if (hot_n <= kHotMax) {
         L.hot = hot;
         L.hot_n = hot_n;
+        L.hot_e0 = rp[hot];
       }
     }
     const size_t nnz = std::max<size_t>(val.size(), 1);
@@ -1279,9 +1280,12 @@ int PoissonEngine::tune_level(std::vector<LevelBuf>& lv, int l, SolveState* tst)
                          (fine ? stream_supported(op_, nu, L.fx, L.fy)
                                : (L.box_radius > 0 && nu == 1));
   if (!fine && L.hdi.empty() && !stream_ok) return kOk;
-  const unsigned per_cell = fine ? 3u * sizeof(T) + 1u : 3u * sizeof(T) + 2u;
-  const unsigned fixed = fine ? 256u * sizeof(T)
-                              : (L.hot >= 0 ? (unsigned)L.hot_n * (unsigned)(sizeof(T) + 4) : 0u);
+  // LDS bytes per region cell: three vectors + flags (level 0); three
+  // vectors, 1/diag, entry range and active flag (stored levels)
+  const unsigned per_cell = fine ? 3u * sizeof(T) + 1u : 4u * sizeof(T) + 9u;
+  const unsigned fixed =
+      fine ? 256u * sizeof(T)
+           : (L.hot >= 0 ? (unsigned)L.hot_n * (unsigned)(sizeof(T) + 4) : 0u) + 16u;
   const int ftx = fine ? cfg_.tile_tx : 0, fty = fine ? cfg_.tile_ty : 0;
   std::vector<FusedTile> cands;
   if (fine || !L.hdi.empty())
@@ -1880,6 +1884,10 @@ static ClassRows<T> class_rows(const LevelBuf& L, int px, int py) {
   D.cactive = L.cactive;
   D.hot = L.hot;
   D.hot_n = L.hot_n;
+  D.rdinv = (const T*)L.rdinv;
+  D.ract = L.ract;
+  D.rr = L.rr;
+  D.hot_e0 = L.hot_e0;
   return D;
 }
 

@@ -174,6 +174,7 @@ struct LevelBuf {
   int* ent_off = nullptr;
   std::vector<int8_t> hdi, hdj;  // host copy of the entry offsets (tile planning)
   int hot = -1, hot_n = 0;       // most frequent class and its entry count
+  int hot_e0 = 0;                // its first entry
 };
 
 class PoissonEngine {

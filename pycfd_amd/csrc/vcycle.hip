@@ -15,7 +15,6 @@ namespace pcfd {
 namespace {
 
 constexpr uint8_t F_DIRB = 0x20;  // region flag: solvable cell on a Dirichlet wall
-constexpr uint16_t kOutside = 0xFFFF;
 
 // Periodic image of i in [0, n).  Halos span a few cells, so the loops run
 // once at most except on grids smaller than the halo.
@@ -761,62 +760,65 @@ k_fine_up_stream(FineOp A, int strips, int L, T w, const T* __restrict__ x_pre,
 // ---------------------------------------------------------------------------
 // Row c of the level applied at region cell idx; four entries per trip with
 // their loads issued together, summed in entry order.
+// Row (entry range rr) at region cell idx from the global dictionary, four
+// partial sums to keep the multiply-add chain short.
 template <typename T>
-__device__ __forceinline__ T class_row(const ClassRows<T>& L, int c, const T* X, int idx) {
-  T acc = (T)0;
-  const int e1 = L.row_ptr[c + 1];
-  int e = L.row_ptr[c];
-  for (; e + 3 < e1; e += 4) {
+__device__ __forceinline__ T class_row(const ClassRows<T>& L, int2 rr, const T* X, int idx) {
+  T s0 = (T)0, s1 = (T)0, s2 = (T)0, s3 = (T)0;
+  int e = rr.x;
+  for (; e + 3 < rr.y; e += 4) {
     const int o0 = L.ent_off[e], o1 = L.ent_off[e + 1], o2 = L.ent_off[e + 2],
               o3 = L.ent_off[e + 3];
     const T a0 = L.ent_val[e], a1 = L.ent_val[e + 1], a2 = L.ent_val[e + 2],
             a3 = L.ent_val[e + 3];
-    const T x0 = X[idx + o0], x1 = X[idx + o1], x2 = X[idx + o2], x3 = X[idx + o3];
-    acc += a0 * x0;
-    acc += a1 * x1;
-    acc += a2 * x2;
-    acc += a3 * x3;
+    s0 += a0 * X[idx + o0];
+    s1 += a1 * X[idx + o1];
+    s2 += a2 * X[idx + o2];
+    s3 += a3 * X[idx + o3];
   }
-  for (; e < e1; ++e) acc += L.ent_val[e] * X[idx + L.ent_off[e]];
-  return acc;
+  for (; e < rr.y; ++e) s0 += L.ent_val[e] * X[idx + L.ent_off[e]];
+  return (s0 + s1) + (s2 + s3);
 }
 
 // LDS of a stored-level tile: the hot row's coefficients and offsets, three
-// vectors and the row classes.
+// vectors, per cell 1/diag, entry range and active flag.
 template <typename T>
 struct ClassLds {
-  T *HV, *P, *Q, *S;
+  T *HV, *P, *Q, *S, *D;
+  int2* RR;
   int* HO;
-  uint16_t* C;
+  uint8_t* A;
   __device__ ClassLds(unsigned char* smem, int rn, int hot_n) {
     HV = (T*)smem;
     P = HV + hot_n;
     Q = P + rn;
     S = Q + rn;
-    HO = (int*)(S + rn);
-    C = (uint16_t*)(HO + hot_n);
+    D = S + rn;
+    size_t off = (size_t)((unsigned char*)(D + rn) - smem);
+    off = (off + 7) & ~(size_t)7;
+    RR = (int2*)(smem + off);  // (-1, -1) outside the grid
+    HO = (int*)(RR + rn);
+    A = (uint8_t*)(HO + hot_n);
   }
 };
 
-// Row c at region cell idx, the hot row from LDS (the common case: all lanes
+// Row at region cell idx: the hot row from LDS (the common case: all lanes
 // of an interior wavefront share it), any other row from the global dictionary.
 template <typename T>
-__device__ __forceinline__ T class_row_lds(const ClassRows<T>& L, const ClassLds<T>& M, int c,
+__device__ __forceinline__ T class_row_lds(const ClassRows<T>& L, const ClassLds<T>& M, int2 rr,
                                            const T* X, int idx) {
-  if (c != L.hot) return class_row(L, c, X, idx);
-  T acc = (T)0;
+  if (L.hot < 0 || rr.x != L.hot_e0) return class_row(L, rr, X, idx);
+  T s0 = (T)0, s1 = (T)0, s2 = (T)0, s3 = (T)0;
   const int n = L.hot_n;
   int e = 0;
   for (; e + 3 < n; e += 4) {
-    const T x0 = X[idx + M.HO[e]], x1 = X[idx + M.HO[e + 1]], x2 = X[idx + M.HO[e + 2]],
-            x3 = X[idx + M.HO[e + 3]];
-    acc += M.HV[e] * x0;
-    acc += M.HV[e + 1] * x1;
-    acc += M.HV[e + 2] * x2;
-    acc += M.HV[e + 3] * x3;
+    s0 += M.HV[e] * X[idx + M.HO[e]];
+    s1 += M.HV[e + 1] * X[idx + M.HO[e + 1]];
+    s2 += M.HV[e + 2] * X[idx + M.HO[e + 2]];
+    s3 += M.HV[e + 3] * X[idx + M.HO[e + 3]];
   }
-  for (; e < n; ++e) acc += M.HV[e] * X[idx + M.HO[e]];
-  return acc;
+  for (; e < n; ++e) s0 += M.HV[e] * X[idx + M.HO[e]];
+  return (s0 + s1) + (s2 + s3);
 }
 
 template <typename T>
@@ -841,15 +843,18 @@ k_class_down(ClassRows<T> L, FusedTile ft, int rxs, int rys, T w, int nu, const 
   T* B = M.P;
   T* X = M.Q;
   T* Y = M.S;
-  uint16_t* C = M.C;
+  int2* RR = M.RR;
   const TileOrigin o = tile_origin(ft);
   stage_hot_row(L, M);
 
   // Load b and the row classes, and start from zero: x = w D^-1 b.
+  T* D = M.D;
+  uint8_t* A = M.A;
   struct ClsLd {
     bool ok;
-    uint16_t cl;
-    T bv;
+    int2 rr;
+    uint8_t act;
+    T bv, dv;
   };
   for_region_batched<kLoadBatch>(
       ft.rx, ry,
@@ -857,30 +862,35 @@ k_class_down(ClassRows<T> L, FusedTile ft, int rxs, int rys, T w, int nu, const 
         ClsLd v;
         int i, j;
         v.ok = resolve(o.gi0 + a, o.gj0 + c, L.nx, L.ny, L.px, L.py, i, j);
-        v.cl = kOutside;
-        v.bv = (T)0;
+        v.rr = make_int2(-1, -1);
+        v.act = 0;
+        v.bv = v.dv = (T)0;
         if (v.ok) {
           const int k = i * L.ny + j;
-          v.cl = L.cls[k];
+          v.rr = L.rr[k];
           v.bv = b[k];
+          v.dv = L.rdinv[k];
+          v.act = L.ract[k];
         }
         return v;
       },
       [&](int a, int c, int idx, const ClsLd& v) {
-        C[idx] = v.cl;
+        RR[idx] = v.rr;
         B[idx] = v.bv;
-        X[idx] = v.ok ? w * L.cdinv[v.cl] * v.bv : (T)0;
+        D[idx] = v.dv;
+        A[idx] = v.act;
+        X[idx] = w * v.dv * v.bv;
       });
   __syncthreads();
   for (int s = 1; s < nu; ++s) {
     for_ring(ring(ft, nu + 1 - s, rxs, rys), [&](int a, int c) {
       const int idx = a * ry + c;
-      const uint16_t cl = C[idx];
-      if (cl == kOutside) {
+      const int2 rr = RR[idx];
+      if (rr.x < 0) {
         Y[idx] = (T)0;
         return;
       }
-      Y[idx] = X[idx] + w * L.cdinv[cl] * (B[idx] - class_row_lds(L, M, cl, X, idx));
+      Y[idx] = X[idx] + w * D[idx] * (B[idx] - class_row_lds(L, M, rr, X, idx));
     });
     __syncthreads();
     T* t = X;
@@ -890,8 +900,8 @@ k_class_down(ClassRows<T> L, FusedTile ft, int rxs, int rys, T w, int nu, const 
   // Residual on ring 1 into Y.
   for_ring(ring(ft, 1, rxs, rys), [&](int a, int c) {
     const int idx = a * ry + c;
-    const uint16_t cl = C[idx];
-    Y[idx] = (cl == kOutside) ? (T)0 : B[idx] - class_row_lds(L, M, cl, X, idx);
+    const int2 rr = RR[idx];
+    Y[idx] = (rr.x < 0) ? (T)0 : B[idx] - class_row_lds(L, M, rr, X, idx);
   });
   __syncthreads();
   for_ring(ring(ft, 1, rxs, rys), [&](int a, int c) {
@@ -901,8 +911,7 @@ k_class_down(ClassRows<T> L, FusedTile ft, int rxs, int rys, T w, int nu, const 
       const int i = o.ti * ft.tx + ia, j = o.tj * ft.ty + jc;
       if (i < L.nx && j < L.ny) x_out[i * L.ny + j] = X[idx];
     }
-    const uint16_t cl = C[idx];
-    X[idx] = (cl == kOutside) ? (T)0 : L.cdinv[cl] * Y[idx];
+    X[idx] = D[idx] * Y[idx];  // zero outside the grid (D = 0 there)
   });
   __syncthreads();
   const int ctx = ft.tx / fx, cty = ft.ty / fy;
@@ -918,8 +927,7 @@ k_class_down(ClassRows<T> L, FusedTile ft, int rxs, int rys, T w, int nu, const 
         const int j = J * fy + c2;
         if (j >= L.ny) break;
         const int idx = (ft.hx + ca * fx + a2) * ry + (ft.hy + cc * fy + c2);
-        const uint16_t cl = C[idx];
-        acc += L.cactive[cl] ? Y[idx] - w * class_row_lds(L, M, cl, X, idx) : (T)0;
+        acc += A[idx] ? Y[idx] - w * class_row_lds(L, M, RR[idx], X, idx) : (T)0;
       }
     }
     bc[I * cny + J] = acc;
@@ -938,14 +946,16 @@ k_class_up(ClassRows<T> L, FusedTile ft, int rxs, int rys, T w, int nu, const T*
   T* E = M.P;
   T* X = M.Q;
   T* B = M.S;
-  uint16_t* C = M.C;
+  int2* RR = M.RR;
   const TileOrigin o = tile_origin(ft);
   stage_hot_row(L, M);
 
+  T* D = M.D;
   struct ClsUpLd {
     bool ok;
-    uint16_t cl;
-    T xv, bv, ev;
+    int2 rr;
+    uint8_t act;
+    T xv, bv, ev, dv;
   };
   for_region_batched<kLoadBatch>(
       ft.rx, ry,
@@ -953,29 +963,33 @@ k_class_up(ClassRows<T> L, FusedTile ft, int rxs, int rys, T w, int nu, const T*
         ClsUpLd v;
         int i, j;
         v.ok = resolve(o.gi0 + a, o.gj0 + c, L.nx, L.ny, L.px, L.py, i, j);
-        v.cl = kOutside;
-        v.xv = v.bv = v.ev = (T)0;
+        v.rr = make_int2(-1, -1);
+        v.act = 0;
+        v.xv = v.bv = v.ev = v.dv = (T)0;
         if (v.ok) {
           const int k = i * L.ny + j;
-          v.cl = L.cls[k];
+          v.rr = L.rr[k];
           v.xv = x_pre[k];
           v.bv = b[k];
+          v.dv = L.rdinv[k];
+          v.act = L.ract[k];
           v.ev = xc[(i / fx) * cny + (j / fy)];  // in range for every grid cell
         }
         return v;
       },
       [&](int a, int c, int idx, const ClsUpLd& v) {
-        C[idx] = v.cl;
+        RR[idx] = v.rr;
         X[idx] = v.xv;
         B[idx] = v.bv;
-        E[idx] = (v.ok && L.cactive[v.cl]) ? v.ev : (T)0;
+        D[idx] = v.dv;
+        E[idx] = v.act ? v.ev : (T)0;
       });
   __syncthreads();
   for_ring(ring(ft, nu, rxs, rys), [&](int a, int c) {
     const int idx = a * ry + c;
-    const uint16_t cl = C[idx];
-    if (cl == kOutside) return;
-    X[idx] = X[idx] + E[idx] - w * L.cdinv[cl] * class_row_lds(L, M, cl, E, idx);
+    const int2 rr = RR[idx];
+    if (rr.x < 0) return;
+    X[idx] = X[idx] + E[idx] - w * D[idx] * class_row_lds(L, M, rr, E, idx);
   });
   __syncthreads();
   T* Y = E;
@@ -983,9 +997,9 @@ k_class_up(ClassRows<T> L, FusedTile ft, int rxs, int rys, T w, int nu, const T*
     const bool last = sw == nu;
     for_ring(ring(ft, nu - sw, rxs, rys), [&](int a, int c) {
       const int idx = a * ry + c;
-      const uint16_t cl = C[idx];
+      const int2 rr = RR[idx];
       T val = (T)0;
-      if (cl != kOutside) val = X[idx] + w * L.cdinv[cl] * (B[idx] - class_row_lds(L, M, cl, X, idx));
+      if (rr.x >= 0) val = X[idx] + w * D[idx] * (B[idx] - class_row_lds(L, M, rr, X, idx));
       if (!last) {
         Y[idx] = val;
         return;

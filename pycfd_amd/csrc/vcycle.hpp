@@ -32,6 +32,10 @@ struct ClassRows {
   const uint8_t* cactive;
   int hot;    // the level's most frequent class (interior rows), staged in LDS; -1: none
   int hot_n;  // its entry count
+  const T* rdinv;        // per cell 1/diag
+  const uint8_t* ract;   // per cell active flag
+  const int2* rr;        // per cell entry range of its class row
+  int hot_e0;            // first entry of the hot class
 };
 
 // Largest row the fused class kernels stage in LDS as the level's hot class.
