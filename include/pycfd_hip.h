@@ -118,6 +118,10 @@ typedef struct pcfd_solver_desc {
    * 0 measured choice, -1 never, k > 0 always, k rows per wavefront (a
    * multiple of 4). */
   int32_t stream_rows;
+  /* Stored-level passes (when a level runs pass by pass): 0 register-resident
+   * rows wherever a row fits 8 entries per lane on 64 lanes, -1 always the
+   * looping row kernels. */
+  int32_t row_passes;
 } pcfd_solver_desc;
 
 /* One level of the aggregation hierarchy (multigrid.py:293-322).  Level 0 is

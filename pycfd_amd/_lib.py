@@ -65,6 +65,7 @@ class SolverDesc(C.Structure):
         ("omega", C.c_double), ("sweeps", C.c_int32), ("precision", C.c_int32),
         ("fused_levels", C.c_int32), ("tile_tx", C.c_int32), ("tile_ty", C.c_int32),
         ("coarse_chain", C.c_int32), ("stream_rows", C.c_int32),
+        ("row_passes", C.c_int32),
     ]
 
 

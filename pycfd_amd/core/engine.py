@@ -105,10 +105,11 @@ class DeviceContext:
     def set_solver(self, kind: str, tol: float, maxiter: int, omega: float, sweeps: int,
                    precision: int, levels: list[dict] | None, *, fused_levels: bool = True,
                    tile: tuple[int, int] = (0, 0), coarse_chain: int = 0,
-                   stream_rows: int = 0) -> None:
+                   stream_rows: int = 0, row_passes: int = 0) -> None:
         sd = _lib.SolverDesc(_lib.SOLVER_CODES[kind], float(tol), int(maxiter), float(omega),
                              int(sweeps), int(precision), int(bool(fused_levels)),
-                             int(tile[0]), int(tile[1]), int(coarse_chain), int(stream_rows))
+                             int(tile[0]), int(tile[1]), int(coarse_chain), int(stream_rows),
+                             int(row_passes))
         keep = []
         if levels:
             arr = (_lib.MGLevelDesc * len(levels))()

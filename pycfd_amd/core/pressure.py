@@ -183,6 +183,9 @@ class PoissonSolverBase:
         # level 0's Krylov-start legs in streaming form: 0 measured choice,
         # -1 never, k > 0 always with k rows per wavefront
         self.stream_rows = 0
+        # stored-level passes: 0 register-resident rows where they fit, -1 the
+        # looping row kernels
+        self.row_passes = 0
         # hand the stored multigrid levels over with 32-bit class indices even
         # when 16 bits would do (they are chosen by size otherwise)
         self.wide_classes = False
@@ -230,7 +233,8 @@ class PoissonSolverBase:
         self.context.set_solver(self._device_kind, tol, maxiter, self._omega(), self._sweeps(),
                                 1 if self.precision == "mixed" else 0, self._levels(),
                                 fused_levels=self.fused_levels, tile=self.tile,
-                                coarse_chain=self.coarse_chain, stream_rows=self.stream_rows)
+                                coarse_chain=self.coarse_chain, stream_rows=self.stream_rows,
+                                row_passes=self.row_passes)
         self._configured = True
 
     def record(self, iterations: int, residual: float, converged: bool) -> None:

@@ -43,6 +43,52 @@ __device__ __forceinline__ Line line_of(double* arr, int ld, int rows, int cols,
   return Line{arr + idx, (size_t)ld, rows};
 }
 
+// Lines run in chunks of kSlots entries per thread, every read of a chunk
+// issued before any of its writes: one memory round trip per chunk instead
+// of one per entry.  A thread visits its entries k = tid, tid + blockDim, ...
+// in increasing order, as a plain strided loop does, and the reads and writes
+// of one call touch different lines (or the same entry of one line), so
+// every written value and every partial sum is the plain loop's.
+constexpr int kSlots = 8;
+
+// put(k, get(k)) for k in [0, n).
+template <class Get, class Put>
+__device__ __forceinline__ void line_map(int n, const Get& get, const Put& put) {
+  if (n <= 0) return;
+  for (int base = 0; base < n; base += kSlots * (int)blockDim.x) {
+    double val[kSlots];
+#pragma unroll
+    for (int t = 0; t < kSlots; ++t)
+      val[t] = get(min(base + t * (int)blockDim.x + (int)threadIdx.x, n - 1));
+#pragma unroll
+    for (int t = 0; t < kSlots; ++t) {
+      const int k = base + t * (int)blockDim.x + (int)threadIdx.x;
+      if (k < n) put(k, val[t]);
+    }
+  }
+}
+
+// This thread's sum of get(k) over its entries of [0, n) with take(k).
+template <class Get, class Take>
+__device__ __forceinline__ double line_part(int n, const Get& get, const Take& take) {
+  double part = 0.0;
+  if (n <= 0) return part;
+  for (int base = 0; base < n; base += kSlots * (int)blockDim.x) {
+    double val[kSlots];
+    bool use[kSlots];
+#pragma unroll
+    for (int t = 0; t < kSlots; ++t) {
+      const int k = base + t * (int)blockDim.x + (int)threadIdx.x;
+      val[t] = get(min(k, n - 1));
+      use[t] = k < n && take(min(k, n - 1));
+    }
+#pragma unroll
+    for (int t = 0; t < kSlots; ++t)
+      if (use[t]) part += val[t];
+  }
+  return part;
+}
+
 __device__ double block_sum(double x) {
   double v[1] = {x};
   const int ops[1] = {kSum};
@@ -78,54 +124,57 @@ __device__ void apply_wall(const BCParams& P, int w, bool predictor, double* u, 
   const bool prescribes = kind != BC_PRESSURE_OUTLET;
   const bool derives = kind == BC_OUTLET;
   if (prescribes && (predictor || !derives)) {
-    for (int k = threadIdx.x; k < nf.len; k += blockDim.x) {
-      double val = 0.0;
-      if (kind == BC_INLET) val = P.inlet[w][k];
-      else if (kind == BC_OUTLET) val = nin[k];
-      nf[k] = val;
-    }
+    const double* inl = P.inlet[w];
+    line_map(
+        nf.len,
+        [&](int k) {
+          if (kind == BC_INLET) return inl[k];
+          return kind == BC_OUTLET ? nin[k] : 0.0;
+        },
+        [&](int k, double val) { nf[k] = val; });
     __syncthreads();
   }
-  for (int k = threadIdx.x; k < ng.len; k += blockDim.x)
-    ng[k] = (kind == BC_PRESSURE_OUTLET) ? nf[k] : 2.0 * nf[k] - nin[k];
-  for (int k = threadIdx.x; k < tg.len; k += blockDim.x) {
-    const double t = tin[k];
-    double val;
-    if (kind == BC_MOVING_WALL) val = 2.0 * P.vel[w] - t;
-    else if (kind == BC_NO_SLIP || kind == BC_INLET) val = -t;
-    else val = t;  // OUTLET, PRESSURE_OUTLET, SYMMETRY: zero gradient
-    tg[k] = val;
-  }
+  line_map(
+      ng.len, [&](int k) { return (kind == BC_PRESSURE_OUTLET) ? nf[k] : 2.0 * nf[k] - nin[k]; },
+      [&](int k, double val) { ng[k] = val; });
+  line_map(
+      tg.len,
+      [&](int k) {
+        const double t = tin[k];
+        if (kind == BC_MOVING_WALL) return 2.0 * P.vel[w] - t;
+        if (kind == BC_NO_SLIP || kind == BC_INLET) return -t;
+        return t;  // OUTLET, PRESSURE_OUTLET, SYMMETRY: zero gradient
+      },
+      [&](int k, double val) { tg[k] = val; });
   __syncthreads();
 }
 
 // BoundaryManager._wrap_velocity (boundary.py:388-410).
 __device__ void wrap_velocity(const BCParams& P, double* u, double* v) {
   const int nx = P.nx, ny = P.ny;
+  // Copies between distinct lines (the sources are interior lines, the
+  // destinations ghost lines), so all reads may go first.
+  auto copy = [&](double* arr, size_t ld, int n, int axis, int dst, int src) {
+    const size_t sd = axis == 0 ? 1 : ld;
+    double* d = axis == 0 ? arr + (size_t)dst * ld : arr + dst;
+    const double* s = axis == 0 ? arr + (size_t)src * ld : arr + src;
+    line_map(n, [&](int k) { return s[(size_t)k * sd]; },
+             [&](int k, double val) { d[(size_t)k * sd] = val; });
+  };
   if (P.periodic_x) {
-    for (int k = threadIdx.x; k < ny + 2; k += blockDim.x) {
-      u[(size_t)(nx + 1) * P.ldu + k] = u[(size_t)1 * P.ldu + k];
-      u[(size_t)0 * P.ldu + k] = u[(size_t)nx * P.ldu + k];
-      u[(size_t)(nx + 2) * P.ldu + k] = u[(size_t)2 * P.ldu + k];
-    }
-    for (int k = threadIdx.x; k < ny + 3; k += blockDim.x) {
-      v[(size_t)0 * P.ldv + k] = v[(size_t)nx * P.ldv + k];
-      v[(size_t)(nx + 1) * P.ldv + k] = v[(size_t)1 * P.ldv + k];
-    }
+    copy(u, P.ldu, ny + 2, 0, nx + 1, 1);
+    copy(u, P.ldu, ny + 2, 0, 0, nx);
+    copy(u, P.ldu, ny + 2, 0, nx + 2, 2);
+    copy(v, P.ldv, ny + 3, 0, 0, nx);
+    copy(v, P.ldv, ny + 3, 0, nx + 1, 1);
     __syncthreads();
   }
   if (P.periodic_y) {
-    for (int k = threadIdx.x; k < nx + 2; k += blockDim.x) {
-      double* row = v + (size_t)k * P.ldv;
-      row[ny + 1] = row[1];
-      row[0] = row[ny];
-      row[ny + 2] = row[2];
-    }
-    for (int k = threadIdx.x; k < nx + 3; k += blockDim.x) {
-      double* row = u + (size_t)k * P.ldu;
-      row[0] = row[ny];
-      row[ny + 1] = row[1];
-    }
+    copy(v, P.ldv, nx + 2, 1, ny + 1, 1);
+    copy(v, P.ldv, nx + 2, 1, 0, ny);
+    copy(v, P.ldv, nx + 2, 1, ny + 2, 2);
+    copy(u, P.ldu, nx + 3, 1, 0, ny);
+    copy(u, P.ldu, nx + 3, 1, ny + 1, 1);
     __syncthreads();
   }
 }
@@ -133,12 +182,17 @@ __device__ void wrap_velocity(const BCParams& P, double* u, double* v) {
 // ProjectionSolver._wrap_periodic_faces (solver.py:601-607).
 __device__ void wrap_faces(const BCParams& P, double* u, double* v) {
   const int nx = P.nx, ny = P.ny;
-  if (P.periodic_x)
-    for (int k = threadIdx.x; k < ny + 2; k += blockDim.x)
-      u[(size_t)(nx + 1) * P.ldu + k] = u[(size_t)1 * P.ldu + k];
-  if (P.periodic_y)
-    for (int k = threadIdx.x; k < nx + 2; k += blockDim.x)
-      v[(size_t)k * P.ldv + ny + 1] = v[(size_t)k * P.ldv + 1];
+  if (P.periodic_x) {
+    double* d = u + (size_t)(nx + 1) * P.ldu;
+    const double* s = u + (size_t)1 * P.ldu;
+    line_map(ny + 2, [&](int k) { return s[k]; }, [&](int k, double val) { d[k] = val; });
+  }
+  if (P.periodic_y) {
+    const size_t ld = P.ldv;
+    line_map(
+        nx + 2, [&](int k) { return v[(size_t)k * ld + 1]; },
+        [&](int k, double val) { v[(size_t)k * ld + ny + 1] = val; });
+  }
   __syncthreads();
 }
 
@@ -156,12 +210,9 @@ __device__ void mass_balance(const BCParams& P, double* u, double* v) {
     const double width = (wi.axis == 0) ? P.dy : P.dx;
     // stretched mesh: each face weighted by its own height / width
     const double* wk = (wi.axis == 0) ? P.hy : P.hx;
-    double part = 0.0;
-    if (wk) {
-      for (int k = threadIdx.x; k < n; k += blockDim.x) part += f[1 + k] * wk[k];
-    } else {
-      for (int k = threadIdx.x; k < n; k += blockDim.x) part += f[1 + k];
-    }
+    auto all = [](int) { return true; };
+    const double part = wk ? line_part(n, [&](int k) { return f[1 + k] * wk[k]; }, all)
+                           : line_part(n, [&](int k) { return f[1 + k]; }, all);
     const double s = block_sum(part);
     const double flux_out = (double)wi.outward * (wk ? s : s * width);
     if (kind == BC_OUTLET) {
@@ -180,7 +231,8 @@ __device__ void mass_balance(const BCParams& P, double* u, double* v) {
       Line f = (wi.axis == 0) ? line_of(u, P.ldu, P.nx + 3, P.ny + 2, 0, wi.n_face)
                               : line_of(v, P.ldv, P.nx + 2, P.ny + 3, 1, wi.n_face);
       const int n = (wi.axis == 0) ? P.ny : P.nx;
-      for (int k = threadIdx.x; k < n; k += blockDim.x) f[1 + k] *= scale;
+      line_map(n, [&](int k) { return f[1 + k] * scale; },
+               [&](int k, double val) { f[1 + k] = val; });
     }
   } else {
     for (int w = 0; w < 4; ++w) {
@@ -190,8 +242,7 @@ __device__ void mass_balance(const BCParams& P, double* u, double* v) {
       const double width = (wi.axis == 0) ? P.dy : P.dx;
       const double* wk = (wi.axis == 0) ? P.hy : P.hx;
       if (wk) {
-        double part = 0.0;
-        for (int k = threadIdx.x; k < n; k += blockDim.x) part += wk[k];
+        const double part = line_part(n, [&](int k) { return wk[k]; }, [](int) { return true; });
         total_area = total_area + block_sum(part);
       } else {
         total_area = total_area + (double)n * width;
@@ -205,7 +256,8 @@ __device__ void mass_balance(const BCParams& P, double* u, double* v) {
                               : line_of(v, P.ldv, P.nx + 2, P.ny + 3, 1, wi.n_face);
       const int n = (wi.axis == 0) ? P.ny : P.nx;
       const double add = (double)wi.outward * deficit / total_area;
-      for (int k = threadIdx.x; k < n; k += blockDim.x) f[1 + k] += add;
+      line_map(n, [&](int k) { return f[1 + k] + add; },
+               [&](int k, double val) { f[1 + k] = val; });
     }
   }
   __syncthreads();
@@ -223,9 +275,9 @@ __device__ void mask_obstacle(const BCParams& P, int mode, double dt, double* u,
     auto scaled = [&](double sum, bool weighted) -> double {
       return weighted ? sum / dt : sum * P.area / dt;
     };
-    double su = 0.0, sv = 0.0;
-    for (int k = threadIdx.x; k < P.n_uf; k += blockDim.x) su += ua(k);
-    for (int k = threadIdx.x; k < P.n_vf; k += blockDim.x) sv += va(k);
+    auto all = [](int) { return true; };
+    const double su = line_part(P.n_uf, ua, all);
+    const double sv = line_part(P.n_vf, va, all);
     const double fx = scaled(block_sum(su), P.uf_area != nullptr);
     const double fy = scaled(block_sum(sv), P.vf_area != nullptr);
     if (threadIdx.x == 0) {
@@ -238,11 +290,8 @@ __device__ void mask_obstacle(const BCParams& P, int mode, double dt, double* u,
       }
     }
     for (int b = 0; b < P.n_bodies; ++b) {
-      double bu = 0.0, bv = 0.0;
-      for (int k = threadIdx.x; k < P.n_uf; k += blockDim.x)
-        if (P.uf_body[k] == b) bu += ua(k);
-      for (int k = threadIdx.x; k < P.n_vf; k += blockDim.x)
-        if (P.vf_body[k] == b) bv += va(k);
+      const double bu = line_part(P.n_uf, ua, [&](int k) { return P.uf_body[k] == b; });
+      const double bv = line_part(P.n_vf, va, [&](int k) { return P.vf_body[k] == b; });
       const double bx = scaled(block_sum(bu), P.uf_area != nullptr);
       const double by = scaled(block_sum(bv), P.vf_area != nullptr);
       if (threadIdx.x == 0) {
@@ -297,23 +346,28 @@ k_pressure_ghosts(BCParams P, double* p, int ldp) {
     Line g = line_of(p, ldp, nx + 2, ny + 2, wi.axis, wi.p_ghost);
     Line in = line_of(p, ldp, nx + 2, ny + 2, wi.axis, wi.p_in);
     const double pref = P.pref[w];
-    for (int k = threadIdx.x; k < g.len; k += blockDim.x)
-      g[k] = (kind == BC_PRESSURE_OUTLET) ? 2.0 * pref - in[k] : in[k];
+    line_map(
+        g.len, [&](int k) { return (kind == BC_PRESSURE_OUTLET) ? 2.0 * pref - in[k] : in[k]; },
+        [&](int k, double val) { g[k] = val; });
     __syncthreads();
   }
   if (P.periodic_x) {
-    for (int k = threadIdx.x; k < ny + 2; k += blockDim.x) {
-      p[k] = p[(size_t)nx * ldp + k];
-      p[(size_t)(nx + 1) * ldp + k] = p[(size_t)ldp + k];
-    }
+    double* r0 = p;
+    double* r1 = p + (size_t)(nx + 1) * ldp;
+    const double* sn = p + (size_t)nx * ldp;
+    const double* s1 = p + (size_t)ldp;
+    line_map(ny + 2, [&](int k) { return sn[k]; }, [&](int k, double val) { r0[k] = val; });
+    line_map(ny + 2, [&](int k) { return s1[k]; }, [&](int k, double val) { r1[k] = val; });
     __syncthreads();
   }
   if (P.periodic_y) {
-    for (int k = threadIdx.x; k < nx + 2; k += blockDim.x) {
-      double* row = p + (size_t)k * ldp;
-      row[0] = row[ny];
-      row[ny + 1] = row[1];
-    }
+    const size_t ld = ldp;
+    line_map(
+        nx + 2, [&](int k) { return p[(size_t)k * ld + ny]; },
+        [&](int k, double val) { p[(size_t)k * ld] = val; });
+    line_map(
+        nx + 2, [&](int k) { return p[(size_t)k * ld + 1]; },
+        [&](int k, double val) { p[(size_t)k * ld + ny + 1] = val; });
     __syncthreads();
   }
 }

@@ -583,6 +583,7 @@ int pcfd_set_pressure_solver(pcfd_ctx* c, const pcfd_solver_desc* sd,
   cfg.tile_ty = sd->tile_ty;
   cfg.chain = sd->coarse_chain;
   cfg.stream = sd->stream_rows;
+  cfg.row_reg = sd->row_passes;
   if (cfg.stream > 0 && cfg.stream % 4) {
     set_last_error("stream_rows must be a multiple of 4, got %d", cfg.stream);
     return kErrBadArg;

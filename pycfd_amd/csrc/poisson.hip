@@ -716,6 +716,200 @@ k_cls_restrict_serial(ClassLevel<T> L, T w, const T* __restrict__ r, const T* __
 }
 
 // ----------------------------------------------------------------------------
+// The same row passes with every load of a lane issued up front.  A coarse
+// pass is short and bound by its chain of dependent memory round trips (row
+// -> entry -> neighbour), not by bandwidth, so here each lane holds all of
+// its entries (at most kRowNE) in registers: the row's own values, its entry
+// range and the entries of the level's most frequent row (`spec_e0/e1`, the
+// interior rows) go out together, the neighbour values follow in one batch,
+// and only rows of another class pay one more round trip for their entries.
+// Per-lane sums run in the entry order of cls_rows_at at the same lane count.
+// ----------------------------------------------------------------------------
+constexpr int kRowNE = 8;
+
+// Entries e0, e0 + LPR, ... of the row at cell (i, j) whose range ends at
+// e1: coefficient, neighbour cell q and, with AGG, the neighbour's aggregate
+// on the next level.  Lanes past the row's end load its last entry (every
+// row has its diagonal), so all loads are unconditional and go out in one
+// batch; the callers skip those slots.  PER (a periodic level) wraps both
+// axes: on a non-periodic axis the row's own entries stay inside the grid,
+// where the wrap is the identity.  Aggregation factors are 1 or 2.
+template <typename T, int LPR, bool AGG, bool PER>
+__device__ __forceinline__ void row_entries(const ClassLevel<T>& L, int row, int i, int j, int e0,
+                                            int e1, int (&q)[kRowNE], int (&a)[kRowNE],
+                                            T (&v)[kRowNE]) {
+#pragma unroll
+  for (int k = 0; k < kRowNE; ++k) {
+    const int e = min(e0 + k * LPR, e1 - 1);
+    v[k] = L.ent_val[e];
+    int qi = 0, qj = 0;
+    if (PER) {
+      qi = i + (int)L.ent_di[e];
+      qj = j + (int)L.ent_dj[e];
+      qi = (qi % L.nx + L.nx) % L.nx;
+      qj = (qj % L.ny + L.ny) % L.ny;
+      q[k] = qi * L.ny + qj;
+    } else {
+      q[k] = row + L.ent_lin[e];
+      if (AGG) {
+        qi = q[k] / L.ny;
+        qj = q[k] - qi * L.ny;
+      }
+    }
+    a[k] = AGG ? (qi >> (L.fx >> 1)) * L.cny + (qj >> (L.fy >> 1)) : 0;
+  }
+}
+
+// Holds a value in registers at this point of the program, so the loads it
+// comes from are issued before what follows (the compiler may otherwise sink
+// them into the branch that uses them, behind the loads that branch waits for).
+template <typename V>
+__device__ __forceinline__ void issue_here(V& v) {
+  asm volatile("" : "+v"(v));
+}
+
+template <typename T>
+__device__ __forceinline__ void issue_entries(int2& rr, int (&q)[kRowNE], T (&v)[kRowNE]) {
+  issue_here(rr.x);
+  issue_here(rr.y);
+#pragma unroll
+  for (int k = 0; k < kRowNE; ++k) {
+    issue_here(q[k]);
+    issue_here(v[k]);
+  }
+}
+
+template <typename T, int LPR, int OP, bool PER>
+__global__ void __launch_bounds__(kBlock)
+k_cls_rows_reg(ClassLevel<T> L, T w, const T* __restrict__ b, const T* __restrict__ xin,
+               const T* __restrict__ xc, T* __restrict__ o1, T* __restrict__ o2,
+               T* __restrict__ o3, const SolveState* st) {
+  constexpr bool AGG = OP == RO_PROLONG;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = gid % LPR;
+  const bool valid = gid / LPR < L.n;
+  const int row = valid ? gid / LPR : L.n - 1;  // lanes past the grid load row n-1, store nothing
+  const int i = row / L.ny, j = row - i * L.ny;
+  int2 rr = L.rr[row];
+  const T dinv = L.rdinv[row];
+  const T bk = OP == RO_PROLONG ? (T)0 : b[row];
+  const T xk = OP == RO_SMOOTH ? xin[row] : (T)0;
+  T ok = (T)0, own = (T)0;
+  bool own_act = false;
+  if (OP == RO_PROLONG) {
+    ok = o1[row];
+    own = xc[(i >> (L.fx >> 1)) * L.cny + (j >> (L.fy >> 1))];
+    own_act = L.ract[row] != 0;
+  }
+  int q[kRowNE], a[kRowNE];
+  T v[kRowNE];
+  int done = st->done;
+  row_entries<T, LPR, AGG, PER>(L, row, i, j, L.spec_e0 + lane, L.spec_e1, q, a, v);
+  issue_entries(rr, q, v);
+  issue_here(done);
+  if (done) return;
+  if (rr.x != L.spec_e0) row_entries<T, LPR, AGG, PER>(L, row, i, j, rr.x + lane, rr.y, q, a, v);
+  T xv[kRowNE];
+#pragma unroll
+  for (int k = 0; k < kRowNE; ++k) {
+    if (OP == RO_PROLONG) {
+      const T ev = xc[a[k]];
+      xv[k] = L.ract[q[k]] ? ev : (T)0;
+    } else if (OP == RO_SMOOTH0_RES) {
+      xv[k] = w * L.rdinv[q[k]] * b[q[k]];
+    } else {
+      xv[k] = xin[q[k]];
+    }
+  }
+  T acc = (T)0;
+#pragma unroll
+  for (int k = 0; k < kRowNE; ++k) {
+    const T sum = acc + v[k] * xv[k];
+    acc = rr.x + lane + k * LPR < rr.y ? sum : acc;
+  }
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
+  if (!valid || lane != 0) return;
+  if (OP == RO_SMOOTH0_RES) {
+    const T rk = bk - acc;
+    o1[row] = w * dinv * bk;
+    o2[row] = rk;
+    o3[row] = dinv * rk;
+  } else if (OP == RO_SMOOTH) {
+    o1[row] = xk + w * dinv * (bk - acc);
+  } else if (OP == RO_RES) {
+    const T rk = bk - acc;
+    o2[row] = rk;
+    o3[row] = dinv * rk;
+  } else {  // RO_PROLONG
+    o1[row] = ok + (own_act ? own : (T)0) - w * dinv * acc;
+  }
+}
+
+// Restriction with register-resident rows: 4 * LPR lanes per coarse cell, LPR
+// per fine row of its block.  LPR <= 16: the cell inside one wavefront, its
+// rows summed in slot order by shuffles, as cls_restrict_at; LPR = 32, 64: the
+// cell spans wavefronts of one workgroup and its rows are summed through LDS.
+template <typename T, int LPR, bool PER>
+__global__ void __launch_bounds__(kBlock)
+k_cls_restrict_reg(ClassLevel<T> L, T w, const T* __restrict__ r, const T* __restrict__ t,
+                   int fx, int fy, int cnx, int cny, T* __restrict__ bc, const SolveState* st) {
+  constexpr int G = 4 * LPR;
+  static_assert(G <= kBlock && kBlock % G == 0, "a coarse cell's lanes must share a workgroup");
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int crow = gid / G;
+  const int sub = (gid % G) / LPR;
+  const int lane = gid % LPR;
+  const bool valid = crow < cnx * cny;
+  const int I = valid ? crow / cny : 0, J = valid ? crow - I * cny : 0;
+  const int a2 = sub / fy, c2 = sub - a2 * fy;
+  const int i = I * fx + a2, j = J * fy + c2;
+  const bool in = valid && sub < fx * fy && i < L.nx && j < L.ny;
+  const int row = in ? i * L.ny + j : 0;
+  const int ri = in ? i : 0, rj = in ? j : 0;
+  int2 rr = L.rr[row];
+  int ra = L.ract[row];
+  const T rk = r[row];
+  int q[kRowNE], a[kRowNE];
+  T v[kRowNE];
+  int done = st->done;
+  row_entries<T, LPR, false, PER>(L, row, ri, rj, L.spec_e0 + lane, L.spec_e1, q, a, v);
+  issue_entries(rr, q, v);
+  issue_here(ra);
+  issue_here(done);
+  if (done) return;
+  const bool act = in & (ra != 0);
+  if (rr.x != L.spec_e0)
+    row_entries<T, LPR, false, PER>(L, row, ri, rj, rr.x + lane, rr.y, q, a, v);
+  T tv[kRowNE];
+#pragma unroll
+  for (int k = 0; k < kRowNE; ++k) tv[k] = t[q[k]];
+  T acc = (T)0;
+#pragma unroll
+  for (int k = 0; k < kRowNE; ++k) {
+    const T sum = acc + v[k] * tv[k];
+    acc = rr.x + lane + k * LPR < rr.y ? sum : acc;
+  }
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
+  const T y = act ? rk - w * acc : (T)0;
+  T sum = (T)0;
+  if constexpr (G <= kWave) {
+    const int g0 = (int)(threadIdx.x % kWave) - (int)(threadIdx.x % G);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) sum += __shfl(y, g0 + s * LPR, kWave);
+  } else {  // a coarse cell spans several wavefronts: combine through LDS
+    __shared__ T ys[kBlock / LPR];
+    if (lane == 0) ys[threadIdx.x / LPR] = y;
+    __syncthreads();
+    const int c0 = (int)(threadIdx.x / G) * 4;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) sum += ys[c0 + s];
+  }
+  if (valid && sub == 0 && lane == 0) bc[crow] = sum;
+}
+
+// ----------------------------------------------------------------------------
 // The deepest stored levels and the dense coarsest solve in one cooperative
 // launch.  These levels are small (tens of thousands of rows at most) and their
 // passes are bound by launch latency, so the down leg, the coarsest solve and
@@ -1048,7 +1242,7 @@ static int upload_level(const MGLevelHost& h, const MGLevelHost* next, bool fine
     std::vector<int> rp(nc + 1, 0), lin;
     std::vector<int8_t> edi, edj;
     std::vector<T> val;
-    int max_row = 0;
+    int max_row = 0, min_row = 1 << 30;
     for (size_t c = 0; c < nc; ++c) {
       for (size_t q = 0; q < S; ++q) {
         const double a = h.table[c * S + q];
@@ -1060,7 +1254,9 @@ static int upload_level(const MGLevelHost& h, const MGLevelHost* next, bool fine
       }
       rp[c + 1] = (int)val.size();
       max_row = std::max(max_row, rp[c + 1] - rp[c]);
+      min_row = std::min(min_row, rp[c + 1] - rp[c]);
     }
+    L.min_row = nc ? min_row : 0;
     std::vector<T> cd(nc);
     for (size_t k = 0; k < nc; ++k) cd[k] = (T)h.cdinv[k];
     L.rxs = 0;
@@ -1076,6 +1272,17 @@ static int upload_level(const MGLevelHost& h, const MGLevelHost* next, bool fine
     // row length; past radius 3 it costs more than the launches it saves).
     // The tile plan keeps the entry offsets to place them in its LDS region.
     const bool wide = !h.cls32.empty();
+    {  // the most frequent row, whose entries the row passes load speculatively
+      std::vector<int> count(nc, 0);
+      if (wide) {
+        for (uint32_t c : h.cls32) ++count[c];
+      } else {
+        for (uint16_t c : h.cls) ++count[c];
+      }
+      const int sc = (int)(std::max_element(count.begin(), count.end()) - count.begin());
+      L.spec_e0 = rp[sc];
+      L.spec_e1 = rp[sc + 1];
+    }
     if (next && fused_ok && !wide && std::max(L.rxs, L.rys) <= kFuseMaxRadius) {
       L.hdi = edi;
       L.hdj = edj;
@@ -1214,6 +1421,16 @@ int PoissonEngine::configure(const PoissonConfig& cfg, const std::vector<MGLevel
                    : upload_level<double>(h, next, l == 0, cfg.sweeps, cfg.fused != 0, L, s_);
       if (rc) return rc;
       L.lanes = L.max_row <= 16 ? 1 : L.max_row <= 64 ? 4 : L.max_row <= 256 ? 16 : 64;
+      // register-resident passes: the fewest lanes that hold a row in kRowNE
+      // entries each (0: rows too long, the looping passes run the level)
+      L.row_lanes = 0;
+      if (l > 0 && cfg.row_reg >= 0 && L.min_row >= 1 && (L.fx == 1 || L.fx == 2) &&
+          (L.fy == 1 || L.fy == 2))
+        for (int lanes : {1, 2, 4, 8, 16, 32, 64})
+          if (L.max_row <= lanes * kRowNE) {
+            L.row_lanes = lanes;
+            break;
+          }
       dst.push_back(L);
     }
     const int rc = f32 ? plan_level_tiles<float>(dst) : plan_level_tiles<double>(dst);
@@ -1838,15 +2055,49 @@ static ClassLevel<T> class_level(const LevelBuf& L, int px, int py) {
   D.fx = L.fx;
   D.fy = L.fy;
   D.cny = (L.ny + L.fy - 1) / L.fy;
+  D.spec_e0 = L.spec_e0;
+  D.spec_e1 = L.spec_e1;
   return D;
 }
 
-// Launches one row operation on a coarse level with the lane count chosen
-// for its widest row.
+template <typename T, int OP, bool PER>
+static void cls_rows_reg(hipStream_t s, int reg_lanes, const ClassLevel<T>& L, T w, const T* b,
+                         const T* xin, const T* xc, T* o1, T* o2, T* o3, const SolveState* st,
+                         const LaunchClock& clk) {
+  const dim3 grid((unsigned)(((long)L.n * reg_lanes + kBlock - 1) / kBlock));
+#define PCFD_ROWS_REG(LPR)                                                                 \
+  case LPR:                                                                                \
+    PCFD_LAUNCH_TIMED(clk, (k_cls_rows_reg<T, LPR, OP, PER>), grid, dim3(kBlock), 0, s, L, w, \
+                      b, xin, xc, o1, o2, o3, st);                                         \
+    break;
+  switch (reg_lanes) {
+    PCFD_ROWS_REG(1)
+    PCFD_ROWS_REG(2)
+    PCFD_ROWS_REG(4)
+    PCFD_ROWS_REG(8)
+    PCFD_ROWS_REG(16)
+    PCFD_ROWS_REG(32)
+    PCFD_ROWS_REG(64)
+    default:
+      break;
+  }
+#undef PCFD_ROWS_REG
+}
+
+// Launches one row operation on a coarse level: the register-resident kernel
+// when the level's rows fit it (reg_lanes > 0), else the looping one with the
+// lane count chosen for its widest row.
 template <typename T, int OP>
-static void cls_rows(hipStream_t s, int lanes, const ClassLevel<T>& L, T w, const T* b,
-                     const T* xin, const T* xc, const int* agg, T* o1, T* o2, T* o3,
+static void cls_rows(hipStream_t s, int lanes, int reg_lanes, const ClassLevel<T>& L, T w,
+                     const T* b, const T* xin, const T* xc, const int* agg, T* o1, T* o2, T* o3,
                      const SolveState* st, const LaunchClock& clk = LaunchClock{}) {
+  if (reg_lanes > 0) {
+    if (L.px | L.py)
+      cls_rows_reg<T, OP, true>(s, reg_lanes, L, w, b, xin, xc, o1, o2, o3, st, clk);
+    else
+      cls_rows_reg<T, OP, false>(s, reg_lanes, L, w, b, xin, xc, o1, o2, o3, st, clk);
+    return;
+  }
   const long threads = (long)L.n * lanes;
   const dim3 grid((unsigned)((threads + kBlock - 1) / kBlock));
   switch (lanes) {
@@ -1906,12 +2157,44 @@ static BoxRows<T> box_rows(const LevelBuf& L, int px, int py) {
   return D;
 }
 
+template <typename T, bool PER>
+static void cls_restrict_reg(hipStream_t s, int reg_lanes, const ClassLevel<T>& L, T w,
+                             const T* r, const T* t, int fx, int fy, int cnx, int cny, T* bc,
+                             const SolveState* st, const LaunchClock& clk) {
+  const long cells = (long)cnx * cny;
+  auto grid = [&](int per) { return dim3((unsigned)((cells * per + kBlock - 1) / kBlock)); };
+#define PCFD_RESTRICT_REG(LPR)                                                               \
+  case LPR:                                                                                  \
+    PCFD_LAUNCH_TIMED(clk, (k_cls_restrict_reg<T, LPR, PER>), grid(4 * LPR), dim3(kBlock), 0, \
+                      s, L, w, r, t, fx, fy, cnx, cny, bc, st);                              \
+    break;
+  switch (reg_lanes) {
+    PCFD_RESTRICT_REG(1)
+    PCFD_RESTRICT_REG(2)
+    PCFD_RESTRICT_REG(4)
+    PCFD_RESTRICT_REG(8)
+    PCFD_RESTRICT_REG(16)
+    PCFD_RESTRICT_REG(32)
+    PCFD_RESTRICT_REG(64)
+    default:
+      break;
+  }
+#undef PCFD_RESTRICT_REG
+}
+
 template <typename T>
-static void cls_restrict(hipStream_t s, int lanes, const ClassLevel<T>& L, T w, const T* r,
-                         const T* t, int fx, int fy, int cnx, int cny, T* bc,
+static void cls_restrict(hipStream_t s, int lanes, int reg_lanes, const ClassLevel<T>& L, T w,
+                         const T* r, const T* t, int fx, int fy, int cnx, int cny, T* bc,
                          const SolveState* st, const LaunchClock& clk = LaunchClock{}) {
   const long cells = (long)cnx * cny;
   auto grid = [&](int per) { return dim3((unsigned)((cells * per + kBlock - 1) / kBlock)); };
+  if (reg_lanes > 0) {
+    if (L.px | L.py)
+      cls_restrict_reg<T, true>(s, reg_lanes, L, w, r, t, fx, fy, cnx, cny, bc, st, clk);
+    else
+      cls_restrict_reg<T, false>(s, reg_lanes, L, w, r, t, fx, fy, cnx, cny, bc, st, clk);
+    return;
+  }
   switch (lanes) {
     case 1:
       PCFD_LAUNCH_TIMED(clk, (k_cls_restrict<T, 1>), grid(4), dim3(kBlock), 0, s, L, w, r, t, fx,
@@ -1956,22 +2239,23 @@ int PoissonEngine::level_down(LevelBuf& L, LevelBuf& C, const SolveState* cst, i
                                 clock_next(CK_CLASS_DOWN, lev, n * (2 * ts + cb) + nc * ts));
   }
   const ClassLevel<T> D = class_level<T>(L, op_.px, op_.py);
-  cls_rows<T, RO_SMOOTH0_RES>(s_, L.lanes, D, w, (const T*)L.b, nullptr, nullptr, nullptr,
+  const int la = L.lanes, rl = L.row_lanes;
+  cls_rows<T, RO_SMOOTH0_RES>(s_, la, rl, D, w, (const T*)L.b, nullptr, nullptr, nullptr,
                               (T*)L.x, (T*)L.r, (T*)L.t, cst,
                               clock_next(CK_ROWS_SMOOTH0, lev, n * (4 * ts + cb)));
   if (nu > 1) {
     for (int sw = 1; sw < nu; ++sw) {
-      cls_rows<T, RO_SMOOTH>(s_, L.lanes, D, w, (const T*)L.b, (const T*)L.x, nullptr, nullptr,
+      cls_rows<T, RO_SMOOTH>(s_, la, rl, D, w, (const T*)L.b, (const T*)L.x, nullptr, nullptr,
                              (T*)L.y, nullptr, nullptr, cst,
                              clock_next(CK_ROWS_SMOOTH, lev, n * (3 * ts + cb)));
       std::swap(L.x, L.y);
     }
-    cls_rows<T, RO_RES>(s_, L.lanes, D, w, (const T*)L.b, (const T*)L.x, nullptr, nullptr,
+    cls_rows<T, RO_RES>(s_, la, rl, D, w, (const T*)L.b, (const T*)L.x, nullptr, nullptr,
                         nullptr, (T*)L.r, (T*)L.t, cst,
                         clock_next(CK_ROWS_RES, lev, n * (4 * ts + cb)));
   }
   // b_c = P^T r in one pass over the coarse cells
-  cls_restrict<T>(s_, L.lanes, D, w, (const T*)L.r, (const T*)L.t, L.fx, L.fy, C.nx, C.ny,
+  cls_restrict<T>(s_, la, rl, D, w, (const T*)L.r, (const T*)L.t, L.fx, L.fy, C.nx, C.ny,
                   (T*)C.b, cst, clock_next(CK_RESTRICT, lev, n * (2 * ts + cb) + nc * ts));
   PCFD_LAUNCH_CHECK();
   return kOk;
@@ -1997,11 +2281,12 @@ int PoissonEngine::level_up(LevelBuf& L, LevelBuf& C, const SolveState* cst, int
                               clock_next(CK_CLASS_UP, lev, n * (3 * ts + cb) + nc * ts));
   }
   const ClassLevel<T> D = class_level<T>(L, op_.px, op_.py);
-  cls_rows<T, RO_PROLONG>(s_, L.lanes, D, w, nullptr, nullptr, (const T*)C.y, L.agg, (T*)L.x,
+  const int la = L.lanes, rl = L.row_lanes;
+  cls_rows<T, RO_PROLONG>(s_, la, rl, D, w, nullptr, nullptr, (const T*)C.y, L.agg, (T*)L.x,
                           nullptr, nullptr, cst,
                           clock_next(CK_ROWS_PROLONG, lev, n * (2 * ts + 4 + cb) + nc * ts));
   for (int sw = 0; sw < nu; ++sw) {
-    cls_rows<T, RO_SMOOTH>(s_, L.lanes, D, w, (const T*)L.b, (const T*)L.x, nullptr, nullptr,
+    cls_rows<T, RO_SMOOTH>(s_, la, rl, D, w, (const T*)L.b, (const T*)L.x, nullptr, nullptr,
                            (T*)L.y, nullptr, nullptr, cst,
                            clock_next(CK_ROWS_SMOOTH, lev, n * (3 * ts + cb)));
     std::swap(L.x, L.y);

@@ -63,6 +63,7 @@ struct ClassLevel {
   const T* rdinv;
   const uint8_t* ract;
   int fx, fy, cny;  // aggregation towards the next level (prolongation index)
+  int spec_e0, spec_e1;  // entry range of the level's most frequent row
 };
 
 // Device-resident scalars of one solve.  Kernels read them directly so the
@@ -129,6 +130,7 @@ struct PoissonConfig {
   int tile_ty = 0;
   int chain = 0;    // cooperative coarse chain: 0 measured, -1 never, k >= 1 from level k
   int stream = 0;   // level-0 streaming legs: 0 measured, -1 never, k > 0 with k rows
+  int row_reg = 0;  // stored-level row passes: 0 register-resident where rows fit, -1 looping
 };
 
 struct SolveInfo {
@@ -142,7 +144,7 @@ struct SolveInfo {
 struct LevelBuf {
   int nx = 0, ny = 0, n = 0, fx = 1, fy = 1;
   void *b = nullptr, *x = nullptr, *y = nullptr, *r = nullptr, *t = nullptr;
-  int ncls = 0, nnz = 0, max_row = 0, lanes = 1;
+  int ncls = 0, nnz = 0, max_row = 0, min_row = 0, lanes = 1;
   uint16_t* cls = nullptr;
   uint32_t* cls32 = nullptr;  // wide class index (never run through the fused kernels)
   int2* rr = nullptr;         // per cell: entry range of its class row
@@ -175,6 +177,10 @@ struct LevelBuf {
   std::vector<int8_t> hdi, hdj;  // host copy of the entry offsets (tile planning)
   int hot = -1, hot_n = 0;       // most frequent class and its entry count
   int hot_e0 = 0;                // its first entry
+  // Register-resident row passes: lanes per row (0: the looping passes run
+  // the level) and the entry range of the level's most frequent row.
+  int row_lanes = 0;
+  int spec_e0 = 0, spec_e1 = 0;
 };
 
 class PoissonEngine {

@@ -92,31 +92,35 @@ def test_operator_apply(gpu_available, name):
     assert np.abs(y.cpu().numpy() - ref).max() <= 1e-14 * np.abs(ref).max()
 
 
-def _mg_solver(name, fused=True, chain=-1, stream=0, **kw):
+def _mg_solver(name, fused=True, chain=-1, stream=0, rows=0, **kw):
     """fused: stored levels through the fused kernels where they fit; chain:
     the cooperative launch of the deepest levels (-1 never, k from level k);
     stream: level 0's Krylov-start legs in streaming form (0 measured, -1
-    never, k rows per wavefront)."""
+    never, k rows per wavefront); rows: pass-by-pass stored levels with the
+    register-resident row kernels (0) or the looping ones (-1)."""
     s = _solver(name, pressure_solver=kw.pop("pressure_solver", "mgcg"), **kw)
     ps = s.pressure_solver
-    if not fused or chain != 0 or stream != 0:
+    if not fused or chain != 0 or stream != 0 or rows != 0:
         ps.fused_levels = bool(fused)
         ps.coarse_chain = chain
         ps.stream_rows = stream
+        ps.row_passes = rows
         ps.configure()
     return s
 
 
 @pytest.mark.parametrize("name", ["cavity", "periodic", "cyl_outlet", "cyl_pout"])
 @pytest.mark.parametrize("sweeps", [1, 2])
-@pytest.mark.parametrize("fused", [True, False])
-def test_vcycle_matches_reference(gpu_available, name, sweeps, fused):
+@pytest.mark.parametrize("fused,rows", [(True, 0), (False, 0), (False, -1)],
+                         ids=["fused", "passes", "passes_loop"])
+def test_vcycle_matches_reference(gpu_available, name, sweeps, fused, rows):
     """z = M r: one V-cycle of the reference hierarchy (coarsest level at 96
-    unknowns) with projections, FP64, vs the reference's, 1e-10."""
+    unknowns) with projections, FP64, vs the reference's, 1e-10: fused stored
+    levels, pass-by-pass with register-resident rows, and with looping rows."""
     import torch
     g = golden("multigrid")
     tag = f"{name}_s{sweeps}"
-    s = _mg_solver(name, fused, -1, mg_sweeps=sweeps, mg_coarse_max=96)
+    s = _mg_solver(name, fused, -1, rows=rows, mg_sweeps=sweeps, mg_coarse_max=96)
     r = torch.as_tensor(g[f"{tag}_r"]).cuda()
     z = torch.empty_like(r)
     s.ctx.mg_apply(r, z)
@@ -129,8 +133,10 @@ VCYCLE_GRIDS = [("cavity", {}), ("periodic", {}), ("cyl_outlet", {}), ("cyl_pout
                 ("periodic", dict(nx=128, ny=96)), ("cavity", dict(nx=37, ny=23))]
 
 
-CYCLE_MODES = {"fused": (True, -1, -1), "passes": (False, -1, -1), "chain1": (True, 1, -1),
-               "chain2": (True, 2, -1), "stream8": (True, -1, 8), "stream16": (True, -1, 16)}
+CYCLE_MODES = {"fused": (True, -1, -1, 0), "passes": (False, -1, -1, 0),
+               "passes_loop": (False, -1, -1, -1), "chain1": (True, 1, -1, 0),
+               "chain2": (True, 2, -1, 0), "stream8": (True, -1, 8, 0),
+               "stream16": (True, -1, 16, 0)}
 
 
 @pytest.mark.parametrize("name,grid", VCYCLE_GRIDS,
@@ -142,9 +148,10 @@ def test_vcycle_matches_oracle(gpu_available, name, grid, coarse_max, sweeps, mo
     """One V-cycle with the coarsest solve at the first level of at most
     coarse_max unknowns, vs the oracle's cycle on the same hierarchy:
     FP64 to 1e-10, the FP32 cycle to 1e-4 (relative to max|z|).  Modes: fused
-    level kernels, pass-by-pass levels, the cooperative chain from level 1 or 2,
-    level 0's streaming legs (8 or 16 rows per wavefront; one sweep only)."""
-    fused, chain, stream = CYCLE_MODES[mode]
+    level kernels, pass-by-pass levels (register-resident or looping row
+    kernels), the cooperative chain from level 1 or 2, level 0's streaming
+    legs (8 or 16 rows per wavefront; one sweep only)."""
+    fused, chain, stream, rows = CYCLE_MODES[mode]
     if stream > 0 and sweeps != 1:
         pytest.skip("the streaming legs serve one sweep each way")
     import torch
@@ -156,8 +163,8 @@ def test_vcycle_matches_oracle(gpu_available, name, grid, coarse_max, sweeps, mo
     r = rng.standard_normal(S.A.shape[0])
     ref = H.apply(r)
     for precision, tol in (("fp64", 1e-10), ("mixed", 1e-4)):
-        s = _mg_solver(name, fused, chain, stream, mg_sweeps=sweeps, mg_coarse_max=coarse_max,
-                       precision=precision, **grid)
+        s = _mg_solver(name, fused, chain, stream, rows, mg_sweeps=sweeps,
+                       mg_coarse_max=coarse_max, precision=precision, **grid)
         plan = s.ctx.mg_plan()
         if stream > 0 and len(plan) > 1:  # a one-level hierarchy is the dense solve alone
             assert plan[0]["mode"] == "stream"
