@@ -802,23 +802,75 @@ struct ClassLds {
   }
 };
 
-// Row at region cell idx: the hot row from LDS (the common case: all lanes
-// of an interior wavefront share it), any other row from the global dictionary.
-template <typename T>
-__device__ __forceinline__ T class_row_lds(const ClassRows<T>& L, const ClassLds<T>& M, int2 rr,
-                                           const T* X, int idx) {
-  if (L.hot < 0 || rr.x != L.hot_e0) return class_row(L, rr, X, idx);
-  T s0 = (T)0, s1 = (T)0, s2 = (T)0, s3 = (T)0;
-  const int n = L.hot_n;
-  int e = 0;
-  for (; e + 3 < n; e += 4) {
-    s0 += M.HV[e] * X[idx + M.HO[e]];
-    s1 += M.HV[e + 1] * X[idx + M.HO[e + 1]];
-    s2 += M.HV[e + 2] * X[idx + M.HO[e + 2]];
-    s3 += M.HV[e + 3] * X[idx + M.HO[e + 3]];
+// (A X) at every cell of ring g, handed to out(a, c, idx, rr, ax) (rr.x < 0:
+// a cell outside the grid, ax = 0).  A thread takes the cells q = tid,
+// tid + 256, ... in chunks of kRingRows and runs the hot row on all of a
+// chunk in lockstep: one broadcast LDS read of each hot coefficient and
+// offset serves every row, and the rows' X reads are independent.  Rows of
+// another class then take the global dictionary.  Every X read stays inside
+// the region (the ring lies at least one stencil radius inside it), so the
+// hot sums run unpredicated and are used only by hot rows.
+constexpr int kRingRows = 1;
+
+template <typename T, class Out>
+__device__ __forceinline__ void ring_rows(const ClassRows<T>& L, const ClassLds<T>& M,
+                                          const Ring& g, int ry, const T* X, Out&& out) {
+  const int w = g.c1 - g.c0;
+  const int n = (g.a1 - g.a0) * w;
+  if (n <= 0) return;
+  const float inv = 1.0f / (float)w;
+  const int hn = L.hot < 0 ? 0 : L.hot_n;
+  for (int q0 = 0; q0 < n; q0 += kRingRows * kBlock) {
+    int ra[kRingRows], rc[kRingRows], idx[kRingRows];
+    int2 rr[kRingRows];
+    T s[kRingRows][4];
+#pragma unroll
+    for (int r = 0; r < kRingRows; ++r) {
+      const int q = min(q0 + r * kBlock + (int)threadIdx.x, n - 1);
+      int a = (int)(((float)q + 0.5f) * inv);
+      int c = q - a * w;
+      if (c < 0) {
+        --a;
+        c += w;
+      } else if (c >= w) {
+        ++a;
+        c -= w;
+      }
+      ra[r] = g.a0 + a;
+      rc[r] = g.c0 + c;
+      idx[r] = ra[r] * ry + rc[r];
+      rr[r] = M.RR[idx[r]];
+#pragma unroll
+      for (int p = 0; p < 4; ++p) s[r][p] = (T)0;
+    }
+    int e = 0;
+    for (; e + 3 < hn; e += 4) {
+      const int o0 = M.HO[e], o1 = M.HO[e + 1], o2 = M.HO[e + 2], o3 = M.HO[e + 3];
+      const T v0 = M.HV[e], v1 = M.HV[e + 1], v2 = M.HV[e + 2], v3 = M.HV[e + 3];
+#pragma unroll
+      for (int r = 0; r < kRingRows; ++r) {
+        s[r][0] += v0 * X[idx[r] + o0];
+        s[r][1] += v1 * X[idx[r] + o1];
+        s[r][2] += v2 * X[idx[r] + o2];
+        s[r][3] += v3 * X[idx[r] + o3];
+      }
+    }
+    for (; e < hn; ++e) {
+      const int o = M.HO[e];
+      const T v = M.HV[e];
+#pragma unroll
+      for (int r = 0; r < kRingRows; ++r) s[r][0] += v * X[idx[r] + o];
+    }
+#pragma unroll
+    for (int r = 0; r < kRingRows; ++r) {
+      if (q0 + r * kBlock + (int)threadIdx.x >= n) break;
+      T ax = (T)0;
+      if (rr[r].x >= 0)
+        ax = (hn > 0 && rr[r].x == L.hot_e0) ? (s[r][0] + s[r][1]) + (s[r][2] + s[r][3])
+                                             : class_row(L, rr[r], X, idx[r]);
+      out(ra[r], rc[r], idx[r], rr[r], ax);
+    }
   }
-  for (; e < n; ++e) s0 += M.HV[e] * X[idx + M.HO[e]];
-  return (s0 + s1) + (s2 + s3);
 }
 
 template <typename T>
@@ -883,25 +935,18 @@ k_class_down(ClassRows<T> L, FusedTile ft, int rxs, int rys, T w, int nu, const 
       });
   __syncthreads();
   for (int s = 1; s < nu; ++s) {
-    for_ring(ring(ft, nu + 1 - s, rxs, rys), [&](int a, int c) {
-      const int idx = a * ry + c;
-      const int2 rr = RR[idx];
-      if (rr.x < 0) {
-        Y[idx] = (T)0;
-        return;
-      }
-      Y[idx] = X[idx] + w * D[idx] * (B[idx] - class_row_lds(L, M, rr, X, idx));
-    });
+    ring_rows(L, M, ring(ft, nu + 1 - s, rxs, rys), ry, X,
+              [&](int, int, int idx, int2 rr, T ax) {
+                Y[idx] = (rr.x < 0) ? (T)0 : X[idx] + w * D[idx] * (B[idx] - ax);
+              });
     __syncthreads();
     T* t = X;
     X = Y;
     Y = t;
   }
   // Residual on ring 1 into Y.
-  for_ring(ring(ft, 1, rxs, rys), [&](int a, int c) {
-    const int idx = a * ry + c;
-    const int2 rr = RR[idx];
-    Y[idx] = (rr.x < 0) ? (T)0 : B[idx] - class_row_lds(L, M, rr, X, idx);
+  ring_rows(L, M, ring(ft, 1, rxs, rys), ry, X, [&](int, int, int idx, int2 rr, T ax) {
+    Y[idx] = (rr.x < 0) ? (T)0 : B[idx] - ax;
   });
   __syncthreads();
   for_ring(ring(ft, 1, rxs, rys), [&](int a, int c) {
@@ -912,6 +957,12 @@ k_class_down(ClassRows<T> L, FusedTile ft, int rxs, int rys, T w, int nu, const 
       if (i < L.nx && j < L.ny) x_out[i * L.ny + j] = X[idx];
     }
     X[idx] = D[idx] * Y[idx];  // zero outside the grid (D = 0 there)
+  });
+  __syncthreads();
+  // (r - w A t) on the tile's rows into B (free now), then each coarse cell
+  // sums its block's rows in x-major order.
+  ring_rows(L, M, ring(ft, 0, rxs, rys), ry, X, [&](int, int, int idx, int2, T ax) {
+    B[idx] = A[idx] ? Y[idx] - w * ax : (T)0;
   });
   __syncthreads();
   const int ctx = ft.tx / fx, cty = ft.ty / fy;
@@ -926,8 +977,7 @@ k_class_down(ClassRows<T> L, FusedTile ft, int rxs, int rys, T w, int nu, const 
       for (int c2 = 0; c2 < fy; ++c2) {
         const int j = J * fy + c2;
         if (j >= L.ny) break;
-        const int idx = (ft.hx + ca * fx + a2) * ry + (ft.hy + cc * fy + c2);
-        acc += A[idx] ? Y[idx] - w * class_row_lds(L, M, RR[idx], X, idx) : (T)0;
+        acc += B[(ft.hx + ca * fx + a2) * ry + (ft.hy + cc * fy + c2)];
       }
     }
     bc[I * cny + J] = acc;
@@ -985,21 +1035,15 @@ k_class_up(ClassRows<T> L, FusedTile ft, int rxs, int rys, T w, int nu, const T*
         E[idx] = v.act ? v.ev : (T)0;
       });
   __syncthreads();
-  for_ring(ring(ft, nu, rxs, rys), [&](int a, int c) {
-    const int idx = a * ry + c;
-    const int2 rr = RR[idx];
-    if (rr.x < 0) return;
-    X[idx] = X[idx] + E[idx] - w * D[idx] * class_row_lds(L, M, rr, E, idx);
+  ring_rows(L, M, ring(ft, nu, rxs, rys), ry, E, [&](int, int, int idx, int2 rr, T ax) {
+    if (rr.x >= 0) X[idx] = X[idx] + E[idx] - w * D[idx] * ax;
   });
   __syncthreads();
   T* Y = E;
   for (int sw = 1; sw <= nu; ++sw) {
     const bool last = sw == nu;
-    for_ring(ring(ft, nu - sw, rxs, rys), [&](int a, int c) {
-      const int idx = a * ry + c;
-      const int2 rr = RR[idx];
-      T val = (T)0;
-      if (rr.x >= 0) val = X[idx] + w * D[idx] * (B[idx] - class_row_lds(L, M, rr, X, idx));
+    ring_rows(L, M, ring(ft, nu - sw, rxs, rys), ry, X, [&](int a, int c, int idx, int2 rr, T ax) {
+      const T val = (rr.x >= 0) ? X[idx] + w * D[idx] * (B[idx] - ax) : (T)0;
       if (!last) {
         Y[idx] = val;
         return;
