@@ -1061,26 +1061,45 @@ k_class_up(ClassRows<T> L, FusedTile ft, int rxs, int rys, T w, int nu, const T*
 }
 
 // ---------------------------------------------------------------------------
-// Dense coarsest-level operator, one wavefront per row.
+// Dense coarsest-level operator, one workgroup per row: each thread loads its
+// kDensePer entries of the row (and of b, an L1/L2-resident vector) in one
+// batch, so a row costs one memory round trip for rows up to
+// kBlock * kDensePer entries.
 // ---------------------------------------------------------------------------
+constexpr int kDensePer = 8;
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_dense_apply(const double* __restrict__ C, int n, const T* __restrict__ b, T* __restrict__ x,
               const SolveState* __restrict__ st) {
   if (st->done) return;
-  extern __shared__ double sb[];
-  for (int k = threadIdx.x; k < n; k += kBlock) sb[k] = (double)b[k];
-  __syncthreads();
+  __shared__ double part[kBlock / kWave];
   const int lane = threadIdx.x % kWave;
   const int wv = threadIdx.x / kWave;
-  constexpr int kWaves = kBlock / kWave;
-  for (int row = blockIdx.x * kWaves + wv; row < n; row += gridDim.x * kWaves) {
+  for (int row = blockIdx.x; row < n; row += gridDim.x) {
     const double* c = C + (size_t)row * n;
     double acc = 0.0;
-#pragma unroll 8
-    for (int k = lane; k < n; k += kWave) acc += c[k] * sb[k];
+    for (int k0 = 0; k0 < n; k0 += kBlock * kDensePer) {
+      double cv[kDensePer], bv[kDensePer];
+#pragma unroll
+      for (int t = 0; t < kDensePer; ++t) {
+        const int k = min(k0 + t * kBlock + (int)threadIdx.x, n - 1);
+        cv[t] = c[k];
+        bv[t] = (double)b[k];
+      }
+#pragma unroll
+      for (int t = 0; t < kDensePer; ++t)
+        if (k0 + t * kBlock + (int)threadIdx.x < n) acc += cv[t] * bv[t];
+    }
     acc = wave_sum(acc);
-    if (lane == 0) x[row] = (T)acc;
+    if (lane == 0) part[wv] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double tot = part[0];
+      for (int q = 1; q < kBlock / kWave; ++q) tot += part[q];
+      x[row] = (T)tot;
+    }
+    __syncthreads();
   }
 }
 
@@ -1435,10 +1454,8 @@ int launch_fine_dir(hipStream_t s, const FineOp& A, const FusedTile& ft, const T
 template <typename T>
 int launch_dense_apply(hipStream_t s, const double* C, int n, const T* b, T* x,
                        const SolveState* st, const LaunchClock& clk) {
-  constexpr int kWaves = kBlock / kWave;
-  const int blocks = std::max(1, std::min((n + kWaves - 1) / kWaves, 1024));
-  PCFD_LAUNCH_TIMED(clk, k_dense_apply<T>, dim3(blocks), dim3(kBlock), (size_t)n * sizeof(double),
-                    s, C, n, b, x, st);
+  const int blocks = std::max(1, std::min(n, 8192));
+  PCFD_LAUNCH_TIMED(clk, k_dense_apply<T>, dim3(blocks), dim3(kBlock), 0, s, C, n, b, x, st);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
