@@ -11,6 +11,8 @@
 // the reference (the reductions aside, which are deterministic tree sums).
 #include "internal.hpp"
 
+#include <algorithm>
+
 namespace pcfd {
 
 namespace {
@@ -101,51 +103,79 @@ __device__ double block_sum(double x) {
   return r;
 }
 
-// One wall's velocity condition (BoundaryCondition.apply, boundary.py:141-164).
-__device__ void apply_wall(const BCParams& P, int w, bool predictor, double* u, double* v) {
+// The five lines one wall's velocity condition touches and what it does
+// (BoundaryCondition.apply, boundary.py:141-164): the normal face line nf
+// (written when `face`), its ghost line ng from nf and the interior line nin,
+// the tangential ghost line tg from the interior tangential line tin.  Each
+// entry k of a line depends only on entry k of the others.
+struct WallLines {
+  Line nf, nin, ng, tg, tin;
+  int kind, w;
+  bool face;
+};
+
+__device__ __forceinline__ WallLines wall_lines(const BCParams& P, int w, bool predictor,
+                                                double* u, double* v) {
   const int kind = P.kind[w];
-  if (kind == BC_PERIODIC) return;
   const WallIdx wi = wall_index(w, P.nx, P.ny);
   const int ur = P.nx + 3, uc = P.ny + 2, vr = P.nx + 2, vc = P.ny + 3;
-  Line nf, nin, ng, tg, tin;
+  WallLines L;
   if (wi.axis == 0) {
-    nf = line_of(u, P.ldu, ur, uc, 0, wi.n_face);
-    nin = line_of(u, P.ldu, ur, uc, 0, wi.n_in);
-    ng = line_of(u, P.ldu, ur, uc, 0, wi.n_ghost);
-    tg = line_of(v, P.ldv, vr, vc, 0, wi.t_ghost);
-    tin = line_of(v, P.ldv, vr, vc, 0, wi.t_in);
+    L.nf = line_of(u, P.ldu, ur, uc, 0, wi.n_face);
+    L.nin = line_of(u, P.ldu, ur, uc, 0, wi.n_in);
+    L.ng = line_of(u, P.ldu, ur, uc, 0, wi.n_ghost);
+    L.tg = line_of(v, P.ldv, vr, vc, 0, wi.t_ghost);
+    L.tin = line_of(v, P.ldv, vr, vc, 0, wi.t_in);
   } else {
-    nf = line_of(v, P.ldv, vr, vc, 1, wi.n_face);
-    nin = line_of(v, P.ldv, vr, vc, 1, wi.n_in);
-    ng = line_of(v, P.ldv, vr, vc, 1, wi.n_ghost);
-    tg = line_of(u, P.ldu, ur, uc, 1, wi.t_ghost);
-    tin = line_of(u, P.ldu, ur, uc, 1, wi.t_in);
+    L.nf = line_of(v, P.ldv, vr, vc, 1, wi.n_face);
+    L.nin = line_of(v, P.ldv, vr, vc, 1, wi.n_in);
+    L.ng = line_of(v, P.ldv, vr, vc, 1, wi.n_ghost);
+    L.tg = line_of(u, P.ldu, ur, uc, 1, wi.t_ghost);
+    L.tin = line_of(u, P.ldu, ur, uc, 1, wi.t_in);
   }
+  L.kind = kind;
+  L.w = w;
   const bool prescribes = kind != BC_PRESSURE_OUTLET;
   const bool derives = kind == BC_OUTLET;
-  if (prescribes && (predictor || !derives)) {
-    const double* inl = P.inlet[w];
-    line_map(
-        nf.len,
-        [&](int k) {
-          if (kind == BC_INLET) return inl[k];
-          return kind == BC_OUTLET ? nin[k] : 0.0;
-        },
-        [&](int k, double val) { nf[k] = val; });
+  L.face = prescribes && (predictor || !derives);
+  return L;
+}
+
+// INLET: the inlet line; OUTLET: the interior line; any other kind: zero.  The
+// one load goes through a selected pointer (the interior line when the wall
+// has no inlet array), so no branch surrounds it.
+__device__ __forceinline__ double face_value(const BCParams& P, const WallLines& L, int k) {
+  const bool inl = L.kind == BC_INLET;
+  const double* src = inl ? P.inlet[L.w] : L.nin.base;
+  const size_t stride = inl ? 1 : L.nin.stride;
+  const double val = src[(size_t)k * stride];
+  return (inl || L.kind == BC_OUTLET) ? val : 0.0;
+}
+
+__device__ __forceinline__ double ghost_value(const WallLines& L, int k) {
+  return (L.kind == BC_PRESSURE_OUTLET) ? L.nf[k] : 2.0 * L.nf[k] - L.nin[k];
+}
+
+__device__ __forceinline__ double tangential_value(const BCParams& P, const WallLines& L, int k) {
+  const double t = L.tin[k];
+  if (L.kind == BC_MOVING_WALL) return 2.0 * P.vel[L.w] - t;
+  if (L.kind == BC_NO_SLIP || L.kind == BC_INLET) return -t;
+  return t;  // OUTLET, PRESSURE_OUTLET, SYMMETRY: zero gradient
+}
+
+// One wall's velocity condition over the workgroup.
+__device__ void apply_wall(const BCParams& P, int w, bool predictor, double* u, double* v) {
+  if (P.kind[w] == BC_PERIODIC) return;
+  const WallLines L = wall_lines(P, w, predictor, u, v);
+  if (L.face) {
+    line_map(L.nf.len, [&](int k) { return face_value(P, L, k); },
+             [&](int k, double val) { L.nf[k] = val; });
     __syncthreads();
   }
-  line_map(
-      ng.len, [&](int k) { return (kind == BC_PRESSURE_OUTLET) ? nf[k] : 2.0 * nf[k] - nin[k]; },
-      [&](int k, double val) { ng[k] = val; });
-  line_map(
-      tg.len,
-      [&](int k) {
-        const double t = tin[k];
-        if (kind == BC_MOVING_WALL) return 2.0 * P.vel[w] - t;
-        if (kind == BC_NO_SLIP || kind == BC_INLET) return -t;
-        return t;  // OUTLET, PRESSURE_OUTLET, SYMMETRY: zero gradient
-      },
-      [&](int k, double val) { tg[k] = val; });
+  line_map(L.ng.len, [&](int k) { return ghost_value(L, k); },
+           [&](int k, double val) { L.ng[k] = val; });
+  line_map(L.tg.len, [&](int k) { return tangential_value(P, L, k); },
+           [&](int k, double val) { L.tg[k] = val; });
   __syncthreads();
 }
 
@@ -313,6 +343,38 @@ __device__ void mask_obstacle(const BCParams& P, int mode, double dt, double* u,
 
 }  // namespace
 
+// The velocity condition of all four walls (no periodic axis) over many
+// workgroups.  Walls apply in order 0..3 and meet only at the corners: a
+// later wall overwrites, or reads, entries an earlier one wrote within the
+// first or last kWallCorner entries of its lines.  Those entries run on one
+// thread in the serial order (wall by wall; per wall the face line, then the
+// ghost lines); every other entry k of a wall touches entry k of that
+// wall's lines only, nothing any other wall reads or writes, and runs on its
+// own thread (blockIdx.y = wall).
+constexpr int kWallCorner = 4;
+
+__global__ void __launch_bounds__(kBlock)
+k_walls(BCParams P, int predictor, double* u, double* v) {
+  auto corner = [](int k, int len) { return k < kWallCorner || k >= len - kWallCorner; };
+  const int w = blockIdx.y;
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  const WallLines L = wall_lines(P, w, predictor != 0, u, v);
+  if (L.face && k < L.nf.len && !corner(k, L.nf.len)) L.nf[k] = face_value(P, L, k);
+  if (k < L.ng.len && !corner(k, L.ng.len)) L.ng[k] = ghost_value(L, k);
+  if (k < L.tg.len && !corner(k, L.tg.len)) L.tg[k] = tangential_value(P, L, k);
+  if (blockIdx.x != 0 || blockIdx.y != 0 || threadIdx.x != 0) return;
+  for (int ww = 0; ww < 4; ++ww) {
+    const WallLines C = wall_lines(P, ww, predictor != 0, u, v);
+    auto each_corner = [&](int len, auto&& fn) {
+      for (int q = 0; q < len && q < kWallCorner; ++q) fn(q);
+      for (int q = max(len - kWallCorner, kWallCorner); q < len; ++q) fn(q);
+    };
+    if (C.face) each_corner(C.nf.len, [&](int q) { C.nf[q] = face_value(P, C, q); });
+    each_corner(C.ng.len, [&](int q) { C.ng[q] = ghost_value(C, q); });
+    each_corner(C.tg.len, [&](int q) { C.tg[q] = tangential_value(P, C, q); });
+  }
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_boundary_pass(BCParams P, BCOps ops, double dt, double* u, double* v) {
   for (int q = 0; q < ops.n; ++q) {
@@ -372,10 +434,33 @@ k_pressure_ghosts(BCParams P, double* p, int ldp) {
   }
 }
 
+// Without a periodic axis (and on a grid wide enough that opposite walls
+// touch disjoint lines) the wall conditions run on the whole device
+// (k_walls); the rest of the pass (O(perimeter) lines and reductions) stays
+// on one workgroup, in the pass's order.
 int launch_boundary_pass(hipStream_t s, const BCParams& P, const BCOps& ops, double dt, double* u,
                          double* v) {
   if (ops.n == 0) return kOk;
-  hipLaunchKernelGGL(k_boundary_pass, dim3(1), dim3(kBlock), 0, s, P, ops, dt, u, v);
+  const bool walls_apart = !P.periodic_x && !P.periodic_y && P.nx >= 2 * kWallCorner &&
+                           P.ny >= 2 * kWallCorner;
+  BCOps rest{};
+  auto flush = [&]() {
+    if (rest.n == 0) return;
+    hipLaunchKernelGGL(k_boundary_pass, dim3(1), dim3(kBlock), 0, s, P, rest, dt, u, v);
+    rest.n = 0;
+  };
+  for (int q = 0; q < ops.n; ++q) {
+    const int op = ops.op[q];
+    if (walls_apart && (op == BOP_VEL_PRED || op == BOP_VEL_CORR)) {
+      flush();
+      const int len = std::max(P.nx, P.ny) + 3;
+      hipLaunchKernelGGL(k_walls, dim3((len + kBlock - 1) / kBlock, 4), dim3(kBlock), 0, s, P,
+                         op == BOP_VEL_PRED ? 1 : 0, u, v);
+    } else if (op != BOP_WRAP_FACES || !walls_apart) {  // face wrap: periodic axes only
+      rest.op[rest.n++] = op;
+    }
+  }
+  flush();
   PCFD_LAUNCH_CHECK();
   return kOk;
 }

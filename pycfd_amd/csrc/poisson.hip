@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -1476,6 +1477,47 @@ bool autotune_enabled() {
   const char* e = std::getenv("PCFD_AUTOTUNE");
   return !(e && e[0] == '0');
 }
+
+// Optional persistence of the measured choices across processes: with
+// PCFD_TUNE_FILE set, the file's entries (one line of 14 key integers and
+// 4 choice integers each) seed the cache on first use and every new
+// measurement is appended, so later runs of the same workload (a profiled
+// run, say) take the same plan without measuring.  Caller holds g_tune_mutex.
+const char* tune_file() {
+  const char* f = std::getenv("PCFD_TUNE_FILE");
+  return (f && f[0]) ? f : nullptr;
+}
+
+void tune_file_load() {
+  static bool loaded = false;
+  if (loaded) return;
+  loaded = true;
+  const char* f = tune_file();
+  if (!f) return;
+  FILE* fp = std::fopen(f, "r");
+  if (!fp) return;
+  for (;;) {
+    TuneKey k{};
+    TunePick p{};
+    int got = 0;
+    for (int& v : k) got += std::fscanf(fp, "%d", &v) == 1;
+    got += std::fscanf(fp, "%d %d %d %d", &p.fused, &p.tx, &p.ty, &p.stream) == 4 ? 4 : 0;
+    if (got != (int)k.size() + 4) break;
+    g_tune_cache[k] = p;
+  }
+  std::fclose(fp);
+}
+
+void tune_store(const TuneKey& k, const TunePick& p) {
+  g_tune_cache[k] = p;
+  const char* f = tune_file();
+  if (!f || !autotune_enabled()) return;  // only measured choices persist
+  FILE* fp = std::fopen(f, "a");
+  if (!fp) return;
+  for (int v : k) std::fprintf(fp, "%d ", v);
+  std::fprintf(fp, "%d %d %d %d\n", p.fused, p.tx, p.ty, p.stream);
+  std::fclose(fp);
+}
 }  // namespace
 
 // Tiles of the level-l legs.  Every tiling gives the same arithmetic (each
@@ -1598,6 +1640,7 @@ int PoissonEngine::tune_level(std::vector<LevelBuf>& lv, int l, SolveState* tst)
                        op_.px, op_.py, ftx, fty, op_.hxc != nullptr};
   {
     std::lock_guard<std::mutex> g(g_tune_mutex);
+    tune_file_load();
     auto it = g_tune_cache.find(key);
     if (it != g_tune_cache.end()) {
       const TunePick pk = it->second;
@@ -1673,7 +1716,7 @@ int PoissonEngine::tune_level(std::vector<LevelBuf>& lv, int l, SolveState* tst)
     std::lock_guard<std::mutex> g(g_tune_mutex);
     TunePick pk{best_fused, best.tx, best.ty};
     pk.stream = best_stream;
-    if (cfg_.stream == 0) g_tune_cache[key] = pk;
+    if (cfg_.stream == 0) tune_store(key, pk);
   }
   return apply(best_fused, best, best_stream);
 }
@@ -1696,6 +1739,7 @@ int PoissonEngine::tune_dir(SolveState* tst) {
                        op_.hxc != nullptr};
   {
     std::lock_guard<std::mutex> g(g_tune_mutex);
+    tune_file_load();
     auto it = g_tune_cache.find(key);
     if (it != g_tune_cache.end())
       for (const FusedTile& t : cands)
@@ -1733,7 +1777,7 @@ int PoissonEngine::tune_dir(SolveState* tst) {
   }
   {
     std::lock_guard<std::mutex> g(g_tune_mutex);
-    g_tune_cache[key] = TunePick{1, best.tx, best.ty};
+    tune_store(key, TunePick{1, best.tx, best.ty});
   }
   dir_tile_ = best;
   return kOk;

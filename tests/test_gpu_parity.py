@@ -373,3 +373,19 @@ def test_les_trajectory_matches_reference(gpu_available, name, steps):
     assert np.abs(sim.fields.u - g[f"traj_{name}_u"]).max() <= 1e-10 * scale
     assert np.abs(sim.fields.v - g[f"traj_{name}_v"]).max() <= 1e-10 * scale
     assert max(sim.result.history["max_div"]) <= 1e-11
+
+
+def test_tune_file_records_measured_plan(gpu_available, tmp_path, monkeypatch):
+    """With PCFD_TUNE_FILE set, each measured level plan is appended to the
+    file (14 key and 4 choice integers a line), and a solver configured again
+    for the same operator takes the same plan."""
+    f = tmp_path / "tune.txt"
+    monkeypatch.setenv("PCFD_TUNE_FILE", str(f))
+    kw = dict(nx=44, ny=36, mg_coarse_max=96)  # a shape no other test tunes
+    s = _mg_solver("cavity", **kw)
+    rows = [list(map(int, line.split())) for line in f.read_text().splitlines() if line.strip()]
+    assert rows and all(len(r) == 18 for r in rows)
+    assert any(r[0] == 1 and r[1:3] == [44, 36] for r in rows), rows  # level 0 of this grid
+    plan = s.ctx.mg_plan()
+    assert len(plan) > 1
+    assert _mg_solver("cavity", **kw).ctx.mg_plan() == plan
