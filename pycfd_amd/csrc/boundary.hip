@@ -346,11 +346,12 @@ __device__ void mask_obstacle(const BCParams& P, int mode, double dt, double* u,
 // The velocity condition of all four walls (no periodic axis) over many
 // workgroups.  Walls apply in order 0..3 and meet only at the corners: a
 // later wall overwrites, or reads, entries an earlier one wrote within the
-// first or last kWallCorner entries of its lines.  Those entries run on one
-// thread in the serial order (wall by wall; per wall the face line, then the
-// ghost lines); every other entry k of a wall touches entry k of that
-// wall's lines only, nothing any other wall reads or writes, and runs on its
-// own thread (blockIdx.y = wall).
+// first or last kWallCorner entries of its lines.  Those entries run in
+// workgroup (0, 0) wall by wall, a barrier between walls, one thread per
+// entry (per entry the face value, then the ghost value that reads it);
+// every other entry k of a wall touches entry k of that wall's lines only,
+// nothing any other wall reads or writes, and runs on its own thread
+// (blockIdx.y = wall).
 constexpr int kWallCorner = 4;
 
 __global__ void __launch_bounds__(kBlock)
@@ -362,16 +363,27 @@ k_walls(BCParams P, int predictor, double* u, double* v) {
   if (L.face && k < L.nf.len && !corner(k, L.nf.len)) L.nf[k] = face_value(P, L, k);
   if (k < L.ng.len && !corner(k, L.ng.len)) L.ng[k] = ghost_value(L, k);
   if (k < L.tg.len && !corner(k, L.tg.len)) L.tg[k] = tangential_value(P, L, k);
-  if (blockIdx.x != 0 || blockIdx.y != 0 || threadIdx.x != 0) return;
+  if (blockIdx.x != 0 || blockIdx.y != 0) return;  // whole workgroup: barriers below
+  // thread t < 2C: corner entry t of the normal lines (face and ghost have
+  // one length), 2C <= t < 4C: of the tangential line; entry index q and
+  // whether it exists (a line shorter than 2C has fewer corner entries)
+  auto corner_q = [](int t, int len, int& q) {
+    q = t < kWallCorner ? t : len - 2 * kWallCorner + t;
+    return t < kWallCorner ? q < len : q >= max(len - kWallCorner, kWallCorner) && q < len;
+  };
+  const int t = threadIdx.x;
   for (int ww = 0; ww < 4; ++ww) {
     const WallLines C = wall_lines(P, ww, predictor != 0, u, v);
-    auto each_corner = [&](int len, auto&& fn) {
-      for (int q = 0; q < len && q < kWallCorner; ++q) fn(q);
-      for (int q = max(len - kWallCorner, kWallCorner); q < len; ++q) fn(q);
-    };
-    if (C.face) each_corner(C.nf.len, [&](int q) { C.nf[q] = face_value(P, C, q); });
-    each_corner(C.ng.len, [&](int q) { C.ng[q] = ghost_value(C, q); });
-    each_corner(C.tg.len, [&](int q) { C.tg[q] = tangential_value(P, C, q); });
+    int q = 0;
+    if (t < 2 * kWallCorner) {
+      if (corner_q(t, C.nf.len, q)) {
+        if (C.face) C.nf[q] = face_value(P, C, q);
+        C.ng[q] = ghost_value(C, q);
+      }
+    } else if (t < 4 * kWallCorner) {
+      if (corner_q(t - 2 * kWallCorner, C.tg.len, q)) C.tg[q] = tangential_value(P, C, q);
+    }
+    __syncthreads();
   }
 }
 
