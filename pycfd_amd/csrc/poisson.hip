@@ -364,6 +364,62 @@ k_cg_update(FineOp A, double* __restrict__ x, double* __restrict__ r,
   block_partials<2>(s, ops, partial, t.tile);
 }
 
+// Multigrid-CG update with the step length folded in: every workgroup first
+// issues the loads of its kCgTiles tiles, then folds the direction kernel's
+// p.q partials itself in k_fold's fixed order (so each computes the alpha a
+// separate fold launch would, bit for bit), then x += alpha p, r -= alpha q
+// and one row of partials r.r, sum(r) over its tiles.  Saves the alpha fold's
+// launch; workgroup 0 records alpha in the state.
+constexpr int kCgTiles = 8;
+
+__global__ void __launch_bounds__(kBlock)
+k_cg_update_f(FineOp A, double* __restrict__ x, double* __restrict__ r,
+              const double* __restrict__ p, const double* __restrict__ q,
+              const double* __restrict__ dir_partial, int dir_tiles, double* __restrict__ partial,
+              SolveState* st) {
+  if (st->done) return;
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int ty = tiles_y(A.ny), ntile = tiles_x(A.nx) * ty;
+  double xv[kCgTiles], pv[kCgTiles], rv[kCgTiles], qv[kCgTiles];
+  uint8_t fv[kCgTiles];
+  int kk[kCgTiles];
+  bool in[kCgTiles];
+#pragma unroll
+  for (int u = 0; u < kCgTiles; ++u) {
+    const int t = blk * kCgTiles + u;
+    const int i = (t / ty) * kTileX + (int)threadIdx.x / kTileY;
+    const int j = (t % ty) * kTileY + (int)threadIdx.x % kTileY;
+    in[u] = t < ntile && i < A.nx && j < A.ny;
+    kk[u] = in[u] ? i * A.ny + j : 0;
+    xv[u] = x[kk[u]];
+    pv[u] = p[kk[u]];
+    rv[u] = r[kk[u]];
+    qv[u] = q[kk[u]];
+    fv[u] = A.flags[kk[u]];
+  }
+  double pq[1];
+  const int op1[1] = {kSum};
+  fold_partials<1>(pq, op1, dir_partial, dir_tiles);
+  __shared__ double s_alpha;
+  if (threadIdx.x == 0) s_alpha = st->rho / pq[0];
+  __syncthreads();
+  const double alpha = s_alpha;
+  if (blk == 0 && threadIdx.x == 0) st->alpha = alpha;
+  double s[2] = {0.0, 0.0};
+#pragma unroll
+  for (int u = 0; u < kCgTiles; ++u) {
+    if (!in[u]) continue;
+    const double xk = xv[u] + alpha * pv[u];
+    const double rk = rv[u] - alpha * qv[u];
+    x[kk[u]] = xk;
+    r[kk[u]] = rk;
+    s[0] += rk * rk;
+    if (fv[u] & F_SOLV) s[1] += rk;
+  }
+  const int op2[2] = {kSum, kSum};
+  block_partials<2>(s, op2, partial, blk);
+}
+
 // V-cycle input: b0 = r - mean(r) on the active set, 0 elsewhere
 // (MultigridHierarchy._project, multigrid.py:453-458).
 template <typename T>
@@ -2492,8 +2548,10 @@ int PoissonEngine::enqueue_vcycle(int mode, int reduce) {
 }
 
 int PoissonEngine::run_mgcg() {
-  const dim3 g = tiles_for(op_.nx, op_.ny);
   const int nt = ntiles(op_.nx, op_.ny);
+  const int ub = (nt + kCgTiles - 1) / kCgTiles;  // update workgroups
+  const size_t uoff = ((size_t)dir_tile_.ntiles() + 255) / 256 * 256;
+  if (int rc = ensure_partials(uoff + 2 * (size_t)ub)) return rc;
   if (int rc = start(IM_MGCG, r_)) return rc;
   double* p_old = pa_;
   double* p_new = pb_;
@@ -2509,14 +2567,12 @@ int PoissonEngine::run_mgcg() {
                        : launch_fine_dir<double>(s_, op_, dir_tile_, (const double*)lv64_[0].y,
                                                  p_old, p_new, q_, partial_, st_, cd);
     if (rc) return rc;
-    if (int rc2 = fold(s_, partial_, dir_tile_.ntiles(), EpiAlpha{st_},
-                       clock_next(CK_FOLD, 1, dir_tile_.ntiles() * 8.0)))
-      return rc2;
+    // the update folds alpha itself and writes its partials past the direction's
     const LaunchClock cu = clock_next(CK_UPDATE, 0, (double)n_ * (6 * 8 + 1));  // x, p, r, q, flags; x, r
-    PCFD_LAUNCH_TIMED(cu, k_cg_update, g, dim3(kBlock), 0, s_, op_, x_, r_, p_new, q_,
-                      (int)IM_MGCG, partial_, (const SolveState*)st_);
-    if (int rc = fold(s_, partial_, nt, EpiUpdate{st_, (int)IM_MGCG, cfg_.maxiter, n_proj_},
-                      clock_next(CK_FOLD, 2, nt * 16.0)))
+    PCFD_LAUNCH_TIMED(cu, k_cg_update_f, dim3(ub), dim3(kBlock), 0, s_, op_, x_, r_, p_new, q_,
+                      (const double*)partial_, dir_tile_.ntiles(), partial_ + uoff, st_);
+    if (int rc = fold(s_, partial_ + uoff, ub, EpiUpdate{st_, (int)IM_MGCG, cfg_.maxiter, n_proj_},
+                      clock_next(CK_FOLD, 2, ub * 16.0)))
       return rc;
     std::swap(p_old, p_new);
   });
