@@ -870,8 +870,11 @@ k_cls_rows_reg(ClassLevel<T> L, T w, const T* __restrict__ b, const T* __restric
 #pragma unroll
   for (int k = 0; k < kRowNE; ++k) {
     if (OP == RO_PROLONG) {
+      // P_tent x_c at the neighbour: an off-diagonal entry couples active
+      // cells only (an inactive cell's row and column are the identity's),
+      // so only the diagonal needs an active flag, the row's own
       const T ev = xc[a[k]];
-      xv[k] = L.ract[q[k]] ? ev : (T)0;
+      xv[k] = (q[k] != row || own_act) ? ev : (T)0;
     } else if (OP == RO_SMOOTH0_RES) {
       xv[k] = w * L.rdinv[q[k]] * b[q[k]];
     } else {
