@@ -370,7 +370,7 @@ k_cg_update(FineOp A, double* __restrict__ x, double* __restrict__ r,
 // separate fold launch would, bit for bit), then x += alpha p, r -= alpha q
 // and one row of partials r.r, sum(r) over its tiles.  Saves the alpha fold's
 // launch; workgroup 0 records alpha in the state.
-constexpr int kCgTiles = 8;
+constexpr int kCgTiles = 4;
 
 __global__ void __launch_bounds__(kBlock)
 k_cg_update_f(FineOp A, double* __restrict__ x, double* __restrict__ r,
