@@ -342,7 +342,7 @@ k_jpcg_dir(FineOp A, const double* __restrict__ r, const double* __restrict__ p_
 
 // x += alpha p, r -= alpha q; partials ||r||^2 and r.(r/d) or sum(r).
 __global__ void __launch_bounds__(kBlock)
-k_cg_update(FineOp A, double* __restrict__ x, double* __restrict__ r,
+k_jpcg_update(FineOp A, double* __restrict__ x, double* __restrict__ r,
             const double* __restrict__ p, const double* __restrict__ q, int mode,
             double* partial, const SolveState* st) {
   if (st->done) return;
@@ -373,7 +373,7 @@ k_cg_update(FineOp A, double* __restrict__ x, double* __restrict__ r,
 constexpr int kCgTiles = 4;
 
 __global__ void __launch_bounds__(kBlock)
-k_cg_update_f(FineOp A, double* __restrict__ x, double* __restrict__ r,
+k_cg_update(FineOp A, double* __restrict__ x, double* __restrict__ r,
               const double* __restrict__ p, const double* __restrict__ q,
               const double* __restrict__ dir_partial, int dir_tiles, double* __restrict__ partial,
               SolveState* st) {
@@ -2074,7 +2074,7 @@ int PoissonEngine::run_cg() {
     hipLaunchKernelGGL(k_jpcg_dir, g, dim3(kBlock), 0, s_, op_, r_, p_old, p_new, q_, partial_,
                        (const SolveState*)st_);
     if (int rc = fold(s_, partial_, nt, EpiAlpha{st_})) return rc;
-    hipLaunchKernelGGL(k_cg_update, g, dim3(kBlock), 0, s_, op_, x_, r_, p_new, q_,
+    hipLaunchKernelGGL(k_jpcg_update, g, dim3(kBlock), 0, s_, op_, x_, r_, p_new, q_,
                        (int)IM_JPCG, partial_, (const SolveState*)st_);
     if (int rc = fold(s_, partial_, nt, EpiUpdate{st_, (int)IM_JPCG, cfg_.maxiter, n_proj_}))
       return rc;
@@ -2572,7 +2572,7 @@ int PoissonEngine::run_mgcg() {
     if (rc) return rc;
     // the update folds alpha itself and writes its partials past the direction's
     const LaunchClock cu = clock_next(CK_UPDATE, 0, (double)n_ * (6 * 8 + 1));  // x, p, r, q, flags; x, r
-    PCFD_LAUNCH_TIMED(cu, k_cg_update_f, dim3(ub), dim3(kBlock), 0, s_, op_, x_, r_, p_new, q_,
+    PCFD_LAUNCH_TIMED(cu, k_cg_update, dim3(ub), dim3(kBlock), 0, s_, op_, x_, r_, p_new, q_,
                       (const double*)partial_, dir_tile_.ntiles(), partial_ + uoff, st_);
     if (int rc = fold(s_, partial_ + uoff, ub, EpiUpdate{st_, (int)IM_MGCG, cfg_.maxiter, n_proj_},
                       clock_next(CK_FOLD, 2, ub * 16.0)))
@@ -2654,8 +2654,8 @@ int PoissonEngine::time_kernel(int which, int reps, double* avg_ms, double* byte
   double B = 0.0;
   auto launch = [&]() -> int {
     switch (which) {
-      case 0:  // CG update: read x, p, r, q + flags, write x, r
-        hipLaunchKernelGGL(k_cg_update, g, dim3(kBlock), 0, s_, op_, x_, r_, pa_, q_,
+      case 0:  // CG update (the separate-fold form): read x, p, r, q + flags, write x, r
+        hipLaunchKernelGGL(k_jpcg_update, g, dim3(kBlock), 0, s_, op_, x_, r_, pa_, q_,
                            (int)IM_MGCG, partial_, (const SolveState*)st);
         B = n * (6 * 8 + 1);
         break;
