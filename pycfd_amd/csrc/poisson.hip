@@ -237,35 +237,98 @@ struct EpiOpen {  // opens the state for one preconditioner application
 };
 
 // ----------------------------------------------------------------------------
+// The once-per-solve full-grid passes (right-hand side, start, final check)
+// take kMultiTiles tiles per workgroup: every load of a thread's cells is
+// issued before any of its arithmetic and stores, and one row of partials
+// serves the workgroup's tiles (a quarter of the rows to fold).
+// ----------------------------------------------------------------------------
+constexpr int kMultiTiles = 4;
+
+struct MultiTile {
+  int blk;
+  int i[kMultiTiles], j[kMultiTiles], k[kMultiTiles];
+  bool in[kMultiTiles];  // false: past the grid (i, j, k then name cell 0)
+};
+
+__device__ __forceinline__ MultiTile multi_tile(int nx, int ny) {
+  MultiTile m;
+  m.blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int ty = tiles_y(ny), ntile = tiles_x(nx) * ty;
+#pragma unroll
+  for (int u = 0; u < kMultiTiles; ++u) {
+    const int t = m.blk * kMultiTiles + u;
+    const int i = (t / ty) * kTileX + (int)threadIdx.x / kTileY;
+    const int j = (t % ty) * kTileY + (int)threadIdx.x % kTileY;
+    m.in[u] = t < ntile && i < nx && j < ny;
+    m.i[u] = m.in[u] ? i : 0;
+    m.j[u] = m.in[u] ? j : 0;
+    m.k[u] = m.i[u] * ny + m.j[u];
+  }
+  return m;
+}
+
+static inline int multi_blocks(int nx, int ny) {
+  return (num_tiles(nx, ny) + kMultiTiles - 1) / kMultiTiles;
+}
+
+// y = (A x)[k] from the five values already loaded, in fine_apply's order.
+__device__ __forceinline__ double apply5(const FaceC& c, uint8_t f, double d, double xc,
+                                         double xl, double xr, double xb, double xt) {
+  double y = 0.0;
+  if (f & F_L) y += c.l * xl;
+  if (f & F_B) y += c.b * xb;
+  y += d * xc;
+  if (f & F_T) y += c.t * xt;
+  if (f & F_R) y += c.r * xr;
+  return y;
+}
+
+// ----------------------------------------------------------------------------
 // Right-hand side assembly (solver.py:659-661, pressure.py:300-308)
 // ----------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock)
 k_prepare(FineOp A, const double* __restrict__ u, const double* __restrict__ v, int ldu, int ldv,
           double dx, double dy, double dt, const double* __restrict__ rhs, double* __restrict__ b,
           double* partial) {
-  Tile2D t;
-  double s[1] = {0.0};
-  if (cell_of_tile(A.nx, A.ny, t)) {
-    const int i = t.i, j = t.j, k = i * A.ny + j;
-    const uint8_t f = A.flags[k];
-    double val;
+  const MultiTile m = multi_tile(A.nx, A.ny);
+  uint8_t f[kMultiTiles];
+  double g0[kMultiTiles], g1[kMultiTiles], g2[kMultiTiles], g3[kMultiTiles];
+#pragma unroll
+  for (int q = 0; q < kMultiTiles; ++q) {
+    const int i = m.i[q], j = m.j[q];
+    f[q] = A.flags[m.k[q]];
     if (rhs) {
-      val = rhs[k];
+      g0[q] = rhs[m.k[q]];
+      g1[q] = g2[q] = g3[q] = 0.0;
     } else {
       const int a = i + 1, c = j + 1;
+      g0[q] = u[(size_t)(a + 1) * ldu + c];
+      g1[q] = u[(size_t)a * ldu + c];
+      g2[q] = v[(size_t)a * ldv + c + 1];
+      g3[q] = v[(size_t)a * ldv + c];
+    }
+  }
+  double s[1] = {0.0};
+#pragma unroll
+  for (int q = 0; q < kMultiTiles; ++q) {
+    if (!m.in[q]) continue;
+    const int i = m.i[q], j = m.j[q];
+    double val;
+    if (rhs) {
+      val = g0[q];
+    } else {
       const double hx = A.hxc ? A.hxc[i] : dx, hy = A.hyc ? A.hyc[j] : dy;
-      const double div = (u[(size_t)(a + 1) * ldu + c] - u[(size_t)a * ldu + c]) / hx +
-                         (v[(size_t)a * ldv + c + 1] - v[(size_t)a * ldv + c]) / hy;
+      const double div = (g0[q] - g1[q]) / hx + (g2[q] - g3[q]) / hy;
       val = div / dt;
     }
     val = val * fine_row_scale(A, i, j) +
-          ((f & F_SOLV) ? dirichlet_rhs(A, fine_faces(A, i, j), i, j) : 0.0);
-    if (!(f & F_SOLV)) val = 0.0;
-    b[k] = val;
-    s[0] = val;
+          ((f[q] & F_SOLV) ? dirichlet_rhs(A, fine_faces(A, i, j), i, j) : 0.0);
+    if (!(f[q] & F_SOLV)) val = 0.0;
+    b[m.k[q]] = val;
+    s[0] += val;
   }
   const int ops[1] = {kSum};
-  block_partials<1>(s, ops, partial, t.tile);
+  block_partials<1>(s, ops, partial, m.blk);
 }
 
 // Solver start: b <- b - mean_b on fluid rows; r = b - A x (x = guess with
@@ -274,27 +337,41 @@ k_prepare(FineOp A, const double* __restrict__ u, const double* __restrict__ v, 
 __global__ void __launch_bounds__(kBlock)
 k_init(FineOp A, double* __restrict__ b, const double* __restrict__ x, double* __restrict__ r,
        int mode, const SolveState* st, double* partial) {
-  Tile2D t;
-  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  const MultiTile m = multi_tile(A.nx, A.ny);
   const double mb = st->mean_b;
-  if (cell_of_tile(A.nx, A.ny, t)) {
-    const int i = t.i, j = t.j, k = i * A.ny + j;
-    const uint8_t f = A.flags[k];
-    const Nb nb = fine_neighbours(A, i, j);
-    const double bk = (f & F_SOLV) ? b[k] - mb : 0.0;
-    auto xv = [&](int q) -> double { return x[q]; };
+  uint8_t f[kMultiTiles];
+  double bv[kMultiTiles], xc[kMultiTiles], xl[kMultiTiles], xr[kMultiTiles], xb[kMultiTiles],
+      xt[kMultiTiles];
+#pragma unroll
+  for (int u = 0; u < kMultiTiles; ++u) {
+    const int k = m.k[u];
+    const Nb nb = fine_neighbours(A, m.i[u], m.j[u]);
+    f[u] = A.flags[k];
+    bv[u] = b[k];
+    xc[u] = x[k];
+    xl[u] = x[nb.kL];
+    xr[u] = x[nb.kR];
+    xb[u] = x[nb.kB];
+    xt[u] = x[nb.kT];
+  }
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int u = 0; u < kMultiTiles; ++u) {
+    if (!m.in[u]) continue;
+    const int i = m.i[u], j = m.j[u], k = m.k[u];
+    const double bk = (f[u] & F_SOLV) ? bv[u] - mb : 0.0;
     const FaceC fc = fine_faces(A, i, j);
-    const double d = fine_diag(A, fc, f, i, j);
-    const double rk = bk - fine_apply<double>(fc, f, nb, k, d, xv);
-    s[0] = bk * bk;
-    s[1] = rk * rk;
-    if (mode == IM_JPCG) s[2] = rk * (rk / d);
-    if (mode == IM_MGCG && (f & F_SOLV)) s[3] = rk;
+    const double d = fine_diag(A, fc, f[u], i, j);
+    const double rk = bk - apply5(fc, f[u], d, xc[u], xl[u], xr[u], xb[u], xt[u]);
+    s[0] += bk * bk;
+    s[1] += rk * rk;
+    if (mode == IM_JPCG) s[2] += rk * (rk / d);
+    if (mode == IM_MGCG && (f[u] & F_SOLV)) s[3] += rk;
     b[k] = bk;
     if (r) r[k] = rk;
   }
   const int ops[4] = {kSum, kSum, kSum, kSum};
-  block_partials<4>(s, ops, partial, t.tile);
+  block_partials<4>(s, ops, partial, m.blk);
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -504,26 +581,47 @@ __global__ void __launch_bounds__(kBlock)
 k_check(FineOp A, const double* __restrict__ x, const double* __restrict__ b, int mode,
         double* partial, const SolveState* st) {
   if (mode != CM_FINAL && st->done) return;
-  Tile2D t;
-  double s[2] = {0.0, 0.0};
+  const MultiTile mt = multi_tile(A.nx, A.ny);
   const double m = st->mean_x;
   const bool zero = (mode == CM_FINAL) && st->zero_rhs;
-  if (cell_of_tile(A.nx, A.ny, t)) {
-    const int i = t.i, j = t.j, k = i * A.ny + j;
-    const uint8_t f = A.flags[k];
-    const Nb nb = fine_neighbours(A, i, j);
-    auto xv = [&](int q) -> double {
-      if (zero) return 0.0;
-      return (A.flags[q] & F_SOLV) ? x[q] - m : x[q];
-    };
+  // x at a cell less the pending shift on solvable cells (zero for a zero rhs)
+  auto xv = [&](double xq, uint8_t fq) -> double {
+    if (zero) return 0.0;
+    return (fq & F_SOLV) ? xq - m : xq;
+  };
+  uint8_t f[kMultiTiles], fl[kMultiTiles], fr[kMultiTiles], fb[kMultiTiles], ft[kMultiTiles];
+  double bv[kMultiTiles], xc[kMultiTiles], xl[kMultiTiles], xr[kMultiTiles], xb[kMultiTiles],
+      xt[kMultiTiles];
+#pragma unroll
+  for (int u = 0; u < kMultiTiles; ++u) {
+    const int k = mt.k[u];
+    const Nb nb = fine_neighbours(A, mt.i[u], mt.j[u]);
+    f[u] = A.flags[k];
+    fl[u] = A.flags[nb.kL];
+    fr[u] = A.flags[nb.kR];
+    fb[u] = A.flags[nb.kB];
+    ft[u] = A.flags[nb.kT];
+    bv[u] = b[k];
+    xc[u] = x[k];
+    xl[u] = x[nb.kL];
+    xr[u] = x[nb.kR];
+    xb[u] = x[nb.kB];
+    xt[u] = x[nb.kT];
+  }
+  double s[2] = {0.0, 0.0};
+#pragma unroll
+  for (int u = 0; u < kMultiTiles; ++u) {
+    if (!mt.in[u]) continue;
+    const int i = mt.i[u], j = mt.j[u];
     const FaceC fc = fine_faces(A, i, j);
-    const double d = fine_diag(A, fc, f, i, j);
-    const double rk = b[k] - fine_apply<double>(fc, f, nb, k, d, xv);
-    s[0] = rk * rk;
-    if (f & F_SOLV) s[1] = xv(k);
+    const double d = fine_diag(A, fc, f[u], i, j);
+    const double rk = bv[u] - apply5(fc, f[u], d, xv(xc[u], f[u]), xv(xl[u], fl[u]),
+                                     xv(xr[u], fr[u]), xv(xb[u], fb[u]), xv(xt[u], ft[u]));
+    s[0] += rk * rk;
+    if (f[u] & F_SOLV) s[1] += xv(xc[u], f[u]);
   }
   const int ops[2] = {kSum, kSum};
-  block_partials<2>(s, ops, partial, t.tile);
+  block_partials<2>(s, ops, partial, mt.blk);
 }
 
 // Normalised pressure into the ghosted array (solver.py:659-665 with
@@ -1934,18 +2032,19 @@ int PoissonEngine::plan_level_tiles(std::vector<LevelBuf>& lv) {
 
 int PoissonEngine::prepare_from_velocity(const double* u, const double* v, int ldu, int ldv,
                                          double dx, double dy, double dt) {
-  hipLaunchKernelGGL(k_prepare, tiles_for(op_.nx, op_.ny), dim3(kBlock), 0, s_, op_, u, v, ldu,
-                     ldv, dx, dy, dt, (const double*)nullptr, b_, partial_);
+  const int mb = multi_blocks(op_.nx, op_.ny);
+  hipLaunchKernelGGL(k_prepare, dim3(mb), dim3(kBlock), 0, s_, op_, u, v, ldu, ldv, dx, dy, dt,
+                     (const double*)nullptr, b_, partial_);
   PCFD_LAUNCH_CHECK();
-  return fold(s_, partial_, ntiles(op_.nx, op_.ny), EpiPrepare{st_, singular_, n_fluid_});
+  return fold(s_, partial_, mb, EpiPrepare{st_, singular_, n_fluid_});
 }
 
 int PoissonEngine::prepare_from_rhs(const double* rhs) {
-  hipLaunchKernelGGL(k_prepare, tiles_for(op_.nx, op_.ny), dim3(kBlock), 0, s_, op_,
-                     (const double*)nullptr, (const double*)nullptr, 0, 0, 1.0, 1.0, 1.0, rhs,
-                     b_, partial_);
+  const int mb = multi_blocks(op_.nx, op_.ny);
+  hipLaunchKernelGGL(k_prepare, dim3(mb), dim3(kBlock), 0, s_, op_, (const double*)nullptr,
+                     (const double*)nullptr, 0, 0, 1.0, 1.0, 1.0, rhs, b_, partial_);
   PCFD_LAUNCH_CHECK();
-  return fold(s_, partial_, ntiles(op_.nx, op_.ny), EpiPrepare{st_, singular_, n_fluid_});
+  return fold(s_, partial_, mb, EpiPrepare{st_, singular_, n_fluid_});
 }
 
 int PoissonEngine::reset_guess() {
@@ -2005,10 +2104,11 @@ int PoissonEngine::solve(SolveInfo* info, bool sync_info) {
 
 int PoissonEngine::finalize(bool recompute_residual) {
   (void)recompute_residual;
-  hipLaunchKernelGGL(k_check, tiles_for(op_.nx, op_.ny), dim3(kBlock), 0, s_, op_, x_, b_,
-                     (int)CM_FINAL, partial_, (const SolveState*)st_);
+  const int mb = multi_blocks(op_.nx, op_.ny);
+  hipLaunchKernelGGL(k_check, dim3(mb), dim3(kBlock), 0, s_, op_, x_, b_, (int)CM_FINAL,
+                     partial_, (const SolveState*)st_);
   PCFD_LAUNCH_CHECK();
-  return fold(s_, partial_, ntiles(op_.nx, op_.ny),
+  return fold(s_, partial_, mb,
               EpiCheck{st_, (int)CM_FINAL, cfg_.tol, cfg_.maxiter, singular_, n_fluid_});
 }
 
@@ -2057,11 +2157,11 @@ int PoissonEngine::apply_operator(const double* x, double* y) {
 int PoissonEngine::start(int mode, double* r) {
   const dim3 g = tiles_for(op_.nx, op_.ny);
   hipLaunchKernelGGL(k_zero_nonfluid, g, dim3(kBlock), 0, s_, op_, x_);
-  hipLaunchKernelGGL(k_init, g, dim3(kBlock), 0, s_, op_, b_, (const double*)x_, r, mode,
+  const int mb = multi_blocks(op_.nx, op_.ny);
+  hipLaunchKernelGGL(k_init, dim3(mb), dim3(kBlock), 0, s_, op_, b_, (const double*)x_, r, mode,
                      (const SolveState*)st_, partial_);
   PCFD_LAUNCH_CHECK();
-  return fold(s_, partial_, ntiles(op_.nx, op_.ny),
-              EpiInit{st_, mode, cfg_.tol, cfg_.maxiter, n_proj_});
+  return fold(s_, partial_, mb, EpiInit{st_, mode, cfg_.tol, cfg_.maxiter, n_proj_});
 }
 
 int PoissonEngine::run_cg() {
@@ -2098,9 +2198,9 @@ int PoissonEngine::run_sor() {
           return rc;
       }
     }
-    hipLaunchKernelGGL(k_check, g, dim3(kBlock), 0, s_, op_, x_, b_, (int)CM_RELAX, partial_,
-                       (const SolveState*)st_);
-    if (int rc = fold(s_, partial_, nt,
+    hipLaunchKernelGGL(k_check, dim3(multi_blocks(op_.nx, op_.ny)), dim3(kBlock), 0, s_, op_, x_,
+                       b_, (int)CM_RELAX, partial_, (const SolveState*)st_);
+    if (int rc = fold(s_, partial_, multi_blocks(op_.nx, op_.ny),
                       EpiCheck{st_, (int)CM_RELAX, cfg_.tol, cfg_.maxiter, singular_, n_fluid_}))
       return rc;
   });
@@ -2122,9 +2222,9 @@ int PoissonEngine::run_jacobi() {
       if (int rc = fold(s_, partial_, nt, EpiJacobi{st_, singular_, n_fluid_})) return rc;
       std::swap(cur, nxt);
     }
-    hipLaunchKernelGGL(k_check, g, dim3(kBlock), 0, s_, op_, cur, b_, (int)CM_RELAX, partial_,
-                       (const SolveState*)st_);
-    if (int rc = fold(s_, partial_, nt,
+    hipLaunchKernelGGL(k_check, dim3(multi_blocks(op_.nx, op_.ny)), dim3(kBlock), 0, s_, op_, cur,
+                       b_, (int)CM_RELAX, partial_, (const SolveState*)st_);
+    if (int rc = fold(s_, partial_, multi_blocks(op_.nx, op_.ny),
                       EpiCheck{st_, (int)CM_RELAX, cfg_.tol, cfg_.maxiter, singular_, n_fluid_}))
       return rc;
   });
@@ -2584,8 +2684,6 @@ int PoissonEngine::run_mgcg() {
 }
 
 int PoissonEngine::run_multigrid() {
-  const dim3 g = tiles_for(op_.nx, op_.ny);
-  const int nt = ntiles(op_.nx, op_.ny);
   if (int rc = start(IM_MG, nullptr)) return rc;
   LevelBuf& L0 = lv64_[0];
   // the hierarchy's level-0 vectors hold b and the iterate
@@ -2596,9 +2694,9 @@ int PoissonEngine::run_multigrid() {
   PCFD_ITERATE(1, cfg_.maxiter, {
     ++clock_iter_;
     if (int rc = enqueue_vcycle_t<double>(VC_STANDALONE, SR_MG)) return rc;
-    hipLaunchKernelGGL(k_check, g, dim3(kBlock), 0, s_, op_, (const double*)L0.x, b_, (int)CM_MG,
-                       partial_, (const SolveState*)st_);
-    if (int rc = fold(s_, partial_, nt,
+    hipLaunchKernelGGL(k_check, dim3(multi_blocks(op_.nx, op_.ny)), dim3(kBlock), 0, s_, op_,
+                       (const double*)L0.x, b_, (int)CM_MG, partial_, (const SolveState*)st_);
+    if (int rc = fold(s_, partial_, multi_blocks(op_.nx, op_.ny),
                       EpiCheck{st_, (int)CM_MG, cfg_.tol, cfg_.maxiter, singular_, n_fluid_}))
       return rc;
   });
