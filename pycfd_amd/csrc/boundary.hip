@@ -230,20 +230,66 @@ __device__ void wrap_faces(const BCParams& P, double* u, double* v) {
 __device__ void mass_balance(const BCParams& P, double* u, double* v) {
   double influx = 0.0, outflux = 0.0, total_area = 0.0;
   int n_out = 0;
+  // The four walls' face sums: each thread's partial of a wall is the plain
+  // strided loop's (kSlots entries a chunk, increasing k), all walls' loads
+  // issued together when every line fits one chunk, then one block
+  // reduction (the same tree per wall as a reduction of its own).
+  double part[4];
+  const int chunk = kSlots * (int)blockDim.x;
+  if (P.nx <= chunk && P.ny <= chunk) {
+    double val[4][kSlots];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      const WallIdx wi = wall_index(w, P.nx, P.ny);
+      Line f = (wi.axis == 0) ? line_of(u, P.ldu, P.nx + 3, P.ny + 2, 0, wi.n_face)
+                              : line_of(v, P.ldv, P.nx + 2, P.ny + 3, 1, wi.n_face);
+      const int n = (wi.axis == 0) ? P.ny : P.nx;
+      const double* wk = (wi.axis == 0) ? P.hy : P.hx;
+#pragma unroll
+      for (int t = 0; t < kSlots; ++t) {  // in bounds on every wall (ghosted lines)
+        const int k = min(t * (int)blockDim.x + (int)threadIdx.x, n - 1);
+        val[w][t] = wk ? f[1 + k] * wk[k] : f[1 + k];
+      }
+    }
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      const int n = (w < 2) ? P.ny : P.nx;
+      const bool on = P.kind[w] != BC_PERIODIC;
+      part[w] = 0.0;
+#pragma unroll
+      for (int t = 0; t < kSlots; ++t)
+        if (on && t * (int)blockDim.x + (int)threadIdx.x < n) part[w] += val[w][t];
+    }
+  } else {
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      const WallIdx wi = wall_index(w, P.nx, P.ny);
+      Line f = (wi.axis == 0) ? line_of(u, P.ldu, P.nx + 3, P.ny + 2, 0, wi.n_face)
+                              : line_of(v, P.ldv, P.nx + 2, P.ny + 3, 1, wi.n_face);
+      const int n = (wi.axis == 0) ? P.ny : P.nx;
+      // stretched mesh: each face weighted by its own height / width
+      const double* wk = (wi.axis == 0) ? P.hy : P.hx;
+      auto all = [](int) { return true; };
+      part[w] = (P.kind[w] == BC_PERIODIC) ? 0.0
+                : wk ? line_part(n, [&](int k) { return f[1 + k] * wk[k]; }, all)
+                     : line_part(n, [&](int k) { return f[1 + k]; }, all);
+    }
+  }
+  const int ops4[4] = {kSum, kSum, kSum, kSum};
+  block_reduce<4>(part, ops4);
+  __shared__ double wall_sum[4];
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int w = 0; w < 4; ++w) wall_sum[w] = part[w];
+  }
+  __syncthreads();
   for (int w = 0; w < 4; ++w) {
     const int kind = P.kind[w];
     if (kind == BC_PERIODIC) continue;
     const WallIdx wi = wall_index(w, P.nx, P.ny);
-    Line f = (wi.axis == 0) ? line_of(u, P.ldu, P.nx + 3, P.ny + 2, 0, wi.n_face)
-                            : line_of(v, P.ldv, P.nx + 2, P.ny + 3, 1, wi.n_face);
-    const int n = (wi.axis == 0) ? P.ny : P.nx;
     const double width = (wi.axis == 0) ? P.dy : P.dx;
-    // stretched mesh: each face weighted by its own height / width
     const double* wk = (wi.axis == 0) ? P.hy : P.hx;
-    auto all = [](int) { return true; };
-    const double part = wk ? line_part(n, [&](int k) { return f[1 + k] * wk[k]; }, all)
-                           : line_part(n, [&](int k) { return f[1 + k]; }, all);
-    const double s = block_sum(part);
+    const double s = wall_sum[w];
     const double flux_out = (double)wi.outward * (wk ? s : s * width);
     if (kind == BC_OUTLET) {
       outflux += flux_out;
@@ -305,32 +351,63 @@ __device__ void mask_obstacle(const BCParams& P, int mode, double dt, double* u,
     auto scaled = [&](double sum, bool weighted) -> double {
       return weighted ? sum / dt : sum * P.area / dt;
     };
-    auto all = [](int) { return true; };
-    const double su = line_part(P.n_uf, ua, all);
-    const double sv = line_part(P.n_vf, va, all);
-    const double fx = scaled(block_sum(su), P.uf_area != nullptr);
-    const double fy = scaled(block_sum(sv), P.vf_area != nullptr);
-    if (threadIdx.x == 0) {
-      if (mode == BOP_MASK_RESET) {
-        P.force[0] = fx;
-        P.force[1] = fy;
-      } else {
-        P.force[0] = P.force[0] + fx;
-        P.force[1] = P.force[1] + fy;
+    // The total and kMaskBodies bodies' sums of u and v from one read of the
+    // faces per chunk; each thread adds its faces in increasing order (the
+    // plain strided loop's partials), then one block reduction.
+    constexpr int kMaskBodies = 3;
+    constexpr int K = 2 + 2 * kMaskBodies;
+    const int chunk = kSlots * (int)blockDim.x;
+    const int nmax = max(P.n_uf, P.n_vf);
+    for (int b0 = 0; b0 == 0 || b0 < P.n_bodies; b0 += kMaskBodies) {
+      double acc[K];
+#pragma unroll
+      for (int c = 0; c < K; ++c) acc[c] = 0.0;
+      for (int base = 0; base < nmax; base += chunk) {
+        double uval[kSlots], vval[kSlots];
+        int ubody[kSlots], vbody[kSlots];
+#pragma unroll
+        for (int t = 0; t < kSlots; ++t) {
+          const int k = base + t * (int)blockDim.x + (int)threadIdx.x;
+          const int ku = min(k, P.n_uf - 1), kv = min(k, P.n_vf - 1);
+          uval[t] = (ku >= 0) ? ua(ku) : 0.0;
+          ubody[t] = (ku >= 0) ? (int)P.uf_body[ku] : -1;
+          vval[t] = (kv >= 0) ? va(kv) : 0.0;
+          vbody[t] = (kv >= 0) ? (int)P.vf_body[kv] : -1;
+        }
+#pragma unroll
+        for (int t = 0; t < kSlots; ++t) {
+          const int k = base + t * (int)blockDim.x + (int)threadIdx.x;
+          if (k < P.n_uf) {
+            acc[0] += uval[t];
+#pragma unroll
+            for (int c = 0; c < kMaskBodies; ++c)
+              if (ubody[t] == b0 + c) acc[2 + 2 * c] += uval[t];
+          }
+          if (k < P.n_vf) {
+            acc[1] += vval[t];
+#pragma unroll
+            for (int c = 0; c < kMaskBodies; ++c)
+              if (vbody[t] == b0 + c) acc[3 + 2 * c] += vval[t];
+          }
+        }
       }
-    }
-    for (int b = 0; b < P.n_bodies; ++b) {
-      const double bu = line_part(P.n_uf, ua, [&](int k) { return P.uf_body[k] == b; });
-      const double bv = line_part(P.n_vf, va, [&](int k) { return P.vf_body[k] == b; });
-      const double bx = scaled(block_sum(bu), P.uf_area != nullptr);
-      const double by = scaled(block_sum(bv), P.vf_area != nullptr);
+      int ops[K];
+#pragma unroll
+      for (int c = 0; c < K; ++c) ops[c] = kSum;
+      block_reduce<K>(acc, ops);
       if (threadIdx.x == 0) {
-        if (mode == BOP_MASK_RESET) {
-          P.force[2 + 2 * b] = bx;
-          P.force[3 + 2 * b] = by;
-        } else {
-          P.force[2 + 2 * b] = P.force[2 + 2 * b] + bx;
-          P.force[3 + 2 * b] = P.force[3 + 2 * b] + by;
+        const bool wu = P.uf_area != nullptr, wv = P.vf_area != nullptr;
+        const bool reset = mode == BOP_MASK_RESET;
+        if (b0 == 0) {
+          const double fx = scaled(acc[0], wu), fy = scaled(acc[1], wv);
+          P.force[0] = reset ? fx : P.force[0] + fx;
+          P.force[1] = reset ? fy : P.force[1] + fy;
+        }
+        for (int c = 0; c < kMaskBodies && b0 + c < P.n_bodies; ++c) {
+          const int b = b0 + c;
+          const double bx = scaled(acc[2 + 2 * c], wu), by = scaled(acc[3 + 2 * c], wv);
+          P.force[2 + 2 * b] = reset ? bx : P.force[2 + 2 * b] + bx;
+          P.force[3 + 2 * b] = reset ? by : P.force[3 + 2 * b] + by;
         }
       }
     }

@@ -2579,7 +2579,7 @@ int PoissonEngine::launch_chain(std::vector<LevelBuf>& lv, int first, const Solv
 // reads the pre-smoothed copy in lv[0].t, so nothing reads x while it is
 // written) with the partial sums `reduce` asks for.
 template <typename T>
-int PoissonEngine::enqueue_vcycle_t(int mode, int reduce) {
+int PoissonEngine::enqueue_vcycle_t(int mode, int reduce, bool defer_fold) {
   auto& lv = (sizeof(T) == 4) ? lv32_ : lv64_;
   const int nl = (int)lv.size();
   const int nu = cfg_.sweeps;
@@ -2599,6 +2599,7 @@ int PoissonEngine::enqueue_vcycle_t(int mode, int reduce) {
     hipLaunchKernelGGL(k_cycle_sums<T>, tiles_for(op_.nx, op_.ny), dim3(kBlock), 0, s_, op_,
                        (const T*)L0.b, (const T*)out, partial_, cst);
     PCFD_LAUNCH_CHECK();
+    if (defer_fold) return kOk;
     return fold(s_, partial_, nt0, EpiCycle{st_, reduce, n_proj_});
   }
   // Downward leg.  Algorithmic bytes for the clock: r (FP64) or b_in and
@@ -2619,6 +2620,7 @@ int PoissonEngine::enqueue_vcycle_t(int mode, int reduce) {
                                           L0.fy, C.ny, out, reduce, partial_, cst,
                                           clock_next(CK_FINE_UP, 0, bytes_up)))
       return rc;
+    if (defer_fold) return kOk;
     return fold(s_, partial_, L0.stream_nb, EpiCycle{st_, reduce, n_proj_},
                 clock_next(CK_FOLD, 0, L0.stream_nb * 24.0));
   }
@@ -2641,19 +2643,34 @@ int PoissonEngine::enqueue_vcycle_t(int mode, int reduce) {
                                    reduce, partial_, cst, clock_next(CK_FINE_UP, 0, bytes)))
       return rc;
   }
+  if (defer_fold) return kOk;
   return fold(s_, partial_, L0.up.ntiles(), EpiCycle{st_, reduce, n_proj_},
               clock_next(CK_FOLD, 0, L0.up.ntiles() * 24.0));
 }
 
-int PoissonEngine::enqueue_vcycle(int mode, int reduce) {
-  if (cfg_.precision == 1) return enqueue_vcycle_t<float>(mode, reduce);
-  return enqueue_vcycle_t<double>(mode, reduce);
+int PoissonEngine::enqueue_vcycle(int mode, int reduce, bool defer_fold) {
+  if (cfg_.precision == 1) return enqueue_vcycle_t<float>(mode, reduce, defer_fold);
+  return enqueue_vcycle_t<double>(mode, reduce, defer_fold);
+}
+
+// Rows of partials a Krylov-mode cycle leaves (the fold's row count in
+// enqueue_vcycle_t).
+int PoissonEngine::cycle_rows() const {
+  const std::vector<LevelBuf>& lv = (cfg_.precision == 1) ? lv32_ : lv64_;
+  if (lv.size() == 1) return ntiles(op_.nx, op_.ny);
+  if (lv[0].stream_rows) return lv[0].stream_nb;
+  return lv[0].up.ntiles();
 }
 
 int PoissonEngine::run_mgcg() {
   const int nt = ntiles(op_.nx, op_.ny);
   const int ub = (nt + kCgTiles - 1) / kCgTiles;  // update workgroups
-  const size_t uoff = ((size_t)dir_tile_.ntiles() + 255) / 256 * 256;
+  // partials: the cycle's sums (folded by the direction kernel), the
+  // direction's p.q (folded by the update), the update's r.r, sum(r)
+  auto up256 = [](size_t n) { return (n + 255) / 256 * 256; };
+  const int crows = cycle_rows();
+  const size_t doff = up256(3 * (size_t)crows);
+  const size_t uoff = doff + up256((size_t)dir_tile_.ntiles());
   if (int rc = ensure_partials(uoff + 2 * (size_t)ub)) return rc;
   if (int rc = start(IM_MGCG, r_)) return rc;
   double* p_old = pa_;
@@ -2662,18 +2679,20 @@ int PoissonEngine::run_mgcg() {
   const int it0 = clock_iter_;
   PCFD_ITERATE(1, cfg_.maxiter, {
     ++clock_iter_;
-    if (int rc = enqueue_vcycle(VC_KRYLOV, SR_MGCG)) return rc;
-    // direction: z (cycle precision), p_old, flags in; p_new, q out
+    if (int rc = enqueue_vcycle(VC_KRYLOV, SR_MGCG, true)) return rc;
+    // direction (folds the cycle's sums itself): z (cycle precision), p_old,
+    // flags in; p_new, q out
     const LaunchClock cd = clock_next(CK_DIR, 0, (double)n_ * ((f32 ? 4 : 8) + 3 * 8 + 1));
+    const CycleFold cf{partial_, crows, n_proj_};
     const int rc = f32 ? launch_fine_dir<float>(s_, op_, dir_tile_, (const float*)lv32_[0].y,
-                                                p_old, p_new, q_, partial_, st_, cd)
+                                                p_old, p_new, q_, partial_ + doff, st_, cf, cd)
                        : launch_fine_dir<double>(s_, op_, dir_tile_, (const double*)lv64_[0].y,
-                                                 p_old, p_new, q_, partial_, st_, cd);
+                                                 p_old, p_new, q_, partial_ + doff, st_, cf, cd);
     if (rc) return rc;
     // the update folds alpha itself and writes its partials past the direction's
     const LaunchClock cu = clock_next(CK_UPDATE, 0, (double)n_ * (6 * 8 + 1));  // x, p, r, q, flags; x, r
     PCFD_LAUNCH_TIMED(cu, k_cg_update, dim3(ub), dim3(kBlock), 0, s_, op_, x_, r_, p_new, q_,
-                      (const double*)partial_, dir_tile_.ntiles(), partial_ + uoff, st_);
+                      (const double*)partial_ + doff, dir_tile_.ntiles(), partial_ + uoff, st_);
     if (int rc = fold(s_, partial_ + uoff, ub, EpiUpdate{st_, (int)IM_MGCG, cfg_.maxiter, n_proj_},
                       clock_next(CK_FOLD, 2, ub * 16.0)))
       return rc;
@@ -2693,7 +2712,7 @@ int PoissonEngine::run_multigrid() {
   const int it0 = clock_iter_;
   PCFD_ITERATE(1, cfg_.maxiter, {
     ++clock_iter_;
-    if (int rc = enqueue_vcycle_t<double>(VC_STANDALONE, SR_MG)) return rc;
+    if (int rc = enqueue_vcycle_t<double>(VC_STANDALONE, SR_MG, false)) return rc;
     hipLaunchKernelGGL(k_check, dim3(multi_blocks(op_.nx, op_.ny)), dim3(kBlock), 0, s_, op_,
                        (const double*)L0.x, b_, (int)CM_MG, partial_, (const SolveState*)st_);
     if (int rc = fold(s_, partial_, multi_blocks(op_.nx, op_.ny),

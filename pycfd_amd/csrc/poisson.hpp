@@ -81,6 +81,7 @@ struct SolveState {
   double res, res_prev;
   double res_final;  // relative true residual of the returned iterate
   double mean_final; // fluid mean removed by the normalisation
+  double rho_hist[2];  // multigrid-CG rho of iteration i at [i & 1] (read by the next one)
   int iter;          // completed iterations / sweeps / cycles
   int done;
   int converged;
@@ -263,9 +264,12 @@ class PoissonEngine {
   // V-cycle entry modes: from the Krylov residual r_ (zero start, output
   // projected by the caller's sums) or from lv[0].b / lv[0].x (standalone).
   enum CycleMode : int { VC_KRYLOV = 0, VC_STANDALONE = 1 };
-  int enqueue_vcycle(int mode, int reduce);
+  // defer_fold: leave the cycle's partials (cycle_rows() rows of 3) for the
+  // next kernel to fold instead of launching the fold.
+  int enqueue_vcycle(int mode, int reduce, bool defer_fold = false);
+  int cycle_rows() const;
   template <typename T>
-  int enqueue_vcycle_t(int mode, int reduce);
+  int enqueue_vcycle_t(int mode, int reduce, bool defer_fold);
   template <typename T>
   int plan_level_tiles(std::vector<LevelBuf>& lv);
   template <typename T>

@@ -447,11 +447,18 @@ k_fine_up(FineOp A, FusedTile ft, T w, int nu, const T* __restrict__ x_pre,
 // p = zp + beta p_old with zp = z - mean_z on solvable cells (0 elsewhere),
 // q = A p, and the tile's partial of p.q.  p is formed once per region cell
 // in LDS (halo 1), then q on the tile.
+//
+// With cf.rows > 0 every workgroup first folds the V-cycle's sums itself, in
+// k_fold's order (the same bits as the fold launch it replaces): mean_z,
+// rho = r.z_proj and beta = rho / rho_old, with rho_old from the state's
+// rho history (written one launch earlier, so no workgroup reads what this
+// launch writes).  Workgroup 0 records mean_z, rho, rho_prev, beta and the
+// history entry of this iteration.
 template <typename T, bool S>
 __global__ void __launch_bounds__(kBlock)
 k_fine_dir(FineOp A, FusedTile ft, const T* __restrict__ z, const double* __restrict__ p_old,
            double* __restrict__ p_new, double* __restrict__ q, double* partial,
-           const SolveState* __restrict__ st) {
+           SolveState* st, CycleFold cf) {
   if (st->done) return;
   extern __shared__ __align__(16) unsigned char smem[];
   const int ry = ft.ry, rn = ft.rx * ry;
@@ -461,9 +468,39 @@ k_fine_dir(FineOp A, FusedTile ft, const T* __restrict__ z, const double* __rest
   uint8_t* F = (uint8_t*)(P + rn);
   const TileOrigin o = tile_origin(ft);
   fine_tables<double>(A, td, ti);
-  const bool first = st->iter == 0;
-  const double beta = st->beta;
-  const double mz = st->mean_z;
+  const int it = st->iter;
+  const bool first = it == 0;
+  double beta, mz;
+  if (cf.rows > 0) {
+    double v[3];
+    const int op3[3] = {kSum, kSum, kSum};
+    fold_partials<3>(v, op3, cf.partial, cf.rows);
+    __shared__ double s_cyc[2];
+    if (threadIdx.x == 0) {
+      // EpiCycle's arithmetic (r.z_proj = sum(b0 z) - mean_z sum(b0))
+      const double m = (cf.n_proj > 0) ? v[0] / (double)cf.n_proj : 0.0;
+      const double rho = v[1] - m * v[2];
+      double rho_old = 1.0;
+      if (!first) rho_old = st->rho_hist[(it + 1) & 1];
+      else if (blockIdx.x == 0) rho_old = st->rho;
+      const double b = rho / rho_old;
+      s_cyc[0] = m;
+      s_cyc[1] = b;
+      if (blockIdx.x == 0) {
+        st->mean_z = m;
+        st->rho_prev = rho_old;
+        st->rho = rho;
+        st->beta = b;
+        st->rho_hist[it & 1] = rho;
+      }
+    }
+    __syncthreads();
+    mz = s_cyc[0];
+    beta = s_cyc[1];
+  } else {
+    beta = st->beta;
+    mz = st->mean_z;
+  }
   struct DirLd {
     int i, j;
     bool ok;
@@ -1440,13 +1477,13 @@ int launch_class_up(hipStream_t s, const ClassRows<T>& L, const FusedTile& ft, i
 template <typename T>
 int launch_fine_dir(hipStream_t s, const FineOp& A, const FusedTile& ft, const T* z,
                     const double* p_old, double* p_new, double* q, double* partial,
-                    const SolveState* st, const LaunchClock& clk) {
+                    SolveState* st, const CycleFold& cf, const LaunchClock& clk) {
   if (A.hxc)
     PCFD_LAUNCH_TIMED(clk, (k_fine_dir<T, true>), dim3(ft.ntiles()), dim3(kBlock), ft.lds, s, A,
-                      ft, z, p_old, p_new, q, partial, st);
+                      ft, z, p_old, p_new, q, partial, st, cf);
   else
     PCFD_LAUNCH_TIMED(clk, (k_fine_dir<T, false>), dim3(ft.ntiles()), dim3(kBlock), ft.lds, s, A,
-                      ft, z, p_old, p_new, q, partial, st);
+                      ft, z, p_old, p_new, q, partial, st, cf);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
@@ -1543,8 +1580,8 @@ bool choose_tile(int nx, int ny, int rxs, int rys, int halo_mult, int fx, int fy
                                          const T*, const T*, int, int, int, T*,               \
                                          const SolveState*, const LaunchClock&);              \
   template int launch_fine_dir<T>(hipStream_t, const FineOp&, const FusedTile&, const T*,      \
-                                  const double*, double*, double*, double*, const SolveState*, \
-                                  const LaunchClock&);
+                                  const double*, double*, double*, double*, SolveState*,       \
+                                  const CycleFold&, const LaunchClock&);
 PCFD_VCYCLE_INST(float)
 PCFD_VCYCLE_INST(double)
 #undef PCFD_VCYCLE_INST

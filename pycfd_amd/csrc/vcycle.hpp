@@ -115,12 +115,21 @@ int launch_class_up(hipStream_t s, const ClassRows<T>& L, const FusedTile& ft, i
                     T w, int nu, const T* x_pre, const T* b, const T* xc, int fx, int fy, int cny,
                     T* x_out, const SolveState* st, const LaunchClock& clk = LaunchClock{});
 
+// The V-cycle's output sums (rows of sum z, sum b0 z, sum b0) left for the
+// direction kernel to fold; rows = 0: mean_z and beta are already in the state.
+struct CycleFold {
+  const double* partial = nullptr;
+  int rows = 0;
+  int n_proj = 0;
+};
+
 // Multigrid-CG direction and operator product on level 0 (halo-1 tiling):
 // p_new = (z - mean_z) + beta p_old, q = A p_new, per-tile partials of p.q.
 template <typename T>
 int launch_fine_dir(hipStream_t s, const FineOp& A, const FusedTile& ft, const T* z,
                     const double* p_old, double* p_new, double* q, double* partial,
-                    const SolveState* st, const LaunchClock& clk = LaunchClock{});
+                    SolveState* st, const CycleFold& cf = CycleFold{},
+                    const LaunchClock& clk = LaunchClock{});
 
 // x = C b with C an n x n dense FP64 matrix, spread over many workgroups.
 template <typename T>
