@@ -245,10 +245,20 @@ __device__ void mass_balance(const BCParams& P, double* u, double* v) {
                               : line_of(v, P.ldv, P.nx + 2, P.ny + 3, 1, wi.n_face);
       const int n = (wi.axis == 0) ? P.ny : P.nx;
       const double* wk = (wi.axis == 0) ? P.hy : P.hx;
+      // in bounds on every wall (ghosted lines); the weight test is uniform
+      // and stays outside the slots, so no load sits in a branch of its own
+      if (wk) {
 #pragma unroll
-      for (int t = 0; t < kSlots; ++t) {  // in bounds on every wall (ghosted lines)
-        const int k = min(t * (int)blockDim.x + (int)threadIdx.x, n - 1);
-        val[w][t] = wk ? f[1 + k] * wk[k] : f[1 + k];
+        for (int t = 0; t < kSlots; ++t) {
+          const int k = min(t * (int)blockDim.x + (int)threadIdx.x, n - 1);
+          val[w][t] = f[1 + k] * wk[k];
+        }
+      } else {
+#pragma unroll
+        for (int t = 0; t < kSlots; ++t) {
+          const int k = min(t * (int)blockDim.x + (int)threadIdx.x, n - 1);
+          val[w][t] = f[1 + k];
+        }
       }
     }
 #pragma unroll
@@ -346,8 +356,36 @@ __device__ void mask_obstacle(const BCParams& P, int mode, double dt, double* u,
   if (mode != BOP_MASK_ZERO) {
     // Uniform mesh: one area after the sum; stretched: each face weighted
     // by its control volume before it (solver.py:197-207).
-    auto ua = [&](int k) -> double { return P.uf_area ? u[P.uf[k]] * P.uf_area[k] : u[P.uf[k]]; };
-    auto va = [&](int k) -> double { return P.vf_area ? v[P.vf[k]] * P.vf_area[k] : v[P.vf[k]]; };
+    const bool wu = P.uf_area != nullptr, wv = P.vf_area != nullptr;
+    // One chunk of faces: field[idx[k]] (times area[k] when there is an area
+    // table; x * 1.0 == x exactly otherwise) and the body of each, at clamped
+    // faces.  The tests are uniform and sit around whole slot loops, so no
+    // load is in a branch of its own.
+    // First the index, body and area loads of u's and v's faces, then the
+    // field loads that need the indices, the area products after both: two
+    // memory round trips a chunk.
+    auto gather_idx = [&](int n, const int* idx, const double* area, const int8_t* body,
+                          int base, int (&id)[kSlots], double (&ar)[kSlots],
+                          int (&bod)[kSlots]) {
+      if (n <= 0) return;
+#pragma unroll
+      for (int t = 0; t < kSlots; ++t) {
+        const int k = min(base + t * (int)blockDim.x + (int)threadIdx.x, n - 1);
+        id[t] = idx[k];
+        bod[t] = body[k];
+      }
+      if (area) {
+#pragma unroll
+        for (int t = 0; t < kSlots; ++t)
+          ar[t] = area[min(base + t * (int)blockDim.x + (int)threadIdx.x, n - 1)];
+      }
+    };
+    auto gather_val = [&](int n, const double* field, const int (&id)[kSlots],
+                          double (&val)[kSlots]) {
+      if (n <= 0) return;
+#pragma unroll
+      for (int t = 0; t < kSlots; ++t) val[t] = field[id[t]];
+    };
     auto scaled = [&](double sum, bool weighted) -> double {
       return weighted ? sum / dt : sum * P.area / dt;
     };
@@ -363,16 +401,23 @@ __device__ void mask_obstacle(const BCParams& P, int mode, double dt, double* u,
 #pragma unroll
       for (int c = 0; c < K; ++c) acc[c] = 0.0;
       for (int base = 0; base < nmax; base += chunk) {
-        double uval[kSlots], vval[kSlots];
-        int ubody[kSlots], vbody[kSlots];
+        double uval[kSlots], vval[kSlots], uar[kSlots], var[kSlots];
+        int uid[kSlots], vid[kSlots], ubody[kSlots], vbody[kSlots];
 #pragma unroll
         for (int t = 0; t < kSlots; ++t) {
-          const int k = base + t * (int)blockDim.x + (int)threadIdx.x;
-          const int ku = min(k, P.n_uf - 1), kv = min(k, P.n_vf - 1);
-          uval[t] = (ku >= 0) ? ua(ku) : 0.0;
-          ubody[t] = (ku >= 0) ? (int)P.uf_body[ku] : -1;
-          vval[t] = (kv >= 0) ? va(kv) : 0.0;
-          vbody[t] = (kv >= 0) ? (int)P.vf_body[kv] : -1;
+          uval[t] = vval[t] = 0.0;
+          uar[t] = var[t] = 1.0;
+          uid[t] = vid[t] = 0;
+          ubody[t] = vbody[t] = -1;
+        }
+        gather_idx(P.n_uf, P.uf, P.uf_area, P.uf_body, base, uid, uar, ubody);
+        gather_idx(P.n_vf, P.vf, P.vf_area, P.vf_body, base, vid, var, vbody);
+        gather_val(P.n_uf, u, uid, uval);
+        gather_val(P.n_vf, v, vid, vval);
+#pragma unroll
+        for (int t = 0; t < kSlots; ++t) {  // after both, so the two rounds overlap
+          uval[t] *= uar[t];
+          vval[t] *= var[t];
         }
 #pragma unroll
         for (int t = 0; t < kSlots; ++t) {
@@ -396,7 +441,6 @@ __device__ void mask_obstacle(const BCParams& P, int mode, double dt, double* u,
       for (int c = 0; c < K; ++c) ops[c] = kSum;
       block_reduce<K>(acc, ops);
       if (threadIdx.x == 0) {
-        const bool wu = P.uf_area != nullptr, wv = P.vf_area != nullptr;
         const bool reset = mode == BOP_MASK_RESET;
         if (b0 == 0) {
           const double fx = scaled(acc[0], wu), fy = scaled(acc[1], wv);
