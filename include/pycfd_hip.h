@@ -265,8 +265,9 @@ int pcfd_time_kernel(pcfd_ctx* ctx, int32_t which, int32_t reps, double* avg_ms,
  * 4 / 5 fused stored-level down / up leg, 6-9 stored-level row passes
  * (zero-start smooth + residual, smooth, residual, prolongation), 10
  * restriction, 11 dense coarsest solve, 12 cooperative chain, 13 reductions
- * (level 0: cycle, 1: alpha (Jacobi-CG only; multigrid-CG's update kernel folds
- * alpha itself), 2: update).  No reference counterpart:
+ * (level 0: cycle (standalone multigrid; multigrid-CG's direction kernel folds
+ * the cycle's sums itself), 1: alpha (Jacobi-CG only; multigrid-CG's update
+ * kernel folds alpha itself), 2: update).  No reference counterpart:
  * measurement only. */
 #define PCFD_CLOCK_KINDS 14
 #define PCFD_CLOCK_LEVELS 16
