@@ -5,9 +5,10 @@ uniform inflow, zero-gradient OUTLET, slip walls, D = 1 body at 4 D, v seeded
 with a 1e-3 Gaussian blob; SSP-RK3, adaptive dt (ceiling 0.02, CFL 0.4);
 pressure by multigrid-preconditioned CG.  Both precision paths run: plain
 FP64, and mixed (FP32 V-cycle inside FP64 CG), at the same fixed divergence
-target: the relative residual tolerance (2.5e-11 by default) is chosen so
-that max|div u| <= 1e-10 after every step on both paths, and the line says
-whether it held.
+target: the relative residual tolerance (2e-11 by default) is chosen so
+that max|div u| <= 1e-10 after every step on both paths from the first one
+(the start-up steps are the largest: 5.6e-11; 2.5e-11 let steps 2-4 reach
+1.08e-10), and the line says whether it held.
 
 One "step" is one full SSP-RK3 time step (three projections) through
 Simulation.run, per-step diagnostics included, with fields resident in HBM.
@@ -207,7 +208,7 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--paths", default="fp64,mixed")
-    ap.add_argument("--tol", type=float, default=2.5e-11)
+    ap.add_argument("--tol", type=float, default=2e-11)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "

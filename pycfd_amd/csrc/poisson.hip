@@ -2133,16 +2133,17 @@ int PoissonEngine::apply_operator(const double* x, double* y) {
 }
 
 // Iteration driver shared by the solvers: enqueue a chunk of iterations,
-// then poll the device flag.  Kernels turn into no-ops once the flag is set,
-// so over-enqueueing only costs launch slots; the chunk follows the previous
-// solve's count (warm-started solves barely change) to keep polls rare.
-// PER is the number of iterations one body advances; MAXB bounds the bodies.
+// then poll the device flag.  Kernels turn into no-ops once the flag is set
+// (also when the start already decided the solve), so over-enqueueing only
+// costs launch slots; the first chunk is the previous solve's count
+// (warm-started solves barely change), so a solve that needs as many
+// iterations polls once.  Every poll drains the stream and leaves the device
+// idle until the next launch.  PER is the number of iterations one body
+// advances; MAXB bounds the bodies.
 #define PCFD_ITERATE(PER, MAXB, ...)                                          \
   do {                                                                        \
-    int _it = 0, _done = 0, _bodies = 0;                                      \
-    int _rc = poll(&_it, &_done);                                             \
-    if (_rc) return _rc;                                                      \
-    int _chunk = std::max(1, (predicted_ > 0 ? predicted_ - 1 : 4) / (PER));  \
+    int _it = 0, _done = 0, _bodies = 0, _rc = 0;                             \
+    int _chunk = std::max(1, (predicted_ > 0 ? predicted_ : 4) / (PER));      \
     while (!_done && _bodies < (MAXB)) {                                      \
       for (int _c = 0; _c < _chunk && _bodies < (MAXB); ++_c, ++_bodies) {    \
         __VA_ARGS__;                                                          \

@@ -402,45 +402,98 @@ k_divergence(const double* __restrict__ u, const double* __restrict__ v, int nx,
 //   out[3] max|u-u_prev| out[4] max|v-v_prev| out[5] max|div| over fluid
 //   out[6] sum(uc^2+vc^2) over cells, each weighted by its area on a
 //          stretched mesh (mesh.py:213-223)
+// A workgroup covers kDiagTiles tiles of the (nx+3) x (ny+3) cell range:
+// every load of its cells is issued first, at indices clamped into each
+// array's range (masked out afterwards), so no load waits on a branch; one
+// row of partials serves the workgroup.  Optional inputs come as `have`
+// bits (kHave*) with the launcher standing in a readable array of at least
+// the same extent for an absent one, so the loads need no tests.
+constexpr int kDiagTiles = 4;
+constexpr int kHaveP = 1, kHavePrev = 2, kHaveSolid = 4, kHaveHx = 8, kHaveHy = 16;
+
+static inline int diag_blocks(int nx, int ny) {
+  return (num_tiles(nx + 3, ny + 3) + kDiagTiles - 1) / kDiagTiles;
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_diagnostics(const double* __restrict__ u, const double* __restrict__ v,
               const double* __restrict__ p, const double* __restrict__ up,
-              const double* __restrict__ vp, const uint8_t* __restrict__ solid, int nx, int ny,
-              int ldu, int ldv, int ldp, double dx, double dy, Metrics M, double* partial) {
-  const Tile2D t = tile_coords(ny + 3);
-  const int i = t.i, j = t.j;
+              const double* __restrict__ vp, const uint8_t* __restrict__ solid,
+              const double* __restrict__ hx, const double* __restrict__ hy, int have, int nx,
+              int ny, int ldu, int ldv, int ldp, double dx, double dy, double* partial) {
+  const bool has_p = have & kHaveP, has_prev = have & kHavePrev, has_s = have & kHaveSolid;
+  const bool has_hx = have & kHaveHx, has_hy = have & kHaveHy;
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int ty = tiles_y(ny + 3), ntile = tiles_x(nx + 3) * ty;
+  int ci[kDiagTiles], cj[kDiagTiles];
+  bool in[kDiagTiles];
+  double au[kDiagTiles], av[kDiagTiles], aup[kDiagTiles], avp[kDiagTiles], ap[kDiagTiles];
+  double ua[kDiagTiles], ub[kDiagTiles], va[kDiagTiles], vb[kDiagTiles];
+  double hxv[kDiagTiles], hyv[kDiagTiles];
+  uint8_t so[kDiagTiles];
+#pragma unroll
+  for (int q = 0; q < kDiagTiles; ++q) {
+    const int t = blk * kDiagTiles + q;
+    const int i = (t / ty) * kTileX + (int)threadIdx.x / kTileY;
+    const int j = (t % ty) * kTileY + (int)threadIdx.x % kTileY;
+    ci[q] = i;
+    cj[q] = j;
+    in[q] = t < ntile;
+    const size_t ku = (size_t)min(i, nx + 2) * ldu + min(j, ny + 1);
+    const size_t kv = (size_t)min(i, nx + 1) * ldv + min(j, ny + 2);
+    const int ii = max(1, min(i, nx)), jj = max(1, min(j, ny));
+    au[q] = u[ku];
+    av[q] = v[kv];
+    ua[q] = u[(size_t)ii * ldu + jj];
+    ub[q] = u[(size_t)(ii + 1) * ldu + jj];
+    va[q] = v[(size_t)ii * ldv + jj];
+    vb[q] = v[(size_t)ii * ldv + jj + 1];
+    aup[q] = up[ku];
+    avp[q] = vp[kv];
+    ap[q] = p[(size_t)min(i, nx + 1) * ldp + min(j, ny + 1)];
+    so[q] = solid[(size_t)(ii - 1) * ny + (jj - 1)];
+    hxv[q] = hx[ii - 1];
+    hyv[q] = hy[jj - 1];
+  }
+#pragma unroll
+  for (int q = 0; q < kDiagTiles; ++q) {
+    if (!has_s) so[q] = 0;
+    if (!has_hx) hxv[q] = dx;
+    if (!has_hy) hyv[q] = dy;
+  }
   double r[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  if (i <= nx + 2 && j <= ny + 1) {
-    const size_t k = (size_t)i * ldu + j;
-    const double a = u[k];
-    if (!isfinite(a)) r[2] += 1.0;
-    if (i >= 1 && i <= nx + 1 && j >= 1 && j <= ny) r[0] = fabs(a);
-    if (up) r[3] = fabs(a - up[k]);
-  }
-  if (i <= nx + 1 && j <= ny + 2) {
-    const size_t k = (size_t)i * ldv + j;
-    const double a = v[k];
-    if (!isfinite(a)) r[2] += 1.0;
-    if (i >= 1 && i <= nx && j >= 1 && j <= ny + 1) r[1] = fabs(a);
-    if (vp) r[4] = fabs(a - vp[k]);
-  }
-  if (p && i <= nx + 1 && j <= ny + 1) {
-    if (!isfinite(p[(size_t)i * ldp + j])) r[2] += 1.0;
-  }
-  if (i >= 1 && i <= nx && j >= 1 && j <= ny) {
-    const double ua = u[(size_t)i * ldu + j], ub = u[(size_t)(i + 1) * ldu + j];
-    const double va = v[(size_t)i * ldv + j], vb = v[(size_t)i * ldv + j + 1];
-    const double hx = M.hx ? M.hx[i - 1] : dx, hy = M.hy ? M.hy[j - 1] : dy;
-    if (!solid || !solid[(size_t)(i - 1) * ny + (j - 1)]) {
-      const double d = (ub - ua) / hx + (vb - va) / hy;
-      r[5] = fabs(d);
+#pragma unroll
+  for (int q = 0; q < kDiagTiles; ++q) {
+    if (!in[q]) continue;
+    const int i = ci[q], j = cj[q];
+    if (i <= nx + 2 && j <= ny + 1) {
+      const double a = au[q];
+      if (!isfinite(a)) r[2] += 1.0;
+      if (i >= 1 && i <= nx + 1 && j >= 1 && j <= ny) r[0] = fmax(r[0], fabs(a));
+      if (has_prev) r[3] = fmax(r[3], fabs(a - aup[q]));
     }
-    const double uc = 0.5 * (ua + ub), vc = 0.5 * (va + vb);
-    r[6] = uc * uc + vc * vc;
-    if (M.hx) r[6] = r[6] * (hx * hy);
+    if (i <= nx + 1 && j <= ny + 2) {
+      const double a = av[q];
+      if (!isfinite(a)) r[2] += 1.0;
+      if (i >= 1 && i <= nx && j >= 1 && j <= ny + 1) r[1] = fmax(r[1], fabs(a));
+      if (has_prev) r[4] = fmax(r[4], fabs(a - avp[q]));
+    }
+    if (has_p && i <= nx + 1 && j <= ny + 1) {
+      if (!isfinite(ap[q])) r[2] += 1.0;
+    }
+    if (i >= 1 && i <= nx && j >= 1 && j <= ny) {
+      if (!so[q]) {
+        const double d = (ub[q] - ua[q]) / hxv[q] + (vb[q] - va[q]) / hyv[q];
+        r[5] = fmax(r[5], fabs(d));
+      }
+      const double uc = 0.5 * (ua[q] + ub[q]), vc = 0.5 * (va[q] + vb[q]);
+      double ke = uc * uc + vc * vc;
+      if (has_hx) ke = ke * (hxv[q] * hyv[q]);
+      r[6] += ke;
+    }
   }
   const int ops[7] = {kMax, kMax, kSum, kMax, kMax, kMax, kSum};
-  block_partials<7>(r, ops, partial, t.tile);
+  block_partials<7>(r, ops, partial, blk);
 }
 
 struct EpiDiag {
@@ -558,11 +611,21 @@ int launch_diagnostics(hipStream_t s, const double* u, const double* v, const do
                        int ldu, int ldv, int ldp, double dx, double dy, const Metrics& g,
                        double* partial, unsigned int* counter, double* out) {
   (void)counter;
-  hipLaunchKernelGGL(k_diagnostics, grid_for(nx + 3, ny + 3), dim3(kBlock), 0, s, u, v, p, up,
-                     vp, solid, nx, ny, ldu, ldv, ldp, dx, dy, g, partial);
+  const int nb = diag_blocks(nx, ny);
+  // Stand-ins for absent inputs: u's allocation ((nx+3) x ldu doubles,
+  // ldu >= ny+2) holds every index the kernel forms for p (rows <= nx+1,
+  // columns <= ny+1, row pitch ldu), the solid flags (nx*ny bytes), hx (nx)
+  // and hy (ny); v's the same for v_prev.  Their values are never used.
+  const bool prev = up && vp;
+  const int have = (p ? kHaveP : 0) | (prev ? kHavePrev : 0) | (solid ? kHaveSolid : 0) |
+                   (g.hx ? kHaveHx : 0) | (g.hy ? kHaveHy : 0);
+  hipLaunchKernelGGL(k_diagnostics, dim3(nb), dim3(kBlock), 0, s, u, v, p ? p : u,
+                     prev ? up : u, prev ? vp : v, solid ? solid : (const uint8_t*)u,
+                     g.hx ? g.hx : u, g.hy ? g.hy : u, have, nx, ny, ldu, ldv, p ? ldp : ldu, dx,
+                     dy, partial);
   PCFD_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_fold<EpiDiag>, dim3(1), dim3(kBlock), 0, s, (const double*)partial,
-                     num_tiles(nx + 3, ny + 3), EpiDiag{out});
+  hipLaunchKernelGGL(k_fold<EpiDiag>, dim3(1), dim3(kBlock), 0, s, (const double*)partial, nb,
+                     EpiDiag{out});
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
