@@ -186,12 +186,19 @@ __device__ __forceinline__ void fold_partials(double (&v)[K], const int (&ops)[K
   for (int k = 0; k < K; ++k) v[k] = (ops[k] == kSum) ? 0.0 : -INFINITY;
   for (int t0 = threadIdx.x; t0 < ntiles; t0 += kBlock * kBatch) {
     double x[kBatch][K];
+    // unconditional loads of a clamped row, the identity selected after:
+    // no load sits in a branch of its own
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
-      const int t = t0 + u * kBlock;
+      const size_t row = (size_t)min(t0 + u * kBlock, ntiles - 1) * K;
+#pragma unroll
+      for (int k = 0; k < K; ++k) x[u][k] = partial[row + k];
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
 #pragma unroll
       for (int k = 0; k < K; ++k)
-        x[u][k] = (t < ntiles) ? partial[(size_t)t * K + k] : ((ops[k] == kSum) ? 0.0 : -INFINITY);
+        if (t0 + u * kBlock >= ntiles) x[u][k] = (ops[k] == kSum) ? 0.0 : -INFINITY;
     }
     // Same order as a plain strided loop: rows t0, t0 + kBlock, ...
 #pragma unroll
