@@ -205,6 +205,22 @@ def test_solver_failure_is_raised(gpu_available):
         s.pressure_solver.solve(rhs)
 
 
+@pytest.mark.parametrize("kind", ["cg", "mgcg"])
+def test_zero_rhs_after_a_solve(gpu_available, kind):
+    """A zero right-hand side right after a solve that sized the iteration
+    chunk: scipy's cg returns b itself for ||b|| = 0 (pressure.py:389, 620),
+    so the pressure is zero after no iterations; every iteration the host
+    enqueued before polling must be a no-op."""
+    s = _solver("cavity", pressure_solver=kind, poisson_tol=1e-10, poisson_maxiter=2000)
+    rhs = np.random.default_rng(5).standard_normal((24, 20))
+    rhs -= rhs.mean()
+    s.pressure_solver.solve(rhs)
+    assert s.pressure_solver.last_iterations > 0
+    p = s.pressure_solver.solve(np.zeros_like(rhs))
+    assert s.pressure_solver.last_iterations == 0
+    assert np.all(p == 0.0)
+
+
 # ------------------------------------------------------------ whole steps
 TRAJ = [("cavity", "direct", 20), ("cavity", "cg", 5), ("periodic", "direct", 10),
         ("channel", "direct", 10), ("cyl_outlet", "direct", 10), ("cyl_pout", "direct", 10)]
