@@ -85,6 +85,7 @@ def run_path(precision: str, steps: int, warmup: int, nx: int, ny: int, tol: flo
     import torch
 
     from pycfd_amd import cases
+    from pycfd_amd.analysis.postprocess import force_coefficients
     from pycfd_amd.physics.incompressible import Simulation
 
     cfg, body, u0, v0 = cases.cylinder(nx=nx, ny=ny, pressure_solver="mgcg",
@@ -108,6 +109,8 @@ def run_path(precision: str, steps: int, warmup: int, nx: int, ny: int, tol: flo
     wall = time.perf_counter() - t0
     hist = res.history
     max_div = float(max(hist["max_div"]))
+    cd_steps = np.array([force_coefficients((fx, fy), cfg.u_ref, cfg.l_ref)[0]
+                         for fx, fy in zip(hist["fx"], hist["fy"])])
     out = {
         "mlups": nx * ny * steps / wall / 1e6,
         "ms_per_step": 1e3 * wall / steps,
@@ -115,8 +118,11 @@ def run_path(precision: str, steps: int, warmup: int, nx: int, ny: int, tol: flo
         "div_ok": max_div <= DIV_TARGET,
         "iterations_last_solve": int(sim.solver.pressure_solver.last_iterations),
         "cd_last": float(sim.force_coefficients()[0]),
+        # mean drag coefficient over the timed steps, from each step's reaction
+        "cd_mean": float(np.mean(cd_steps)),
         "mg_plan": [[p["nx"], p["ny"], p["mode"], *p["tile"]] for p in plan],
         "t": float(sim.fields.t),
+        "_cd_steps": cd_steps,  # per-step Cd, compared across paths, not printed
     }
     log(f"[{precision}] {steps} steps in {wall:.3f} s -> {out['mlups']:.1f} MLUPS, "
         f"max|div| {out['max_div']:.2e}, {out['iterations_last_solve']} it/solve")
@@ -255,13 +261,23 @@ def main() -> int:
                                "SSP-RK3",
                    "path": main_path, "nx": args.nx, "ny": args.ny},
         "paths": {k: {kk: (float(f"{vv:.5g}") if isinstance(vv, float) else vv)
-                      for kk, vv in v.items()} for k, v in results.items()},
+                      for kk, vv in v.items() if not kk.startswith("_")}
+                  for k, v in results.items()},
         "roofline": dominant,
         "kernels": kernels,
         "cpu_baseline": cpu,
     }
     if "fp64" in results and "mixed" in results:
         line["mixed_speedup_vs_fp64"] = round(results["mixed"]["mlups"] / results["fp64"]["mlups"], 3)
+        # the same steps on both paths: mean Cd of the mixed path against
+        # FP64, and the largest per-step relative difference
+        cf, cm = results["fp64"]["cd_mean"], results["mixed"]["cd_mean"]
+        import numpy as np
+        sf, sm = results["fp64"]["_cd_steps"], results["mixed"]["_cd_steps"]
+        if cf and len(sf) == len(sm):
+            line["cd_mean_rel_diff_mixed_vs_fp64"] = float(f"{abs(cm - cf) / abs(cf):.3g}")
+            line["cd_step_max_rel_diff_mixed_vs_fp64"] = float(
+                f"{float(np.max(np.abs(sm - sf) / np.abs(sf))):.3g}")
     print(json.dumps(line), flush=True)
     return 0
 
