@@ -25,6 +25,12 @@ def device_zeros(shape) -> "object":
     return torch.zeros(shape, dtype=torch.float64, device="cuda")
 
 
+def device_empty(shape) -> "object":
+    """Uninitialised device array, for outputs a kernel writes in full."""
+    torch = _torch()
+    return torch.empty(shape, dtype=torch.float64, device="cuda")
+
+
 def to_device(a) -> "object":
     torch = _torch()
     if isinstance(a, torch.Tensor):

@@ -18,7 +18,7 @@ import numpy as np
 from ..config import AdvectionScheme, BCKind, SimulationConfig, SolverType
 from .boundary import BoundaryManager
 from .engine import DeviceContext
-from .fields import FlowField, device_zeros, to_device, to_host
+from .fields import FlowField, device_empty, device_zeros, to_device, to_host
 from .mesh import StructuredMesh
 from .pressure import assemble_poisson_matrix, make_pressure_solver
 
@@ -250,7 +250,9 @@ class ProjectionSolver:
     def step(self, fields: FlowField, dt: float) -> FlowField:
         """Advance by ``dt`` with the configured scheme (solver.py:686-731)."""
         mesh = self.mesh
-        u, v, p = device_zeros(mesh.u_shape), device_zeros(mesh.v_shape), device_zeros(mesh.p_shape)
+        # Every entry is written by the step: the last substep's predictor
+        # covers the full ghosted u and v, the scatter the full ghosted p.
+        u, v, p = device_empty(mesh.u_shape), device_empty(mesh.v_shape), device_empty(mesh.p_shape)
         info = self.ctx.step(fields.u_dev, fields.v_dev, dt, self._blend, u, v, p)
         self.pressure_solver.record(info.iterations, info.residual, bool(info.converged))
         self._has_guess = True
