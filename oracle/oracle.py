@@ -21,7 +21,12 @@ follows.  Pinning (see tests/test_oracle.py):
   hierarchy / V-cycle ............. golden unknowns, weights, nnz and z = M r
                                     (multigrid.npz); dense symmetry of M
   step / run ...................... golden trajectories (trajectory.npz),
-                                    Taylor-Green decay exp(-2 nu t)
+                                    Taylor-Green decay exp(-2 nu t); Euler,
+                                    RK2 and the upwind blend rule
+                                    (schemes.npz); the 128 x 128 cavity's
+                                    2000 CG steps (cavity_c1.npz)
+  mask, per-body forces ........... two bodies in tandem (schemes.npz):
+                                    per-body forces per step, their sum
 """
 
 from __future__ import annotations
@@ -52,6 +57,7 @@ class Case:
     profile: tuple = ("uniform",) * 4
     p_ref: tuple = (0.0,) * 4
     solid: np.ndarray | None = None
+    bodies: tuple | None = None              # per-body masks (> 1 body), union = solid
     scheme: str = "rk3"
     pressure_solver: str = "direct"
     tol: float = 1e-10
@@ -427,12 +433,13 @@ def upd_ranges(case: Case):
     return u_lo, u_hi, v_lo, v_hi
 
 
-def face_masks(case: Case):
-    """solver.py:62-79, 276-283."""
-    if case.solid is None:
+def face_masks(case: Case, solid=None):
+    """solver.py:62-79, 276-283 (``solid`` defaults to the union of bodies;
+    one body's mask gives that body's face set, solver.py:289-296)."""
+    s = case.solid if solid is None else np.asarray(solid, dtype=bool)
+    if s is None:
         return None, None
     u_lo, u_hi, v_lo, v_hi = upd_ranges(case)
-    s = case.solid
 
     def faces(lo, hi, axis, periodic):
         m = np.arange(lo, hi)
@@ -453,6 +460,7 @@ class State:
 
     p_guess: np.ndarray | None = None
     force: tuple = (0.0, 0.0)
+    forces: tuple = ()   # per body, when Case.bodies holds more than one
     last_iterations: int = 0
     last_residual: float = 0.0
     hierarchy: object = None
@@ -462,7 +470,9 @@ class State:
 
 
 def mask(case: Case, st: State, u, v, dt, reset: bool = False) -> None:
-    """solver.py:553-599 (single body)."""
+    """solver.py:553-599: zero the faces touching a solid; with dt, the
+    removed momentum per unit time is the reaction force, in total and, with
+    several bodies, per body over each body's own face set."""
     if case.solid is None:
         return
     u_lo, u_hi, v_lo, v_hi = upd_ranges(case)
@@ -472,14 +482,26 @@ def mask(case: Case, st: State, u, v, dt, reset: bool = False) -> None:
     if dt is not None:
         if case.uniform:
             area = case.dx * case.dy
-            fx = float(uf[um].sum()) * area / dt
-            fy = float(vf[vm].sum()) * area / dt
+
+            def fsum(faces, m):  # solver.py:197-207, scalar area after the sum
+                return float(faces[m].sum()) * area
+            ua = va = None
         else:  # solver.py:197-207, 271-274: control-volume weighted sums
             ua = _sl(case.hxu, u_lo - 1, u_hi - 1, 0) * case.hy
             va = case.hx * _sl(case.hyv, v_lo - 1, v_hi - 1, 1)
-            fx = float((uf * ua)[um].sum()) / dt
-            fy = float((vf * va)[vm].sum()) / dt
+
+            def fsum(faces, m):
+                return float((faces * (ua if faces is uf else va))[m].sum())
+        fx = fsum(uf, um) / dt
+        fy = fsum(vf, vm) / dt
         st.force = (fx, fy) if reset else (st.force[0] + fx, st.force[1] + fy)
+        if case.bodies:
+            per = []
+            for b in case.bodies:
+                bu, bv = face_masks(case, b)
+                per.append((fsum(uf, bu) / dt, fsum(vf, bv) / dt))
+            st.forces = tuple(per) if reset or not st.forces else tuple(
+                (a[0] + c[0], a[1] + c[1]) for a, c in zip(st.forces, per))
     uf[um] = 0.0
     vf[vm] = 0.0
 
@@ -921,7 +943,8 @@ def run(case: Case, steps: int, u_init=None, v_init=None, st: State | None = Non
     st = st or State()
     u, v, p = initialize(case, st, u_init, v_init)
     t = 0.0
-    hist = {"dt": [], "max_div": [], "kinetic_energy": [], "fx": [], "fy": []}
+    hist = {"dt": [], "max_div": [], "kinetic_energy": [], "fx": [], "fy": [], "fbody": [],
+            "blend": []}
     nx, ny = case.nx, case.ny
     for _ in range(steps):
         dt = compute_dt(case, u, v)
@@ -948,4 +971,6 @@ def run(case: Case, steps: int, u_init=None, v_init=None, st: State | None = Non
         hist["kinetic_energy"].append(0.5 * ke2)
         hist["fx"].append(st.force[0])
         hist["fy"].append(st.force[1])
+        hist["fbody"].append(np.asarray(st.forces, dtype=float).ravel())
+        hist["blend"].append(blend)
     return u, v, p, hist, t

@@ -324,6 +324,100 @@ def les_stretched_fixture():
     np.savez_compressed(OUT / "les_stretched.npz", **out)
 
 
+def schemes_fixture():
+    """Euler and RK2 steps (solver.py:699-707), the upwind blend rule
+    (solver.py:734-756), two bodies with per-body forces (solver.py:136-146,
+    197-207) and the Strouhal reduction (postprocess.py:116-142)."""
+    from pycfd.analysis.postprocess import strouhal_number
+    from pycfd.cases import cylinder_flow
+    from pycfd.core.mesh import StructuredMesh
+    out = {}
+    runs = [("cavity", dict(time_scheme="euler"), 10, "euler"),
+            ("cavity", dict(time_scheme="rk2"), 10, "rk2"),
+            ("cyl_outlet", dict(time_scheme="euler"), 8, "euler"),
+            ("cyl_outlet", dict(time_scheme="rk2"), 8, "rk2"),
+            ("cavity", dict(advection_scheme="upwind"), 10, "upwind"),
+            ("cyl_outlet", dict(advection_scheme="upwind"), 8, "upwind"),
+            ("cyl_outlet", dict(advection_scheme="upwind", upwind_blend=0.4), 8, "upwind04")]
+    for name, kw, steps, label in runs:
+        cfg = config(name, pressure_solver="direct", poisson_tol=1e-12, use_numba=False, **kw)
+        mesh = StructuredMesh.from_config(cfg)
+        body = body_for(name, mesh)
+        u0, v0 = init_for(name, mesh)
+        sim = Simulation(cfg, obstacle=body, u_init=u0, v_init=v0)
+        res = sim.run(max_steps=steps)
+        tag = f"{name}_{label}"
+        out[f"{tag}_u"], out[f"{tag}_v"], out[f"{tag}_p"] = sim.fields.u, sim.fields.v, sim.fields.p
+        for k in ("dt", "max_div", "fx", "fy"):
+            out[f"{tag}_{k}"] = np.asarray(res.history[k])
+        out[f"{tag}_blend"] = np.array(sim.solver._blend)
+    # two bodies in tandem (test_multibody.py:43-50, 238-246) on a short run
+    m = StructuredMesh(96, 48, 24.0, 12.0)
+    from pycfd.geometry.obstacles import ObstacleGroup
+    lead = circle_mask(m, (6.0, 6.0), 0.5, name="lead")
+    trail = circle_mask(m, (10.0, 6.0), 0.5, name="trail")
+    sim = cylinder_flow.build(re=40.0, nx=96, ny=48, domain_length=24.0, domain_height=12.0,
+                              obstacle=ObstacleGroup((lead, trail)), t_end=100.0)
+    fx, fy, fb = [], [], []
+
+    def record(fields, info):
+        fx.append(sim.solver.body_force_reaction[0])
+        fy.append(sim.solver.body_force_reaction[1])
+        fb.append(np.asarray(sim.solver.body_force_reactions, dtype=float).ravel())
+
+    sim.run(max_steps=30, callback=record, callback_every=1)
+    out["tandem_config"] = np.array(sim.config.to_json())
+    out["tandem_lead"], out["tandem_trail"] = lead.mask, trail.mask
+    out["tandem_u"], out["tandem_v"], out["tandem_p"] = sim.fields.u, sim.fields.v, sim.fields.p
+    out["tandem_fx"], out["tandem_fy"] = np.array(fx), np.array(fy)
+    out["tandem_fbody"] = np.array(fb)
+    out["tandem_cd_by_body"] = np.array([sim.force_coefficients_by_body()[k]
+                                         for k in ("lead", "trail")])
+    out["tandem_cd"] = np.array(sim.force_coefficients())
+    # Strouhal on adaptive time stamps: a shedding-like signal with a harmonic
+    # and noise, and a record too short to resolve
+    rng = np.random.default_rng(515)
+    for k, (n, f0) in enumerate(((400, 0.19), (1500, 0.1653), (64, 0.5), (12, 0.2))):
+        t = np.cumsum(0.02 + 0.01 * rng.random(n))
+        s = (np.sin(2 * np.pi * f0 * t) + 0.3 * np.sin(4 * np.pi * f0 * t + 0.7)
+             + 0.05 * rng.standard_normal(n) + 1.3)
+        out[f"st{k}_t"], out[f"st{k}_s"] = t, s
+        out[f"st{k}_St"] = np.array(strouhal_number(t, s, 1.0, 1.0))
+        out[f"st{k}_St_scaled"] = np.array(strouhal_number(t, s, 0.5, 2.0))
+    np.savez_compressed(OUT / "schemes.npz", **out)
+
+
+def cavity_c1_fixture():
+    """The 128 x 128 lid-driven cavity exactly as the CPU-runnable benchmark
+    states it: Re 100, CG to relative residual 1e-10 (and 1e-13, the tight
+    pin), CFL 0.5 with the case's dt ceiling 0.01, 2000 SSP-RK3 steps
+    (lid_driven_cavity.py:59-85, pressure.py:366-398)."""
+    import time
+    out = {}
+    for tol, tag in ((1e-10, "t10"), (1e-13, "t13")):
+        cfg = SimulationConfig(
+            name="cavity_c1", nx=128, ny=128, lx=1.0, ly=1.0, re=100.0, u_ref=1.0, l_ref=1.0,
+            dt=0.01, cfl_max=0.5, t_end=1e9, pressure_solver="cg", poisson_tol=tol,
+            use_numba=False, boundary_config=walls(top=BCSpec(BCKind.MOVING_WALL, velocity=1.0)))
+        sim = Simulation(cfg)
+        its = []
+
+        def record(fields, info):
+            its.append(sim.solver.pressure_solver.last_iterations)
+
+        w0 = time.perf_counter()
+        res = sim.run(max_steps=2000, callback=record, callback_every=1)
+        wall = time.perf_counter() - w0
+        print(f"C1 tol {tol:g}: {res.steps} steps in {wall:.1f} s", flush=True)
+        out[f"{tag}_u"], out[f"{tag}_v"], out[f"{tag}_p"] = sim.fields.u, sim.fields.v, sim.fields.p
+        for k in ("dt", "max_div", "kinetic_energy", "residual"):
+            out[f"{tag}_{k}"] = np.asarray(res.history[k])
+        out[f"{tag}_iters"] = np.array(its[:-1])
+        out[f"{tag}_t"] = np.array(sim.fields.t)
+        out[f"{tag}_wall"] = np.array(wall)
+    np.savez_compressed(OUT / "cavity_c1.npz", **out)
+
+
 def mask_fixture():
     from pycfd.core.mesh import StructuredMesh
     out = {}
@@ -337,9 +431,10 @@ def mask_fixture():
 
 if __name__ == "__main__":
     which = sys.argv[1:] or ["rhs", "boundary", "trajectory", "multigrid", "mask", "les",
-                             "stretched", "les_stretched"]
+                             "stretched", "les_stretched", "schemes", "cavity_c1"]
     for w in which:
         print("writing", w, flush=True)
         {"rhs": rhs_fixture, "boundary": boundary_fixture, "trajectory": trajectory_fixture,
          "multigrid": multigrid_fixture, "mask": mask_fixture, "les": les_fixture,
-         "stretched": stretched_fixture, "les_stretched": les_stretched_fixture}[w]()
+         "stretched": stretched_fixture, "les_stretched": les_stretched_fixture,
+         "schemes": schemes_fixture, "cavity_c1": cavity_c1_fixture}[w]()
