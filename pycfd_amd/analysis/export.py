@@ -33,13 +33,26 @@ def provenance_record(config: SimulationConfig | None = None, extra: dict | None
         "generated_utc": datetime.now(timezone.utc).isoformat(timespec="seconds"),
         "python": platform.python_version(),
         "platform": platform.platform(),
-        "units": {"system": "nondimensional"},
     }
     if config is not None:
         record["config"] = config.to_dict()
     if extra:
         record.update(extra)
     return record
+
+
+def units_record() -> dict:
+    """The ``units`` block of every written record: the solver's own
+    non-dimensional units (export.py:69-77, no rescaling on this path)."""
+    return {"system": "solver", "note": "non-dimensional; u_ref = l_ref = 1"}
+
+
+def with_units(record: dict) -> dict:
+    """A copy of ``record`` carrying the units block, also for a record the
+    caller supplied (export.py:80-84)."""
+    out = dict(record)
+    out["units"] = units_record()
+    return out
 
 
 def save_checkpoint(fields: FlowField, config: SimulationConfig, path: str | Path,
@@ -53,7 +66,7 @@ def save_checkpoint(fields: FlowField, config: SimulationConfig, path: str | Pat
     if path.suffix != ".npz":
         path = path.with_suffix(".npz")
     path.parent.mkdir(parents=True, exist_ok=True)
-    record = provenance_record(config) if provenance is None else dict(provenance)
+    record = with_units(provenance_record(config) if provenance is None else provenance)
     arrays = dict(u=fields.u, v=fields.v, p=fields.p, t=np.array(fields.t),
                   step=np.array(fields.step), config=np.array(config.to_json()),
                   provenance=np.array(json.dumps(record)))
@@ -108,4 +121,4 @@ def load_checkpoint_with_guess(path: str | Path):
 
 
 __all__ = ["checkpoint_provenance", "load_checkpoint", "load_checkpoint_with_guess",
-           "provenance_record", "save_checkpoint"]
+           "provenance_record", "save_checkpoint", "units_record", "with_units"]

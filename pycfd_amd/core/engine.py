@@ -4,11 +4,38 @@ arithmetic all happens in the HIP kernels."""
 
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import sys
+import weakref
 
 import numpy as np
 
 from .. import _lib
+
+# Contexts still open.  Each holds device memory, events and pinned host
+# memory of the HIP runtime, so each is destroyed while the runtime is
+# certainly alive: by close(), by garbage collection during the program, or
+# by the exit hook below -- never from a finaliser that runs after the
+# interpreter (and with it torch's and the runtime's own exit handlers) has
+# begun to shut down, where a HIP call can touch a finalised runtime.
+_LIVE: "weakref.WeakSet[DeviceContext]" = weakref.WeakSet()
+_EXIT_HOOKED = False
+
+
+def _close_all() -> None:
+    for ctx in list(_LIVE):
+        ctx.close()
+
+
+def _track(ctx: "DeviceContext") -> None:
+    """Register the exit hook on first use: after torch initialised the GPU,
+    so (exit hooks run last-registered first) it runs before torch's."""
+    global _EXIT_HOOKED
+    if not _EXIT_HOOKED:
+        atexit.register(_close_all)
+        _EXIT_HOOKED = True
+    _LIVE.add(ctx)
 
 
 def _ptr(t) -> C.c_void_p:
@@ -86,13 +113,18 @@ class DeviceContext:
                                              C.byref(handle)), "pcfd_ctx_create")
         self.handle = handle
         self._keep = []
+        _track(self)
 
     def close(self) -> None:
+        """Destroy the device context (waits for its stream first)."""
         if getattr(self, "handle", None) is not None and self.handle.value:
             self._lib.pcfd_ctx_destroy(self.handle)
             self.handle = None
+        _LIVE.discard(self)
 
     def __del__(self):
+        if sys.is_finalizing():
+            return  # the exit hook closed every context the program left open
         try:
             self.close()
         except Exception:

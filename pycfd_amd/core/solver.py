@@ -73,6 +73,7 @@ class ProjectionSolver:
             for k, b in enumerate(self.body_masks):
                 if b.shape != (nx, ny):
                     raise ValueError(f"body {k} mask must have shape {(nx, ny)}, got {b.shape}")
+            self._check_bodies(nx, ny)
         self._build_slices()
         self.dirichlet_pressure = self.boundaries.dirichlet_pressure()
         self.poisson = assemble_poisson_matrix(
@@ -119,6 +120,34 @@ class ProjectionSolver:
         self._has_guess = False
         self._force = np.zeros(2 + 2 * len(self.body_masks))
         self._force_stale = False
+
+    MAX_BODIES = 128  # the device keeps one signed byte of body index per cell
+
+    def _check_bodies(self, nx: int, ny: int) -> None:
+        """Each face is charged to exactly one body (solver.py:289-296 builds
+        disjoint face sets for bodies that neither overlap nor touch), so
+        bodies that overlap or share a face are refused, as are more bodies
+        than the device's body index holds."""
+        if len(self.body_masks) > self.MAX_BODIES:
+            raise ValueError(f"at most {self.MAX_BODIES} bodies are supported, "
+                             f"got {len(self.body_masks)}")
+        ids = np.full((nx, ny), -1, dtype=np.int32)
+        for k, b in enumerate(self.body_masks):
+            if np.any(ids[b] >= 0):
+                raise ValueError(f"body {k} overlaps body {int(ids[b].max())}")
+            ids[b] = k
+        for axis, periodic in ((0, self.periodic_x), (1, self.periodic_y)):
+            a = ids
+            b = np.roll(ids, -1, axis=axis)
+            pair = (a >= 0) & (b >= 0) & (a != b)
+            if not periodic:  # the wrap-around pair is not a face
+                last = [slice(None)] * 2
+                last[axis] = -1
+                pair[tuple(last)] = False
+            if pair.any():
+                i = np.argwhere(pair)[0]
+                raise ValueError(f"bodies {int(a[tuple(i)])} and {int(b[tuple(i)])} touch: "
+                                 "they share a face with no fluid between them")
 
     # ------------------------------------------------------------------ #
     def _build_slices(self) -> None:

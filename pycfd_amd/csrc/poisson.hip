@@ -2065,12 +2065,13 @@ int PoissonEngine::read_state(SolveState* st) {
   return kOk;
 }
 
-int PoissonEngine::poll(int* iter, int* done) {
+int PoissonEngine::poll(int* iter, int* done, int* converged) {
   PCFD_HIP_CHECK(hipMemcpyAsync(st_host_, st_, sizeof(SolveState), hipMemcpyDeviceToHost, s_));
   PCFD_HIP_CHECK(hipEventRecord(ev_, s_));
   PCFD_HIP_CHECK(hipEventSynchronize(ev_));
   *iter = st_host_->iter;
   *done = st_host_->done;
+  *converged = st_host_->converged;
   return kOk;
 }
 
@@ -2135,24 +2136,26 @@ int PoissonEngine::apply_operator(const double* x, double* y) {
 // Iteration driver shared by the solvers: enqueue a chunk of iterations,
 // then poll the device flag.  Kernels turn into no-ops once the flag is set
 // (also when the start already decided the solve), so over-enqueueing only
-// costs launch slots; the first chunk is the previous solve's count
+// costs launch slots; the first chunk is the last converged solve's count
 // (warm-started solves barely change), so a solve that needs as many
-// iterations polls once.  Every poll drains the stream and leaves the device
-// idle until the next launch.  PER is the number of iterations one body
-// advances; MAXB bounds the bodies.
+// iterations polls once.  A solve that ran out of iterations does not seed
+// the next one's chunk (it would enqueue the whole budget).  Every poll
+// drains the stream and leaves the device idle until the next launch.  PER
+// is the number of iterations one body advances; MAXB bounds the bodies.
 #define PCFD_ITERATE(PER, MAXB, ...)                                          \
   do {                                                                        \
-    int _it = 0, _done = 0, _bodies = 0, _rc = 0;                             \
+    int _it = 0, _done = 0, _conv = 0, _bodies = 0, _rc = 0;                  \
     int _chunk = std::max(1, (predicted_ > 0 ? predicted_ : 4) / (PER));      \
     while (!_done && _bodies < (MAXB)) {                                      \
       for (int _c = 0; _c < _chunk && _bodies < (MAXB); ++_c, ++_bodies) {    \
         __VA_ARGS__;                                                          \
       }                                                                       \
-      _rc = poll(&_it, &_done);                                               \
+      _rc = poll(&_it, &_done, &_conv);                                       \
       if (_rc) return _rc;                                                    \
       _chunk = std::max(1, std::min(64, _it / (8 * (PER))));                  \
     }                                                                         \
-    predicted_ = _it;                                                         \
+    last_run_ = _it;                                                          \
+    if (_conv) predicted_ = _it;                                              \
   } while (0)
 
 int PoissonEngine::start(int mode, double* r) {
@@ -2699,7 +2702,7 @@ int PoissonEngine::run_mgcg() {
       return rc;
     std::swap(p_old, p_new);
   });
-  clock_trim(it0, predicted_);
+  clock_trim(it0, last_run_);
   return kOk;
 }
 
@@ -2720,7 +2723,7 @@ int PoissonEngine::run_multigrid() {
                       EpiCheck{st_, (int)CM_MG, cfg_.tol, cfg_.maxiter, singular_, n_fluid_}))
       return rc;
   });
-  clock_trim(it0, predicted_);
+  clock_trim(it0, last_run_);
   // The pending mean shift is applied by the final residual check and the scatter.
   PCFD_HIP_CHECK(hipMemcpyAsync(x_, L0.x, bytes, hipMemcpyDeviceToDevice, s_));
   return kOk;

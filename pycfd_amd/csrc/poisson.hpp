@@ -287,7 +287,7 @@ class PoissonEngine {
   int launch_chain(std::vector<LevelBuf>& lv, int first, const SolveState* st);
   template <typename T>
   int tune_chain(std::vector<LevelBuf>& lv, SolveState* tst);
-  int poll(int* iter, int* done);
+  int poll(int* iter, int* done, int* converged);
   int start(int mode, double* r);
   void free_levels();
 
@@ -320,7 +320,8 @@ class PoissonEngine {
   double *b_ = nullptr, *x_ = nullptr, *r_ = nullptr, *pa_ = nullptr, *pb_ = nullptr,
          *q_ = nullptr, *z_ = nullptr;
   int last_iters_ = 0;
-  int predicted_ = 0;
+  int predicted_ = 0;  // iterations of the last converged solve (first chunk)
+  int last_run_ = 0;   // iterations of the last solve
 
   // multigrid
   std::vector<LevelBuf> lv64_;

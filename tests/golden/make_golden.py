@@ -418,6 +418,25 @@ def cavity_c1_fixture():
     np.savez_compressed(OUT / "cavity_c1.npz", **out)
 
 
+def checkpoint_fixture():
+    """A checkpoint written by the reference's save_checkpoint
+    (export.py:214-235): a 24 x 20 cavity after 3 steps, once with its own
+    provenance and once with a caller-supplied record."""
+    import shutil
+    import tempfile
+    from pycfd.analysis.export import save_checkpoint
+    cfg = config("cavity", pressure_solver="direct", use_numba=False)
+    sim = Simulation(cfg)
+    sim.run(max_steps=3)
+    tmp = Path(tempfile.mkdtemp())
+    save_checkpoint(sim.fields, cfg, tmp / "ref_ckpt.npz")
+    save_checkpoint(sim.fields, cfg, tmp / "ref_ckpt_custom.npz",
+                    provenance={"study": "supplied by the caller"})
+    for name in ("ref_ckpt.npz", "ref_ckpt_custom.npz"):
+        shutil.copy(tmp / name, OUT / name)
+    shutil.rmtree(tmp)
+
+
 def mask_fixture():
     from pycfd.core.mesh import StructuredMesh
     out = {}
@@ -431,10 +450,11 @@ def mask_fixture():
 
 if __name__ == "__main__":
     which = sys.argv[1:] or ["rhs", "boundary", "trajectory", "multigrid", "mask", "les",
-                             "stretched", "les_stretched", "schemes", "cavity_c1"]
+                             "stretched", "les_stretched", "schemes", "cavity_c1", "checkpoint"]
     for w in which:
         print("writing", w, flush=True)
         {"rhs": rhs_fixture, "boundary": boundary_fixture, "trajectory": trajectory_fixture,
          "multigrid": multigrid_fixture, "mask": mask_fixture, "les": les_fixture,
          "stretched": stretched_fixture, "les_stretched": les_stretched_fixture,
-         "schemes": schemes_fixture, "cavity_c1": cavity_c1_fixture}[w]()
+         "schemes": schemes_fixture, "cavity_c1": cavity_c1_fixture,
+         "checkpoint": checkpoint_fixture}[w]()
