@@ -236,6 +236,20 @@ int pcfd_pressure_guess(pcfd_ctx* ctx, double* buf, int32_t store);
  * out_host holds 9 doubles.  u_prev/v_prev/p may be NULL. */
 int pcfd_diagnostics(pcfd_ctx* ctx, const double* u, const double* v, const double* p,
                      const double* u_prev, const double* v_prev, double* out_host);
+/* No counterpart in the reference.  The reductions of pcfd_diagnostics for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): the same nine values
+ * (out_dev[0..6] as out_host above, out_dev[7..8] the total force), written
+ * to the caller's DEVICE buffer out_dev (>= 9 doubles) on the context's
+ * stream, without synchronising.  The reduction's partial sums live in a
+ * fixed device array of PCFD_NOALLOC_SCRATCH_BYTES per GPU (held until the
+ * process ends), so this call allocates nothing; grids whose partials exceed
+ * it (about 19 M cells) are refused with PCFD_ERR_BAD_ARG.  All calls on one
+ * GPU share that array: calls that overlap in time on one GPU (other
+ * streams, other threads) race. */
+#define PCFD_NOALLOC_SCRATCH_BYTES (1u << 20)
+int pcfd_diagnostics_noalloc(pcfd_ctx* ctx, const double* u, const double* v, const double* p,
+                             const double* u_prev, const double* v_prev, double* out_dev);
 /* body_force_reaction then body_force_reactions (solver.py:173-178):
  * out_host has 2 + 2*n_bodies doubles. */
 int pcfd_get_force(pcfd_ctx* ctx, double* out_host);

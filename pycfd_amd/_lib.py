@@ -115,6 +115,8 @@ _SIGNATURES = {
     "pcfd_diagnostics": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                          C.c_void_p, _dp],
     "pcfd_get_force": [C.c_void_p, _dp],
+    "pcfd_diagnostics_noalloc": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                 C.c_void_p, C.c_void_p],
     "pcfd_poisson_apply": [C.c_void_p, C.c_void_p, C.c_void_p],
     "pcfd_mg_apply": [C.c_void_p, C.c_void_p, C.c_void_p],
     "pcfd_time_kernel": [C.c_void_p, C.c_int32, C.c_int32, _dp, _dp],

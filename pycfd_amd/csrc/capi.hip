@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -27,6 +28,47 @@ void set_last_error(const char* fmt, ...) {
 }  // namespace pcfd
 
 using namespace pcfd;
+
+// Scratch of the allocation-free diagnostics (pcfd_diagnostics_noalloc): the
+// per-workgroup partial sums of the reduction live in this fixed array,
+// which every GPU the library runs on holds until the process ends.  Host
+// code never uses the array's name as an address (on the host it is only a
+// stand-in for the symbol): each device's own copy is resolved once with
+// hipGetSymbolAddress and that pointer is what kernels receive.
+constexpr size_t kNoAllocScratchBytes = PCFD_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's copy of `scratch`, resolved on the first call for
+// that device.
+int scratch_for_current_device(double** out) {
+  int dev = 0;
+  PCFD_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_last_error("device %d is beyond the %d devices the scratch table holds", dev,
+                   kMaxDevices);
+    return kErrUnsupported;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      set_last_error("hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+      return kErrHip;
+    }
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return kOk;
+}
+
+}  // namespace
 
 struct pcfd_ctx {
   int nx = 0, ny = 0;
@@ -64,6 +106,7 @@ struct pcfd_ctx {
   double* diag_out = nullptr;
   double* guess_backup = nullptr;
   double* host_out = nullptr;  // pinned
+  double* scratch = nullptr;   // this device's copy of the no-allocation scratch
   PoissonEngine engine;
   bool solver_set = false;
 
@@ -543,6 +586,7 @@ int pcfd_ctx_create(const pcfd_case_desc* d, void* hip_stream, pcfd_ctx** out) {
   PCFD_HIP_CHECK(hipMalloc(&c->diag_out, 16 * sizeof(double)));
   PCFD_HIP_CHECK(hipMalloc(&c->guess_backup, (size_t)nx * ny * sizeof(double)));
   PCFD_HIP_CHECK(hipHostMalloc(&c->host_out, 64 * sizeof(double), hipHostMallocDefault));
+  if ((rc = scratch_for_current_device(&c->scratch))) return rc;
   PCFD_HIP_CHECK(hipStreamSynchronize(c->stream));
   *out = c.release();
   return kOk;
@@ -869,6 +913,31 @@ int pcfd_diagnostics(pcfd_ctx* c, const double* u, const double* v, const double
                                 hipMemcpyDeviceToHost, c->stream));
   PCFD_HIP_CHECK(hipStreamSynchronize(c->stream));
   if (out_host) std::memcpy(out_host, c->host_out, 9 * sizeof(double));
+  return kOk;
+}
+
+int pcfd_diagnostics_noalloc(pcfd_ctx* c, const double* u, const double* v, const double* p,
+                             const double* u_prev, const double* v_prev, double* out_dev) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!u || !v || !out_dev) {
+    set_last_error("pcfd_diagnostics_noalloc: u, v and out_dev must be device pointers");
+    return kErrBadArg;
+  }
+  const size_t need = diagnostics_partials(c->nx, c->ny) * sizeof(double);
+  if (need > kNoAllocScratchBytes) {
+    set_last_error(
+        "pcfd_diagnostics_noalloc: a %dx%d grid needs %zu bytes of reduction scratch, more than "
+        "the %zu-byte maximum; use pcfd_diagnostics",
+        c->nx, c->ny, need, kNoAllocScratchBytes);
+    return kErrBadArg;
+  }
+  // Every partial the fold reads is written by this call's first kernel, so
+  // the scratch needs no clearing.
+  int rc = launch_diagnostics(c->stream, u, v, p, u_prev, v_prev, c->solid, c->nx, c->ny, c->ldu,
+                              c->ldv, c->ldp, c->dx, c->dy, c->g, c->scratch, nullptr, out_dev);
+  if (rc) return rc;
+  PCFD_HIP_CHECK(hipMemcpyAsync(out_dev + 7, c->force, 2 * sizeof(double),
+                                hipMemcpyDeviceToDevice, c->stream));
   return kOk;
 }
 

@@ -123,6 +123,8 @@ int launch_correct(hipStream_t s, const double* p, int ldp, int nx, int ny, int 
                    double* v);
 int launch_divergence(hipStream_t s, const double* u, const double* v, int nx, int ny, int ldu,
                       int ldv, double dx, double dy, const Metrics& g, double* out);
+// Doubles of partial sums launch_diagnostics writes for an nx x ny grid.
+size_t diagnostics_partials(int nx, int ny);
 int launch_diagnostics(hipStream_t s, const double* u, const double* v, const double* p,
                        const double* up, const double* vp, const uint8_t* solid, int nx, int ny,
                        int ldu, int ldv, int ldp, double dx, double dy, const Metrics& g,

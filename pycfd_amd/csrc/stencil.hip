@@ -606,6 +606,8 @@ int launch_divergence(hipStream_t s, const double* u, const double* v, int nx, i
   return kOk;
 }
 
+size_t diagnostics_partials(int nx, int ny) { return (size_t)diag_blocks(nx, ny) * 7; }
+
 int launch_diagnostics(hipStream_t s, const double* u, const double* v, const double* p,
                        const double* up, const double* vp, const uint8_t* solid, int nx, int ny,
                        int ldu, int ldv, int ldp, double dx, double dy, const Metrics& g,
