@@ -1,0 +1,134 @@
+"""The two fully specified benchmark cases that fit a test, on the GPU.
+
+* The CPU-runnable cavity: 128 x 128, Re 100, lid u = 1, CG to relative
+  residual 1e-10, CFL 0.5 under the case's dt ceiling 0.01, 2000 SSP-RK3 steps
+  (lid_driven_cavity.py:59-85, pressure.py:366-398), against the reference's
+  own run of exactly that, recorded in tests/golden/cavity_c1.npz, and the
+  same run with CG to 1e-13.
+* Taylor-Green on the periodic [0, 2 pi]^2 at 1024 x 1024, nu 0.01, fixed dt
+  1e-3, 500 steps, standalone multigrid V(2,2): against the exact decay
+  exp(-2 nu t) (validation.py:47-65, taylor_green.py:118-132), with the
+  second-order convergence from 512^2, and at 64^2 against the oracle's
+  multigrid V(2,2) run.
+"""
+
+from __future__ import annotations
+
+import time
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+
+def _cavity_c1(tol):
+    from pycfd_amd import cases
+    from pycfd_amd.physics.incompressible import Simulation
+    cfg, _, _, _ = cases.cavity(nx=128, re=100.0, pressure_solver="cg", cfl_max=0.5, dt=0.01,
+                                poisson_tol=tol)
+    return Simulation(cfg)
+
+
+@pytest.mark.parametrize("tag,tol", [("t13", 1e-13), ("t10", 1e-10)])
+def test_cavity_128_cg_2000_steps_matches_reference(gpu_available, tag, tol):
+    """Every dt of the 2000 steps equal to the reference's (rtol 1e-12), and
+    the final u, v within 1e-10 relative and p within 1e-8 of the reference's
+    run at the same CG tolerance.  At 1e-10 the two CG runs stop on their own
+    recursive residuals at different iterates (the dot products reduce in a
+    different order), and their divergences are each bounded by that
+    tolerance, not equal: max|div u| <= 1e-9 where the reference reaches
+    4.0e-10; at 1e-13 both are at round-off (<= 5e-12, reference 8.9e-13)."""
+    g = golden("cavity_c1")
+    sim = _cavity_c1(tol)
+    w0 = time.perf_counter()
+    res = sim.run(max_steps=2000)
+    wall = time.perf_counter() - w0
+    assert res.steps == 2000
+    np.testing.assert_allclose(res.history["dt"], g[f"{tag}_dt"], rtol=1e-12)
+    u, v, p = sim.fields.u, sim.fields.v, sim.fields.p
+    du = np.abs(u - g[f"{tag}_u"]).max()
+    dv = np.abs(v - g[f"{tag}_v"]).max()
+    dp = np.abs(p - g[f"{tag}_p"]).max()
+    div = max(res.history["max_div"])
+    print(f"cavity 128^2 CG tol {tol:g}: {wall:.2f} s for 2000 steps "
+          f"({128 * 128 * 2000 / wall / 1e6:.2f} MLUPS; reference {float(g[f'{tag}_wall']):.0f} s), "
+          f"|du| {du:.3e} |dv| {dv:.3e} |dp| {dp:.3e}, max|div| {div:.3e} "
+          f"(reference {g[f'{tag}_max_div'].max():.3e})")
+    scale = np.abs(g[f"{tag}_u"]).max()
+    assert du <= 1e-10 * scale and dv <= 1e-10 * scale
+    assert dp <= 1e-8 * max(np.abs(g[f"{tag}_p"]).max(), 1.0)
+    assert div <= (5e-12 if tol < 1e-12 else 1e-9)
+    np.testing.assert_allclose(res.history["kinetic_energy"], g[f"{tag}_kinetic_energy"],
+                               rtol=1e-9)
+
+
+def _taylor_green(n, steps):
+    from pycfd_amd import cases
+    from pycfd_amd.physics.incompressible import Simulation
+    cfg, _, u0, v0 = cases.taylor_green(n=n, nu=0.01, dt=1e-3, pressure_solver="multigrid",
+                                        mg_sweeps=2)
+    sim = Simulation(cfg, u_init=u0, v_init=v0)
+    w0 = time.perf_counter()
+    res = sim.run(max_steps=steps)
+    return sim, res, time.perf_counter() - w0
+
+
+def _tg_errors(sim, nu=0.01):
+    """taylor_green.py:118-132: the far-end faces of a periodic axis
+    duplicate the near-end ones and are dropped."""
+    from pycfd_amd.cases import taylor_green_exact
+    ue, ve = taylor_green_exact(sim.mesh, sim.fields.t, nu)
+    du = sim.fields.u_phys[:-1, :] - ue[:-1, :]
+    dv = sim.fields.v_phys[:, :-1] - ve[:, :-1]
+    l2 = float(np.sqrt((np.sum(du ** 2) + np.sum(dv ** 2)) / (du.size + dv.size)))
+    return l2, float(max(np.abs(du).max(), np.abs(dv).max()))
+
+
+def test_taylor_green_1024_multigrid_v22(gpu_available):
+    """500 steps of dt 1e-3 at 1024^2 and 512^2: velocity error against the
+    exact decay, second order between the two grids, kinetic energy decaying
+    as exp(-4 nu t) (test_solver.py:181-197), max|div u| at round-off."""
+    errs = {}
+    for n in (512, 1024):
+        sim, res, wall = _taylor_green(n, 500)
+        t = sim.fields.t
+        assert t == pytest.approx(0.5, rel=1e-12)
+        l2, linf = _tg_errors(sim)
+        ke0 = 0.5 * (2 * np.pi) ** 2 * 0.5
+        ke_ratio = res.history["kinetic_energy"][-1] / ke0 / np.exp(-4 * 0.01 * t)
+        div = max(res.history["max_div"])
+        its = sim.solver.pressure_solver.last_iterations
+        print(f"taylor-green {n}^2 multigrid V(2,2): {wall:.2f} s for 500 steps "
+              f"({n * n * 500 / wall / 1e6:.1f} MLUPS), L2 {l2:.3e} Linf {linf:.3e}, "
+              f"KE/exact {ke_ratio:.8f}, max|div| {div:.2e}, cycles/solve {its}")
+        errs[n] = (l2, linf)
+        assert abs(ke_ratio - 1.0) < 2e-3
+        assert div < 1e-10
+        assert linf < 1e-4
+    order = np.log2(errs[512][0] / errs[1024][0])
+    print(f"observed L2 order 512 -> 1024: {order:.3f}")
+    assert order > 1.8
+
+
+def test_taylor_green_multigrid_v22_matches_oracle(gpu_available):
+    """64^2, 20 steps, standalone multigrid V(2,2) on the GPU against the
+    oracle's run of the same (its multigrid pinned to the reference's solver
+    outputs): velocity within 1e-10 relative, pressure within 1e-8."""
+    from pycfd_amd.cases import taylor_green
+    n, steps = 64, 20
+    cfg, _, u0, v0 = taylor_green(n=n, nu=0.01, dt=1e-3, pressure_solver="multigrid",
+                                  mg_sweeps=2, poisson_tol=1e-12)
+    ui, vi = u0, v0
+    from pycfd_amd.physics.incompressible import Simulation
+    sim = Simulation(cfg, u_init=ui, v_init=vi)
+    sim.run(max_steps=steps)
+    case = O.Case(nx=n, ny=n, lx=2 * np.pi, ly=2 * np.pi, re=100.0, dt=1e-3, adaptive_dt=False,
+                  kinds=("periodic",) * 4, pressure_solver="multigrid", mg_sweeps=2, tol=1e-12)
+    u, v, p, hist, t = O.run(case, steps, ui, vi)
+    assert np.abs(sim.fields.u - u).max() <= 1e-10 * np.abs(u).max()
+    assert np.abs(sim.fields.v - v).max() <= 1e-10 * np.abs(u).max()
+    assert np.abs(sim.fields.p - p).max() <= 1e-8 * max(np.abs(p).max(), 1.0)
