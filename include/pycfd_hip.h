@@ -221,6 +221,21 @@ int pcfd_initialize(pcfd_ctx* ctx, double* u, double* v, double* p, pcfd_solve_i
 /* ProjectionSolver.step (solver.py:686-731): (u0, v0) -> (u, v, p). */
 int pcfd_step(pcfd_ctx* ctx, const double* u0, const double* v0, double dt, double blend,
               double* u, double* v, double* p, pcfd_solve_info* info);
+/* Whole steps as HIP graphs (no counterpart in the reference, whose loop is
+ * Python).  With enable = 1, pcfd_step captures each (u0, v0) -> (u, v, p)
+ * buffer set once (up to four sets are kept) and replays it: dt and the
+ * blend are written by the graph's first node, and each solve runs a fixed
+ * iteration budget (the last measured count plus a margin; iterations past
+ * convergence are no-op kernels), so a step is one launch and one
+ * synchronisation instead of a launch per kernel and a poll per solve.  The
+ * first step after enabling (or after pcfd_set_pressure_solver) runs
+ * eagerly to measure the counts; a replay whose solve ran out of budget is
+ * redone eagerly from the same warm start and recaptured, so results are
+ * those of the eager path.  Steps with u0 == u, and steps while the kernel
+ * clock runs, are never captured. */
+int pcfd_set_step_graphs(pcfd_ctx* ctx, int32_t enable);
+/* Graph replays, captures and eager redos since the context was created. */
+int pcfd_step_graph_stats(pcfd_ctx* ctx, int32_t* out3);
 /* Forget the warm-start pressure (ProjectionSolver._p_guess = None). */
 int pcfd_reset_guess(pcfd_ctx* ctx);
 /* The warm-start pressure (ProjectionSolver._p_guess, solver.py:171,662-663)

@@ -112,6 +112,8 @@ _SIGNATURES = {
     "pcfd_step": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_void_p,
                   C.c_void_p, C.c_void_p, C.POINTER(SolveInfo)],
     "pcfd_reset_guess": [C.c_void_p],
+    "pcfd_set_step_graphs": [C.c_void_p, C.c_int32],
+    "pcfd_step_graph_stats": [C.c_void_p, C.POINTER(C.c_int32)],
     "pcfd_diagnostics": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                          C.c_void_p, _dp],
     "pcfd_get_force": [C.c_void_p, _dp],

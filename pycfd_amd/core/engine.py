@@ -236,6 +236,16 @@ class DeviceContext:
     def reset_guess(self) -> None:
         self._call("pcfd_reset_guess")
 
+    def set_step_graphs(self, enable: bool) -> None:
+        """Capture whole steps as HIP graphs and replay them (pcfd_set_step_graphs)."""
+        self._call("pcfd_set_step_graphs", int(bool(enable)))
+
+    def step_graph_stats(self) -> dict:
+        """Graph replays, captures and eager redos of this context so far."""
+        out = (C.c_int32 * 3)()
+        self._call("pcfd_step_graph_stats", out)
+        return {"launches": int(out[0]), "captures": int(out[1]), "fallbacks": int(out[2])}
+
     def pressure_guess(self, buf, store: bool) -> None:
         """Copy the warm-start pressure out to (store=False) or in from ``buf``."""
         self._call("pcfd_pressure_guess", _ptr(buf), int(bool(store)))

@@ -11,6 +11,7 @@ the reference can call them unchanged.
 from __future__ import annotations
 
 import logging
+import os
 from collections.abc import Sequence
 
 import numpy as np
@@ -117,6 +118,8 @@ class ProjectionSolver:
             mg_sweeps=config.mg_sweeps, context=self.ctx, precision=config.precision,
             mg_coarse_max=config.mg_coarse_max)
         self.pressure_solver.configure()
+        self._use_graphs = False
+        self.use_graphs = os.environ.get("PCFD_STEP_GRAPHS", "0") == "1"
         self._has_guess = False
         self._force = np.zeros(2 + 2 * len(self.body_masks))
         self._force_stale = False
@@ -173,6 +176,21 @@ class ProjectionSolver:
                                  for b in self.body_masks)
         self.v_face_body = tuple(_face_solid_mask(b, v_lo, v_hi, 1, self.periodic_y)
                                  for b in self.body_masks)
+
+    # -- captured steps ------------------------------------------------- #
+    @property
+    def use_graphs(self) -> bool:
+        """Whether ``step`` replays captured HIP graphs (the same arithmetic
+        as launching each kernel; see pcfd_set_step_graphs).  Off unless the
+        environment sets ``PCFD_STEP_GRAPHS=1``: on MI355X a replay runs its
+        kernels no faster than the stream does (the per-kernel dispatch on
+        the device, not the host's launches, sets the pace)."""
+        return self._use_graphs
+
+    @use_graphs.setter
+    def use_graphs(self, enable: bool) -> None:
+        self.ctx.set_step_graphs(bool(enable))
+        self._use_graphs = bool(enable)
 
     # -- forces --------------------------------------------------------- #
     @property

@@ -509,7 +509,8 @@ k_walls(BCParams P, int predictor, double* u, double* v) {
 }
 
 __global__ void __launch_bounds__(kBlock)
-k_boundary_pass(BCParams P, BCOps ops, double dt, double* u, double* v) {
+k_boundary_pass(BCParams P, BCOps ops, StepScalars T, double* u, double* v) {
+  const double dt = T.get_dt();
   for (int q = 0; q < ops.n; ++q) {
     switch (ops.op[q]) {
       case BOP_VEL_PRED:
@@ -571,15 +572,15 @@ k_pressure_ghosts(BCParams P, double* p, int ldp) {
 // touch disjoint lines) the wall conditions run on the whole device
 // (k_walls); the rest of the pass (O(perimeter) lines and reductions) stays
 // on one workgroup, in the pass's order.
-int launch_boundary_pass(hipStream_t s, const BCParams& P, const BCOps& ops, double dt, double* u,
-                         double* v) {
+int launch_boundary_pass(hipStream_t s, const BCParams& P, const BCOps& ops,
+                         const StepScalars& T, double* u, double* v) {
   if (ops.n == 0) return kOk;
   const bool walls_apart = !P.periodic_x && !P.periodic_y && P.nx >= 2 * kWallCorner &&
                            P.ny >= 2 * kWallCorner;
   BCOps rest{};
   auto flush = [&]() {
     if (rest.n == 0) return;
-    hipLaunchKernelGGL(k_boundary_pass, dim3(1), dim3(kBlock), 0, s, P, rest, dt, u, v);
+    hipLaunchKernelGGL(k_boundary_pass, dim3(1), dim3(kBlock), 0, s, P, rest, T, u, v);
     rest.n = 0;
   };
   for (int q = 0; q < ops.n; ++q) {
@@ -594,6 +595,19 @@ int launch_boundary_pass(hipStream_t s, const BCParams& P, const BCOps& ops, dou
     }
   }
   flush();
+  PCFD_LAUNCH_CHECK();
+  return kOk;
+}
+
+__global__ void k_set_step_scalars(double* dst, double dt, double blend) {
+  if (threadIdx.x == 0) {
+    dst[0] = dt;
+    dst[1] = blend;
+  }
+}
+
+int launch_set_step_scalars(hipStream_t s, double* dst, double dt, double blend) {
+  hipLaunchKernelGGL(k_set_step_scalars, dim3(1), dim3(64), 0, s, dst, dt, blend);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }

@@ -109,6 +109,33 @@ struct pcfd_ctx {
   double* scratch = nullptr;   // this device's copy of the no-allocation scratch
   PoissonEngine engine;
   bool solver_set = false;
+  // Captured steps (pcfd_set_step_graphs): one graph per buffer set, the
+  // iteration budget of each substep's solve from the last measured counts.
+  struct StepGraph {
+    const double *u0 = nullptr, *v0 = nullptr;
+    double *u = nullptr, *v = nullptr, *p = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraphNode_t scalars = nullptr;  // k_set_step_scalars, updated per replay
+    hipKernelNodeParams kp{};
+    int budget[3] = {0, 0, 0};
+    int partial_gen = 0;
+    uint64_t used = 0;
+    bool stale = false;
+    void release() {
+      if (exec) hipGraphExecDestroy(exec);
+      if (graph) hipGraphDestroy(graph);
+      exec = nullptr;
+      graph = nullptr;
+    }
+  };
+  std::vector<StepGraph> graphs;
+  uint64_t graph_tick = 0;
+  bool use_graphs = false;
+  int pred_sub[3] = {0, 0, 0};    // iterations each substep's solve took last
+  hipStream_t cap_stream = nullptr;
+  double* scal = nullptr;         // device [dt, blend] read by captured kernels
+  int graph_launches = 0, graph_captures = 0, graph_fallbacks = 0;
 
   ~pcfd_ctx() {
     for (auto* p : inlet) hipFree(p);
@@ -126,6 +153,9 @@ struct pcfd_ctx {
     hipFree(nuc);
     hipFree(metric);
     if (host_out) hipHostFree(host_out);
+    for (StepGraph& g : graphs) g.release();
+    hipFree(scal);
+    if (cap_stream) hipStreamDestroy(cap_stream);
   }
 };
 
@@ -137,8 +167,13 @@ BCOps ops_of(std::initializer_list<int> l) {
   return o;
 }
 
+int boundary(pcfd_ctx* c, std::initializer_list<int> l, const StepScalars& S, double* u,
+             double* v) {
+  return launch_boundary_pass(c->stream, c->bc, ops_of(l), S, u, v);
+}
+
 int boundary(pcfd_ctx* c, std::initializer_list<int> l, double dt, double* u, double* v) {
-  return launch_boundary_pass(c->stream, c->bc, ops_of(l), dt, u, v);
+  return boundary(c, l, StepScalars{dt, 0.0, nullptr}, u, v);
 }
 
 template <typename T>
@@ -175,32 +210,250 @@ void fill_info(pcfd_solve_info* info, const SolveInfo& s) {
 // One projected Euler substep (solver.py:612-640).  `ue` non-null selects
 // the fused stage combination ca*u + cb*ue as the substep input.
 int substep(pcfd_ctx* c, const double* u, const double* v, const double* ue, const double* ve,
-            double ca, double cb, double dt, double* us, double* vs, double* p, SolveInfo* info,
-            bool sync) {
+            double ca, double cb, const StepScalars& S, double* us, double* vs, double* p,
+            SolveInfo* info, bool sync) {
   int rc;
   if (c->use_les &&
       (rc = launch_eddy_viscosity(c->stream, c->les, u, ue, v, ve, ca, cb, c->nuc, nullptr)))
     return rc;
   if (ue) {
-    rc = launch_predictor_comb(c->stream, u, ue, v, ve, ca, cb, c->mom, c->upd, dt, us, vs);
+    rc = launch_predictor_comb(c->stream, u, ue, v, ve, ca, cb, c->mom, c->upd, S, us, vs);
   } else {
-    rc = launch_predictor(c->stream, u, v, c->mom, c->upd, dt, us, vs);
+    rc = launch_predictor(c->stream, u, v, c->mom, c->upd, S, us, vs);
   }
   if (rc) return rc;
   if (c->any_outlet) {
-    rc = boundary(c, {BOP_VEL_PRED, BOP_MASS, BOP_MASK_RESET, BOP_WRAP_FACES}, dt, us, vs);
+    rc = boundary(c, {BOP_VEL_PRED, BOP_MASS, BOP_MASK_RESET, BOP_WRAP_FACES}, S, us, vs);
   } else {
-    rc = boundary(c, {BOP_VEL_PRED, BOP_MASK_RESET, BOP_WRAP_FACES}, dt, us, vs);
+    rc = boundary(c, {BOP_VEL_PRED, BOP_MASK_RESET, BOP_WRAP_FACES}, S, us, vs);
   }
   if (rc) return rc;
-  if ((rc = c->engine.prepare_from_velocity(us, vs, c->ldu, c->ldv, c->dx, c->dy, dt))) return rc;
+  if ((rc = c->engine.prepare_from_velocity(us, vs, c->ldu, c->ldv, c->dx, c->dy, S))) return rc;
   if ((rc = c->engine.solve(info, sync))) return rc;
   if ((rc = c->engine.scatter_pressure(p, c->ldp))) return rc;
   if ((rc = launch_pressure_ghosts(c->stream, c->bc, p, c->ldp))) return rc;
   if ((rc = launch_correct(c->stream, p, c->ldp, c->nx, c->ny, c->ldu, c->ldv, c->dx, c->dy,
-                           c->g, c->upd, dt, us, vs)))
+                           c->g, c->upd, S, us, vs)))
     return rc;
-  return boundary(c, {BOP_MASK_ACC, BOP_WRAP_FACES, BOP_VEL_CORR}, dt, us, vs);
+  return boundary(c, {BOP_MASK_ACC, BOP_WRAP_FACES, BOP_VEL_CORR}, S, us, vs);
+}
+
+int substeps_of(int scheme) {
+  return scheme == PCFD_SCHEME_EULER ? 1 : scheme == PCFD_SCHEME_RK2 ? 2 : 3;
+}
+
+// ProjectionSolver.step (solver.py:686-731) enqueued on c->stream.  With
+// `log`, substep s's solve writes its record to the engine's log slot s.
+// `want` fills *s from the last solve (a sync); a captured step passes false
+// and the engine's budget is set per substep from `budget` (null: polling).
+int step_body(pcfd_ctx* c, const double* u0, const double* v0, const StepScalars& S, double* u,
+              double* v, double* p, SolveInfo* s, bool want, bool log, const int* budget) {
+  int rc;
+  double* A[2] = {c->stage_a[0], c->stage_a[1]};
+  double* B[2] = {c->stage_b[0], c->stage_b[1]};
+  PoissonEngine& E = c->engine;
+  auto sub = [&](int k, const double* a, const double* b, const double* ae, const double* be,
+                 double ca, double cb, double* ua, double* va, bool last) -> int {
+    E.set_log_slot(log ? k : -1);
+    E.set_budget(budget ? budget[k] : 0);
+    const int r = substep(c, a, b, ae, be, ca, cb, S, ua, va, p, s, last && want);
+    E.set_log_slot(-1);
+    E.set_budget(0);
+    return r;
+  };
+  switch (c->scheme) {
+    case PCFD_SCHEME_EULER:
+      if ((rc = sub(0, u0, v0, nullptr, nullptr, 0, 0, u, v, true))) return rc;
+      break;
+    case PCFD_SCHEME_RK2:  // solver.py:702-707
+      if ((rc = sub(0, u0, v0, nullptr, nullptr, 0, 0, A[0], A[1], false))) return rc;
+      if ((rc = sub(1, A[0], A[1], nullptr, nullptr, 0, 0, B[0], B[1], true))) return rc;
+      if ((rc = launch_combine(c->stream, u0, B[0], v0, B[1], 0.5, 0.5, c->nx, c->ny, c->ldu,
+                               c->ldv, u, v)))
+        return rc;
+      break;
+    default:  // SSP-RK3, solver.py:709-721
+      if ((rc = sub(0, u0, v0, nullptr, nullptr, 0, 0, A[0], A[1], false))) return rc;
+      if ((rc = sub(1, A[0], A[1], nullptr, nullptr, 0, 0, B[0], B[1], false))) return rc;
+      if ((rc = sub(2, u0, v0, B[0], B[1], 0.75, 0.25, A[0], A[1], true))) return rc;
+      if ((rc = launch_combine(c->stream, u0, A[0], v0, A[1], 1.0 / 3.0, 2.0 / 3.0, c->nx, c->ny,
+                               c->ldu, c->ldv, u, v)))
+        return rc;
+      break;
+  }
+  if (c->scheme != PCFD_SCHEME_EULER) {
+    if ((rc = boundary(c, {BOP_VEL_CORR, BOP_WRAP_FACES}, S, u, v))) return rc;
+  }
+  return kOk;
+}
+
+// Budget of a captured solve: the last count plus a margin (the warm-started
+// counts drift by an iteration or two from step to step), within maxiter.
+int solve_budget(const pcfd_ctx* c, int last) {
+  const int b = last + std::max(2, last / 8);
+  return std::min(b, std::max(1, c->engine.maxiter()));
+}
+
+void set_streams(pcfd_ctx* c, hipStream_t s) {
+  c->stream = s;
+  c->engine.set_stream(s);
+}
+
+// Capture step_body into g (c->stream is the replay stream; the capture runs
+// on a private non-blocking stream).
+int capture_step(pcfd_ctx* c, pcfd_ctx::StepGraph& g) {
+  hipStream_t replay = c->stream;
+  set_streams(c, c->cap_stream);
+  hipGraph_t graph = nullptr;
+  int rc = kOk;
+  hipError_t e = hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeRelaxed);
+  if (e != hipSuccess) {
+    set_streams(c, replay);
+    set_last_error("hipStreamBeginCapture: %s", hipGetErrorString(e));
+    return kErrHip;
+  }
+  const size_t bytes = (size_t)c->nx * c->ny * sizeof(double);
+  rc = launch_set_step_scalars(c->cap_stream, c->scal, 0.0, 0.0);
+  if (!rc) {
+    // the warm-start pressure the step starts from, kept for a replay that
+    // runs out of iterations (its step is then redone from the same start)
+    e = hipMemcpyAsync(c->guess_backup, c->engine.guess(), bytes, hipMemcpyDeviceToDevice,
+                       c->cap_stream);
+    if (e != hipSuccess) {
+      set_last_error("hipMemcpyAsync in capture: %s", hipGetErrorString(e));
+      rc = kErrHip;
+    }
+  }
+  if (!rc)
+    rc = step_body(c, g.u0, g.v0, StepScalars{0.0, 0.0, c->scal}, g.u, g.v, g.p, nullptr, false,
+                   true, g.budget);
+  e = hipStreamEndCapture(c->cap_stream, &graph);
+  set_streams(c, replay);
+  if (rc) {
+    if (graph) hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess || !graph) {
+    set_last_error("hipStreamEndCapture: %s", hipGetErrorString(e));
+    return kErrHip;
+  }
+  size_t nroot = 0;
+  hipGraphNode_t root = nullptr;
+  e = hipGraphGetRootNodes(graph, nullptr, &nroot);
+  if (e == hipSuccess && nroot == 1) e = hipGraphGetRootNodes(graph, &root, &nroot);
+  hipGraphNodeType type = hipGraphNodeTypeEmpty;
+  if (e == hipSuccess && root) e = hipGraphNodeGetType(root, &type);
+  if (e != hipSuccess || nroot != 1 || type != hipGraphNodeTypeKernel) {
+    hipGraphDestroy(graph);
+    set_last_error("captured step: expected one kernel root node (got %zu)", nroot);
+    return kErrHip;
+  }
+  hipKernelNodeParams kp{};
+  if ((e = hipGraphKernelNodeGetParams(root, &kp)) != hipSuccess) {
+    hipGraphDestroy(graph);
+    set_last_error("hipGraphKernelNodeGetParams: %s", hipGetErrorString(e));
+    return kErrHip;
+  }
+  hipGraphExec_t exec = nullptr;
+  if ((e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0)) != hipSuccess) {
+    hipGraphDestroy(graph);
+    set_last_error("hipGraphInstantiate: %s", hipGetErrorString(e));
+    return kErrHip;
+  }
+  g.release();
+  g.graph = graph;
+  g.exec = exec;
+  g.scalars = root;
+  g.kp = kp;
+  g.partial_gen = c->engine.partial_gen_;
+  g.stale = false;
+  ++c->graph_captures;
+  return kOk;
+}
+
+void info_from_log(const pcfd_ctx* c, const SolveLog& L, SolveInfo* s) {
+  s->iterations = L.iter;
+  s->residual = c->engine.reports_final_residual() ? L.res_final : L.res;
+  s->converged = L.converged;
+  s->status = kOk;
+}
+
+// The step run eagerly (polling each solve) with every solve logged, so the
+// next capture knows each substep's iteration count.
+int step_eager_logged(pcfd_ctx* c, const double* u0, const double* v0, double dt, double blend,
+                      double* u, double* v, double* p, SolveInfo* s) {
+  int rc = step_body(c, u0, v0, StepScalars{dt, blend, nullptr}, u, v, p, s, true, true, nullptr);
+  if (rc) return rc;
+  SolveLog L[3];
+  const int ns = substeps_of(c->scheme);
+  if ((rc = c->engine.read_log(L, ns))) return rc;
+  for (int k = 0; k < ns; ++k) c->pred_sub[k] = L[k].iter;
+  return kOk;
+}
+
+// ProjectionSolver.step through a captured graph: replayed with this step's
+// dt and blend; a solve that ran out of its budget redoes the step eagerly.
+int step_graph(pcfd_ctx* c, const double* u0, const double* v0, double dt, double blend,
+               double* u, double* v, double* p, SolveInfo* s) {
+  const int ns = substeps_of(c->scheme);
+  bool known = true;
+  for (int k = 0; k < ns; ++k) known &= c->pred_sub[k] > 0;
+  if (!known) return step_eager_logged(c, u0, v0, dt, blend, u, v, p, s);
+  pcfd_ctx::StepGraph* g = nullptr;
+  for (auto& x : c->graphs)
+    if (x.u0 == u0 && x.v0 == v0 && x.u == u && x.v == v && x.p == p) g = &x;
+  if (g && (g->stale || g->partial_gen != c->engine.partial_gen_)) g->release();
+  if (!g) {
+    constexpr size_t kMaxGraphs = 4;
+    if (c->graphs.size() < kMaxGraphs) {
+      c->graphs.emplace_back();
+      g = &c->graphs.back();
+    } else {  // replace the least recently used
+      g = &c->graphs[0];
+      for (auto& x : c->graphs)
+        if (x.used < g->used) g = &x;
+      g->release();
+    }
+    g->u0 = u0;
+    g->v0 = v0;
+    g->u = u;
+    g->v = v;
+    g->p = p;
+  }
+  if (!g->exec) {
+    for (int k = 0; k < ns; ++k) g->budget[k] = solve_budget(c, c->pred_sub[k]);
+    if (int rc = capture_step(c, *g)) return rc;
+  }
+  g->used = ++c->graph_tick;
+  void* args[3] = {(void*)&c->scal, (void*)&dt, (void*)&blend};
+  hipKernelNodeParams kp = g->kp;
+  kp.kernelParams = args;
+  kp.extra = nullptr;
+  PCFD_HIP_CHECK(hipGraphExecKernelNodeSetParams(g->exec, g->scalars, &kp));
+  PCFD_HIP_CHECK(hipGraphLaunch(g->exec, c->stream));
+  ++c->graph_launches;
+  SolveLog L[3];
+  if (int rc = c->engine.read_log(L, ns)) return rc;
+  bool short_budget = false;
+  for (int k = 0; k < ns; ++k) short_budget |= !L[k].done;
+  if (short_budget) {
+    // Redo the step eagerly from the same warm start; the next replay is
+    // captured again with the new counts.
+    ++c->graph_fallbacks;
+    g->stale = true;
+    PCFD_HIP_CHECK(hipMemcpyAsync(c->engine.guess(), c->guess_backup,
+                                  (size_t)c->nx * c->ny * sizeof(double),
+                                  hipMemcpyDeviceToDevice, c->stream));
+    return step_eager_logged(c, u0, v0, dt, blend, u, v, p, s);
+  }
+  for (int k = 0; k < ns; ++k) {
+    c->pred_sub[k] = L[k].iter;
+    // recapture with a tighter budget once the counts have dropped well
+    // below it (the extra iterations are no-op launches)
+    if (g->budget[k] > solve_budget(c, L[k].iter) + std::max(2, L[k].iter / 4)) g->stale = true;
+  }
+  info_from_log(c, L[ns - 1], s);
+  return kOk;
 }
 
 // Per-cell flag byte of the fine Poisson operator (pressure.py:209-251):
@@ -241,12 +494,13 @@ int poisson_flags(const pcfd_case_desc& d, const std::vector<uint8_t>& solid, ui
 
 int project_impl(pcfd_ctx* c, double* u, double* v, double dt, double* p, SolveInfo* info) {
   int rc;
-  if ((rc = c->engine.prepare_from_velocity(u, v, c->ldu, c->ldv, c->dx, c->dy, dt))) return rc;
+  const StepScalars S{dt, 0.0, nullptr};
+  if ((rc = c->engine.prepare_from_velocity(u, v, c->ldu, c->ldv, c->dx, c->dy, S))) return rc;
   if ((rc = c->engine.solve(info, true))) return rc;
   if ((rc = c->engine.scatter_pressure(p, c->ldp))) return rc;
   if ((rc = launch_pressure_ghosts(c->stream, c->bc, p, c->ldp))) return rc;
   return launch_correct(c->stream, p, c->ldp, c->nx, c->ny, c->ldu, c->ldv, c->dx, c->dy,
-                        c->g, c->upd, dt, u, v);
+                        c->g, c->upd, S, u, v);
 }
 
 }  // namespace
@@ -698,6 +952,10 @@ int pcfd_set_pressure_solver(pcfd_ctx* c, const pcfd_solver_desc* sd,
     }
     lv.push_back(std::move(h));
   }
+  // captured steps hold the old engine's buffers and iteration counts
+  for (auto& g : c->graphs) g.release();
+  c->graphs.clear();
+  for (int& k : c->pred_sub) k = 0;
   int rc = c->engine.configure(cfg, lv);
   if (rc) return rc;
   c->solver_set = true;
@@ -840,40 +1098,41 @@ int pcfd_step(pcfd_ctx* c, const double* u0, const double* v0, double dt, double
   if (int rc = check_solver(c)) return rc;
   c->mom.blend = blend;
   SolveInfo s{};
-  int rc;
-  double* A[2] = {c->stage_a[0], c->stage_a[1]};
-  double* B[2] = {c->stage_b[0], c->stage_b[1]};
   const bool want = info != nullptr;
-  switch (c->scheme) {
-    case PCFD_SCHEME_EULER:
-      rc = substep(c, u0, v0, nullptr, nullptr, 0, 0, dt, u, v, p, &s, want);
-      if (rc) return rc;
-      break;
-    case PCFD_SCHEME_RK2:  // solver.py:702-707
-      if ((rc = substep(c, u0, v0, nullptr, nullptr, 0, 0, dt, A[0], A[1], p, &s, false)))
-        return rc;
-      if ((rc = substep(c, A[0], A[1], nullptr, nullptr, 0, 0, dt, B[0], B[1], p, &s, want)))
-        return rc;
-      if ((rc = launch_combine(c->stream, u0, B[0], v0, B[1], 0.5, 0.5, c->nx, c->ny, c->ldu,
-                               c->ldv, u, v)))
-        return rc;
-      break;
-    default:  // SSP-RK3, solver.py:709-721
-      if ((rc = substep(c, u0, v0, nullptr, nullptr, 0, 0, dt, A[0], A[1], p, &s, false)))
-        return rc;
-      if ((rc = substep(c, A[0], A[1], nullptr, nullptr, 0, 0, dt, B[0], B[1], p, &s, false)))
-        return rc;
-      if ((rc = substep(c, u0, v0, B[0], B[1], 0.75, 0.25, dt, A[0], A[1], p, &s, want)))
-        return rc;
-      if ((rc = launch_combine(c->stream, u0, A[0], v0, A[1], 1.0 / 3.0, 2.0 / 3.0, c->nx, c->ny,
-                               c->ldu, c->ldv, u, v)))
-        return rc;
-      break;
-  }
-  if (c->scheme != PCFD_SCHEME_EULER) {
-    if ((rc = boundary(c, {BOP_VEL_CORR, BOP_WRAP_FACES}, dt, u, v))) return rc;
-  }
+  // A captured step needs distinct input and output buffers (the replay
+  // reads u0 while it writes u) and no live kernel clock.
+  const bool graph = c->use_graphs && !c->engine.clock_on() && u0 != u && v0 != v;
+  int rc = graph ? step_graph(c, u0, v0, dt, blend, u, v, p, &s)
+                 : step_body(c, u0, v0, StepScalars{dt, blend, nullptr}, u, v, p, &s, want,
+                             false, nullptr);
+  if (rc) return rc;
   if (want) fill_info(info, s);
+  return kOk;
+}
+
+int pcfd_set_step_graphs(pcfd_ctx* c, int32_t enable) {
+  if (int rc = check_ctx(c)) return rc;
+  if (enable && !c->cap_stream) {
+    PCFD_HIP_CHECK(hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking));
+    PCFD_HIP_CHECK(hipMalloc(&c->scal, 4 * sizeof(double)));
+  }
+  if (!enable) {
+    for (auto& g : c->graphs) g.release();
+    c->graphs.clear();
+  }
+  c->use_graphs = enable != 0;
+  return kOk;
+}
+
+int pcfd_step_graph_stats(pcfd_ctx* c, int32_t* out3) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!out3) {
+    set_last_error("pcfd_step_graph_stats: null output");
+    return kErrBadArg;
+  }
+  out3[0] = c->graph_launches;
+  out3[1] = c->graph_captures;
+  out3[2] = c->graph_fallbacks;
   return kOk;
 }
 

@@ -93,6 +93,16 @@ struct LesParams {
   Metrics g;
 };
 
+// The time step and upwind blend of a substep: by value, or, when `dev` is
+// set, read from device memory when the kernel runs (a captured step graph
+// is replayed with new values written there by its first node).
+struct StepScalars {
+  double dt = 0.0, blend = 0.0;
+  const double* dev = nullptr;  // [dt, blend] or null
+  __device__ __forceinline__ double get_dt() const { return dev ? dev[0] : dt; }
+  __device__ __forceinline__ double get_blend() const { return dev ? dev[1] : blend; }
+};
+
 // Solvable face ranges u_upd / v_upd (solver.py:235-263): u faces i in
 // [u_lo, u_hi) x rows 1..ny; v faces columns 1..nx x j in [v_lo, v_hi).
 struct UpdRange {
@@ -108,19 +118,20 @@ int launch_eddy_viscosity(hipStream_t s, const LesParams& P, const double* u0, c
                           double* nu_corner);
 int launch_momentum_rhs(hipStream_t s, const double* u, const double* v, const MomParams& P,
                         double* rhs_u, double* rhs_v);
+// The predictors take dt and the blend from `S` (P.blend is ignored).
 int launch_predictor(hipStream_t s, const double* u, const double* v, const MomParams& P,
-                     const UpdRange& R, double dt, double* us, double* vs);
+                     const UpdRange& R, const StepScalars& S, double* us, double* vs);
 int launch_predictor_comb(hipStream_t s, const double* u0, const double* ue, const double* v0,
                           const double* ve, double ca, double cb, const MomParams& P,
-                          const UpdRange& R, double dt, double* us, double* vs);
+                          const UpdRange& R, const StepScalars& S, double* us, double* vs);
 int launch_combine(hipStream_t s, const double* u0, const double* ue, const double* v0,
                    const double* ve, double ca, double cb, int nx, int ny, int ldu, int ldv,
                    double* uo, double* vo);
 // The launchers below take the mesh's Metrics: null arrays select the
 // uniform arithmetic on dx, dy.
 int launch_correct(hipStream_t s, const double* p, int ldp, int nx, int ny, int ldu, int ldv,
-                   double dx, double dy, const Metrics& g, const UpdRange& R, double dt, double* u,
-                   double* v);
+                   double dx, double dy, const Metrics& g, const UpdRange& R,
+                   const StepScalars& S, double* u, double* v);
 int launch_divergence(hipStream_t s, const double* u, const double* v, int nx, int ny, int ldu,
                       int ldv, double dx, double dy, const Metrics& g, double* out);
 // Doubles of partial sums launch_diagnostics writes for an nx x ny grid.
@@ -131,8 +142,10 @@ int launch_diagnostics(hipStream_t s, const double* u, const double* v, const do
                        double* partial, unsigned int* counter, double* out);
 
 // boundary.hip
-int launch_boundary_pass(hipStream_t s, const BCParams& P, const BCOps& ops, double dt, double* u,
-                         double* v);
+int launch_boundary_pass(hipStream_t s, const BCParams& P, const BCOps& ops,
+                         const StepScalars& S, double* u, double* v);
 int launch_pressure_ghosts(hipStream_t s, const BCParams& P, double* p, int ldp);
+// dst[0] = dt, dst[1] = blend: the first node of a captured step.
+int launch_set_step_scalars(hipStream_t s, double* dst, double dt, double blend);
 
 }  // namespace pcfd

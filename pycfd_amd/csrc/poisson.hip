@@ -166,6 +166,7 @@ struct EpiCheck {  // relative residual; termination of the relaxation loops
   int mode;
   double tol;
   int maxiter, singular, nfluid;
+  SolveLog* log = nullptr;  // CM_FINAL: the solve's record, or none
   __device__ void ops(int (&o)[K]) const { o[0] = o[1] = kSum; }
   __device__ bool active() const { return mode == CM_FINAL || !st->done; }
   __device__ void operator()(const double (&v)[K]) const {
@@ -174,6 +175,13 @@ struct EpiCheck {  // relative residual; termination of the relaxation loops
     if (mode == CM_FINAL) {
       st->res_final = res;
       st->mean_final = (singular && nfluid > 0) ? v[1] / (double)nfluid : 0.0;
+      if (log) {
+        log->iter = st->iter;
+        log->converged = st->converged;
+        log->done = st->done;
+        log->res = st->res;
+        log->res_final = res;
+      }
       return;
     }
     const double prev = st->res;
@@ -288,8 +296,9 @@ __device__ __forceinline__ double apply5(const FaceC& c, uint8_t f, double d, do
 // ----------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock)
 k_prepare(FineOp A, const double* __restrict__ u, const double* __restrict__ v, int ldu, int ldv,
-          double dx, double dy, double dt, const double* __restrict__ rhs, double* __restrict__ b,
-          double* partial) {
+          double dx, double dy, StepScalars T, const double* __restrict__ rhs,
+          double* __restrict__ b, double* partial) {
+  const double dt = T.get_dt();
   const MultiTile m = multi_tile(A.nx, A.ny);
   uint8_t f[kMultiTiles];
   double g0[kMultiTiles], g1[kMultiTiles], g2[kMultiTiles], g3[kMultiTiles];
@@ -1272,6 +1281,8 @@ PoissonEngine::~PoissonEngine() {
   free_levels();
   for (double* v : {b_, x_, r_, pa_, pb_, q_, z_}) hipFree(v);
   hipFree(st_);
+  hipFree(log_);
+  if (log_host_) hipHostFree(log_host_);
   hipFree(partial_);
   hipFree(counter_);
   hipFree(fine_active_);
@@ -1374,6 +1385,9 @@ int PoissonEngine::init(const FineOp& op, int n_fluid, int singular, hipStream_t
   PCFD_HIP_CHECK(hipMemsetAsync(counter_, 0, 64 * sizeof(unsigned int), s_));
   PCFD_HIP_CHECK(hipMalloc(&st_, sizeof(SolveState)));
   PCFD_HIP_CHECK(hipMemsetAsync(st_, 0, sizeof(SolveState), s_));
+  PCFD_HIP_CHECK(hipMalloc(&log_, kLogSlots * sizeof(SolveLog)));
+  PCFD_HIP_CHECK(hipMemsetAsync(log_, 0, kLogSlots * sizeof(SolveLog), s_));
+  PCFD_HIP_CHECK(hipHostMalloc(&log_host_, kLogSlots * sizeof(SolveLog), hipHostMallocDefault));
   PCFD_HIP_CHECK(hipHostMalloc(&st_host_, sizeof(SolveState), hipHostMallocDefault));
   PCFD_HIP_CHECK(hipEventCreateWithFlags(&ev_, hipEventDisableTiming));
   return kOk;
@@ -1616,6 +1630,7 @@ int PoissonEngine::ensure_partials(size_t n) {
   partial_cap_ = 0;
   PCFD_HIP_CHECK(hipMalloc(&partial_, n * sizeof(double)));
   partial_cap_ = n;
+  ++partial_gen_;
   return kOk;
 }
 
@@ -2031,9 +2046,9 @@ int PoissonEngine::plan_level_tiles(std::vector<LevelBuf>& lv) {
 }
 
 int PoissonEngine::prepare_from_velocity(const double* u, const double* v, int ldu, int ldv,
-                                         double dx, double dy, double dt) {
+                                         double dx, double dy, const StepScalars& T) {
   const int mb = multi_blocks(op_.nx, op_.ny);
-  hipLaunchKernelGGL(k_prepare, dim3(mb), dim3(kBlock), 0, s_, op_, u, v, ldu, ldv, dx, dy, dt,
+  hipLaunchKernelGGL(k_prepare, dim3(mb), dim3(kBlock), 0, s_, op_, u, v, ldu, ldv, dx, dy, T,
                      (const double*)nullptr, b_, partial_);
   PCFD_LAUNCH_CHECK();
   return fold(s_, partial_, mb, EpiPrepare{st_, singular_, n_fluid_});
@@ -2042,7 +2057,8 @@ int PoissonEngine::prepare_from_velocity(const double* u, const double* v, int l
 int PoissonEngine::prepare_from_rhs(const double* rhs) {
   const int mb = multi_blocks(op_.nx, op_.ny);
   hipLaunchKernelGGL(k_prepare, dim3(mb), dim3(kBlock), 0, s_, op_, (const double*)nullptr,
-                     (const double*)nullptr, 0, 0, 1.0, 1.0, 1.0, rhs, b_, partial_);
+                     (const double*)nullptr, 0, 0, 1.0, 1.0, StepScalars{1.0, 0.0, nullptr}, rhs,
+                     b_, partial_);
   PCFD_LAUNCH_CHECK();
   return fold(s_, partial_, mb, EpiPrepare{st_, singular_, n_fluid_});
 }
@@ -2109,8 +2125,17 @@ int PoissonEngine::finalize(bool recompute_residual) {
   hipLaunchKernelGGL(k_check, dim3(mb), dim3(kBlock), 0, s_, op_, x_, b_, (int)CM_FINAL,
                      partial_, (const SolveState*)st_);
   PCFD_LAUNCH_CHECK();
+  SolveLog* log = (log_slot_ >= 0 && log_slot_ < kLogSlots) ? log_ + log_slot_ : nullptr;
   return fold(s_, partial_, mb,
-              EpiCheck{st_, (int)CM_FINAL, cfg_.tol, cfg_.maxiter, singular_, n_fluid_});
+              EpiCheck{st_, (int)CM_FINAL, cfg_.tol, cfg_.maxiter, singular_, n_fluid_, log});
+}
+
+int PoissonEngine::read_log(SolveLog* out, int n) {
+  n = std::min(n, kLogSlots);
+  PCFD_HIP_CHECK(hipMemcpyAsync(log_host_, log_, n * sizeof(SolveLog), hipMemcpyDeviceToHost, s_));
+  PCFD_HIP_CHECK(hipStreamSynchronize(s_));
+  std::memcpy(out, log_host_, n * sizeof(SolveLog));
+  return kOk;
 }
 
 int PoissonEngine::scatter_pressure(double* p, int ldp) {
@@ -2142,8 +2167,18 @@ int PoissonEngine::apply_operator(const double* x, double* y) {
 // the next one's chunk (it would enqueue the whole budget).  Every poll
 // drains the stream and leaves the device idle until the next launch.  PER
 // is the number of iterations one body advances; MAXB bounds the bodies.
+// Under a budget (a captured solve) exactly enough bodies for that many
+// iterations are enqueued and nothing is polled.
 #define PCFD_ITERATE(PER, MAXB, ...)                                          \
   do {                                                                        \
+    if (budget_ > 0) {                                                        \
+      const int _nb = std::min((MAXB), (budget_ + (PER) - 1) / (PER));        \
+      for (int _b = 0; _b < _nb; ++_b) {                                      \
+        __VA_ARGS__;                                                          \
+      }                                                                       \
+      last_run_ = budget_;                                                    \
+      break;                                                                  \
+    }                                                                         \
     int _it = 0, _done = 0, _conv = 0, _bodies = 0, _rc = 0;                  \
     int _chunk = std::max(1, (predicted_ > 0 ? predicted_ : 4) / (PER));      \
     while (!_done && _bodies < (MAXB)) {                                      \

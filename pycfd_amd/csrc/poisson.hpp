@@ -90,6 +90,17 @@ struct SolveState {
   int pad[3];
 };
 
+// What the final residual check of one solve records for the host: a
+// captured step reads these after it ran (one record per solve).
+struct SolveLog {
+  int iter;        // iterations / sweeps / cycles completed
+  int converged;
+  int done;        // 0: the iteration budget ran out before the solve decided
+  int pad;
+  double res;      // the last check's relative residual (relaxation solvers)
+  double res_final;  // the relative true residual of the returned iterate
+};
+
 // Partial sums taken over the output of a V-cycle: none, sum(z) for the
 // standalone solver's mean removal, or sum(z), sum(b*z), sum(b) for the
 // preconditioned-CG inner product.
@@ -193,7 +204,7 @@ class PoissonEngine {
   // b_raw: assembled right-hand side (fluid rows, Dirichlet terms added, not
   // yet mean-free).  The solution overwrites `x_` (the warm-start buffer).
   int prepare_from_velocity(const double* u, const double* v, int ldu, int ldv, double dx,
-                            double dy, double dt);
+                            double dy, const StepScalars& T);
   int prepare_from_rhs(const double* rhs);
   int solve(SolveInfo* info, bool sync_info);
   int scatter_pressure(double* p, int ldp);   // ghosted p, ghost ring zeroed
@@ -203,6 +214,22 @@ class PoissonEngine {
   int apply_operator(const double* x, double* y);
   int apply_preconditioner(const double* r, double* z);
   int read_state(SolveState* st);
+  // Captured solves: `iters` iterations are enqueued without polling (the
+  // kernels past convergence are no-ops); 0 polls as usual.
+  void set_budget(int iters) { budget_ = iters; }
+  // The final check of the solves that follow writes its record to log
+  // slot `slot` (-1: none); read_log copies n records to the host (syncs).
+  void set_log_slot(int slot) { log_slot_ = slot; }
+  int read_log(SolveLog* out, int n);
+  // Whether info.residual reports the final true residual (Krylov solvers)
+  // or the last check's (relaxation solvers and standalone multigrid).
+  bool reports_final_residual() const {
+    return cfg_.kind == SK_CG || cfg_.kind == SK_MGCG || cfg_.kind == SK_DIRECT;
+  }
+  int maxiter() const { return cfg_.maxiter; }
+  bool clock_on() const { return clock_on_; }
+  void set_stream(hipStream_t s) { s_ = s; }
+  static constexpr int kLogSlots = 8;
   // Times `reps` back-to-back launches of one hot kernel on this engine's
   // buffers with HIP events; reports the mean duration and the kernel's
   // algorithmic bytes per launch (arrays read + written once, at stored size).
@@ -252,6 +279,7 @@ class PoissonEngine {
 
   double *partial_ = nullptr;  // per-tile partial sums of the two-launch reductions
   size_t partial_cap_ = 0;     // capacity in doubles
+  int partial_gen_ = 0;        // bumped when partial_ moves (captured graphs hold it)
   unsigned int* counter_ = nullptr;
 
  private:
@@ -320,6 +348,10 @@ class PoissonEngine {
   double *b_ = nullptr, *x_ = nullptr, *r_ = nullptr, *pa_ = nullptr, *pb_ = nullptr,
          *q_ = nullptr, *z_ = nullptr;
   int last_iters_ = 0;
+  int budget_ = 0;     // captured solves: iterations enqueued without polling
+  int log_slot_ = -1;
+  SolveLog* log_ = nullptr;       // kLogSlots records (device)
+  SolveLog* log_host_ = nullptr;  // pinned mirror
   int predicted_ = 0;  // iterations of the last converged solve (first chunk)
   int last_run_ = 0;   // iterations of the last solve
 

@@ -329,18 +329,20 @@ __device__ __forceinline__ void predictor_body(const FU& U, const FV& V, const M
 template <bool S>
 __global__ void __launch_bounds__(kBlock)
 k_predictor(const double* __restrict__ u, const double* __restrict__ v, MomParams P,
-            UpdRange R, double dt, double* __restrict__ us, double* __restrict__ vs) {
-  predictor_body<S>(PlainLoad{u, P.ldu}, PlainLoad{v, P.ldv}, P, R, dt, us, vs);
+            UpdRange R, StepScalars T, double* __restrict__ us, double* __restrict__ vs) {
+  P.blend = T.get_blend();
+  predictor_body<S>(PlainLoad{u, P.ldu}, PlainLoad{v, P.ldv}, P, R, T.get_dt(), us, vs);
 }
 
 template <bool S>
 __global__ void __launch_bounds__(kBlock)
 k_predictor_comb(const double* __restrict__ u0, const double* __restrict__ ue,
                  const double* __restrict__ v0, const double* __restrict__ ve, double ca,
-                 double cb, MomParams P, UpdRange R, double dt, double* __restrict__ us,
+                 double cb, MomParams P, UpdRange R, StepScalars T, double* __restrict__ us,
                  double* __restrict__ vs) {
-  predictor_body<S>(CombLoad{u0, ue, P.ldu, ca, cb}, CombLoad{v0, ve, P.ldv, ca, cb}, P, R, dt,
-                    us, vs);
+  P.blend = T.get_blend();
+  predictor_body<S>(CombLoad{u0, ue, P.ldu, ca, cb}, CombLoad{v0, ve, P.ldv, ca, cb}, P, R,
+                    T.get_dt(), us, vs);
 }
 
 // out = ca*a + cb*b over both ghosted velocity arrays (solver.py:706-721).
@@ -365,8 +367,9 @@ k_combine(const double* __restrict__ u0, const double* __restrict__ ue,
 // stretched mesh.
 __global__ void __launch_bounds__(kBlock)
 k_correct(const double* __restrict__ p, int ldp, int nx, int ny, int ldu, int ldv, double dx,
-          double dy, Metrics M, UpdRange R, double dt, double* __restrict__ u,
+          double dy, Metrics M, UpdRange R, StepScalars T, double* __restrict__ u,
           double* __restrict__ v) {
+  const double dt = T.get_dt();
   const Tile2D t = tile_coords(ny + 3);
   const int i = t.i, j = t.j;
   if (i >= R.u_lo && i < R.u_hi && j >= 1 && j <= ny) {
@@ -556,26 +559,26 @@ int launch_momentum_rhs(hipStream_t s, const double* u, const double* v, const M
 }
 
 int launch_predictor(hipStream_t s, const double* u, const double* v, const MomParams& P,
-                     const UpdRange& R, double dt, double* us, double* vs) {
+                     const UpdRange& R, const StepScalars& T, double* us, double* vs) {
   const dim3 g = grid_for(P.nx + 3, P.ny + 3);
   if (P.g.hx)
-    hipLaunchKernelGGL(k_predictor<true>, g, dim3(kBlock), 0, s, u, v, P, R, dt, us, vs);
+    hipLaunchKernelGGL(k_predictor<true>, g, dim3(kBlock), 0, s, u, v, P, R, T, us, vs);
   else
-    hipLaunchKernelGGL(k_predictor<false>, g, dim3(kBlock), 0, s, u, v, P, R, dt, us, vs);
+    hipLaunchKernelGGL(k_predictor<false>, g, dim3(kBlock), 0, s, u, v, P, R, T, us, vs);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
 
 int launch_predictor_comb(hipStream_t s, const double* u0, const double* ue, const double* v0,
                           const double* ve, double ca, double cb, const MomParams& P,
-                          const UpdRange& R, double dt, double* us, double* vs) {
+                          const UpdRange& R, const StepScalars& T, double* us, double* vs) {
   const dim3 g = grid_for(P.nx + 3, P.ny + 3);
   if (P.g.hx)
     hipLaunchKernelGGL(k_predictor_comb<true>, g, dim3(kBlock), 0, s, u0, ue, v0, ve, ca, cb, P,
-                       R, dt, us, vs);
+                       R, T, us, vs);
   else
     hipLaunchKernelGGL(k_predictor_comb<false>, g, dim3(kBlock), 0, s, u0, ue, v0, ve, ca, cb, P,
-                       R, dt, us, vs);
+                       R, T, us, vs);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
@@ -590,10 +593,10 @@ int launch_combine(hipStream_t s, const double* u0, const double* ue, const doub
 }
 
 int launch_correct(hipStream_t s, const double* p, int ldp, int nx, int ny, int ldu, int ldv,
-                   double dx, double dy, const Metrics& g, const UpdRange& R, double dt, double* u,
-                   double* v) {
+                   double dx, double dy, const Metrics& g, const UpdRange& R,
+                   const StepScalars& T, double* u, double* v) {
   hipLaunchKernelGGL(k_correct, grid_for(nx + 3, ny + 3), dim3(kBlock), 0, s, p, ldp, nx, ny, ldu,
-                     ldv, dx, dy, g, R, dt, u, v);
+                     ldv, dx, dy, g, R, T, u, v);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
