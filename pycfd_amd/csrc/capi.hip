@@ -231,9 +231,8 @@ int substep(pcfd_ctx* c, const double* u, const double* v, const double* ue, con
   if ((rc = c->engine.prepare_from_velocity(us, vs, c->ldu, c->ldv, c->dx, c->dy, S))) return rc;
   if ((rc = c->engine.solve(info, sync))) return rc;
   if ((rc = c->engine.scatter_pressure(p, c->ldp))) return rc;
-  if ((rc = launch_pressure_ghosts(c->stream, c->bc, p, c->ldp))) return rc;
-  if ((rc = launch_correct(c->stream, p, c->ldp, c->nx, c->ny, c->ldu, c->ldv, c->dx, c->dy,
-                           c->g, c->upd, S, us, vs)))
+  if ((rc = launch_correct_ghosts(c->stream, p, c->ldp, c->bc, c->ldu, c->ldv, c->dx, c->dy,
+                                  c->g, c->upd, S, us, vs)))
     return rc;
   return boundary(c, {BOP_MASK_ACC, BOP_WRAP_FACES, BOP_VEL_CORR}, S, us, vs);
 }
@@ -498,9 +497,8 @@ int project_impl(pcfd_ctx* c, double* u, double* v, double dt, double* p, SolveI
   if ((rc = c->engine.prepare_from_velocity(u, v, c->ldu, c->ldv, c->dx, c->dy, S))) return rc;
   if ((rc = c->engine.solve(info, true))) return rc;
   if ((rc = c->engine.scatter_pressure(p, c->ldp))) return rc;
-  if ((rc = launch_pressure_ghosts(c->stream, c->bc, p, c->ldp))) return rc;
-  return launch_correct(c->stream, p, c->ldp, c->nx, c->ny, c->ldu, c->ldv, c->dx, c->dy,
-                        c->g, c->upd, S, u, v);
+  return launch_correct_ghosts(c->stream, p, c->ldp, c->bc, c->ldu, c->ldv, c->dx, c->dy, c->g,
+                               c->upd, S, u, v);
 }
 
 }  // namespace
@@ -784,6 +782,11 @@ int pcfd_ctx_create(const pcfd_case_desc* d, void* hip_stream, pcfd_ctx** out) {
   M.dy = d->dy;
   M.dx2 = d->dx * d->dx;
   M.dy2 = d->dy * d->dy;
+  M.pow2 = !stretched && is_pow2(d->dx) && is_pow2(d->dy);
+  M.idx = 1.0 / d->dx;
+  M.idy = 1.0 / d->dy;
+  M.idx2 = 1.0 / M.dx2;
+  M.idy2 = 1.0 / M.dy2;
   M.nu = d->nu;
   M.fx = d->body_fx;
   M.fy = d->body_fy;

@@ -1,6 +1,8 @@
 // Declarations shared by the translation units of libpycfd_hip.so.
 #pragma once
 
+#include <cmath>
+
 #include "common.hpp"
 
 namespace pcfd {
@@ -77,6 +79,10 @@ struct BCParams {
 struct MomParams {
   int nx, ny, ldu, ldv;
   double dx, dy, dx2, dy2, nu, fx, fy, blend;
+  // 1/dx, 1/dy, 1/dx^2, 1/dy^2: used in place of the divisions when dx and
+  // dy are powers of two (pow2), where both give the same bits
+  double idx, idy, idx2, idy2;
+  int pow2;
   // Variable viscosity (Smagorinsky LES): cell-centred nu + nu_t on the
   // ghosted (nx+2, ny+2) layout, row pitch ny + 2; nullptr: constant nu.
   const double* nuc;
@@ -134,6 +140,17 @@ int launch_correct(hipStream_t s, const double* p, int ldp, int nx, int ny, int 
                    const StepScalars& S, double* u, double* v);
 int launch_divergence(hipStream_t s, const double* u, const double* v, int nx, int ny, int ldu,
                       int ldv, double dx, double dy, const Metrics& g, double* out);
+// The pressure ghost layer of the normalised interior p (apply_pressure)
+// and the gradient correction of u, v, in one pass.
+int launch_correct_ghosts(hipStream_t s, double* p, int ldp, const BCParams& B, int ldu, int ldv,
+                          double dx, double dy, const Metrics& g, const UpdRange& R,
+                          const StepScalars& T, double* u, double* v);
+// A spacing whose reciprocal is exact (a power of two): division by it and
+// multiplication by its reciprocal give the same bits, so kernels multiply.
+inline bool is_pow2(double h) {
+  int e = 0;
+  return h > 0.0 && std::frexp(h, &e) == 0.5 && std::ldexp(1.0, -e + 1) * h == 1.0;
+}
 // Doubles of partial sums launch_diagnostics writes for an nx x ny grid.
 size_t diagnostics_partials(int nx, int ny);
 int launch_diagnostics(hipStream_t s, const double* u, const double* v, const double* p,

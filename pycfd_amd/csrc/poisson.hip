@@ -294,10 +294,12 @@ __device__ __forceinline__ double apply5(const FaceC& c, uint8_t f, double d, do
 // ----------------------------------------------------------------------------
 // Right-hand side assembly (solver.py:659-661, pressure.py:300-308)
 // ----------------------------------------------------------------------------
+template <bool P2>
 __global__ void __launch_bounds__(kBlock)
 k_prepare(FineOp A, const double* __restrict__ u, const double* __restrict__ v, int ldu, int ldv,
-          double dx, double dy, StepScalars T, const double* __restrict__ rhs,
-          double* __restrict__ b, double* partial) {
+          double dx, double dy, double idx, double idy, StepScalars T,
+          const double* __restrict__ rhs, double* __restrict__ b, double* __restrict__ x,
+          double* partial) {
   const double dt = T.get_dt();
   const MultiTile m = multi_tile(A.nx, A.ny);
   uint8_t f[kMultiTiles];
@@ -326,13 +328,18 @@ k_prepare(FineOp A, const double* __restrict__ u, const double* __restrict__ v, 
     if (rhs) {
       val = g0[q];
     } else {
-      const double hx = A.hxc ? A.hxc[i] : dx, hy = A.hyc ? A.hyc[j] : dy;
-      const double div = (g0[q] - g1[q]) / hx + (g2[q] - g3[q]) / hy;
+      double div;
+      if (A.hxc) div = (g0[q] - g1[q]) / A.hxc[i] + (g2[q] - g3[q]) / A.hyc[j];
+      else if constexpr (P2) div = (g0[q] - g1[q]) * idx + (g2[q] - g3[q]) * idy;
+      else div = (g0[q] - g1[q]) / dx + (g2[q] - g3[q]) / dy;
       val = div / dt;
     }
     val = val * fine_row_scale(A, i, j) +
           ((f[q] & F_SOLV) ? dirichlet_rhs(A, fine_faces(A, i, j), i, j) : 0.0);
-    if (!(f[q] & F_SOLV)) val = 0.0;
+    if (!(f[q] & F_SOLV)) {
+      val = 0.0;
+      x[m.k[q]] = 0.0;  // the start guess holds zero off the solvable set
+    }
     b[m.k[q]] = val;
     s[0] += val;
   }
@@ -381,15 +388,6 @@ k_init(FineOp A, double* __restrict__ b, const double* __restrict__ x, double* _
   }
   const int ops[4] = {kSum, kSum, kSum, kSum};
   block_partials<4>(s, ops, partial, m.blk);
-}
-
-__global__ void __launch_bounds__(kBlock)
-k_zero_nonfluid(FineOp A, double* __restrict__ x) {
-  Tile2D t;
-  if (cell_of_tile(A.nx, A.ny, t)) {
-    const int k = t.i * A.ny + t.j;
-    if (!(A.flags[k] & F_SOLV)) x[k] = 0.0;
-  }
 }
 
 // ----------------------------------------------------------------------------
@@ -2048,17 +2046,21 @@ int PoissonEngine::plan_level_tiles(std::vector<LevelBuf>& lv) {
 int PoissonEngine::prepare_from_velocity(const double* u, const double* v, int ldu, int ldv,
                                          double dx, double dy, const StepScalars& T) {
   const int mb = multi_blocks(op_.nx, op_.ny);
-  hipLaunchKernelGGL(k_prepare, dim3(mb), dim3(kBlock), 0, s_, op_, u, v, ldu, ldv, dx, dy, T,
-                     (const double*)nullptr, b_, partial_);
+  if (is_pow2(dx) && is_pow2(dy))
+    hipLaunchKernelGGL(k_prepare<true>, dim3(mb), dim3(kBlock), 0, s_, op_, u, v, ldu, ldv, dx, dy,
+                       1.0 / dx, 1.0 / dy, T, (const double*)nullptr, b_, x_, partial_);
+  else
+    hipLaunchKernelGGL(k_prepare<false>, dim3(mb), dim3(kBlock), 0, s_, op_, u, v, ldu, ldv, dx,
+                       dy, 1.0 / dx, 1.0 / dy, T, (const double*)nullptr, b_, x_, partial_);
   PCFD_LAUNCH_CHECK();
   return fold(s_, partial_, mb, EpiPrepare{st_, singular_, n_fluid_});
 }
 
 int PoissonEngine::prepare_from_rhs(const double* rhs) {
   const int mb = multi_blocks(op_.nx, op_.ny);
-  hipLaunchKernelGGL(k_prepare, dim3(mb), dim3(kBlock), 0, s_, op_, (const double*)nullptr,
-                     (const double*)nullptr, 0, 0, 1.0, 1.0, StepScalars{1.0, 0.0, nullptr}, rhs,
-                     b_, partial_);
+  hipLaunchKernelGGL(k_prepare<false>, dim3(mb), dim3(kBlock), 0, s_, op_,
+                     (const double*)nullptr, (const double*)nullptr, 0, 0, 1.0, 1.0, 1.0, 1.0,
+                     StepScalars{1.0, 0.0, nullptr}, rhs, b_, x_, partial_);
   PCFD_LAUNCH_CHECK();
   return fold(s_, partial_, mb, EpiPrepare{st_, singular_, n_fluid_});
 }
@@ -2193,9 +2195,9 @@ int PoissonEngine::apply_operator(const double* x, double* y) {
     if (_conv) predicted_ = _it;                                              \
   } while (0)
 
+// The right-hand side pass (k_prepare) has zeroed the guess off the
+// solvable set.
 int PoissonEngine::start(int mode, double* r) {
-  const dim3 g = tiles_for(op_.nx, op_.ny);
-  hipLaunchKernelGGL(k_zero_nonfluid, g, dim3(kBlock), 0, s_, op_, x_);
   const int mb = multi_blocks(op_.nx, op_.ny);
   hipLaunchKernelGGL(k_init, dim3(mb), dim3(kBlock), 0, s_, op_, b_, (const double*)x_, r, mode,
                      (const SolveState*)st_, partial_);

@@ -88,7 +88,15 @@ __device__ __forceinline__ double visc_v_face(const FU& U, const FV& V, int i, i
 // `U(i, j)` / `V(i, j)` return the ghosted velocity arrays' entries; they are
 // functors so a Runge-Kutta stage combination can be folded into the loads.
 // ----------------------------------------------------------------------------
-template <class FU, class FV>
+// a / h on a uniform mesh; a * (1 / h), the same bits, when h is a power of
+// two (P2)
+template <bool P2>
+__device__ __forceinline__ double dvd(double a, double h, double inv_h) {
+  if constexpr (P2) return a * inv_h;
+  else return a / h;
+}
+
+template <bool P2, class FU, class FV>
 __device__ __forceinline__ double rhs_u_uniform(const FU& U, const FV& V, int m, int j,
                                                 const MomParams& P) {
   const double u0 = U(m, j);
@@ -97,23 +105,26 @@ __device__ __forceinline__ double rhs_u_uniform(const FU& U, const FV& V, int m,
   // cell-centred interpolants right/left of the face (solver.py:376, 382-383)
   const double uu_r = 0.5 * (u0 + up);
   const double uu_l = 0.5 * (um + u0);
-  double duudx = (uu_r * uu_r - uu_l * uu_l) / P.dx;
+  double duudx = dvd<P2>(uu_r * uu_r - uu_l * uu_l, P.dx, P.idx);
   // corner products above/below the face (solver.py:351-357, 385)
   const double u_top = 0.5 * (u0 + uu);
   const double v_top = 0.5 * (V(m - 1, j + 1) + V(m, j + 1));
   const double u_bot = 0.5 * (ud + u0);
   const double v_bot = 0.5 * (V(m - 1, j) + V(m, j));
-  double duvdy = (u_top * v_top - u_bot * v_bot) / P.dy;
+  double duvdy = dvd<P2>(u_top * v_top - u_bot * v_bot, P.dy, P.idy);
   if (P.blend > 0.0) {  // donor-cell correction (solver.py:393-407)
-    duudx += P.blend / P.dx * (fabs(uu_r) * (u0 - up) / 2.0 - fabs(uu_l) * (um - u0) / 2.0);
-    duvdy += P.blend / P.dy * (fabs(v_top) * (u0 - uu) / 2.0 - fabs(v_bot) * (ud - u0) / 2.0);
+    duudx += dvd<P2>(P.blend, P.dx, P.idx) *
+             (fabs(uu_r) * (u0 - up) / 2.0 - fabs(uu_l) * (um - u0) / 2.0);
+    duvdy += dvd<P2>(P.blend, P.dy, P.idy) *
+             (fabs(v_top) * (u0 - uu) / 2.0 - fabs(v_bot) * (ud - u0) / 2.0);
   }
   if (P.nuc) return -(duudx + duvdy) + visc_u_face<false>(U, V, m, j, P) + P.fx;
-  const double d2u = (um - 2.0 * u0 + up) / P.dx2 + (ud - 2.0 * u0 + uu) / P.dy2;
+  const double d2u =
+      dvd<P2>(um - 2.0 * u0 + up, P.dx2, P.idx2) + dvd<P2>(ud - 2.0 * u0 + uu, P.dy2, P.idy2);
   return -(duudx + duvdy) + P.nu * d2u + P.fx;
 }
 
-template <class FU, class FV>
+template <bool P2, class FU, class FV>
 __device__ __forceinline__ double rhs_v_uniform(const FU& U, const FV& V, int i, int n,
                                                 const MomParams& P) {
   const double v0 = V(i, n);
@@ -121,18 +132,21 @@ __device__ __forceinline__ double rhs_v_uniform(const FU& U, const FV& V, int i,
   const double vl = V(i - 1, n), vr = V(i + 1, n);
   const double vv_t = 0.5 * (v0 + vp);
   const double vv_b = 0.5 * (vm + v0);
-  double dvvdy = (vv_t * vv_t - vv_b * vv_b) / P.dy;
+  double dvvdy = dvd<P2>(vv_t * vv_t - vv_b * vv_b, P.dy, P.idy);
   const double u_right = 0.5 * (U(i + 1, n - 1) + U(i + 1, n));
   const double v_right = 0.5 * (v0 + vr);
   const double u_left = 0.5 * (U(i, n - 1) + U(i, n));
   const double v_left = 0.5 * (vl + v0);
-  double duvdx = (u_right * v_right - u_left * v_left) / P.dx;
+  double duvdx = dvd<P2>(u_right * v_right - u_left * v_left, P.dx, P.idx);
   if (P.blend > 0.0) {  // solver.py:409-417
-    dvvdy += P.blend / P.dy * (fabs(vv_t) * (v0 - vp) / 2.0 - fabs(vv_b) * (vm - v0) / 2.0);
-    duvdx += P.blend / P.dx * (fabs(u_right) * (v0 - vr) / 2.0 - fabs(u_left) * (vl - v0) / 2.0);
+    dvvdy += dvd<P2>(P.blend, P.dy, P.idy) *
+             (fabs(vv_t) * (v0 - vp) / 2.0 - fabs(vv_b) * (vm - v0) / 2.0);
+    duvdx += dvd<P2>(P.blend, P.dx, P.idx) *
+             (fabs(u_right) * (v0 - vr) / 2.0 - fabs(u_left) * (vl - v0) / 2.0);
   }
   if (P.nuc) return -(duvdx + dvvdy) + visc_v_face<false>(U, V, i, n, P) + P.fy;
-  const double d2v = (vl - 2.0 * v0 + vr) / P.dx2 + (vm - 2.0 * v0 + vp) / P.dy2;
+  const double d2v =
+      dvd<P2>(vl - 2.0 * v0 + vr, P.dx2, P.idx2) + dvd<P2>(vm - 2.0 * v0 + vp, P.dy2, P.idy2);
   return -(duvdx + dvvdy) + P.nu * d2v + P.fy;
 }
 
@@ -207,18 +221,18 @@ __device__ __forceinline__ double rhs_v_stretched(const FU& U, const FV& V, int 
   return -(duvdx + dvvdy) + P.nu * d2v + P.fy;
 }
 
-template <bool S, class FU, class FV>
+template <bool S, bool P2, class FU, class FV>
 __device__ __forceinline__ double rhs_u_face(const FU& U, const FV& V, int m, int j,
                                              const MomParams& P) {
   if constexpr (S) return rhs_u_stretched(U, V, m, j, P);
-  else return rhs_u_uniform(U, V, m, j, P);
+  else return rhs_u_uniform<P2>(U, V, m, j, P);
 }
 
-template <bool S, class FU, class FV>
+template <bool S, bool P2, class FU, class FV>
 __device__ __forceinline__ double rhs_v_face(const FU& U, const FV& V, int i, int n,
                                              const MomParams& P) {
   if constexpr (S) return rhs_v_stretched(U, V, i, n, P);
-  else return rhs_v_uniform(U, V, i, n, P);
+  else return rhs_v_uniform<P2>(U, V, i, n, P);
 }
 
 struct PlainLoad {
@@ -294,21 +308,22 @@ k_corner_viscosity(const double* __restrict__ nuc, int nx, int ny, double* __res
 }
 
 // rhs_u (nx+1, ny) and rhs_v (nx, ny+1), contiguous (solver.py:501-529).
-template <bool S>
+template <bool S, bool P2>
 __global__ void __launch_bounds__(kBlock)
 k_momentum_rhs(const double* __restrict__ u, const double* __restrict__ v, MomParams P,
                double* __restrict__ rhs_u, double* __restrict__ rhs_v) {
   const Tile2D t = tile_coords(P.ny + 1);
   const int i = t.i, j = t.j;  // i: 0..nx, j: 0..ny
   PlainLoad U{u, P.ldu}, V{v, P.ldv};
-  if (i <= P.nx && j < P.ny) rhs_u[(size_t)i * P.ny + j] = rhs_u_face<S>(U, V, i + 1, j + 1, P);
+  if (i <= P.nx && j < P.ny)
+    rhs_u[(size_t)i * P.ny + j] = rhs_u_face<S, P2>(U, V, i + 1, j + 1, P);
   if (i < P.nx && j <= P.ny)
-    rhs_v[(size_t)i * (P.ny + 1) + j] = rhs_v_face<S>(U, V, i + 1, j + 1, P);
+    rhs_v[(size_t)i * (P.ny + 1) + j] = rhs_v_face<S, P2>(U, V, i + 1, j + 1, P);
 }
 
 // Predictor us = u; us[u_upd] += dt*rhs_u (solver.py:617-622), both
 // components, over the full ghosted arrays.
-template <bool S, class FU, class FV>
+template <bool S, bool P2, class FU, class FV>
 __device__ __forceinline__ void predictor_body(const FU& U, const FV& V, const MomParams& P,
                                                const UpdRange& R, double dt,
                                                double* __restrict__ us, double* __restrict__ vs) {
@@ -316,33 +331,35 @@ __device__ __forceinline__ void predictor_body(const FU& U, const FV& V, const M
   const int i = t.i, j = t.j;
   if (i <= P.nx + 2 && j <= P.ny + 1) {
     double val = U(i, j);
-    if (i >= R.u_lo && i < R.u_hi && j >= 1 && j <= P.ny) val += dt * rhs_u_face<S>(U, V, i, j, P);
+    if (i >= R.u_lo && i < R.u_hi && j >= 1 && j <= P.ny)
+      val += dt * rhs_u_face<S, P2>(U, V, i, j, P);
     us[(size_t)i * P.ldu + j] = val;
   }
   if (i <= P.nx + 1 && j <= P.ny + 2) {
     double val = V(i, j);
-    if (i >= 1 && i <= P.nx && j >= R.v_lo && j < R.v_hi) val += dt * rhs_v_face<S>(U, V, i, j, P);
+    if (i >= 1 && i <= P.nx && j >= R.v_lo && j < R.v_hi)
+      val += dt * rhs_v_face<S, P2>(U, V, i, j, P);
     vs[(size_t)i * P.ldv + j] = val;
   }
 }
 
-template <bool S>
+template <bool S, bool P2>
 __global__ void __launch_bounds__(kBlock)
 k_predictor(const double* __restrict__ u, const double* __restrict__ v, MomParams P,
             UpdRange R, StepScalars T, double* __restrict__ us, double* __restrict__ vs) {
   P.blend = T.get_blend();
-  predictor_body<S>(PlainLoad{u, P.ldu}, PlainLoad{v, P.ldv}, P, R, T.get_dt(), us, vs);
+  predictor_body<S, P2>(PlainLoad{u, P.ldu}, PlainLoad{v, P.ldv}, P, R, T.get_dt(), us, vs);
 }
 
-template <bool S>
+template <bool S, bool P2>
 __global__ void __launch_bounds__(kBlock)
 k_predictor_comb(const double* __restrict__ u0, const double* __restrict__ ue,
                  const double* __restrict__ v0, const double* __restrict__ ve, double ca,
                  double cb, MomParams P, UpdRange R, StepScalars T, double* __restrict__ us,
                  double* __restrict__ vs) {
   P.blend = T.get_blend();
-  predictor_body<S>(CombLoad{u0, ue, P.ldu, ca, cb}, CombLoad{v0, ve, P.ldv, ca, cb}, P, R,
-                    T.get_dt(), us, vs);
+  predictor_body<S, P2>(CombLoad{u0, ue, P.ldu, ca, cb}, CombLoad{v0, ve, P.ldv, ca, cb}, P, R,
+                        T.get_dt(), us, vs);
 }
 
 // out = ca*a + cb*b over both ghosted velocity arrays (solver.py:706-721).
@@ -382,6 +399,73 @@ k_correct(const double* __restrict__ p, int ldp, int nx, int ny, int ldu, int ld
     const double g = dt * (p[(size_t)i * ldp + j] - p[(size_t)i * ldp + j - 1]) / h;
     v[(size_t)i * ldv + j] -= g;
   }
+}
+
+// Gradient correction with the pressure ghost layer filled in the same pass
+// (apply_pressure then the correction, solver.py:664-682, boundary.py:166-169,
+// 293-303, 381-385, 412-420).  p holds the normalised interior; every ghost
+// the correction or the output needs is formed from it: a wall's ghost is
+// the adjacent interior value (Neumann) or 2 p_ref - p_in (pressure outlet),
+// a periodic axis wraps, and the corner values follow the reference's wall
+// order (left, right, bottom, top, then the x and y wraps), so the ghost
+// ring written equals what the separate ghost pass writes.  The interior of
+// p is only read, the ring only written: no thread reads what another
+// writes.
+struct PGhost {
+  int nx, ny, ldp;
+  int kind[4];
+  double pref[4];
+  int px, py;
+};
+
+__device__ __forceinline__ double p_wall(const PGhost& G, int w, double in) {
+  return G.kind[w] == BC_PRESSURE_OUTLET ? 2.0 * G.pref[w] - in : in;
+}
+
+// Rows 1..ny of ghosted column i (0..nx+1).
+__device__ __forceinline__ double p_col(const PGhost& G, const double* __restrict__ p, int i,
+                                        int j) {
+  if (i >= 1 && i <= G.nx) return p[(size_t)i * G.ldp + j];
+  if (G.px) return p[(size_t)(i == 0 ? G.nx : 1) * G.ldp + j];
+  const int src = (i == 0) ? 1 : G.nx;
+  return p_wall(G, i == 0 ? 0 : 1, p[(size_t)src * G.ldp + j]);
+}
+
+// Any ghosted cell (0..nx+1, 0..ny+1).
+__device__ __forceinline__ double p_full(const PGhost& G, const double* __restrict__ p, int i,
+                                         int j) {
+  if (j >= 1 && j <= G.ny) return p_col(G, p, i, j);
+  if (G.py) return p_col(G, p, i, j == 0 ? G.ny : 1);
+  return p_wall(G, j == 0 ? 2 : 3, p_col(G, p, i, j == 0 ? 1 : G.ny));
+}
+
+template <bool P2>
+__global__ void __launch_bounds__(kBlock)
+k_correct_ghosts(double* __restrict__ p, PGhost G, int ldu, int ldv, double dx, double dy,
+                 double idx, double idy, Metrics M, UpdRange R, StepScalars T,
+                 double* __restrict__ u, double* __restrict__ v) {
+  const double dt = T.get_dt();
+  const int nx = G.nx, ny = G.ny;
+  const Tile2D t = tile_coords(ny + 3);
+  const int i = t.i, j = t.j;
+  if (i >= R.u_lo && i < R.u_hi && j >= 1 && j <= ny) {
+    const double d = dt * (p_col(G, p, i, j) - p_col(G, p, i - 1, j));
+    double g;
+    if (M.hxu) g = d / M.hxu[i - 1];
+    else if constexpr (P2) g = d * idx;
+    else g = d / dx;
+    u[(size_t)i * ldu + j] -= g;
+  }
+  if (i >= 1 && i <= nx && j >= R.v_lo && j < R.v_hi) {
+    const double d = dt * (p_full(G, p, i, j) - p_full(G, p, i, j - 1));
+    double g;
+    if (M.hyv) g = d / M.hyv[j - 1];
+    else if constexpr (P2) g = d * idy;
+    else g = d / dy;
+    v[(size_t)i * ldv + j] -= g;
+  }
+  if (i <= nx + 1 && j <= ny + 1 && (i == 0 || i == nx + 1 || j == 0 || j == ny + 1))
+    p[(size_t)i * G.ldp + j] = p_full(G, p, i, j);
 }
 
 // Raw divergence (nx, ny) (solver.py:532-543), over each cell's own width
@@ -551,9 +635,14 @@ int launch_momentum_rhs(hipStream_t s, const double* u, const double* v, const M
                         double* rhs_u, double* rhs_v) {
   const dim3 g = grid_for(P.nx + 1, P.ny + 1);
   if (P.g.hx)
-    hipLaunchKernelGGL(k_momentum_rhs<true>, g, dim3(kBlock), 0, s, u, v, P, rhs_u, rhs_v);
+    hipLaunchKernelGGL((k_momentum_rhs<true, false>), g, dim3(kBlock), 0, s, u, v, P, rhs_u,
+                       rhs_v);
+  else if (P.pow2)
+    hipLaunchKernelGGL((k_momentum_rhs<false, true>), g, dim3(kBlock), 0, s, u, v, P, rhs_u,
+                       rhs_v);
   else
-    hipLaunchKernelGGL(k_momentum_rhs<false>, g, dim3(kBlock), 0, s, u, v, P, rhs_u, rhs_v);
+    hipLaunchKernelGGL((k_momentum_rhs<false, false>), g, dim3(kBlock), 0, s, u, v, P, rhs_u,
+                       rhs_v);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
@@ -562,9 +651,11 @@ int launch_predictor(hipStream_t s, const double* u, const double* v, const MomP
                      const UpdRange& R, const StepScalars& T, double* us, double* vs) {
   const dim3 g = grid_for(P.nx + 3, P.ny + 3);
   if (P.g.hx)
-    hipLaunchKernelGGL(k_predictor<true>, g, dim3(kBlock), 0, s, u, v, P, R, T, us, vs);
+    hipLaunchKernelGGL((k_predictor<true, false>), g, dim3(kBlock), 0, s, u, v, P, R, T, us, vs);
+  else if (P.pow2)
+    hipLaunchKernelGGL((k_predictor<false, true>), g, dim3(kBlock), 0, s, u, v, P, R, T, us, vs);
   else
-    hipLaunchKernelGGL(k_predictor<false>, g, dim3(kBlock), 0, s, u, v, P, R, T, us, vs);
+    hipLaunchKernelGGL((k_predictor<false, false>), g, dim3(kBlock), 0, s, u, v, P, R, T, us, vs);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
@@ -574,11 +665,14 @@ int launch_predictor_comb(hipStream_t s, const double* u0, const double* ue, con
                           const UpdRange& R, const StepScalars& T, double* us, double* vs) {
   const dim3 g = grid_for(P.nx + 3, P.ny + 3);
   if (P.g.hx)
-    hipLaunchKernelGGL(k_predictor_comb<true>, g, dim3(kBlock), 0, s, u0, ue, v0, ve, ca, cb, P,
-                       R, T, us, vs);
+    hipLaunchKernelGGL((k_predictor_comb<true, false>), g, dim3(kBlock), 0, s, u0, ue, v0, ve, ca,
+                       cb, P, R, T, us, vs);
+  else if (P.pow2)
+    hipLaunchKernelGGL((k_predictor_comb<false, true>), g, dim3(kBlock), 0, s, u0, ue, v0, ve, ca,
+                       cb, P, R, T, us, vs);
   else
-    hipLaunchKernelGGL(k_predictor_comb<false>, g, dim3(kBlock), 0, s, u0, ue, v0, ve, ca, cb, P,
-                       R, T, us, vs);
+    hipLaunchKernelGGL((k_predictor_comb<false, false>), g, dim3(kBlock), 0, s, u0, ue, v0, ve,
+                       ca, cb, P, R, T, us, vs);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
@@ -597,6 +691,26 @@ int launch_correct(hipStream_t s, const double* p, int ldp, int nx, int ny, int 
                    const StepScalars& T, double* u, double* v) {
   hipLaunchKernelGGL(k_correct, grid_for(nx + 3, ny + 3), dim3(kBlock), 0, s, p, ldp, nx, ny, ldu,
                      ldv, dx, dy, g, R, T, u, v);
+  PCFD_LAUNCH_CHECK();
+  return kOk;
+}
+
+int launch_correct_ghosts(hipStream_t s, double* p, int ldp, const BCParams& B, int ldu, int ldv,
+                          double dx, double dy, const Metrics& g, const UpdRange& R,
+                          const StepScalars& T, double* u, double* v) {
+  PGhost G{B.nx, B.ny, ldp, {}, {}, B.periodic_x, B.periodic_y};
+  for (int w = 0; w < 4; ++w) {
+    G.kind[w] = B.kind[w];
+    G.pref[w] = B.pref[w];
+  }
+  const bool p2 = is_pow2(dx) && is_pow2(dy);
+  const dim3 grid = grid_for(B.nx + 3, B.ny + 3);
+  if (p2)
+    hipLaunchKernelGGL(k_correct_ghosts<true>, grid, dim3(kBlock), 0, s, p, G, ldu, ldv, dx, dy,
+                       1.0 / dx, 1.0 / dy, g, R, T, u, v);
+  else
+    hipLaunchKernelGGL(k_correct_ghosts<false>, grid, dim3(kBlock), 0, s, p, G, ldu, ldv, dx, dy,
+                       1.0 / dx, 1.0 / dy, g, R, T, u, v);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }

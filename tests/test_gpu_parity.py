@@ -405,3 +405,75 @@ def test_tune_file_records_measured_plan(gpu_available, tmp_path, monkeypatch):
     plan = s.ctx.mg_plan()
     assert len(plan) > 1
     assert _mg_solver("cavity", **kw).ctx.mg_plan() == plan
+
+
+# ------------------------------------------------------------ fused passes, power-of-two spacing
+@pytest.mark.parametrize("name", ["cavity", "periodic", "channel", "cyl_outlet", "cyl_pout"])
+def test_projection_ghost_ring_equals_apply_pressure(gpu_available, name):
+    """The correction kernel fills the pressure ghost layer itself: the ring
+    it writes equals the oracle's apply_pressure (boundary.py:166-169,
+    293-303, 412-420, pinned bit for bit to the reference) applied to the
+    same interior, corners included, and the velocity correction equals the
+    oracle's on the same pressure."""
+    from pycfd_amd.core.fields import device_zeros, to_device, to_host
+    rng = np.random.default_rng(11)
+    s = _solver(name, pressure_solver="mgcg", poisson_tol=1e-12)
+    case = oracle_case(name)
+    nx, ny = case.nx, case.ny
+    u0 = rng.standard_normal((nx + 3, ny + 2))
+    v0 = rng.standard_normal((nx + 2, ny + 3))
+    ud, vd = to_device(u0), to_device(v0)
+    p = device_zeros((nx + 2, ny + 2))
+    dt = 0.0123
+    s.ctx.project(ud, vd, dt, p)
+    ph = to_host(p)
+    want = np.zeros_like(ph)
+    want[1:nx + 1, 1:ny + 1] = ph[1:nx + 1, 1:ny + 1]
+    O.apply_pressure(case, want)
+    assert np.array_equal(ph, want)
+    u_lo, u_hi, v_lo, v_hi = O.upd_ranges(case)
+    uw, vw = u0.copy(), v0.copy()
+    uw[u_lo:u_hi, 1:ny + 1] -= dt * (want[u_lo:u_hi, 1:ny + 1] - want[u_lo - 1:u_hi - 1, 1:ny + 1]) / case.dx
+    vw[1:nx + 1, v_lo:v_hi] -= dt * (want[1:nx + 1, v_lo:v_hi] - want[1:nx + 1, v_lo - 1:v_hi - 1]) / case.dy
+    assert np.array_equal(to_host(ud), uw) and np.array_equal(to_host(vd), vw)
+
+
+POW2 = [("cavity", dict(nx=32, ny=16, lx=1.0, ly=0.5)),
+        ("periodic", dict(nx=32, ny=16, lx=2.0, ly=1.0)),
+        ("cyl_outlet", dict(nx=64, ny=32, lx=16.0, ly=8.0))]
+
+
+@pytest.mark.parametrize("name,grid", POW2, ids=[p[0] for p in POW2])
+@pytest.mark.parametrize("blend", [0.0, 0.7])
+def test_power_of_two_spacing_bitwise(gpu_available, name, grid, blend):
+    """On a grid whose spacings are powers of two the kernels multiply by the
+    exact reciprocals instead of dividing: the momentum RHS and divergence
+    stay bit-identical to the oracle's NumPy arithmetic (itself pinned bit for
+    bit to the reference), and so does one Euler predictor step."""
+    from pycfd_amd.core.fields import to_device, to_host
+    rng = np.random.default_rng(5)
+    s = _solver(name, **grid)
+    assert s.ctx is not None
+    case = oracle_case(name, **grid)
+    assert case.dx in (2.0 ** -5, 2.0 ** -4, 2.0 ** -2) and case.dy == case.dx
+    u = rng.standard_normal(s.mesh.u_shape)
+    v = rng.standard_normal(s.mesh.v_shape)
+    ru, rv = s.momentum_rhs(u, v, blend)
+    wu, wv = O.momentum_rhs(case, u, v, blend)
+    assert np.array_equal(ru, wu) and np.array_equal(rv, wv)
+    assert np.array_equal(s.divergence(u, v), O.divergence(case, u, v))
+
+
+@pytest.mark.parametrize("name,grid", POW2, ids=[p[0] for p in POW2])
+def test_power_of_two_trajectory_matches_oracle(gpu_available, name, grid):
+    """Whole RK3 steps on a power-of-two grid against the oracle: velocity
+    within 1e-10 relative, pressure within 1e-8."""
+    case = oracle_case(name, pressure_solver="direct", tol=1e-12, **grid)
+    u0, v0 = initial(name, case)
+    sim = _sim(name, u0, v0, pressure_solver="mgcg", poisson_tol=1e-13, **grid)
+    sim.run(max_steps=6)
+    u, v, p, hist, t = O.run(case, 6, u0, v0)
+    scale = max(np.abs(u).max(), 1.0)
+    assert np.abs(sim.fields.u - u).max() <= 1e-10 * scale
+    assert np.abs(sim.fields.v - v).max() <= 1e-10 * scale
+    assert np.abs(sim.fields.p - p).max() <= 1e-8 * max(np.abs(p).max(), 1.0)
