@@ -3,7 +3,9 @@
 Workload: flow past a cylinder, 2048 x 1024 channel (32 x 16 D), Re = 100,
 uniform inflow, zero-gradient OUTLET, slip walls, D = 1 body at 4 D, v seeded
 with a 1e-3 Gaussian blob; SSP-RK3, adaptive dt (ceiling 0.02, CFL 0.4);
-pressure by multigrid-preconditioned CG.  Both precision paths run: plain
+pressure by multigrid-preconditioned CG, its smoothing sweeps (V(s,s), s in
+1..3) picked by the in-project tuner on whole steps of the warmed-up run.
+Both precision paths run: plain
 FP64, and mixed (FP32 V-cycle inside FP64 CG), at the same fixed divergence
 target: the relative residual tolerance (2e-11 by default) is chosen so
 that max|div u| <= 1e-10 after every step on both paths from the first one
@@ -80,7 +82,7 @@ def load_traffic() -> dict:
 
 
 def run_path(precision: str, steps: int, warmup: int, nx: int, ny: int, tol: float,
-             traffic: dict):
+             traffic: dict, tune_sweeps: bool = True):
     import numpy as np
     import torch
 
@@ -102,6 +104,18 @@ def run_path(precision: str, steps: int, warmup: int, nx: int, ny: int, tol: flo
         for p in plan))
     if warmup:
         sim.run(max_steps=warmup)
+    sweeps = {}
+    if tune_sweeps:
+        # the in-project tuner: multigrid smoothing sweeps timed on whole
+        # steps from the warmed-up state, the fastest kept
+        sweeps = sim.solver.tune_smoother_sweeps(sim.fields, sim.stepper.compute_dt(sim.fields),
+                                                 candidates=(1, 2, 3), steps=3)
+        log(f"[{precision}] smoother sweeps tuned: " + ", ".join(
+            f"V({k},{k}) {v['ms_per_step']:.3f} ms/step {v['iterations']} it"
+            for k, v in sweeps.items()) +
+            f" -> V({sim.solver.pressure_solver.smooth_sweeps},"
+            f"{sim.solver.pressure_solver.smooth_sweeps})")
+        plan = sim.solver.ctx.mg_plan()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     res = sim.run(max_steps=steps)
@@ -121,6 +135,9 @@ def run_path(precision: str, steps: int, warmup: int, nx: int, ny: int, tol: flo
         # mean drag coefficient over the timed steps, from each step's reaction
         "cd_mean": float(np.mean(cd_steps)),
         "mg_plan": [[p["nx"], p["ny"], p["mode"], *p["tile"]] for p in plan],
+        "smoother_sweeps": int(sim.solver.pressure_solver.smooth_sweeps),
+        "sweep_tuning_ms_per_step": {str(k): round(v["ms_per_step"], 3)
+                                     for k, v in sweeps.items()},
         "t": float(sim.fields.t),
         "_cd_steps": cd_steps,  # per-step Cd, compared across paths, not printed
     }
@@ -215,6 +232,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--paths", default="fp64,mixed")
     ap.add_argument("--tol", type=float, default=2e-11)
+    ap.add_argument("--no-tune-sweeps", action="store_true",
+                    help="keep V(1,1) (mg_sweeps=1) instead of the tuned sweeps")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
@@ -233,7 +252,8 @@ def main() -> int:
     results, kernels = {}, {}
     for prec in args.paths.split(","):
         _, results[prec], kernels[prec] = run_path(prec, args.steps, args.warmup, args.nx,
-                                                   args.ny, args.tol, traffic)
+                                                   args.ny, args.tol, traffic,
+                                                   not args.no_tune_sweeps)
     main_path = "mixed" if "mixed" in results else "fp64"
     r = results[main_path]
     # the dominant kernel of the reported path: largest share of its window

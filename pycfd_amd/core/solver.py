@@ -329,6 +329,51 @@ class ProjectionSolver:
         self.ctx.pressure_guess(to_device(arr), store=True)
         self._has_guess = True
 
+    # -- in-project tuning ------------------------------------------------ #
+    def tune_smoother_sweeps(self, fields: FlowField, dt: float, candidates=(1, 2, 3),
+                             steps: int = 3) -> dict[int, dict]:
+        """Pick the multigrid smoothing sweeps (V(s, s)) that make a step
+        fastest, by measurement: for each candidate the engine is configured
+        with it and `steps` whole steps are timed from the same `fields` and
+        the same warm-start pressure; the fastest stays configured.  The
+        reference exposes the choice as ``mg_sweeps`` and documents the
+        trade-off (more sweeps, fewer iterations, docs/usage.md:473-476);
+        this measures it on the device for the case at hand.  Only the
+        multigrid solvers have sweeps; returns ``{sweeps: {"ms_per_step",
+        "iterations"}}``, empty for the others."""
+        import time
+
+        import torch
+        from .pressure import MultigridSolverBase
+        ps = self.pressure_solver
+        if not isinstance(ps, MultigridSolverBase):
+            return {}
+        guess = self.pressure_guess()
+        graphs = self.use_graphs
+        self.use_graphs = False
+        table: dict[int, dict] = {}
+        for sw in candidates:
+            ps.smooth_sweeps = int(sw)
+            ps.configure()
+            times, its = [], 0
+            for rep in range(steps + 1):  # the first step warms the tuned plan up
+                self.set_pressure_guess(guess)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                self.step(fields, dt)
+                torch.cuda.synchronize()
+                if rep:
+                    times.append(time.perf_counter() - t0)
+                its = ps.last_iterations
+            table[int(sw)] = {"ms_per_step": 1e3 * min(times), "iterations": int(its)}
+        best = min(table, key=lambda k: table[k]["ms_per_step"])
+        ps.smooth_sweeps = best
+        ps.configure()
+        self.set_pressure_guess(guess)
+        self.use_graphs = graphs
+        log.info("smoother sweeps tuned: %s -> V(%d,%d)", table, best, best)
+        return table
+
     # ------------------------------------------------------------------ #
     def set_upwind_blend(self, blend: float) -> None:
         self._blend = float(np.clip(blend, 0.0, 1.0))
