@@ -128,6 +128,31 @@ class Level:
         return self.A.shape[0]
 
 
+def _distinct_rows(rows: np.ndarray):
+    """``np.unique(rows, axis=0, return_inverse=True)`` for float rows, by
+    a 64-bit hash of each row's bit pattern: one sort of n hashes instead of a
+    lexicographic sort of n rows.  Rows that share a hash are verified equal
+    bit for bit, so a collision cannot merge distinct rows (it falls back to
+    the exact sort).  Returns the distinct rows in ascending lexicographic
+    order, as np.unique does, and each row's index into them."""
+    n, w = rows.shape
+    bits = np.ascontiguousarray(rows).view(np.uint64)
+    # +0.0 and -0.0 compare equal, as np.unique treats them
+    bits = np.where(bits == np.uint64(1 << 63), np.uint64(0), bits)
+    mult = np.random.default_rng(12345).integers(1, 2 ** 63, size=w, dtype=np.uint64) | np.uint64(1)
+    with np.errstate(over="ignore"):
+        h = (bits * mult[None, :]).sum(axis=1, dtype=np.uint64)
+    uh, first, inv = np.unique(h, return_index=True, return_inverse=True)
+    reps = bits[first]
+    if not np.array_equal(bits, reps[inv]):  # a hash shared by two different rows
+        return np.unique(rows, axis=0, return_inverse=True)
+    table = rows[first]
+    order = np.lexsort(table.T[::-1])
+    rank = np.empty_like(order)
+    rank[order] = np.arange(order.size)
+    return table[order], rank[inv.ravel()]
+
+
 def to_class_stencil(A: sp.csr_matrix, shape: tuple[int, int], periodic: tuple[bool, bool],
                      active: np.ndarray, wide: bool = False):
     """CSR operator on a rectangular grid -> stencil dictionary.
@@ -160,7 +185,7 @@ def to_class_stencil(A: sp.csr_matrix, shape: tuple[int, int], periodic: tuple[b
     rows = np.zeros((nx * ny, S + 1))
     rows[r, inv] = v
     rows[:, S] = np.asarray(active, dtype=float).ravel()
-    table, cls = np.unique(rows, axis=0, return_inverse=True)
+    table, cls = _distinct_rows(rows)
     if table.shape[0] >= 2 ** 32:
         raise ValueError(f"{table.shape[0]} distinct operator rows exceed the class index range")
     wide = wide or table.shape[0] > 65535
