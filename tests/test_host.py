@@ -264,3 +264,25 @@ def test_stretched_periodic_axis_is_refused():
     cfg, _ = product_config("pchannel", stretch_x=1.05)
     with pytest.raises(ValueError, match="periodic and stretched"):
         ProjectionSolver(cfg)
+
+
+# ---------------------------------------------------------------- batched cavities
+def test_cavity_batch_configs_are_seeded_and_in_range():
+    """The batched-cavity workload: sizes drawn from the stated set, Re
+    log-uniform in [100, 1000], reproducible from the seed; each instance is
+    lid_driven_cavity.build's configuration (lid_driven_cavity.py:59-85)."""
+    from pycfd_amd.batch import CAVITY_SIZES, cavity_batch_configs
+    a = cavity_batch_configs(96, seed=7)
+    b = cavity_batch_configs(96, seed=7)
+    c = cavity_batch_configs(96, seed=8)
+    assert [(x.nx, x.re) for x in a] == [(x.nx, x.re) for x in b]
+    assert [(x.nx, x.re) for x in a] != [(x.nx, x.re) for x in c]
+    assert {x.nx for x in a} <= set(CAVITY_SIZES) and len({x.nx for x in a}) == len(CAVITY_SIZES)
+    res = np.array([x.re for x in a])
+    assert res.min() >= 100.0 and res.max() <= 1000.0
+    # log-uniform: the median sits near sqrt(100 * 1000)
+    assert 200.0 < np.median(res) < 500.0
+    x = a[0]
+    assert x.nx == x.ny and x.dt == 0.01 and x.cfl_max == 0.4 and x.lx == x.ly == 1.0
+    assert x.boundary_config["top"].kind.value == "moving_wall"
+    assert x.pressure_solver.value == "direct" and not x.use_les
