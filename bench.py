@@ -172,6 +172,55 @@ def run_path(precision: str, steps: int, warmup: int, nx: int, ny: int, tol: flo
     return sim, out, kernels
 
 
+def other_workloads(streams: int, batch_steps: int) -> dict:
+    """The other configured workloads, each a short timed window after a
+    warm-up (not the headline): the 128 x 128 CG cavity, Taylor-Green 1024^2
+    with multigrid V(2,2), and the 96 batched cavities advanced together."""
+    import torch
+
+    from pycfd_amd import cases
+    from pycfd_amd.batch import SimulationBatch, cavity_batch_configs
+    from pycfd_amd.physics.incompressible import Simulation
+    out = {}
+
+    def timed(sim, steps, warm):
+        sim.run(max_steps=warm)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim.run(max_steps=steps)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    cfg, _, _, _ = cases.cavity(nx=128, pressure_solver="cg", cfl_max=0.5, dt=0.01,
+                                poisson_tol=1e-10)
+    sim = Simulation(cfg)
+    w = timed(sim, 50, 5)
+    out["cavity128_cg"] = {"mlups": round(128 * 128 * 50 / w / 1e6, 3),
+                           "ms_per_step": round(1e3 * w / 50, 3),
+                           "cg_iterations_last_solve": sim.solver.pressure_solver.last_iterations}
+    log(f"[cavity 128 cg] {out['cavity128_cg']}")
+    cfg, _, u0, v0 = cases.taylor_green(n=1024, nu=0.01, dt=1e-3, pressure_solver="multigrid",
+                                        mg_sweeps=2)
+    sim = Simulation(cfg, u_init=u0, v_init=v0)
+    w = timed(sim, 40, 5)
+    out["taylor_green1024_mg_v22"] = {
+        "mlups": round(1024 * 1024 * 40 / w / 1e6, 2), "ms_per_step": round(1e3 * w / 40, 3),
+        "cycles_last_solve": sim.solver.pressure_solver.last_iterations}
+    log(f"[taylor-green 1024 V(2,2)] {out['taylor_green1024_mg_v22']}")
+    del sim
+    cfgs = cavity_batch_configs(96, seed=2024)
+    batch = SimulationBatch(cfgs, streams=streams)
+    batch.run(max_steps=2)
+    r = batch.run(max_steps=batch_steps)
+    out["batched_cavities96"] = {
+        "mlups": round(r.mlups, 2), "instances": len(cfgs), "streams": batch.n_streams,
+        "steps": batch_steps, "wall_s": round(r.wall_time, 3),
+        "setup_s": round(batch.setup_time, 2), "cells": int(sum(r.cells)),
+        "sizes": sorted({c.nx for c in cfgs}), "seed": 2024}
+    log(f"[batched cavities] {out['batched_cavities96']}")
+    return out
+
+
 def copy_calibration(nbytes: int = 1 << 26) -> None:
     """A plain device copy of known size (nbytes read + nbytes written), run
     once so that a counter profile of this script holds a reference kernel
@@ -234,6 +283,10 @@ def main() -> int:
     ap.add_argument("--tol", type=float, default=2e-11)
     ap.add_argument("--no-tune-sweeps", action="store_true",
                     help="keep V(1,1) (mg_sweeps=1) instead of the tuned sweeps")
+    ap.add_argument("--no-others", action="store_true",
+                    help="skip the other workloads (cavity 128 CG, Taylor-Green 1024, batch)")
+    ap.add_argument("--batch-streams", type=int, default=8)
+    ap.add_argument("--batch-steps", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
@@ -258,6 +311,7 @@ def main() -> int:
     r = results[main_path]
     # the dominant kernel of the reported path: largest share of its window
     dominant = dict(max(kernels[main_path], key=lambda d: d["share_of_window"]))
+    others = None if args.no_others else other_workloads(args.batch_streams, args.batch_steps)
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle) ...")
@@ -286,6 +340,7 @@ def main() -> int:
         "roofline": dominant,
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "other_workloads": others,
     }
     if "fp64" in results and "mixed" in results:
         line["mixed_speedup_vs_fp64"] = round(results["mixed"]["mlups"] / results["fp64"]["mlups"], 3)

@@ -93,3 +93,78 @@ def cylinder(nx: int = 2048, ny: int = 1024, re: float = 100.0,
     Xv, Yv = mesh.v_grid()
     blob = np.exp(-(((Xv - (cylinder_x + 1.0)) / 1.0) ** 2 + ((Yv - cy) / 0.5) ** 2))
     return cfg, body, 1.0, perturbation * blob
+
+
+def run_cylinder_record(steps: int, precision: str = "fp64", nx: int = 2048, ny: int = 1024,
+                        tol: float = 2e-11, progress_every: int = 1000, **overrides) -> dict:
+    """The cylinder case run as cylinder_flow.run measures it
+    (cylinder_flow.py:458-492, 590-600): Cd and Cl from the body reaction
+    every 10 steps, a wake probe of v at (lx/2, ly/2 + D/4), mean Cd and Cl
+    rms over t > 0.5 t_end (timeseries.py:255-259), and the Strouhal number
+    of the whole probe record (postprocess.py:116-142)."""
+    import sys
+    import time
+
+    from .analysis.postprocess import force_coefficients, late_time_mean, strouhal_number
+    from .physics.incompressible import Simulation
+    cfg, body, u0, v0 = cylinder(nx=nx, ny=ny, precision=precision, poisson_tol=tol,
+                                 **overrides)
+    sim = Simulation(cfg, obstacle=body, u_init=u0, v_init=v0)
+    probe = sim.add_probe(cfg.lx * 0.5, cfg.ly * 0.5 + 0.25, "wake")
+    hist = {"t": [], "cd": [], "cl": []}
+    t0 = time.perf_counter()
+
+    def record(fields, info):
+        cd, cl = force_coefficients(sim.solver.body_force_reaction, cfg.u_ref, cfg.l_ref)
+        hist["t"].append(fields.t)
+        hist["cd"].append(cd)
+        hist["cl"].append(cl)
+        if progress_every and fields.step % progress_every == 0:
+            print(f"[{precision}] step {fields.step} t {fields.t:.3f} Cd {cd:.5f} Cl {cl:+.5f} "
+                  f"{time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
+
+    res = sim.run(max_steps=steps, callback=record, callback_every=10)
+    wall = time.perf_counter() - t0
+    t = np.asarray(hist["t"])
+    cd, cl = np.asarray(hist["cd"]), np.asarray(hist["cl"])
+    settled = t > 0.5 * t[-1]
+    series = {k: np.asarray(v) for k, v in (("t", probe.t), ("v", probe.v))}
+    return {
+        "precision": precision, "steps": res.steps, "t_end": float(sim.fields.t),
+        "wall_s": round(wall, 2), "mlups": round(nx * ny * res.steps / wall / 1e6, 2),
+        "cd_mean": late_time_mean(t, cd), "cl_rms": float(np.sqrt(np.mean(cl[settled] ** 2))),
+        "strouhal": strouhal_number(series["t"], series["v"], cfg.l_ref, cfg.u_ref),
+        "max_div": float(max(res.history["max_div"])), "records": int(t.size),
+        "cd_series": cd.tolist(), "cl_series": cl.tolist(), "t_series": t.tolist(),
+    }
+
+
+if __name__ == "__main__":  # python -m pycfd_amd.cases cylinder-record --steps N ...
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description="long cylinder run: Cd, Cl rms and St per path")
+    ap.add_argument("case", choices=["cylinder-record"])
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--precision", default="fp64,mixed")
+    ap.add_argument("--nx", type=int, default=2048)
+    ap.add_argument("--ny", type=int, default=1024)
+    ap.add_argument("--tol", type=float, default=2e-11)
+    ap.add_argument("--out", default=None, help="write the full records (JSON) here")
+    a = ap.parse_args()
+    runs = {p: run_cylinder_record(a.steps, p, a.nx, a.ny, a.tol) for p in a.precision.split(",")}
+    summary = {p: {k: v for k, v in r.items() if not k.endswith("_series")} for p, r in runs.items()}
+    if "fp64" in runs and "mixed" in runs:
+        f, m = runs["fp64"], runs["mixed"]
+        summary["mixed_vs_fp64"] = {
+            "cd_mean_rel_diff": abs(m["cd_mean"] - f["cd_mean"]) / abs(f["cd_mean"]),
+            "cl_rms_rel_diff": abs(m["cl_rms"] - f["cl_rms"]) / abs(f["cl_rms"]),
+            "strouhal_diff": abs(m["strouhal"] - f["strouhal"]),
+            "cd_series_max_rel_diff": float(np.max(np.abs(np.asarray(m["cd_series"])
+                                                         - np.asarray(f["cd_series"]))
+                                                  / np.abs(np.asarray(f["cd_series"])))),
+        }
+    print(json.dumps(summary, indent=1))
+    if a.out:
+        with open(a.out, "w") as fh:
+            json.dump(runs, fh)
