@@ -96,6 +96,10 @@ typedef struct pcfd_case_desc {
   const double* dy_cells;
   double area_mean;
   double les_cs;
+  /* 1: the body is forced by a soft mask passed to the pcfd_soft_* calls;
+     `solid` then only fixes the Poisson operator's topology (no binary face
+     forcing).  0: the reference's binary forcing (solver.py:553-599). */
+  int32_t soft_ib;
 } pcfd_case_desc;
 
 typedef struct pcfd_solver_desc {
@@ -268,6 +272,42 @@ int pcfd_diagnostics_noalloc(pcfd_ctx* ctx, const double* u, const double* v, co
 /* body_force_reaction then body_force_reactions (solver.py:173-178):
  * out_host has 2 + 2*n_bodies doubles. */
 int pcfd_get_force(pcfd_ctx* ctx, double* out_host);
+
+/* --- differentiable step with a soft immersed-boundary mask -------------
+ * No counterpart in the reference (its mask is binary, solver.py:553-599).
+ * A context created with soft_ib = 1: the body is a per-cell field m in
+ * [0, 1]; a solvable face's weight is m_f = 1 - (1 - m_a)(1 - m_b) of its
+ * two cells (solver.py:62-79 with the OR made soft), the forcing u_f <-
+ * (1 - m_f) u_f after the predictor and after the projection, the reaction
+ * force sum m_f u_f A / dt.  m in {0, 1} gives the binary step exactly.
+ * The Poisson operator keeps the topology of the context's obstacle (the
+ * binary body); faces touching an obstacle cell keep weight 1, so the soft
+ * weights act on the faces between two fluid cells around the body.
+ * SSP-RK3, uniform mesh, constant viscosity, central advection, walls of
+ * kind inlet / no-slip / moving wall / symmetry / pressure outlet (p_ref 0).
+ * All arrays are device arrays; mf_u, mf_v, lam_*, g_mf_* have the ghosted
+ * shapes of u and v, m and g_theta (nx, ny). */
+/* Face weights from the cell mask. */
+int pcfd_soft_face_masks(pcfd_ctx* ctx, const double* m, double* mf_u, double* mf_v);
+/* Doubles of one step's tape (the states the adjoint needs). */
+int pcfd_soft_tape_size(pcfd_ctx* ctx, int64_t* doubles);
+/* One step (u0, v0) -> (u, v, p) with fixed dt; the tape (or NULL) records
+ * it; force_dev (2 doubles, or NULL) receives the last substep's force. */
+int pcfd_soft_step(pcfd_ctx* ctx, const double* u0, const double* v0, double dt,
+                   const double* mf_u, const double* mf_v, double* u, double* v, double* p,
+                   double* tape, double* force_dev);
+/* Transpose of a recorded step: lam_u, lam_v enter as the adjoint of its
+ * output (u, v) and leave as that of its input (u0, v0); dL_dFx weights the
+ * step's x-force; g_mf_u, g_mf_v accumulate the adjoint of the face
+ * weights.  The transposed projection solves with the configured solver
+ * from a zero start (the forward's warm start is kept). */
+int pcfd_soft_step_adjoint(pcfd_ctx* ctx, const double* tape, double dt, const double* mf_u,
+                           const double* mf_v, double dL_dFx, double* lam_u, double* lam_v,
+                           double* g_mf_u, double* g_mf_v);
+/* g_theta += (d m_f / d m)^T g_mf * m (1 - m): the face adjoints on the
+ * cells, through m = sigmoid(theta). */
+int pcfd_soft_cell_gradient(pcfd_ctx* ctx, const double* m, const double* g_mf_u,
+                            const double* g_mf_v, double* g_theta);
 
 /* --- Poisson building blocks (tests, autotuning) ----------------------- */
 /* y = A x with the fine operator (pressure.py:110-270); both (nx, ny). */

@@ -56,6 +56,7 @@ class CaseDesc(C.Structure):
         ("dx_cells", _dp), ("dy_cells", _dp),
         ("area_mean", C.c_double),
         ("les_cs", C.c_double),
+        ("soft_ib", C.c_int32),
     ]
 
 
@@ -113,6 +114,13 @@ _SIGNATURES = {
                   C.c_void_p, C.c_void_p, C.POINTER(SolveInfo)],
     "pcfd_reset_guess": [C.c_void_p],
     "pcfd_set_step_graphs": [C.c_void_p, C.c_int32],
+    "pcfd_soft_face_masks": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+    "pcfd_soft_tape_size": [C.c_void_p, C.POINTER(C.c_int64)],
+    "pcfd_soft_step": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p,
+                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+    "pcfd_soft_step_adjoint": [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p,
+                               C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+    "pcfd_soft_cell_gradient": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "pcfd_step_graph_stats": [C.c_void_p, C.POINTER(C.c_int32)],
     "pcfd_diagnostics": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                          C.c_void_p, _dp],

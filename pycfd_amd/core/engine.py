@@ -62,7 +62,7 @@ class DeviceContext:
                  body_id: np.ndarray | None = None, n_bodies: int = 0,
                  les_coeff: float | None = None, *, les_cs: float = 0.0,
                  dx_cells: np.ndarray | None = None, dy_cells: np.ndarray | None = None,
-                 area_mean: float | None = None) -> None:
+                 area_mean: float | None = None, soft_ib: bool = False) -> None:
         require_device()
         import torch
         self._lib = _lib.load()
@@ -107,6 +107,8 @@ class DeviceContext:
             d.dx_cells = wx.ctypes.data_as(C.POINTER(C.c_double))
             d.dy_cells = wy.ctypes.data_as(C.POINTER(C.c_double))
             d.area_mean = float(area_mean)
+        d.soft_ib = int(bool(soft_ib))
+        self.soft_ib = bool(soft_ib)
         self.stream = torch.cuda.current_stream()
         handle = C.c_void_p(0)
         _lib.check(self._lib.pcfd_ctx_create(C.byref(d), C.c_void_p(self.stream.cuda_stream),

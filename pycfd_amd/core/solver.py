@@ -45,7 +45,11 @@ class ProjectionSolver:
 
     def __init__(self, config: SimulationConfig, mesh: StructuredMesh | None = None,
                  obstacle: np.ndarray | None = None,
-                 bodies: Sequence[np.ndarray] | None = None) -> None:
+                 bodies: Sequence[np.ndarray] | None = None, *, soft_body: bool = False) -> None:
+        """``soft_body``: the obstacle mask only fixes the Poisson operator's
+        topology, and the body is forced by a soft mask through the
+        differentiable step (pycfd_amd.adjoint) instead of the binary face
+        forcing; ``step`` then does not force the body at all."""
         if config.solver_type is SolverType.SIMPLE:
             raise NotImplementedError(
                 "solver_type='simple' is reserved for a future SIMPLE "
@@ -109,7 +113,7 @@ class ProjectionSolver:
             les_cs=self.turbulence.cs if self.turbulence is not None else 0.0,
             dx_cells=self.mesh.dx_cells if stretched else None,
             dy_cells=self.mesh.dy_cells if stretched else None,
-            area_mean=self.mesh.area_mean if stretched else None)
+            area_mean=self.mesh.area_mean if stretched else None, soft_ib=soft_body)
         if self.turbulence is not None:
             self.turbulence.bind(self.ctx, self.nu)
         self.pressure_solver = make_pressure_solver(

@@ -179,6 +179,44 @@ __device__ void apply_wall(const BCParams& P, int w, bool predictor, double* u, 
   __syncthreads();
 }
 
+// Transpose of the four walls' velocity conditions (apply_wall in wall
+// order): each condition is a sequence of line assignments target = a *
+// source + constant, so its transpose runs them in reverse, adding a times
+// the target's adjoint into the source's and zeroing the target's (a
+// constant assignment, the prescribed normal face, only zeroes).  lu, lv
+// are adjoints in the ghosted layouts of u and v.  Walls without a periodic
+// axis and without a derived (OUTLET) normal face only.
+__device__ void apply_wall_adjoint(const BCParams& P, int w, bool predictor, double* lu,
+                                   double* lv) {
+  if (P.kind[w] == BC_PERIODIC) return;
+  const WallLines L = wall_lines(P, w, predictor, lu, lv);
+  const bool po = L.kind == BC_PRESSURE_OUTLET;
+  const double tb = (L.kind == BC_MOVING_WALL || L.kind == BC_NO_SLIP || L.kind == BC_INLET)
+                        ? -1.0 : 1.0;
+  // ng = 2 nf - nin (PO: ng = nf); tg = tb * tin (+ constant)
+  for (int k = threadIdx.x; k < L.ng.len; k += blockDim.x) {
+    const double g = L.ng[k];
+    L.ng[k] = 0.0;
+    L.nf[k] += po ? g : 2.0 * g;
+    if (!po) L.nin[k] -= g;
+  }
+  for (int k = threadIdx.x; k < L.tg.len; k += blockDim.x) {
+    const double g = L.tg[k];
+    L.tg[k] = 0.0;
+    L.tin[k] += tb * g;
+  }
+  __syncthreads();
+  if (L.face) {
+    for (int k = threadIdx.x; k < L.nf.len; k += blockDim.x) L.nf[k] = 0.0;
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_walls_adjoint(BCParams P, int predictor, double* lu, double* lv) {
+  for (int w = 3; w >= 0; --w) apply_wall_adjoint(P, w, predictor != 0, lu, lv);
+}
+
 // BoundaryManager._wrap_velocity (boundary.py:388-410).
 __device__ void wrap_velocity(const BCParams& P, double* u, double* v) {
   const int nx = P.nx, ny = P.ny;
@@ -595,6 +633,13 @@ int launch_boundary_pass(hipStream_t s, const BCParams& P, const BCOps& ops,
     }
   }
   flush();
+  PCFD_LAUNCH_CHECK();
+  return kOk;
+}
+
+int launch_walls_adjoint(hipStream_t s, const BCParams& P, bool predictor, double* lu,
+                         double* lv) {
+  hipLaunchKernelGGL(k_walls_adjoint, dim3(1), dim3(kBlock), 0, s, P, predictor ? 1 : 0, lu, lv);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }

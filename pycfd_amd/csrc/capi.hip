@@ -107,6 +107,8 @@ struct pcfd_ctx {
   double* guess_backup = nullptr;
   double* host_out = nullptr;  // pinned
   double* scratch = nullptr;   // this device's copy of the no-allocation scratch
+  double* soft_work = nullptr; // (nx*ny) cells: the transposed projection's right-hand side
+  bool soft_ib = false;
   PoissonEngine engine;
   bool solver_set = false;
   // Captured steps (pcfd_set_step_graphs): one graph per buffer set, the
@@ -155,6 +157,7 @@ struct pcfd_ctx {
     if (host_out) hipHostFree(host_out);
     for (StepGraph& g : graphs) g.release();
     hipFree(scal);
+    hipFree(soft_work);
     if (cap_stream) hipStreamDestroy(cap_stream);
   }
 };
@@ -625,11 +628,14 @@ int pcfd_ctx_create(const pcfd_case_desc* d, void* hip_stream, pcfd_ctx** out) {
       if (d->body_id) body[k] = d->body_id[k];
     }
   }
+  // a soft body: the solid set shapes the Poisson operator only
+  c->soft_ib = d->soft_ib != 0;
+  const bool binary_faces = has_obstacle && !c->soft_ib;
   auto sol = [&](int i, int j) -> bool { return solid[(size_t)i * ny + j] != 0; };
   auto bod = [&](int i, int j) -> int8_t { return body[(size_t)i * ny + j]; };
   std::vector<int> uf, vf;
   std::vector<int8_t> ufb, vfb;
-  if (has_obstacle) {
+  if (binary_faces) {
     for (int m = R.u_lo; m < R.u_hi; ++m)
       for (int j = 0; j < ny; ++j) {
         int a = m - 2, b = m - 1;
@@ -762,7 +768,7 @@ int pcfd_ctx_create(const pcfd_case_desc* d, void* hip_stream, pcfd_ctx** out) {
   B.dy = d->dy;
   B.periodic_x = px;
   B.periodic_y = py;
-  B.has_obstacle = has_obstacle;
+  B.has_obstacle = binary_faces;
   B.n_uf = (int)uf.size();
   B.n_vf = (int)vf.size();
   B.uf = c->uf;
@@ -843,6 +849,7 @@ int pcfd_ctx_create(const pcfd_case_desc* d, void* hip_stream, pcfd_ctx** out) {
   PCFD_HIP_CHECK(hipMalloc(&c->diag_out, 16 * sizeof(double)));
   PCFD_HIP_CHECK(hipMalloc(&c->guess_backup, (size_t)nx * ny * sizeof(double)));
   PCFD_HIP_CHECK(hipHostMalloc(&c->host_out, 64 * sizeof(double), hipHostMallocDefault));
+  if (c->soft_ib) PCFD_HIP_CHECK(hipMalloc(&c->soft_work, (size_t)nx * ny * sizeof(double)));
   if ((rc = scratch_for_current_device(&c->scratch))) return rc;
   PCFD_HIP_CHECK(hipStreamSynchronize(c->stream));
   *out = c.release();
@@ -1112,6 +1119,74 @@ int pcfd_step(pcfd_ctx* c, const double* u0, const double* v0, double dt, double
   if (want) fill_info(info, s);
   return kOk;
 }
+
+}  // extern "C"
+
+// What the differentiable step (adjoint.hip) needs of a context.
+struct SoftView {
+  hipStream_t stream;
+  int nx, ny, ldu, ldv, ldp;
+  double dx, dy, nu;
+  UpdRange upd;
+  const BCParams* bc;
+  const MomParams* mom;
+  const Metrics* g;
+  PoissonEngine* engine;
+  double* force;
+  double* scratch_a[2];
+  double* scratch_b[2];
+  double* work;
+  const uint8_t* flags;
+  double* guess_backup;
+  int kind[4];
+  int scheme;
+  const uint8_t* solid;
+};
+
+int pcfd_soft_view(pcfd_ctx* c, SoftView* out) {
+  if (!c->soft_ib) {
+    set_last_error("the context was not created with soft_ib = 1");
+    return kErrBadArg;
+  }
+  if (!c->solver_set) {
+    set_last_error("no pressure solver configured (pcfd_set_pressure_solver)");
+    return kErrBadArg;
+  }
+  for (int w = 0; w < 4; ++w)
+    if (c->kind[w] == PCFD_BC_PRESSURE_OUTLET && c->bc.pref[w] != 0.0) {
+      set_last_error("the differentiable step needs p_ref = 0 on pressure outlets");
+      return kErrUnsupported;
+    }
+  SoftView& V = *out;
+  V.stream = c->stream;
+  V.nx = c->nx;
+  V.ny = c->ny;
+  V.ldu = c->ldu;
+  V.ldv = c->ldv;
+  V.ldp = c->ldp;
+  V.dx = c->dx;
+  V.dy = c->dy;
+  V.nu = c->nu;
+  V.upd = c->upd;
+  V.bc = &c->bc;
+  V.mom = &c->mom;
+  V.g = &c->g;
+  V.engine = &c->engine;
+  V.force = c->force;
+  V.scratch_a[0] = c->stage_a[0];
+  V.scratch_a[1] = c->stage_a[1];
+  V.scratch_b[0] = c->stage_b[0];
+  V.scratch_b[1] = c->stage_b[1];
+  V.work = c->soft_work;
+  V.flags = c->flags;
+  V.guess_backup = c->guess_backup;
+  for (int w = 0; w < 4; ++w) V.kind[w] = c->kind[w];
+  V.scheme = c->scheme;
+  V.solid = c->solid;
+  return kOk;
+}
+
+extern "C" {
 
 int pcfd_set_step_graphs(pcfd_ctx* c, int32_t enable) {
   if (int rc = check_ctx(c)) return rc;

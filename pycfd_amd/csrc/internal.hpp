@@ -162,6 +162,10 @@ int launch_diagnostics(hipStream_t s, const double* u, const double* v, const do
 int launch_boundary_pass(hipStream_t s, const BCParams& P, const BCOps& ops,
                          const StepScalars& S, double* u, double* v);
 int launch_pressure_ghosts(hipStream_t s, const BCParams& P, double* p, int ldp);
+// Transpose of apply_velocity(predictor) on adjoint fields lu, lv (walls
+// with no periodic axis and no OUTLET).
+int launch_walls_adjoint(hipStream_t s, const BCParams& P, bool predictor, double* lu,
+                         double* lv);
 // dst[0] = dt, dst[1] = blend: the first node of a captured step.
 int launch_set_step_scalars(hipStream_t s, double* dst, double dt, double blend);
 
