@@ -311,6 +311,11 @@ def main() -> int:
     r = results[main_path]
     # the dominant kernel of the reported path: largest share of its window
     dominant = dict(max(kernels[main_path], key=lambda d: d["share_of_window"]))
+    # traffic is not measured in this run: it is read from the committed
+    # counter summary (two rocprofv3 --pmc passes, scaled by the copy kernel)
+    dominant["traffic_source"] = (f"{TRAFFIC_FILE.relative_to(ROOT)} (rocprofv3 --pmc "
+                                  "FETCH_SIZE / WRITE_SIZE per dispatch, scaled by a copy "
+                                  "kernel of known size)" if dominant.get("traffic") else None)
     others = None if args.no_others else other_workloads(args.batch_streams, args.batch_steps)
     cpu = None
     if not args.no_cpu:
