@@ -4,7 +4,9 @@ Contexts are closed by the library's exit hook (registered after torch has
 initialised the device), and a context's finaliser does nothing while the
 interpreter is finalising, so no device call runs after the HIP runtime or a
 profiler's tool library has shut down.  The child program leaves two contexts
-open, one with the cooperative chain mode and the live kernel clock.
+open, one with the live kernel clock on and the cooperative chain mode asked
+for (kept when the device accepts the launch; it may refuse one while this test
+runner holds the device).
 """
 
 from __future__ import annotations
