@@ -1465,10 +1465,6 @@ static int upload_level(const MGLevelHost& h, const MGLevelHost* next, bool fine
         L.hot_n = hot_n;
         L.hot_e0 = rp[hot];
       }
-      int longest = 0;
-      for (size_t c = 0; c < nc; ++c) longest = std::max(longest, rp[c + 1] - rp[c]);
-      L.nh_lanes = 1;
-      while (L.nh_lanes < 16 && L.nh_lanes * nh_per<T>() < longest) L.nh_lanes *= 2;
     }
     const size_t nnz = std::max<size_t>(val.size(), 1);
     if (wide) {
@@ -1714,12 +1710,11 @@ int PoissonEngine::tune_level(std::vector<LevelBuf>& lv, int l, SolveState* tst)
                                : (L.box_radius > 0 && nu == 1));
   if (!fine && L.hdi.empty() && !stream_ok) return kOk;
   // LDS bytes per region cell: three vectors + flags (level 0); three
-  // vectors, 1/diag, entry range, active flag and a deferred-row slot
-  // (stored levels; plus two counters and alignment)
-  const unsigned per_cell = fine ? 3u * sizeof(T) + 1u : 4u * sizeof(T) + 13u;
+  // vectors, 1/diag, entry range and active flag (stored levels)
+  const unsigned per_cell = fine ? 3u * sizeof(T) + 1u : 4u * sizeof(T) + 9u;
   const unsigned fixed =
       fine ? 256u * sizeof(T)
-           : (L.hot >= 0 ? (unsigned)L.hot_n * (unsigned)(sizeof(T) + 4) : 0u) + 28u;
+           : (L.hot >= 0 ? (unsigned)L.hot_n * (unsigned)(sizeof(T) + 4) : 0u) + 16u;
   const int ftx = fine ? cfg_.tile_tx : 0, fty = fine ? cfg_.tile_ty : 0;
   std::vector<FusedTile> cands;
   if (fine || !L.hdi.empty())
@@ -2393,7 +2388,6 @@ static ClassRows<T> class_rows(const LevelBuf& L, int px, int py) {
   D.ract = L.ract;
   D.rr = L.rr;
   D.hot_e0 = L.hot_e0;
-  D.nh_lanes = L.nh_lanes;
   return D;
 }
 

@@ -189,7 +189,6 @@ struct LevelBuf {
   std::vector<int8_t> hdi, hdj;  // host copy of the entry offsets (tile planning)
   int hot = -1, hot_n = 0;       // most frequent class and its entry count
   int hot_e0 = 0;                // its first entry
-  int nh_lanes = 1;              // fused kernels: lanes per row not staged in LDS
   // Register-resident row passes: lanes per row (0: the looping passes run
   // the level) and the entry range of the level's most frequent row.
   int row_lanes = 0;

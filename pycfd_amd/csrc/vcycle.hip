@@ -34,19 +34,6 @@ __device__ __forceinline__ bool resolve(int gi, int gj, int nx, int ny, int px, 
   return i >= 0 && i < nx && j >= 0 && j < ny;
 }
 
-// The same for a load issued whether or not the cell exists: `k` is always a
-// valid cell index (the cell clamped into the grid), and the result says
-// whether the loaded values count.  No load then sits in a branch, so a
-// batch of cells keeps all its loads in flight together.
-__device__ __forceinline__ bool resolve_k(int gi, int gj, int nx, int ny, int px, int py,
-                                          int& i, int& j, int& k) {
-  const bool ok = resolve(gi, gj, nx, ny, px, py, i, j);
-  i = min(max(i, 0), nx - 1);
-  j = min(max(j, 0), ny - 1);
-  k = i * ny + j;
-  return ok;
-}
-
 struct TileOrigin {
   int t;         // linear tile index (partials slot)
   int ti, tj;    // tile coordinates
@@ -95,17 +82,11 @@ __device__ __forceinline__ void for_ring(const Ring& g, Fn&& fn) {
 // Visits every cell of a whole region (ring 0) in batches of KB cells per
 // thread: `ld(a, c)` issues a cell's global loads and returns them, and all
 // KB loads of a batch are in flight before `st(a, c, idx, v)` consumes them
-// (one load round trip per batch instead of one per cell).  `first()` runs in
-// every thread once the first batch's loads are issued and before any of
-// them is consumed, so loads issued earlier (a solver flag, staged rows) share
-// that round trip; it returns false to stop before any store (uniformly
-// across the workgroup), and the visit returns what it returned.
-template <int KB, class Ld, class St, class First>
-__device__ __forceinline__ bool for_region_batched(int rx, int ry, Ld&& ld, St&& st,
-                                                   First&& first) {
+// (one load round trip per batch instead of one per cell).
+template <int KB, class Ld, class St>
+__device__ __forceinline__ void for_region_batched(int rx, int ry, Ld&& ld, St&& st) {
   using V = decltype(ld(0, 0));
   const int n = rx * ry;
-  if (n <= 0) return first();
   const float inv = 1.0f / (float)ry;
   auto split = [&](int q, int& a, int& c) {
     a = (int)(((float)q + 0.5f) * inv);
@@ -118,16 +99,17 @@ __device__ __forceinline__ bool for_region_batched(int rx, int ry, Ld&& ld, St&&
       c -= ry;
     }
   };
-  for (int b0 = 0; b0 < n; b0 += KB * kBlock) {
-    const int base = b0 + (int)threadIdx.x;
+  for (int base = threadIdx.x; base < n; base += KB * kBlock) {
     V vals[KB];
 #pragma unroll
-    for (int u = 0; u < KB; ++u) {  // every slot loads (a clamped cell past the end)
-      int a, c;
-      split(min(base + u * kBlock, n - 1), a, c);
-      vals[u] = ld(a, c);
+    for (int u = 0; u < KB; ++u) {
+      const int q = base + u * kBlock;
+      if (q < n) {
+        int a, c;
+        split(q, a, c);
+        vals[u] = ld(a, c);
+      }
     }
-    if (b0 == 0 && !first()) return false;
 #pragma unroll
     for (int u = 0; u < KB; ++u) {
       const int q = base + u * kBlock;
@@ -138,12 +120,6 @@ __device__ __forceinline__ bool for_region_batched(int rx, int ry, Ld&& ld, St&&
       }
     }
   }
-  return true;
-}
-
-template <int KB, class Ld, class St>
-__device__ __forceinline__ void for_region_batched(int rx, int ry, Ld&& ld, St&& st) {
-  for_region_batched<KB>(rx, ry, ld, st, []() { return true; });
 }
 
 constexpr int kLoadBatch = 4;
@@ -819,6 +795,28 @@ k_fine_up_stream(FineOp A, int strips, int L, T w, const T* __restrict__ x_pre,
 // ---------------------------------------------------------------------------
 // Levels >= 1: stored Galerkin rows (class dictionary).
 // ---------------------------------------------------------------------------
+// Row c of the level applied at region cell idx; four entries per trip with
+// their loads issued together, summed in entry order.
+// Row (entry range rr) at region cell idx from the global dictionary, four
+// partial sums to keep the multiply-add chain short.
+template <typename T>
+__device__ __forceinline__ T class_row(const ClassRows<T>& L, int2 rr, const T* X, int idx) {
+  T s0 = (T)0, s1 = (T)0, s2 = (T)0, s3 = (T)0;
+  int e = rr.x;
+  for (; e + 3 < rr.y; e += 4) {
+    const int o0 = L.ent_off[e], o1 = L.ent_off[e + 1], o2 = L.ent_off[e + 2],
+              o3 = L.ent_off[e + 3];
+    const T a0 = L.ent_val[e], a1 = L.ent_val[e + 1], a2 = L.ent_val[e + 2],
+            a3 = L.ent_val[e + 3];
+    s0 += a0 * X[idx + o0];
+    s1 += a1 * X[idx + o1];
+    s2 += a2 * X[idx + o2];
+    s3 += a3 * X[idx + o3];
+  }
+  for (; e < rr.y; ++e) s0 += L.ent_val[e] * X[idx + L.ent_off[e]];
+  return (s0 + s1) + (s2 + s3);
+}
+
 // LDS of a stored-level tile: the hot row's coefficients and offsets, three
 // vectors, per cell 1/diag, entry range and active flag.
 template <typename T>
@@ -827,8 +825,6 @@ struct ClassLds {
   int2* RR;
   int* HO;
   uint8_t* A;
-  int* NH;   // a ring pass's deferred (not hot) cells, ring-linear indices
-  int* CNT;  // two counters of deferred cells, used by alternate passes
   __device__ ClassLds(unsigned char* smem, int rn, int hot_n) {
     HV = (T*)smem;
     P = HV + hot_n;
@@ -840,153 +836,87 @@ struct ClassLds {
     RR = (int2*)(smem + off);  // (-1, -1) outside the grid
     HO = (int*)(RR + rn);
     A = (uint8_t*)(HO + hot_n);
-    off = (size_t)((unsigned char*)(A + rn) - smem);
-    off = (off + 3) & ~(size_t)3;
-    NH = (int*)(smem + off);
-    CNT = NH + rn;
-  }
-  // Before the first ring pass (ordered by the barrier that follows).
-  __device__ void clear_counters() const {
-    if (threadIdx.x < 2) CNT[threadIdx.x] = 0;
   }
 };
 
-// Deferred rows of a ring pass (ring-linear indices NH[0, nn)): G lanes per
-// row, each loading its nh_per<T>() entries (stride G) in one batch, clamped into
-// the row (a slot past the row adds 0 * X, its X read the clamped entry's),
-// then the lanes' sums reduced by shuffles.
-template <int G, typename T, class Cell, class Out>
-__device__ __forceinline__ void deferred_rows(const ClassRows<T>& L, const ClassLds<T>& M, int nn,
-                                              int ry, const T* X, const Cell& cell, Out& out) {
-  constexpr int kNhPer = nh_per<T>();
-  const int lane = (int)threadIdx.x % G;
-  for (int b0 = 0; b0 < nn; b0 += kBlock / G) {
-    const int r = b0 + (int)threadIdx.x / G;
-    int ra, rc;
-    cell(M.NH[min(r, nn - 1)], ra, rc);
-    const int idx = ra * ry + rc;
-    const int2 rr = M.RR[idx];
-    T acc = (T)0;
-    for (int e0 = rr.x + lane; e0 < rr.y; e0 += G * kNhPer) {
-      int off[kNhPer];
-      T val[kNhPer];
-#pragma unroll
-      for (int u = 0; u < kNhPer; ++u) {
-        const int ec = min(e0 + u * G, rr.y - 1);
-        off[u] = L.ent_off[ec];
-        val[u] = L.ent_val[ec];
-      }
-#pragma unroll
-      for (int u = 0; u < kNhPer; ++u)
-        acc += ((e0 + u * G < rr.y) ? val[u] : (T)0) * X[idx + off[u]];
-    }
-#pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, G);
-    if (lane == 0 && r < nn) out(ra, rc, idx, rr, acc);
-  }
-}
-
 // (A X) at every cell of ring g, handed to out(a, c, idx, rr, ax) (rr.x < 0:
 // a cell outside the grid, ax = 0).  A thread takes the cells q = tid,
-// tid + 256, ... and runs the hot row on each: one broadcast LDS read of each
-// hot coefficient and offset serves the wavefront, and every X read stays
-// inside the region (the ring lies at least one stencil radius inside it), so
-// the hot sums run unpredicated and are used only by hot rows.  Rows of
-// another class (walls, the body) are deferred: listed in LDS and, after a
-// barrier, evaluated by deferred_rows with the level's lanes per row -- a
-// dictionary round trip per kBlock / nh_lanes rows, where one thread walking
-// its row costs a trip per few entries, and the workgroups holding such rows
-// set the launch's time.  `call` counts the ring passes of the kernel; pass k
-// lists into counter k % 2 and clears the other one for pass k + 1 (every
-// caller has a barrier after each pass).
+// tid + 256, ... in chunks of kRingRows and runs the hot row on all of a
+// chunk in lockstep: one broadcast LDS read of each hot coefficient and
+// offset serves every row, and the rows' X reads are independent.  Rows of
+// another class then take the global dictionary.  Every X read stays inside
+// the region (the ring lies at least one stencil radius inside it), so the
+// hot sums run unpredicated and are used only by hot rows.
+constexpr int kRingRows = 1;
+
 template <typename T, class Out>
 __device__ __forceinline__ void ring_rows(const ClassRows<T>& L, const ClassLds<T>& M,
-                                          const Ring& g, int ry, const T* X, int& call,
-                                          Out&& out) {
-  const int slot = call & 1;
-  ++call;
-  if (threadIdx.x == 0) M.CNT[slot ^ 1] = 0;
+                                          const Ring& g, int ry, const T* X, Out&& out) {
   const int w = g.c1 - g.c0;
   const int n = (g.a1 - g.a0) * w;
-  if (n <= 0) return;  // uniform
+  if (n <= 0) return;
   const float inv = 1.0f / (float)w;
-  auto cell = [&](int q, int& a, int& c) {
-    a = (int)(((float)q + 0.5f) * inv);
-    c = q - a * w;
-    if (c < 0) {
-      --a;
-      c += w;
-    } else if (c >= w) {
-      ++a;
-      c -= w;
-    }
-    a += g.a0;
-    c += g.c0;
-  };
   const int hn = L.hot < 0 ? 0 : L.hot_n;
-  for (int q0 = 0; q0 < n; q0 += kBlock) {
-    const int qt = q0 + (int)threadIdx.x;
-    int ra, rc;
-    cell(min(qt, n - 1), ra, rc);
-    const int idx = ra * ry + rc;
-    const int2 rr = M.RR[idx];
-    T s0 = (T)0, s1 = (T)0, s2 = (T)0, s3 = (T)0;
+  for (int q0 = 0; q0 < n; q0 += kRingRows * kBlock) {
+    int ra[kRingRows], rc[kRingRows], idx[kRingRows];
+    int2 rr[kRingRows];
+    T s[kRingRows][4];
+#pragma unroll
+    for (int r = 0; r < kRingRows; ++r) {
+      const int q = min(q0 + r * kBlock + (int)threadIdx.x, n - 1);
+      int a = (int)(((float)q + 0.5f) * inv);
+      int c = q - a * w;
+      if (c < 0) {
+        --a;
+        c += w;
+      } else if (c >= w) {
+        ++a;
+        c -= w;
+      }
+      ra[r] = g.a0 + a;
+      rc[r] = g.c0 + c;
+      idx[r] = ra[r] * ry + rc[r];
+      rr[r] = M.RR[idx[r]];
+#pragma unroll
+      for (int p = 0; p < 4; ++p) s[r][p] = (T)0;
+    }
     int e = 0;
     for (; e + 3 < hn; e += 4) {
       const int o0 = M.HO[e], o1 = M.HO[e + 1], o2 = M.HO[e + 2], o3 = M.HO[e + 3];
       const T v0 = M.HV[e], v1 = M.HV[e + 1], v2 = M.HV[e + 2], v3 = M.HV[e + 3];
-      s0 += v0 * X[idx + o0];
-      s1 += v1 * X[idx + o1];
-      s2 += v2 * X[idx + o2];
-      s3 += v3 * X[idx + o3];
+#pragma unroll
+      for (int r = 0; r < kRingRows; ++r) {
+        s[r][0] += v0 * X[idx[r] + o0];
+        s[r][1] += v1 * X[idx[r] + o1];
+        s[r][2] += v2 * X[idx[r] + o2];
+        s[r][3] += v3 * X[idx[r] + o3];
+      }
     }
-    for (; e < hn; ++e) s0 += M.HV[e] * X[idx + M.HO[e]];
-    if (qt < n) {
-      if (rr.x < 0) out(ra, rc, idx, rr, (T)0);
-      else if (hn > 0 && rr.x == L.hot_e0) out(ra, rc, idx, rr, (s0 + s1) + (s2 + s3));
-      else M.NH[atomicAdd(&M.CNT[slot], 1)] = qt;
+    for (; e < hn; ++e) {
+      const int o = M.HO[e];
+      const T v = M.HV[e];
+#pragma unroll
+      for (int r = 0; r < kRingRows; ++r) s[r][0] += v * X[idx[r] + o];
     }
-  }
-  __syncthreads();
-  const int nn = M.CNT[slot];  // uniform
-  if (nn == 0) return;
-  switch (L.nh_lanes) {
-    case 1: deferred_rows<1>(L, M, nn, ry, X, cell, out); break;
-    case 2: deferred_rows<2>(L, M, nn, ry, X, cell, out); break;
-    case 4: deferred_rows<4>(L, M, nn, ry, X, cell, out); break;
-    case 8: deferred_rows<8>(L, M, nn, ry, X, cell, out); break;
-    default: deferred_rows<16>(L, M, nn, ry, X, cell, out); break;
+#pragma unroll
+    for (int r = 0; r < kRingRows; ++r) {
+      if (q0 + r * kBlock + (int)threadIdx.x >= n) break;
+      T ax = (T)0;
+      if (rr[r].x >= 0)
+        ax = (hn > 0 && rr[r].x == L.hot_e0) ? (s[r][0] + s[r][1]) + (s[r][2] + s[r][3])
+                                             : class_row(L, rr[r], X, idx[r]);
+      out(ra[r], rc[r], idx[r], rr[r], ax);
+    }
   }
 }
 
-// The level's hot row, one entry per thread (kHotMax <= kBlock), loaded from
-// its known first entry once the region's loads are in flight, so both share
-// one round trip.
-static_assert(kHotMax <= kBlock, "one hot-row entry per thread");
-
 template <typename T>
-struct HotEntry {
-  T v = (T)0;
-  int o = 0;
-  bool on = false;
-};
-
-template <typename T>
-__device__ __forceinline__ HotEntry<T> load_hot_entry(const ClassRows<T>& L) {
-  HotEntry<T> h;
-  h.on = L.hot >= 0 && (int)threadIdx.x < L.hot_n;
-  if (h.on) {
-    h.v = L.ent_val[L.hot_e0 + threadIdx.x];
-    h.o = L.ent_off[L.hot_e0 + threadIdx.x];
-  }
-  return h;
-}
-
-template <typename T>
-__device__ __forceinline__ void store_hot_entry(const ClassLds<T>& M, const HotEntry<T>& h) {
-  if (h.on) {
-    M.HV[threadIdx.x] = h.v;
-    M.HO[threadIdx.x] = h.o;
+__device__ __forceinline__ void stage_hot_row(const ClassRows<T>& L, const ClassLds<T>& M) {
+  if (L.hot < 0) return;
+  const int e0 = L.row_ptr[L.hot];
+  for (int e = threadIdx.x; e < L.hot_n; e += kBlock) {
+    M.HV[e] = L.ent_val[e0 + e];
+    M.HO[e] = L.ent_off[e0 + e];
   }
 }
 
@@ -995,9 +925,7 @@ __global__ void __launch_bounds__(kBlock)
 k_class_down(ClassRows<T> L, FusedTile ft, int rxs, int rys, T w, int nu, const T* __restrict__ b,
              T* __restrict__ x_out, T* __restrict__ bc, int fx, int fy, int cnx, int cny,
              const SolveState* __restrict__ st) {
-  // The done flag, the region and the hot row are loaded together (one
-  // memory round trip); the flag only gates what follows.
-  const int done = st->done;
+  if (st->done) return;
   extern __shared__ __align__(16) unsigned char smem[];
   const int ry = ft.ry;
   ClassLds<T> M(smem, ft.rx * ry, L.hot < 0 ? 0 : L.hot_n);
@@ -1006,6 +934,7 @@ k_class_down(ClassRows<T> L, FusedTile ft, int rxs, int rys, T w, int nu, const 
   T* Y = M.S;
   int2* RR = M.RR;
   const TileOrigin o = tile_origin(ft);
+  stage_hot_row(L, M);
 
   // Load b and the row classes, and start from zero: x = w D^-1 b.
   T* D = M.D;
@@ -1016,37 +945,34 @@ k_class_down(ClassRows<T> L, FusedTile ft, int rxs, int rys, T w, int nu, const 
     uint8_t act;
     T bv, dv;
   };
-  const bool go = for_region_batched<kLoadBatch>(
+  for_region_batched<kLoadBatch>(
       ft.rx, ry,
-      [&](int a, int c) {  // raw values: masked only when stored, after the flag
+      [&](int a, int c) {
         ClsLd v;
-        int i, j, k;
-        v.ok = resolve_k(o.gi0 + a, o.gj0 + c, L.nx, L.ny, L.px, L.py, i, j, k);
-        v.rr = L.rr[k];
-        v.bv = b[k];
-        v.dv = L.rdinv[k];
-        v.act = L.ract[k];
+        int i, j;
+        v.ok = resolve(o.gi0 + a, o.gj0 + c, L.nx, L.ny, L.px, L.py, i, j);
+        v.rr = make_int2(-1, -1);
+        v.act = 0;
+        v.bv = v.dv = (T)0;
+        if (v.ok) {
+          const int k = i * L.ny + j;
+          v.rr = L.rr[k];
+          v.bv = b[k];
+          v.dv = L.rdinv[k];
+          v.act = L.ract[k];
+        }
         return v;
       },
       [&](int a, int c, int idx, const ClsLd& v) {
-        const T bv = v.ok ? v.bv : (T)0, dv = v.ok ? v.dv : (T)0;
-        RR[idx] = v.ok ? v.rr : make_int2(-1, -1);
-        B[idx] = bv;
-        D[idx] = dv;
-        A[idx] = v.ok ? v.act : (uint8_t)0;
-        X[idx] = w * dv * bv;
-      },
-      [&]() {  // the hot row's loads join the region's in flight
-        if (done) return false;
-        store_hot_entry(M, load_hot_entry(L));
-        M.clear_counters();
-        return true;
+        RR[idx] = v.rr;
+        B[idx] = v.bv;
+        D[idx] = v.dv;
+        A[idx] = v.act;
+        X[idx] = w * v.dv * v.bv;
       });
-  if (!go) return;
-  int call = 0;  // ring passes so far (deferred-row counters)
   __syncthreads();
   for (int s = 1; s < nu; ++s) {
-    ring_rows(L, M, ring(ft, nu + 1 - s, rxs, rys), ry, X, call,
+    ring_rows(L, M, ring(ft, nu + 1 - s, rxs, rys), ry, X,
               [&](int, int, int idx, int2 rr, T ax) {
                 Y[idx] = (rr.x < 0) ? (T)0 : X[idx] + w * D[idx] * (B[idx] - ax);
               });
@@ -1056,7 +982,7 @@ k_class_down(ClassRows<T> L, FusedTile ft, int rxs, int rys, T w, int nu, const 
     Y = t;
   }
   // Residual on ring 1 into Y.
-  ring_rows(L, M, ring(ft, 1, rxs, rys), ry, X, call, [&](int, int, int idx, int2 rr, T ax) {
+  ring_rows(L, M, ring(ft, 1, rxs, rys), ry, X, [&](int, int, int idx, int2 rr, T ax) {
     Y[idx] = (rr.x < 0) ? (T)0 : B[idx] - ax;
   });
   __syncthreads();
@@ -1072,7 +998,7 @@ k_class_down(ClassRows<T> L, FusedTile ft, int rxs, int rys, T w, int nu, const 
   __syncthreads();
   // (r - w A t) on the tile's rows into B (free now), then each coarse cell
   // sums its block's rows in x-major order.
-  ring_rows(L, M, ring(ft, 0, rxs, rys), ry, X, call, [&](int, int, int idx, int2, T ax) {
+  ring_rows(L, M, ring(ft, 0, rxs, rys), ry, X, [&](int, int, int idx, int2, T ax) {
     B[idx] = A[idx] ? Y[idx] - w * ax : (T)0;
   });
   __syncthreads();
@@ -1100,7 +1026,7 @@ __global__ void __launch_bounds__(kBlock)
 k_class_up(ClassRows<T> L, FusedTile ft, int rxs, int rys, T w, int nu, const T* __restrict__ x_pre,
            const T* __restrict__ b, const T* __restrict__ xc, int fx, int fy, int cny,
            T* __restrict__ x_out, const SolveState* __restrict__ st) {
-  const int done = st->done;  // gates the stores; loaded with the region
+  if (st->done) return;
   extern __shared__ __align__(16) unsigned char smem[];
   const int ry = ft.ry;
   ClassLds<T> M(smem, ft.rx * ry, L.hot < 0 ? 0 : L.hot_n);
@@ -1109,6 +1035,7 @@ k_class_up(ClassRows<T> L, FusedTile ft, int rxs, int rys, T w, int nu, const T*
   T* B = M.S;
   int2* RR = M.RR;
   const TileOrigin o = tile_origin(ft);
+  stage_hot_row(L, M);
 
   T* D = M.D;
   struct ClsUpLd {
@@ -1117,45 +1044,42 @@ k_class_up(ClassRows<T> L, FusedTile ft, int rxs, int rys, T w, int nu, const T*
     uint8_t act;
     T xv, bv, ev, dv;
   };
-  const bool go = for_region_batched<kLoadBatch>(
+  for_region_batched<kLoadBatch>(
       ft.rx, ry,
-      [&](int a, int c) {  // raw values: masked only when stored, after the flag
+      [&](int a, int c) {
         ClsUpLd v;
-        int i, j, k;
-        v.ok = resolve_k(o.gi0 + a, o.gj0 + c, L.nx, L.ny, L.px, L.py, i, j, k);
-        v.rr = L.rr[k];
-        v.xv = x_pre[k];
-        v.bv = b[k];
-        v.dv = L.rdinv[k];
-        v.act = L.ract[k];
-        v.ev = xc[(i / fx) * cny + (j / fy)];  // in range for every grid cell
+        int i, j;
+        v.ok = resolve(o.gi0 + a, o.gj0 + c, L.nx, L.ny, L.px, L.py, i, j);
+        v.rr = make_int2(-1, -1);
+        v.act = 0;
+        v.xv = v.bv = v.ev = v.dv = (T)0;
+        if (v.ok) {
+          const int k = i * L.ny + j;
+          v.rr = L.rr[k];
+          v.xv = x_pre[k];
+          v.bv = b[k];
+          v.dv = L.rdinv[k];
+          v.act = L.ract[k];
+          v.ev = xc[(i / fx) * cny + (j / fy)];  // in range for every grid cell
+        }
         return v;
       },
       [&](int a, int c, int idx, const ClsUpLd& v) {
-        RR[idx] = v.ok ? v.rr : make_int2(-1, -1);
-        X[idx] = v.ok ? v.xv : (T)0;
-        B[idx] = v.ok ? v.bv : (T)0;
-        D[idx] = v.ok ? v.dv : (T)0;
-        E[idx] = (v.ok && v.act) ? v.ev : (T)0;
-      },
-      [&]() {  // the hot row's loads join the region's in flight
-        if (done) return false;
-        store_hot_entry(M, load_hot_entry(L));
-        M.clear_counters();
-        return true;
+        RR[idx] = v.rr;
+        X[idx] = v.xv;
+        B[idx] = v.bv;
+        D[idx] = v.dv;
+        E[idx] = v.act ? v.ev : (T)0;
       });
-  if (!go) return;
-  int call = 0;  // ring passes so far (deferred-row counters)
   __syncthreads();
-  ring_rows(L, M, ring(ft, nu, rxs, rys), ry, E, call, [&](int, int, int idx, int2 rr, T ax) {
+  ring_rows(L, M, ring(ft, nu, rxs, rys), ry, E, [&](int, int, int idx, int2 rr, T ax) {
     if (rr.x >= 0) X[idx] = X[idx] + E[idx] - w * D[idx] * ax;
   });
   __syncthreads();
   T* Y = E;
   for (int sw = 1; sw <= nu; ++sw) {
     const bool last = sw == nu;
-    ring_rows(L, M, ring(ft, nu - sw, rxs, rys), ry, X, call,
-              [&](int a, int c, int idx, int2 rr, T ax) {
+    ring_rows(L, M, ring(ft, nu - sw, rxs, rys), ry, X, [&](int a, int c, int idx, int2 rr, T ax) {
       const T val = (rr.x >= 0) ? X[idx] + w * D[idx] * (B[idx] - ax) : (T)0;
       if (!last) {
         Y[idx] = val;

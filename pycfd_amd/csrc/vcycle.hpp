@@ -36,14 +36,7 @@ struct ClassRows {
   const uint8_t* ract;   // per cell active flag
   const int2* rr;        // per cell entry range of its class row
   int hot_e0;            // first entry of the hot class
-  int nh_lanes;          // lanes per row of another class (1, 2, 4, 8, 16): the
-                         // fewest with kNhPer entries each covering the longest row
 };
-
-// Entries a lane of the fused kernels' deferred rows loads per trip (half as
-// many 8-byte values, to keep the batch's registers alike).
-template <typename T>
-constexpr int nh_per() { return sizeof(T) == 4 ? 16 : 8; }
 
 // Largest row the fused class kernels stage in LDS as the level's hot class.
 constexpr int kHotMax = 256;
