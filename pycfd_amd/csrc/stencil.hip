@@ -9,6 +9,8 @@
 // reference's array-metric arithmetic; the uniform ones are unchanged.
 #include "internal.hpp"
 
+#include <type_traits>
+
 namespace pcfd {
 
 // ----------------------------------------------------------------------------
@@ -362,6 +364,82 @@ k_predictor_comb(const double* __restrict__ u0, const double* __restrict__ ue,
                         T.get_dt(), us, vs);
 }
 
+// Row-blocked predictor on a uniform mesh: a thread evaluates the u and v
+// faces of kPredRows consecutive x rows at one y index.  The values its faces
+// read -- rows i0-1 .. i0+kPredRows, columns j-1 .. j+1 of u and of v, the
+// stage combination folded in -- are loaded once, at indices clamped into
+// each array (a value at a clamped index is only read by faces outside the
+// update ranges, which copy their own, in-range value), and the faces take
+// them from registers: about 3.75 loads a face instead of 9.  The arithmetic
+// of every face is rhs_u_uniform / rhs_v_uniform's, so the results are the
+// one-face-per-thread kernel's bit for bit.
+constexpr int kPredRows = 4;      // rows per thread, plain loads
+constexpr int kPredRowsComb = 4;  // with the stage combination (2 rows measured slower)
+
+// A view of a register cache c[a][b] = value (i0 - 1 + a, j0 - 1 + b).  The
+// offsets live outside the array, so once the rows are unrolled every index
+// folds to a constant and the array is promoted to registers.
+struct RowCacheView {
+  const double (*c)[3];
+  int i0, j0;
+  __device__ __forceinline__ double operator()(int i, int j) const {
+    return c[i - i0 + 1][j - j0 + 1];
+  }
+};
+
+template <bool P2, bool LES, int NR, class Src>
+__global__ void __launch_bounds__(kBlock)
+k_predictor_rows(Src Us, Src Vs, MomParams P, UpdRange R, StepScalars T, double* __restrict__ us,
+                 double* __restrict__ vs) {
+  P.blend = T.get_blend();
+  if constexpr (!LES) P.nuc = nullptr;  // the constant-viscosity form, folded at compile time
+  const double dt = T.get_dt();
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  const int ty = tiles_y(P.ny + 3);
+  const int i0 = (tile / ty) * (kTileX * NR) + ((int)threadIdx.x / kTileY) * NR;
+  const int j = (tile % ty) * kTileY + (int)threadIdx.x % kTileY;
+  double cu[NR + 2][3], cv[NR + 2][3];
+#pragma unroll
+  for (int a = 0; a < NR + 2; ++a) {
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      const int ii = i0 - 1 + a, jj = j - 1 + b;
+      cu[a][b] = Us(min(max(ii, 0), P.nx + 2), min(max(jj, 0), P.ny + 1));
+      cv[a][b] = Vs(min(max(ii, 0), P.nx + 1), min(max(jj, 0), P.ny + 2));
+    }
+  }
+  const RowCacheView U{cu, i0, j}, V{cv, i0, j};
+  // one row's two faces; r is a compile-time constant, so every cache index
+  // is one and the cache stays in registers
+  auto row = [&](auto rc) {
+    constexpr int r = decltype(rc)::value;
+    const int i = i0 + r;
+    if (i <= P.nx + 2 && j <= P.ny + 1) {
+      double val = cu[r + 1][1];
+      if (i >= R.u_lo && i < R.u_hi && j >= 1 && j <= P.ny)
+        val += dt * rhs_u_uniform<P2>(U, V, i, j, P);
+      us[(size_t)i * P.ldu + j] = val;
+    }
+    if (i <= P.nx + 1 && j <= P.ny + 2) {
+      double val = cv[r + 1][1];
+      if (i >= 1 && i <= P.nx && j >= R.v_lo && j < R.v_hi)
+        val += dt * rhs_v_uniform<P2>(U, V, i, j, P);
+      vs[(size_t)i * P.ldv + j] = val;
+    }
+  };
+  static_assert(NR == 2 || NR == 4, "the rows below");
+  row(std::integral_constant<int, 0>{});
+  row(std::integral_constant<int, 1>{});
+  if constexpr (NR == 4) {
+    row(std::integral_constant<int, 2>{});
+    row(std::integral_constant<int, 3>{});
+  }
+}
+
+static dim3 grid_rows(int nxr, int nyr, int nr) {
+  return dim3((unsigned)(((nxr + kTileX * nr - 1) / (kTileX * nr)) * tiles_y(nyr)));
+}
+
 // out = ca*a + cb*b over both ghosted velocity arrays (solver.py:706-721).
 __global__ void __launch_bounds__(kBlock)
 k_combine(const double* __restrict__ u0, const double* __restrict__ ue,
@@ -650,12 +728,19 @@ int launch_momentum_rhs(hipStream_t s, const double* u, const double* v, const M
 int launch_predictor(hipStream_t s, const double* u, const double* v, const MomParams& P,
                      const UpdRange& R, const StepScalars& T, double* us, double* vs) {
   const dim3 g = grid_for(P.nx + 3, P.ny + 3);
+  const dim3 gr = grid_rows(P.nx + 3, P.ny + 3, kPredRows);
+  const PlainLoad U{u, P.ldu}, V{v, P.ldv};
   if (P.g.hx)
     hipLaunchKernelGGL((k_predictor<true, false>), g, dim3(kBlock), 0, s, u, v, P, R, T, us, vs);
+  else if (P.nuc)
+    hipLaunchKernelGGL((k_predictor_rows<false, true, kPredRows, PlainLoad>), gr, dim3(kBlock), 0, s, U, V, P,
+                       R, T, us, vs);
   else if (P.pow2)
-    hipLaunchKernelGGL((k_predictor<false, true>), g, dim3(kBlock), 0, s, u, v, P, R, T, us, vs);
+    hipLaunchKernelGGL((k_predictor_rows<true, false, kPredRows, PlainLoad>), gr, dim3(kBlock), 0, s, U, V, P,
+                       R, T, us, vs);
   else
-    hipLaunchKernelGGL((k_predictor<false, false>), g, dim3(kBlock), 0, s, u, v, P, R, T, us, vs);
+    hipLaunchKernelGGL((k_predictor_rows<false, false, kPredRows, PlainLoad>), gr, dim3(kBlock), 0, s, U, V,
+                       P, R, T, us, vs);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
@@ -664,15 +749,20 @@ int launch_predictor_comb(hipStream_t s, const double* u0, const double* ue, con
                           const double* ve, double ca, double cb, const MomParams& P,
                           const UpdRange& R, const StepScalars& T, double* us, double* vs) {
   const dim3 g = grid_for(P.nx + 3, P.ny + 3);
+  const dim3 gr = grid_rows(P.nx + 3, P.ny + 3, kPredRowsComb);
+  const CombLoad U{u0, ue, P.ldu, ca, cb}, V{v0, ve, P.ldv, ca, cb};
   if (P.g.hx)
     hipLaunchKernelGGL((k_predictor_comb<true, false>), g, dim3(kBlock), 0, s, u0, ue, v0, ve, ca,
                        cb, P, R, T, us, vs);
+  else if (P.nuc)
+    hipLaunchKernelGGL((k_predictor_rows<false, true, kPredRowsComb, CombLoad>), gr, dim3(kBlock), 0, s, U, V, P,
+                       R, T, us, vs);
   else if (P.pow2)
-    hipLaunchKernelGGL((k_predictor_comb<false, true>), g, dim3(kBlock), 0, s, u0, ue, v0, ve, ca,
-                       cb, P, R, T, us, vs);
+    hipLaunchKernelGGL((k_predictor_rows<true, false, kPredRowsComb, CombLoad>), gr, dim3(kBlock), 0, s, U, V, P,
+                       R, T, us, vs);
   else
-    hipLaunchKernelGGL((k_predictor_comb<false, false>), g, dim3(kBlock), 0, s, u0, ue, v0, ve,
-                       ca, cb, P, R, T, us, vs);
+    hipLaunchKernelGGL((k_predictor_rows<false, false, kPredRowsComb, CombLoad>), gr, dim3(kBlock), 0, s, U, V,
+                       P, R, T, us, vs);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
