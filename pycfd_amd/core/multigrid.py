@@ -21,6 +21,8 @@ the reference's own coarsest level.
 from __future__ import annotations
 
 import logging
+import os
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass
 
 import numpy as np
@@ -36,6 +38,54 @@ POWER_ITERATIONS = 12
 STALL_FACTOR = 0.995
 STALL_PATIENCE = 3
 ANISOTROPY_THRESHOLD = 2.0
+
+
+# Sparse products are split into row blocks run on threads (SciPy's sparse
+# kernels release the interpreter lock).  Each output row is computed by one
+# block exactly as the whole product computes it, so the result does not
+# depend on the split.
+_SETUP_THREADS = max(1, min(8, os.cpu_count() or 1))
+_BLOCKED_MIN_NNZ = 200_000
+_POOL: ThreadPoolExecutor | None = None
+
+
+def _pool() -> ThreadPoolExecutor:
+    global _POOL
+    if _POOL is None:
+        _POOL = ThreadPoolExecutor(_SETUP_THREADS, thread_name_prefix="pcfd-mg-setup")
+    return _POOL
+
+
+def _row_cuts(X: sp.csr_matrix) -> np.ndarray:
+    """Row block boundaries with about equal nonzeros per block."""
+    n = X.shape[0]
+    targets = np.linspace(0, X.nnz, _SETUP_THREADS + 1)
+    cuts = np.searchsorted(X.indptr, targets).clip(0, n)
+    cuts[0], cuts[-1] = 0, n
+    return np.unique(cuts)
+
+
+def _rows_view(X: sp.csr_matrix, a: int, b: int) -> sp.csr_matrix:
+    """Rows [a, b) of X sharing X's data and column arrays (no copy)."""
+    s, e = X.indptr[a], X.indptr[b]
+    return sp.csr_matrix((X.data[s:e], X.indices[s:e], X.indptr[a:b + 1] - s),
+                         shape=(b - a, X.shape[1]), copy=False)
+
+
+def _matmul(X: sp.csr_matrix, Y: sp.csr_matrix) -> sp.csr_matrix:
+    """X @ Y (both CSR), by row blocks of X on threads."""
+    X, Y = sp.csr_matrix(X), sp.csr_matrix(Y)
+    if _SETUP_THREADS == 1 or X.nnz < _BLOCKED_MIN_NNZ:
+        return sp.csr_matrix(X @ Y)
+    cuts = _row_cuts(X)
+    parts = list(_pool().map(lambda ab: sp.csr_matrix(_rows_view(X, ab[0], ab[1]) @ Y),
+                             zip(cuts[:-1], cuts[1:])))
+    offs = np.cumsum([0] + [q.nnz for q in parts])
+    indptr = np.concatenate([parts[0].indptr[:1]] +
+                            [q.indptr[1:] + o for q, o in zip(parts, offs[:-1])])
+    return sp.csr_matrix((np.concatenate([q.data for q in parts]),
+                          np.concatenate([q.indices for q in parts]), indptr),
+                         shape=(X.shape[0], Y.shape[1]), copy=False)
 
 
 def spectral_radius(A: sp.csr_matrix, dinv: np.ndarray, seed: int = 0) -> float:
@@ -103,7 +153,7 @@ def tentative_prolongator(labels: np.ndarray, n_coarse: int) -> sp.csr_matrix:
 
 
 def smooth_prolongator(A, dinv, weight, P) -> sp.csr_matrix:
-    return (P - sp.diags(weight * dinv) @ (A @ P)).tocsr()
+    return (P - sp.diags(weight * dinv) @ _matmul(A, P)).tocsr()
 
 
 def _identity_on_inactive(A: sp.csr_matrix, active: np.ndarray) -> sp.csr_matrix:
@@ -175,12 +225,18 @@ def to_class_stencil(A: sp.csr_matrix, shape: tuple[int, int], periodic: tuple[b
         di = (di + nx // 2) % nx - nx // 2
     if periodic[1] and ny > 1:
         dj = (dj + ny // 2) % ny - ny // 2
-    key = di.astype(np.int64) * 4096 + dj
-    uniq, inv = np.unique(key, return_inverse=True)
-    udi = np.round(uniq / 4096).astype(np.int64)
-    udj = uniq - udi * 4096
-    if np.abs(udi).max(initial=0) > 127 or np.abs(udj).max(initial=0) > 127:
+    if np.abs(di).max(initial=0) > 127 or np.abs(dj).max(initial=0) > 127:
         raise ValueError("coarse stencil offset exceeds the int8 range")
+    # distinct offsets in (di, dj) order and each entry's index among them, by
+    # a presence table over the int8 offset range (no sort of the entries)
+    key = (di.astype(np.int64) + 128) * 256 + (dj.astype(np.int64) + 128)
+    present = np.zeros(256 * 256, dtype=bool)
+    present[key] = True
+    uniq = np.flatnonzero(present)
+    slot = np.cumsum(present) - 1
+    inv = slot[key]
+    udi = uniq // 256 - 128
+    udj = uniq % 256 - 128
     S = uniq.size
     rows = np.zeros((nx * ny, S + 1))
     rows[r, inv] = v
@@ -270,7 +326,9 @@ class MultigridHierarchy:
                 break
             P = tentative_prolongator(labels, n_coarse)
             P = smooth_prolongator(level.A, level.dinv, level.weight, P)
-            coarse_A = _identity_on_inactive(sp.csr_matrix(P.T @ level.A @ P), coarse_active)
+            # P^T (A P): both products by row blocks (P^T held as CSR)
+            coarse_A = _identity_on_inactive(
+                _matmul(sp.csr_matrix(P.T), _matmul(level.A, P)), coarse_active)
             coarse_A.sum_duplicates()
             level.factors = (fx, fy)
             level = self._make_level(coarse_A, next_shape, coarse_active, singular)
