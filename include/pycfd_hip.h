@@ -324,21 +324,26 @@ int pcfd_mg_apply(pcfd_ctx* ctx, const double* r, double* z);
  * reference counterpart: measurement only. */
 int pcfd_time_kernel(pcfd_ctx* ctx, int32_t which, int32_t reps, double* avg_ms,
                      double* bytes_per_launch);
-/* Every kernel of the pressure solves that follow, timed live: each launch
- * records events at the kernel's own start and end (hipExtLaunchKernelGGL),
- * up to max_launches launches in all; launches of iterations enqueued past
- * convergence (no-ops) are dropped.  _read stops the clock and fills, per
- * slot (arrays of PCFD_CLOCK_SLOTS), launches timed, mean duration in ms and
- * algorithmic bytes per launch.  slot = kind * PCFD_CLOCK_LEVELS + level,
- * kinds: 0 CG update, 1 multigrid-CG direction, 2 / 3 level-0 down / up leg,
- * 4 / 5 fused stored-level down / up leg, 6-9 stored-level row passes
- * (zero-start smooth + residual, smooth, residual, prolongation), 10
- * restriction, 11 dense coarsest solve, 12 cooperative chain, 13 reductions
- * (level 0: cycle (standalone multigrid; multigrid-CG's direction kernel folds
- * the cycle's sums itself), 1: alpha (Jacobi-CG only; multigrid-CG's update
- * kernel folds alpha itself), 2: update).  No reference counterpart:
- * measurement only. */
-#define PCFD_CLOCK_KINDS 14
+/* Every kernel of the time steps and pressure solves that follow, timed
+ * live: each launch records events at the kernel's own start and end
+ * (hipExtLaunchKernelGGL), up to max_launches launches in all; launches of
+ * iterations enqueued past convergence (no-ops) are dropped.  _read stops the
+ * clock and fills, per slot (arrays of PCFD_CLOCK_SLOTS), launches timed, mean
+ * duration in ms and algorithmic bytes per launch.  slot = kind *
+ * PCFD_CLOCK_LEVELS + level, kinds: 0 CG update, 1 multigrid-CG direction,
+ * 2 / 3 level-0 down / up leg, 4 / 5 fused stored-level down / up leg, 6-9
+ * stored-level row passes (zero-start smooth + residual, smooth, residual,
+ * prolongation), 10 restriction, 11 dense coarsest solve, 12 cooperative
+ * chain, 13 reductions (level 0: cycle (standalone multigrid; multigrid-CG's
+ * direction kernel folds the cycle's sums itself), 1: alpha (Jacobi-CG only;
+ * multigrid-CG's update kernel folds alpha itself), 2: update, 3: right-hand
+ * side mean, 4: solve start, 5: final check); the step's other kernels at
+ * level 0: 14 predictor, 15 stage-combining predictor, 16 wall conditions,
+ * 17 one-workgroup boundary pass, 18 right-hand side, 19 solve start, 20
+ * final residual check, 21 pressure scatter, 22 gradient correction, 23
+ * stage combination, 24 diagnostics.  No reference counterpart: measurement
+ * only. */
+#define PCFD_CLOCK_KINDS 25
 #define PCFD_CLOCK_LEVELS 16
 #define PCFD_CLOCK_SLOTS (PCFD_CLOCK_KINDS * PCFD_CLOCK_LEVELS)
 int pcfd_kernel_clock_start(pcfd_ctx* ctx, int32_t max_launches);

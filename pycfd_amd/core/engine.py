@@ -278,11 +278,13 @@ class DeviceContext:
     CLOCK_KINDS = ("k_cg_update", "k_fine_dir", "k_fine_down", "k_fine_up", "k_class_down",
                    "k_class_up", "k_cls_rows[smooth0+res]", "k_cls_rows[smooth]",
                    "k_cls_rows[res]", "k_cls_rows[prolong]", "k_cls_restrict", "k_dense_apply",
-                   "k_chain", "k_fold")
+                   "k_chain", "k_fold", "k_predictor", "k_predictor_comb", "k_walls",
+                   "k_boundary_pass", "k_prepare", "k_init", "k_check", "k_scatter",
+                   "k_correct_ghosts", "k_combine", "k_diagnostics")
     CLOCK_LEVELS = 16
 
     def kernel_clock_start(self, max_launches: int) -> None:
-        """Time every kernel of the pressure solves that follow, live."""
+        """Time every kernel of the steps and pressure solves that follow, live."""
         self._call("pcfd_kernel_clock_start", int(max_launches))
 
     def kernel_clock_read(self) -> list[dict]:

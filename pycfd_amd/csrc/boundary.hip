@@ -513,8 +513,14 @@ __device__ void mask_obstacle(const BCParams& P, int mode, double dt, double* u,
 // (blockIdx.y = wall).
 constexpr int kWallCorner = 4;
 
+// With mask_op set, workgroup (0, 4) runs that immersed-boundary mask beside
+// the walls (the faces lie apart from every wall line: BCParams::mask_apart).
 __global__ void __launch_bounds__(kBlock)
-k_walls(BCParams P, int predictor, double* u, double* v) {
+k_walls(BCParams P, int predictor, double* u, double* v, int mask_op, StepScalars T) {
+  if (blockIdx.y == 4) {
+    if (blockIdx.x == 0) mask_obstacle(P, mask_op, T.get_dt(), u, v);
+    return;
+  }
   auto corner = [](int k, int len) { return k < kWallCorner || k >= len - kWallCorner; };
   const int w = blockIdx.y;
   const int k = blockIdx.x * kBlock + threadIdx.x;
@@ -609,16 +615,31 @@ k_pressure_ghosts(BCParams P, double* p, int ldp) {
 // Without a periodic axis (and on a grid wide enough that opposite walls
 // touch disjoint lines) the wall conditions run on the whole device
 // (k_walls); the rest of the pass (O(perimeter) lines and reductions) stays
-// on one workgroup, in the pass's order.
+// on one workgroup, in the pass's order.  A mask whose faces lie apart from
+// the wall lines commutes with the walls and the mass balance, so it moves
+// into the pass's walls launch (one workgroup beside the wall entries).
 int launch_boundary_pass(hipStream_t s, const BCParams& P, const BCOps& ops,
-                         const StepScalars& T, double* u, double* v) {
+                         const StepScalars& T, double* u, double* v, const ClockSource& clk) {
   if (ops.n == 0) return kOk;
   const bool walls_apart = !P.periodic_x && !P.periodic_y && P.nx >= 2 * kWallCorner &&
                            P.ny >= 2 * kWallCorner;
+  auto is_mask = [](int op) {
+    return op == BOP_MASK_RESET || op == BOP_MASK_ACC || op == BOP_MASK_ZERO;
+  };
+  int n_walls = 0, n_mask = 0, mask_op = 0;
+  for (int q = 0; q < ops.n; ++q) {
+    n_walls += ops.op[q] == BOP_VEL_PRED || ops.op[q] == BOP_VEL_CORR;
+    if (is_mask(ops.op[q])) {
+      ++n_mask;
+      mask_op = ops.op[q];
+    }
+  }
+  const bool fuse_mask = walls_apart && P.mask_apart && n_walls == 1 && n_mask == 1;
   BCOps rest{};
   auto flush = [&]() {
     if (rest.n == 0) return;
-    hipLaunchKernelGGL(k_boundary_pass, dim3(1), dim3(kBlock), 0, s, P, rest, T, u, v);
+    const LaunchClock c = clk.get(kClockBoundaryPass, 0.0);
+    PCFD_LAUNCH_TIMED(c, k_boundary_pass, dim3(1), dim3(kBlock), 0, s, P, rest, T, u, v);
     rest.n = 0;
   };
   for (int q = 0; q < ops.n; ++q) {
@@ -626,8 +647,12 @@ int launch_boundary_pass(hipStream_t s, const BCParams& P, const BCOps& ops,
     if (walls_apart && (op == BOP_VEL_PRED || op == BOP_VEL_CORR)) {
       flush();
       const int len = std::max(P.nx, P.ny) + 3;
-      hipLaunchKernelGGL(k_walls, dim3((len + kBlock - 1) / kBlock, 4), dim3(kBlock), 0, s, P,
-                         op == BOP_VEL_PRED ? 1 : 0, u, v);
+      const LaunchClock c = clk.get(kClockWalls, 0.0);
+      PCFD_LAUNCH_TIMED(c, k_walls, dim3((len + kBlock - 1) / kBlock, fuse_mask ? 5 : 4),
+                        dim3(kBlock), 0, s, P, op == BOP_VEL_PRED ? 1 : 0, u, v,
+                        fuse_mask ? mask_op : 0, T);
+    } else if (fuse_mask && is_mask(op)) {
+      continue;  // runs in the walls launch
     } else if (op != BOP_WRAP_FACES || !walls_apart) {  // face wrap: periodic axes only
       rest.op[rest.n++] = op;
     }

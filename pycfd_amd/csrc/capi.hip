@@ -172,7 +172,13 @@ BCOps ops_of(std::initializer_list<int> l) {
 
 int boundary(pcfd_ctx* c, std::initializer_list<int> l, const StepScalars& S, double* u,
              double* v) {
-  return launch_boundary_pass(c->stream, c->bc, ops_of(l), S, u, v);
+  return launch_boundary_pass(c->stream, c->bc, ops_of(l), S, u, v, c->engine.boundary_clocks());
+}
+
+// The live kernel clock's entry for one step kernel, with its algorithmic
+// bytes per cell (each array read or written once at its stored size).
+LaunchClock step_clock(pcfd_ctx* c, int kind, double bytes_per_cell) {
+  return c->engine.clock_step(kind, bytes_per_cell * (double)c->nx * (double)c->ny);
 }
 
 int boundary(pcfd_ctx* c, std::initializer_list<int> l, double dt, double* u, double* v) {
@@ -214,30 +220,39 @@ void fill_info(pcfd_solve_info* info, const SolveInfo& s) {
 // the fused stage combination ca*u + cb*ue as the substep input.
 int substep(pcfd_ctx* c, const double* u, const double* v, const double* ue, const double* ve,
             double ca, double cb, const StepScalars& S, double* us, double* vs, double* p,
-            SolveInfo* info, bool sync) {
+            SolveInfo* info, bool sync, bool last = true) {
+  // The reaction force a substep books is overwritten by the next substep's
+  // (solver.py:577-583, reset after each predictor), so only the step's last
+  // substep sums it; the others just zero the faces.
+  const int mask_pred = last ? BOP_MASK_RESET : BOP_MASK_ZERO;
+  const int mask_corr = last ? BOP_MASK_ACC : BOP_MASK_ZERO;
   int rc;
   if (c->use_les &&
       (rc = launch_eddy_viscosity(c->stream, c->les, u, ue, v, ve, ca, cb, c->nuc, nullptr)))
     return rc;
-  if (ue) {
-    rc = launch_predictor_comb(c->stream, u, ue, v, ve, ca, cb, c->mom, c->upd, S, us, vs);
-  } else {
-    rc = launch_predictor(c->stream, u, v, c->mom, c->upd, S, us, vs);
+  if (ue) {  // u0, ue, v0, ve in; u*, v* out
+    rc = launch_predictor_comb(c->stream, u, ue, v, ve, ca, cb, c->mom, c->upd, S, us, vs,
+                               step_clock(c, PoissonEngine::CK_PREDICTOR_COMB, 48));
+  } else {  // u, v in; u*, v* out
+    rc = launch_predictor(c->stream, u, v, c->mom, c->upd, S, us, vs,
+                          step_clock(c, PoissonEngine::CK_PREDICTOR, 32));
   }
   if (rc) return rc;
   if (c->any_outlet) {
-    rc = boundary(c, {BOP_VEL_PRED, BOP_MASS, BOP_MASK_RESET, BOP_WRAP_FACES}, S, us, vs);
+    rc = boundary(c, {BOP_VEL_PRED, BOP_MASS, mask_pred, BOP_WRAP_FACES}, S, us, vs);
   } else {
-    rc = boundary(c, {BOP_VEL_PRED, BOP_MASK_RESET, BOP_WRAP_FACES}, S, us, vs);
+    rc = boundary(c, {BOP_VEL_PRED, mask_pred, BOP_WRAP_FACES}, S, us, vs);
   }
   if (rc) return rc;
   if ((rc = c->engine.prepare_from_velocity(us, vs, c->ldu, c->ldv, c->dx, c->dy, S))) return rc;
   if ((rc = c->engine.solve(info, sync))) return rc;
   if ((rc = c->engine.scatter_pressure(p, c->ldp))) return rc;
+  // p, u*, v* in; u, v out
   if ((rc = launch_correct_ghosts(c->stream, p, c->ldp, c->bc, c->ldu, c->ldv, c->dx, c->dy,
-                                  c->g, c->upd, S, us, vs)))
+                                  c->g, c->upd, S, us, vs,
+                                  step_clock(c, PoissonEngine::CK_CORRECT, 40))))
     return rc;
-  return boundary(c, {BOP_MASK_ACC, BOP_WRAP_FACES, BOP_VEL_CORR}, S, us, vs);
+  return boundary(c, {mask_corr, BOP_WRAP_FACES, BOP_VEL_CORR}, S, us, vs);
 }
 
 int substeps_of(int scheme) {
@@ -258,7 +273,7 @@ int step_body(pcfd_ctx* c, const double* u0, const double* v0, const StepScalars
                  double ca, double cb, double* ua, double* va, bool last) -> int {
     E.set_log_slot(log ? k : -1);
     E.set_budget(budget ? budget[k] : 0);
-    const int r = substep(c, a, b, ae, be, ca, cb, S, ua, va, p, s, last && want);
+    const int r = substep(c, a, b, ae, be, ca, cb, S, ua, va, p, s, last && want, last);
     E.set_log_slot(-1);
     E.set_budget(0);
     return r;
@@ -271,7 +286,7 @@ int step_body(pcfd_ctx* c, const double* u0, const double* v0, const StepScalars
       if ((rc = sub(0, u0, v0, nullptr, nullptr, 0, 0, A[0], A[1], false))) return rc;
       if ((rc = sub(1, A[0], A[1], nullptr, nullptr, 0, 0, B[0], B[1], true))) return rc;
       if ((rc = launch_combine(c->stream, u0, B[0], v0, B[1], 0.5, 0.5, c->nx, c->ny, c->ldu,
-                               c->ldv, u, v)))
+                               c->ldv, u, v, step_clock(c, PoissonEngine::CK_COMBINE, 48))))
         return rc;
       break;
     default:  // SSP-RK3, solver.py:709-721
@@ -279,7 +294,8 @@ int step_body(pcfd_ctx* c, const double* u0, const double* v0, const StepScalars
       if ((rc = sub(1, A[0], A[1], nullptr, nullptr, 0, 0, B[0], B[1], false))) return rc;
       if ((rc = sub(2, u0, v0, B[0], B[1], 0.75, 0.25, A[0], A[1], true))) return rc;
       if ((rc = launch_combine(c->stream, u0, A[0], v0, A[1], 1.0 / 3.0, 2.0 / 3.0, c->nx, c->ny,
-                               c->ldu, c->ldv, u, v)))
+                               c->ldu, c->ldv, u, v,
+                               step_clock(c, PoissonEngine::CK_COMBINE, 48))))
         return rc;
       break;
   }
@@ -778,6 +794,15 @@ int pcfd_ctx_create(const pcfd_case_desc* d, void* hip_stream, pcfd_ctx** out) {
   B.n_bodies = c->n_bodies;
   B.area = d->dx * d->dy;  // mesh.py:372 (_dx * _dy)
   B.force = c->force;
+  {
+    auto inside = [&](int a, int b, int na, int nb) {  // a in [0, na), b in [0, nb)
+      return a >= kMaskApart && a < na - kMaskApart && b >= kMaskApart && b < nb - kMaskApart;
+    };
+    bool apart = binary_faces;
+    for (int f : uf) apart = apart && inside(f / c->ldu, f % c->ldu, nx + 3, ny + 2);
+    for (int f : vf) apart = apart && inside(f / c->ldv, f % c->ldv, nx + 2, ny + 3);
+    B.mask_apart = apart ? 1 : 0;
+  }
 
   MomParams& M = c->mom;
   M.nx = nx;
@@ -1240,9 +1265,11 @@ int pcfd_pressure_guess(pcfd_ctx* c, double* buf, int32_t store) {
 int pcfd_diagnostics(pcfd_ctx* c, const double* u, const double* v, const double* p,
                      const double* u_prev, const double* v_prev, double* out_host) {
   if (int rc = check_ctx(c)) return rc;
+  // u, v, p, u_prev, v_prev, the solid byte
   int rc = launch_diagnostics(c->stream, u, v, p, u_prev, v_prev, c->solid, c->nx, c->ny, c->ldu,
                               c->ldv, c->ldp, c->dx, c->dy, c->g, c->engine.partial_,
-                              c->engine.counter_, c->diag_out);
+                              c->engine.counter_, c->diag_out,
+                              step_clock(c, PoissonEngine::CK_DIAG, 41));
   if (rc) return rc;
   PCFD_HIP_CHECK(
       hipMemcpyAsync(c->host_out, c->diag_out, 7 * sizeof(double), hipMemcpyDeviceToHost, c->stream));

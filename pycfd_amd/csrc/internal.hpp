@@ -74,7 +74,12 @@ struct BCParams {
   const double* vf_area;
   const double* hx;        // nx cell widths
   const double* hy;        // ny cell heights
+  // Every immersed-boundary face lies at least kMaskApart lines inside the
+  // arrays, away from all that the wall conditions and the mass balance read
+  // or write: masking then commutes with both and runs beside the walls.
+  int mask_apart;
 };
+constexpr int kMaskApart = 4;
 
 struct MomParams {
   int nx, ny, ldu, ldv;
@@ -124,15 +129,18 @@ int launch_eddy_viscosity(hipStream_t s, const LesParams& P, const double* u0, c
                           double* nu_corner);
 int launch_momentum_rhs(hipStream_t s, const double* u, const double* v, const MomParams& P,
                         double* rhs_u, double* rhs_v);
-// The predictors take dt and the blend from `S` (P.blend is ignored).
+// The predictors take dt and the blend from `S` (P.blend is ignored).  The
+// step's launchers take an optional LaunchClock (the live kernel clock).
 int launch_predictor(hipStream_t s, const double* u, const double* v, const MomParams& P,
-                     const UpdRange& R, const StepScalars& S, double* us, double* vs);
+                     const UpdRange& R, const StepScalars& S, double* us, double* vs,
+                     const LaunchClock& clk = LaunchClock{});
 int launch_predictor_comb(hipStream_t s, const double* u0, const double* ue, const double* v0,
                           const double* ve, double ca, double cb, const MomParams& P,
-                          const UpdRange& R, const StepScalars& S, double* us, double* vs);
+                          const UpdRange& R, const StepScalars& S, double* us, double* vs,
+                          const LaunchClock& clk = LaunchClock{});
 int launch_combine(hipStream_t s, const double* u0, const double* ue, const double* v0,
                    const double* ve, double ca, double cb, int nx, int ny, int ldu, int ldv,
-                   double* uo, double* vo);
+                   double* uo, double* vo, const LaunchClock& clk = LaunchClock{});
 // The launchers below take the mesh's Metrics: null arrays select the
 // uniform arithmetic on dx, dy.
 int launch_correct(hipStream_t s, const double* p, int ldp, int nx, int ny, int ldu, int ldv,
@@ -144,7 +152,8 @@ int launch_divergence(hipStream_t s, const double* u, const double* v, int nx, i
 // and the gradient correction of u, v, in one pass.
 int launch_correct_ghosts(hipStream_t s, double* p, int ldp, const BCParams& B, int ldu, int ldv,
                           double dx, double dy, const Metrics& g, const UpdRange& R,
-                          const StepScalars& T, double* u, double* v);
+                          const StepScalars& T, double* u, double* v,
+                          const LaunchClock& clk = LaunchClock{});
 // A spacing whose reciprocal is exact (a power of two): division by it and
 // multiplication by its reciprocal give the same bits, so kernels multiply.
 inline bool is_pow2(double h) {
@@ -156,11 +165,24 @@ size_t diagnostics_partials(int nx, int ny);
 int launch_diagnostics(hipStream_t s, const double* u, const double* v, const double* p,
                        const double* up, const double* vp, const uint8_t* solid, int nx, int ny,
                        int ldu, int ldv, int ldp, double dx, double dy, const Metrics& g,
-                       double* partial, unsigned int* counter, double* out);
+                       double* partial, unsigned int* counter, double* out,
+                       const LaunchClock& clk = LaunchClock{});
 
-// boundary.hip
+// Clocks for launchers that issue several kernels: next(ctx, kind, bytes)
+// hands out one LaunchClock per launch (none without a source).
+struct ClockSource {
+  LaunchClock (*next)(void* ctx, int kind, double bytes) = nullptr;
+  void* ctx = nullptr;
+  LaunchClock get(int kind, double bytes) const {
+    return next ? next(ctx, kind, bytes) : LaunchClock{};
+  }
+};
+
+// boundary.hip: the ClockSource kinds of its two kernels
+constexpr int kClockWalls = 0, kClockBoundaryPass = 1;
 int launch_boundary_pass(hipStream_t s, const BCParams& P, const BCOps& ops,
-                         const StepScalars& S, double* u, double* v);
+                         const StepScalars& S, double* u, double* v,
+                         const ClockSource& clk = ClockSource{});
 int launch_pressure_ghosts(hipStream_t s, const BCParams& P, double* p, int ldp);
 // Transpose of apply_velocity(predictor) on adjoint fields lu, lv (walls
 // with no periodic axis and no OUTLET).

@@ -27,6 +27,7 @@
 #include <array>
 #include <cstdio>
 #include <cstdlib>
+#include <climits>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -40,17 +41,6 @@ namespace pcfd {
 // ----------------------------------------------------------------------------
 enum InitMode : int { IM_JPCG = 0, IM_MGCG = 1, IM_RELAX = 2, IM_MG = 3 };
 enum CheckMode : int { CM_RELAX = 0, CM_MG = 1, CM_FINAL = 2 };
-
-struct EpiPrepare {  // fluid mean of the right-hand side (pressure.py:306-307)
-  static constexpr int K = 1;
-  SolveState* st;
-  int singular, nfluid;
-  __device__ void ops(int (&o)[K]) const { o[0] = kSum; }
-  __device__ bool active() const { return true; }
-  __device__ void operator()(const double (&v)[K]) const {
-    st->mean_b = (singular && nfluid > 0) ? v[0] / (double)nfluid : 0.0;
-  }
-};
 
 struct EpiInit {  // ||b||, ||r||, first rho, termination at iteration 0
   static constexpr int K = 4;
@@ -347,14 +337,24 @@ k_prepare(FineOp A, const double* __restrict__ u, const double* __restrict__ v, 
   block_partials<1>(s, ops, partial, m.blk);
 }
 
+// The right-hand side pass's partials, folded by the solve start into the
+// fluid mean of the right-hand side (pressure.py:306-307); rows == the pass's
+// workgroup count; partial == null: the mean is already in the state.
+struct PrepFold {
+  const double* partial;
+  int rows, singular, nfluid;
+};
+
 // Solver start: b <- b - mean_b on fluid rows; r = b - A x (x = guess with
 // non-fluid rows already zeroed); partials ||b||^2, ||r||^2 and, per mode,
-// r.(r/d) (Jacobi-CG) or sum(r) (input projection of multigrid-CG).
+// r.(r/d) (Jacobi-CG) or sum(r) (input projection of multigrid-CG).  With
+// pf.partial set, every workgroup first folds the right-hand side pass's sums
+// itself in k_fold's fixed order (the mean a fold launch would give, bit for bit,
+// without its launch); workgroup 0 records it.
 __global__ void __launch_bounds__(kBlock)
 k_init(FineOp A, double* __restrict__ b, const double* __restrict__ x, double* __restrict__ r,
-       int mode, const SolveState* st, double* partial) {
+       int mode, SolveState* st, double* partial, PrepFold pf) {
   const MultiTile m = multi_tile(A.nx, A.ny);
-  const double mb = st->mean_b;
   uint8_t f[kMultiTiles];
   double bv[kMultiTiles], xc[kMultiTiles], xl[kMultiTiles], xr[kMultiTiles], xb[kMultiTiles],
       xt[kMultiTiles];
@@ -369,6 +369,22 @@ k_init(FineOp A, double* __restrict__ b, const double* __restrict__ x, double* _
     xr[u] = x[nb.kR];
     xb[u] = x[nb.kB];
     xt[u] = x[nb.kT];
+  }
+  // the fold after the cell loads are issued, so the two overlap
+  double mb;
+  if (pf.partial) {
+    double v[1];
+    const int op1[1] = {kSum};
+    fold_partials<1>(v, op1, pf.partial, pf.rows);
+    __shared__ double s_mb;
+    if (threadIdx.x == 0) {
+      s_mb = (pf.singular && pf.nfluid > 0) ? v[0] / (double)pf.nfluid : 0.0;
+      if (blockIdx.x == 0) st->mean_b = s_mb;
+    }
+    __syncthreads();
+    mb = s_mb;
+  } else {
+    mb = st->mean_b;
   }
   double s[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -1282,6 +1298,7 @@ PoissonEngine::~PoissonEngine() {
   hipFree(log_);
   if (log_host_) hipHostFree(log_host_);
   hipFree(partial_);
+  hipFree(prep_partial_);
   hipFree(counter_);
   hipFree(fine_active_);
   if (st_host_) hipHostFree(st_host_);
@@ -1318,6 +1335,20 @@ LaunchClock PoissonEngine::clock_next(int kind, int level, double bytes) {
   clock_pool_used_ += 2;
   clock_log_.push_back(ClockEntry{kind * kClockLevels + lev, clock_iter_, c.start, c.stop, bytes});
   return c;
+}
+
+LaunchClock PoissonEngine::clock_step(int kind, double bytes, int level) {
+  if (!clock_on_) return LaunchClock{};
+  const int it = clock_iter_;
+  clock_iter_ = INT_MIN;  // an entry no trim drops
+  const LaunchClock c = clock_next(kind, level, bytes);
+  clock_iter_ = it;
+  return c;
+}
+
+LaunchClock PoissonEngine::boundary_clock(void* self, int kind, double bytes) {
+  return static_cast<PoissonEngine*>(self)->clock_step(kind == kClockWalls ? CK_WALLS : CK_BPASS,
+                                                       bytes);
 }
 
 int PoissonEngine::clock_read(int32_t* launches, double* avg_ms, double* bytes) {
@@ -1379,6 +1410,7 @@ int PoissonEngine::init(const FineOp& op, int n_fluid, int singular, hipStream_t
   const size_t nt = (size_t)num_tiles(op.nx + 3, op.ny + 3) + 64;
   partial_cap_ = nt * 8;
   PCFD_HIP_CHECK(hipMalloc(&partial_, partial_cap_ * sizeof(double)));
+  PCFD_HIP_CHECK(hipMalloc(&prep_partial_, (size_t)multi_blocks(op.nx, op.ny) * sizeof(double)));
   PCFD_HIP_CHECK(hipMalloc(&counter_, 64 * sizeof(unsigned int)));
   PCFD_HIP_CHECK(hipMemsetAsync(counter_, 0, 64 * sizeof(unsigned int), s_));
   PCFD_HIP_CHECK(hipMalloc(&st_, sizeof(SolveState)));
@@ -2051,23 +2083,29 @@ int PoissonEngine::plan_level_tiles(std::vector<LevelBuf>& lv) {
 int PoissonEngine::prepare_from_velocity(const double* u, const double* v, int ldu, int ldv,
                                          double dx, double dy, const StepScalars& T) {
   const int mb = multi_blocks(op_.nx, op_.ny);
+  // algorithmic bytes: u*, v* (one face each per cell), flags; b
+  const LaunchClock clk = clock_step(CK_PREPARE, (double)n_ * (8 + 8 + 1 + 8));
   if (is_pow2(dx) && is_pow2(dy))
-    hipLaunchKernelGGL(k_prepare<true>, dim3(mb), dim3(kBlock), 0, s_, op_, u, v, ldu, ldv, dx, dy,
-                       1.0 / dx, 1.0 / dy, T, (const double*)nullptr, b_, x_, partial_);
+    PCFD_LAUNCH_TIMED(clk, k_prepare<true>, dim3(mb), dim3(kBlock), 0, s_, op_, u, v, ldu, ldv,
+                      dx, dy, 1.0 / dx, 1.0 / dy, T, (const double*)nullptr, b_, x_,
+                      prep_partial_);
   else
-    hipLaunchKernelGGL(k_prepare<false>, dim3(mb), dim3(kBlock), 0, s_, op_, u, v, ldu, ldv, dx,
-                       dy, 1.0 / dx, 1.0 / dy, T, (const double*)nullptr, b_, x_, partial_);
+    PCFD_LAUNCH_TIMED(clk, k_prepare<false>, dim3(mb), dim3(kBlock), 0, s_, op_, u, v, ldu, ldv,
+                      dx, dy, 1.0 / dx, 1.0 / dy, T, (const double*)nullptr, b_, x_,
+                      prep_partial_);
   PCFD_LAUNCH_CHECK();
-  return fold(s_, partial_, mb, EpiPrepare{st_, singular_, n_fluid_});
+  prep_pending_ = true;  // the solve start folds the partials (k_init)
+  return kOk;
 }
 
 int PoissonEngine::prepare_from_rhs(const double* rhs) {
   const int mb = multi_blocks(op_.nx, op_.ny);
   hipLaunchKernelGGL(k_prepare<false>, dim3(mb), dim3(kBlock), 0, s_, op_,
                      (const double*)nullptr, (const double*)nullptr, 0, 0, 1.0, 1.0, 1.0, 1.0,
-                     StepScalars{1.0, 0.0, nullptr}, rhs, b_, x_, partial_);
+                     StepScalars{1.0, 0.0, nullptr}, rhs, b_, x_, prep_partial_);
   PCFD_LAUNCH_CHECK();
-  return fold(s_, partial_, mb, EpiPrepare{st_, singular_, n_fluid_});
+  prep_pending_ = true;  // the solve start folds the partials (k_init)
+  return kOk;
 }
 
 int PoissonEngine::reset_guess() {
@@ -2129,12 +2167,15 @@ int PoissonEngine::solve(SolveInfo* info, bool sync_info) {
 int PoissonEngine::finalize(bool recompute_residual) {
   (void)recompute_residual;
   const int mb = multi_blocks(op_.nx, op_.ny);
-  hipLaunchKernelGGL(k_check, dim3(mb), dim3(kBlock), 0, s_, op_, x_, b_, (int)CM_FINAL,
-                     partial_, (const SolveState*)st_);
+  // algorithmic bytes: x, b, flags
+  const LaunchClock clk = clock_step(CK_CHECK, (double)n_ * (8 + 8 + 1));
+  PCFD_LAUNCH_TIMED(clk, k_check, dim3(mb), dim3(kBlock), 0, s_, op_, x_, b_, (int)CM_FINAL,
+                    partial_, (const SolveState*)st_);
   PCFD_LAUNCH_CHECK();
   SolveLog* log = (log_slot_ >= 0 && log_slot_ < kLogSlots) ? log_ + log_slot_ : nullptr;
   return fold(s_, partial_, mb,
-              EpiCheck{st_, (int)CM_FINAL, cfg_.tol, cfg_.maxiter, singular_, n_fluid_, log});
+              EpiCheck{st_, (int)CM_FINAL, cfg_.tol, cfg_.maxiter, singular_, n_fluid_, log},
+              clock_step(CK_FOLD, 0.0, kFoldCheck));
 }
 
 int PoissonEngine::read_log(SolveLog* out, int n) {
@@ -2146,8 +2187,10 @@ int PoissonEngine::read_log(SolveLog* out, int n) {
 }
 
 int PoissonEngine::scatter_pressure(double* p, int ldp) {
-  hipLaunchKernelGGL(k_scatter, tiles_for(op_.nx + 2, op_.ny + 2), dim3(kBlock), 0, s_, op_, x_,
-                     (const SolveState*)st_, p, ldp, (double*)nullptr);
+  // algorithmic bytes: x, flags; x (normalised guess), p
+  const LaunchClock clk = clock_step(CK_SCATTER, (double)n_ * (8 + 1 + 8 + 8));
+  PCFD_LAUNCH_TIMED(clk, k_scatter, tiles_for(op_.nx + 2, op_.ny + 2), dim3(kBlock), 0, s_, op_,
+                    x_, (const SolveState*)st_, p, ldp, (double*)nullptr);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
@@ -2204,10 +2247,15 @@ int PoissonEngine::apply_operator(const double* x, double* y) {
 // solvable set.
 int PoissonEngine::start(int mode, double* r) {
   const int mb = multi_blocks(op_.nx, op_.ny);
-  hipLaunchKernelGGL(k_init, dim3(mb), dim3(kBlock), 0, s_, op_, b_, (const double*)x_, r, mode,
-                     (const SolveState*)st_, partial_);
+  // algorithmic bytes: b, x, flags; b, r
+  const LaunchClock clk = clock_step(CK_INIT, (double)n_ * (8 + 8 + 1 + 8 + (r ? 8 : 0)));
+  const PrepFold pf{prep_pending_ ? prep_partial_ : nullptr, mb, singular_, n_fluid_};
+  prep_pending_ = false;
+  PCFD_LAUNCH_TIMED(clk, k_init, dim3(mb), dim3(kBlock), 0, s_, op_, b_, (const double*)x_, r,
+                    mode, st_, partial_, pf);
   PCFD_LAUNCH_CHECK();
-  return fold(s_, partial_, mb, EpiInit{st_, mode, cfg_.tol, cfg_.maxiter, n_proj_});
+  return fold(s_, partial_, mb, EpiInit{st_, mode, cfg_.tol, cfg_.maxiter, n_proj_},
+              clock_step(CK_FOLD, 0.0, kFoldInit));
 }
 
 int PoissonEngine::run_cg() {

@@ -239,13 +239,27 @@ class PoissonEngine {
   // slot = kind * kClockLevels + level (ClockKind); clock_read stops it and
   // reports per slot the launches, mean duration and algorithmic bytes per
   // launch.  Iterations enqueued past convergence (no-ops) are dropped.
+  // The step's other kernels (predictors, boundary passes, right-hand side,
+  // start, final check, scatter, correction, stage combination,
+  // diagnostics) are filed under their own kinds at level 0; the solve's
+  // once-per-solve folds under CK_FOLD at levels kFoldPrepare.. .
   enum ClockKind : int {
     CK_UPDATE = 0, CK_DIR, CK_FINE_DOWN, CK_FINE_UP, CK_CLASS_DOWN, CK_CLASS_UP,
     CK_ROWS_SMOOTH0, CK_ROWS_SMOOTH, CK_ROWS_RES, CK_ROWS_PROLONG, CK_RESTRICT, CK_DENSE,
-    CK_CHAIN, CK_FOLD, kClockKinds
+    CK_CHAIN, CK_FOLD, CK_PREDICTOR, CK_PREDICTOR_COMB, CK_WALLS, CK_BPASS, CK_PREPARE, CK_INIT,
+    CK_CHECK, CK_SCATTER, CK_CORRECT, CK_COMBINE, CK_DIAG, kClockKinds
   };
+  enum FoldLevel : int { kFoldPrepare = 3, kFoldInit, kFoldCheck, kFoldDiag };
   static constexpr int kClockLevels = 16, kClockSlots = kClockKinds * kClockLevels;
   int clock_start(int max_launches);
+  // A clock for one launch outside the iteration loop (never trimmed); no
+  // clock when the live clock is off.
+  LaunchClock clock_step(int kind, double bytes, int level = 0);
+  // The same as a ClockSource for launchers that issue several kernels:
+  // walls and boundary-pass launches (kClockWalls, kClockBoundaryPass).
+  ClockSource boundary_clocks() {
+    return clock_on_ ? ClockSource{&PoissonEngine::boundary_clock, this} : ClockSource{};
+  }
   int clock_read(int32_t* launches, double* avg_ms, double* bytes);
   // Per level: nx, ny, fused, tile x, tile y (see pcfd_mg_plan).
   int plan(int32_t* out, int cap) const {
@@ -278,6 +292,11 @@ class PoissonEngine {
   int levels() const { return (int)lv64_.size(); }
 
   double *partial_ = nullptr;  // per-tile partial sums of the two-launch reductions
+  // The right-hand side pass's partials (sum of b), folded by the solve
+  // start that follows it (prep_pending_): a buffer of their own, since
+  // partial_ may be reallocated in between.
+  double* prep_partial_ = nullptr;
+  bool prep_pending_ = false;
   size_t partial_cap_ = 0;     // capacity in doubles
   int partial_gen_ = 0;        // bumped when partial_ moves (captured graphs hold it)
   unsigned int* counter_ = nullptr;
@@ -338,6 +357,7 @@ class PoissonEngine {
   std::vector<ClockEntry> clock_log_;
   int clock_iter_ = 0;  // iteration tag of the launches being enqueued
   LaunchClock clock_next(int kind, int level, double bytes);
+  static LaunchClock boundary_clock(void* self, int kind, double bytes);
   void clock_free();
   // Launches of iterations past the solve's `iters` (tagged after `it0`) ran
   // as no-ops: drop them.

@@ -726,52 +726,55 @@ int launch_momentum_rhs(hipStream_t s, const double* u, const double* v, const M
 }
 
 int launch_predictor(hipStream_t s, const double* u, const double* v, const MomParams& P,
-                     const UpdRange& R, const StepScalars& T, double* us, double* vs) {
+                     const UpdRange& R, const StepScalars& T, double* us, double* vs,
+                     const LaunchClock& clk) {
   const dim3 g = grid_for(P.nx + 3, P.ny + 3);
   const dim3 gr = grid_rows(P.nx + 3, P.ny + 3, kPredRows);
   const PlainLoad U{u, P.ldu}, V{v, P.ldv};
   if (P.g.hx)
-    hipLaunchKernelGGL((k_predictor<true, false>), g, dim3(kBlock), 0, s, u, v, P, R, T, us, vs);
+    PCFD_LAUNCH_TIMED(clk, (k_predictor<true, false>), g, dim3(kBlock), 0, s, u, v, P, R, T, us,
+                      vs);
   else if (P.nuc)
-    hipLaunchKernelGGL((k_predictor_rows<false, true, kPredRows, PlainLoad>), gr, dim3(kBlock), 0, s, U, V, P,
-                       R, T, us, vs);
+    PCFD_LAUNCH_TIMED(clk, (k_predictor_rows<false, true, kPredRows, PlainLoad>), gr,
+                      dim3(kBlock), 0, s, U, V, P, R, T, us, vs);
   else if (P.pow2)
-    hipLaunchKernelGGL((k_predictor_rows<true, false, kPredRows, PlainLoad>), gr, dim3(kBlock), 0, s, U, V, P,
-                       R, T, us, vs);
+    PCFD_LAUNCH_TIMED(clk, (k_predictor_rows<true, false, kPredRows, PlainLoad>), gr,
+                      dim3(kBlock), 0, s, U, V, P, R, T, us, vs);
   else
-    hipLaunchKernelGGL((k_predictor_rows<false, false, kPredRows, PlainLoad>), gr, dim3(kBlock), 0, s, U, V,
-                       P, R, T, us, vs);
+    PCFD_LAUNCH_TIMED(clk, (k_predictor_rows<false, false, kPredRows, PlainLoad>), gr,
+                      dim3(kBlock), 0, s, U, V, P, R, T, us, vs);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
 
 int launch_predictor_comb(hipStream_t s, const double* u0, const double* ue, const double* v0,
                           const double* ve, double ca, double cb, const MomParams& P,
-                          const UpdRange& R, const StepScalars& T, double* us, double* vs) {
+                          const UpdRange& R, const StepScalars& T, double* us, double* vs,
+                          const LaunchClock& clk) {
   const dim3 g = grid_for(P.nx + 3, P.ny + 3);
   const dim3 gr = grid_rows(P.nx + 3, P.ny + 3, kPredRowsComb);
   const CombLoad U{u0, ue, P.ldu, ca, cb}, V{v0, ve, P.ldv, ca, cb};
   if (P.g.hx)
-    hipLaunchKernelGGL((k_predictor_comb<true, false>), g, dim3(kBlock), 0, s, u0, ue, v0, ve, ca,
-                       cb, P, R, T, us, vs);
+    PCFD_LAUNCH_TIMED(clk, (k_predictor_comb<true, false>), g, dim3(kBlock), 0, s, u0, ue, v0, ve,
+                      ca, cb, P, R, T, us, vs);
   else if (P.nuc)
-    hipLaunchKernelGGL((k_predictor_rows<false, true, kPredRowsComb, CombLoad>), gr, dim3(kBlock), 0, s, U, V, P,
-                       R, T, us, vs);
+    PCFD_LAUNCH_TIMED(clk, (k_predictor_rows<false, true, kPredRowsComb, CombLoad>), gr,
+                      dim3(kBlock), 0, s, U, V, P, R, T, us, vs);
   else if (P.pow2)
-    hipLaunchKernelGGL((k_predictor_rows<true, false, kPredRowsComb, CombLoad>), gr, dim3(kBlock), 0, s, U, V, P,
-                       R, T, us, vs);
+    PCFD_LAUNCH_TIMED(clk, (k_predictor_rows<true, false, kPredRowsComb, CombLoad>), gr,
+                      dim3(kBlock), 0, s, U, V, P, R, T, us, vs);
   else
-    hipLaunchKernelGGL((k_predictor_rows<false, false, kPredRowsComb, CombLoad>), gr, dim3(kBlock), 0, s, U, V,
-                       P, R, T, us, vs);
+    PCFD_LAUNCH_TIMED(clk, (k_predictor_rows<false, false, kPredRowsComb, CombLoad>), gr,
+                      dim3(kBlock), 0, s, U, V, P, R, T, us, vs);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
 
 int launch_combine(hipStream_t s, const double* u0, const double* ue, const double* v0,
                    const double* ve, double ca, double cb, int nx, int ny, int ldu, int ldv,
-                   double* uo, double* vo) {
-  hipLaunchKernelGGL(k_combine, grid_for(nx + 3, ny + 3), dim3(kBlock), 0, s, u0, ue, v0, ve, ca,
-                     cb, nx, ny, ldu, ldv, uo, vo);
+                   double* uo, double* vo, const LaunchClock& clk) {
+  PCFD_LAUNCH_TIMED(clk, k_combine, grid_for(nx + 3, ny + 3), dim3(kBlock), 0, s, u0, ue, v0, ve,
+                    ca, cb, nx, ny, ldu, ldv, uo, vo);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
@@ -787,7 +790,7 @@ int launch_correct(hipStream_t s, const double* p, int ldp, int nx, int ny, int 
 
 int launch_correct_ghosts(hipStream_t s, double* p, int ldp, const BCParams& B, int ldu, int ldv,
                           double dx, double dy, const Metrics& g, const UpdRange& R,
-                          const StepScalars& T, double* u, double* v) {
+                          const StepScalars& T, double* u, double* v, const LaunchClock& clk) {
   PGhost G{B.nx, B.ny, ldp, {}, {}, B.periodic_x, B.periodic_y};
   for (int w = 0; w < 4; ++w) {
     G.kind[w] = B.kind[w];
@@ -796,11 +799,11 @@ int launch_correct_ghosts(hipStream_t s, double* p, int ldp, const BCParams& B, 
   const bool p2 = is_pow2(dx) && is_pow2(dy);
   const dim3 grid = grid_for(B.nx + 3, B.ny + 3);
   if (p2)
-    hipLaunchKernelGGL(k_correct_ghosts<true>, grid, dim3(kBlock), 0, s, p, G, ldu, ldv, dx, dy,
-                       1.0 / dx, 1.0 / dy, g, R, T, u, v);
+    PCFD_LAUNCH_TIMED(clk, k_correct_ghosts<true>, grid, dim3(kBlock), 0, s, p, G, ldu, ldv, dx,
+                      dy, 1.0 / dx, 1.0 / dy, g, R, T, u, v);
   else
-    hipLaunchKernelGGL(k_correct_ghosts<false>, grid, dim3(kBlock), 0, s, p, G, ldu, ldv, dx, dy,
-                       1.0 / dx, 1.0 / dy, g, R, T, u, v);
+    PCFD_LAUNCH_TIMED(clk, k_correct_ghosts<false>, grid, dim3(kBlock), 0, s, p, G, ldu, ldv, dx,
+                      dy, 1.0 / dx, 1.0 / dy, g, R, T, u, v);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
@@ -818,7 +821,8 @@ size_t diagnostics_partials(int nx, int ny) { return (size_t)diag_blocks(nx, ny)
 int launch_diagnostics(hipStream_t s, const double* u, const double* v, const double* p,
                        const double* up, const double* vp, const uint8_t* solid, int nx, int ny,
                        int ldu, int ldv, int ldp, double dx, double dy, const Metrics& g,
-                       double* partial, unsigned int* counter, double* out) {
+                       double* partial, unsigned int* counter, double* out,
+                       const LaunchClock& clk) {
   (void)counter;
   const int nb = diag_blocks(nx, ny);
   // Stand-ins for absent inputs: u's allocation ((nx+3) x ldu doubles,
@@ -828,10 +832,10 @@ int launch_diagnostics(hipStream_t s, const double* u, const double* v, const do
   const bool prev = up && vp;
   const int have = (p ? kHaveP : 0) | (prev ? kHavePrev : 0) | (solid ? kHaveSolid : 0) |
                    (g.hx ? kHaveHx : 0) | (g.hy ? kHaveHy : 0);
-  hipLaunchKernelGGL(k_diagnostics, dim3(nb), dim3(kBlock), 0, s, u, v, p ? p : u,
-                     prev ? up : u, prev ? vp : v, solid ? solid : (const uint8_t*)u,
-                     g.hx ? g.hx : u, g.hy ? g.hy : u, have, nx, ny, ldu, ldv, p ? ldp : ldu, dx,
-                     dy, partial);
+  PCFD_LAUNCH_TIMED(clk, k_diagnostics, dim3(nb), dim3(kBlock), 0, s, u, v, p ? p : u,
+                    prev ? up : u, prev ? vp : v, solid ? solid : (const uint8_t*)u,
+                    g.hx ? g.hx : u, g.hy ? g.hy : u, have, nx, ny, ldu, ldv, p ? ldp : ldu, dx,
+                    dy, partial);
   PCFD_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_fold<EpiDiag>, dim3(1), dim3(kBlock), 0, s, (const double*)partial, nb,
                      EpiDiag{out});
