@@ -2218,14 +2218,17 @@ int PoissonEngine::apply_operator(const double* x, double* y) {
 // drains the stream and leaves the device idle until the next launch.  PER
 // is the number of iterations one body advances; MAXB bounds the bodies.
 // Under a budget (a captured solve) exactly enough bodies for that many
-// iterations are enqueued and nothing is polled.
-#define PCFD_ITERATE(PER, MAXB, ...)                                          \
+// iterations are enqueued and nothing is polled.  HOOK runs after each chunk,
+// before its poll (and after a budget's bodies): it enqueues what the state
+// the host reads still waits for.
+#define PCFD_ITERATE_HOOK(PER, MAXB, HOOK, ...)                               \
   do {                                                                        \
     if (budget_ > 0) {                                                        \
       const int _nb = std::min((MAXB), (budget_ + (PER) - 1) / (PER));        \
       for (int _b = 0; _b < _nb; ++_b) {                                      \
         __VA_ARGS__;                                                          \
       }                                                                       \
+      if (int _hrc = (HOOK)) return _hrc;                                     \
       last_run_ = budget_;                                                    \
       break;                                                                  \
     }                                                                         \
@@ -2235,6 +2238,7 @@ int PoissonEngine::apply_operator(const double* x, double* y) {
       for (int _c = 0; _c < _chunk && _bodies < (MAXB); ++_c, ++_bodies) {    \
         __VA_ARGS__;                                                          \
       }                                                                       \
+      if (int _hrc = (HOOK)) return _hrc;                                     \
       _rc = poll(&_it, &_done, &_conv);                                       \
       if (_rc) return _rc;                                                    \
       _chunk = std::max(1, std::min(64, _it / (8 * (PER))));                  \
@@ -2242,6 +2246,7 @@ int PoissonEngine::apply_operator(const double* x, double* y) {
     last_run_ = _it;                                                          \
     if (_conv) predicted_ = _it;                                              \
   } while (0)
+#define PCFD_ITERATE(PER, MAXB, ...) PCFD_ITERATE_HOOK(PER, MAXB, kOk, __VA_ARGS__)
 
 // The right-hand side pass (k_prepare) has zeroed the guess off the
 // solvable set.
@@ -2681,6 +2686,8 @@ int PoissonEngine::enqueue_vcycle_t(int mode, int reduce, bool defer_fold) {
   const bool krylov = mode == VC_KRYLOV;
   LevelBuf& L0 = lv[0];
   (void)nu;
+  if (!(krylov && L0.stream_rows && nl > 1))
+    if (int rc = flush_update_fold()) return rc;  // no streaming leg to fold it
   T* out = krylov ? (T*)L0.y : (T*)L0.x;
   const int nt0 = ntiles(op_.nx, op_.ny);
   if (nl == 1) {
@@ -2702,9 +2709,12 @@ int PoissonEngine::enqueue_vcycle_t(int mode, int reduce, bool defer_fold) {
   if (krylov && L0.stream_rows) {
     LevelBuf& C = lv[1];
     const double bytes_down = n0 * (8 + 1 + 2 * ts) + C.n * ts;
+    // the streaming down leg folds a pending update's partials itself
+    const UpdateFold uf{upd_partial_, upd_rows_, upd_iter_, cfg_.maxiter, n_proj_};
+    upd_rows_ = 0;
     if (int rc = launch_fine_down_stream<T>(s_, op_, L0.stream_rows, (T)L0.weight,
                                             (const double*)r_, (T*)L0.b, (T*)L0.t, (T*)C.b,
-                                            L0.fx, L0.fy, C.nx, C.ny, cst,
+                                            L0.fx, L0.fy, C.nx, C.ny, st_, uf,
                                             clock_next(CK_FINE_DOWN, 0, bytes_down)))
       return rc;
     if (int rc = coarse_legs<T>(lv, 1, cst)) return rc;
@@ -2771,7 +2781,9 @@ int PoissonEngine::run_mgcg() {
   double* p_new = pb_;
   const bool f32 = cfg_.precision == 1;
   const int it0 = clock_iter_;
-  PCFD_ITERATE(1, cfg_.maxiter, {
+  int n_upd = 0;  // updates enqueued in this solve (their iteration counts)
+  upd_rows_ = 0;
+  PCFD_ITERATE_HOOK(1, cfg_.maxiter, flush_update_fold(), {
     ++clock_iter_;
     if (int rc = enqueue_vcycle(VC_KRYLOV, SR_MGCG, true)) return rc;
     // direction (folds the cycle's sums itself): z (cycle precision), p_old,
@@ -2787,13 +2799,26 @@ int PoissonEngine::run_mgcg() {
     const LaunchClock cu = clock_next(CK_UPDATE, 0, (double)n_ * (6 * 8 + 1));  // x, p, r, q, flags; x, r
     PCFD_LAUNCH_TIMED(cu, k_cg_update, dim3(ub), dim3(kBlock), 0, s_, op_, x_, r_, p_new, q_,
                       (const double*)partial_ + doff, dir_tile_.ntiles(), partial_ + uoff, st_);
-    if (int rc = fold(s_, partial_ + uoff, ub, EpiUpdate{st_, (int)IM_MGCG, cfg_.maxiter, n_proj_},
-                      clock_next(CK_FOLD, 2, ub * 16.0)))
-      return rc;
+    // its partials are folded by the next cycle's down leg, or by
+    // flush_update_fold before the host reads the state
+    upd_partial_ = partial_ + uoff;
+    upd_rows_ = ub;
+    upd_iter_ = ++n_upd;
     std::swap(p_old, p_new);
   });
   clock_trim(it0, last_run_);
   return kOk;
+}
+
+// The last multigrid-CG update's partials folded by a launch of its own
+// (EpiUpdate), when no streaming down leg follows to fold them.
+int PoissonEngine::flush_update_fold() {
+  if (upd_rows_ <= 0) return kOk;
+  const int rows = upd_rows_;
+  upd_rows_ = 0;
+  return fold(s_, (double*)upd_partial_, rows,
+              EpiUpdate{st_, (int)IM_MGCG, cfg_.maxiter, n_proj_},
+              clock_next(CK_FOLD, 2, rows * 16.0));
 }
 
 int PoissonEngine::run_multigrid() {

@@ -297,6 +297,12 @@ class PoissonEngine {
   // partial_ may be reallocated in between.
   double* prep_partial_ = nullptr;
   bool prep_pending_ = false;
+  // Multigrid-CG: the last update's partials not yet folded (rows > 0), for
+  // the next cycle's streaming down leg to fold (UpdateFold), with the
+  // iteration count that update completes.
+  const double* upd_partial_ = nullptr;
+  int upd_rows_ = 0, upd_iter_ = 0;
+  int flush_update_fold();
   size_t partial_cap_ = 0;     // capacity in doubles
   int partial_gen_ = 0;        // bumped when partial_ moves (captured graphs hold it)
   unsigned int* counter_ = nullptr;

@@ -618,14 +618,11 @@ template <typename T, int FX, int FY>
 __global__ void __launch_bounds__(kBlock)
 k_fine_down_stream(FineOp A, int strips, int L, T w, const double* __restrict__ r,
                    T* __restrict__ b_out, T* __restrict__ x_out, T* __restrict__ bc, int cnx,
-                   int cny, const SolveState* __restrict__ st) {
+                   int cny, SolveState* __restrict__ st, UpdateFold uf) {
   if (st->done) return;
   __shared__ T td[128], ti[128];
   fine_tables<T>(A, td, ti);
-  __syncthreads();
   const StreamPos P = stream_pos(A, strips, L);
-  if (!P.active) return;  // whole wavefront: no shuffles are left half-done
-  const double mr = st->mean_r;
   const T cx = (T)A.cx, cy = (T)A.cy;
   const int p_begin = P.i0 - 2, p_end = P.i0 + L + 2;  // L % 4 == 0: a whole number of batches
   // prefetch slots
@@ -644,8 +641,53 @@ k_fine_down_stream(FineOp A, int strips, int L, T w, const double* __restrict__ 
       pf_i[slot] = ir;
     }
   };
+  if (P.active) {
 #pragma unroll
-  for (int u = 0; u < kStreamPrefetch; ++u) issue(u, p_begin + u);
+    for (int u = 0; u < kStreamPrefetch; ++u) issue(u, p_begin + u);
+  }
+  // With uf.rows > 0 every workgroup folds the previous update's partials
+  // itself, in k_fold's fixed order, while its first loads are in flight:
+  // the update epilogue's arithmetic (iteration count, r.r, mean of r, the
+  // stopping test), bit for bit, without the fold launch.  Workgroup 0
+  // records it in the state.  The count comes in uf.iter; the one field read
+  // above that this launch may write is `done`, and a workgroup that sees it
+  // set here would have stopped after its own fold anyway.
+  double mr;
+  if (uf.rows > 0) {
+    double v[2];
+    const int op2[2] = {kSum, kSum};
+    fold_partials<2>(v, op2, uf.partial, uf.rows);
+    __shared__ double s_mr;
+    __shared__ int s_done;
+    if (threadIdx.x == 0) {
+      const double m = (uf.n_proj > 0) ? v[1] / (double)uf.n_proj : 0.0;
+      int done = 0, conv = 0;
+      if (sqrt(v[0]) < st->atol) {
+        done = 1;
+        conv = 1;
+      } else if (uf.iter >= uf.maxiter) {
+        done = 1;
+      }
+      s_mr = m;
+      s_done = done;
+      if (blockIdx.x == 0) {
+        st->iter = uf.iter;
+        st->rr = v[0];
+        st->mean_r = m;
+        if (done) {
+          st->done = 1;
+          st->converged = conv;
+        }
+      }
+    }
+    __syncthreads();
+    if (s_done) return;
+    mr = s_mr;
+  } else {
+    __syncthreads();
+    mr = st->mean_r;
+  }
+  if (!P.active) return;  // whole wavefront: no shuffles are left half-done
   // pipeline registers: rows p-3 .. p
   T x0 = 0, x1 = 0, b1 = 0, d0 = 1, d1 = 1, di1 = 1, res0 = 0, t0 = 0, t1 = 0, acc = 0;
   bool s0 = false, s1 = false;
@@ -1425,14 +1467,15 @@ bool stream_supported(const FineOp& A, int nu, int fx, int fy) {
 template <typename T>
 int launch_fine_down_stream(hipStream_t s, const FineOp& A, int rows, T w, const double* r,
                             T* b_out, T* x_out, T* bc, int fx, int fy, int cnx, int cny,
-                            const SolveState* st, const LaunchClock& clk) {
+                            SolveState* st, const UpdateFold& uf, const LaunchClock& clk) {
   if (rows < 4 || rows % 4 || (fx == 2 && rows % 2)) {
     set_last_error("streaming down leg: %d rows per wavefront (need a multiple of 4)", rows);
     return kErrBadArg;
   }
   const dim3 grid(stream_blocks(A.nx, A.ny, rows));
   const int strips = stream_strips(A.ny);
-  PCFD_STREAM_DISPATCH(k_fine_down_stream, A, strips, rows, w, r, b_out, x_out, bc, cnx, cny, st);
+  PCFD_STREAM_DISPATCH(k_fine_down_stream, A, strips, rows, w, r, b_out, x_out, bc, cnx, cny, st,
+                       uf);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
@@ -1568,8 +1611,8 @@ bool choose_tile(int nx, int ny, int rxs, int rys, int halo_mult, int fx, int fy
   template int launch_dense_apply<T>(hipStream_t, const double*, int, const T*, T*,            \
                                      const SolveState*, const LaunchClock&);                  \
   template int launch_fine_down_stream<T>(hipStream_t, const FineOp&, int, T, const double*,   \
-                                          T*, T*, T*, int, int, int, int, const SolveState*,  \
-                                          const LaunchClock&);                                \
+                                          T*, T*, T*, int, int, int, int, SolveState*,        \
+                                          const UpdateFold&, const LaunchClock&);             \
   template int launch_fine_up_stream<T>(hipStream_t, const FineOp&, int, T, const T*, const T*, \
                                         const T*, int, int, int, T*, int, double*,            \
                                         const SolveState*, const LaunchClock&);               \

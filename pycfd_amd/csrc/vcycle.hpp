@@ -81,10 +81,22 @@ int launch_fine_up(hipStream_t s, const FineOp& A, const FusedTile& ft, T w, int
 // the up leg writes one row of partials per workgroup (stream_blocks).
 bool stream_supported(const FineOp& A, int nu, int fx, int fy);
 int stream_blocks(int nx, int ny, int rows);
+// The multigrid-CG update's partials (r.r, sum r) of the previous iteration,
+// left for the next streaming down leg to fold (rows = 0: the state is
+// current).  `iter` is the iteration count once that update is counted.
+struct UpdateFold {
+  const double* partial = nullptr;
+  int rows = 0;
+  int iter = 0;
+  int maxiter = 0;
+  int n_proj = 0;
+};
+
 template <typename T>
 int launch_fine_down_stream(hipStream_t s, const FineOp& A, int rows, T w, const double* r,
                             T* b_out, T* x_out, T* bc, int fx, int fy, int cnx, int cny,
-                            const SolveState* st, const LaunchClock& clk = LaunchClock{});
+                            SolveState* st, const UpdateFold& uf = UpdateFold{},
+                            const LaunchClock& clk = LaunchClock{});
 template <typename T>
 int launch_fine_up_stream(hipStream_t s, const FineOp& A, int rows, T w, const T* x_pre,
                           const T* b0, const T* xc, int fx, int fy, int cny, T* z, int reduce,
