@@ -65,7 +65,7 @@ def kernel_name(entry: dict, precision: str, plan: list[dict]) -> str:
     kind, level = entry["kind"], entry["level"]
     if kind == "k_fold":
         return f"k_fold[{('cycle', 'alpha', 'update', 'prepare', 'init', 'check')[level]}]"
-    if kind in STEP_KERNELS:
+    if kind in STEP_KERNELS or kind == "k_jpcg_small":
         return kind
     if kind in ("k_fine_down", "k_fine_up"):
         stream = plan and plan[0]["mode"] == "stream"

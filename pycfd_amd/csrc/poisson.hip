@@ -66,6 +66,7 @@ struct EpiInit {  // ||b||, ||r||, first rho, termination at iteration 0
     int done = 0, conv = 0;
     if (mode == IM_JPCG || mode == IM_MGCG) {
       st->rho = v[2];
+      if (mode == IM_JPCG) st->rho_hist[0] = v[2];  // rho of iteration 0 (k_jpcg_dir)
       st->mean_r = (n_proj > 0) ? v[3] / (double)n_proj : 0.0;
       if (bn == 0.0 || sqrt(v[1]) < st->atol) {
         done = 1;
@@ -86,14 +87,6 @@ struct EpiInit {  // ||b||, ||r||, first rho, termination at iteration 0
     st->done = done;
     st->converged = conv;
   }
-};
-
-struct EpiAlpha {  // alpha = rho / p.q
-  static constexpr int K = 1;
-  SolveState* st;
-  __device__ void ops(int (&o)[K]) const { o[0] = kSum; }
-  __device__ bool active() const { return !st->done; }
-  __device__ void operator()(const double (&v)[K]) const { st->alpha = st->rho / v[0]; }
 };
 
 struct EpiUpdate {  // after x += alpha p, r -= alpha q (scipy cg loop)
@@ -117,6 +110,29 @@ struct EpiUpdate {  // after x += alpha p, r -= alpha q (scipy cg loop)
       st->done = 1;
       st->converged = 1;
     } else if (it >= maxiter) {
+      st->done = 1;
+    }
+  }
+};
+
+struct EpiUpdateJ {  // Jacobi-CG: the last update's partials when no direction kernel follows
+  static constexpr int K = 2;
+  SolveState* st;
+  int k, maxiter;  // updates done (the iteration count once this one is counted)
+  __device__ void ops(int (&o)[K]) const { o[0] = o[1] = kSum; }
+  __device__ bool active() const { return !st->done; }
+  __device__ void operator()(const double (&v)[K]) const {
+    const double rho_prev = st->rho_hist[(k - 1) & 1];
+    st->iter = k;
+    st->rr = v[0];
+    st->rho_prev = rho_prev;
+    st->rho = v[1];
+    st->beta = v[1] / rho_prev;
+    st->rho_hist[k & 1] = v[1];
+    if (sqrt(v[0]) < st->atol) {
+      st->done = 1;
+      st->converged = 1;
+    } else if (k >= maxiter) {
       st->done = 1;
     }
   }
@@ -409,16 +425,70 @@ k_init(FineOp A, double* __restrict__ b, const double* __restrict__ x, double* _
 // ----------------------------------------------------------------------------
 // Jacobi-preconditioned CG (scipy.sparse.linalg.cg iteration)
 // ----------------------------------------------------------------------------
-// p = z + beta*p_old (z = r/d), q = A p, and p.q.
+// Jacobi-CG state across launches: rho of iteration k (r.z after k updates)
+// sits in st->rho_hist[k & 1]; the start writes rho_hist[0].  Iteration k's
+// direction kernel reads rho_hist[(k - 1) & 1] and writes rho_hist[k & 1],
+// so no workgroup reads what its own launch writes.
+//
+// The previous update's partials (r.r, r.z), folded by the next direction
+// kernel (rows > 0), or already folded by EpiUpdateJ (rows = 0).  `k` is the
+// iteration the direction kernel starts (updates done so far).
+struct JpcgFold {
+  const double* partial;
+  int rows, k, maxiter;
+};
+
+// p = z + beta*p_old (z = r/d), q = A p, and p.q.  For k > 0 with rows > 0
+// every workgroup first folds the previous update's partials itself in
+// k_fold's fixed order: the update epilogue's arithmetic (iteration count,
+// r.r, rho, beta = rho_k / rho_(k-1), the stopping test), bit for bit,
+// without its launch; workgroup 0 records it.
 __global__ void __launch_bounds__(kBlock)
 k_jpcg_dir(FineOp A, const double* __restrict__ r, const double* __restrict__ p_old,
            double* __restrict__ p_new, double* __restrict__ q, double* partial,
-           const SolveState* st) {
+           SolveState* st, JpcgFold jf) {
   if (st->done) return;
   Tile2D t;
   double s[1] = {0.0};
-  const bool first = st->iter == 0;
-  const double beta = st->beta;
+  const bool first = jf.k == 0;
+  double beta = 0.0;
+  if (!first && jf.rows > 0) {
+    double v[2];
+    const int op2[2] = {kSum, kSum};
+    fold_partials<2>(v, op2, jf.partial, jf.rows);
+    __shared__ double s_beta;
+    __shared__ int s_done;
+    if (threadIdx.x == 0) {
+      const double rho_prev = st->rho_hist[(jf.k - 1) & 1];
+      const double rho = v[1];
+      int done = 0, conv = 0;
+      if (sqrt(v[0]) < st->atol) {
+        done = 1;
+        conv = 1;
+      } else if (jf.k >= jf.maxiter) {
+        done = 1;
+      }
+      s_beta = rho / rho_prev;
+      s_done = done;
+      if (blockIdx.x == 0) {
+        st->iter = jf.k;
+        st->rr = v[0];
+        st->rho_prev = rho_prev;
+        st->rho = rho;
+        st->beta = s_beta;
+        st->rho_hist[jf.k & 1] = rho;
+        if (done) {
+          st->done = 1;
+          st->converged = conv;
+        }
+      }
+    }
+    __syncthreads();
+    if (s_done) return;
+    beta = s_beta;
+  } else if (!first) {
+    beta = st->rho_hist[jf.k & 1] / st->rho_hist[(jf.k - 1) & 1];
+  }
   if (cell_of_tile(A.nx, A.ny, t)) {
     const int i = t.i, j = t.j, k = i * A.ny + j;
     const uint8_t f = A.flags[k];
@@ -440,7 +510,44 @@ k_jpcg_dir(FineOp A, const double* __restrict__ r, const double* __restrict__ p_
   block_partials<1>(s, ops, partial, t.tile);
 }
 
-// x += alpha p, r -= alpha q; partials ||r||^2 and r.(r/d) or sum(r).
+// Jacobi-CG update with the step length folded in: every workgroup first
+// issues its loads, then folds the direction kernel's p.q partials itself in
+// k_fold's fixed order (alpha = rho_k / p.q, the bits a fold launch would
+// give), then x += alpha p, r -= alpha q, partials r.r and r.(r/d).
+__global__ void __launch_bounds__(kBlock)
+k_jpcg_update_f(FineOp A, double* __restrict__ x, double* __restrict__ r,
+                const double* __restrict__ p, const double* __restrict__ q,
+                const double* __restrict__ dir_partial, int dir_tiles, int k_iter,
+                double* partial, SolveState* st) {
+  if (st->done) return;
+  Tile2D t;
+  const bool in = cell_of_tile(A.nx, A.ny, t);
+  const int i = in ? t.i : 0, j = in ? t.j : 0, k = i * A.ny + j;
+  const uint8_t f = A.flags[k];
+  const double xv = x[k], pv = p[k], rv = r[k], qv = q[k];
+  double pq[1];
+  const int op1[1] = {kSum};
+  fold_partials<1>(pq, op1, dir_partial, dir_tiles);
+  __shared__ double s_alpha;
+  if (threadIdx.x == 0) s_alpha = st->rho_hist[k_iter & 1] / pq[0];
+  __syncthreads();
+  const double alpha = s_alpha;
+  if (t.tile == 0 && threadIdx.x == 0) st->alpha = alpha;
+  double s[2] = {0.0, 0.0};
+  if (in) {
+    const double xk = xv + alpha * pv;
+    const double rk = rv - alpha * qv;
+    x[k] = xk;
+    r[k] = rk;
+    s[0] = rk * rk;
+    s[1] = rk * (rk / fine_diag(A, f, i, j));
+  }
+  const int ops[2] = {kSum, kSum};
+  block_partials<2>(s, ops, partial, t.tile);
+}
+
+// The update with alpha from the state (no fold of its own): what
+// pcfd_time_kernel times as the plain CG update.
 __global__ void __launch_bounds__(kBlock)
 k_jpcg_update(FineOp A, double* __restrict__ x, double* __restrict__ r,
             const double* __restrict__ p, const double* __restrict__ q, int mode,
@@ -2263,20 +2370,227 @@ int PoissonEngine::start(int mode, double* r) {
               clock_step(CK_FOLD, 0.0, kFoldInit));
 }
 
+// ----------------------------------------------------------------------------
+// Jacobi-CG of a small grid in one workgroup (scipy.sparse.linalg.cg with
+// M = 1/diag, pressure.py:366-398, the same iteration as k_jpcg_dir /
+// k_jpcg_update_f and their epilogues).  On a tiny grid the multi-launch
+// iteration is bound by its four dependent launches per iteration; here the
+// whole solve is one launch on one CU.  1024 threads hold x and r of CPT
+// cells each in registers; the direction p and the flag bytes sit in LDS,
+// where the operator product reads the neighbours (q = A p is formed twice
+// per iteration, for p.q and for the update, rather than held in registers).
+// Each inner product is a wavefront tree plus a fixed-order sum of the 16
+// wavefront totals that every thread forms itself (one barrier).  The solve
+// iterates until the recursive residual meets the stopping test or maxiter,
+// then writes x, r and the state.  One CU's FP64 rate bounds it: measured on
+// MI355X, 5.9 us per iteration at 32 x 32 against 12.5 for the two-launch
+// iteration below, but 13.8 against 12.0 at 64 x 64, so it serves grids of at
+// most 2048 cells.
+// ----------------------------------------------------------------------------
+constexpr int kSmallCgThreads = 1024;
+constexpr int kSmallCgWaves = kSmallCgThreads / kWave;
+constexpr int kSmallCgCells = 2 * kSmallCgThreads;
+
+template <int K>
+__device__ __forceinline__ void small_cg_sum(double (&v)[K], double (*slot)[kSmallCgWaves]) {
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const double w = wave_sum(v[k]);
+    if (lane == 0) slot[k][wid] = w;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double a = slot[k][0];
+    for (int w = 1; w < kSmallCgWaves; ++w) a += slot[k][w];
+    v[k] = a;
+  }
+}
+
+// Cell m of this thread: c = m * 1024 + tid, (i, j) stepped from cell m - 1.
+struct SmallCell {
+  int c, i, j;
+};
+
+template <int CPT>
+__global__ void __launch_bounds__(kSmallCgThreads)
+k_jpcg_small(FineOp A, double* __restrict__ x, double* __restrict__ r, int maxiter,
+             SolveState* st) {
+  __shared__ double P[CPT * kSmallCgThreads];
+  __shared__ uint8_t F[CPT * kSmallCgThreads];
+  // two slot sets: the p.q sum and the (r.r, r.z) sum of one iteration use
+  // different ones, so no sum overwrites slots another thread still reads
+  __shared__ double slots_pq[1][kSmallCgWaves];
+  __shared__ double slots_rr[2][kSmallCgWaves];
+  if (st->done) return;  // decided by the start (zero rhs, met at once, maxiter 0)
+  const int n = A.nx * A.ny, ny = A.ny;
+  const int step_i = kSmallCgThreads / ny, step_j = kSmallCgThreads - step_i * ny;
+  // The cells' indices are formed afresh in every pass from a thread index the
+  // compiler cannot see through (an empty asm on the register): hoisted out of
+  // the iteration loop, the per-cell indices and neighbours of CPT cells would
+  // stay live across it and spill.
+  auto cell = [&](int m, SmallCell& q) {  // q holds cell m - 1 (or is fresh for m = 0)
+    if (m == 0) {
+      int t = (int)threadIdx.x;
+      asm volatile("" : "+v"(t));
+      const int i0 = t / ny;
+      q = SmallCell{t, i0, t - i0 * ny};
+      return;
+    }
+    q.c += kSmallCgThreads;
+    q.i += step_i;
+    q.j += step_j;
+    if (q.j >= ny) {
+      q.j -= ny;
+      ++q.i;
+    }
+  };
+  double xv[CPT], rv[CPT];
+  {
+    SmallCell q{};
+#pragma unroll
+    for (int m = 0; m < CPT; ++m) {
+      cell(m, q);
+      const bool in = q.c < n;
+      xv[m] = in ? x[q.c] : 0.0;
+      rv[m] = in ? r[q.c] : 0.0;
+      if (in) F[q.c] = A.flags[q.c];
+    }
+  }
+  // (A p)[c] from the LDS copies, in fine_apply's term order
+  auto ap = [&](const SmallCell& q) {
+    const uint8_t f = F[q.c];
+    const FaceC fc = fine_faces(A, q.i, q.j);
+    const double d = fine_diag(A, fc, f, q.i, q.j);
+    return fine_apply<double>(fc, f, fine_neighbours(A, q.i, q.j), q.c, d,
+                              [&](int k) { return P[k]; });
+  };
+  double rho = st->rho, rho_prev = st->rho_prev, beta = 0.0, alpha = 0.0, rr = st->rr;
+  const double atol = st->atol;
+  int it = st->iter, conv = 0;
+  __syncthreads();
+  for (bool first = true;; first = false) {
+    // p = z + beta p_old, z = r / diag (k_jpcg_dir's arithmetic)
+    {
+      SmallCell q{};
+#pragma unroll
+      for (int m = 0; m < CPT; ++m) {
+        cell(m, q);
+        if (q.c >= n) continue;
+        const double z = rv[m] / fine_diag(A, F[q.c], q.i, q.j);
+        P[q.c] = first ? z : P[q.c] * beta + z;
+      }
+    }
+    __syncthreads();
+    double pq[1] = {0.0};
+    {
+      SmallCell q{};
+#pragma unroll 4
+      for (int m = 0; m < CPT; ++m) {
+        cell(m, q);
+        if (q.c < n) pq[0] += P[q.c] * ap(q);
+      }
+    }
+    small_cg_sum<1>(pq, slots_pq);
+    alpha = rho / pq[0];
+    // x += alpha p, r -= alpha q; r.r and r.(r / diag) (k_jpcg_update_f)
+    double s[2] = {0.0, 0.0};
+    {
+      SmallCell q{};
+#pragma unroll
+      for (int m = 0; m < CPT; ++m) {
+        cell(m, q);
+        if (q.c >= n) continue;
+        xv[m] = xv[m] + alpha * P[q.c];
+        rv[m] = rv[m] - alpha * ap(q);
+        s[0] += rv[m] * rv[m];
+        s[1] += rv[m] * (rv[m] / fine_diag(A, F[q.c], q.i, q.j));
+      }
+    }
+    small_cg_sum<2>(s, slots_rr);
+    // the update epilogue (EpiUpdate, Jacobi-CG)
+    ++it;
+    rr = s[0];
+    rho_prev = rho;
+    rho = s[1];
+    beta = rho / rho_prev;
+    if (sqrt(rr) < atol) {
+      conv = 1;
+      break;
+    }
+    if (it >= maxiter) break;
+  }
+  {
+    SmallCell q{};
+#pragma unroll
+    for (int m = 0; m < CPT; ++m) {
+      cell(m, q);
+      if (q.c >= n) continue;
+      x[q.c] = xv[m];
+      r[q.c] = rv[m];
+    }
+  }
+  if (threadIdx.x == 0) {
+    st->iter = it;
+    st->rr = rr;
+    st->rho = rho;
+    st->rho_prev = rho_prev;
+    st->beta = beta;
+    st->alpha = alpha;
+    st->done = 1;
+    st->converged = conv;
+  }
+}
+
+int PoissonEngine::small_cg_cells_per_thread() const {
+  const long n = (long)op_.nx * op_.ny;
+  const char* e = std::getenv("PCFD_SMALL_CG");  // "0": always the multi-launch iteration
+  if (n > kSmallCgCells || (e && e[0] == '0')) return 0;
+  int cpt = 1;
+  while ((long)cpt * kSmallCgThreads < n) cpt *= 2;
+  return cpt;
+}
+
 int PoissonEngine::run_cg() {
   const dim3 g = tiles_for(op_.nx, op_.ny);
   const int nt = ntiles(op_.nx, op_.ny);
   if (int rc = start(IM_JPCG, r_)) return rc;
+  if (const int cpt = small_cg_cells_per_thread()) {
+    const dim3 one(1), blk(kSmallCgThreads);
+    const LaunchClock clk = clock_next(CK_SMALL_CG, 0, (double)n_ * 8 * 4);  // x, r in and out
+    if (cpt == 1)
+      PCFD_LAUNCH_TIMED(clk, k_jpcg_small<1>, one, blk, 0, s_, op_, x_, r_, cfg_.maxiter, st_);
+    else
+      PCFD_LAUNCH_TIMED(clk, k_jpcg_small<2>, one, blk, 0, s_, op_, x_, r_, cfg_.maxiter, st_);
+    PCFD_LAUNCH_CHECK();
+    last_run_ = 0;
+    return kOk;
+  }
+  // Two launches per iteration: the direction kernel folds the previous
+  // update's partials, the update folds the direction's p.q; a chunk's last
+  // update is folded by EpiUpdateJ before the host reads the state.
+  const size_t uoff = (nt + 255) / 256 * 256;
+  if (int rc = ensure_partials(uoff + 2 * (size_t)nt)) return rc;
   double* p_old = pa_;
   double* p_new = pb_;
-  PCFD_ITERATE(1, cfg_.maxiter, {
+  int k = 0;             // updates enqueued
+  bool pending = false;  // the last one's partials not folded yet
+  auto flush = [&]() -> int {
+    if (!pending) return kOk;
+    pending = false;
+    return fold(s_, partial_ + uoff, nt, EpiUpdateJ{st_, k, cfg_.maxiter});
+  };
+  PCFD_ITERATE_HOOK(1, cfg_.maxiter, flush(), {
+    const JpcgFold jf{partial_ + uoff, pending ? nt : 0, k, cfg_.maxiter};
+    pending = false;
     hipLaunchKernelGGL(k_jpcg_dir, g, dim3(kBlock), 0, s_, op_, r_, p_old, p_new, q_, partial_,
-                       (const SolveState*)st_);
-    if (int rc = fold(s_, partial_, nt, EpiAlpha{st_})) return rc;
-    hipLaunchKernelGGL(k_jpcg_update, g, dim3(kBlock), 0, s_, op_, x_, r_, p_new, q_,
-                       (int)IM_JPCG, partial_, (const SolveState*)st_);
-    if (int rc = fold(s_, partial_, nt, EpiUpdate{st_, (int)IM_JPCG, cfg_.maxiter, n_proj_}))
-      return rc;
+                       st_, jf);
+    hipLaunchKernelGGL(k_jpcg_update_f, g, dim3(kBlock), 0, s_, op_, x_, r_, p_new, q_,
+                       (const double*)partial_, nt, k, partial_ + uoff, st_);
+    PCFD_LAUNCH_CHECK();
+    ++k;
+    pending = true;
     std::swap(p_old, p_new);
   });
   return kOk;

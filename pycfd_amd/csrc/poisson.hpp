@@ -247,7 +247,7 @@ class PoissonEngine {
     CK_UPDATE = 0, CK_DIR, CK_FINE_DOWN, CK_FINE_UP, CK_CLASS_DOWN, CK_CLASS_UP,
     CK_ROWS_SMOOTH0, CK_ROWS_SMOOTH, CK_ROWS_RES, CK_ROWS_PROLONG, CK_RESTRICT, CK_DENSE,
     CK_CHAIN, CK_FOLD, CK_PREDICTOR, CK_PREDICTOR_COMB, CK_WALLS, CK_BPASS, CK_PREPARE, CK_INIT,
-    CK_CHECK, CK_SCATTER, CK_CORRECT, CK_COMBINE, CK_DIAG, kClockKinds
+    CK_CHECK, CK_SCATTER, CK_CORRECT, CK_COMBINE, CK_DIAG, CK_SMALL_CG, kClockKinds
   };
   enum FoldLevel : int { kFoldPrepare = 3, kFoldInit, kFoldCheck, kFoldDiag };
   static constexpr int kClockLevels = 16, kClockSlots = kClockKinds * kClockLevels;
@@ -309,6 +309,9 @@ class PoissonEngine {
 
  private:
   int run_cg();
+  // Jacobi-CG on grids of at most kSmallCgCells cells as one workgroup that
+  // keeps the whole solve on one CU (0: not eligible; else cells per thread).
+  int small_cg_cells_per_thread() const;
   int run_mgcg();
   int run_sor();
   int run_jacobi();
