@@ -42,6 +42,17 @@ namespace pcfd {
 enum InitMode : int { IM_JPCG = 0, IM_MGCG = 1, IM_RELAX = 2, IM_MG = 3 };
 enum CheckMode : int { CM_RELAX = 0, CM_MG = 1, CM_FINAL = 2 };
 
+struct EpiPrepare {  // fluid mean of the right-hand side (pressure.py:306-307)
+  static constexpr int K = 1;
+  SolveState* st;
+  int singular, nfluid;
+  __device__ void ops(int (&o)[K]) const { o[0] = kSum; }
+  __device__ bool active() const { return true; }
+  __device__ void operator()(const double (&v)[K]) const {
+    st->mean_b = (singular && nfluid > 0) ? v[0] / (double)nfluid : 0.0;
+  }
+};
+
 struct EpiInit {  // ||b||, ||r||, first rho, termination at iteration 0
   static constexpr int K = 4;
   SolveState* st;
@@ -353,24 +364,14 @@ k_prepare(FineOp A, const double* __restrict__ u, const double* __restrict__ v, 
   block_partials<1>(s, ops, partial, m.blk);
 }
 
-// The right-hand side pass's partials, folded by the solve start into the
-// fluid mean of the right-hand side (pressure.py:306-307); rows == the pass's
-// workgroup count; partial == null: the mean is already in the state.
-struct PrepFold {
-  const double* partial;
-  int rows, singular, nfluid;
-};
-
 // Solver start: b <- b - mean_b on fluid rows; r = b - A x (x = guess with
 // non-fluid rows already zeroed); partials ||b||^2, ||r||^2 and, per mode,
-// r.(r/d) (Jacobi-CG) or sum(r) (input projection of multigrid-CG).  With
-// pf.partial set, every workgroup first folds the right-hand side pass's sums
-// itself in k_fold's fixed order (the mean a fold launch would give, bit for bit,
-// without its launch); workgroup 0 records it.
+// r.(r/d) (Jacobi-CG) or sum(r) (input projection of multigrid-CG).
 __global__ void __launch_bounds__(kBlock)
 k_init(FineOp A, double* __restrict__ b, const double* __restrict__ x, double* __restrict__ r,
-       int mode, SolveState* st, double* partial, PrepFold pf) {
+       int mode, const SolveState* st, double* partial) {
   const MultiTile m = multi_tile(A.nx, A.ny);
+  const double mb = st->mean_b;
   uint8_t f[kMultiTiles];
   double bv[kMultiTiles], xc[kMultiTiles], xl[kMultiTiles], xr[kMultiTiles], xb[kMultiTiles],
       xt[kMultiTiles];
@@ -385,22 +386,6 @@ k_init(FineOp A, double* __restrict__ b, const double* __restrict__ x, double* _
     xr[u] = x[nb.kR];
     xb[u] = x[nb.kB];
     xt[u] = x[nb.kT];
-  }
-  // the fold after the cell loads are issued, so the two overlap
-  double mb;
-  if (pf.partial) {
-    double v[1];
-    const int op1[1] = {kSum};
-    fold_partials<1>(v, op1, pf.partial, pf.rows);
-    __shared__ double s_mb;
-    if (threadIdx.x == 0) {
-      s_mb = (pf.singular && pf.nfluid > 0) ? v[0] / (double)pf.nfluid : 0.0;
-      if (blockIdx.x == 0) st->mean_b = s_mb;
-    }
-    __syncthreads();
-    mb = s_mb;
-  } else {
-    mb = st->mean_b;
   }
   double s[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -719,6 +704,9 @@ k_check(FineOp A, const double* __restrict__ x, const double* __restrict__ b, in
     if (zero) return 0.0;
     return (fq & F_SOLV) ? xq - m : xq;
   };
+  // Without a pending shift (the Krylov solvers' case) x needs no flags of the
+  // neighbours: their loads are skipped (a uniform branch).
+  const bool plain = m == 0.0 && !zero;
   uint8_t f[kMultiTiles], fl[kMultiTiles], fr[kMultiTiles], fb[kMultiTiles], ft[kMultiTiles];
   double bv[kMultiTiles], xc[kMultiTiles], xl[kMultiTiles], xr[kMultiTiles], xb[kMultiTiles],
       xt[kMultiTiles];
@@ -727,10 +715,13 @@ k_check(FineOp A, const double* __restrict__ x, const double* __restrict__ b, in
     const int k = mt.k[u];
     const Nb nb = fine_neighbours(A, mt.i[u], mt.j[u]);
     f[u] = A.flags[k];
-    fl[u] = A.flags[nb.kL];
-    fr[u] = A.flags[nb.kR];
-    fb[u] = A.flags[nb.kB];
-    ft[u] = A.flags[nb.kT];
+    fl[u] = fr[u] = fb[u] = ft[u] = 0;
+    if (!plain) {
+      fl[u] = A.flags[nb.kL];
+      fr[u] = A.flags[nb.kR];
+      fb[u] = A.flags[nb.kB];
+      ft[u] = A.flags[nb.kT];
+    }
     bv[u] = b[k];
     xc[u] = x[k];
     xl[u] = x[nb.kL];
@@ -1405,7 +1396,6 @@ PoissonEngine::~PoissonEngine() {
   hipFree(log_);
   if (log_host_) hipHostFree(log_host_);
   hipFree(partial_);
-  hipFree(prep_partial_);
   hipFree(counter_);
   hipFree(fine_active_);
   if (st_host_) hipHostFree(st_host_);
@@ -1517,7 +1507,6 @@ int PoissonEngine::init(const FineOp& op, int n_fluid, int singular, hipStream_t
   const size_t nt = (size_t)num_tiles(op.nx + 3, op.ny + 3) + 64;
   partial_cap_ = nt * 8;
   PCFD_HIP_CHECK(hipMalloc(&partial_, partial_cap_ * sizeof(double)));
-  PCFD_HIP_CHECK(hipMalloc(&prep_partial_, (size_t)multi_blocks(op.nx, op.ny) * sizeof(double)));
   PCFD_HIP_CHECK(hipMalloc(&counter_, 64 * sizeof(unsigned int)));
   PCFD_HIP_CHECK(hipMemsetAsync(counter_, 0, 64 * sizeof(unsigned int), s_));
   PCFD_HIP_CHECK(hipMalloc(&st_, sizeof(SolveState)));
@@ -2194,25 +2183,22 @@ int PoissonEngine::prepare_from_velocity(const double* u, const double* v, int l
   const LaunchClock clk = clock_step(CK_PREPARE, (double)n_ * (8 + 8 + 1 + 8));
   if (is_pow2(dx) && is_pow2(dy))
     PCFD_LAUNCH_TIMED(clk, k_prepare<true>, dim3(mb), dim3(kBlock), 0, s_, op_, u, v, ldu, ldv,
-                      dx, dy, 1.0 / dx, 1.0 / dy, T, (const double*)nullptr, b_, x_,
-                      prep_partial_);
+                      dx, dy, 1.0 / dx, 1.0 / dy, T, (const double*)nullptr, b_, x_, partial_);
   else
     PCFD_LAUNCH_TIMED(clk, k_prepare<false>, dim3(mb), dim3(kBlock), 0, s_, op_, u, v, ldu, ldv,
-                      dx, dy, 1.0 / dx, 1.0 / dy, T, (const double*)nullptr, b_, x_,
-                      prep_partial_);
+                      dx, dy, 1.0 / dx, 1.0 / dy, T, (const double*)nullptr, b_, x_, partial_);
   PCFD_LAUNCH_CHECK();
-  prep_pending_ = true;  // the solve start folds the partials (k_init)
-  return kOk;
+  return fold(s_, partial_, mb, EpiPrepare{st_, singular_, n_fluid_},
+              clock_step(CK_FOLD, 0.0, kFoldPrepare));
 }
 
 int PoissonEngine::prepare_from_rhs(const double* rhs) {
   const int mb = multi_blocks(op_.nx, op_.ny);
   hipLaunchKernelGGL(k_prepare<false>, dim3(mb), dim3(kBlock), 0, s_, op_,
                      (const double*)nullptr, (const double*)nullptr, 0, 0, 1.0, 1.0, 1.0, 1.0,
-                     StepScalars{1.0, 0.0, nullptr}, rhs, b_, x_, prep_partial_);
+                     StepScalars{1.0, 0.0, nullptr}, rhs, b_, x_, partial_);
   PCFD_LAUNCH_CHECK();
-  prep_pending_ = true;  // the solve start folds the partials (k_init)
-  return kOk;
+  return fold(s_, partial_, mb, EpiPrepare{st_, singular_, n_fluid_});
 }
 
 int PoissonEngine::reset_guess() {
@@ -2361,10 +2347,8 @@ int PoissonEngine::start(int mode, double* r) {
   const int mb = multi_blocks(op_.nx, op_.ny);
   // algorithmic bytes: b, x, flags; b, r
   const LaunchClock clk = clock_step(CK_INIT, (double)n_ * (8 + 8 + 1 + 8 + (r ? 8 : 0)));
-  const PrepFold pf{prep_pending_ ? prep_partial_ : nullptr, mb, singular_, n_fluid_};
-  prep_pending_ = false;
   PCFD_LAUNCH_TIMED(clk, k_init, dim3(mb), dim3(kBlock), 0, s_, op_, b_, (const double*)x_, r,
-                    mode, st_, partial_, pf);
+                    mode, (const SolveState*)st_, partial_);
   PCFD_LAUNCH_CHECK();
   return fold(s_, partial_, mb, EpiInit{st_, mode, cfg_.tol, cfg_.maxiter, n_proj_},
               clock_step(CK_FOLD, 0.0, kFoldInit));

@@ -292,11 +292,6 @@ class PoissonEngine {
   int levels() const { return (int)lv64_.size(); }
 
   double *partial_ = nullptr;  // per-tile partial sums of the two-launch reductions
-  // The right-hand side pass's partials (sum of b), folded by the solve
-  // start that follows it (prep_pending_): a buffer of their own, since
-  // partial_ may be reallocated in between.
-  double* prep_partial_ = nullptr;
-  bool prep_pending_ = false;
   // Multigrid-CG: the last update's partials not yet folded (rows > 0), for
   // the next cycle's streaming down leg to fold (UpdateFold), with the
   // iteration count that update completes.
