@@ -20,8 +20,11 @@ the reference's own coarsest level.
 
 from __future__ import annotations
 
+import hashlib
 import logging
 import os
+import threading
+import weakref
 from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass
 
@@ -398,3 +401,36 @@ class MultigridHierarchy:
     def stencil_sizes(self) -> list[tuple[int, int]]:
         """(offsets, distinct rows) per coarse level."""
         return [(int(d["di"].size), int(d["class_dinv"].size)) for d in self.device_levels()[1:]]
+
+
+# Hierarchies already built in this process, by operator: many simulations of
+# one grid and boundary set (a batch of same-size cavities, a run repeated in
+# another precision) share one operator, so they share its hierarchy, which is
+# read-only once built.  An entry lives as long as some solver holds it.
+_SHARED: "weakref.WeakValueDictionary[bytes, MultigridHierarchy]" = weakref.WeakValueDictionary()
+_SHARED_LOCK = threading.Lock()
+
+
+def _operator_key(A: sp.csr_matrix, shape, active, singular, sweeps, periodic) -> bytes:
+    h = hashlib.blake2b(digest_size=32)
+    A = sp.csr_matrix(A)
+    for arr in (A.indptr, A.indices, A.data, np.asarray(active, dtype=bool)):
+        h.update(np.ascontiguousarray(arr).view(np.uint8))
+    h.update(repr((tuple(shape), bool(singular), int(sweeps), tuple(map(bool, periodic)),
+                   A.shape)).encode())
+    return h.digest()
+
+
+def shared_hierarchy(A: sp.csr_matrix, shape, active, singular: bool, sweeps: int = 1,
+                     periodic: tuple[bool, bool] = (False, False)) -> "MultigridHierarchy":
+    """The hierarchy of this operator, built once per process while in use
+    (the same object for every solver of an identical operator)."""
+    key = _operator_key(A, shape, active, singular, sweeps, periodic)
+    with _SHARED_LOCK:
+        hit = _SHARED.get(key)
+    if hit is not None:
+        return hit
+    built = MultigridHierarchy(A, shape, active, singular, presmooth=sweeps,
+                               postsmooth=sweeps, periodic=periodic)
+    with _SHARED_LOCK:
+        return _SHARED.setdefault(key, built)

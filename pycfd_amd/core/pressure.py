@@ -358,12 +358,11 @@ class MultigridSolverBase(PoissonSolverBase):
     def __init__(self, system, tol=DEFAULT_POISSON_TOL, maxiter=DEFAULT_POISSON_MAXITER,
                  smooth_sweeps: int = DEFAULT_MG_SWEEPS, **kw):
         super().__init__(system, tol, maxiter, **kw)
-        from .multigrid import MultigridHierarchy
+        from .multigrid import shared_hierarchy
         self.smooth_sweeps = smooth_sweeps
-        self.hierarchy = MultigridHierarchy(system.A, system.shape, system.fluid,
-                                            system.singular, presmooth=smooth_sweeps,
-                                            postsmooth=smooth_sweeps,
-                                            periodic=(system.periodic_x, system.periodic_y))
+        self.hierarchy = shared_hierarchy(system.A, system.shape, system.fluid, system.singular,
+                                          sweeps=smooth_sweeps,
+                                          periodic=(system.periodic_x, system.periodic_y))
 
     def _levels(self):
         return self.hierarchy.device_levels(self.coarse_max, self.wide_classes)

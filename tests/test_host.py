@@ -286,3 +286,31 @@ def test_cavity_batch_configs_are_seeded_and_in_range():
     assert x.nx == x.ny and x.dt == 0.01 and x.cfl_max == 0.4 and x.lx == x.ly == 1.0
     assert x.boundary_config["top"].kind.value == "moving_wall"
     assert x.pressure_solver.value == "direct" and not x.use_les
+
+
+def test_identical_operators_share_one_hierarchy():
+    """Solvers of one operator get the same hierarchy object (built once); a
+    different operator, sweep count or a freed hierarchy gets its own build,
+    equal to a fresh one."""
+    import gc
+
+    from pycfd_amd.core.multigrid import shared_hierarchy
+    mesh = StructuredMesh(48, 40, 1.0, 1.0)
+    s1 = assemble_poisson_matrix(mesh)
+    s2 = assemble_poisson_matrix(StructuredMesh(48, 40, 1.0, 1.0))
+    h1 = shared_hierarchy(s1.A, s1.shape, s1.fluid, s1.singular)
+    h2 = shared_hierarchy(s2.A, s2.shape, s2.fluid, s2.singular)
+    assert h1 is h2
+    assert shared_hierarchy(s1.A, s1.shape, s1.fluid, s1.singular, sweeps=2) is not h1
+    s3 = assemble_poisson_matrix(mesh, dirichlet={"right": 0.0})
+    h3 = shared_hierarchy(s3.A, s3.shape, s3.fluid, s3.singular)
+    assert h3 is not h1
+    fresh = MultigridHierarchy(s1.A, s1.shape, s1.fluid, s1.singular)
+    assert fresh.unknowns == h1.unknowns
+    for a, b in zip(fresh.levels, h1.levels):
+        assert (a.A != b.A).nnz == 0 and a.weight == b.weight
+    del h1, h2
+    gc.collect()  # no solver holds it any more: the next request builds it again
+    h4 = shared_hierarchy(s1.A, s1.shape, s1.fluid, s1.singular)
+    assert h4.unknowns == fresh.unknowns
+    assert [lv.weight for lv in h4.levels] == [lv.weight for lv in fresh.levels]
