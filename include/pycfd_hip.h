@@ -162,6 +162,43 @@ typedef struct pcfd_solve_info {
   int32_t status;
 } pcfd_solve_info;
 
+/* The fine operator handed to the hierarchy build: the assembled Poisson
+ * matrix (pressure.py:110-270) in compressed rows, the per-cell active mask,
+ * and the power iteration's seeded start vector (numpy
+ * default_rng(0).standard_normal(nx*ny); each level uses its first n
+ * entries, which is what a fresh generator of that seed draws for n,
+ * multigrid.py:147-149).  Host pointers, read during the call only. */
+typedef struct pcfd_mg_operator {
+  int32_t nx, ny;
+  const int64_t* indptr;   /* (nx*ny + 1) */
+  const int32_t* indices;  /* (indptr[nx*ny]) */
+  const double* data;      /* (indptr[nx*ny]) */
+  const uint8_t* active;   /* (nx*ny) */
+  const double* x0;        /* (nx*ny) */
+  int32_t singular;
+  int32_t periodic_x, periodic_y;
+  int32_t threads;         /* host threads for the build */
+  int32_t coarse_max;      /* stop at the first level with at most this many
+                              unknowns (at most 96: the whole hierarchy) */
+  int32_t every_row;       /* 1: form every coarse row; 0: each distinct one
+                              once, translated to the equal rows (the same
+                              operator bit for bit, less work) */
+} pcfd_mg_operator;
+
+typedef struct pcfd_mg_setup pcfd_mg_setup;
+
+/* One level of a built hierarchy. */
+typedef struct pcfd_mg_built_level {
+  int32_t n_levels;
+  int32_t nx, ny;
+  int32_t fx, fy;          /* aggregation factors towards the next level */
+  int32_t pin;             /* first active cell (the coarsest level's pin) */
+  int32_t n_offsets;       /* stencil offsets of the row dictionary (levels >= 1) */
+  int64_t nnz;             /* stored entries of the level's operator */
+  int64_t n_classes;       /* distinct operator rows (levels >= 1) */
+  double weight;           /* damped-Jacobi weight (multigrid.py:354-362) */
+} pcfd_mg_built_level;
+
 /* --- lifetime --------------------------------------------------------- */
 /* ProjectionSolver.__init__ (solver.py:101-195): device state, IB face lists,
  * Poisson operator flags (pressure.py:110-270).  hip_stream NULL = the
@@ -177,6 +214,23 @@ int pcfd_synchronize(pcfd_ctx* ctx);
  * no device; returns the solvable-cell count in *n_solvable. */
 int pcfd_poisson_flags(const pcfd_case_desc* desc, uint8_t* flags_out, int32_t* n_solvable);
 
+/* Host-only: MultigridHierarchy.__init__ (multigrid.py:324-474) -- the
+ * aggregation hierarchy of an operator, every coarse level also turned into
+ * its row dictionary (pcfd_mg_level_desc).  Built natively on `threads` host
+ * threads; the result does not depend on the thread count. */
+int pcfd_mg_build(const pcfd_mg_operator* op, pcfd_mg_setup** out);
+int pcfd_mg_free(pcfd_mg_setup* h);
+int pcfd_mg_level_info(const pcfd_mg_setup* h, int32_t level, pcfd_mg_built_level* info);
+/* Copies one level out (any pointer may be NULL): its operator in compressed
+ * rows (nnz entries, columns ascending), 1/diag and the active mask (nx*ny),
+ * and for levels >= 1 the row dictionary: offsets (n_offsets), the class of
+ * every cell (nx*ny), the class table (n_classes*n_offsets, row-major), and per
+ * class 1/diag and active flag.  Classes are numbered in the lexicographic
+ * order of their rows. */
+int pcfd_mg_level_copy(const pcfd_mg_setup* h, int32_t level, int64_t* indptr,
+                       int32_t* indices, double* data, double* dinv, uint8_t* active,
+                       int8_t* di, int8_t* dj, uint32_t* cls, double* table,
+                       double* class_dinv, uint8_t* class_active);
 /* make_pressure_solver (pressure.py:645-660) with a hierarchy built by the
  * host setup (MultigridHierarchy._build, multigrid.py:364-395). */
 int pcfd_set_pressure_solver(pcfd_ctx* ctx, const pcfd_solver_desc* desc,

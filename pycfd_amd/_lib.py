@@ -81,6 +81,24 @@ class MGLevelDesc(C.Structure):
     ]
 
 
+class MGOperator(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32),
+        ("indptr", C.POINTER(C.c_int64)), ("indices", C.POINTER(C.c_int32)),
+        ("data", _dp), ("active", _u8p), ("x0", _dp),
+        ("singular", C.c_int32), ("periodic_x", C.c_int32), ("periodic_y", C.c_int32),
+        ("threads", C.c_int32), ("coarse_max", C.c_int32), ("every_row", C.c_int32),
+    ]
+
+
+class MGBuiltLevel(C.Structure):
+    _fields_ = [
+        ("n_levels", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32),
+        ("fx", C.c_int32), ("fy", C.c_int32), ("pin", C.c_int32), ("n_offsets", C.c_int32),
+        ("nnz", C.c_int64), ("n_classes", C.c_int64), ("weight", C.c_double),
+    ]
+
+
 class SolveInfo(C.Structure):
     _fields_ = [
         ("iterations", C.c_int32), ("residual", C.c_double),
@@ -95,6 +113,10 @@ _SIGNATURES = {
     "pcfd_device_count": [C.POINTER(C.c_int32)],
     "pcfd_synchronize": [C.c_void_p],
     "pcfd_poisson_flags": [C.POINTER(CaseDesc), _u8p, C.POINTER(C.c_int32)],
+    "pcfd_mg_build": [C.POINTER(MGOperator), C.POINTER(C.c_void_p)],
+    "pcfd_mg_free": [C.c_void_p],
+    "pcfd_mg_level_info": [C.c_void_p, C.c_int32, C.POINTER(MGBuiltLevel)],
+    "pcfd_mg_level_copy": [C.c_void_p, C.c_int32] + [C.c_void_p] * 11,
     "pcfd_set_pressure_solver": [C.c_void_p, C.POINTER(SolverDesc), C.POINTER(MGLevelDesc),
                                  C.c_int32],
     "pcfd_momentum_rhs": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,

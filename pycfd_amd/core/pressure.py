@@ -362,7 +362,8 @@ class MultigridSolverBase(PoissonSolverBase):
         self.smooth_sweeps = smooth_sweeps
         self.hierarchy = shared_hierarchy(system.A, system.shape, system.fluid, system.singular,
                                           sweeps=smooth_sweeps,
-                                          periodic=(system.periodic_x, system.periodic_y))
+                                          periodic=(system.periodic_x, system.periodic_y),
+                                          coarse_max=self.coarse_max)
 
     def _levels(self):
         return self.hierarchy.device_levels(self.coarse_max, self.wide_classes)

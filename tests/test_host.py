@@ -314,3 +314,134 @@ def test_identical_operators_share_one_hierarchy():
     h4 = shared_hierarchy(s1.A, s1.shape, s1.fluid, s1.singular)
     assert h4.unknowns == fresh.unknowns
     assert [lv.weight for lv in h4.levels] == [lv.weight for lv in fresh.levels]
+
+
+# --- the native hierarchy build (csrc/hierarchy.hip) ---------------------
+
+def _native_levels(S, threads=4, coarse_max=96, every_row=0):
+    """Every level of a native build: operator arrays, weight, dictionary."""
+    import ctypes as C
+
+    from pycfd_amd import _lib
+    from pycfd_amd.core.multigrid import _Built, _start_vector
+    nx, ny = S.shape
+    A = sp.csr_matrix(S.A)
+    keep = [np.ascontiguousarray(A.indptr, dtype=np.int64),
+            np.ascontiguousarray(A.indices, dtype=np.int32),
+            np.ascontiguousarray(A.data, dtype=np.float64),
+            np.ascontiguousarray(S.fluid, dtype=np.uint8), _start_vector(nx * ny)]
+    op = _lib.MGOperator()
+    op.nx, op.ny = nx, ny
+    op.indptr = keep[0].ctypes.data_as(C.POINTER(C.c_int64))
+    op.indices = keep[1].ctypes.data_as(C.POINTER(C.c_int32))
+    op.data = keep[2].ctypes.data_as(C.POINTER(C.c_double))
+    op.active = keep[3].ctypes.data_as(C.POINTER(C.c_uint8))
+    op.x0 = keep[4].ctypes.data_as(C.POINTER(C.c_double))
+    op.singular = int(S.singular)
+    op.periodic_x, op.periodic_y = int(S.periodic_x), int(S.periodic_y)
+    op.threads, op.coarse_max, op.every_row = threads, coarse_max, every_row
+    h = C.c_void_p(0)
+    _lib.check(_lib.load().pcfd_mg_build(C.byref(op), C.byref(h)), "pcfd_mg_build")
+    built = _Built(h)
+    out = []
+    for l in range(built.info(0).n_levels):
+        d = built.copy(l, "indptr", "indices", "data", "dinv", "cls", "table", "class_dinv")
+        d["weight"] = built.info(l).weight
+        out.append(d)
+    return out
+
+
+def _cylinder_system(nx, ny):
+    from pycfd_amd import cases
+    cfg, body, _, _ = cases.cylinder(nx=nx, ny=ny)
+    mesh = StructuredMesh.from_config(cfg)
+    bm = BoundaryManager(cfg.boundary_config, mesh)
+    return assemble_poisson_matrix(mesh, bm.periodic_x, bm.periodic_y, body.mask,
+                                   bm.dirichlet_pressure() or None)
+
+
+def _native_cases():
+    yield from ((n, product_system(n)) for n in ("cavity", "periodic", "channel", "cyl_pout",
+                                                 "s_cyl", "s_pchannel"))
+    yield "cylinder256x128", _cylinder_system(256, 128)
+
+
+@pytest.mark.parametrize("name,S", list(_native_cases()))
+def test_native_translated_rows_equal_every_row(name, S):
+    """Forming each distinct coarse row once and translating it gives the
+    operator that forming every row gives, bit for bit; so does any thread
+    count."""
+    a = _native_levels(S, threads=4, every_row=1)
+    b = _native_levels(S, threads=4, every_row=0)
+    c = _native_levels(S, threads=1, every_row=0)
+    assert len(a) == len(b) == len(c)
+    for x, y, z in zip(a, b, c):
+        assert x["weight"] == y["weight"] == z["weight"]
+        for k in ("indptr", "indices", "data", "dinv", "cls", "table", "class_dinv"):
+            assert np.array_equal(x[k], y[k]) and np.array_equal(y[k], z[k]), k
+
+
+@pytest.mark.parametrize("name", ["cavity", "periodic", "channel", "cyl_outlet", "cyl_pout",
+                                  "s_cyl", "s_pchannel"])
+def test_native_hierarchy_matches_oracle(name):
+    """Against the oracle's SciPy build of the reference algorithm: the same
+    grids, factors and sparsity, weights and operators equal to round-off."""
+    S = product_system(name)
+    R = O.Hierarchy(O.assemble(oracle_case(name)).A, S.shape, S.fluid, S.singular, 1)
+    H = MultigridHierarchy(S.A, S.shape, S.fluid, S.singular, 1, 1,
+                           (S.periodic_x, S.periodic_y))
+    assert H.unknowns == [lv["A"].shape[0] for lv in R.levels]
+    for lv, rl in zip(H.levels, R.levels):
+        assert lv.shape == rl["shape"]
+        assert lv.A.nnz == rl["A"].nnz
+        assert lv.weight == pytest.approx(rl["weight"], rel=1e-13)
+        scale = abs(rl["A"]).max()
+        assert abs(lv.A - rl["A"]).max() <= 1e-13 * scale
+        np.testing.assert_array_equal(lv.active, rl["active"])
+
+
+@pytest.mark.parametrize("name", ["cavity", "periodic", "cyl_pout", "s_cyl"])
+def test_native_coarse_operators_keep_symmetry_and_null_space(name):
+    """Galerkin operators of a symmetric operator are symmetric; where the
+    fine operator is singular (row sums zero on the active set) every coarse
+    operator keeps the constant null space on its active cells."""
+    S = product_system(name)
+    H = MultigridHierarchy(S.A, S.shape, S.fluid, S.singular, 1, 1,
+                           (S.periodic_x, S.periodic_y))
+    for lv in H.levels[1:]:
+        A = lv.A
+        scale = abs(A).max()
+        assert abs(A - A.T).max() <= 1e-12 * scale
+        if S.singular:
+            rows = np.asarray(A @ lv.active.astype(float))
+            assert np.abs(rows[lv.active]).max() <= 1e-12 * scale
+
+
+def test_native_build_stops_at_coarse_max():
+    """A build stopped at the first level with at most 2048 unknowns holds the
+    same levels as the whole hierarchy down to there, bit for bit, and a
+    solver asking for a coarser device level gets the hierarchy deepened."""
+    S = _cylinder_system(256, 128)
+    full = _native_levels(S, coarse_max=96)
+    cut = _native_levels(S, coarse_max=2048)
+    assert len(cut) < len(full)
+    assert cut[-1]["dinv"].size <= 2048 < cut[-2]["dinv"].size
+    for x, y in zip(cut, full):
+        assert x["weight"] == y["weight"]
+        for k in ("indptr", "indices", "data", "cls", "table"):
+            assert np.array_equal(x[k], y[k]), k
+    H = MultigridHierarchy(S.A, S.shape, S.fluid, S.singular, coarse_max=2048)
+    assert H.unknowns == [d["dinv"].size for d in cut]
+    d = H.device_levels(96)
+    assert [L["nx"] * L["ny"] for L in d] == [x["dinv"].size for x in full]
+
+
+def test_native_build_rejects_bad_operators():
+    from pycfd_amd.core.multigrid import MultigridHierarchy as MH
+    S = product_system("cavity")
+    bad = S.A.copy().tocsr()
+    bad.indices[3] = S.A.shape[0] + 5
+    with pytest.raises(ValueError, match="out of range"):
+        MH(bad, S.shape, S.fluid, S.singular)
+    with pytest.raises(ValueError, match="do not fit"):
+        MH(S.A, (S.shape[0] + 1, S.shape[1]), S.fluid, S.singular)
