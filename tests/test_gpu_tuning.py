@@ -28,7 +28,9 @@ def test_tuned_sweeps_keep_the_step(gpu_available, ps):
     sim = Simulation(cfg, obstacle=body, u_init=u0, v_init=v0)
     table = sim.solver.tune_smoother_sweeps(sim.fields, 0.01, candidates=(1, 2, 3), steps=2)
     assert sorted(table) == [1, 2, 3]
-    best = min(table, key=lambda k: table[k]["ms_per_step"])
+    # the fewest sweeps within 3% of the fastest
+    fastest = min(t["ms_per_step"] for t in table.values())
+    best = min(k for k, t in table.items() if t["ms_per_step"] <= 1.03 * fastest)
     assert sim.solver.pressure_solver.smooth_sweeps == best
     assert all(t["iterations"] > 0 and t["ms_per_step"] > 0 for t in table.values())
     # more smoothing per cycle never needs more cycles on this case
