@@ -162,6 +162,21 @@ typedef struct pcfd_solve_info {
   int32_t status;
 } pcfd_solve_info;
 
+/* The fine Poisson operator to assemble (pressure.py:110-270): the grid, the
+ * body, the face coefficients (1/dx^2 and 1/dy^2 on a uniform mesh; per face
+ * on a stretched one: face area / centre distance / mean cell area) and the
+ * walls with a pressure value.  Host pointers, read during the call only. */
+typedef struct pcfd_poisson_assembly {
+  int32_t nx, ny;
+  int32_t periodic_x, periodic_y;
+  const uint8_t* solid;    /* (nx*ny) nonzero in a body, or NULL */
+  double cx, cy;           /* uniform face coefficients */
+  const double* cx_face;   /* ((nx+1)*ny) x-face coefficients of a stretched mesh, or NULL */
+  const double* cy_face;   /* (nx*(ny+1)) y-face coefficients, or NULL */
+  int32_t dirichlet[4];    /* walls left, right, bottom, top with a pressure value */
+  double p_ref[4];         /* their values */
+} pcfd_poisson_assembly;
+
 /* The fine operator handed to the hierarchy build: the assembled Poisson
  * matrix (pressure.py:110-270) in compressed rows, the per-cell active mask,
  * and the power iteration's seeded start vector (numpy
@@ -214,6 +229,13 @@ int pcfd_synchronize(pcfd_ctx* ctx);
  * no device; returns the solvable-cell count in *n_solvable. */
 int pcfd_poisson_flags(const pcfd_case_desc* desc, uint8_t* flags_out, int32_t* n_solvable);
 
+/* Host-only: assemble_poisson_matrix (pressure.py:110-270) in compressed
+ * rows, columns ascending.  With indices == NULL only indptr (nx*ny + 1, the
+ * row starts) is written; a second call with indptr from the first fills
+ * indices and data (indptr[nx*ny] entries), the boundary right-hand side bc
+ * (nx*ny) and the solvable mask (nx*ny; the other cells get identity rows). */
+int pcfd_poisson_assemble(const pcfd_poisson_assembly* a, int64_t* indptr, int32_t* indices,
+                          double* data, double* bc, uint8_t* solvable);
 /* Host-only: MultigridHierarchy.__init__ (multigrid.py:324-474) -- the
  * aggregation hierarchy of an operator, every coarse level also turned into
  * its row dictionary (pcfd_mg_level_desc).  Built natively on `threads` host
@@ -222,7 +244,8 @@ int pcfd_mg_build(const pcfd_mg_operator* op, pcfd_mg_setup** out);
 int pcfd_mg_free(pcfd_mg_setup* h);
 int pcfd_mg_level_info(const pcfd_mg_setup* h, int32_t level, pcfd_mg_built_level* info);
 /* Copies one level out (any pointer may be NULL): its operator in compressed
- * rows (nnz entries, columns ascending), 1/diag and the active mask (nx*ny),
+ * rows (nnz entries, columns ascending; not for level 0, the caller's own
+ * operator, which the build does not keep), 1/diag and the active mask (nx*ny),
  * and for levels >= 1 the row dictionary: offsets (n_offsets), the class of
  * every cell (nx*ny), the class table (n_classes*n_offsets, row-major), and per
  * class 1/diag and active flag.  Classes are numbered in the lexicographic

@@ -81,6 +81,16 @@ class MGLevelDesc(C.Structure):
     ]
 
 
+class PoissonAssembly(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32),
+        ("periodic_x", C.c_int32), ("periodic_y", C.c_int32),
+        ("solid", _u8p), ("cx", C.c_double), ("cy", C.c_double),
+        ("cx_face", _dp), ("cy_face", _dp),
+        ("dirichlet", C.c_int32 * 4), ("p_ref", C.c_double * 4),
+    ]
+
+
 class MGOperator(C.Structure):
     _fields_ = [
         ("nx", C.c_int32), ("ny", C.c_int32),
@@ -113,6 +123,7 @@ _SIGNATURES = {
     "pcfd_device_count": [C.POINTER(C.c_int32)],
     "pcfd_synchronize": [C.c_void_p],
     "pcfd_poisson_flags": [C.POINTER(CaseDesc), _u8p, C.POINTER(C.c_int32)],
+    "pcfd_poisson_assemble": [C.POINTER(PoissonAssembly)] + [C.c_void_p] * 5,
     "pcfd_mg_build": [C.POINTER(MGOperator), C.POINTER(C.c_void_p)],
     "pcfd_mg_free": [C.c_void_p],
     "pcfd_mg_level_info": [C.c_void_p, C.c_int32, C.POINTER(MGBuiltLevel)],

@@ -141,6 +141,30 @@ class Level:
         return self._cache["active_bool"]
 
 
+def _inverse_negated(B: np.ndarray) -> np.ndarray:
+    """inv(-B) for a symmetric positive definite B (overwritten), by Cholesky
+    (half the work of a general inverse); a general inverse when the
+    factorisation fails."""
+    from scipy.linalg import lapack
+    n = B.shape[0]
+    if n == 0:
+        return B
+    # B^T is B's memory in column order, so LAPACK works on it in place
+    c, info = lapack.dpotrf(B.T, lower=1, overwrite_a=1, clean=0)
+    if info == 0:
+        inv, info = lapack.dpotri(c, lower=1, overwrite_c=1)
+        if info == 0:
+            for i0 in range(0, n, 256):  # the lower triangle onto the upper
+                i1 = min(n, i0 + 256)
+                blk = inv[i0:i1, i0:i1]
+                iu = np.triu_indices(i1 - i0, 1)
+                blk[iu] = blk.T[iu]
+                inv[i0:i1, i1:] = inv[i1:, i0:i1].T
+            np.negative(inv, out=inv)
+            return inv
+    return np.linalg.inv(-B)
+
+
 def coarsest_pseudo_inverse(level: Level) -> np.ndarray:
     """Dense operator of the coarsest solve, projections included (multigrid.py:514-543).
 
@@ -148,26 +172,39 @@ def coarsest_pseudo_inverse(level: Level) -> np.ndarray:
     Z zeroing the pinned right-hand side entry and Pi the mean removal on the
     active set (zero on inactive cells); the two projections are applied as
     column and row mean subtractions.  Otherwise inactive rows are zeroed.
+
+    Inactive cells carry identity rows that no active row couples to, and
+    pinning cell p leaves inv(M) with column p of no effect once Z drops it,
+    so only the inverse of the operator on the active cells (without p) is
+    needed; that block is symmetric negative definite and is inverted by
+    Cholesky.
     """
     n = level.n
-    A = level.A.toarray()
-    act = level.active
-    if level.singular and n:
-        M = A.copy()
-        M[level.pin, :] = 0.0
-        M[level.pin, level.pin] = 1.0
-        Y = np.linalg.inv(M)
-        Y[:, level.pin] = 0.0
-        ia = np.flatnonzero(act)
-        out = np.zeros((n, n))
-        if ia.size:
-            Yaa = Y[np.ix_(ia, ia)]
-            Yaa -= Yaa.mean(axis=0, keepdims=True)
-            Yaa -= Yaa.mean(axis=1, keepdims=True)
-            out[np.ix_(ia, ia)] = Yaa
-        return out
-    zero_inactive = np.diag(act.astype(float))
-    return zero_inactive @ np.linalg.inv(A)
+    ia = np.flatnonzero(level.active)
+    if ia.size == 0:
+        return np.zeros((n, n))
+    A = level.A
+    m = ia.size
+    if level.singular:
+        p = int(np.searchsorted(ia, level.pin))  # the pin's place among the active cells
+        keep = np.delete(ia, p)
+        Z = _inverse_negated((-A[keep][:, keep]).toarray())
+        Y = np.empty((m, m))  # Z with a zero row and column at p
+        Y[p, :] = 0.0
+        Y[:, p] = 0.0
+        Y[:p, :p] = Z[:p, :p]
+        Y[:p, p + 1:] = Z[:p, p:]
+        Y[p + 1:, :p] = Z[p:, :p]
+        Y[p + 1:, p + 1:] = Z[p:, p:]
+        Y -= Y.mean(axis=0, keepdims=True)
+        Y -= Y.mean(axis=1, keepdims=True)
+    else:
+        Y = _inverse_negated((-A[ia][:, ia]).toarray())
+    if m == n:
+        return np.ascontiguousarray(Y)
+    out = np.zeros((n, n))
+    out[np.ix_(ia, ia)] = Y
+    return out
 
 
 class MultigridHierarchy:
@@ -229,6 +266,7 @@ class MultigridHierarchy:
         first = built.info(0)
         self.levels = [Level(built, l, first if l == 0 else built.info(l), self.singular)
                        for l in range(first.n_levels)]
+        self.levels[0]._cache["A"] = A  # the build keeps no copy of the fine operator
         self._built = built
         self._depth = coarse_max
         log.debug("multigrid hierarchy: %s", self)

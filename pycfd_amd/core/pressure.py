@@ -88,46 +88,50 @@ def assemble_poisson_matrix(mesh: StructuredMesh, periodic_x: bool = False,
         vbar = mesh.area_mean
         cxf = m.hy / (m.hxu * vbar)   # (nx+1, ny), by x face
         cyf = m.hx / (m.hyv * vbar)   # (nx, ny+1), by y face
-        coeffs = (cxf[0:nx, :], cxf[1:nx + 1, :], cyf[:, 0:ny], cyf[:, 1:ny + 1])
         row_scale = (m.cell_areas / vbar).ravel()
         dx, dy = float(mesh.dx_cells[0]), float(mesh.dy_cells[0])
     else:
         dx, dy = mesh.dx, mesh.dy
         cx, cy = 1.0 / (dx * dx), 1.0 / (dy * dy)
-        coeffs = (cx, cx, cy, cy)
         row_scale = 1.0
-    idx = np.arange(n, dtype=np.int64).reshape(nx, ny)
-    diag = np.zeros((nx, ny))
-    bc = np.zeros((nx, ny))
-    rows, cols, vals = [], [], []
-    for axis, shift, coeff, periodic, wall in ((0, -1, coeffs[0], periodic_x, "left"),
-                                              (0, +1, coeffs[1], periodic_x, "right"),
-                                              (1, -1, coeffs[2], periodic_y, "bottom"),
-                                              (1, +1, coeffs[3], periodic_y, "top")):
-        nb = np.roll(idx, -shift, axis=axis)
-        inside = np.ones((nx, ny), dtype=bool)
-        if not periodic:
-            edge = [slice(None), slice(None)]
-            edge[axis] = -1 if shift > 0 else 0
-            inside[tuple(edge)] = False
-        active = fluid & inside & fluid.ravel()[nb]
-        rows.append(idx[active])
-        cols.append(nb[active])
-        vals.append(np.full(int(active.sum()), coeff) if np.isscalar(coeff) else coeff[active])
-        diag -= coeff * active
-        if wall in dirichlet:
-            on_wall = fluid & ~inside
-            diag -= 2.0 * coeff * on_wall
-            bc -= 2.0 * coeff * dirichlet[wall] * on_wall
-    solvable = fluid & ~(diag == 0.0)
-    diag = np.where(solvable, diag, 1.0)
-    bc = np.where(solvable, bc, 0.0)
-    rows.append(idx.ravel())
-    cols.append(idx.ravel())
-    vals.append(diag.ravel())
-    A = sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
-                      shape=(n, n)).tocsr()
-    A.sum_duplicates()
+    # the rows are assembled natively (csrc/assembly.hip)
+    import ctypes as C
+
+    from .. import _lib
+    desc = _lib.PoissonAssembly()
+    desc.nx, desc.ny = nx, ny
+    desc.periodic_x, desc.periodic_y = int(bool(periodic_x)), int(bool(periodic_y))
+    keep = []
+    if solid.any():
+        sarr = np.ascontiguousarray(solid, dtype=np.uint8)
+        keep.append(sarr)
+        desc.solid = sarr.ctypes.data_as(C.POINTER(C.c_uint8))
+    if stretched:
+        fx_ = np.ascontiguousarray(cxf, dtype=np.float64)
+        fy_ = np.ascontiguousarray(cyf, dtype=np.float64)
+        keep += [fx_, fy_]
+        desc.cx_face = fx_.ctypes.data_as(C.POINTER(C.c_double))
+        desc.cy_face = fy_.ctypes.data_as(C.POINTER(C.c_double))
+    else:
+        desc.cx, desc.cy = cx, cy
+    for w, wall in enumerate(("left", "right", "bottom", "top")):
+        desc.dirichlet[w] = int(wall in dirichlet)
+        desc.p_ref[w] = float(dirichlet.get(wall, 0.0))
+    lib = _lib.load()
+    indptr = np.empty(n + 1, dtype=np.int64)
+    _lib.check(lib.pcfd_poisson_assemble(C.byref(desc), indptr.ctypes.data, None, None, None,
+                                         None), "pcfd_poisson_assemble")
+    nnz = int(indptr[n])
+    indices = np.empty(nnz, dtype=np.int32)
+    data = np.empty(nnz)
+    bc = np.empty(n)
+    solvable = np.empty(n, dtype=np.uint8)
+    _lib.check(lib.pcfd_poisson_assemble(C.byref(desc), indptr.ctypes.data, indices.ctypes.data,
+                                         data.ctypes.data, bc.ctypes.data,
+                                         solvable.ctypes.data), "pcfd_poisson_assemble")
+    A = sp.csr_matrix((data, indices, indptr.astype(np.int32) if nnz < 2 ** 31 else indptr),
+                      shape=(n, n))
+    solvable = solvable.astype(bool).reshape(nx, ny)
     flat = solvable.ravel()
     pin = int(np.flatnonzero(flat)[0]) if flat.any() else 0
     return PoissonSystem(A, (nx, ny), flat.copy(), not dirichlet, pin, bc.ravel().copy(),

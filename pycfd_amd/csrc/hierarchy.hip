@@ -134,25 +134,33 @@ class Pool {
 };
 
 // A host array left uninitialised on allocation (the build writes every
-// element; zero-filling the large ones first would cost a serial pass).
+// element; zero-filling the large ones first would cost a serial pass), or a
+// read-only view of the caller's memory.
 template <typename T>
 class Buf {
  public:
   Buf() = default;
-  Buf(Buf&& o) noexcept : p_(o.p_), n_(o.n_) {
+  Buf(Buf&& o) noexcept : p_(o.p_), n_(o.n_), own_(o.own_) {
     o.p_ = nullptr;
     o.n_ = 0;
   }
   Buf& operator=(Buf&& o) noexcept {
     std::swap(p_, o.p_);
     std::swap(n_, o.n_);
+    std::swap(own_, o.own_);
     return *this;
   }
   Buf(const Buf&) = delete;
   Buf& operator=(const Buf&) = delete;
-  ~Buf() { std::free(p_); }
+  ~Buf() { release(); }
+  void release() {
+    if (own_) std::free(p_);
+    p_ = nullptr;
+    n_ = 0;
+    own_ = true;
+  }
   void alloc(int64_t n) {
-    std::free(p_);
+    release();
     p_ = static_cast<T*>(std::malloc(sizeof(T) * (size_t)std::max<int64_t>(n, 1)));
     if (!p_) throw std::bad_alloc();
     n_ = n;
@@ -160,6 +168,12 @@ class Buf {
   void assign(const T* src, int64_t n) {
     alloc(n);
     if (n) std::memcpy(p_, src, sizeof(T) * (size_t)n);
+  }
+  void view(const T* src, int64_t n) {
+    release();
+    p_ = const_cast<T*>(src);
+    n_ = n;
+    own_ = false;
   }
   int64_t size() const { return n_; }
   bool empty() const { return n_ == 0; }
@@ -171,6 +185,7 @@ class Buf {
  private:
   T* p_ = nullptr;
   int64_t n_ = 0;
+  bool own_ = true;
 };
 
 // Compressed rows, columns ascending within a row, no stored zeros.
@@ -359,6 +374,7 @@ struct Level {
   int fx = 1, fy = 1;  // towards the next level
   int pin = 0;
   int rx = 0, ry = 0;  // stencil radii of the operator
+  int64_t nnz0 = -1;   // level 0: entries of the caller's operator (not kept)
   // row dictionary; cls is empty when a level has too many distinct rows
   // for its classes to pay (the fine level of a stretched mesh)
   std::vector<int8_t> di, dj;
@@ -785,11 +801,15 @@ bool coarsen(Pool& pool, Level& F, Level& C, bool px, bool py, bool every_row) {
   // aggregates (multigrid.py:212-245)
   std::vector<int32_t> label(nf);
   std::vector<uint8_t> cact(nc, 0);
-  for (int64_t k = 0; k < nf; ++k) {
-    const int64_t i = k / ny, j = k - i * ny;
-    label[k] = F.active[k] ? (int32_t)((i / fx) * ncy + j / fy) : -1;
+  const FastDiv by_ny((uint64_t)ny);
+  pool.run(nf, [&](int, int64_t a, int64_t b) {
+    for (int64_t k = a; k < b; ++k) {
+      const int64_t i = (int64_t)by_ny.div((uint64_t)k), j = k - i * ny;
+      label[k] = F.active[k] ? (int32_t)((i / fx) * ncy + j / fy) : -1;
+    }
+  });
+  for (int64_t k = 0; k < nf; ++k)  // one writer per block
     if (label[k] >= 0) cact[label[k]] = 1;
-  }
   const Blocks B{nx, ny, fx, fy, ncx, ncy, px, py};
   std::vector<int64_t> rep;
   if (every_row) {
@@ -960,27 +980,39 @@ int pcfd_mg_build(const pcfd_mg_operator* op, pcfd_mg_setup** out) {
     L.nx = op->nx;
     L.ny = op->ny;
     L.A.n = L.A.m = n;
-    L.A.ptr.assign(op->indptr, n + 1);
-    L.A.col.assign(op->indices, nnz);
-    L.A.val.assign(op->data, nnz);
-    // rows in ascending column order (each coarse row's sums then run in the
-    // order of its window, whatever the input's order)
+    // the caller's arrays serve as level 0's operator during the build,
+    // unless a row's columns are out of order (each coarse row's sums run in
+    // the order of its window): then a sorted copy
+    std::atomic<bool> unsorted{false};
     pool.run(n, [&](int, int64_t a, int64_t b) {
-      std::vector<std::pair<int32_t, double>> row;
-      for (int64_t i = a; i < b; ++i) {
-        const int64_t e0 = L.A.ptr[i], e1 = L.A.ptr[i + 1];
-        if (std::is_sorted(L.A.col.data() + e0, L.A.col.data() + e1)) continue;
-        row.clear();
-        for (int64_t e = e0; e < e1; ++e) row.push_back(std::make_pair(L.A.col[e], L.A.val[e]));
-        std::stable_sort(row.begin(), row.end(),
-                         [](const std::pair<int32_t, double>& x,
-                            const std::pair<int32_t, double>& y) { return x.first < y.first; });
-        for (int64_t e = e0; e < e1; ++e) {
-          L.A.col[e] = row[e - e0].first;
-          L.A.val[e] = row[e - e0].second;
-        }
-      }
+      for (int64_t i = a; i < b && !unsorted; ++i)
+        if (!std::is_sorted(op->indices + op->indptr[i], op->indices + op->indptr[i + 1]))
+          unsorted = true;
     });
+    if (!unsorted) {
+      L.A.ptr.view(op->indptr, n + 1);
+      L.A.col.view(op->indices, nnz);
+      L.A.val.view(op->data, nnz);
+    } else {
+      L.A.ptr.assign(op->indptr, n + 1);
+      L.A.col.assign(op->indices, nnz);
+      L.A.val.assign(op->data, nnz);
+      pool.run(n, [&](int, int64_t a, int64_t b) {
+        std::vector<std::pair<int32_t, double>> row;
+        for (int64_t i = a; i < b; ++i) {
+          const int64_t e0 = L.A.ptr[i], e1 = L.A.ptr[i + 1];
+          row.clear();
+          for (int64_t e = e0; e < e1; ++e) row.push_back(std::make_pair(L.A.col[e], L.A.val[e]));
+          std::stable_sort(row.begin(), row.end(),
+                           [](const std::pair<int32_t, double>& x,
+                              const std::pair<int32_t, double>& y) { return x.first < y.first; });
+          for (int64_t e = e0; e < e1; ++e) {
+            L.A.col[e] = row[e - e0].first;
+            L.A.val[e] = row[e - e0].second;
+          }
+        }
+      });
+    }
     L.active.assign(op->active, op->active + n);
     for (auto& a : L.active) a = a ? 1 : 0;
     // the fine level's classes pay only when its rows repeat (a stretched
@@ -1007,6 +1039,9 @@ int pcfd_mg_build(const pcfd_mg_operator* op, pcfd_mg_setup** out) {
     set_last_error("pcfd_mg_build: out of host memory");
     return kErrBadArg;
   }
+  // level 0's operator is the caller's: not kept past the call
+  H->levels[0].nnz0 = H->levels[0].A.nnz();
+  H->levels[0].A = Csr{};
   Level& last = H->levels.back();
   last.pin = 0;
   for (size_t k = 0; k < last.active.size(); ++k)
@@ -1040,7 +1075,7 @@ int pcfd_mg_level_info(const pcfd_mg_setup* h, int32_t level, pcfd_mg_built_leve
   info->ny = L.ny;
   info->fx = L.fx;
   info->fy = L.fy;
-  info->nnz = L.A.nnz();
+  info->nnz = L.nnz0 >= 0 ? L.nnz0 : L.A.nnz();
   info->weight = L.weight;
   info->pin = L.pin;
   info->n_offsets = (int32_t)L.di.size();

@@ -57,6 +57,7 @@ int main(void) {{
   O(pcfd_mg_level_desc, coarse_inv)
   S(pcfd_solve_info) O(pcfd_solve_info, residual) O(pcfd_solve_info, status)
   S(pcfd_mg_operator) O(pcfd_mg_operator, x0) O(pcfd_mg_operator, every_row)
+  S(pcfd_poisson_assembly) O(pcfd_poisson_assembly, cx_face) O(pcfd_poisson_assembly, p_ref)
   S(pcfd_mg_built_level) O(pcfd_mg_built_level, nnz) O(pcfd_mg_built_level, weight)
   return 0;
 }}
@@ -69,6 +70,7 @@ int main(void) {{
         "pcfd_case_desc": _lib.CaseDesc, "pcfd_solver_desc": _lib.SolverDesc,
         "pcfd_mg_level_desc": _lib.MGLevelDesc, "pcfd_solve_info": _lib.SolveInfo,
         "pcfd_mg_operator": _lib.MGOperator, "pcfd_mg_built_level": _lib.MGBuiltLevel,
+        "pcfd_poisson_assembly": _lib.PoissonAssembly,
     }
     for key, val in got.items():
         if "." in key:
