@@ -192,6 +192,14 @@ def load() -> C.CDLL:
             f"{LIB_PATH} is missing: build it with `python -m pycfd_amd.build` "
             "(the HIP engine has no CPU fallback)"
         )
+    # torch ships its own HIP runtime; loading it first makes the library's
+    # libamdhip64 dependency resolve to that one, so the process holds a
+    # single runtime (a second one, loaded first, leaves torch's unable to
+    # see the device)
+    try:
+        import torch  # noqa: F401
+    except ImportError:  # pragma: no cover - torch is a dependency
+        pass
     lib = C.CDLL(str(LIB_PATH))
     for name, args in _SIGNATURES.items():
         fn = getattr(lib, name)

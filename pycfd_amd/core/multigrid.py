@@ -64,10 +64,11 @@ class _Built:
 
     def __init__(self, handle: C.c_void_p) -> None:
         self.handle = handle
+        self._free = _lib.load().pcfd_mg_free  # still reachable at interpreter exit
 
     def __del__(self):
         if getattr(self, "handle", None) is not None and self.handle.value:
-            _lib.load().pcfd_mg_free(self.handle)
+            self._free(self.handle)
             self.handle = None
 
     def info(self, level: int) -> _lib.MGBuiltLevel:

@@ -565,8 +565,10 @@ int pcfd_ctx_create(const pcfd_case_desc* d, void* hip_stream, pcfd_ctx** out) {
     return kErrBadArg;
   }
   int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
-    set_last_error("no HIP device available");
+  const hipError_t derr = hipGetDeviceCount(&ndev);
+  if (derr != hipSuccess || ndev == 0) {
+    set_last_error("no HIP device available (hipGetDeviceCount: %s, %d devices)",
+                   hipGetErrorString(derr), ndev);
     return kErrNoDevice;
   }
   std::unique_ptr<pcfd_ctx> c(new (std::nothrow) pcfd_ctx());

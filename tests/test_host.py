@@ -361,17 +361,13 @@ def _cylinder_system(nx, ny):
                                    bm.dirichlet_pressure() or None)
 
 
-def _native_cases():
-    yield from ((n, product_system(n)) for n in ("cavity", "periodic", "channel", "cyl_pout",
-                                                 "s_cyl", "s_pchannel"))
-    yield "cylinder256x128", _cylinder_system(256, 128)
-
-
-@pytest.mark.parametrize("name,S", list(_native_cases()))
-def test_native_translated_rows_equal_every_row(name, S):
+@pytest.mark.parametrize("name", ["cavity", "periodic", "channel", "cyl_pout", "s_cyl",
+                                  "s_pchannel", "cylinder256x128"])
+def test_native_translated_rows_equal_every_row(name):
     """Forming each distinct coarse row once and translating it gives the
     operator that forming every row gives, bit for bit; so does any thread
     count."""
+    S = _cylinder_system(256, 128) if name == "cylinder256x128" else product_system(name)
     a = _native_levels(S, threads=4, every_row=1)
     b = _native_levels(S, threads=4, every_row=0)
     c = _native_levels(S, threads=1, every_row=0)
