@@ -495,8 +495,22 @@ __device__ void mask_obstacle(const BCParams& P, int mode, double dt, double* u,
     }
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < P.n_uf; k += blockDim.x) u[P.uf[k]] = 0.0;
-  for (int k = threadIdx.x; k < P.n_vf; k += blockDim.x) v[P.vf[k]] = 0.0;
+  // the faces zeroed: a chunk's u and v face indices loaded together, then
+  // the stores (one memory round trip a chunk)
+  for (int base = 0; base < max(P.n_uf, P.n_vf); base += kSlots * (int)blockDim.x) {
+    int uid[kSlots], vid[kSlots];
+#pragma unroll
+    for (int t = 0; t < kSlots; ++t) {
+      const int k = base + t * (int)blockDim.x + (int)threadIdx.x;
+      uid[t] = k < P.n_uf ? P.uf[k] : -1;
+      vid[t] = k < P.n_vf ? P.vf[k] : -1;
+    }
+#pragma unroll
+    for (int t = 0; t < kSlots; ++t) {
+      if (uid[t] >= 0) u[uid[t]] = 0.0;
+      if (vid[t] >= 0) v[vid[t]] = 0.0;
+    }
+  }
   __syncthreads();
 }
 
