@@ -88,6 +88,11 @@ class Pool {
     std::unique_lock<std::mutex> lk(mu_);
     done_.wait(lk, [this] { return pending_ == 0; });
     job_ = nullptr;
+    if (error_) {  // the first failure of any thread, once all have finished
+      std::exception_ptr e = error_;
+      error_ = nullptr;
+      std::rethrow_exception(e);
+    }
   }
   // The range of thread t for n items.
   void bounds(int64_t n, int t, int64_t& a, int64_t& b, int64_t serial = kSerialRows) const {
@@ -101,7 +106,13 @@ class Pool {
   void range(int t, const std::function<void(int, int64_t, int64_t)>& f) {
     int64_t a, b;
     bounds(n_, t, a, b, serial_);
-    if (a < b) f(t, a, b);
+    if (a >= b) return;
+    try {
+      f(t, a, b);
+    } catch (...) {
+      std::lock_guard<std::mutex> g(mu_);
+      if (!error_) error_ = std::current_exception();
+    }
   }
   void loop(int t) {
     int seen = 0;
@@ -131,6 +142,7 @@ class Pool {
   int64_t n_ = 0, serial_ = kSerialRows;
   int pending_ = 0, gen_ = 0;
   bool stop_ = false;
+  std::exception_ptr error_;
 };
 
 // A host array left uninitialised on allocation (the build writes every
@@ -955,6 +967,10 @@ int pcfd_mg_build(const pcfd_mg_operator* op, pcfd_mg_setup** out) {
   }
   *out = nullptr;
   const int64_t n = (int64_t)op->nx * op->ny;
+  if (n >= ((int64_t)1 << 31)) {
+    set_last_error("pcfd_mg_build: %lld cells exceed 32-bit columns", (long long)n);
+    return kErrUnsupported;
+  }
   const int64_t nnz = op->indptr[n];
   if (op->indptr[0] != 0 || nnz < 0) {
     set_last_error("pcfd_mg_build: indptr must start at 0");
@@ -974,8 +990,8 @@ int pcfd_mg_build(const pcfd_mg_operator* op, pcfd_mg_setup** out) {
   H->px = op->periodic_x != 0;
   H->py = op->periodic_y != 0;
   H->singular = op->singular != 0;
-  Pool pool(std::max(1, std::min(op->threads, 64)));
   try {
+    Pool pool(std::max(1, std::min(op->threads, 64)));
     Level L;
     L.nx = op->nx;
     L.ny = op->ny;
@@ -1037,6 +1053,10 @@ int pcfd_mg_build(const pcfd_mg_operator* op, pcfd_mg_setup** out) {
   } catch (const std::bad_alloc&) {
     delete H;
     set_last_error("pcfd_mg_build: out of host memory");
+    return kErrBadArg;
+  } catch (const std::exception& e) {
+    delete H;
+    set_last_error("pcfd_mg_build: %s", e.what());
     return kErrBadArg;
   }
   // level 0's operator is the caller's: not kept past the call
