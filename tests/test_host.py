@@ -442,3 +442,26 @@ def test_native_build_rejects_bad_operators():
         MH(bad, S.shape, S.fluid, S.singular)
     with pytest.raises(ValueError, match="do not fit"):
         MH(S.A, (S.shape[0] + 1, S.shape[1]), S.fluid, S.singular)
+
+
+@pytest.mark.parametrize("nx,ny,lx,ly", [(45, 31, 1.0, 1.0),     # odd extents: singleton blocks
+                                         (40, 24, 4.0, 0.5),     # strongly anisotropic cells
+                                         (33, 96, 1.0, 1.0)])
+def test_native_hierarchy_odd_and_anisotropic_grids(nx, ny, lx, ly):
+    """Odd extents (a last block of one cell on every level) and cells with an
+    aspect ratio past the semi-coarsening threshold: the native build gives
+    the oracle's grids, factors, sparsity and weights."""
+    mesh = StructuredMesh(nx, ny, lx, ly)
+    S = assemble_poisson_matrix(mesh)
+    R = O.Hierarchy(O.assemble(O.Case(nx=nx, ny=ny, lx=lx, ly=ly)).A, S.shape, S.fluid,
+                    S.singular, 1)
+    H = MultigridHierarchy(S.A, S.shape, S.fluid, S.singular)
+    assert H.unknowns == [lv["A"].shape[0] for lv in R.levels]
+    for lv, rl in zip(H.levels, R.levels):
+        assert lv.shape == rl["shape"] and lv.A.nnz == rl["A"].nnz
+        assert lv.weight == pytest.approx(rl["weight"], rel=1e-13)
+        assert abs(lv.A - rl["A"]).max() <= 1e-13 * abs(rl["A"]).max()
+    factors = [lv.factors for lv in H.levels[:-1]]
+    ratio = (lx / nx) ** 2 / (ly / ny) ** 2  # y strength over x strength on the fine grid
+    if ratio > 2.5 or ratio < 0.4:
+        assert (2, 1) in factors or (1, 2) in factors  # semi-coarsening taken
