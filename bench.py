@@ -4,8 +4,12 @@ Workload: flow past a cylinder, 2048 x 1024 channel (32 x 16 D), Re = 100,
 uniform inflow, zero-gradient OUTLET, slip walls, D = 1 body at 4 D, v seeded
 with a 1e-3 Gaussian blob; SSP-RK3, adaptive dt (ceiling 0.02, CFL 0.4);
 pressure by multigrid-preconditioned CG, its smoothing sweeps (V(s,s), s in
-1..3) picked by the in-project tuner on whole steps of the warmed-up run.
-Both precision paths run: plain
+1..3) picked by the in-project tuner on whole steps of the warmed-up run,
+each substep's solve started from the cubic extrapolation in time of that
+substep's last four solutions (--warm-start 3; 0 starts from the last
+pressure, the reference's way; the tolerance and the divergence target are
+the same either way, only the iteration count drops).  Both precision paths
+run it: plain
 FP64, and mixed (FP32 V-cycle inside FP64 CG), at the same fixed divergence
 target: the relative residual tolerance (2e-11 by default) is chosen so
 that max|div u| <= 1e-10 after every step on both paths from the first one
@@ -93,7 +97,7 @@ def load_traffic() -> dict:
 
 
 def run_path(precision: str, steps: int, warmup: int, nx: int, ny: int, tol: float,
-             traffic: dict, tune_sweeps: bool = True):
+             traffic: dict, tune_sweeps: bool = True, warm_start: int = 0):
     import numpy as np
     import torch
 
@@ -106,6 +110,7 @@ def run_path(precision: str, steps: int, warmup: int, nx: int, ny: int, tol: flo
                                        precision=precision)
     t0 = time.perf_counter()
     sim = Simulation(cfg, obstacle=body, u_init=u0, v_init=v0)
+    sim.solver.warm_start_order = warm_start
     log(f"[{precision}] setup {time.perf_counter() - t0:.1f} s "
         f"({sim.solver.pressure_solver.hierarchy})")
     plan = sim.solver.ctx.mg_plan()
@@ -304,6 +309,9 @@ def main() -> int:
                     help="keep V(1,1) (mg_sweeps=1) instead of the tuned sweeps")
     ap.add_argument("--no-others", action="store_true",
                     help="skip the other workloads (cavity 128 CG, Taylor-Green 1024, batch)")
+    ap.add_argument("--warm-start", type=int, default=3,
+                    help="start each substep's solve from the extrapolation in time of its "
+                         "last solutions (order 1-3; 0: the last pressure, the reference's)")
     ap.add_argument("--batch-streams", type=int, default=8)
     ap.add_argument("--batch-steps", type=int, default=10)
     args = ap.parse_args()
@@ -325,7 +333,7 @@ def main() -> int:
     for prec in args.paths.split(","):
         _, results[prec], kernels[prec] = run_path(prec, args.steps, args.warmup, args.nx,
                                                    args.ny, args.tol, traffic,
-                                                   not args.no_tune_sweeps)
+                                                   not args.no_tune_sweeps, args.warm_start)
     main_path = "mixed" if "mixed" in results else "fp64"
     r = results[main_path]
     # the dominant kernel of the reported path: largest share of its window
@@ -359,7 +367,9 @@ def main() -> int:
         "config": {"workload": f"cylinder Re=100 {args.nx}x{args.ny} (32x16 D), OUTLET, "
                                f"mgcg to max|div u| <= {DIV_TARGET:g} (tol {args.tol:g}), "
                                "SSP-RK3",
-                   "path": main_path, "nx": args.nx, "ny": args.ny},
+                   "path": main_path, "nx": args.nx, "ny": args.ny,
+                   "warm_start": (f"order-{args.warm_start} extrapolation in time per substep"
+                                  if args.warm_start else "previous pressure")},
         "paths": {k: {kk: (float(f"{vv:.5g}") if isinstance(vv, float) else vv)
                       for kk, vv in v.items() if not kk.startswith("_")}
                   for k, v in results.items()},

@@ -147,6 +147,7 @@ _SIGNATURES = {
                   C.c_void_p, C.c_void_p, C.POINTER(SolveInfo)],
     "pcfd_reset_guess": [C.c_void_p],
     "pcfd_set_step_graphs": [C.c_void_p, C.c_int32],
+    "pcfd_set_warm_start": [C.c_void_p, C.c_int32],
     "pcfd_soft_face_masks": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "pcfd_soft_tape_size": [C.c_void_p, C.POINTER(C.c_int64)],
     "pcfd_soft_step": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p,

@@ -238,6 +238,12 @@ class DeviceContext:
     def reset_guess(self) -> None:
         self._call("pcfd_reset_guess")
 
+    def set_warm_start(self, order: int) -> None:
+        """Warm start of each substep's solve by extrapolation in time of
+        that substep's last solutions (order 1-3; 0: the previous pressure,
+        the reference's) (pcfd_set_warm_start)."""
+        self._call("pcfd_set_warm_start", int(order))
+
     def set_step_graphs(self, enable: bool) -> None:
         """Capture whole steps as HIP graphs and replay them (pcfd_set_step_graphs)."""
         self._call("pcfd_set_step_graphs", int(bool(enable)))

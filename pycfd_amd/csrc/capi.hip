@@ -105,6 +105,13 @@ struct pcfd_ctx {
   double* stage_b[2] = {nullptr, nullptr};
   double* diag_out = nullptr;
   double* guess_backup = nullptr;
+  // warm start by extrapolation (pcfd_set_warm_start): per substep a ring of
+  // kWarmSlots solutions and its state (4 ints); a backup of the state for a
+  // captured step that is redone
+  int warm_order = 0;
+  double* warm_slots = nullptr;
+  int* warm_idx = nullptr;
+  int* warm_idx_backup = nullptr;
   double* host_out = nullptr;  // pinned
   double* scratch = nullptr;   // this device's copy of the no-allocation scratch
   double* soft_work = nullptr; // (nx*ny) cells: the transposed projection's right-hand side
@@ -152,6 +159,9 @@ struct pcfd_ctx {
     for (auto* p : stage_b) hipFree(p);
     hipFree(diag_out);
     hipFree(guess_backup);
+    hipFree(warm_slots);
+    hipFree(warm_idx);
+    hipFree(warm_idx_backup);
     hipFree(nuc);
     hipFree(metric);
     if (host_out) hipHostFree(host_out);
@@ -218,9 +228,26 @@ void fill_info(pcfd_solve_info* info, const SolveInfo& s) {
 
 // One projected Euler substep (solver.py:612-640).  `ue` non-null selects
 // the fused stage combination ca*u + cb*ue as the substep input.
+// The warm-start history of substep k (none when the warm start is the
+// previous pressure).
+WarmHistory warm_of(const pcfd_ctx* c, int k) {
+  WarmHistory h;
+  if (c->warm_order > 0 && c->warm_slots) {
+    h.slots = c->warm_slots + (size_t)k * kWarmSlots * c->nx * c->ny;
+    h.idx = c->warm_idx + 4 * k;
+  }
+  return h;
+}
+
+// Forget the extrapolation history (a new start for the guesses).
+int warm_reset(pcfd_ctx* c) {
+  if (c->warm_idx) PCFD_HIP_CHECK(hipMemsetAsync(c->warm_idx, 0, 12 * sizeof(int), c->stream));
+  return kOk;
+}
+
 int substep(pcfd_ctx* c, const double* u, const double* v, const double* ue, const double* ve,
             double ca, double cb, const StepScalars& S, double* us, double* vs, double* p,
-            SolveInfo* info, bool sync, bool last = true) {
+            SolveInfo* info, bool sync, bool last = true, int sidx = 0) {
   // The reaction force a substep books is overwritten by the next substep's
   // (solver.py:577-583, reset after each predictor), so only the step's last
   // substep sums it; the others just zero the faces.
@@ -244,9 +271,11 @@ int substep(pcfd_ctx* c, const double* u, const double* v, const double* ue, con
     rc = boundary(c, {BOP_VEL_PRED, mask_pred, BOP_WRAP_FACES}, S, us, vs);
   }
   if (rc) return rc;
+  const WarmHistory wh = warm_of(c, sidx);
+  if (wh.slots && (rc = c->engine.warm_guess(wh, c->warm_order))) return rc;
   if ((rc = c->engine.prepare_from_velocity(us, vs, c->ldu, c->ldv, c->dx, c->dy, S))) return rc;
   if ((rc = c->engine.solve(info, sync))) return rc;
-  if ((rc = c->engine.scatter_pressure(p, c->ldp))) return rc;
+  if ((rc = c->engine.scatter_pressure(p, c->ldp, wh))) return rc;
   // p, u*, v* in; u, v out
   if ((rc = launch_correct_ghosts(c->stream, p, c->ldp, c->bc, c->ldu, c->ldv, c->dx, c->dy,
                                   c->g, c->upd, S, us, vs,
@@ -273,7 +302,7 @@ int step_body(pcfd_ctx* c, const double* u0, const double* v0, const StepScalars
                  double ca, double cb, double* ua, double* va, bool last) -> int {
     E.set_log_slot(log ? k : -1);
     E.set_budget(budget ? budget[k] : 0);
-    const int r = substep(c, a, b, ae, be, ca, cb, S, ua, va, p, s, last && want, last);
+    const int r = substep(c, a, b, ae, be, ca, cb, S, ua, va, p, s, last && want, last, k);
     E.set_log_slot(-1);
     E.set_budget(0);
     return r;
@@ -337,6 +366,9 @@ int capture_step(pcfd_ctx* c, pcfd_ctx::StepGraph& g) {
     // runs out of iterations (its step is then redone from the same start)
     e = hipMemcpyAsync(c->guess_backup, c->engine.guess(), bytes, hipMemcpyDeviceToDevice,
                        c->cap_stream);
+    if (e == hipSuccess && c->warm_idx)  // and the extrapolation history's state
+      e = hipMemcpyAsync(c->warm_idx_backup, c->warm_idx, 12 * sizeof(int),
+                         hipMemcpyDeviceToDevice, c->cap_stream);
     if (e != hipSuccess) {
       set_last_error("hipMemcpyAsync in capture: %s", hipGetErrorString(e));
       rc = kErrHip;
@@ -462,6 +494,11 @@ int step_graph(pcfd_ctx* c, const double* u0, const double* v0, double dt, doubl
     PCFD_HIP_CHECK(hipMemcpyAsync(c->engine.guess(), c->guess_backup,
                                   (size_t)c->nx * c->ny * sizeof(double),
                                   hipMemcpyDeviceToDevice, c->stream));
+    // the history's slots the replay wrote are ones the redo's guesses do
+    // not read (kWarmSlots leaves one spare); its state goes back
+    if (c->warm_idx)
+      PCFD_HIP_CHECK(hipMemcpyAsync(c->warm_idx, c->warm_idx_backup, 12 * sizeof(int),
+                                    hipMemcpyDeviceToDevice, c->stream));
     return step_eager_logged(c, u0, v0, dt, blend, u, v, p, s);
   }
   for (int k = 0; k < ns; ++k) {
@@ -1122,6 +1159,7 @@ int pcfd_initialize(pcfd_ctx* c, double* u, double* v, double* p, pcfd_solve_inf
   }
   if (rc) return rc;
   SolveInfo s{};
+  if ((rc = warm_reset(c))) return rc;
   if ((rc = project_impl(c, u, v, 1.0, p, &s))) return rc;
   if ((rc = boundary(c, {BOP_MASK_ZERO, BOP_WRAP_FACES, BOP_VEL_CORR}, 1.0, u, v))) return rc;
   PCFD_HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -1243,7 +1281,25 @@ int pcfd_step_graph_stats(pcfd_ctx* c, int32_t* out3) {
 
 int pcfd_reset_guess(pcfd_ctx* c) {
   if (int rc = check_ctx(c)) return rc;
+  if (int rc = warm_reset(c)) return rc;
   return c->engine.reset_guess();
+}
+
+int pcfd_set_warm_start(pcfd_ctx* c, int32_t order) {
+  if (int rc = check_ctx(c)) return rc;
+  if (order < 0 || order > 3) {
+    set_last_error("warm-start extrapolation order must be 0-3, got %d", order);
+    return kErrBadArg;
+  }
+  if (order > 0 && !c->warm_slots) {
+    const size_t n = (size_t)c->nx * c->ny;
+    PCFD_HIP_CHECK(hipMalloc(&c->warm_slots, 3 * kWarmSlots * n * sizeof(double)));
+    PCFD_HIP_CHECK(hipMalloc(&c->warm_idx, 12 * sizeof(int)));
+    PCFD_HIP_CHECK(hipMalloc(&c->warm_idx_backup, 12 * sizeof(int)));
+  }
+  c->warm_order = order;
+  for (auto& g : c->graphs) g.stale = true;  // the captured steps change
+  return warm_reset(c);
 }
 
 int pcfd_pressure_guess(pcfd_ctx* c, double* buf, int32_t store) {
@@ -1254,6 +1310,7 @@ int pcfd_pressure_guess(pcfd_ctx* c, double* buf, int32_t store) {
   }
   const size_t bytes = (size_t)c->nx * c->ny * sizeof(double);
   if (store) {
+    if (int rc = warm_reset(c)) return rc;
     PCFD_HIP_CHECK(
         hipMemcpyAsync(c->engine.guess(), buf, bytes, hipMemcpyDeviceToDevice, c->stream));
   } else {

@@ -750,11 +750,13 @@ k_check(FineOp A, const double* __restrict__ x, const double* __restrict__ b, in
 // ring zero.  The same values become the next warm-start guess.
 __global__ void __launch_bounds__(kBlock)
 k_scatter(FineOp A, double* __restrict__ x, const SolveState* st, double* __restrict__ p,
-          int ldp, double* __restrict__ out) {
+          int ldp, double* __restrict__ out, WarmHistory wh) {
   const Tile2D t = tile_coords(A.ny + 2);
   const int i = t.i, j = t.j;
   if (i > A.nx + 1 || j > A.ny + 1) return;
   const bool interior = i >= 1 && i <= A.nx && j >= 1 && j <= A.ny;
+  // the history slot this solution goes to (chosen by the guess kernel)
+  double* hist = wh.slots ? wh.slots + (size_t)wh.idx[2] * A.nx * A.ny : nullptr;
   double val = 0.0;
   if (interior) {
     const int k = (i - 1) * A.ny + (j - 1);
@@ -764,8 +766,46 @@ k_scatter(FineOp A, double* __restrict__ x, const SolveState* st, double* __rest
     }
     x[k] = val;
     if (out) out[k] = val;
+    if (hist) hist[k] = val;
   }
   if (p) p[(size_t)i * ldp + j] = val;
+  if (wh.slots && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+    wh.idx[0] = wh.idx[2];  // the newest solution and the count, for the next guess
+    wh.idx[1] = wh.idx[3];
+  }
+}
+
+// Warm start by polynomial extrapolation in time: the guess of this solve is
+// sum_j w_j x_(n-j) over the last order+1 solutions of the same substep (a
+// ring of kWarmSlots), with the weights of equal steps: 2, -1; 3, -3, 1;
+// 4, -6, 4, -1.  Fewer stored solutions lower the order; with one or none the
+// guess is left as it is (the previous solve's pressure).  The ring's state
+// is read from idx[0..1] (newest slot, count) and the next state written to
+// idx[2..3], which only the scatter reads (and copies back): no launch reads
+// what it writes.
+__global__ void __launch_bounds__(kBlock)
+k_warm_guess(int n, double* __restrict__ x, WarmHistory wh, int order) {
+  const int head = wh.idx[0], cnt = wh.idx[1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    wh.idx[2] = (head + 1) % kWarmSlots;
+    wh.idx[3] = min(cnt + 1, kWarmSlots);
+  }
+  const int o = min(order, cnt - 1);
+  if (o < 1) return;
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= n) return;
+  auto at = [&](int back) {
+    return wh.slots[(size_t)((head - back + kWarmSlots) % kWarmSlots) * n + k];
+  };
+  double g;
+  if (o == 1) {
+    g = 2.0 * at(0) - at(1);
+  } else if (o == 2) {
+    g = 3.0 * at(0) - 3.0 * at(1) + at(2);
+  } else {
+    g = 4.0 * at(0) - 6.0 * at(1) + 4.0 * at(2) - at(3);
+  }
+  x[k] = g;
 }
 
 // ----------------------------------------------------------------------------
@@ -2279,18 +2319,26 @@ int PoissonEngine::read_log(SolveLog* out, int n) {
   return kOk;
 }
 
-int PoissonEngine::scatter_pressure(double* p, int ldp) {
-  // algorithmic bytes: x, flags; x (normalised guess), p
-  const LaunchClock clk = clock_step(CK_SCATTER, (double)n_ * (8 + 1 + 8 + 8));
+int PoissonEngine::scatter_pressure(double* p, int ldp, const WarmHistory& wh) {
+  // algorithmic bytes: x, flags; x (normalised guess), p (and the history slot)
+  const LaunchClock clk =
+      clock_step(CK_SCATTER, (double)n_ * (8 + 1 + 8 + 8 + (wh.slots ? 8 : 0)));
   PCFD_LAUNCH_TIMED(clk, k_scatter, tiles_for(op_.nx + 2, op_.ny + 2), dim3(kBlock), 0, s_, op_,
-                    x_, (const SolveState*)st_, p, ldp, (double*)nullptr);
+                    x_, (const SolveState*)st_, p, ldp, (double*)nullptr, wh);
+  PCFD_LAUNCH_CHECK();
+  return kOk;
+}
+
+int PoissonEngine::warm_guess(const WarmHistory& wh, int order) {
+  hipLaunchKernelGGL(k_warm_guess, dim3((n_ + kBlock - 1) / kBlock), dim3(kBlock), 0, s_, n_, x_,
+                     wh, order);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }
 
 int PoissonEngine::write_interior(double* out) {
   hipLaunchKernelGGL(k_scatter, tiles_for(op_.nx + 2, op_.ny + 2), dim3(kBlock), 0, s_, op_, x_,
-                     (const SolveState*)st_, (double*)nullptr, 0, out);
+                     (const SolveState*)st_, (double*)nullptr, 0, out, WarmHistory{});
   PCFD_LAUNCH_CHECK();
   return kOk;
 }

@@ -66,6 +66,15 @@ struct ClassLevel {
   int spec_e0, spec_e1;  // entry range of the level's most frequent row
 };
 
+// The last solutions of one substep's solve, for the warm start by
+// extrapolation in time: kWarmSlots arrays of nx*ny (a ring) and its state
+// idx = {newest slot, count, next slot, next count}.  Null slots: no history.
+constexpr int kWarmSlots = 5;  // order 3 reads four; the slot written next is never read
+struct WarmHistory {
+  double* slots = nullptr;
+  int* idx = nullptr;
+};
+
 // Device-resident scalars of one solve.  Kernels read them directly so the
 // host never has to synchronise inside an iteration.
 struct SolveState {
@@ -207,7 +216,11 @@ class PoissonEngine {
                             double dy, const StepScalars& T);
   int prepare_from_rhs(const double* rhs);
   int solve(SolveInfo* info, bool sync_info);
-  int scatter_pressure(double* p, int ldp);   // ghosted p, ghost ring zeroed
+  // ghosted p, ghost ring zeroed; with a history the solution also goes to
+  // its ring (the warm start by extrapolation)
+  int scatter_pressure(double* p, int ldp, const WarmHistory& wh = WarmHistory{});
+  // the guess of the next solve extrapolated from the history (order 1-3)
+  int warm_guess(const WarmHistory& wh, int order);
   int write_interior(double* out);            // normalised (nx, ny)
   int reset_guess();
   int set_guess(const double* x0);
