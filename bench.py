@@ -58,7 +58,7 @@ def log(msg: str) -> None:
 # each); k_walls and k_boundary_pass touch O(nx + ny) values: no fraction
 STEP_KERNELS = ("k_cg_update", "k_predictor", "k_predictor_comb", "k_walls", "k_boundary_pass",
                 "k_prepare", "k_init", "k_check", "k_scatter", "k_correct_ghosts", "k_combine",
-                "k_diagnostics")
+                "k_diagnostics", "k_warm_guess")
 
 
 def kernel_name(entry: dict, precision: str, plan: list[dict]) -> str:

@@ -427,9 +427,9 @@ int pcfd_time_kernel(pcfd_ctx* ctx, int32_t which, int32_t reps, double* avg_ms,
  * level 0: 14 predictor, 15 stage-combining predictor, 16 wall conditions,
  * 17 one-workgroup boundary pass, 18 right-hand side, 19 solve start, 20
  * final residual check, 21 pressure scatter, 22 gradient correction, 23
- * stage combination, 24 diagnostics, 25 one-workgroup Jacobi-CG solve.  No
- * reference counterpart: measurement only. */
-#define PCFD_CLOCK_KINDS 26
+ * stage combination, 24 diagnostics, 25 one-workgroup Jacobi-CG solve, 26
+ * warm-start extrapolation.  No reference counterpart: measurement only. */
+#define PCFD_CLOCK_KINDS 27
 #define PCFD_CLOCK_LEVELS 16
 #define PCFD_CLOCK_SLOTS (PCFD_CLOCK_KINDS * PCFD_CLOCK_LEVELS)
 int pcfd_kernel_clock_start(pcfd_ctx* ctx, int32_t max_launches);

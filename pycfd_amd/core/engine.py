@@ -286,7 +286,8 @@ class DeviceContext:
                    "k_cls_rows[res]", "k_cls_rows[prolong]", "k_cls_restrict", "k_dense_apply",
                    "k_chain", "k_fold", "k_predictor", "k_predictor_comb", "k_walls",
                    "k_boundary_pass", "k_prepare", "k_init", "k_check", "k_scatter",
-                   "k_correct_ghosts", "k_combine", "k_diagnostics", "k_jpcg_small")
+                   "k_correct_ghosts", "k_combine", "k_diagnostics", "k_jpcg_small",
+                   "k_warm_guess")
     CLOCK_LEVELS = 16
 
     def kernel_clock_start(self, max_launches: int) -> None:

@@ -2330,8 +2330,11 @@ int PoissonEngine::scatter_pressure(double* p, int ldp, const WarmHistory& wh) {
 }
 
 int PoissonEngine::warm_guess(const WarmHistory& wh, int order) {
-  hipLaunchKernelGGL(k_warm_guess, dim3((n_ + kBlock - 1) / kBlock), dim3(kBlock), 0, s_, n_, x_,
-                     wh, order);
+  // algorithmic bytes with the history full: order + 1 solutions read, the
+  // guess written (the first calls after a restart read fewer)
+  const LaunchClock clk = clock_step(CK_WARM, (double)n_ * 8 * (order + 2));
+  PCFD_LAUNCH_TIMED(clk, k_warm_guess, dim3((n_ + kBlock - 1) / kBlock), dim3(kBlock), 0, s_, n_,
+                    x_, wh, order);
   PCFD_LAUNCH_CHECK();
   return kOk;
 }

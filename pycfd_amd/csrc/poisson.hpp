@@ -254,13 +254,13 @@ class PoissonEngine {
   // launch.  Iterations enqueued past convergence (no-ops) are dropped.
   // The step's other kernels (predictors, boundary passes, right-hand side,
   // start, final check, scatter, correction, stage combination,
-  // diagnostics) are filed under their own kinds at level 0; the solve's
+  // diagnostics, warm-start guess) are filed under their own kinds at level 0; the solve's
   // once-per-solve folds under CK_FOLD at levels kFoldPrepare.. .
   enum ClockKind : int {
     CK_UPDATE = 0, CK_DIR, CK_FINE_DOWN, CK_FINE_UP, CK_CLASS_DOWN, CK_CLASS_UP,
     CK_ROWS_SMOOTH0, CK_ROWS_SMOOTH, CK_ROWS_RES, CK_ROWS_PROLONG, CK_RESTRICT, CK_DENSE,
     CK_CHAIN, CK_FOLD, CK_PREDICTOR, CK_PREDICTOR_COMB, CK_WALLS, CK_BPASS, CK_PREPARE, CK_INIT,
-    CK_CHECK, CK_SCATTER, CK_CORRECT, CK_COMBINE, CK_DIAG, CK_SMALL_CG, kClockKinds
+    CK_CHECK, CK_SCATTER, CK_CORRECT, CK_COMBINE, CK_DIAG, CK_SMALL_CG, CK_WARM, kClockKinds
   };
   enum FoldLevel : int { kFoldPrepare = 3, kFoldInit, kFoldCheck, kFoldDiag };
   static constexpr int kClockLevels = 16, kClockSlots = kClockKinds * kClockLevels;

@@ -41,6 +41,21 @@ def test_exports_every_declared_symbol():
     assert set(_lib.EXPORTED) == declared()
 
 
+def test_clock_slots_match_header():
+    """The live clock's slot table: the binding names as many kinds as the
+    header counts (the C side files its kinds in the same order)."""
+    from pycfd_amd.core.engine import DeviceContext as Engine
+    text = HEADER.read_text()
+    kinds = int(re.search(r"#define PCFD_CLOCK_KINDS (\d+)", text).group(1))
+    levels = int(re.search(r"#define PCFD_CLOCK_LEVELS (\d+)", text).group(1))
+    assert len(Engine.CLOCK_KINDS) == kinds
+    assert Engine.CLOCK_LEVELS == levels
+    src = (ROOT / "pycfd_amd" / "csrc" / "poisson.hpp").read_text()
+    enum = re.search(r"enum ClockKind : int \{(.*?)\};", src, flags=re.S).group(1)
+    names = [x.split("=")[0].strip() for x in enum.split(",") if x.strip()]
+    assert names[-1] == "kClockKinds" and len(names) - 1 == kinds
+
+
 def test_struct_layouts_match_header(tmp_path):
     src = tmp_path / "layout.c"
     src.write_text(f"""
