@@ -21,9 +21,9 @@ pytestmark = pytest.mark.gpu
 GRID = dict(nx=192, ny=96)
 
 
-def _sim(order, graphs=False, scheme=None, name="cyl_outlet"):
+def _sim(order, graphs=False, scheme=None, name="cyl_outlet", solver="mgcg", tol=1e-12):
     from pycfd_amd.physics.incompressible import Simulation
-    kw = dict(pressure_solver="mgcg", poisson_tol=1e-12, **GRID)
+    kw = dict(pressure_solver=solver, poisson_tol=tol, **GRID)
     if scheme:
         kw["time_scheme"] = scheme
     cfg, body = product_config(name, **kw)
@@ -59,6 +59,25 @@ def test_extrapolated_start_keeps_the_solution(gpu_available, order):
     late = slice(order + 2, None)
     assert all(a <= b for a, b in zip(it_ext[late], it_ref[late])), (it_ext, it_ref)
     assert sum(it_ext[late]) < sum(it_ref[late]), (it_ext, it_ref)
+
+
+@pytest.mark.parametrize("solver", ["cg", "sor", "multigrid", "direct"])
+def test_extrapolated_start_other_solvers(gpu_available, solver):
+    """Every solver starts from the engine's guess, so the extrapolation
+    applies to all of them: the fields stay those of the reference-started
+    run to the solver's tolerance and the solves take no more iterations in
+    all (SOR tests its residual every fifth sweep)."""
+    steps = 10
+    ref = _sim(0, solver=solver, tol=1e-11)
+    ext = _sim(3, solver=solver, tol=1e-11)
+    it_ref = _iterations(ref, steps)
+    it_ext = _iterations(ext, steps)
+    print(f"{solver}: iterations {it_ext} against {it_ref}")
+    scale = max(1.0, float(np.abs(ref.fields.u).max()))
+    assert np.abs(ext.fields.u - ref.fields.u).max() <= 1e-7 * scale
+    assert np.abs(ext.fields.v - ref.fields.v).max() <= 1e-7 * scale
+    late = slice(5, None)
+    assert sum(it_ext[late]) <= sum(it_ref[late]), (it_ext, it_ref)
 
 
 def test_extrapolated_start_matches_oracle(gpu_available):
