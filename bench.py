@@ -5,12 +5,11 @@ uniform inflow, zero-gradient OUTLET, slip walls, D = 1 body at 4 D, v seeded
 with a 1e-3 Gaussian blob; SSP-RK3, adaptive dt (ceiling 0.02, CFL 0.4);
 pressure by multigrid-preconditioned CG, its smoothing sweeps (V(s,s), s in
 1..3) picked by the in-project tuner on whole steps of the warmed-up run,
-each substep's solve started from the cubic extrapolation in time of that
-substep's last four solutions (--warm-start 3; 0 starts from the last
+each substep's solve started from the quartic extrapolation in time of that
+substep's last five solutions (--warm-start 4; 0 starts from the last
 pressure, the reference's way; the tolerance and the divergence target are
 the same either way, only the iteration count drops).  Both precision paths
-run it: plain
-FP64, and mixed (FP32 V-cycle inside FP64 CG), at the same fixed divergence
+run it: plain FP64, and mixed (FP32 V-cycle inside FP64 CG), at the same fixed divergence
 target: the relative residual tolerance (2e-11 by default) is chosen so
 that max|div u| <= 1e-10 after every step on both paths from the first one
 (the start-up steps are the largest: 5.6e-11; 2.5e-11 let steps 2-4 reach
@@ -309,9 +308,9 @@ def main() -> int:
                     help="keep V(1,1) (mg_sweeps=1) instead of the tuned sweeps")
     ap.add_argument("--no-others", action="store_true",
                     help="skip the other workloads (cavity 128 CG, Taylor-Green 1024, batch)")
-    ap.add_argument("--warm-start", type=int, default=3,
+    ap.add_argument("--warm-start", type=int, default=4,
                     help="start each substep's solve from the extrapolation in time of its "
-                         "last solutions (order 1-3; 0: the last pressure, the reference's)")
+                         "last solutions (order 1-4; 0: the last pressure, the reference's)")
     ap.add_argument("--batch-streams", type=int, default=8)
     ap.add_argument("--batch-steps", type=int, default=10)
     args = ap.parse_args()

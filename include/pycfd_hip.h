@@ -317,12 +317,12 @@ int pcfd_step(pcfd_ctx* ctx, const double* u0, const double* v0, double dt, doub
 int pcfd_set_step_graphs(pcfd_ctx* ctx, int32_t enable);
 /* Warm start of each substep's pressure solve (no counterpart in the
  * reference, whose ProjectionSolver._p_guess (solver.py:171, 662-663) is the
- * last pressure solved, order 0 here and the default): order 1-3 starts the
+ * last pressure solved, order 0 here and the default): order 1-4 starts the
  * solve of RK substep s from the polynomial extrapolation in time of substep
  * s's last order+1 solutions (equal steps assumed), fewer while fewer are
  * stored.  The solve's tolerance is unchanged; only its iteration count
  * drops.  pcfd_reset_guess, pcfd_pressure_guess (store) and pcfd_initialize
- * restart the history.  Holds 15 nx*ny FP64 arrays once enabled. */
+ * restart the history.  Holds 18 nx*ny FP64 arrays once enabled. */
 int pcfd_set_warm_start(pcfd_ctx* ctx, int32_t order);
 /* Graph replays, captures and eager redos since the context was created. */
 int pcfd_step_graph_stats(pcfd_ctx* ctx, int32_t* out3);

@@ -200,20 +200,23 @@ class ProjectionSolver:
     @property
     def warm_start_order(self) -> int:
         """How each substep's pressure solve is started: 0 (the default, the
-        reference's ``_p_guess``) from the last pressure solved; 1-3 from the
-        polynomial extrapolation in time of the same substep's last 2-4
+        reference's ``_p_guess``) from the last pressure solved; 1-4 from the
+        polynomial extrapolation in time of the same substep's last 2-5
         solutions, equal steps assumed (2 p1 - p2; 3 p1 - 3 p2 + p3;
-        4 p1 - 6 p2 + 4 p3 - p4).  The solves still stop on the same
+        4 p1 - 6 p2 + 4 p3 - p4; 5 p1 - 10 p2 + 10 p3 - 5 p4 + p5).  Weights
+        from the actual step times change nothing measurable (the adaptive
+        dt varies by a fraction of a percent a step), and orders above 4
+        saved no further iterations.  The solves still stop on the same
         tolerance; a closer start only saves iterations (on the 2048x1024
-        cylinder 11 -> 7 per solve at order 3 once four steps are stored).
+        cylinder 11 -> 6 per solve at order 4 once five steps are stored).
         Resetting or setting the pressure guess restarts the history."""
         return self._warm_order
 
     @warm_start_order.setter
     def warm_start_order(self, order: int) -> None:
         order = int(order)
-        if not 0 <= order <= 3:
-            raise ValueError(f"warm_start_order must be 0-3, got {order}")
+        if not 0 <= order <= 4:
+            raise ValueError(f"warm_start_order must be 0-4, got {order}")
         self.ctx.set_warm_start(order)
         self._warm_order = order
 

@@ -1287,8 +1287,8 @@ int pcfd_reset_guess(pcfd_ctx* c) {
 
 int pcfd_set_warm_start(pcfd_ctx* c, int32_t order) {
   if (int rc = check_ctx(c)) return rc;
-  if (order < 0 || order > 3) {
-    set_last_error("warm-start extrapolation order must be 0-3, got %d", order);
+  if (order < 0 || order > kWarmMaxOrder) {
+    set_last_error("warm-start extrapolation order must be 0-%d, got %d", kWarmMaxOrder, order);
     return kErrBadArg;
   }
   if (order > 0 && !c->warm_slots) {

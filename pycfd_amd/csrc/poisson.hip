@@ -777,12 +777,28 @@ k_scatter(FineOp A, double* __restrict__ x, const SolveState* st, double* __rest
 
 // Warm start by polynomial extrapolation in time: the guess of this solve is
 // sum_j w_j x_(n-j) over the last order+1 solutions of the same substep (a
-// ring of kWarmSlots), with the weights of equal steps: 2, -1; 3, -3, 1;
-// 4, -6, 4, -1.  Fewer stored solutions lower the order; with one or none the
-// guess is left as it is (the previous solve's pressure).  The ring's state
-// is read from idx[0..1] (newest slot, count) and the next state written to
-// idx[2..3], which only the scatter reads (and copies back): no launch reads
-// what it writes.
+// ring of kWarmSlots), with the weights of equal steps, w_j = (-1)^j
+// C(order + 1, j + 1): 2, -1; 3, -3, 1; 4, -6, 4, -1; ...  Fewer stored
+// solutions lower the order; with one or none the guess is left as it is
+// (the previous solve's pressure).  The ring's state is read from idx[0..1]
+// (newest slot, count) and the next state written to idx[2..3], which only
+// the scatter reads (and copies back): no launch reads what it writes.
+template <int O>
+__device__ __forceinline__ double extrapolate(const double* __restrict__ slots, int head, int n,
+                                              int k) {
+  double v[O + 1];
+#pragma unroll
+  for (int j = 0; j <= O; ++j)  // all loads first, then the sum in a fixed order
+    v[j] = slots[(size_t)((head - j + kWarmSlots) % kWarmSlots) * n + k];
+  double g = 0.0, c = 1.0;
+#pragma unroll
+  for (int j = 0; j <= O; ++j) {
+    c = c * (O + 1 - j) / (j + 1);  // C(O + 1, j + 1), an exact integer
+    g += ((j & 1) ? -c : c) * v[j];
+  }
+  return g;
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_warm_guess(int n, double* __restrict__ x, WarmHistory wh, int order) {
   const int head = wh.idx[0], cnt = wh.idx[1];
@@ -794,16 +810,13 @@ k_warm_guess(int n, double* __restrict__ x, WarmHistory wh, int order) {
   if (o < 1) return;
   const int k = blockIdx.x * kBlock + threadIdx.x;
   if (k >= n) return;
-  auto at = [&](int back) {
-    return wh.slots[(size_t)((head - back + kWarmSlots) % kWarmSlots) * n + k];
-  };
+  static_assert(kWarmMaxOrder == 4, "one case per order");
   double g;
-  if (o == 1) {
-    g = 2.0 * at(0) - at(1);
-  } else if (o == 2) {
-    g = 3.0 * at(0) - 3.0 * at(1) + at(2);
-  } else {
-    g = 4.0 * at(0) - 6.0 * at(1) + 4.0 * at(2) - at(3);
+  switch (o) {
+    case 1: g = extrapolate<1>(wh.slots, head, n, k); break;
+    case 2: g = extrapolate<2>(wh.slots, head, n, k); break;
+    case 3: g = extrapolate<3>(wh.slots, head, n, k); break;
+    default: g = extrapolate<4>(wh.slots, head, n, k); break;
   }
   x[k] = g;
 }

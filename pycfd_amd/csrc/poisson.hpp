@@ -69,7 +69,9 @@ struct ClassLevel {
 // The last solutions of one substep's solve, for the warm start by
 // extrapolation in time: kWarmSlots arrays of nx*ny (a ring) and its state
 // idx = {newest slot, count, next slot, next count}.  Null slots: no history.
-constexpr int kWarmSlots = 5;  // order 3 reads four; the slot written next is never read
+constexpr int kWarmMaxOrder = 4;  // orders 5 and 6 measured no better
+// order o reads o + 1; the slot written next is never read
+constexpr int kWarmSlots = kWarmMaxOrder + 2;
 struct WarmHistory {
   double* slots = nullptr;
   int* idx = nullptr;

@@ -42,7 +42,7 @@ def _iterations(sim, steps):
     return its
 
 
-@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
 def test_extrapolated_start_keeps_the_solution(gpu_available, order):
     steps = 14
     ref = _sim(0)
@@ -94,13 +94,13 @@ def test_extrapolated_start_matches_oracle(gpu_available):
     assert np.abs(sim.fields.v - v).max() <= 1e-10 * scale
 
 
-@pytest.mark.parametrize("scheme", ["rk3", "rk2"])
-def test_extrapolated_start_graphs_equal_eager(gpu_available, scheme):
+@pytest.mark.parametrize("scheme,order", [("rk3", 3), ("rk2", 3), ("rk3", 4)])
+def test_extrapolated_start_graphs_equal_eager(gpu_available, scheme, order):
     """Captured steps keep the history's state on the device: replays give the
     eager run's fields and iteration counts bit for bit."""
     steps = 12
-    a = _sim(3, graphs=False, scheme=scheme)
-    b = _sim(3, graphs=True, scheme=scheme)
+    a = _sim(order, graphs=False, scheme=scheme)
+    b = _sim(order, graphs=True, scheme=scheme)
     ia = _iterations(a, steps)
     ib = _iterations(b, steps)
     assert b.solver.ctx.step_graph_stats()["launches"] >= steps - 2
@@ -132,6 +132,6 @@ def test_setting_the_guess_restarts_the_history(gpu_available):
 
 def test_order_is_checked(gpu_available):
     sim = _sim(0)
-    with pytest.raises(ValueError, match="0-3"):
-        sim.solver.warm_start_order = 4
+    with pytest.raises(ValueError, match="0-4"):
+        sim.solver.warm_start_order = 5
     assert sim.solver.warm_start_order == 0
