@@ -195,10 +195,13 @@ def run_path(precision: str, steps: int, warmup: int, nx: int, ny: int, tol: flo
     return sim, out, kernels
 
 
-def other_workloads(streams: int, batch_steps: int) -> dict:
+def other_workloads(streams: int, batch_steps: int, warm_start: int = 0) -> dict:
     """The other configured workloads, each a short timed window after a
     warm-up (not the headline): the 128 x 128 CG cavity, Taylor-Green 1024^2
-    with multigrid V(2,2), and the 96 batched cavities advanced together."""
+    with multigrid V(2,2), and the 96 batched cavities advanced together.
+    Each runs from the reference's start (the last pressure) and, when
+    ``warm_start`` > 0, again with that extrapolation order (keys suffixed
+    ``_warm``)."""
     import torch
 
     from pycfd_amd import cases
@@ -214,33 +217,44 @@ def other_workloads(streams: int, batch_steps: int) -> dict:
         torch.cuda.synchronize()
         return time.perf_counter() - t0
 
-    cfg, _, _, _ = cases.cavity(nx=128, pressure_solver="cg", cfl_max=0.5, dt=0.01,
-                                poisson_tol=1e-10)
-    sim = Simulation(cfg)
-    w = timed(sim, 50, 5)
-    out["cavity128_cg"] = {"mlups": round(128 * 128 * 50 / w / 1e6, 3),
-                           "ms_per_step": round(1e3 * w / 50, 3),
-                           "cg_iterations_last_solve": sim.solver.pressure_solver.last_iterations}
-    log(f"[cavity 128 cg] {out['cavity128_cg']}")
-    cfg, _, u0, v0 = cases.taylor_green(n=1024, nu=0.01, dt=1e-3, pressure_solver="multigrid",
-                                        mg_sweeps=2)
-    sim = Simulation(cfg, u_init=u0, v_init=v0)
-    w = timed(sim, 40, 5)
-    out["taylor_green1024_mg_v22"] = {
-        "mlups": round(1024 * 1024 * 40 / w / 1e6, 2), "ms_per_step": round(1e3 * w / 40, 3),
-        "cycles_last_solve": sim.solver.pressure_solver.last_iterations}
-    log(f"[taylor-green 1024 V(2,2)] {out['taylor_green1024_mg_v22']}")
-    del sim
-    cfgs = cavity_batch_configs(96, seed=2024)
-    batch = SimulationBatch(cfgs, streams=streams)
-    batch.run(max_steps=2)
-    r = batch.run(max_steps=batch_steps)
-    out["batched_cavities96"] = {
-        "mlups": round(r.mlups, 2), "instances": len(cfgs), "streams": batch.n_streams,
-        "steps": batch_steps, "wall_s": round(r.wall_time, 3),
-        "setup_s": round(batch.setup_time, 2), "cells": int(sum(r.cells)),
-        "sizes": sorted({c.nx for c in cfgs}), "seed": 2024}
-    log(f"[batched cavities] {out['batched_cavities96']}")
+    orders = [0] + ([warm_start] if warm_start else [])
+    for order in orders:
+        sfx = "_warm" if order else ""
+        cfg, _, _, _ = cases.cavity(nx=128, pressure_solver="cg", cfl_max=0.5, dt=0.01,
+                                    poisson_tol=1e-10)
+        sim = Simulation(cfg)
+        sim.solver.warm_start_order = order
+        w = timed(sim, 50, 5)
+        key = "cavity128_cg" + sfx
+        out[key] = {"mlups": round(128 * 128 * 50 / w / 1e6, 3),
+                    "ms_per_step": round(1e3 * w / 50, 3),
+                    "cg_iterations_last_solve": sim.solver.pressure_solver.last_iterations,
+                    "warm_start_order": order}
+        log(f"[{key}] {out[key]}")
+        cfg, _, u0, v0 = cases.taylor_green(n=1024, nu=0.01, dt=1e-3,
+                                            pressure_solver="multigrid", mg_sweeps=2)
+        sim = Simulation(cfg, u_init=u0, v_init=v0)
+        sim.solver.warm_start_order = order
+        w = timed(sim, 40, 5)
+        key = "taylor_green1024_mg_v22" + sfx
+        out[key] = {
+            "mlups": round(1024 * 1024 * 40 / w / 1e6, 2), "ms_per_step": round(1e3 * w / 40, 3),
+            "cycles_last_solve": sim.solver.pressure_solver.last_iterations,
+            "warm_start_order": order}
+        log(f"[{key}] {out[key]}")
+        del sim
+        cfgs = cavity_batch_configs(96, seed=2024)
+        batch = SimulationBatch(cfgs, streams=streams, warm_start_order=order)
+        batch.run(max_steps=2)
+        r = batch.run(max_steps=batch_steps)
+        key = "batched_cavities96" + sfx
+        out[key] = {
+            "mlups": round(r.mlups, 2), "instances": len(cfgs), "streams": batch.n_streams,
+            "steps": batch_steps, "wall_s": round(r.wall_time, 3),
+            "setup_s": round(batch.setup_time, 2), "cells": int(sum(r.cells)),
+            "sizes": sorted({c.nx for c in cfgs}), "seed": 2024, "warm_start_order": order}
+        log(f"[{key}] {out[key]}")
+        del batch
     return out
 
 
@@ -344,7 +358,8 @@ def main() -> int:
     dominant["traffic_source"] = (f"{TRAFFIC_FILE.relative_to(ROOT)} (rocprofv3 --pmc "
                                   "FETCH_SIZE / WRITE_SIZE per dispatch, scaled by a copy "
                                   "kernel of known size)" if dominant.get("traffic") else None)
-    others = None if args.no_others else other_workloads(args.batch_streams, args.batch_steps)
+    others = None if args.no_others else other_workloads(args.batch_streams, args.batch_steps,
+                                                         args.warm_start)
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle) ...")

@@ -70,9 +70,12 @@ class BatchResult:
 
 class SimulationBatch:
     """Independent simulations, one HIP stream each (shared round-robin when
-    there are more instances than ``streams``)."""
+    there are more instances than ``streams``).  ``warm_start_order`` sets
+    every instance's ``ProjectionSolver.warm_start_order`` (0: the
+    reference's start from the last pressure)."""
 
-    def __init__(self, configs, obstacles=None, inits=None, streams: int = 8) -> None:
+    def __init__(self, configs, obstacles=None, inits=None, streams: int = 8,
+                 warm_start_order: int = 0) -> None:
         import torch
 
         from .physics.incompressible import Simulation
@@ -90,7 +93,10 @@ class SimulationBatch:
         for k, cfg in enumerate(configs):
             with torch.cuda.stream(self.streams[k % self.n_streams]):
                 u0, v0 = inits[k]
-                self.sims.append(Simulation(cfg, obstacle=obstacles[k], u_init=u0, v_init=v0))
+                sim = Simulation(cfg, obstacle=obstacles[k], u_init=u0, v_init=v0)
+                if warm_start_order:
+                    sim.solver.warm_start_order = warm_start_order
+                self.sims.append(sim)
         torch.cuda.synchronize()
         self.setup_time = time.perf_counter() - t0
 

@@ -16,17 +16,23 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 
-def test_batched_instances_equal_lone_runs(gpu_available):
+@pytest.mark.parametrize("warm", [0, 4])
+def test_batched_instances_equal_lone_runs(gpu_available, warm):
+    """``warm``: every instance's solves started from the extrapolation in
+    time (its history lives in the instance's own context)."""
     from pycfd_amd.batch import SimulationBatch, cavity_batch_configs
     from pycfd_amd.physics.incompressible import Simulation
     cfgs = cavity_batch_configs(count=7, seed=5, sizes=(24, 32, 40, 48),
                                 pressure_solver="mgcg", poisson_tol=1e-12)
-    batch = SimulationBatch(cfgs, streams=3)
-    res = batch.run(max_steps=6)
-    assert res.steps == [6] * 7 and res.mlups > 0
+    batch = SimulationBatch(cfgs, streams=3, warm_start_order=warm)
+    steps = 8 if warm else 6
+    res = batch.run(max_steps=steps)
+    assert res.steps == [steps] * 7 and res.mlups > 0
     for cfg, sim in zip(cfgs, batch.sims):
+        assert sim.solver.warm_start_order == warm
         lone = Simulation(cfg)
-        lone.run(max_steps=6)
+        lone.solver.warm_start_order = warm
+        lone.run(max_steps=steps)
         assert np.array_equal(lone.fields.u, sim.fields.u), cfg.name
         assert np.array_equal(lone.fields.v, sim.fields.v), cfg.name
         assert np.array_equal(lone.fields.p, sim.fields.p), cfg.name

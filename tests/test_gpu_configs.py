@@ -33,17 +33,21 @@ def _cavity_c1(tol):
     return Simulation(cfg)
 
 
-@pytest.mark.parametrize("tag,tol", [("t13", 1e-13), ("t10", 1e-10)])
-def test_cavity_128_cg_2000_steps_matches_reference(gpu_available, tag, tol):
+@pytest.mark.parametrize("tag,tol,warm", [("t13", 1e-13, 0), ("t10", 1e-10, 0), ("t13", 1e-13, 4)])
+def test_cavity_128_cg_2000_steps_matches_reference(gpu_available, tag, tol, warm):
     """Every dt of the 2000 steps equal to the reference's (rtol 1e-12), and
     the final u, v within 1e-10 relative and p within 1e-8 of the reference's
     run at the same CG tolerance.  At 1e-10 the two CG runs stop on their own
     recursive residuals at different iterates (the dot products reduce in a
     different order), and their divergences are each bounded by that
     tolerance, not equal: max|div u| <= 1e-9 where the reference reaches
-    4.0e-10; at 1e-13 both are at round-off (<= 5e-12, reference 8.9e-13)."""
+    4.0e-10; at 1e-13 both are at round-off (<= 5e-12, reference 8.9e-13).
+    ``warm``: each solve started from the quartic extrapolation in time of
+    its substep's last solutions instead of the last pressure; the reference's
+    run still bounds it."""
     g = golden("cavity_c1")
     sim = _cavity_c1(tol)
+    sim.solver.warm_start_order = warm
     w0 = time.perf_counter()
     res = sim.run(max_steps=2000)
     wall = time.perf_counter() - w0
@@ -54,7 +58,7 @@ def test_cavity_128_cg_2000_steps_matches_reference(gpu_available, tag, tol):
     dv = np.abs(v - g[f"{tag}_v"]).max()
     dp = np.abs(p - g[f"{tag}_p"]).max()
     div = max(res.history["max_div"])
-    print(f"cavity 128^2 CG tol {tol:g}: {wall:.2f} s for 2000 steps "
+    print(f"cavity 128^2 CG tol {tol:g} warm start {warm}: {wall:.2f} s for 2000 steps "
           f"({128 * 128 * 2000 / wall / 1e6:.2f} MLUPS; reference {float(g[f'{tag}_wall']):.0f} s), "
           f"|du| {du:.3e} |dv| {dv:.3e} |dp| {dp:.3e}, max|div| {div:.3e} "
           f"(reference {g[f'{tag}_max_div'].max():.3e})")
@@ -66,12 +70,13 @@ def test_cavity_128_cg_2000_steps_matches_reference(gpu_available, tag, tol):
                                rtol=1e-9)
 
 
-def _taylor_green(n, steps):
+def _taylor_green(n, steps, warm=0):
     from pycfd_amd import cases
     from pycfd_amd.physics.incompressible import Simulation
     cfg, _, u0, v0 = cases.taylor_green(n=n, nu=0.01, dt=1e-3, pressure_solver="multigrid",
                                         mg_sweeps=2)
     sim = Simulation(cfg, u_init=u0, v_init=v0)
+    sim.solver.warm_start_order = warm
     w0 = time.perf_counter()
     res = sim.run(max_steps=steps)
     return sim, res, time.perf_counter() - w0
@@ -112,6 +117,26 @@ def test_taylor_green_1024_multigrid_v22(gpu_available):
     order = np.log2(errs[512][0] / errs[1024][0])
     print(f"observed L2 order 512 -> 1024: {order:.3f}")
     assert order > 1.8
+
+
+def test_taylor_green_1024_warm_start(gpu_available):
+    """The same 500 steps at 1024^2 with each solve started from the quartic
+    extrapolation in time: the fields agree with the reference-started run to
+    the solver's tolerance, the error against the exact decay is unchanged,
+    and the V(2,2) cycles a solve drop (the decay is smooth in time)."""
+    a, ra, wa = _taylor_green(1024, 500)
+    b, rb, wb = _taylor_green(1024, 500, warm=4)
+    la, _ = _tg_errors(a)
+    lb, linf = _tg_errors(b)
+    ca, cb = a.solver.pressure_solver.last_iterations, b.solver.pressure_solver.last_iterations
+    print(f"taylor-green 1024^2 V(2,2): {wa:.2f} s -> {wb:.2f} s for 500 steps, cycles/solve "
+          f"{ca} -> {cb}, L2 {la:.6e} -> {lb:.6e}")
+    scale = np.abs(a.fields.u).max()
+    assert np.abs(a.fields.u - b.fields.u).max() <= 1e-8 * scale
+    assert np.abs(a.fields.v - b.fields.v).max() <= 1e-8 * scale
+    assert abs(lb - la) <= 1e-3 * la and linf < 1e-4
+    assert max(rb.history["max_div"]) < 1e-10
+    assert cb < ca
 
 
 def test_taylor_green_multigrid_v22_matches_oracle(gpu_available):
