@@ -370,7 +370,9 @@ class ProjectionSolver:
         trade-off (more sweeps, fewer iterations, docs/usage.md:473-476);
         this measures it on the device for the case at hand.  Only the
         multigrid solvers have sweeps; returns ``{sweeps: {"ms_per_step",
-        "iterations"}}``, empty for the others."""
+        "iterations"}}``, empty for the others.  Setting the pressure guess
+        before each timed step restarts the warm-start history, so every
+        candidate is timed with solves started from the last pressure."""
         import time
 
         import torch
