@@ -96,12 +96,14 @@ def cylinder(nx: int = 2048, ny: int = 1024, re: float = 100.0,
 
 
 def run_cylinder_record(steps: int, precision: str = "fp64", nx: int = 2048, ny: int = 1024,
-                        tol: float = 2e-11, progress_every: int = 1000, **overrides) -> dict:
+                        tol: float = 2e-11, progress_every: int = 1000, warm_start: int = 0,
+                        **overrides) -> dict:
     """The cylinder case run as cylinder_flow.run measures it
     (cylinder_flow.py:458-492, 590-600): Cd and Cl from the body reaction
     every 10 steps, a wake probe of v at (lx/2, ly/2 + D/4), mean Cd and Cl
     rms over t > 0.5 t_end (timeseries.py:255-259), and the Strouhal number
-    of the whole probe record (postprocess.py:116-142)."""
+    of the whole probe record (postprocess.py:116-142).  ``warm_start`` is
+    the solver's ``warm_start_order`` (0: the reference's start)."""
     import sys
     import time
 
@@ -110,6 +112,7 @@ def run_cylinder_record(steps: int, precision: str = "fp64", nx: int = 2048, ny:
     cfg, body, u0, v0 = cylinder(nx=nx, ny=ny, precision=precision, poisson_tol=tol,
                                  **overrides)
     sim = Simulation(cfg, obstacle=body, u_init=u0, v_init=v0)
+    sim.solver.warm_start_order = warm_start
     probe = sim.add_probe(cfg.lx * 0.5, cfg.ly * 0.5 + 0.25, "wake")
     hist = {"t": [], "cd": [], "cl": []}
     t0 = time.perf_counter()
@@ -130,7 +133,7 @@ def run_cylinder_record(steps: int, precision: str = "fp64", nx: int = 2048, ny:
     settled = t > 0.5 * t[-1]
     series = {k: np.asarray(v) for k, v in (("t", probe.t), ("v", probe.v))}
     return {
-        "precision": precision, "steps": res.steps, "t_end": float(sim.fields.t),
+        "precision": precision, "warm_start": warm_start, "steps": res.steps, "t_end": float(sim.fields.t),
         "wall_s": round(wall, 2), "mlups": round(nx * ny * res.steps / wall / 1e6, 2),
         "cd_mean": late_time_mean(t, cd), "cl_rms": float(np.sqrt(np.mean(cl[settled] ** 2))),
         "strouhal": strouhal_number(series["t"], series["v"], cfg.l_ref, cfg.u_ref),
@@ -150,9 +153,12 @@ if __name__ == "__main__":  # python -m pycfd_amd.cases cylinder-record --steps 
     ap.add_argument("--nx", type=int, default=2048)
     ap.add_argument("--ny", type=int, default=1024)
     ap.add_argument("--tol", type=float, default=2e-11)
+    ap.add_argument("--warm-start", type=int, default=0,
+                    help="extrapolation order of each solve's start (0: the last pressure)")
     ap.add_argument("--out", default=None, help="write the full records (JSON) here")
     a = ap.parse_args()
-    runs = {p: run_cylinder_record(a.steps, p, a.nx, a.ny, a.tol) for p in a.precision.split(",")}
+    runs = {p: run_cylinder_record(a.steps, p, a.nx, a.ny, a.tol, warm_start=a.warm_start)
+            for p in a.precision.split(",")}
     summary = {p: {k: v for k, v in r.items() if not k.endswith("_series")} for p, r in runs.items()}
     if "fp64" in runs and "mixed" in runs:
         f, m = runs["fp64"], runs["mixed"]
